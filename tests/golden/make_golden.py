@@ -1,0 +1,279 @@
+"""Write the golden fixtures in tests/golden/ from the reference implementation.
+
+Run in the development container only (the reference is not shipped anywhere):
+
+    python tests/golden/make_golden.py /root/reference
+
+It imports the reference's own Python (CPU mode; CuPy is absent) and records
+inputs and outputs of the functions on the PathFinder hot path:
+
+  lattice_*.npz    Lattice3D.build_graph -> CSR arrays      UPF:1195-1338, 647-792
+  via_pairs.json   Lattice3D.get_legal_via_pairs order      UPF:1136-1181
+  roi_bfs.npz      ROIExtractor.extract_roi                 UPF:1619-1734
+  dijkstra.npz     SimpleDijkstra.find_path_roi             UPF:1801-1887
+  costs.npz        EdgeAccountant update_costs / ema / history / overuse
+  negotiation.npz  route_multiple_nets trajectory on a small board
+
+The fixtures are data only: inputs and the outputs the reference produced.
+"""
+import json
+import logging
+import os
+import random
+import sys
+import types
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _import_reference(root):
+    sys.path.insert(0, root)
+    logging.disable(logging.CRITICAL)
+    from orthoroute.algorithms.manhattan import unified_pathfinder as upf
+    from orthoroute.domain.models.board import Board, Net, Pad, Coordinate
+    return upf, Board, Net, Pad, Coordinate
+
+
+def make_lattice(upf, X, Y, L, via_cost=3.0):
+    lat = upf.Lattice3D((0.0, 0.0, float(X - 1), float(Y - 1)), 1.0, L)
+    assert (lat.x_steps, lat.y_steps) == (X, Y)
+    g = lat.build_graph(via_cost)
+    return lat, g
+
+
+def gen_lattices(upf):
+    out = {}
+    for (X, Y, L) in [(5, 4, 2), (6, 5, 3), (8, 6, 4), (12, 10, 6), (7, 6, 8)]:
+        lat, g = make_lattice(upf, X, Y, L)
+        np.savez_compressed(
+            os.path.join(HERE, f"lattice_{X}x{Y}x{L}.npz"),
+            X=X, Y=Y, L=L, via_cost=3.0, pitch=1.0,
+            indptr=np.asarray(g.indptr), indices=np.asarray(g.indices),
+            costs=np.asarray(g.base_costs), edge_layer=np.asarray(g.edge_layer),
+            edge_kind=np.asarray(g.edge_kind))
+        out[f"{X}x{Y}x{L}"] = int(len(g.indices))
+    pairs = {}
+    for L in range(2, 33):
+        lat = upf.Lattice3D((0.0, 0.0, 3.0, 3.0), 1.0, L)
+        pairs[str(L)] = [list(p) for p in lat.get_legal_via_pairs(L)]
+    with open(os.path.join(HERE, "via_pairs.json"), "w") as f:
+        json.dump(pairs, f)
+    return out
+
+
+def gen_roi(upf):
+    lat, g = make_lattice(upf, 30, 24, 6)
+    rng = np.random.default_rng(11)
+    plane = 30 * 24
+    cases = []
+    for i in range(14):
+        sx, sy = rng.integers(0, 30), rng.integers(0, 24)
+        dx, dy = rng.integers(0, 30), rng.integers(0, 24)
+        sz = int(rng.integers(0, 6)) if i % 3 else 0
+        dz = int(rng.integers(0, 6)) if i % 3 else 0
+        src = int(sz * plane + sy * 30 + sx)
+        dst = int(dz * plane + dy * 30 + dx)
+        if src == dst:
+            continue
+        bonus = [0.0, 0.6, 1.2, 3.0][i % 4]
+        max_nodes = [50_000, 300, 1000, 2500][i % 4]
+        ext = upf.ROIExtractor(g, lattice=lat)
+        ext.max_roi_nodes = max_nodes
+        roi, g2r = ext.extract_roi(src, dst, initial_radius=40, stagnation_bonus=bonus)
+        cases.append((src, dst, bonus, max_nodes, np.asarray(roi, dtype=np.int64)))
+    np.savez_compressed(
+        os.path.join(HERE, "roi_bfs.npz"), X=30, Y=24, L=6,
+        src=np.array([c[0] for c in cases]), dst=np.array([c[1] for c in cases]),
+        bonus=np.array([c[2] for c in cases]), max_nodes=np.array([c[3] for c in cases]),
+        roi_len=np.array([len(c[4]) for c in cases]),
+        roi=np.concatenate([c[4] for c in cases]))
+
+
+def gen_dijkstra(upf):
+    lat, g = make_lattice(upf, 20, 16, 6)
+    E = len(g.indices)
+    N = 20 * 16 * 6
+    rng = np.random.default_rng(12)
+    solver = upf.SimpleDijkstra(g, lat)
+    ext = upf.ROIExtractor(g, lattice=lat)
+    base = np.asarray(g.base_costs, dtype=np.float32)
+    snaps, srcs, dsts, modes, paths, plen = [], [], [], [], [], []
+    for s in range(4):
+        if s == 0:
+            costs = base * np.float32(0.3)
+        elif s == 1:   # many exact ties
+            costs = (base * np.float32(0.5) * rng.integers(1, 3, E).astype(np.float32)).astype(np.float32)
+        elif s == 2:
+            costs = (base * (1 + 4 * rng.random(E))).astype(np.float32)
+        else:          # reference-like: 0.3*base + jitter, some +inf blocked vias
+            costs = (base * np.float32(0.3)).astype(np.float32)
+            costs += (np.arange(E, dtype=np.float32) % 9973) * np.float32(1e-6)
+            kind = np.asarray(g.edge_kind)
+            blk = (kind == 1) & (rng.random(E) < 0.2)
+            costs[blk] = np.float32(np.inf)
+        snaps.append(costs.astype(np.float32))
+        for k in range(12):
+            src = int(rng.integers(0, N)); dst = int(rng.integers(0, N))
+            if src == dst:
+                continue
+            if k % 2 == 0:
+                roi, g2r = ext.extract_roi(src, dst, initial_radius=40)
+                mode = 0
+            else:
+                roi = np.arange(N, dtype=np.int32); g2r = np.arange(N, dtype=np.int32)
+                mode = 1
+            p = solver.find_path_roi(src, dst, snaps[-1], roi, g2r)
+            p = [] if p is None else list(p)
+            srcs.append(s * 1_000_000 + src); dsts.append(dst); modes.append(mode)
+            paths.append(np.asarray(p, dtype=np.int64)); plen.append(len(p))
+    np.savez_compressed(
+        os.path.join(HERE, "dijkstra.npz"), X=20, Y=16, L=6,
+        snaps=np.stack(snaps), src=np.array(srcs), dst=np.array(dsts),
+        mode=np.array(modes), path_len=np.array(plen),
+        paths=np.concatenate(paths) if paths else np.zeros(0, np.int64))
+
+
+def gen_costs(upf):
+    lat, g = make_lattice(upf, 12, 10, 6)
+    E = len(g.indices)
+    rng = np.random.default_rng(13)
+    acc = upf.EdgeAccountant(E)
+    acc.present[:] = rng.integers(0, 4, E).astype(np.float32)
+    acc.present_ema[:] = (rng.random(E) * 3).astype(np.float32)
+    acc.history[:] = (rng.random(E) * 2).astype(np.float32)
+    present0, ema0, hist0 = acc.present.copy(), acc.present_ema.copy(), acc.history.copy()
+    layer_bias = (1.0 + 0.8 * rng.random(6)).astype(np.float32)
+    acc.update_costs(g.base_costs, 3.7, 12.0, via_cost_multiplier=0.5,
+                     base_cost_weight=0.3, edge_layer=g.edge_layer,
+                     layer_bias_per_layer=layer_bias, edge_kind=g.edge_kind)
+    total_a = acc.total_cost.copy()
+    acc.update_costs(g.base_costs, 1.0, 10.0)
+    total_b = acc.total_cost.copy()
+    acc.update_present_ema(beta=0.4)
+    ema1 = acc.present_ema.copy()
+    acc.update_history(0.25, base_costs=g.base_costs, history_cap_multiplier=15.0,
+                       decay_factor=0.98, use_raw_present=True)
+    hist1 = acc.history.copy()
+    acc.update_history(1.2, base_costs=g.base_costs, history_cap_multiplier=15.0,
+                       decay_factor=1.0, use_raw_present=False)
+    hist2 = acc.history.copy()
+    acc.present[:5] += np.float32(0.37)
+    over_sum, over_cnt = acc.compute_overuse()
+    np.savez_compressed(
+        os.path.join(HERE, "costs.npz"), X=12, Y=10, L=6,
+        present0=present0, ema0=ema0, hist0=hist0, layer_bias=layer_bias,
+        total_a=total_a, total_b=total_b, ema1=ema1, hist1=hist1, hist2=hist2,
+        present_final=acc.present.copy(), over_sum=over_sum, over_cnt=over_cnt)
+
+
+def gen_negotiation(upf, Board, Net, Pad, Coordinate, X=20, Y=20, L=6, n_nets=60,
+                    seed=7, cap=30, name="negotiation"):
+    corner = Net(id="corner", name="corner", pads=[
+        Pad(id="c0", component_id="C", net_id="corner", position=Coordinate(3.0, 3.0), size=(0.2, 0.2)),
+        Pad(id="c1", component_id="C", net_id="corner", position=Coordinate(float(X - 4), float(Y - 4)), size=(0.2, 0.2)),
+    ])
+    board = Board(id="b", name="b", nets=[corner], layer_count=L)
+    cfg = upf.PathFinderConfig()
+    cfg.grid_pitch = 1.0
+    cfg.via_cost = 3.0
+    router = upf.PathFinderRouter(cfg, use_gpu=False)
+    router.initialize_graph(board)
+    assert (router.lattice.x_steps, router.lattice.y_steps) == (X, Y)
+
+    rng = np.random.default_rng(seed)
+    plane = X * Y
+    cells = rng.permutation(plane)[: 2 * n_nets]
+    reqs = [(f"N{i:03d}", int(cells[2 * i]), int(cells[2 * i + 1])) for i in range(n_nets)]
+
+    orig_apply = upf.apply_derived_parameters
+
+    def capped_apply(config, derived):
+        orig_apply(config, derived)
+        config.max_iterations = cap
+    upf.apply_derived_parameters = capped_apply
+
+    rec = {"order": [], "total_cost": [], "over": [], "history": [], "ema": [],
+           "paths_iter": [], "layer_bias": [], "pres_fac": []}
+    orig_order = router._order_nets_by_difficulty
+
+    def order_wrap(tasks):
+        o = orig_order(tasks)
+        rec["order"].append(list(o))
+        rec["total_cost"].append(np.asarray(router.accounting.total_cost, dtype=np.float32).copy())
+        return o
+    router._order_nets_by_difficulty = order_wrap
+
+    def it_cb(it, tracks, vias, over_sum, over_cnt):
+        rec["over"].append((it, int(over_sum), int(over_cnt)))
+        rec["history"].append(router.accounting.history.copy())
+        rec["ema"].append(router.accounting.present_ema.copy())
+        rec["layer_bias"].append(np.asarray(router.layer_bias, dtype=np.float64).copy())
+        snap = []
+        for k in sorted(router.net_paths):
+            snap.append((k, list(router.net_paths[k])))
+        rec["paths_iter"].append(snap)
+
+    res = router.route_multiple_nets(reqs, iteration_cb=it_cb)
+    upf.apply_derived_parameters = orig_apply
+
+    names = [r[0] for r in reqs]
+    name_idx = {n: i for i, n in enumerate(names)}
+    order_flat, order_len = [], []
+    for o in rec["order"]:
+        order_len.append(len(o)); order_flat.extend(name_idx[n] for n in o)
+    pi_net, pi_len, pi_nodes, pi_iter = [], [], [], []
+    for it_i, snap in enumerate(rec["paths_iter"]):
+        for k, p in snap:
+            pi_iter.append(it_i); pi_net.append(name_idx[k]); pi_len.append(len(p)); pi_nodes.extend(p)
+    final_paths = res.get("paths", {})
+    fp_len, fp_nodes = [], []
+    for n in names:
+        p = list(final_paths.get(n, []))
+        fp_len.append(len(p)); fp_nodes.extend(p)
+    np.savez_compressed(
+        os.path.join(HERE, f"{name}.npz"), X=X, Y=Y, L=L, seed=seed, cap=cap,
+        src=np.array([r[1] for r in reqs]), dst=np.array([r[2] for r in reqs]),
+        over=np.array(rec["over"], dtype=np.int64),
+        order_len=np.array(order_len), order=np.array(order_flat, dtype=np.int64),
+        total_cost=np.stack(rec["total_cost"]), history=np.stack(rec["history"]),
+        ema=np.stack(rec["ema"]), layer_bias=np.stack(rec["layer_bias"]),
+        pi_iter=np.array(pi_iter), pi_net=np.array(pi_net), pi_len=np.array(pi_len),
+        pi_nodes=np.array(pi_nodes, dtype=np.int64),
+        final_len=np.array(fp_len), final_nodes=np.array(fp_nodes, dtype=np.int64),
+        success=bool(res.get("success", False)),
+        overuse_sum=int(res.get("overuse_sum", 0) or 0),
+        overuse_edges=int(res.get("overuse_edges", 0) or 0),
+        failed_nets=int(res.get("failed_nets", 0) or 0))
+    return res
+
+
+def main():
+    root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    which = sys.argv[2:] or ["lattice", "roi", "dijkstra", "costs", "negotiation"]
+    upf, Board, Net, Pad, Coordinate = _import_reference(root)
+    random.seed(0)
+    if "lattice" in which:
+        print("lattices", gen_lattices(upf))
+    if "roi" in which:
+        gen_roi(upf); print("roi done")
+    if "dijkstra" in which:
+        gen_dijkstra(upf); print("dijkstra done")
+    if "costs" in which:
+        gen_costs(upf); print("costs done")
+    if "negotiation" in which:
+        r = gen_negotiation(upf, Board, Net, Pad, Coordinate)
+        print("negotiation", {k: r.get(k) for k in ("success", "overuse_sum", "overuse_edges", "failed_nets")})
+    if "negotiation_easy" in which:
+        r = gen_negotiation(upf, Board, Net, Pad, Coordinate, X=24, Y=22, L=6, n_nets=14,
+                            seed=5, cap=30, name="negotiation_easy")
+        print("negotiation_easy", {k: r.get(k) for k in ("success", "overuse_sum", "overuse_edges", "failed_nets")})
+    if "negotiation2" in which:
+        r = gen_negotiation(upf, Board, Net, Pad, Coordinate, X=16, Y=14, L=4, n_nets=45,
+                            seed=9, cap=25, name="negotiation_l4")
+        print("negotiation_l4", {k: r.get(k) for k in ("success", "overuse_sum", "overuse_edges", "failed_nets")})
+
+
+if __name__ == "__main__":
+    main()
