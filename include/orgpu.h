@@ -1,0 +1,220 @@
+/*
+ * orgpu: C ABI of the MI355X (gfx950) PathFinder inner loop.
+ *
+ * Plain pointers and sizes only.  Every large array is caller-owned device
+ * memory (allocated by the caller, e.g. a torch tensor on cuda:0); it must
+ * stay alive until the handle's stream has been synchronised.  The handle owns
+ * only its scratch (search keys, bucket queues, bitmaps, staging), sized by
+ * orgpu_reserve.  Calls are asynchronous on the handle's stream unless the
+ * comment says otherwise.  Return value: 0 on success, a negative ORGPU_E*
+ * code on failure (detail in orgpu_last_error(), thread-local).  "No path" is
+ * not an error: the path length is 0 and the Python layer maps it to None.
+ *
+ * One handle = one HIP stream = one host thread at a time.  Several handles
+ * may coexist (serving slots).  No C++ exception crosses this boundary.
+ *
+ * Reference interfaces each entry point replaces are cited as
+ * UPF = orthoroute/algorithms/manhattan/unified_pathfinder.py.
+ */
+#ifndef ORGPU_H
+#define ORGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORGPU_OK 0
+#define ORGPU_EINVAL -1    /* invalid argument */
+#define ORGPU_ENOMEM -2    /* device allocation failed */
+#define ORGPU_EHIP -3      /* HIP runtime error */
+#define ORGPU_ECAPACITY -4 /* reserved capacity exceeded (queue, path length, slots) */
+
+#define ORGPU_MAX_LAYERS 64
+
+typedef struct orgpu_ctx orgpu_t;
+
+/* ---- handle -------------------------------------------------------------- */
+
+/* Creates a handle on `device` bound to `hip_stream` (a hipStream_t; NULL =
+ * the null stream).  Replaces nothing in the reference: there the CuPy
+ * module-level state of CUDADijkstra (UPF:2205-2218) plays this role. */
+int orgpu_create(int device, void *hip_stream, orgpu_t **out);
+int orgpu_destroy(orgpu_t *h);
+const char *orgpu_last_error(void);
+/* Blocks until all work queued on the handle's stream has finished. */
+int orgpu_sync(orgpu_t *h);
+
+/* ---- A1 lattice ------------------------------------------------------------ */
+
+/* Lattice3D(bounds, pitch, layers) + build_graph(via_cost) (UPF:1075-1338,
+ * 647-792).  node = z*X*Y + y*X + x.  dir[z]: 0 = 'v', 1 = 'h', 2 = 'hv'.
+ * planar[z] != 0: layer z carries planar edges.  Via pairs (from, to) are
+ * given in the order the reference iterates them (UPF:1266-1276); each pair
+ * yields u->v and v->u with cost via_cost*(1+0.15*(|to-from|-1)). */
+typedef struct orgpu_lattice_desc {
+    int32_t X, Y, L;
+    float pitch;
+    double via_cost;
+    uint8_t planar[ORGPU_MAX_LAYERS];
+    uint8_t dir[ORGPU_MAX_LAYERS];
+    int32_t npairs;
+    const int32_t *pair_from; /* host, npairs */
+    const int32_t *pair_to;   /* host, npairs */
+} orgpu_lattice_desc;
+
+/* Host-only: node and edge counts (UPF:1206-1222). */
+int orgpu_lattice_count(const orgpu_lattice_desc *d, int64_t *n_nodes, int64_t *n_edges);
+
+/* CSR arrays exactly as CSRGraph.finalize leaves them (rows sorted by
+ * (dst, cost), UPF:723): indptr i32[N+1], indices i32[E], base_cost f32[E],
+ * edge_layer u8[E] (source layer), edge_kind u8[E] (1 = via). */
+int orgpu_lattice_build(orgpu_t *h, const orgpu_lattice_desc *d, int32_t *indptr,
+                        int32_t *indices, float *base_cost, uint8_t *edge_layer,
+                        uint8_t *edge_kind);
+
+/* ---- A2/A3/A4 search -------------------------------------------------------- */
+
+typedef struct orgpu_graph {
+    int64_t n_nodes, n_edges;
+    const int32_t *indptr;    /* device i32[N+1] */
+    const int32_t *indices;   /* device i32[E]  */
+    const float *cost;        /* device f32[E]: this iteration's total_cost */
+    const uint8_t *edge_kind; /* device u8[E]  */
+    int32_t X, Y, L;          /* lattice shape (node = z*X*Y + y*X + x) */
+} orgpu_graph;
+
+/* One search request: SimpleDijkstra.find_path_roi(src, dst, costs, roi,
+ * global_to_roi) (UPF:1801-1887) with the per-net owner-aware keepout of
+ * _apply_owner_aware_via_keepouts (UPF:3150-3228) folded into the edge
+ * weight.  roi_index < 0: whole graph; otherwise row roi_index of the ROI
+ * bitmap array passed to orgpu_route.  use_keepout != 0: out-edges of a node
+ * whose keepout owner is neither -1 nor net_id, other than via edges, cost
+ * the keepout penalty (1e9 in the reference) instead of their cost. */
+typedef struct orgpu_net_req {
+    int32_t src, dst;
+    int32_t net_id;
+    int32_t roi_index;
+    int32_t use_keepout;
+    int32_t reserved;
+} orgpu_net_req;
+
+typedef struct orgpu_sssp_params {
+    float delta;            /* bucket width (cost units) */
+    int32_t n_buckets;      /* bucket ring size (>= 1) */
+    int32_t slots;          /* concurrent searches K (1 = plain sequential) */
+    int32_t rounds_per_poll;/* kernel rounds queued between host status polls */
+    int32_t grid;           /* workgroups per round launch (0 = auto) */
+    int32_t count_minimal;  /* 1: count Dijkstra-minimal settled nodes/edges per net */
+} orgpu_sssp_params;
+
+typedef struct orgpu_sssp_stats {
+    int64_t rounds;          /* round launches executed */
+    int64_t items;           /* queue items expanded (incl. re-expansions) */
+    int64_t edges;           /* edges relaxed (incl. re-relaxations) */
+    int64_t minimal_nodes;   /* Dijkstra-minimal settled nodes (count_minimal) */
+    int64_t minimal_edges;   /* their out-edges (count_minimal) */
+    double kernel_ms;        /* device time of the round kernels (HIP events) */
+} orgpu_sssp_stats;
+
+/* Reserves scratch for graphs of up to n_nodes nodes, `slots` concurrent
+ * searches and `n_buckets` ring buckets; queue_capacity items per queue
+ * (0 = slots*n_nodes).  Blocks (allocation). */
+int orgpu_reserve(orgpu_t *h, int64_t n_nodes, int32_t slots, int32_t n_buckets,
+                  int64_t queue_capacity, int32_t max_path_len);
+
+/* Routes n_nets requests (host array) to completion: K = params->slots
+ * searches run concurrently and a finished slot is refilled with the next
+ * request.  Writes each path (src..dst, global node ids) and the CSR edge id
+ * of each hop (first match in the row, UPF:5463-5476) into the host arrays
+ * path_nodes / path_edges at path_offset[i] (host i64[n_nets+1], filled by
+ * this call); capacity = path_capacity entries each.  path_len 0 = no path.
+ * roi_bitmaps: device u32[n_roi][ceil(N/32)], bit set = node in ROI.
+ * keepout_owner: device i32[N] or NULL.  Blocks until done. */
+int orgpu_route(orgpu_t *h, const orgpu_graph *g, const orgpu_net_req *nets, int32_t n_nets,
+                const uint32_t *roi_bitmaps, int32_t n_roi, const int32_t *keepout_owner,
+                float keepout_cost,
+                const orgpu_sssp_params *params, int32_t *path_nodes, int32_t *path_edges,
+                int64_t path_capacity, int64_t *path_offset, float *path_dist,
+                orgpu_sssp_stats *stats);
+
+/* ---- A5/A6 congestion state (EdgeAccountant in HBM) ------------------------- */
+
+/* Nonzero entries of a per-edge quantity, copied to host arrays (idx, val)
+ * of `capacity` entries, in no particular order; *count receives the number of
+ * nonzeros (may exceed capacity: then only `capacity` were written and the
+ * call returns ORGPU_ECAPACITY). */
+typedef struct orgpu_sparse_out {
+    int32_t *idx;      /* host */
+    float *val;        /* host */
+    int64_t capacity;
+    int64_t count;     /* out */
+} orgpu_sparse_out;
+
+/* canonical usage count[e] += delta for every edge id in the host array
+ * (commit_path / clear_path / _rebuild_usage_from_committed_nets,
+ * UPF:821-836, 4971-4987).  count: device i32[E]. */
+int orgpu_usage_scatter(orgpu_t *h, int32_t *count, int64_t n_edges, const int32_t *edge_ids,
+                        int64_t n, int32_t delta);
+
+/* present[e] = float(count[e]) (refresh_from_canonical, UPF:814-819), then
+ * present[e] += penalty once per edge of the host list (the barrel-conflict
+ * penalty, UPF:3863-3871; numpy fancy += applies once per unique index). */
+int orgpu_present_build(orgpu_t *h, int64_t n_edges, const int32_t *count, float *present,
+                        const int32_t *penalty_edges, int64_t n_penalty, float penalty);
+
+/* present_ema = beta*present + (1-beta)*present_ema with present = count
+ * (update_present_ema, UPF:963-971; first != 0 starts from
+ * present_ema = present, UPF:3650-3652).  beta and one_minus_beta are the
+ * float32 constants numpy would use.  Also reports the nonzeros of
+ * max(0, present_ema - 1) (input of _compute_layer_bias, UPF:4825-4878). */
+int orgpu_present_ema(orgpu_t *h, int64_t n_edges, const int32_t *count, float *present_ema,
+                      int first, float beta, float one_minus_beta, orgpu_sparse_out *over_out);
+
+/* Per-iteration cost (EdgeAccountant.update_costs UPF:973-1043, then the
+ * via pooling penalties UPF:3291-3387 and, when hard_block != 0, the via
+ * capacity hard block UPF:3483-3546), one fused pass.  All float32 constants
+ * are the values numpy would use; via_discount[zlo*L+zhi] is the leave-hot-
+ * layer factor (double; applied in double when discount_f64, else rounded to
+ * float first). */
+typedef struct orgpu_cost_params {
+    int32_t X, Y, L;
+    float via_cost_mult, base_cost_weight, pres_fac, hist_weight, jitter_scale;
+    int32_t add_jitter;
+    const float *layer_bias;      /* device f32[L] */
+    int32_t via_pooling;          /* apply pooling penalties */
+    int32_t seg_z;                /* number of via segments per column (L-2, 0 = none) */
+    const float *col_pres;        /* device f32[X*Y]   ([x][y]) */
+    const float *seg_prefix;      /* device f32[X*Y*seg_z] ([x][y][s]) */
+    float col_weight, seg_weight;
+    const double *via_discount;   /* device f64[L*L] */
+    int32_t discount_f64;
+    int32_t hard_block;
+    const int16_t *col_use;       /* device i16[X*Y] */
+    const int16_t *seg_use;       /* device i16[X*Y*seg_z] */
+    int32_t col_cap, seg_cap;
+} orgpu_cost_params;
+
+int orgpu_cost_update(orgpu_t *h, const orgpu_cost_params *p, int64_t n_nodes, int64_t n_edges,
+                      const int32_t *indptr, const int32_t *indices, const uint8_t *edge_kind,
+                      const float *base_cost, const float *present_ema, const float *history,
+                      float *total_cost);
+
+/* history = history*decay; history += min(gain*max(0, present-1), cap_mult*base)
+ * (EdgeAccountant.update_history with use_raw_present, UPF:898-961; the cap
+ * is skipped when use_cap == 0). */
+int orgpu_history_update(orgpu_t *h, int64_t n_edges, const float *present, const float *base_cost,
+                         float *history, float decay, float gain, float cap_mult, int use_cap);
+
+/* Nonzeros of max(0, present-1) (mode 0), or the hot-edge set of
+ * _build_hotset, over + 0.1*history > 0.5, with val = over (mode 1,
+ * UPF:5021-5024). */
+int orgpu_overuse_sparse(orgpu_t *h, int64_t n_edges, const float *present, const float *history,
+                         int mode, float hist_scale, float threshold, orgpu_sparse_out *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ORGPU_H */

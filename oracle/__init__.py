@@ -58,7 +58,7 @@ def lib():
         _lib.orc_lattice_build.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, f32, f64,
                                            P, P, ctypes.c_int, P, P, P, P, P]
         _lib.orc_sssp_canon.restype = ctypes.c_int
-        _lib.orc_sssp_canon.argtypes = [i64, P, P, P, P, P, P, i32, f32, i32, i32, P, P, P, P]
+        _lib.orc_sssp_canon.argtypes = [i64, P, P, P, P, P, P, i32, f32, i32, i32, P, P, P, P, P]
         _lib.orc_backtrace.restype = i64
         _lib.orc_backtrace.argtypes = [P, i32, i32, P, i64]
         _lib.orc_sssp_ref.restype = i64
@@ -163,17 +163,19 @@ KEEPOUT_COST = np.float32(1e9)   # UPF:3182
 
 def sssp_canon(lat, costs, src, dst, roi_mask=None, keepout_owner=None, net_id=-1):
     """FP32 SSSP with lowest-predecessor tie-break; returns (path list or None,
-    dist f32[N], pred i32[N], expanded, traversed)."""
+    dist f32[N], pred i32[N], expanded, traversed).  Paths are ordered by
+    (number of keepout edges, fp32 cost of the other edges)."""
     N = lat.N
     dist = np.empty(N, dtype=np.float32)
     pred = np.empty(N, dtype=np.int32)
     costs = np.ascontiguousarray(costs, dtype=np.float32)
     rm = None if roi_mask is None else np.ascontiguousarray(roi_mask, dtype=np.uint8)
     ko = None if keepout_owner is None else np.ascontiguousarray(keepout_owner, dtype=np.int32)
+    nko = np.empty(N, dtype=np.uint8)
     ex, tr = ctypes.c_int64(0), ctypes.c_int64(0)
     r = lib().orc_sssp_canon(N, _p(lat.indptr), _p(lat.indices), _p(costs), _p(lat.edge_kind),
                              _p(rm), _p(ko), int(net_id), float(KEEPOUT_COST), int(src), int(dst),
-                             _p(dist), _p(pred), ctypes.byref(ex), ctypes.byref(tr))
+                             _p(dist), _p(pred), _p(nko), ctypes.byref(ex), ctypes.byref(tr))
     if r < 0:
         raise MemoryError("oracle sssp failed")
     path = None

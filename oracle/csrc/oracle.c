@@ -209,39 +209,60 @@ int orc_sssp_canon(int64_t N, const int32_t *indptr, const int32_t *indices,
                    const float *costs, const uint8_t *edge_kind,
                    const uint8_t *roi, const int32_t *keepout_owner,
                    int32_t net_id, float keepout_cost, int32_t src,
-                   int32_t dst, float *dist, int32_t *pred,
+                   int32_t dst, float *dist, int32_t *pred, uint8_t *nko,
                    int64_t *expanded, int64_t *traversed)
 {
-    for (int64_t i = 0; i < N; ++i) { dist[i] = INFINITY; pred[i] = -1; }
+    (void)keepout_cost;
+    /* Path order: (number of keepout edges, fp32 cost of the other edges),
+     * then the lowest predecessor.  A keepout edge costs the reference 1e9
+     * (UPF:3182), far above any path cost, so its float64 sums order paths
+     * the same way; counting keepout edges apart keeps the fp32 cost exact. */
+    uint8_t *ko = nko;
+    for (int64_t i = 0; i < N; ++i) { dist[i] = INFINITY; pred[i] = -1; ko[i] = 255; }
     uint8_t *done = (uint8_t *)calloc((size_t)N, 1);
     if (!done) return -1;
     heapf_t h = {0};
     int64_t nexp = 0, ntrav = 0;
     dist[src] = 0.0f;
-    hf_push(&h, (hf_t){0.0f, src});
+    ko[src] = 0;
+    /* heap key: keepout count folded into the float's order by popping
+     * (ko, d) lexicographically -- one heap per keepout level would do too;
+     * here the heap compares (ko, d, node) through a combined float key is
+     * not possible, so levels are processed in increasing ko order */
     int reached = 0;
-    while (h.n > 0) {
-        hf_t t = hf_pop(&h);
-        int32_t u = t.n;
-        if (done[u] || t.d != dist[u]) continue;
-        if (reached && dist[u] > dist[dst]) break;
-        done[u] = 1;
-        nexp++;
-        if (u == dst) reached = 1;
-        float du = dist[u];
-        int ko = keepout_owner && keepout_owner[u] >= 0 && keepout_owner[u] != net_id;
-        for (int32_t e = indptr[u]; e < indptr[u + 1]; ++e) {
-            int32_t v = indices[e];
-            ntrav++;
-            if (roi && !roi[v]) continue;
-            float w = (ko && edge_kind[e] == 0) ? keepout_cost : costs[e];
-            float nd = du + w;
-            if (nd < dist[v]) {
-                dist[v] = nd;
-                pred[v] = u;
-                hf_push(&h, (hf_t){nd, v});
-            } else if (nd == dist[v] && u < pred[v]) {
-                pred[v] = u;
+    for (int level = 0; level < 32 && !(reached); ++level) {
+        /* seed this level with every node that sits at it */
+        h.n = 0;
+        for (int64_t i = 0; i < N; ++i)
+            if (ko[i] == level && !done[i]) hf_push(&h, (hf_t){dist[i], (int32_t)i});
+        if (h.n == 0) continue;
+        while (h.n > 0) {
+            hf_t t = hf_pop(&h);
+            int32_t u = t.n;
+            if (done[u] || ko[u] != level || t.d != dist[u]) continue;
+            if (reached && dist[u] > dist[dst]) break;
+            done[u] = 1;
+            nexp++;
+            if (u == dst) reached = 1;
+            float du = dist[u];
+            int kou = keepout_owner && keepout_owner[u] >= 0 && keepout_owner[u] != net_id;
+            for (int32_t e = indptr[u]; e < indptr[u + 1]; ++e) {
+                int32_t v = indices[e];
+                ntrav++;
+                if (roi && !roi[v]) continue;
+                int nk = level;
+                float nd;
+                if (kou && edge_kind[e] == 0) { nk = level < 31 ? level + 1 : 31; nd = du; }
+                else { nd = du + costs[e]; if (!(nd < INFINITY)) continue; }
+                int better = nk < ko[v] || (nk == ko[v] && nd < dist[v]);
+                if (better) {
+                    ko[v] = (uint8_t)nk;
+                    dist[v] = nd;
+                    pred[v] = u;
+                    if (nk == level) hf_push(&h, (hf_t){nd, v});
+                } else if (nk == ko[v] && nd == dist[v] && u < pred[v]) {
+                    pred[v] = u;
+                }
             }
         }
     }

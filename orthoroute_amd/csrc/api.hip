@@ -1,0 +1,102 @@
+// C ABI entry points, error reporting and the handle (see include/orgpu.h).
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+
+#include "common.h"
+#include "engine.h"
+
+static thread_local char g_err[1024] = "";
+
+int orgpu_fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int orgpu_fail_hip(hipError_t e, const char *what, const char *file, int line)
+{
+    snprintf(g_err, sizeof(g_err), "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    return e == hipErrorOutOfMemory ? ORGPU_ENOMEM : ORGPU_EHIP;
+}
+
+hipStream_t orgpu_stream(orgpu_t *h) { return h->stream; }
+
+int orgpu_set_device(orgpu_t *h)
+{
+    HIP_CHECK_RET(hipSetDevice(h->device));
+    return ORGPU_OK;
+}
+
+extern "C" const char *orgpu_last_error(void) { return g_err; }
+
+extern "C" int orgpu_create(int device, void *hip_stream, orgpu_t **out)
+{
+    if (!out) return orgpu_fail(ORGPU_EINVAL, "create: null output pointer");
+    *out = nullptr;
+    int n = 0;
+    HIP_CHECK_RET(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return orgpu_fail(ORGPU_EINVAL, "create: device %d of %d", device, n);
+    HIP_CHECK_RET(hipSetDevice(device));
+    orgpu_t *h = new (std::nothrow) orgpu_ctx();
+    if (!h) return orgpu_fail(ORGPU_ENOMEM, "create: host allocation failed");
+    h->device = device;
+    h->stream = (hipStream_t)hip_stream;
+    h->en.device = device;
+    h->en.stream = h->stream;
+    *out = h;
+    return ORGPU_OK;
+}
+
+extern "C" int orgpu_destroy(orgpu_t *h)
+{
+    if (!h) return ORGPU_OK;
+    hipSetDevice(h->device);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    else hipDeviceSynchronize();
+    engine_release(h->en);
+    delete h;
+    return ORGPU_OK;
+}
+
+extern "C" int orgpu_sync(orgpu_t *h)
+{
+    if (!h) return orgpu_fail(ORGPU_EINVAL, "sync: null handle");
+    HIP_CHECK_RET(hipSetDevice(h->device));
+    HIP_CHECK_RET(hipStreamSynchronize(h->stream));
+    return ORGPU_OK;
+}
+
+extern "C" int orgpu_reserve(orgpu_t *h, int64_t n_nodes, int32_t slots, int32_t n_buckets,
+                             int64_t queue_capacity, int32_t max_path_len)
+{
+    if (!h) return orgpu_fail(ORGPU_EINVAL, "reserve: null handle");
+    HIP_CHECK_RET(hipSetDevice(h->device));
+    HIP_CHECK_RET(hipStreamSynchronize(h->stream));
+    return engine_reserve(h->en, n_nodes, slots, n_buckets, queue_capacity, max_path_len);
+}
+
+extern "C" int orgpu_route(orgpu_t *h, const orgpu_graph *g, const orgpu_net_req *nets, int32_t n_nets,
+                           const uint32_t *roi_bitmaps, int32_t n_roi, const int32_t *keepout_owner,
+                           float keepout_cost,
+                           const orgpu_sssp_params *params, int32_t *path_nodes, int32_t *path_edges,
+                           int64_t path_capacity, int64_t *path_offset, float *path_dist,
+                           orgpu_sssp_stats *stats)
+{
+    if (!h) return orgpu_fail(ORGPU_EINVAL, "route: null handle");
+    if (n_nets < 0) return orgpu_fail(ORGPU_EINVAL, "route: negative request count");
+    HIP_CHECK_RET(hipSetDevice(h->device));
+    if (h->en.cap_slots == 0) return orgpu_fail(ORGPU_EINVAL, "route: call orgpu_reserve first");
+    if (n_nets == 0) {
+        if (path_offset) path_offset[0] = 0;
+        return ORGPU_OK;
+    }
+    if (!path_nodes) return orgpu_fail(ORGPU_EINVAL, "route: null path output");
+    return engine_route(h->en, g, nets, n_nets, roi_bitmaps, n_roi, keepout_owner, keepout_cost, params,
+                        path_nodes, path_edges, path_capacity, path_offset, path_dist, stats);
+}

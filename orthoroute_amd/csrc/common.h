@@ -1,0 +1,116 @@
+// Shared device/host definitions of the orgpu library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/orgpu.h"
+
+// ---------------------------------------------------------------------------
+// Packed search key (one u64 per (slot, node)), minimised with atomicMin:
+//
+//   bits 63..59  keepout edges on the path (saturating at 31)
+//   bits 58..28  FP32 bit pattern of the path cost without keepout edges
+//                (non-negative floats order like their bit patterns)
+//   bits 27..0   predecessor node index (lowest index wins ties)
+//
+// The reference replaces the cost of a keepout edge by 1e9
+// (UPF:3182, 3220) and sums in float64; with path costs far below 1e9 that
+// orders paths by (number of keepout edges, remaining cost), which is what the
+// two top fields encode without the FP32 absorption a 1e9 summand would cause.
+// ---------------------------------------------------------------------------
+#define KEY_INF 0xFFFFFFFFFFFFFFFFull
+#define KEY_PRED_BITS 28
+#define KEY_PRED_MASK 0x0FFFFFFFu
+#define KEY_NOPRED 0x0FFFFFFFu
+#define KEY_MAX_NODES (1ll << 28) - 1
+
+__host__ __device__ inline uint64_t key_pack(uint32_t ko, float d, uint32_t pred)
+{
+    uint32_t db;
+    __builtin_memcpy(&db, &d, 4);
+    return ((uint64_t)ko << 59) | ((uint64_t)(db & 0x7FFFFFFFu) << 28) | (uint64_t)(pred & KEY_PRED_MASK);
+}
+__host__ __device__ inline uint32_t key_ko(uint64_t k) { return (uint32_t)(k >> 59); }
+__host__ __device__ inline float key_dist(uint64_t k)
+{
+    uint32_t db = (uint32_t)((k >> 28) & 0x7FFFFFFFu);
+    float d;
+    __builtin_memcpy(&d, &db, 4);
+    return d;
+}
+__host__ __device__ inline uint32_t key_pred(uint64_t k) { return (uint32_t)(k & KEY_PRED_MASK); }
+// (keepout count, distance) part, comparable as an integer
+__host__ __device__ inline uint64_t key_cost(uint64_t k) { return k >> 28; }
+
+// bucket of a key relative to the slot's base bucket
+__host__ __device__ inline uint64_t key_bucket_rel(uint64_t k, float inv_delta)
+{
+    float q = key_dist(k) * inv_delta;
+    uint32_t b = q >= 4.0e9f ? 4000000000u : (uint32_t)q;
+    return ((uint64_t)key_ko(k) << 32) | b;
+}
+
+// ---------------------------------------------------------------------------
+// Search engine layout
+// ---------------------------------------------------------------------------
+#define MAX_RING 32
+// physical queue buffers: ring [0, nb), OUT, OVF, OVF_NEXT
+#define MAX_BUFS (MAX_RING + 3)
+
+enum { MODE_IDLE = 0, MODE_RELAX = 1, MODE_REDIST = 2 };
+
+struct Ctrl {
+    uint64_t cur;              // current global bucket
+    uint64_t ovf_min;          // lower bound of buckets held in OVF
+    uint32_t mode;
+    uint32_t finished;         // last-workgroup counter
+    uint32_t ring[MAX_RING];   // ring position -> physical buffer
+    uint32_t out_buf, ovf_buf, ovf_next_buf;
+    uint32_t overflow;         // a queue exceeded its capacity
+    uint32_t count[MAX_BUFS];  // items per physical buffer
+    uint32_t active;           // slots seeded and not finished
+    uint32_t epoch;            // bumps when any slot finishes
+    uint64_t rounds, items, edges;
+};
+
+struct SlotInfo {
+    uint64_t base;      // bucket of distance 0 for this slot's search
+    int32_t src, dst;
+    int32_t net_id;
+    int32_t roi_index;  // -1: whole graph
+    int32_t use_keepout;
+    uint32_t state;     // 0 free/draining, 1 searching, 2 dst reached, 3 dst unreachable
+    int32_t req;        // request index
+    int32_t pending;    // items of this slot still queued (pushes - pops)
+};
+
+struct RoundArgs {
+    Ctrl *ctrl;
+    SlotInfo *slots;
+    int32_t K;
+    int64_t N;
+    uint32_t nb;
+    float inv_delta;
+    const int32_t *indptr;
+    const int32_t *indices;
+    const float *cost;
+    const uint8_t *kind;
+    const uint32_t *roi;      // [n_roi][words]
+    int64_t roi_words;
+    const int32_t *ko_owner;  // [N] or null
+    uint64_t *keys;           // [K][N]
+    uint32_t *queues;         // [MAX_BUFS][qcap]
+    uint64_t qcap;
+    uint32_t *bits;           // [MAX_BUFS][bitmap_words]
+    uint64_t bitmap_words;
+};
+
+#define HIP_CHECK_RET(expr)                                                   \
+    do {                                                                      \
+        hipError_t _e = (expr);                                               \
+        if (_e != hipSuccess) return orgpu_fail_hip(_e, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+int orgpu_fail(int code, const char *fmt, ...);
+int orgpu_fail_hip(hipError_t e, const char *what, const char *file, int line);

@@ -1,0 +1,55 @@
+// Host-side state of one orgpu handle: stream and the search engine scratch.
+#pragma once
+
+#include "common.h"
+
+struct SeedReq;
+
+struct Engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // capacities reserved
+    int64_t cap_nodes = 0, cap_q = 0;
+    int32_t cap_slots = 0, cap_nb = 0, cap_path = 0;
+    uint64_t words = 0;
+    // device scratch
+    uint64_t *keys = nullptr;
+    uint32_t *queues = nullptr;
+    uint32_t *bits = nullptr;
+    Ctrl *ctrl = nullptr;
+    SlotInfo *slots = nullptr;
+    SeedReq *seed = nullptr;
+    int32_t *slot_list = nullptr;
+    int32_t *stage_nodes = nullptr, *stage_edges = nullptr;
+    int32_t *lens = nullptr;
+    float *dists = nullptr;
+    unsigned long long *counters = nullptr;
+    // pinned host mirrors
+    Ctrl *h_ctrl = nullptr;
+    SlotInfo *h_slots = nullptr;
+    SeedReq *h_seed = nullptr;
+    int32_t *h_slot_list = nullptr, *h_lens = nullptr, *h_stage = nullptr;
+    float *h_dists = nullptr;
+    unsigned long long *h_counters = nullptr;
+    hipEvent_t ev_poll = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
+    // congestion-state helpers: nonzero staging and host->device uploads
+    int32_t *sp_idx = nullptr;
+    float *sp_val = nullptr;
+    unsigned long long *sp_count = nullptr, *h_sp_count = nullptr;
+    int64_t sp_cap = 0;
+    int32_t *up_buf = nullptr;
+    int64_t up_cap = 0;
+};
+
+struct orgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Engine en;
+};
+
+int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets, int64_t qcap, int32_t max_path);
+void engine_release(Engine &en);
+int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, int32_t n_nets,
+                 const uint32_t *roi_bitmaps, int32_t n_roi, const int32_t *keepout_owner, float keepout_cost,
+                 const orgpu_sssp_params *p, int32_t *path_nodes, int32_t *path_edges,
+                 int64_t path_capacity, int64_t *path_offset, float *path_dist, orgpu_sssp_stats *stats);

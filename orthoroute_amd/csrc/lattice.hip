@@ -1,0 +1,187 @@
+// A1: Manhattan lattice -> CSR, one thread per node, closed-form row offsets.
+//
+// Reference: Lattice3D.build_graph + CSRGraph.finalize (UPF:1195-1338,
+// 647-792).  The reference inserts edges layer by layer and via pair by via
+// pair, then sorts the (src, dst, cost) records; every row therefore ends up
+// ordered by (dst, cost).  For node (x, y, z) that order is: via targets on
+// lower layers, the planar neighbours y-1, x-1, x+1, y+1 (those its layer
+// direction allows), via targets on higher layers.  Row offsets follow from
+// per-layer closed forms, so no scan or sort is needed: the kernel is a pure
+// streaming write (10 bytes per edge + 4 bytes per node).
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+
+#define MAX_VIA_PER_LAYER 128
+
+struct LatticeTables {
+    int32_t X, Y, L;
+    float pitch;
+    int64_t E;
+    int64_t layer_off[ORGPU_MAX_LAYERS];
+    int32_t nvia[ORGPU_MAX_LAYERS];
+    int32_t nbelow[ORGPU_MAX_LAYERS];
+    uint8_t planar[ORGPU_MAX_LAYERS];
+    uint8_t dir[ORGPU_MAX_LAYERS];
+};
+
+struct ViaEntry {
+    int32_t z;
+    float cost;
+};
+
+static int64_t planar_layer_edges(const orgpu_lattice_desc *d, int z)
+{
+    if (!d->planar[z]) return 0;
+    int64_t e = 0;
+    if (d->dir[z] == 1 || d->dir[z] == 2) e += 2LL * d->Y * (d->X - 1);
+    if (d->dir[z] == 0 || d->dir[z] == 2) e += 2LL * d->X * (d->Y - 1);
+    return e;
+}
+
+static int validate_desc(const orgpu_lattice_desc *d)
+{
+    if (!d) return orgpu_fail(ORGPU_EINVAL, "lattice: null descriptor");
+    if (d->X < 1 || d->Y < 1 || d->L < 1 || d->L > ORGPU_MAX_LAYERS)
+        return orgpu_fail(ORGPU_EINVAL, "lattice: bad shape %d x %d x %d", d->X, d->Y, d->L);
+    int64_t N = (int64_t)d->X * d->Y * d->L;
+    if (N >= KEY_MAX_NODES) return orgpu_fail(ORGPU_EINVAL, "lattice: %lld nodes exceed 2^28", (long long)N);
+    if (d->npairs < 0 || (d->npairs > 0 && (!d->pair_from || !d->pair_to)))
+        return orgpu_fail(ORGPU_EINVAL, "lattice: bad via pair list");
+    for (int p = 0; p < d->npairs; ++p) {
+        int a = d->pair_from[p], b = d->pair_to[p];
+        if (a < 0 || b < 0 || a >= d->L || b >= d->L || a == b)
+            return orgpu_fail(ORGPU_EINVAL, "lattice: bad via pair (%d, %d)", a, b);
+    }
+    for (int z = 0; z < d->L; ++z)
+        if (d->dir[z] > 2) return orgpu_fail(ORGPU_EINVAL, "lattice: bad direction code %d", d->dir[z]);
+    return ORGPU_OK;
+}
+
+extern "C" int orgpu_lattice_count(const orgpu_lattice_desc *d, int64_t *n_nodes, int64_t *n_edges)
+{
+    int rc = validate_desc(d);
+    if (rc) return rc;
+    int64_t E = 0;
+    for (int z = 0; z < d->L; ++z) E += planar_layer_edges(d, z);
+    E += 2LL * d->X * d->Y * d->npairs;
+    if (E >= (1ll << 31)) return orgpu_fail(ORGPU_EINVAL, "lattice: %lld edges exceed int32 CSR", (long long)E);
+    if (n_nodes) *n_nodes = (int64_t)d->X * d->Y * d->L;
+    if (n_edges) *n_edges = E;
+    return ORGPU_OK;
+}
+
+__global__ void __launch_bounds__(256)
+k_lattice_build(LatticeTables t, const ViaEntry *__restrict__ via, int32_t *__restrict__ indptr,
+                int32_t *__restrict__ indices, float *__restrict__ cost, uint8_t *__restrict__ layer,
+                uint8_t *__restrict__ kind)
+{
+    const int64_t plane = (int64_t)t.X * t.Y;
+    const int64_t N = plane * t.L;
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(n / plane);
+        const int64_t r = n - (int64_t)z * plane;
+        const int y = (int)(r / t.X);
+        const int x = (int)(r - (int64_t)y * t.X);
+        const int pl = t.planar[z];
+        const int dz = t.dir[z];
+        const bool hx = pl && (dz == 1 || dz == 2);
+        const bool vy = pl && (dz == 0 || dz == 2);
+        int64_t off = t.layer_off[z] + (int64_t)t.nvia[z] * r;
+        if (hx) off += (int64_t)y * 2 * (t.X - 1) + (x > 0 ? x - 1 : 0) + (x < t.X - 1 ? x : t.X - 1);
+        if (vy) {
+            const int vdeg = (y > 0) + (y < t.Y - 1);
+            off += (int64_t)t.X * ((y > 0 ? y - 1 : 0) + (y < t.Y - 1 ? y : t.Y - 1)) + (int64_t)x * vdeg;
+        }
+        indptr[n] = (int32_t)off;
+        if (n == N - 1) indptr[N] = (int32_t)t.E;
+        int64_t e = off;
+        const ViaEntry *vz = via + (int64_t)z * MAX_VIA_PER_LAYER;
+        const int nb = t.nbelow[z], nv = t.nvia[z];
+        for (int j = 0; j < nb; ++j) {
+            indices[e] = (int32_t)((int64_t)vz[j].z * plane + r);
+            cost[e] = vz[j].cost; layer[e] = (uint8_t)z; kind[e] = 1; ++e;
+        }
+        if (vy && y > 0) { indices[e] = (int32_t)(n - t.X); cost[e] = t.pitch; layer[e] = (uint8_t)z; kind[e] = 0; ++e; }
+        if (hx && x > 0) { indices[e] = (int32_t)(n - 1); cost[e] = t.pitch; layer[e] = (uint8_t)z; kind[e] = 0; ++e; }
+        if (hx && x < t.X - 1) { indices[e] = (int32_t)(n + 1); cost[e] = t.pitch; layer[e] = (uint8_t)z; kind[e] = 0; ++e; }
+        if (vy && y < t.Y - 1) { indices[e] = (int32_t)(n + t.X); cost[e] = t.pitch; layer[e] = (uint8_t)z; kind[e] = 0; ++e; }
+        for (int j = nb; j < nv; ++j) {
+            indices[e] = (int32_t)((int64_t)vz[j].z * plane + r);
+            cost[e] = vz[j].cost; layer[e] = (uint8_t)z; kind[e] = 1; ++e;
+        }
+    }
+}
+
+struct orgpu_ctx;
+hipStream_t orgpu_stream(orgpu_t *h);
+int orgpu_set_device(orgpu_t *h);
+
+extern "C" int orgpu_lattice_build(orgpu_t *h, const orgpu_lattice_desc *d, int32_t *indptr,
+                                   int32_t *indices, float *base_cost, uint8_t *edge_layer,
+                                   uint8_t *edge_kind)
+{
+    if (!h) return orgpu_fail(ORGPU_EINVAL, "lattice_build: null handle");
+    int64_t N = 0, E = 0;
+    int rc = orgpu_lattice_count(d, &N, &E);
+    if (rc) return rc;
+    if (!indptr || (E > 0 && (!indices || !base_cost || !edge_layer || !edge_kind)))
+        return orgpu_fail(ORGPU_EINVAL, "lattice_build: null output array");
+    rc = orgpu_set_device(h);
+    if (rc) return rc;
+
+    // per-layer via targets, sorted by (target layer, cost) like the CSR rows
+    std::vector<std::vector<ViaEntry>> per(d->L);
+    for (int p = 0; p < d->npairs; ++p) {
+        int a = d->pair_from[p], b = d->pair_to[p];
+        int span = a > b ? a - b : b - a;
+        float c = (float)(d->via_cost * (1.0 + 0.15 * (double)(span - 1)));
+        per[a].push_back({b, c});
+        per[b].push_back({a, c});
+    }
+    LatticeTables t;
+    memset(&t, 0, sizeof(t));
+    t.X = d->X; t.Y = d->Y; t.L = d->L; t.pitch = d->pitch; t.E = E;
+    std::vector<ViaEntry> table((size_t)d->L * MAX_VIA_PER_LAYER, ViaEntry{0, 0.0f});
+    int64_t acc = 0;
+    const int64_t plane = (int64_t)d->X * d->Y;
+    for (int z = 0; z < d->L; ++z) {
+        auto &v = per[z];
+        std::stable_sort(v.begin(), v.end(), [](const ViaEntry &a, const ViaEntry &b) {
+            return a.z < b.z || (a.z == b.z && a.cost < b.cost);
+        });
+        if ((int)v.size() > MAX_VIA_PER_LAYER)
+            return orgpu_fail(ORGPU_ECAPACITY, "lattice: %zu via edges per node on layer %d (max %d)",
+                              v.size(), z, MAX_VIA_PER_LAYER);
+        t.nvia[z] = (int32_t)v.size();
+        int nb = 0;
+        for (auto &e : v) nb += e.z < z;
+        t.nbelow[z] = nb;
+        t.planar[z] = d->planar[z];
+        t.dir[z] = d->dir[z];
+        for (size_t j = 0; j < v.size(); ++j) table[(size_t)z * MAX_VIA_PER_LAYER + j] = v[j];
+        t.layer_off[z] = acc;
+        acc += planar_layer_edges(d, z) + (int64_t)t.nvia[z] * plane;
+    }
+    if (acc != E) return orgpu_fail(ORGPU_EINVAL, "lattice: internal edge count mismatch");
+
+    hipStream_t s = orgpu_stream(h);
+    ViaEntry *dvia = nullptr;
+    HIP_CHECK_RET(hipMalloc((void **)&dvia, table.size() * sizeof(ViaEntry)));
+    HIP_CHECK_RET(hipMemcpyAsync(dvia, table.data(), table.size() * sizeof(ViaEntry), hipMemcpyHostToDevice, s));
+    int64_t blocks = (N + 255) / 256;
+    if (blocks > 65535 * 8) blocks = 65535 * 8;
+    hipLaunchKernelGGL(k_lattice_build, dim3((unsigned)blocks), dim3(256), 0, s, t, dvia, indptr, indices,
+                       base_cost, edge_layer, edge_kind);
+    HIP_CHECK_RET(hipGetLastError());
+    // the host table must outlive the async copy
+    HIP_CHECK_RET(hipStreamSynchronize(s));
+    HIP_CHECK_RET(hipFree(dvia));
+    return ORGPU_OK;
+}

@@ -1,0 +1,763 @@
+// A3/A4: batched exact SSSP (bucketed delta-stepping on packed 64-bit keys)
+// and the backtrace, for K concurrent searches ("slots") on one CSR graph.
+//
+// Reference behaviour replaced: SimpleDijkstra.find_path_roi (UPF:1801-1887),
+// the owner-aware keepout (UPF:3150-3228), the parent walk (UPF:1880-1887)
+// and _path_to_edges (UPF:5463-5476).  The result is the FP32 shortest path
+// with the lowest-predecessor-index tie-break (see common.h for the key).
+//
+// Design (one HIP stream, no host sync inside a round):
+//  * state: keys u64[K][N]; work items are u32 (slot*N + node).
+//  * buckets: a ring of nb queues for buckets cur .. cur+nb-1, an OUT queue
+//    for re-relaxations inside the current bucket and an overflow queue for
+//    buckets >= cur+nb.  Each queue has a dedupe bitmap (one bit per item).
+//  * one kernel launch = one round.  The last workgroup to finish (atomic
+//    counter) runs the control step: swap OUT in, advance the bucket, detect
+//    finished slots, schedule an overflow redistribution or go idle.  The host
+//    queues rounds blindly and polls a status block every few dozen rounds.
+//  * every slot shares the global bucket index; a slot seeded at bucket c
+//    measures its distances from c (per-slot base), so finished slots are
+//    refilled with the next request while the others keep running.
+//  * a wavefront expands 64 items at once: it scans their degrees, then each
+//    lane takes one edge of the flattened edge list (consecutive lanes read
+//    consecutive CSR entries), relaxes it with a 64-bit atomicMin, and the
+//    improved targets are appended with one atomicAdd per wave and queue
+//    (ballot + popcount compaction).
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+
+#define WG 256
+#define WAVES_PER_WG (WG / 64)
+#define MAX_SLOTS 1024
+
+__device__ __forceinline__ uint64_t atomic_load_u64(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// agent-scope loads for values other workgroups of the same launch updated
+__device__ __forceinline__ uint32_t ld32(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t logical_block()
+{
+    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of logical blocks (and so of queue items) so that
+    // neighbouring items share that XCD's L2.
+    const uint32_t g = gridDim.x, b = blockIdx.x;
+    if (g % 8 != 0) return b;
+    return (b % 8) * (g / 8) + b / 8;
+}
+
+struct PushCtx {
+    uint32_t out_buf, ovf_buf;
+    uint32_t ring[MAX_RING];
+};
+
+// Wave-aggregated append: one atomicAdd per (wave, destination queue).
+__device__ __forceinline__ void wave_push(const RoundArgs &a, bool want, uint32_t buf, uint32_t item)
+{
+    uint64_t pending = __ballot(want);
+    const uint32_t lane = lane_id();
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lb = __shfl(buf, leader);
+        const bool mine = want && buf == lb;
+        const uint64_t grp = __ballot(mine);
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&a.ctrl->count[lb], (uint32_t)__popcll(grp));
+        base = __shfl(base, leader);
+        if (mine) {
+            const uint64_t pos = (uint64_t)base + (uint64_t)__popcll(grp & ((1ull << lane) - 1ull));
+            if (pos < a.qcap) a.queues[(uint64_t)lb * a.qcap + pos] = item;
+            else a.ctrl->overflow = 1;
+            want = false;
+        }
+        pending &= ~grp;
+    }
+}
+
+__device__ __forceinline__ bool test_and_set_bit(uint32_t *bm, uint32_t item)
+{
+    const uint32_t m = 1u << (item & 31);
+    return (atomicOr(&bm[item >> 5], m) & m) != 0;
+}
+
+// inclusive wave scan
+__device__ __forceinline__ int wave_scan(int x)
+{
+    const int lane = (int)lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(x, o, 64);
+        if (lane >= o) x += t;
+    }
+    return x;
+}
+
+__device__ void control_step(const RoundArgs &a, uint32_t mode, uint64_t cur);
+
+__global__ void __launch_bounds__(WG)
+k_round(RoundArgs a)
+{
+    Ctrl *c = a.ctrl;
+    const uint32_t mode = __hip_atomic_load(&c->mode, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (mode == MODE_IDLE) return;
+    const uint64_t cur = c->cur;
+    const uint32_t nb = a.nb;
+
+    __shared__ int32_t s_slot_delta[MAX_SLOTS];
+    __shared__ uint32_t s_ring[MAX_RING];
+    __shared__ uint32_t s_out, s_ovf, s_ovfn, s_in;
+    __shared__ unsigned long long s_items, s_edges;
+    __shared__ int s_last;
+    for (int i = threadIdx.x; i < a.K; i += WG) s_slot_delta[i] = 0;
+    if (threadIdx.x < nb) s_ring[threadIdx.x] = c->ring[threadIdx.x];
+    if (threadIdx.x == 0) {
+        s_out = c->out_buf; s_ovf = c->ovf_buf; s_ovfn = c->ovf_next_buf;
+        s_in = mode == MODE_RELAX ? c->ring[cur % nb] : c->ovf_buf;
+        s_items = 0; s_edges = 0;
+    }
+    __syncthreads();
+
+    const uint32_t in_buf = s_in;
+    // a queue that overflowed holds only its first qcap items (the host
+    // reports the overflow); never read past them
+    const uint32_t n_in = (uint32_t)std::min<uint64_t>(c->count[in_buf], a.qcap);
+    const uint32_t *qin = a.queues + (uint64_t)in_buf * a.qcap;
+    uint32_t *bits_in = a.bits + (uint64_t)in_buf * a.bitmap_words;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x / 64;
+    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES_PER_WG;
+    const uint64_t lw = (uint64_t)logical_block() * WAVES_PER_WG + wave;
+    const uint64_t chunk = ((n_in + nwaves * 64 - 1) / (nwaves * 64)) * 64;
+    const uint64_t start = lw * chunk;
+    const uint64_t end = std::min<uint64_t>(start + chunk, n_in);
+    const uint64_t N = (uint64_t)a.N;
+    unsigned long long my_items = 0, my_edges = 0;
+
+    for (uint64_t base = start; base < end; base += 64) {
+        const uint64_t i = base + lane;
+        uint32_t item = 0, slot = 0, u = 0;
+        uint64_t ku = KEY_INF;
+        int row = 0, deg = 0;
+        bool live = false;
+        if (i < end) {
+            item = qin[i];
+            atomicAnd(&bits_in[item >> 5], ~(1u << (item & 31)));
+            slot = item / (uint32_t)N;
+            u = item - slot * (uint32_t)N;
+            atomicAdd(&s_slot_delta[slot], -1);
+            const SlotInfo si = a.slots[slot];
+            if (si.state == 1) {
+                ku = atomic_load_u64(&a.keys[item]);
+                if (ku != KEY_INF) {
+                    const uint64_t b = si.base + key_bucket_rel(ku, a.inv_delta);
+                    if (mode == MODE_RELAX) {
+                        if (b == cur) {
+                            row = a.indptr[u];
+                            deg = a.indptr[u + 1] - row;
+                            live = true;
+                            my_items++;
+                        }
+                    } else if (b >= cur) {
+                        // redistribution: move into the ring or keep in overflow
+                        uint32_t dst;
+                        if (b < cur + nb) dst = s_ring[b % nb];
+                        else { dst = s_ovfn; atomicMin((unsigned long long *)&c->ovf_min, (unsigned long long)b); }
+                        live = !test_and_set_bit(a.bits + (uint64_t)dst * a.bitmap_words, item);
+                        if (live) atomicAdd(&s_slot_delta[slot], 1);
+                        // reuse `row` to carry the destination queue id
+                        row = (int)dst;
+                    }
+                }
+            }
+        }
+        if (mode == MODE_REDIST) {
+            wave_push(a, live, (uint32_t)row, item);
+            continue;
+        }
+        // ---- relax: flatten the edges of the 64 items over the lanes ----
+        const int incl = wave_scan(deg);
+        const int total = __shfl(incl, 63);
+        const int excl = incl - deg;
+        my_edges += (unsigned long long)deg;
+        for (int t0 = 0; t0 < total; t0 += 64) {
+            const int t = t0 + (int)lane;
+            int j = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const int cand = j + step;
+                const int ec = __shfl(excl, cand);
+                if (ec <= t) j = cand;
+            }
+            const int row_j = __shfl(row, j);
+            const int excl_j = __shfl(excl, j);
+            const uint32_t item_j = __shfl(item, j);
+            const uint32_t slot_j = __shfl(slot, j);
+            const uint32_t ulo = __shfl((uint32_t)ku, j), uhi = __shfl((uint32_t)(ku >> 32), j);
+            const uint64_t ku_j = ((uint64_t)uhi << 32) | ulo;
+            bool want = false;
+            uint32_t buf = 0, vitem = 0;
+            if (t < total) {
+                const int e = row_j + (t - excl_j);
+                const uint32_t v = (uint32_t)a.indices[e];
+                const uint32_t u_j = item_j - slot_j * (uint32_t)N;
+                const SlotInfo si = a.slots[slot_j];
+                bool ok = true;
+                if (si.roi_index >= 0) {
+                    const uint32_t w = a.roi[(uint64_t)si.roi_index * a.roi_words + (v >> 5)];
+                    ok = (w >> (v & 31)) & 1u;
+                }
+                if (ok) {
+                    uint32_t ko = key_ko(ku_j);
+                    float nd = key_dist(ku_j);
+                    if (si.use_keepout && a.ko_owner) {
+                        const int32_t own = a.ko_owner[u_j];
+                        if (own >= 0 && own != si.net_id && a.kind[e] == 0) {
+                            ko = ko < 31 ? ko + 1 : 31;
+                        } else {
+                            nd = nd + a.cost[e];
+                        }
+                    } else {
+                        nd = nd + a.cost[e];
+                    }
+                    if (nd < __builtin_inff()) {
+                        const uint64_t nk = key_pack(ko, nd, u_j);
+                        vitem = slot_j * (uint32_t)N + v;
+                        uint64_t *kp = &a.keys[vitem];
+                        if (nk < *kp) {
+                            const uint64_t old = atomicMin((unsigned long long *)kp, (unsigned long long)nk);
+                            if (nk < old && key_cost(nk) < key_cost(old)) {
+                                uint64_t b = si.base + key_bucket_rel(nk, a.inv_delta);
+                                if (b <= cur) buf = s_out;
+                                else if (b < cur + nb) buf = s_ring[b % nb];
+                                else {
+                                    buf = s_ovf;
+                                    atomicMin((unsigned long long *)&c->ovf_min, (unsigned long long)b);
+                                }
+                                want = !test_and_set_bit(a.bits + (uint64_t)buf * a.bitmap_words, vitem);
+                                if (want) atomicAdd(&s_slot_delta[slot_j], 1);
+                            }
+                        }
+                    }
+                }
+            }
+            wave_push(a, want, buf, vitem);
+        }
+    }
+
+    // ---- per-workgroup flush, then the last workgroup runs the control step
+    atomicAdd(&s_items, my_items);
+    atomicAdd(&s_edges, my_edges);
+    __syncthreads();
+    for (int s = threadIdx.x; s < a.K; s += WG) {
+        const int dlt = s_slot_delta[s];
+        if (dlt != 0) atomicAdd(&a.slots[s].pending, dlt);
+    }
+    // every thread's queue appends and counter updates must be visible
+    // before this workgroup counts itself finished
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_items) atomicAdd((unsigned long long *)&c->items, s_items);
+        if (s_edges) atomicAdd((unsigned long long *)&c->edges, s_edges);
+        __threadfence();
+        const uint32_t t = atomicAdd(&c->finished, 1u);
+        s_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (s_last) {
+        __threadfence();
+        control_step(a, mode, cur);
+    }
+}
+
+// Runs in one workgroup after every other workgroup of the round finished.
+__device__ void control_step(const RoundArgs &a, uint32_t mode, uint64_t cur)
+{
+    Ctrl *c = a.ctrl;
+    const uint32_t nb = a.nb;
+    __shared__ uint64_t s_newcur;
+    __shared__ uint32_t s_ring_total, s_mode, s_any_active;
+    if (threadIdx.x == 0) {
+        uint64_t newcur = cur;
+        if (mode == MODE_RELAX) {
+            const uint32_t pos = (uint32_t)(cur % nb);
+            const uint32_t in = c->ring[pos];
+            c->count[in] = 0;
+            if (ld32(&c->count[c->out_buf]) > 0) {
+                c->ring[pos] = c->out_buf;
+                c->out_buf = in;
+                s_mode = MODE_RELAX;
+            } else {
+                newcur = cur + 1;
+                s_mode = 0xFFFFFFFFu; // decide below
+            }
+        } else {
+            // redistribution consumed OVF; OVF_NEXT holds what stays
+            c->count[c->ovf_buf] = 0;
+            const uint32_t t = c->ovf_buf;
+            c->ovf_buf = c->ovf_next_buf;
+            c->ovf_next_buf = t;
+            s_mode = 0xFFFFFFFFu;
+        }
+        uint32_t tot = 0;
+        for (uint32_t p = 0; p < nb; ++p) tot += ld32(&c->count[c->ring[p]]);
+        tot += ld32(&c->count[c->out_buf]);
+        s_ring_total = tot;
+        if (s_mode == 0xFFFFFFFFu) {
+            const uint32_t ovf = ld32(&c->count[c->ovf_buf]);
+            if (tot == 0) {
+                if (ovf == 0) {
+                    s_mode = MODE_IDLE;
+                } else {
+                    // nothing inside the window: jump to the lowest overflow
+                    // bucket and redistribute
+                    const uint64_t m = atomic_load_u64(&c->ovf_min);
+                    if (m > newcur) newcur = m;
+                    s_mode = MODE_REDIST;
+                }
+            } else if (mode == MODE_RELAX && ovf > 0 && newcur % nb == 0) {
+                // the window slid by a full ring: pull overflow items into it
+                s_mode = MODE_REDIST;
+            } else {
+                s_mode = MODE_RELAX;
+            }
+        }
+        s_newcur = newcur;
+        s_any_active = 0;
+    }
+    __syncthreads();
+    const uint64_t newcur = s_newcur;
+    // slot completion: dst settled (its bucket is below the new current
+    // bucket), or no pending items left (dst unreachable)
+    for (int s = threadIdx.x; s < a.K; s += WG) {
+        SlotInfo *si = &a.slots[s];
+        if (si->state != 1) continue;
+        const uint64_t kd = atomic_load_u64(&a.keys[(uint64_t)s * (uint64_t)a.N + (uint64_t)si->dst]);
+        bool found = false;
+        if (kd != KEY_INF && si->base + key_bucket_rel(kd, a.inv_delta) < newcur) found = true;
+        if (found) { si->state = 2; atomicSub(&c->active, 1u); atomicAdd(&c->epoch, 1u); }
+        else if ((int32_t)ld32((const uint32_t *)&si->pending) <= 0) { si->state = 3; atomicSub(&c->active, 1u); atomicAdd(&c->epoch, 1u); }
+        else atomicOr(&s_any_active, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        c->rounds++;
+        c->cur = newcur;
+        const uint32_t m = s_mode;
+        if (m == MODE_REDIST) c->ovf_min = ~0ull;
+        c->finished = 0;
+        __threadfence();
+        __hip_atomic_store(&c->mode, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// slot management
+// ---------------------------------------------------------------------------
+
+__global__ void k_fill_keys(uint64_t *keys, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        keys[i] = KEY_INF;
+}
+
+struct SeedReq {
+    int32_t slot, src, dst, net_id, roi_index, use_keepout, req, pad;
+};
+
+__global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n)
+{
+    Ctrl *c = a.ctrl;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const SeedReq r = reqs[i];
+        SlotInfo *si = &a.slots[r.slot];
+        si->base = c->cur;
+        si->src = r.src; si->dst = r.dst; si->net_id = r.net_id;
+        si->roi_index = r.roi_index; si->use_keepout = r.use_keepout;
+        si->req = r.req;
+        si->state = 1;
+        const uint32_t item = (uint32_t)r.slot * (uint32_t)a.N + (uint32_t)r.src;
+        a.keys[item] = key_pack(0, 0.0f, KEY_NOPRED);
+        const uint32_t buf = c->ring[c->cur % a.nb];
+        const uint32_t m = 1u << (item & 31);
+        uint32_t *bm = a.bits + (uint64_t)buf * a.bitmap_words;
+        // a slot is reseeded only once its previous search has no queued
+        // items left, so the item's dedupe bit is clear here
+        si->pending = 1;
+        if (!(atomicOr(&bm[item >> 5], m) & m)) {
+            const uint32_t pos = atomicAdd(&c->count[buf], 1u);
+            if (pos < a.qcap) a.queues[(uint64_t)buf * a.qcap + pos] = item;
+            else c->overflow = 1;
+        }
+        atomicAdd(&c->active, 1u);
+    }
+    __syncthreads();
+    if (i == 0) {
+        __threadfence();
+        c->mode = c->mode == MODE_IDLE ? MODE_RELAX : c->mode;
+    }
+}
+
+// One thread per finished slot: walk predecessors dst -> src into the
+// staging row (reversed), record the CSR edge id of each hop (first match in
+// the sorted row, UPF:5463-5476).  len = 0: unreachable; -1: broken chain or
+// longer than the staging row.
+__global__ void k_backtrace(RoundArgs a, const int32_t *slot_list, int n, int32_t *stage_nodes,
+                            int32_t *stage_edges, int32_t *lens, float *dists, int32_t max_len)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = slot_list[i];
+    const SlotInfo si = a.slots[s];
+    int32_t *pn = stage_nodes + (int64_t)s * max_len;
+    int32_t *pe = stage_edges + (int64_t)s * max_len;
+    const uint64_t *ks = a.keys + (uint64_t)s * (uint64_t)a.N;
+    if (si.state != 2) { lens[s] = 0; dists[s] = __builtin_inff(); return; }
+    dists[s] = key_dist(ks[si.dst]);
+    int len = 0;
+    int32_t cur = si.dst;
+    while (true) {
+        if (len >= max_len) { len = -1; break; }
+        pn[len++] = cur;
+        if (cur == si.src) break;
+        const uint32_t p = key_pred(ks[cur]);
+        if (p == KEY_NOPRED) { len = -1; break; }
+        // first CSR edge p -> cur
+        const int r0 = a.indptr[p], r1 = a.indptr[p + 1];
+        int ee = -1;
+        for (int e = r0; e < r1; ++e)
+            if (a.indices[e] == cur) { ee = e; break; }
+        pe[len - 1] = ee;   // edge entering pn[len-1]
+        cur = (int32_t)p;
+    }
+    lens[s] = len;
+}
+
+// Optional Dijkstra-minimal work count: nodes whose (keepout, distance) is at
+// most the destination's, and the sum of their out-degrees.
+__global__ void k_count_minimal(RoundArgs a, int slot, unsigned long long *out)
+{
+    const SlotInfo si = a.slots[slot];
+    const uint64_t *ks = a.keys + (uint64_t)slot * (uint64_t)a.N;
+    const uint64_t lim = key_cost(ks[si.dst]);
+    unsigned long long nn = 0, ne = 0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < a.N; v += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = ks[v];
+        if (k != KEY_INF && key_cost(k) <= lim) { nn++; ne += (unsigned long long)(a.indptr[v + 1] - a.indptr[v]); }
+    }
+    for (int o = 32; o > 0; o >>= 1) { nn += __shfl_down(nn, o, 64); ne += __shfl_down(ne, o, 64); }
+    if (lane_id() == 0) { atomicAdd(&out[0], nn); atomicAdd(&out[1], ne); }
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+#include "engine.h"
+
+static int launch_round(Engine &en, const RoundArgs &a, int grid)
+{
+    hipLaunchKernelGGL(k_round, dim3(grid), dim3(WG), 0, en.stream, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets, int64_t qcap, int32_t max_path)
+{
+    if (slots < 1 || slots > MAX_SLOTS) return orgpu_fail(ORGPU_EINVAL, "reserve: slots %d outside [1, %d]", slots, MAX_SLOTS);
+    if (n_buckets < 1 || n_buckets > MAX_RING) return orgpu_fail(ORGPU_EINVAL, "reserve: n_buckets %d outside [1, %d]", n_buckets, MAX_RING);
+    if (n_nodes < 1 || n_nodes >= KEY_MAX_NODES) return orgpu_fail(ORGPU_EINVAL, "reserve: bad node count");
+    if ((uint64_t)n_nodes * (uint64_t)slots >= (1ull << 32))
+        return orgpu_fail(ORGPU_EINVAL, "reserve: slots*nodes = %lld exceeds 2^32 work items",
+                          (long long)n_nodes * slots);
+    if (max_path < 2) max_path = 1 << 16;
+    const uint64_t items = (uint64_t)n_nodes * (uint64_t)slots;
+    if (qcap <= 0 || (uint64_t)qcap > items) qcap = (int64_t)items;
+    if (en.cap_nodes >= n_nodes && en.cap_slots >= slots && en.cap_nb >= n_buckets && en.cap_q >= qcap &&
+        en.cap_path >= max_path)
+        return ORGPU_OK;
+    engine_release(en);
+    const uint64_t words = (items + 31) / 32;
+    const int nbufs = n_buckets + 3;
+    HIP_CHECK_RET(hipMalloc((void **)&en.keys, items * sizeof(uint64_t)));
+    HIP_CHECK_RET(hipMalloc((void **)&en.queues, (uint64_t)nbufs * (uint64_t)qcap * sizeof(uint32_t)));
+    HIP_CHECK_RET(hipMalloc((void **)&en.bits, (uint64_t)nbufs * words * sizeof(uint32_t)));
+    HIP_CHECK_RET(hipMemset(en.bits, 0, (uint64_t)nbufs * words * sizeof(uint32_t)));
+    HIP_CHECK_RET(hipMalloc((void **)&en.ctrl, sizeof(Ctrl)));
+    HIP_CHECK_RET(hipMalloc((void **)&en.slots, sizeof(SlotInfo) * slots));
+    HIP_CHECK_RET(hipMalloc((void **)&en.seed, sizeof(SeedReq) * slots));
+    HIP_CHECK_RET(hipMalloc((void **)&en.slot_list, sizeof(int32_t) * slots));
+    HIP_CHECK_RET(hipMalloc((void **)&en.stage_nodes, (size_t)slots * max_path * sizeof(int32_t)));
+    HIP_CHECK_RET(hipMalloc((void **)&en.stage_edges, (size_t)slots * max_path * sizeof(int32_t)));
+    HIP_CHECK_RET(hipMalloc((void **)&en.lens, sizeof(int32_t) * slots));
+    HIP_CHECK_RET(hipMalloc((void **)&en.dists, sizeof(float) * slots));
+    HIP_CHECK_RET(hipMalloc((void **)&en.counters, sizeof(unsigned long long) * 2));
+    HIP_CHECK_RET(hipHostMalloc((void **)&en.h_ctrl, sizeof(Ctrl), hipHostMallocDefault));
+    HIP_CHECK_RET(hipHostMalloc((void **)&en.h_slots, sizeof(SlotInfo) * slots, hipHostMallocDefault));
+    HIP_CHECK_RET(hipHostMalloc((void **)&en.h_seed, sizeof(SeedReq) * slots, hipHostMallocDefault));
+    HIP_CHECK_RET(hipHostMalloc((void **)&en.h_slot_list, sizeof(int32_t) * slots, hipHostMallocDefault));
+    HIP_CHECK_RET(hipHostMalloc((void **)&en.h_lens, sizeof(int32_t) * slots, hipHostMallocDefault));
+    HIP_CHECK_RET(hipHostMalloc((void **)&en.h_dists, sizeof(float) * slots, hipHostMallocDefault));
+    HIP_CHECK_RET(hipHostMalloc((void **)&en.h_stage, (size_t)2 * max_path * sizeof(int32_t), hipHostMallocDefault));
+    HIP_CHECK_RET(hipHostMalloc((void **)&en.h_counters, sizeof(unsigned long long) * 2, hipHostMallocDefault));
+    HIP_CHECK_RET(hipEventCreate(&en.ev_poll));
+    HIP_CHECK_RET(hipEventCreate(&en.ev_t0));
+    HIP_CHECK_RET(hipEventCreate(&en.ev_t1));
+    en.cap_nodes = n_nodes; en.cap_slots = slots; en.cap_nb = n_buckets; en.cap_q = qcap; en.cap_path = max_path;
+    en.words = words;
+    return ORGPU_OK;
+}
+
+void engine_release(Engine &en)
+{
+    if (en.keys) hipFree(en.keys);
+    if (en.queues) hipFree(en.queues);
+    if (en.bits) hipFree(en.bits);
+    if (en.ctrl) hipFree(en.ctrl);
+    if (en.slots) hipFree(en.slots);
+    if (en.seed) hipFree(en.seed);
+    if (en.slot_list) hipFree(en.slot_list);
+    if (en.stage_nodes) hipFree(en.stage_nodes);
+    if (en.stage_edges) hipFree(en.stage_edges);
+    if (en.lens) hipFree(en.lens);
+    if (en.dists) hipFree(en.dists);
+    if (en.counters) hipFree(en.counters);
+    if (en.h_ctrl) hipHostFree(en.h_ctrl);
+    if (en.h_slots) hipHostFree(en.h_slots);
+    if (en.h_seed) hipHostFree(en.h_seed);
+    if (en.h_slot_list) hipHostFree(en.h_slot_list);
+    if (en.h_lens) hipHostFree(en.h_lens);
+    if (en.h_dists) hipHostFree(en.h_dists);
+    if (en.h_stage) hipHostFree(en.h_stage);
+    if (en.h_counters) hipHostFree(en.h_counters);
+    if (en.ev_poll) hipEventDestroy(en.ev_poll);
+    if (en.ev_t0) hipEventDestroy(en.ev_t0);
+    if (en.ev_t1) hipEventDestroy(en.ev_t1);
+    if (en.sp_idx) hipFree(en.sp_idx);
+    if (en.sp_val) hipFree(en.sp_val);
+    if (en.sp_count) hipFree(en.sp_count);
+    if (en.h_sp_count) hipHostFree(en.h_sp_count);
+    if (en.up_buf) hipFree(en.up_buf);
+    Engine fresh;
+    fresh.stream = en.stream;
+    fresh.device = en.device;
+    en = fresh;
+}
+
+int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, int32_t n_nets,
+                 const uint32_t *roi_bitmaps, int32_t n_roi, const int32_t *keepout_owner, float keepout_cost,
+                 const orgpu_sssp_params *p,
+                 int32_t *path_nodes, int32_t *path_edges, int64_t path_capacity, int64_t *path_offset,
+                 float *path_dist, orgpu_sssp_stats *stats)
+{
+    if (!g || !p || (n_nets > 0 && !nets) || !path_offset) return orgpu_fail(ORGPU_EINVAL, "route: null argument");
+    if (!g->indptr || !g->indices || !g->cost || !g->edge_kind) return orgpu_fail(ORGPU_EINVAL, "route: null graph array");
+    const int64_t N = g->n_nodes;
+    const int K = p->slots;
+    if (N <= 0 || N > en.cap_nodes || K < 1 || K > en.cap_slots || p->n_buckets < 1 || p->n_buckets > en.cap_nb)
+        return orgpu_fail(ORGPU_EINVAL, "route: request (N=%lld, K=%d, nb=%d) outside the reserved capacity "
+                          "(N=%lld, K=%d, nb=%d)", (long long)N, K, p->n_buckets, (long long)en.cap_nodes,
+                          en.cap_slots, en.cap_nb);
+    if (!(p->delta > 0.0f)) return orgpu_fail(ORGPU_EINVAL, "route: delta must be > 0");
+    // keepout edges are ranked by count ahead of the remaining cost, which is
+    // what a penalty far above every path cost (the reference's 1e9) does
+    if (keepout_owner && !(keepout_cost >= 1.0e6f))
+        return orgpu_fail(ORGPU_EINVAL, "route: keepout_cost %g must dominate path costs (>= 1e6)", (double)keepout_cost);
+    for (int i = 0; i < n_nets; ++i) {
+        const orgpu_net_req &r = nets[i];
+        if (r.src < 0 || r.src >= N || r.dst < 0 || r.dst >= N)
+            return orgpu_fail(ORGPU_EINVAL, "route: request %d endpoint outside [0, %lld)", i, (long long)N);
+        if (r.roi_index >= 0 && (!roi_bitmaps || r.roi_index >= n_roi))
+            return orgpu_fail(ORGPU_EINVAL, "route: request %d ROI row %d outside [0, %d)", i, r.roi_index, n_roi);
+        if (r.use_keepout && !keepout_owner) return orgpu_fail(ORGPU_EINVAL, "route: request %d needs keepout_owner", i);
+    }
+    hipStream_t s = en.stream;
+    const int nb = p->n_buckets;
+    RoundArgs a;
+    a.ctrl = en.ctrl; a.slots = en.slots; a.K = K; a.N = N; a.nb = (uint32_t)nb;
+    a.inv_delta = 1.0f / p->delta;
+    a.indptr = g->indptr; a.indices = g->indices; a.cost = g->cost; a.kind = g->edge_kind;
+    a.roi = roi_bitmaps; a.roi_words = (N + 31) / 32;
+    a.ko_owner = keepout_owner;
+    a.keys = en.keys; a.queues = en.queues; a.qcap = (uint64_t)en.cap_q;
+    a.bits = en.bits; a.bitmap_words = (uint64_t)((N * (int64_t)K + 31) / 32);
+
+    // control block init
+    Ctrl hc;
+    memset(&hc, 0, sizeof(hc));
+    hc.mode = MODE_IDLE;
+    hc.ovf_min = ~0ull;
+    for (int i = 0; i < nb; ++i) hc.ring[i] = (uint32_t)i;
+    hc.out_buf = (uint32_t)nb; hc.ovf_buf = (uint32_t)nb + 1; hc.ovf_next_buf = (uint32_t)nb + 2;
+    *en.h_ctrl = hc;
+    HIP_CHECK_RET(hipMemcpyAsync(en.ctrl, en.h_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, s));
+    HIP_CHECK_RET(hipMemsetAsync(en.slots, 0, sizeof(SlotInfo) * K, s));
+    // queue bitmaps must be clear: they are cleared as items are consumed, but
+    // a previous call may have stopped with items left (finished slots)
+    HIP_CHECK_RET(hipMemsetAsync(en.bits, 0, (size_t)(nb + 3) * a.bitmap_words * sizeof(uint32_t), s));
+
+    int grid = p->grid > 0 ? p->grid : 2048;
+    grid = (grid + 7) / 8 * 8;
+    const int rpp = p->rounds_per_poll > 0 ? p->rounds_per_poll : 32;
+    std::vector<int> slot_req(K, -1);
+    int next = 0, done_nets = 0;
+    int64_t out_pos = 0;
+    path_offset[0] = 0;
+    std::vector<int64_t> lens(n_nets, 0), offs(n_nets, 0);
+    unsigned long long min_nodes = 0, min_edges = 0;
+    HIP_CHECK_RET(hipEventRecord(en.ev_t0, s));
+
+    // slot_drained[k]: the slot's previous search has no queued items left
+    std::vector<char> slot_drained(K, 1);
+    auto seed_free_slots = [&]() -> int {
+        int ns = 0;
+        for (int k = 0; k < K && next < n_nets; ++k) {
+            if (slot_req[k] >= 0 || !slot_drained[k]) continue;
+            const orgpu_net_req &r = nets[next];
+            en.h_seed[ns] = SeedReq{k, r.src, r.dst, r.net_id, r.roi_index, r.use_keepout, next, 0};
+            slot_req[k] = next;
+            ++next; ++ns;
+        }
+        if (ns == 0) return 0;
+        for (int i = 0; i < ns; ++i) {
+            const int k = en.h_seed[i].slot;
+            hipLaunchKernelGGL(k_fill_keys, dim3(1024), dim3(256), 0, s, en.keys + (uint64_t)k * (uint64_t)N, N);
+        }
+        HIP_CHECK_RET(hipMemcpyAsync(en.seed, en.h_seed, sizeof(SeedReq) * ns, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_seed, dim3((ns + 255) / 256), dim3(256), 0, s, a, en.seed, ns);
+        HIP_CHECK_RET(hipGetLastError());
+        return 0;
+    };
+    int rc = seed_free_slots();
+    if (rc) return rc;
+
+    while (done_nets < n_nets) {
+        for (int r = 0; r < rpp; ++r) launch_round(en, a, grid);
+        HIP_CHECK_RET(hipGetLastError());
+        HIP_CHECK_RET(hipMemcpyAsync(en.h_ctrl, en.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+        HIP_CHECK_RET(hipMemcpyAsync(en.h_slots, en.slots, sizeof(SlotInfo) * K, hipMemcpyDeviceToHost, s));
+        HIP_CHECK_RET(hipEventRecord(en.ev_poll, s));
+        HIP_CHECK_RET(hipEventSynchronize(en.ev_poll));
+        if (en.h_ctrl->overflow)
+            return orgpu_fail(ORGPU_ECAPACITY, "route: a bucket queue exceeded %lld items; reserve a larger queue_capacity",
+                              (long long)en.cap_q);
+        int nf = 0, searching = 0;
+        for (int k = 0; k < K; ++k) {
+            if (slot_req[k] < 0) {
+                // free slot: reusable once the device reports no queued items
+                slot_drained[k] = en.h_slots[k].pending <= 0;
+                continue;
+            }
+            const uint32_t st = en.h_slots[k].state;
+            if (st == 2 || st == 3) en.h_slot_list[nf++] = k;
+            else ++searching;
+        }
+        if (en.h_ctrl->rounds > (1ull << 32))
+            return orgpu_fail(ORGPU_EHIP, "route: no completion after %llu rounds",
+                              (unsigned long long)en.h_ctrl->rounds);
+        if (nf == 0 && searching > 0 && en.h_ctrl->mode == MODE_IDLE)
+            return orgpu_fail(ORGPU_EHIP, "route: engine idle while %d searches are unfinished", searching);
+        if (nf == 0) {
+            if (searching == 0) {
+                rc = seed_free_slots();
+                if (rc) return rc;
+                if (en.h_ctrl->mode == MODE_IDLE && next >= n_nets)
+                    return orgpu_fail(ORGPU_EHIP, "route: engine idle with %d unfinished requests",
+                                      n_nets - done_nets);
+            }
+            continue;
+        }
+        HIP_CHECK_RET(hipMemcpyAsync(en.slot_list, en.h_slot_list, sizeof(int32_t) * nf, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_backtrace, dim3((nf + 63) / 64), dim3(64), 0, s, a, en.slot_list, nf,
+                           en.stage_nodes, en.stage_edges, en.lens, en.dists, en.cap_path);
+        HIP_CHECK_RET(hipGetLastError());
+        HIP_CHECK_RET(hipMemcpyAsync(en.h_lens, en.lens, sizeof(int32_t) * K, hipMemcpyDeviceToHost, s));
+        HIP_CHECK_RET(hipMemcpyAsync(en.h_dists, en.dists, sizeof(float) * K, hipMemcpyDeviceToHost, s));
+        HIP_CHECK_RET(hipStreamSynchronize(s));
+        for (int i = 0; i < nf; ++i) {
+            const int k = en.h_slot_list[i];
+            const int req = slot_req[k];
+            int len = en.h_lens[k];
+            if (len < 0) return orgpu_fail(ORGPU_ECAPACITY, "route: request %d: predecessor chain broken or longer than %d",
+                                           req, en.cap_path);
+            if (p->count_minimal && en.h_slots[k].state == 2) {
+                HIP_CHECK_RET(hipMemsetAsync(en.counters, 0, sizeof(unsigned long long) * 2, s));
+                hipLaunchKernelGGL(k_count_minimal, dim3(512), dim3(256), 0, s, a, k, en.counters);
+                HIP_CHECK_RET(hipMemcpyAsync(en.h_counters, en.counters, sizeof(unsigned long long) * 2,
+                                             hipMemcpyDeviceToHost, s));
+                HIP_CHECK_RET(hipStreamSynchronize(s));
+                min_nodes += en.h_counters[0];
+                min_edges += en.h_counters[1];
+            }
+            if (len > 0) {
+                if (out_pos + len > path_capacity)
+                    return orgpu_fail(ORGPU_ECAPACITY, "route: path output capacity %lld exceeded", (long long)path_capacity);
+                HIP_CHECK_RET(hipMemcpyAsync(en.h_stage, en.stage_nodes + (int64_t)k * en.cap_path, sizeof(int32_t) * len,
+                                             hipMemcpyDeviceToHost, s));
+                HIP_CHECK_RET(hipMemcpyAsync(en.h_stage + en.cap_path, en.stage_edges + (int64_t)k * en.cap_path,
+                                             sizeof(int32_t) * len, hipMemcpyDeviceToHost, s));
+                HIP_CHECK_RET(hipStreamSynchronize(s));
+                // staging holds dst..src; the edge entering node i sits at i
+                for (int t = 0; t < len; ++t) path_nodes[out_pos + t] = en.h_stage[len - 1 - t];
+                if (path_edges) {
+                    for (int t = 0; t + 1 < len; ++t) path_edges[out_pos + t] = en.h_stage[en.cap_path + len - 2 - t];
+                    path_edges[out_pos + len - 1] = -1;
+                }
+            }
+            lens[req] = len;
+            offs[req] = out_pos;
+            out_pos += len;
+            if (path_dist) path_dist[req] = en.h_dists[k];
+            slot_req[k] = -1;
+            en.h_slots[k].state = 0;
+            ++done_nets;
+        }
+        // mark the finished slots free on the device before reseeding
+        for (int i = 0; i < nf; ++i) {
+            const int k = en.h_slot_list[i];
+            HIP_CHECK_RET(hipMemsetAsync(&en.slots[k].state, 0, sizeof(uint32_t), s));
+            slot_drained[k] = en.h_slots[k].pending <= 0;
+        }
+        rc = seed_free_slots();
+        if (rc) return rc;
+    }
+    HIP_CHECK_RET(hipEventRecord(en.ev_t1, s));
+    HIP_CHECK_RET(hipMemcpyAsync(en.h_ctrl, en.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+    HIP_CHECK_RET(hipStreamSynchronize(s));
+    // outputs in request order
+    {
+        std::vector<int32_t> tmp_n(out_pos), tmp_e(out_pos);
+        std::copy(path_nodes, path_nodes + out_pos, tmp_n.begin());
+        if (path_edges) std::copy(path_edges, path_edges + out_pos, tmp_e.begin());
+        int64_t pos = 0;
+        for (int i = 0; i < n_nets; ++i) {
+            path_offset[i] = pos;
+            std::copy(tmp_n.begin() + offs[i], tmp_n.begin() + offs[i] + lens[i], path_nodes + pos);
+            if (path_edges) std::copy(tmp_e.begin() + offs[i], tmp_e.begin() + offs[i] + lens[i], path_edges + pos);
+            pos += lens[i];
+        }
+        path_offset[n_nets] = pos;
+    }
+    if (stats) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, en.ev_t0, en.ev_t1);
+        stats->rounds = (int64_t)en.h_ctrl->rounds;
+        stats->items = (int64_t)en.h_ctrl->items;
+        stats->edges = (int64_t)en.h_ctrl->edges;
+        stats->minimal_nodes = (int64_t)min_nodes;
+        stats->minimal_edges = (int64_t)min_edges;
+        stats->kernel_ms = ms;
+    }
+    return ORGPU_OK;
+}

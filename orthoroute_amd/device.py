@@ -1,0 +1,176 @@
+"""Device side of the router: HBM-resident lattice and search calls.
+
+PyTorch is used only to own device memory (tensors on cuda:0) and to provide
+the HIP stream; every computation runs in liborgpu.so's hand-written gfx950
+kernels through the C ABI in include/orgpu.h.  There is no CPU fallback: on a
+machine without the library or a GPU these calls raise.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from .lattice import lattice_spec
+
+KEEPOUT_COST = 1e9  # UPF:3182
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("orthoroute_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Context:
+    """One orgpu handle bound to torch's current HIP stream on a device."""
+
+    def __init__(self, device=0):
+        torch = _torch()
+        self.torch = torch
+        self.device = torch.device("cuda", device)
+        self.lib = _abi.load()
+        with torch.cuda.device(self.device):
+            self.stream = torch.cuda.current_stream(self.device)
+        h = ctypes.c_void_p()
+        _abi.check(self.lib.orgpu_create(device, ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(h)))
+        self.handle = h
+        self._reserved = None
+
+    def close(self):
+        if self.handle:
+            self.lib.orgpu_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _abi.check(self.lib.orgpu_sync(self.handle))
+
+    # ---- A1 ---------------------------------------------------------------
+    def build_lattice(self, X, Y, L, policy="reference", pitch=1.0, via_cost=3.0):
+        return DeviceLattice(self, X, Y, L, policy, pitch, via_cost)
+
+    # ---- A3 ---------------------------------------------------------------
+    def reserve(self, n_nodes, slots, n_buckets, queue_capacity=0, max_path_len=1 << 16):
+        key = (int(n_nodes), int(slots), int(n_buckets), int(queue_capacity), int(max_path_len))
+        if self._reserved is not None and all(a >= b for a, b in zip(self._reserved, key)):
+            return
+        _abi.check(self.lib.orgpu_reserve(self.handle, key[0], key[1], key[2], key[3], key[4]))
+        self._reserved = key
+
+
+def _lattice_desc(X, Y, L, policy, pitch, via_cost):
+    planar, dirs, pairs = lattice_spec(L, policy)
+    d = _abi.LatticeDesc()
+    d.X, d.Y, d.L = X, Y, L
+    d.pitch = pitch
+    d.via_cost = via_cost
+    for z in range(L):
+        d.planar[z] = planar[z]
+        d.dir[z] = dirs[z]
+    pf = np.ascontiguousarray([p[0] for p in pairs], dtype=np.int32)
+    pt = np.ascontiguousarray([p[1] for p in pairs], dtype=np.int32)
+    d.npairs = len(pairs)
+    d.pair_from = pf.ctypes.data_as(ctypes.c_void_p)
+    d.pair_to = pt.ctypes.data_as(ctypes.c_void_p)
+    return d, (pf, pt)
+
+
+def lattice_counts(X, Y, L, policy="reference"):
+    d, keep = _lattice_desc(X, Y, L, policy, 1.0, 3.0)
+    n, e = ctypes.c_int64(), ctypes.c_int64()
+    _abi.check(_abi.load().orgpu_lattice_count(ctypes.byref(d), ctypes.byref(n), ctypes.byref(e)))
+    return n.value, e.value
+
+
+class DeviceLattice:
+    """CSR of the lattice in HBM (indptr, indices, base_cost, edge_layer, edge_kind)."""
+
+    def __init__(self, ctx, X, Y, L, policy="reference", pitch=1.0, via_cost=3.0):
+        torch = ctx.torch
+        self.ctx = ctx
+        self.X, self.Y, self.L, self.policy = int(X), int(Y), int(L), policy
+        self.pitch, self.via_cost = float(pitch), float(via_cost)
+        d, keep = _lattice_desc(self.X, self.Y, self.L, policy, self.pitch, self.via_cost)
+        n, e = ctypes.c_int64(), ctypes.c_int64()
+        _abi.check(ctx.lib.orgpu_lattice_count(ctypes.byref(d), ctypes.byref(n), ctypes.byref(e)))
+        self.N, self.E = n.value, e.value
+        dev = ctx.device
+        self.indptr = torch.empty(self.N + 1, dtype=torch.int32, device=dev)
+        self.indices = torch.empty(max(self.E, 1), dtype=torch.int32, device=dev)
+        self.base_cost = torch.empty(max(self.E, 1), dtype=torch.float32, device=dev)
+        self.edge_layer = torch.empty(max(self.E, 1), dtype=torch.uint8, device=dev)
+        self.edge_kind = torch.empty(max(self.E, 1), dtype=torch.uint8, device=dev)
+        _abi.check(ctx.lib.orgpu_lattice_build(ctx.handle, ctypes.byref(d), _ptr(self.indptr),
+                                               _ptr(self.indices), _ptr(self.base_cost),
+                                               _ptr(self.edge_layer), _ptr(self.edge_kind)))
+
+    def graph(self, cost):
+        g = _abi.Graph()
+        g.n_nodes, g.n_edges = self.N, self.E
+        g.indptr, g.indices = self.indptr.data_ptr(), self.indices.data_ptr()
+        g.cost, g.edge_kind = cost.data_ptr(), self.edge_kind.data_ptr()
+        g.X, g.Y, g.L = self.X, self.Y, self.L
+        return g
+
+
+class SearchParams:
+    def __init__(self, slots=1, delta=1.0, n_buckets=8, rounds_per_poll=32, grid=0, count_minimal=False):
+        self.slots, self.delta, self.n_buckets = int(slots), float(delta), int(n_buckets)
+        self.rounds_per_poll, self.grid, self.count_minimal = int(rounds_per_poll), int(grid), bool(count_minimal)
+
+    def c(self):
+        p = _abi.SsspParams()
+        p.delta, p.n_buckets, p.slots = self.delta, self.n_buckets, self.slots
+        p.rounds_per_poll, p.grid, p.count_minimal = self.rounds_per_poll, self.grid, int(self.count_minimal)
+        return p
+
+
+def route(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=None, path_capacity=None):
+    """Route requests = list of (src, dst, net_id, roi_index, use_keepout) on
+    the lattice with edge costs `cost` (device f32[E]).  Returns
+    (list of (nodes, edges, dist) or None per request, stats dict)."""
+    n = len(requests)
+    ctx.reserve(lat.N, params.slots, params.n_buckets)
+    reqs = (_abi.NetReq * max(n, 1))()
+    for i, (s, d, nid, roi, ko) in enumerate(requests):
+        reqs[i].src, reqs[i].dst, reqs[i].net_id = int(s), int(d), int(nid)
+        reqs[i].roi_index, reqs[i].use_keepout = int(roi), int(ko)
+    if path_capacity is None:
+        path_capacity = max(1024, n * 4 * (lat.X + lat.Y + 2 * lat.L))
+    nodes = np.empty(path_capacity, dtype=np.int32)
+    edges = np.empty(path_capacity, dtype=np.int32)
+    offs = np.zeros(n + 1, dtype=np.int64)
+    dist = np.zeros(max(n, 1), dtype=np.float32)
+    st = _abi.SsspStats()
+    g = lat.graph(cost)
+    n_roi = 0 if roi_bitmaps is None else int(roi_bitmaps.shape[0])
+    if roi_bitmaps is not None and roi_bitmaps.shape[1] * 32 < lat.N:
+        raise ValueError("roi bitmap rows are shorter than the node count")
+    rc = ctx.lib.orgpu_route(ctx.handle, ctypes.byref(g), ctypes.cast(reqs, ctypes.c_void_p), n,
+                             _ptr(roi_bitmaps), n_roi, _ptr(keepout_owner), ctypes.c_float(KEEPOUT_COST),
+                             ctypes.byref(params.c()), nodes.ctypes.data_as(ctypes.c_void_p),
+                             edges.ctypes.data_as(ctypes.c_void_p), path_capacity,
+                             offs.ctypes.data_as(ctypes.c_void_p), dist.ctypes.data_as(ctypes.c_void_p),
+                             ctypes.byref(st))
+    _abi.check(rc)
+    out = []
+    for i in range(n):
+        a, b = offs[i], offs[i + 1]
+        if b - a > 1:
+            out.append((nodes[a:b].copy(), edges[a:b - 1].copy(), float(dist[i])))
+        else:
+            out.append(None)
+    stats = {"rounds": st.rounds, "items": st.items, "edges": st.edges,
+             "minimal_nodes": st.minimal_nodes, "minimal_edges": st.minimal_edges,
+             "kernel_ms": st.kernel_ms}
+    return out, stats

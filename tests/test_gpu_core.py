@@ -1,0 +1,128 @@
+"""GPU parity: lattice builder and batched SSSP vs the CPU oracle (bit-exact)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from orthoroute_amd.device import Context
+    return Context(0)
+
+
+def to_np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("policy", ["reference", "adjacent"])
+@pytest.mark.parametrize("shape", [(5, 4, 2), (6, 5, 3), (8, 6, 4), (12, 10, 6), (7, 6, 8), (96, 96, 6), (33, 17, 12)])
+def test_lattice_bit_exact(ctx, policy, shape):
+    X, Y, L = shape
+    say(f"[lattice] {policy} {shape}")
+    g = ctx.build_lattice(X, Y, L, policy, pitch=1.0, via_cost=3.0)
+    o = oracle.Lattice(X, Y, L, policy, pitch=1.0, via_cost=3.0)
+    np.testing.assert_array_equal(to_np(g.indptr), o.indptr)
+    np.testing.assert_array_equal(to_np(g.indices)[:o.E], o.indices)
+    np.testing.assert_array_equal(to_np(g.base_cost)[:o.E].view(np.uint32), o.costs.view(np.uint32))
+    np.testing.assert_array_equal(to_np(g.edge_layer)[:o.E], o.edge_layer)
+    np.testing.assert_array_equal(to_np(g.edge_kind)[:o.E], o.edge_kind)
+
+
+def pack_bits(mask):
+    b = np.packbits(mask.astype(np.uint8), bitorder="little")
+    pad = (-len(b)) % 4
+    return np.concatenate([b, np.zeros(pad, np.uint8)]).view(np.uint32)
+
+
+def make_costs(o, rng, kind="jitter"):
+    if kind == "jitter":   # reference-like: 0.3*base + (e % 9973)*1e-6 + congestion
+        c = (o.costs * np.float32(0.3)).astype(np.float32)
+        c += (np.arange(o.E, dtype=np.float32) % 9973) * np.float32(1e-6)
+        hot = rng.random(o.E) < 0.05
+        c[hot] += rng.integers(1, 20, hot.sum()).astype(np.float32) * np.float32(0.7)
+    elif kind == "ties":
+        c = (o.costs * np.float32(0.5) * rng.integers(1, 3, o.E).astype(np.float32)).astype(np.float32)
+    else:
+        c = (o.costs * (0.2 + 3 * rng.random(o.E))).astype(np.float32)
+    blk = (o.edge_kind == 1) & (rng.random(o.E) < 0.15)
+    c[blk] = np.inf
+    return c
+
+
+def run_case(ctx, X, Y, L, policy, slots, n_nets, seed, cost_kind, roi_frac=0.5, keepout=False):
+    import torch
+    from orthoroute_amd.device import SearchParams, route
+    say(f"[route] {policy} {X}x{Y}x{L} slots={slots} nets={n_nets} seed={seed} costs={cost_kind} keepout={keepout}")
+    rng = np.random.default_rng(seed)
+    o = oracle.Lattice(X, Y, L, policy)
+    g = ctx.build_lattice(X, Y, L, policy)
+    c = make_costs(o, rng, cost_kind)
+    cost_d = torch.from_numpy(c).to(ctx.device)
+    owner = None
+    if keepout:
+        owner = np.full(o.N, -1, np.int32)
+        sel = rng.random(o.N) < 0.08
+        owner[sel] = rng.integers(0, 6, sel.sum())
+    reqs, rois, masks = [], [], []
+    for i in range(n_nets):
+        s, d = rng.integers(0, o.N, 2)
+        while s == d:
+            d = rng.integers(0, o.N)
+        roi_idx = -1
+        mask = None
+        if rng.random() < roi_frac:
+            roi = oracle.roi_bfs(o, int(s), int(d), max_nodes=int(rng.choice([300, 2000, 50_000])))
+            mask = np.zeros(o.N, np.uint8); mask[roi] = 1; mask[s] = 1; mask[d] = 1
+            roi_idx = len(rois)
+            rois.append(pack_bits(mask))
+        masks.append(mask)
+        reqs.append((int(s), int(d), int(i % 6), roi_idx, int(keepout)))
+    roi_d = torch.from_numpy(np.stack(rois)).to(ctx.device) if rois else None
+    own_d = torch.from_numpy(owner).to(ctx.device) if owner is not None else None
+    p = SearchParams(slots=slots, delta=float(rng.choice([0.3, 1.0, 4.0])), n_buckets=int(rng.choice([1, 4, 8])),
+                     rounds_per_poll=16)
+    res, stats = route(ctx, g, cost_d, reqs, p, roi_bitmaps=roi_d, keepout_owner=own_d)
+    for i, (s, d, nid, roi_idx, ko) in enumerate(reqs):
+        path, dist, pred, _, _ = oracle.sssp_canon(o, c, s, d, roi_mask=masks[i],
+                                                   keepout_owner=owner if ko else None, net_id=nid)
+        got = None if res[i] is None else res[i][0].tolist()
+        assert got == path, f"net {i} ({s}->{d}) roi={roi_idx}: gpu {got} oracle {path}"
+        if path:
+            # first-match CSR edge per hop
+            exp_e = []
+            for u, v in zip(path, path[1:]):
+                row = o.indices[o.indptr[u]:o.indptr[u + 1]]
+                exp_e.append(int(o.indptr[u] + np.nonzero(row == v)[0][0]))
+            assert res[i][1].tolist() == exp_e
+            assert np.float32(res[i][2]) == dist[d]
+    return stats
+
+
+@pytest.mark.parametrize("policy", ["reference", "adjacent"])
+@pytest.mark.parametrize("slots", [1, 7])
+def test_route_matches_oracle(ctx, policy, slots):
+    for seed, kind in [(1, "jitter"), (2, "ties"), (3, "random")]:
+        run_case(ctx, 24, 20, 6, policy, slots, 24, seed + 10 * slots, kind)
+
+
+def test_route_keepout_matches_oracle(ctx):
+    run_case(ctx, 20, 18, 6, "reference", 5, 20, 77, "jitter", keepout=True)
+    run_case(ctx, 20, 18, 6, "adjacent", 1, 10, 78, "random", keepout=True)
+
+
+def test_route_unreachable_and_refill(ctx):
+    # B.Cu (top layer) of the reference lattice has no edges: unreachable
+    # destinations must finish as "no path" without blocking refills
+    stats = run_case(ctx, 16, 14, 6, "reference", 3, 30, 5, "jitter", roi_frac=0.0)
+    assert stats["rounds"] > 0
