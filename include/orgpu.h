@@ -139,6 +139,37 @@ int orgpu_route(orgpu_t *h, const orgpu_graph *g, const orgpu_net_req *nets, int
                 int64_t path_capacity, int64_t *path_offset, float *path_dist,
                 orgpu_sssp_stats *stats);
 
+/* ---- A2 ROI extraction -------------------------------------------------------- */
+
+/* ROIExtractor.extract_roi_bfs (UPF:1636-1734) plus the must-keep re-add of
+ * _route_all (UPF:4683-4700): bidirectional level-synchronous BFS from src
+ * (wave 1) and dst (wave 2) until max_depth, both frontiers empty, the waves
+ * meeting, or more than 1.5*max_nodes nodes seen; then the reference's
+ * truncation to max_nodes (lowest node ids first, must-keep swaps) and src/dst
+ * re-added.  The ROI is written as bitmap row roi_index (bit set = in ROI). */
+typedef struct orgpu_roi_req {
+    int32_t src, dst;
+    int32_t max_depth;
+    int32_t roi_index;
+    int64_t max_nodes;
+    int32_t readd;    /* 1: re-add src/dst (main search); 0: as extract_roi returns it (fallback) */
+    int32_t reserved;
+} orgpu_roi_req;
+
+typedef struct orgpu_roi_info {
+    int64_t seen;     /* nodes the BFS marked */
+    int64_t kept;     /* nodes in the final ROI */
+    int32_t depth;    /* BFS depth reached */
+    int32_t met;      /* waves met */
+    int32_t src_in;   /* src in the final ROI */
+    int32_t dst_in;   /* dst in the final ROI */
+} orgpu_roi_info;
+
+/* reqs: host array; roi_bitmaps: device u32[n_roi][ceil(N/32)], rows written
+ * completely; info: host array (may be NULL).  Blocks. */
+int orgpu_roi_bfs(orgpu_t *h, const orgpu_graph *g, const orgpu_roi_req *reqs, int32_t n,
+                  uint32_t *roi_bitmaps, int32_t n_roi, orgpu_roi_info *info);
+
 /* ---- A5/A6 congestion state (EdgeAccountant in HBM) ------------------------- */
 
 /* Nonzero entries of a per-edge quantity, copied to host arrays (idx, val)

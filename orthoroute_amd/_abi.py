@@ -17,7 +17,7 @@ ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ECAPACITY"}
 EXPORTED = [
     "orgpu_create", "orgpu_destroy", "orgpu_last_error", "orgpu_sync",
     "orgpu_lattice_count", "orgpu_lattice_build",
-    "orgpu_reserve", "orgpu_route",
+    "orgpu_reserve", "orgpu_route", "orgpu_roi_bfs",
     "orgpu_usage_scatter", "orgpu_present_build", "orgpu_present_ema", "orgpu_cost_update",
     "orgpu_history_update", "orgpu_overuse_sparse",
 ]
@@ -61,6 +61,17 @@ class SsspStats(ctypes.Structure):
     _fields_ = [("rounds", ctypes.c_int64), ("items", ctypes.c_int64), ("edges", ctypes.c_int64),
                 ("minimal_nodes", ctypes.c_int64), ("minimal_edges", ctypes.c_int64),
                 ("kernel_ms", ctypes.c_double)]
+
+
+class RoiReq(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_int32), ("dst", ctypes.c_int32), ("max_depth", ctypes.c_int32),
+                ("roi_index", ctypes.c_int32), ("max_nodes", ctypes.c_int64), ("readd", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class RoiInfo(ctypes.Structure):
+    _fields_ = [("seen", ctypes.c_int64), ("kept", ctypes.c_int64), ("depth", ctypes.c_int32),
+                ("met", ctypes.c_int32), ("src_in", ctypes.c_int32), ("dst_in", ctypes.c_int32)]
 
 
 class SparseOut(ctypes.Structure):
@@ -108,6 +119,7 @@ def load():
         "orgpu_reserve": [P, i64, i32, i32, i64, i32],
         "orgpu_route": [P, ctypes.POINTER(Graph), P, i32, P, i32, P, f32, ctypes.POINTER(SsspParams),
                         P, P, i64, P, P, ctypes.POINTER(SsspStats)],
+        "orgpu_roi_bfs": [P, ctypes.POINTER(Graph), P, i32, P, i32, P],
         "orgpu_usage_scatter": [P, P, i64, P, i64, i32],
         "orgpu_present_build": [P, i64, P, P, P, i64, f32],
         "orgpu_present_ema": [P, i64, P, P, ctypes.c_int, f32, f32, ctypes.POINTER(SparseOut)],

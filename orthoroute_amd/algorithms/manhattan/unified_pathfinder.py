@@ -1,0 +1,1201 @@
+"""PathFinder negotiated-congestion router on MI355X.
+
+Drop-in for orthoroute/algorithms/manhattan/unified_pathfinder.py (UPF): the
+same classes and entry points (PathFinderRouter / UnifiedPathFinder,
+PathFinderConfig, GPUConfig, Lattice3D, CSRGraph, EdgeAccountant,
+ROIExtractor, SimpleDijkstra) and the same negotiation schedule, with the
+per-edge state (usage counts, present, present EMA, history, total cost) and
+every per-edge / per-node pass running in hand-written gfx950 kernels
+(liborgpu.so, include/orgpu.h).  The host keeps the per-net bookkeeping
+(paths, edge lists, via columns, ownership) and the small decision
+reductions, which it evaluates exactly as numpy would on the dense arrays
+(orthoroute_amd.reductions).
+
+Shortest paths are the FP32 packed-key semantics of the kernels: path cost
+within 1e-5 of the reference's float64 Dijkstra, ties broken by the lowest
+predecessor index.
+
+Batching: within one negotiation iteration every net sees the same cost
+snapshot (UPF:3722-3741 recomputes costs once per iteration; keepouts are
+fixed per iteration), so all nets of the iteration are searched together,
+K at a time, and committed afterwards in the reference's order.  The only
+order-dependent step, the wider-ROI fallback for the first five failures
+(UPF:4751-4781), is replayed in that order after the batch.  `batch_slots=1`
+gives the plain one-net-at-a-time form.
+"""
+import ctypes
+import logging
+import math
+import os
+import random
+import time
+from collections import defaultdict
+from typing import Dict, List, Optional, Set, Tuple
+
+import numpy as np
+
+from ... import _abi
+from ...device import (Context, DeviceLattice, EdgeState, SearchParams, roi_bfs, route as device_route)
+from ...lattice import DIR_H, DIR_HV, DIR_V, layer_directions, reference_via_pairs
+from ...reductions import sum_f32
+from .board_analyzer import analyze_board_characteristics
+from .parameter_derivation import apply_derived_parameters, derive_routing_parameters
+
+logger = logging.getLogger(__name__)
+f32 = np.float32
+
+ROI_THRESHOLD_STEPS = 125      # UPF:4617
+MAX_ROI_NODES = 50_000         # UPF:1659
+KEEPOUT_COST = 1e9             # UPF:3182
+
+
+class GeometryPayload:
+    def __init__(self, tracks, vias):
+        self.tracks = tracks
+        self.vias = vias
+
+
+class GPUConfig:
+    """UPF:549-561 (the flags of the reference's CuPy path; kept for callers)."""
+    GPU_MODE = True
+    DEBUG_INVARIANTS = True
+    USE_PERSISTENT_KERNEL = False
+    USE_GPU_COMPACTION = True
+    USE_DELTA_STEPPING = True
+    DELTA_VALUE = 0.5
+
+
+class PathFinderConfig:
+    """UPF:564-634 defaults, plus this build's execution knobs:
+    lattice_policy ('reference' | 'adjacent'), batch_slots (concurrent
+    searches per launch), sssp_delta / sssp_buckets (bucket width and ring
+    size; None = autotuned), max_iterations_cap."""
+    max_iterations: int = 40
+    pres_fac_init: float = 1.0
+    pres_fac_mult: float = 1.10
+    pres_fac_max: float = 64.0
+    hist_gain: float = 0.2
+    base_cost_weight: float = 0.3
+    grid_pitch: float = 0.4
+    via_cost: float = 0.7
+    portal_discount: float = 0.4
+    span_alpha: float = 0.15
+    iter1_always_connect: bool = True
+    portal_enabled: bool = True
+    portal_delta_min: int = 3
+    portal_delta_max: int = 12
+    portal_delta_pref: int = 6
+    portal_x_snap_max: float = 0.5
+    portal_via_discount: float = 0.15
+    portal_retarget_patience: int = 3
+    stagnation_patience: int = 5
+    use_gpu: bool = True
+    batch_size: int = 32
+    layer_count: int = 6
+    strict_drc: bool = False
+    mode: str = "near_far"
+    roi_parallel: bool = False
+    per_net_budget_s: float = 5.0
+    max_roi_nodes: int = 750000
+    delta_multiplier: float = 4.0
+    adaptive_delta: bool = True
+    strict_capacity: bool = True
+    reroute_only_offenders: bool = True
+    layer_shortfall_percentile: float = 95.0
+    layer_shortfall_cap: int = 16
+    enable_profiling: bool = False
+    enable_instrumentation: bool = False
+    strict_overuse_block: bool = False
+    hist_cost_weight: float = 10.0
+    log_iteration_details: bool = False
+    acc_fac: float = 0.0
+    phase_block_after: int = 2
+    congestion_multiplier: float = 1.0
+    max_search_nodes: int = 2000000
+    layer_names = None
+    hotset_cap: int = 512
+    allowed_via_spans = None
+    # execution knobs of this build
+    lattice_policy: str = "reference"
+    batch_slots: int = 64
+    sssp_delta = None
+    sssp_buckets = None
+    max_iterations_cap = None
+
+
+# ----------------------------------------------------------------------------
+# geometry / lattice
+# ----------------------------------------------------------------------------
+
+class KiCadGeometry:
+    """pathfinder/kicad_geometry.py:12-72: grid snapped to pitch multiples."""
+
+    def __init__(self, bounds, pitch=0.4, layer_count=2):
+        self.min_x, self.min_y, self.max_x, self.max_y = bounds
+        self.pitch = pitch
+        self.grid_min_x = round(self.min_x / pitch) * pitch
+        self.grid_min_y = round(self.min_y / pitch) * pitch
+        self.grid_max_x = round(self.max_x / pitch) * pitch
+        self.grid_max_y = round(self.max_y / pitch) * pitch
+        self.x_steps = int((self.grid_max_x - self.grid_min_x) / pitch) + 1
+        self.y_steps = int((self.grid_max_y - self.grid_min_y) / pitch) + 1
+        self.layer_count = layer_count
+
+    def lattice_to_world(self, x, y):
+        return (self.grid_min_x + x * self.pitch, self.grid_min_y + y * self.pitch)
+
+    def world_to_lattice(self, wx, wy):
+        return (round((wx - self.grid_min_x) / self.pitch), round((wy - self.grid_min_y) / self.pitch))
+
+    def node_index(self, x, y, z):
+        return z * (self.x_steps * self.y_steps) + y * self.x_steps + x
+
+    def index_to_coords(self, n):
+        plane = self.x_steps * self.y_steps
+        z = n // plane
+        r = n % plane
+        return (r % self.x_steps, r // self.x_steps, z)
+
+
+class CSRGraph:
+    """The lattice CSR in HBM (UPF:647-792): indptr, indices, base_costs,
+    edge_layer, edge_kind are torch tensors on the GPU."""
+
+    def __init__(self, dlat: DeviceLattice):
+        self.dlat = dlat
+        self.indptr = dlat.indptr
+        self.indices = dlat.indices
+        self.base_costs = dlat.base_cost
+        self.edge_layer = dlat.edge_layer
+        self.edge_kind = dlat.edge_kind
+        self.num_nodes, self.num_edges = dlat.N, dlat.E
+
+
+class Lattice3D:
+    """UPF:1075-1338: H/V discipline, via pairs, and build_graph on the GPU."""
+
+    def __init__(self, bounds, pitch, layers, policy="reference"):
+        self.bounds = bounds
+        self.pitch = pitch
+        self.layers = layers
+        self.policy = policy
+        self.geom = KiCadGeometry(bounds, pitch, layer_count=layers)
+        self.x_steps = self.geom.x_steps
+        self.y_steps = self.geom.y_steps
+        self.num_nodes = self.x_steps * self.y_steps * layers
+        self.layer_dir = ["hv", "hv"] if layers == 2 else ["v" if z == 0 else ("h" if z % 2 == 1 else "v")
+                                                           for z in range(layers)]
+
+    def get_legal_axis(self, layer):
+        if layer >= len(self.layer_dir):
+            return "h" if layer % 2 == 1 else "v"
+        return self.layer_dir[layer]
+
+    def is_legal_planar_edge(self, fx, fy, fl, tx, ty, tl):
+        if fl != tl:
+            return True
+        dx, dy = abs(tx - fx), abs(ty - fy)
+        if dx + dy != 1:
+            return False
+        d = self.get_legal_axis(fl)
+        if d == "hv":
+            return True
+        if d == "h":
+            return dy == 0 and dx == 1
+        return dx == 0 and dy == 1
+
+    def get_legal_via_pairs(self, layer_count):
+        return set(reference_via_pairs(layer_count))
+
+    def node_idx(self, x, y, z):
+        return self.geom.node_index(x, y, z)
+
+    def idx_to_coord(self, idx):
+        return self.geom.index_to_coords(int(idx))
+
+    def world_to_lattice(self, x_mm, y_mm):
+        return self.geom.world_to_lattice(x_mm, y_mm)
+
+    def build_graph(self, via_cost, allowed_via_spans=None, use_gpu=True, ctx=None) -> CSRGraph:
+        ctx = ctx or Context(0)
+        dlat = ctx.build_lattice(self.x_steps, self.y_steps, self.layers, self.policy,
+                                 pitch=float(self.pitch), via_cost=float(via_cost))
+        return CSRGraph(dlat)
+
+
+# ----------------------------------------------------------------------------
+# accountant
+# ----------------------------------------------------------------------------
+
+class EdgeAccountant:
+    """UPF:799-1043 with the arrays in HBM.  `count` is the canonical usage
+    store (UPF:807); present/present_ema/history/total_cost are float32 as in
+    the reference; capacity is 1 for every edge (UPF:811) and is not stored."""
+
+    def __init__(self, ctx, dlat):
+        self.E = dlat.E
+        self.state = EdgeState(ctx, dlat)
+        self.present = self.state.present
+        self.history = self.state.history
+
+    @property
+    def present_ema(self):
+        return self.state.present_ema
+
+    @property
+    def total_cost(self):
+        return self.state.total_cost
+
+    def commit_path(self, edge_indices):
+        self.state.scatter(edge_indices, +1)
+
+    def clear_path(self, edge_indices):
+        self.state.scatter(edge_indices, -1)
+
+    def refresh_from_canonical(self):
+        self.state.build_present()
+
+    def compute_overuse(self, router_instance=None):
+        self.state.build_present()
+        idx, val = self.state.overuse_sparse(0)
+        s = int(sum_f32(self.E, idx, val))
+        col = seg = 0
+        if router_instance is not None:
+            col = int(np.maximum(0, router_instance.via_col_use - router_instance.via_col_cap).sum())
+            if getattr(router_instance, "_segZ", 0) > 0:
+                seg = int(np.maximum(0, router_instance.via_seg_use - router_instance.via_seg_cap).sum())
+        return s + col + seg, int(len(idx))
+
+
+# ----------------------------------------------------------------------------
+# ROI / search front ends (API compatibility: single-net calls)
+# ----------------------------------------------------------------------------
+
+class ROIExtractor:
+    """UPF:1345-1734.  extract_roi runs the GPU BFS for one net and returns
+    (roi_nodes ascending, global_to_roi) like the reference."""
+
+    def __init__(self, graph: CSRGraph, use_gpu=True, lattice=None, ctx=None):
+        self.graph = graph
+        self.lattice = lattice
+        self.ctx = ctx or graph.dlat.ctx
+        self.N = graph.num_nodes
+        self.max_roi_nodes = MAX_ROI_NODES
+
+    def roi_depth(self, src, dst, stagnation_bonus=0.0):
+        sx, sy, _ = self.lattice.idx_to_coord(src)
+        dx, dy, _ = self.lattice.idx_to_coord(dst)
+        md = abs(dx - sx) + abs(dy - sy)
+        radius = max(60, int(md * 0.75 + stagnation_bonus * 2.0))
+        return min(int(radius + stagnation_bonus * 2.0), 150)
+
+    def extract_roi(self, src, dst, initial_radius=40, stagnation_bonus=0.0, portal_seeds=None):
+        torch = self.ctx.torch
+        words = (self.N + 31) // 32
+        rows = torch.zeros((1, words), dtype=torch.int32, device=self.ctx.device)
+        depth = self.roi_depth(src, dst, stagnation_bonus)
+        roi_bfs(self.ctx, self.graph.dlat, [(src, dst, depth, self.max_roi_nodes, 0)], rows, readd=False)
+        bits = np.unpackbits(rows.cpu().numpy().view(np.uint8), bitorder="little")[:self.N]
+        roi = np.nonzero(bits)[0].astype(np.int32)
+        g2r = np.full(self.N, -1, dtype=np.int32)
+        g2r[roi] = np.arange(len(roi), dtype=np.int32)
+        return roi, g2r
+
+
+class SimpleDijkstra:
+    """UPF:1787-1989 surface: find_path_roi(src, dst, costs, roi_nodes,
+    global_to_roi) runs one GPU search restricted to the ROI."""
+
+    def __init__(self, graph: CSRGraph, lattice=None, ctx=None):
+        self.graph = graph
+        self.ctx = ctx or graph.dlat.ctx
+        self.N = graph.num_nodes
+        self.plane_size = lattice.x_steps * lattice.y_steps if lattice else None
+        self.params = SearchParams(slots=1, delta=1.0, n_buckets=8)
+
+    def find_path_roi(self, src, dst, costs, roi_nodes, global_to_roi):
+        g2r = np.asarray(global_to_roi)
+        if g2r[src] < 0 or g2r[dst] < 0:
+            return None
+        torch = self.ctx.torch
+        mask = (g2r >= 0).astype(np.uint8)
+        bits = np.packbits(mask, bitorder="little")
+        bits = np.concatenate([bits, np.zeros((-len(bits)) % 4, np.uint8)]).view(np.int32)
+        rows = torch.from_numpy(bits.reshape(1, -1)).to(self.ctx.device)
+        if not isinstance(costs, torch.Tensor):
+            costs = torch.from_numpy(np.ascontiguousarray(costs, dtype=np.float32)).to(self.ctx.device)
+        res, _ = device_route(self.ctx, self.graph.dlat, costs, [(src, dst, -1, 0, 0)], self.params, roi_bitmaps=rows)
+        r = res[0]
+        return None if r is None else [int(v) for v in r[0]]
+
+
+# ----------------------------------------------------------------------------
+# router
+# ----------------------------------------------------------------------------
+
+class PathFinderRouter:
+    """PathFinder negotiated congestion routing (UPF:1996-5914 surface)."""
+
+    def __init__(self, config: PathFinderConfig = None, use_gpu: bool = None):
+        self.config = config or PathFinderConfig()
+        if use_gpu is not None:
+            self.config.use_gpu = use_gpu
+        env = os.getenv("USE_GPU")
+        if env is not None:
+            self.config.use_gpu = env == "1"
+        self.lattice: Optional[Lattice3D] = None
+        self.graph: Optional[CSRGraph] = None
+        self.accounting: Optional[EdgeAccountant] = None
+        self.solver: Optional[SimpleDijkstra] = None
+        self.roi_extractor: Optional[ROIExtractor] = None
+        self.ctx: Optional[Context] = None
+        self.pad_to_node: Dict[str, int] = {}
+        self.net_paths: Dict[str, List[int]] = {}
+        self.iteration = 0
+        self._negotiation_ran = False
+        self._geometry_payload = GeometryPayload([], [])
+        self.locked_nets: Set[str] = set()
+        self.net_clean_streak: Dict[str, int] = defaultdict(int)
+        self._net_to_edges: Dict[str, np.ndarray] = {}
+        self.portals = {}
+        self.net_pad_ids: Dict[str, Tuple[str, str]] = {}
+        self.stagnation_counter = 0
+        self.full_graph_fallback_count = 0
+        self._last_ripped: Set[str] = set()
+        self._freeze_pres_fac_until = 0
+        self.stats = defaultdict(float)
+        self.iteration_log = []
+
+    # ---------------- initialisation
+    def initialize_graph(self, board) -> bool:
+        """UPF:2088-2252 for a Board (pads on nets give the bounds)."""
+        bounds = self._calc_bounds(board)
+        layers = getattr(board, "layer_count", None) or len(getattr(board, "layers", []) or []) \
+            or self.config.layer_count
+        self.config.layer_count = int(layers)
+        names = self.config.layer_names
+        if not names or len(names) < self.config.layer_count:
+            self.config.layer_names = (getattr(board, "layers", None) or
+                                       (["F.Cu"] + [f"In{i}.Cu" for i in range(1, self.config.layer_count - 1)]
+                                        + ["B.Cu"]))
+        self._init_lattice(bounds, self.config.grid_pitch, self.config.layer_count)
+        self._map_pads(board)
+        return True
+
+    def initialize_lattice(self, x_steps, y_steps, layers, pitch=1.0, via_cost=None):
+        """Synthetic boards: the lattice a board whose pads span
+        (3*pitch, 3*pitch)..((x_steps-4)*pitch, (y_steps-4)*pitch) produces."""
+        if via_cost is not None:
+            self.config.via_cost = via_cost
+        self.config.grid_pitch = pitch
+        self.config.layer_count = layers
+        bounds = (0.0, 0.0, (x_steps - 1) * pitch, (y_steps - 1) * pitch)
+        self._init_lattice(bounds, pitch, layers)
+        assert (self.lattice.x_steps, self.lattice.y_steps) == (x_steps, y_steps)
+        return True
+
+    def _init_lattice(self, bounds, pitch, layers):
+        t0 = time.perf_counter()
+        self.ctx = self.ctx or Context(0)
+        self.lattice = Lattice3D(bounds, pitch, layers, getattr(self.config, "lattice_policy", "reference"))
+        self.graph = self.lattice.build_graph(self.config.via_cost, ctx=self.ctx)
+        self.N = self.lattice.num_nodes
+        self.accounting = EdgeAccountant(self.ctx, self.graph.dlat)
+        X, Y, L = self.lattice.x_steps, self.lattice.y_steps, layers
+        self._Nx, self._Ny, self._Nz = X, Y, L
+        self.layer_bias = np.ones(L, dtype=np.float32)
+        self.via_col_cap = np.full((X, Y), int(getattr(self.config, "via_column_capacity", 4)), dtype=np.int16)
+        self.via_col_use = np.zeros((X, Y), dtype=np.int16)
+        self.via_col_pres = np.zeros((X, Y), dtype=np.float32)
+        self._segZ = max(L - 2, 0)
+        if self._segZ > 0:
+            cap = int(getattr(self.config, "via_segment_capacity", 2))
+            self.via_seg_cap = np.full((X, Y, self._segZ), cap, dtype=np.int8)
+            self.via_seg_use = np.zeros((X, Y, self._segZ), dtype=np.int16)
+            self.via_seg_pres = np.zeros((X, Y, self._segZ), dtype=np.float32)
+            self.via_seg_prefix = np.zeros((X, Y, self._segZ), dtype=np.float32)
+        self.node_owner = np.full(self.N, -1, dtype=np.int32)
+        self.net_id_map = {}
+        self.next_net_id = 0
+        self.solver = SimpleDijkstra(self.graph, self.lattice, self.ctx)
+        self.roi_extractor = ROIExtractor(self.graph, lattice=self.lattice, ctx=self.ctx)
+        # PadEscapePlanner(random_seed=42) seeds Python's random at the end of
+        # initialize_graph (UPF:2239-2240, pad_escape_planner.py:124); the
+        # negotiation's draws continue from that state
+        self._rng = random.Random(getattr(self.config, "escape_random_seed", 42))
+        self._via_rank = None
+        self.ctx.sync()
+        self.stats["lattice_build_s"] = time.perf_counter() - t0
+
+    def _calc_bounds(self, board):
+        margin = 3.0
+        pads = []
+        for net in getattr(board, "nets", []) or []:
+            if len(getattr(net, "pads", [])) >= 2:
+                pads.extend(p for p in net.pads if getattr(p, "position", None) is not None)
+        if pads:
+            xs = [p.position.x for p in pads]
+            ys = [p.position.y for p in pads]
+            return (min(xs) - margin, min(ys) - margin, max(xs) + margin, max(ys) + margin)
+        if hasattr(board, "_kicad_bounds"):
+            b = board._kicad_bounds
+            return (b[0] - margin, b[1] - margin, b[2] + margin, b[3] + margin)
+        return (0, 0, 100, 100)
+
+    def _pad_key(self, pad, comp=None):
+        cid = getattr(pad, "component_id", None) or (getattr(comp, "id", None) if comp else None) \
+            or "GENERIC_COMPONENT"
+        if cid == "GENERIC_COMPONENT" and hasattr(pad, "position"):
+            return f"{cid}_{pad.id}@{int(round(pad.position.x * 1000))},{int(round(pad.position.y * 1000))}"
+        return f"{cid}_{pad.id}"
+
+    def _get_pad_layer(self, pad):
+        names = self.config.layer_names or []
+        if getattr(pad, "layer", None):
+            if str(pad.layer) in names:
+                return names.index(str(pad.layer))
+        if getattr(pad, "layers", None):
+            if str(pad.layers[0]) in names:
+                return names.index(str(pad.layers[0]))
+        return 0
+
+    def _map_pads(self, board):
+        def snap(x_mm, y_mm, layer):
+            x, y = self.lattice.world_to_lattice(x_mm, y_mm)
+            x = max(0, min(x, self.lattice.x_steps - 1))
+            y = max(0, min(y, self.lattice.y_steps - 1))
+            return self.lattice.node_idx(x, y, layer)
+        for comp in getattr(board, "components", []) or []:
+            for pad in getattr(comp, "pads", []) or []:
+                self.pad_to_node[self._pad_key(pad, comp)] = snap(pad.position.x, pad.position.y,
+                                                                   self._get_pad_layer(pad))
+        for pad in getattr(board, "pads", []) or []:
+            k = self._pad_key(pad)
+            if k not in self.pad_to_node:
+                self.pad_to_node[k] = snap(pad.position.x, pad.position.y, self._get_pad_layer(pad))
+        for net in getattr(board, "nets", []) or []:
+            for pad in getattr(net, "pads", []) or []:
+                k = self._pad_key(pad)
+                if k not in self.pad_to_node and getattr(pad, "position", None) is not None:
+                    self.pad_to_node[k] = snap(pad.position.x, pad.position.y, self._get_pad_layer(pad))
+
+    # ---------------- legacy no-op API (UPF:5588-5594, 5619)
+    def map_all_pads(self, board):
+        return None
+
+    def prepare_routing_runtime(self):
+        return None
+
+    def precompute_all_pad_escapes(self, board, nets_to_route=None):
+        """Pad escape planning (portals) is outside this build; nets route
+        pad node to pad node as the reference does when no portals exist."""
+        return [], []
+
+    # ---------------- small helpers
+    def _get_net_id(self, name):
+        if name not in self.net_id_map:
+            self.net_id_map[name] = self.next_net_id
+            self.next_net_id += 1
+        return self.net_id_map[name]
+
+    def _coords(self, n):
+        return self.lattice.idx_to_coord(n)
+
+    def _via_hops(self, path):
+        """(x, y, z_lo, z_hi) of every via hop of a node path."""
+        p = np.asarray(path, dtype=np.int64)
+        plane = self._Nx * self._Ny
+        z = p // plane
+        r = p % plane
+        hop = (r[1:] == r[:-1]) & (z[1:] != z[:-1])
+        if not hop.any():
+            return np.zeros((0, 4), np.int64)
+        rr = r[:-1][hop]
+        za, zb = z[:-1][hop], z[1:][hop]
+        return np.stack([rr % self._Nx, rr // self._Nx, np.minimum(za, zb), np.maximum(za, zb)], axis=1)
+
+    # ---------------- requests
+    def route_multiple_nets(self, requests: List, progress_cb=None, iteration_cb=None) -> Dict:
+        """UPF:2530-2598.  Multi-terminal nets route as their first two pads
+        (the reference's live path does the same: multi_terminal_enabled is
+        not a config attribute, UPF:2558)."""
+        tasks = self._parse_requests(requests)
+        if not tasks:
+            self._negotiation_ran = True
+            return {}
+        return self._pathfinder_negotiation(tasks, progress_cb, iteration_cb)
+
+    def _parse_requests(self, requests):
+        tasks = {}
+        for req in requests:
+            if hasattr(req, "name") and hasattr(req, "pads"):
+                if len(req.pads) < 2:
+                    continue
+                k1, k2 = self._pad_key(req.pads[0]), self._pad_key(req.pads[1])
+                if k1 in self.pad_to_node and k2 in self.pad_to_node:
+                    s, d = self.pad_to_node[k1], self.pad_to_node[k2]
+                    self.net_pad_ids[req.name] = (k1, k2)
+                    if s != d:
+                        tasks[req.name] = (s, d)
+                    else:
+                        self.net_paths[req.name] = [s]
+            elif isinstance(req, tuple) and len(req) == 3:
+                name, s, d = req
+                if isinstance(s, (int, np.integer)) and isinstance(d, (int, np.integer)):
+                    s, d = int(s), int(d)
+                    if s != d:
+                        tasks[name] = (s, d)
+                    else:
+                        self.net_paths[name] = [s]
+        return tasks
+
+    # ---------------- per-iteration device steps
+    def _update_present_ema(self, first):
+        idx, val = self.accounting.state.update_ema(first, f32(0.40), f32(1.0 - 0.40))
+        return idx, val
+
+    def _edge_layers(self, idx):
+        if len(idx) == 0:
+            return np.zeros(0, np.int64)
+        torch = self.ctx.torch
+        t = torch.from_numpy(np.asarray(idx, dtype=np.int64)).to(self.ctx.device)
+        return self.graph.edge_layer[t].cpu().numpy().astype(np.int64)
+
+    def _compute_layer_bias(self, ema_idx, ema_val, alpha=0.88, max_boost=1.80):
+        """UPF:4825-4878 (bincount in float64 over the nonzeros, same order)."""
+        L = self._Nz
+        layers = self._edge_layers(ema_idx)
+        per = np.bincount(layers, weights=ema_val.astype(np.float64), minlength=L)
+        maxv = float(per.max()) if len(per) else 0.0
+        if maxv <= 1e-9:
+            raw = np.ones(L, dtype=np.float32)
+        else:
+            raw = 1.0 + (1.5 if L <= 2 else 0.75) * (per / maxv)
+        if not hasattr(self, "_layer_bias_ema"):
+            self._layer_bias_ema = raw.astype(np.float32)
+        else:
+            self._layer_bias_ema = (alpha * self._layer_bias_ema + (1.0 - alpha) * raw).astype(np.float32)
+        self._layer_bias_ema = np.clip(self._layer_bias_ema, 1.0, max_boost)
+        return self._layer_bias_ema
+
+    def _via_discount_table(self):
+        """Leave-hot-layer factor of the via pooling penalty (UPF:3356-3367)
+        per (z_lo, z_hi), in the dtype self.layer_bias has at this point."""
+        k = float(getattr(self.config, "via_hot_layer_discount", 0.20))
+        b = self.layer_bias
+        L = self._Nz
+        zl = np.repeat(np.arange(L), L)
+        zh = np.tile(np.arange(L), L)
+        d = (1.0 - k * np.maximum(b[zl], 0.0)) * (1.0 + 0.5 * k * np.maximum(b[zh], 0.0))
+        return d.astype(np.float64), b.dtype == np.float64
+
+    def _update_costs(self, pres_fac, hist_weight, vcm, bcw, layer_bias, it, with_vias=True):
+        torch = self.ctx.torch
+        dev = self.ctx.device
+        keep = []
+
+        def dptr(a):
+            t = torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+            keep.append(t)
+            return ctypes.c_void_p(t.data_ptr())
+        p = _abi.CostParams()
+        p.X, p.Y, p.L = self._Nx, self._Ny, self._Nz
+        p.via_cost_mult, p.base_cost_weight = f32(vcm), f32(bcw)
+        p.pres_fac, p.hist_weight = f32(pres_fac), f32(hist_weight)
+        p.jitter_scale, p.add_jitter = f32(1e-6), 1
+        p.layer_bias = dptr(np.asarray(layer_bias, dtype=np.float32))
+        if with_vias:
+            p.via_pooling = 1
+            p.seg_z = self._segZ
+            p.col_pres = dptr(self.via_col_pres)
+            if self._segZ > 0:
+                p.seg_prefix = dptr(self.via_seg_prefix)
+            p.col_weight = f32(float(getattr(self.config, "via_column_weight", 1.0)))
+            p.seg_weight = f32(float(getattr(self.config, "via_segment_weight", 1.0)))
+            disc, is64 = self._via_discount_table()
+            p.via_discount = dptr(disc)
+            p.discount_f64 = int(is64)
+            if it > 1:
+                p.hard_block = 1
+                p.col_use = dptr(self.via_col_use)
+                if self._segZ > 0:
+                    p.seg_use = dptr(self.via_seg_use)
+                p.col_cap, p.seg_cap = 4, 2
+        self.accounting.state.update_costs(p)
+        self.ctx.sync()
+
+    # ---------------- via usage / ownership (UPF:2678-2904)
+    def _rebuild_via_usage_from_committed(self):
+        self.via_col_use.fill(0)
+        if self._segZ > 0:
+            self.via_seg_use.fill(0)
+        L = self._Nz
+        keep = {}
+        if getattr(self, "_via_keepouts_map", None) is not None:
+            keep = {k: v for k, v in self._via_keepouts_map.items() if v not in self.net_paths}
+        km = dict(keep) if (getattr(self, "_via_keepouts_map", None) is not None) else None
+        for nid, path in self.net_paths.items():
+            if path and len(path) > 1:
+                if km is None:
+                    km = {}
+                hops = self._via_hops(path)
+                for x, y, zlo, zhi in hops:
+                    x, y, zlo, zhi = int(x), int(y), int(zlo), int(zhi)
+                    self.via_col_use[x, y] += 1
+                    if self._segZ > 0:
+                        a, b = max(1, min(zlo, L - 2)), max(1, min(zhi, L - 2))
+                        if b > a:
+                            self.via_seg_use[x, y, a - 1:b - 1] += 1
+                    for z in range(zlo, zhi + 1):
+                        km.setdefault((z, x, y), nid)
+        self._via_keepouts_map = km
+        self._rebuild_node_owner()
+
+    def _rebuild_node_owner(self):
+        self.node_owner.fill(-1)
+        for name, path in self.net_paths.items():
+            if not path or len(path) < 2:
+                continue
+            self._mark_barrels(path, self._get_net_id(name))
+        if self._via_keepouts_map:
+            X, plane = self._Nx, self._Nx * self._Ny
+            for (z, x, y), owner in self._via_keepouts_map.items():
+                if z <= 0:
+                    continue
+                n = z * plane + y * X + x
+                nid = self._get_net_id(owner)
+                if self.node_owner[n] == -1:
+                    self.node_owner[n] = nid
+
+    def _mark_barrels(self, path, nid):
+        X, plane = self._Nx, self._Nx * self._Ny
+        for x, y, zlo, zhi in self._via_hops(path):
+            zs = np.arange(int(zlo), int(zhi) + 1)
+            self.node_owner[zs * plane + int(y) * X + int(x)] = nid
+
+    def _keepout_owner_array(self):
+        own = np.full(self.N, -1, dtype=np.int32)
+        X, plane = self._Nx, self._Nx * self._Ny
+        for (z, x, y), owner in self._via_keepouts_map.items():
+            own[z * plane + y * X + x] = self._get_net_id(owner)
+        return own
+
+    def _detect_barrel_conflicts(self):
+        """UPF:5478-5569: committed edges touching a node owned by another net."""
+        edges, nets, srcs, dsts = [], [], [], []
+        for name, path in self.net_paths.items():
+            if not path or len(path) < 2:
+                continue
+            e = self._net_to_edges.get(name)
+            if e is None or len(e) != len(path) - 1:
+                continue
+            nid = self._get_net_id(name)
+            p = np.asarray(path, dtype=np.int64)
+            edges.append(e); srcs.append(p[:-1]); dsts.append(p[1:])
+            nets.append(np.full(len(e), nid, np.int32))
+        if not edges:
+            return np.zeros(0, np.int32), 0
+        e = np.concatenate(edges); n = np.concatenate(nets)
+        so = self.node_owner[np.concatenate(srcs)]
+        do = self.node_owner[np.concatenate(dsts)]
+        m = ((so != -1) & (so != n)) | ((do != -1) & (do != n))
+        ce = e[m]
+        return ce, int(len(ce))
+
+    # ---------------- bookkeeping (UPF:4989-5007)
+    def _update_net_edge_tracking(self, nid, edges):
+        self._net_to_edges.pop(nid, None)
+        self._net_to_edges[nid] = np.asarray(edges, dtype=np.int64)
+
+    def _clear_net_edge_tracking(self, nid):
+        self._net_to_edges.pop(nid, None)
+
+    # ---------------- ordering / hotset / rip (UPF:4401-4439, 5009-5140, 5300-5335)
+    def _over_lookup(self, idx, val):
+        return dict(zip(idx.tolist(), val.tolist()))
+
+    def _order_nets_by_difficulty(self, tasks, over_map):
+        scores = []
+        for nid, (s, d) in tasks.items():
+            sx, sy, sz = self._coords(s)
+            dx, dy, dz = self._coords(d)
+            dist = abs(dx - sx) + abs(dy - sy) + abs(dz - sz)
+            cong = 0.0
+            if nid in self._net_to_edges:
+                edges = self._net_to_edges[nid]
+                cong = sum(over_map.get(int(e), 0.0) for e in edges) / max(1, len(edges))
+            scores.append((dist * 3 * (cong + 1), nid))
+        scores.sort(reverse=True)
+        rot = self._rng.randint(0, min(5, len(scores) // 10))
+        ordered = [n for _, n in scores]
+        if rot > 0:
+            ordered = ordered[rot:] + ordered[:rot]
+        return ordered
+
+    def _build_hotset(self, tasks, hot_idx, hot_over, ripped=None):
+        ripped = ripped or set()
+        over_idx = set(hot_idx.tolist())
+        over_map = dict(zip(hot_idx.tolist(), hot_over.tolist()))
+        if not hasattr(self, "_net_clean_iters"):
+            self._net_clean_iters = {}
+        if len(over_idx) == 0:
+            unrouted = {n for n in tasks if not self.net_paths.get(n)}
+            return unrouted | ripped
+        offenders = set()
+        hot_sorted = np.asarray(sorted(over_idx), dtype=np.int64)
+        for n, e in self._net_to_edges.items():
+            if len(e) and np.isin(e, hot_sorted, assume_unique=False).any():
+                offenders.add(n)
+        for n in tasks:
+            if n in offenders:
+                self._net_clean_iters[n] = 0
+            else:
+                self._net_clean_iters[n] = self._net_clean_iters.get(n, 0) + 1
+        frozen = {n for n in offenders if self._net_clean_iters.get(n, 0) >= int(getattr(self.config, "freeze_after_clean", 3))}
+        offenders -= frozen
+        offenders |= ripped
+        unrouted = {n for n in tasks if not self.net_paths.get(n)}
+        scores = []
+        for n in offenders:
+            if n in self._net_to_edges:
+                imp = sum(over_map[int(e)] for e in self._net_to_edges[n] if int(e) in over_idx)
+                scores.append((imp, n))
+        for n in unrouted:
+            if n not in offenders:
+                scores.append((0.0, n))
+        scores.sort(reverse=True)
+        cap = min(self.config.hotset_cap, 100)
+        rng = random.Random(42 + self.iteration)
+        k_top = int(0.6 * cap)
+        top = [n for _, n in scores[:k_top]]
+        rest = [n for _, n in scores[k_top:]]
+        k_rand = min(cap - k_top, len(rest))
+        sample = rng.sample(rest, k_rand) if rest else []
+        if not hasattr(self, "_last_reroute_iter"):
+            self._last_reroute_iter = {}
+        cool = [n for n in top + sample if self.iteration - self._last_reroute_iter.get(n, -999) > 1]
+        for n in cool:
+            self._last_reroute_iter[n] = self.iteration
+        return set(cool)
+
+    def _rip_top_k_offenders(self, over_map, k=20):
+        over_idx = set(over_map)
+        scores = []
+        for n, path in self.net_paths.items():
+            if not path or n in self.locked_nets:
+                continue
+            if n in self._net_to_edges:
+                imp = sum(over_map[int(e)] for e in self._net_to_edges[n] if int(e) in over_idx)
+                if imp > 0:
+                    scores.append((imp, n))
+        scores.sort(reverse=True)
+        victims = {n for _, n in scores[:k]}
+        for n in victims:
+            if self.net_paths.get(n) and n in self._net_to_edges:
+                self.accounting.clear_path(self._net_to_edges[n])
+                self.net_paths[n] = []
+                self._clear_net_edge_tracking(n)
+                self.net_clean_streak[n] = 0
+        return victims
+
+    # ---------------- batched search of one iteration (UPF:4441-4823)
+    def _search_params(self, slots):
+        d = self.config.sssp_delta
+        nb = self.config.sssp_buckets
+        if d is None or nb is None:
+            from ...autotune import choose_params
+            d2, nb2 = choose_params(self, slots)
+            d = d if d is not None else d2
+            nb = nb if nb is not None else nb2
+        return SearchParams(slots=slots, delta=float(d), n_buckets=int(nb), rounds_per_poll=32)
+
+    def _search_batch(self, reqs, keepout_owner=None, readd=True):
+        """reqs: list of (src, dst, net_int, roi, stagnation_bonus).  roi
+        False = whole graph.  Returns per request (nodes, edges) or None."""
+        torch = self.ctx.torch
+        dlat = self.graph.dlat
+        out = [None] * len(reqs)
+        roi_reqs, rows_of = [], {}
+        for i, (s, d, nid, roi, bonus) in enumerate(reqs):
+            if roi:
+                rows_of[i] = len(roi_reqs)
+                roi_reqs.append((s, d, self.roi_extractor.roi_depth(s, d, bonus), MAX_ROI_NODES, len(roi_reqs)))
+        rows = None
+        skip = set()
+        t0 = time.perf_counter()
+        if roi_reqs:
+            words = (self.N + 31) // 32
+            rows = torch.empty((len(roi_reqs), words), dtype=torch.int32, device=self.ctx.device)
+            info = roi_bfs(self.ctx, dlat, roi_reqs, rows, readd=readd)
+            for i, r in rows_of.items():
+                if not (info[r][4] and info[r][5]):
+                    skip.add(i)
+        self.stats["roi_s"] += time.perf_counter() - t0
+        todo = [i for i in range(len(reqs)) if i not in skip]
+        if not todo:
+            return out
+        slots = max(1, min(int(self.config.batch_slots), len(todo)))
+        params = self._search_params(slots)
+        sreqs = []
+        for i in todo:
+            s, d, nid, roi, _ = reqs[i]
+            sreqs.append((s, d, nid, rows_of[i] if roi else -1, int(keepout_owner is not None)))
+        ko = None
+        if keepout_owner is not None:
+            ko = torch.from_numpy(keepout_owner).to(self.ctx.device)
+        t0 = time.perf_counter()
+        res, st = device_route(self.ctx, dlat, self.accounting.total_cost, sreqs, params, roi_bitmaps=rows,
+                               keepout_owner=ko)
+        self.stats["sssp_s"] += time.perf_counter() - t0
+        self.stats["sssp_kernel_ms"] += st["kernel_ms"]
+        self.stats["sssp_rounds"] += st["rounds"]
+        self.stats["sssp_edges"] += st["edges"]
+        self.stats["sssp_searches"] += len(sreqs)
+        for j, i in enumerate(todo):
+            r = res[j]
+            out[i] = None if r is None else (r[0].tolist(), r[1])
+        return out
+
+    def _route_all(self, tasks, all_tasks=None, pres_fac=1.0, iteration=0, over_map=None):
+        if all_tasks is None:
+            all_tasks = tasks
+        self.full_graph_fallback_count = 0
+        bonus = self.stagnation_counter * 0.6
+        ordered = self._order_nets_by_difficulty(tasks, over_map or {})
+        total = len(tasks)
+        km = getattr(self, "_via_keepouts_map", None) or {}
+        use_ko = total < 100 and len(km) > 0
+        ko_owner = self._keepout_owner_array() if use_ko else None
+        reqs = []
+        for nid in ordered:
+            s, d = tasks[nid]
+            sx, sy, _ = self._coords(s)
+            dx, dy, _ = self._coords(d)
+            md = abs(dx - sx) + abs(dy - sy)
+            reqs.append((s, d, self._get_net_id(nid) if use_ko else -1, md < ROI_THRESHOLD_STEPS, bonus))
+        results = self._search_batch(reqs, ko_owner, readd=True)
+        # wider-ROI fallback for the first five failures, in routing order
+        fb = [i for i, r in enumerate(results) if r is None or len(r[0]) <= 1][:5]
+        self.full_graph_fallback_count = len(fb)
+        if fb:
+            freqs = [(reqs[i][0], reqs[i][1], -1, True, bonus * 2) for i in fb]
+            fres = self._search_batch(freqs, None, readd=False)
+            for i, r in zip(fb, fres):
+                results[i] = r
+        # commit in routing order (UPF:4783-4812)
+        for nid, r in zip(ordered, results):
+            if nid in self.net_paths and self.net_paths[nid]:
+                old = self._net_to_edges.get(nid)
+                if old is not None:
+                    self.accounting.clear_path(old)
+                self._clear_net_edge_tracking(nid)
+            if r is not None and len(r[0]) > 1:
+                path, edges = r
+                self.accounting.commit_path(edges)
+                self.net_paths[nid] = path
+                self._mark_barrels(path, self._get_net_id(nid))
+                self._update_net_edge_tracking(nid, edges)
+            else:
+                self.net_paths[nid] = []
+                self._clear_net_edge_tracking(nid)
+        routed = sum(1 for p in self.net_paths.values() if p)
+        return routed, len(all_tasks) - routed
+
+    # ---------------- negotiation (UPF:3548-4259)
+    def _pathfinder_negotiation(self, tasks, progress_cb=None, iteration_cb=None):
+        cfg = self.config
+        t_start = time.perf_counter()
+        ch = analyze_board_characteristics(self.lattice, tasks)
+        apply_derived_parameters(cfg, derive_routing_parameters(ch))
+        self.h_layers, self.v_layers = ch.h_layers, ch.v_layers
+        n_sig = self._Nz - 2
+        pres_fac = float(cfg.pres_fac_init)
+        pres_fac_mult = float(cfg.pres_fac_mult)
+        hist_gain = float(cfg.hist_gain)
+        if n_sig <= 12:
+            pres_fac_max, hmult = 32.0, 1.2
+        elif n_sig <= 20:
+            pres_fac_max, hmult = 64.0, 1.0
+        else:
+            pres_fac_max, hmult = 128.0, 0.8
+        pres_fac_mult = float(os.getenv("ORTHO_PRES_FAC_MULT", pres_fac_mult))
+        pres_fac_max = float(os.getenv("ORTHO_PRES_FAC_MAX", pres_fac_max))
+        hist_gain = float(os.getenv("ORTHO_HIST_GAIN", hist_gain))
+        best = float("inf")
+        stagnant = 0
+        self._negotiation_ran = True
+        self._net_failure_count = {}
+        self._excluded_nets = set()
+        routed = failed = 0
+        over_sum = over_cnt = 0
+        jam = None
+        E = self.accounting.E
+        for it in range(1, cfg.max_iterations + 1):
+            t_it = time.perf_counter()
+            self.iteration = it
+            # STEP 0/1: usage counts are maintained incrementally in HBM
+            self.accounting.refresh_from_canonical()
+            self._rebuild_via_usage_from_committed()
+            over = np.maximum(0, self.via_col_use - self.via_col_cap).astype(np.float32)
+            self.via_col_pres = 0.6 * over + 0.4 * self.via_col_pres
+            if self._segZ > 0:
+                over = np.maximum(0, self.via_seg_use - self.via_seg_cap).astype(np.float32)
+                self.via_seg_pres = 0.6 * over + 0.4 * self.via_seg_pres
+                np.cumsum(self.via_seg_pres, axis=2, out=self.via_seg_prefix)
+            # over = max(0, present - 1) at iteration start (counts)
+            p_idx, p_val = self.accounting.state.overuse_sparse(0)
+            over_map = self._over_lookup(p_idx, p_val)
+            ema_idx, ema_val = self._update_present_ema(first=(it == 1))
+            lb = self._compute_layer_bias(ema_idx, ema_val)
+            vcm = 1.0
+            if pres_fac >= 64:
+                if self._via_rank is None:
+                    k = self.graph.edge_kind[:E].to(dtype=self.ctx.torch.int64)
+                    self._via_rank = (self.ctx.torch.cumsum(k, 0) - k).cpu().numpy()
+                    self._n_via = int(k.sum().item())
+                kind = self.graph.edge_kind[:E]
+                kk = kind[self.ctx.torch.from_numpy(p_idx).to(self.ctx.device)].cpu().numpy() if len(p_idx) else np.zeros(0, np.uint8)
+                vsel = kk == 1
+                via_over = float(sum_f32(self._n_via, self._via_rank[p_idx[vsel]], p_val[vsel]))
+                tot_over = float(sum_f32(E, p_idx, p_val))
+                if tot_over > 0:
+                    vcm = 1.5 if via_over / tot_over > 0.7 else 0.5
+            self._update_costs(pres_fac, cfg.hist_cost_weight * hmult, vcm, cfg.base_cost_weight, lb, it)
+            if it > 10 and it % 10 == 0 and self._excluded_nets:
+                self._excluded_nets.clear()
+                self._net_failure_count.clear()
+            if it > 4:
+                for n in tasks:
+                    if n in self._last_ripped:
+                        continue
+                    if not self.net_paths.get(n):
+                        self._net_failure_count[n] = self._net_failure_count.get(n, 0) + 1
+                    else:
+                        self._net_failure_count[n] = 0
+                for n, f in self._net_failure_count.items():
+                    if f >= 5 and n not in self._excluded_nets:
+                        self._excluded_nets.add(n)
+            if cfg.reroute_only_offenders and it > 1:
+                h_idx, h_over = self.accounting.state.overuse_sparse(1, 0.1, 0.5)
+                offenders = self._build_hotset(tasks, h_idx, h_over, ripped=self._last_ripped)
+                offenders -= self._excluded_nets
+                sub = {k: v for k, v in tasks.items() if k in offenders and k not in self._excluded_nets}
+                self._last_ripped = set()
+            else:
+                sub = {k: v for k, v in tasks.items() if k not in self._excluded_nets}
+            if it > 1:
+                ids = list(sub.keys())
+                self._rng.seed(42 + it)
+                self._rng.shuffle(ids)
+                sub = {k: sub[k] for k in ids}
+            t_r = time.perf_counter()
+            routed, failed = self._route_all(sub, all_tasks=tasks, pres_fac=pres_fac, iteration=it, over_map=over_map)
+            self.stats["route_s"] += time.perf_counter() - t_r
+            # present = counts, then the barrel-conflict penalty (UPF:3833-3877)
+            if it <= 50 or it % 10 == 0:
+                ce, cc = self._detect_barrel_conflicts()
+            else:
+                ce, cc = np.zeros(0, np.int32), getattr(self, "_last_barrel_conflict_count", 0)
+            self._last_barrel_conflict_count = cc
+            if cc > 0 and it <= 50:
+                self.accounting.state.build_present(np.unique(ce), min(10.0 * pres_fac, 100.0))
+            else:
+                self.accounting.state.build_present()
+            o_idx, o_val = self.accounting.state.overuse_sparse(0)
+            over_sum = int(sum_f32(E, o_idx, o_val))
+            over_cnt = int(len(o_idx))
+            over_sum += int(np.maximum(0, self.via_col_use - self.via_col_cap).sum())
+            if self._segZ > 0:
+                over_sum += int(np.maximum(0, self.via_seg_use - self.via_seg_cap).sum())
+            by_layer = None
+            if over_sum > 0 and len(o_idx):
+                lay = self._edge_layers(o_idx)
+                by_layer = {}
+                for z in range(1, self._Nz + 1):
+                    m = lay == z
+                    if np.any(m):
+                        by_layer[z] = float(np.sum(o_val[m]))
+            if by_layer and sum(by_layer.values()) > 0:
+                lo = np.array([by_layer.get(z, 0.0) for z in range(self._Nz)])
+                mean = np.mean(lo[lo > 0]) + 1e-9
+                pressure = lo / mean
+                if n_sig <= 12:
+                    a, bmin, bmax = 0.20, 0.60, 1.80
+                elif n_sig <= 20:
+                    a, bmin, bmax = 0.12, 0.75, 1.50
+                else:
+                    a, bmin, bmax = 0.08, 0.85, 1.20
+                target = 1.0 + a * (pressure - 1.0)
+                self.layer_bias = 0.85 * self.layer_bias + 0.15 * target
+                self.layer_bias = np.clip(self.layer_bias, bmin, bmax)
+                tot_l = sum(by_layer.values())
+                if tot_l > 0:
+                    mi, mo = max(by_layer.items(), key=lambda x: x[1])
+                    pct = (mo / tot_l) * 100
+                    if jam is None:
+                        jam = {"layer": None, "count": 0, "boost_until": 0}
+                    if pct >= 60.0:
+                        if jam["layer"] == mi:
+                            jam["count"] += 1
+                        else:
+                            jam = {"layer": mi, "count": 1, "boost_until": 0}
+                        if jam["count"] >= 3 and it > jam["boost_until"]:
+                            old = self.layer_bias[mi]
+                            self.layer_bias[mi] = min(old * 1.8, bmax)
+                            jam["boost_until"] = it + 3
+                    else:
+                        jam = {"layer": None, "count": 0, "boost_until": 0}
+            self.iteration_log.append({"it": it, "over_sum": over_sum, "over_cnt": over_cnt, "pres_fac": pres_fac,
+                                       "routed": routed, "failed": failed, "rerouted": len(sub),
+                                       "t": time.perf_counter() - t_it})
+            if over_sum == 0:
+                unrouted = {n for n in tasks if not self.net_paths.get(n)}
+                if not unrouted:
+                    break
+                pres_fac = min(pres_fac, 10.0)
+            hg = hist_gain
+            if it >= 8 and over_sum > 0:
+                hg = min(1.2, hist_gain * 1.25)
+            use_cap = os.getenv("ORTHO_NO_HISTORY_CAP", "0") == "0"
+            decay = 0.98 if it >= 10 else 1.0
+            self.accounting.state.update_history(f32(decay), f32(hg), f32(15.0 if use_cap else 1.0), use_cap)
+            if progress_cb:
+                try:
+                    progress_cb(it, cfg.max_iterations, f"Iteration {it}")
+                except Exception:
+                    pass
+            if iteration_cb:
+                try:
+                    iteration_cb(it, [], [], over_sum, over_cnt)
+                except Exception as e:
+                    logger.warning(f"[ITER {it}] iteration callback failed: {e}")
+            if failed == 0 and over_sum == 0 and self._last_barrel_conflict_count == 0:
+                self.stats["negotiation_s"] = time.perf_counter() - t_start
+                return {"success": True, "paths": self.net_paths, "converged": True}
+            if over_sum < best:
+                best = over_sum
+                stagnant = 0
+            else:
+                stagnant += 1
+            if stagnant == 2:
+                pres_fac = min(pres_fac * 1.5, pres_fac_max)
+            if stagnant >= cfg.stagnation_patience:
+                self.stagnation_counter += 1
+                self._last_ripped = self._rip_top_k_offenders(self._over_lookup(o_idx, o_val), k=20)
+                self._freeze_pres_fac_until = it + 2
+                stagnant = 0
+                continue
+            if it > self._freeze_pres_fac_until:
+                pres_fac = min(pres_fac * pres_fac_mult, pres_fac_max)
+        if 0 < over_sum <= 100:
+            r = self._detail_pass(tasks, over_sum, over_cnt, o_idx, o_val)
+            if r["success"]:
+                self.stats["negotiation_s"] = time.perf_counter() - t_start
+                return r
+        self.stats["negotiation_s"] = time.perf_counter() - t_start
+        success = failed == 0 and over_sum == 0
+        excluded = getattr(self, "_excluded_nets", set())
+        return {"success": success, "converged": success, "paths": self.net_paths,
+                "barrel_conflicts": getattr(self, "_last_barrel_conflict_count", 0),
+                "excluded_nets": len(excluded), "excluded_net_ids": list(excluded),
+                "error_code": None if success else "ROUTING-FAILED",
+                "message": "Complete" if success else f"{failed} unrouted, {over_cnt} overused",
+                "overuse_sum": over_sum, "overuse_edges": over_cnt, "failed_nets": failed,
+                "layer_recommendation": self._analyze_layer_requirements(failed, over_cnt, over_sum)}
+
+    def _analyze_layer_requirements(self, failed, over_edges, over_sum):
+        """UPF:4261-4304."""
+        L = self.lattice.layers
+        fail_rate = failed / max(1, len(self.net_pad_ids))
+        if fail_rate > 0.4 and over_edges > 200:
+            add = max(4, int(failed / 50))
+            return {"needs_more": True, "additional": add, "recommended_total": L + add,
+                    "reason": f"High failure rate ({fail_rate * 100:.1f}%) with {over_edges} congested edges"}
+        if over_sum > 800 and over_edges > 400:
+            return {"needs_more": True, "additional": 6, "recommended_total": L + 6,
+                    "reason": f"Severe congestion ({over_sum} conflicts across {over_edges} edges)"}
+        if fail_rate > 0.3:
+            add = 4 if fail_rate > 0.35 else 2
+            return {"needs_more": True, "additional": add, "recommended_total": L + add,
+                    "reason": f"Moderate routing failure ({fail_rate * 100:.1f}%)"}
+        return {"needs_more": False, "additional": 0, "recommended_total": L,
+                "reason": f"Failure rate ({fail_rate * 100:.1f}%) and overuse ({over_edges} edges) acceptable"}
+
+    def _detail_pass(self, tasks, initial_overuse, initial_edges, o_idx, o_val):
+        """UPF:4306-4399.  The reference picks the first 60 of a Python set of
+        net names (order depends on the string hash seed); sorted names are used
+        here."""
+        conflict = set(o_idx.tolist())
+        nets = set()
+        for n, path in self.net_paths.items():
+            e = self._net_to_edges.get(n)
+            if path and e is not None and any(int(x) in conflict for x in e):
+                nets.add(n)
+        if not nets:
+            return {"success": False, "error_code": "NO-CONFLICT-NETS"}
+        ctasks = {n: tasks[n] for n in sorted(nets) if n in tasks}
+        cfg = self.config
+        pres_fac = cfg.pres_fac_max * 0.5
+        best = initial_overuse
+        E = self.accounting.E
+        for _ in range(1, 11):
+            self.accounting.state.build_present()
+            ones = np.ones(self._Nz, dtype=np.float32)
+            self._update_costs(pres_fac, cfg.hist_cost_weight * 1.5, 0.3, cfg.base_cost_weight, ones, 0,
+                               with_vias=False)
+            h_idx, h_over = self.accounting.state.overuse_sparse(1, 0.1, 0.5)
+            hs = self._build_hotset(ctasks, h_idx, h_over)
+            hs = set(sorted(hs)[:60]) or nets
+            sub = {k: v for k, v in ctasks.items() if k in hs}
+            p_idx, p_val = self.accounting.state.overuse_sparse(0)
+            old = self.stagnation_counter
+            self.stagnation_counter += 3
+            self._route_all(sub, all_tasks=tasks, pres_fac=pres_fac, over_map=self._over_lookup(p_idx, p_val))
+            self.stagnation_counter = old
+            self.accounting.state.build_present()
+            s, _ = self.accounting.compute_overuse(self)
+            if s == 0:
+                return {"success": True, "paths": self.net_paths}
+            best = min(best, s)
+            # update_history(gain*2, decay 0.98) on present_ema (UPF:4385-4390)
+            self._history_from_ema(f32(0.98), f32(cfg.hist_gain * 2.0))
+            pres_fac = min(pres_fac * 1.5, cfg.pres_fac_max)
+        return {"success": False, "error_code": "DETAIL-INCOMPLETE", "overuse_sum": best}
+
+    def _history_from_ema(self, decay, gain):
+        st = self.accounting.state
+        torch = self.ctx.torch
+        saved = st.present.clone()
+        st.present.copy_(st.present_ema)
+        st.update_history(decay, gain, f32(15.0), True)
+        st.present.copy_(saved)
+
+    # ---------------- geometry (UPF:5677-5914 surface)
+    def emit_geometry(self, board):
+        tracks, vias = self._generate_geometry_from_paths()
+        self._geometry_payload = GeometryPayload(tracks, vias)
+        return len(tracks), len(vias)
+
+    def get_geometry_payload(self):
+        return self._geometry_payload
+
+    def _generate_geometry_from_paths(self):
+        """Track and via records per committed path (the KiCad geometry
+        emission details of the reference are outside this build)."""
+        tracks, vias = [], []
+        names = self.config.layer_names or []
+        for net, path in self.net_paths.items():
+            for a, b in zip(path or [], (path or [])[1:]):
+                xa, ya, za = self._coords(a)
+                xb, yb, zb = self._coords(b)
+                wa = self.lattice.geom.lattice_to_world(xa, ya)
+                wb = self.lattice.geom.lattice_to_world(xb, yb)
+                if za == zb:
+                    tracks.append({"net": net, "layer": names[za] if za < len(names) else za,
+                                   "start": wa, "end": wb})
+                else:
+                    vias.append({"net": net, "at": wa, "from_layer": min(za, zb), "to_layer": max(za, zb)})
+        return tracks, vias
+
+
+UnifiedPathFinder = PathFinderRouter

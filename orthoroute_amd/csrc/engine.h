@@ -39,6 +39,11 @@ struct Engine {
     int64_t sp_cap = 0;
     int32_t *up_buf = nullptr;
     int64_t up_cap = 0;
+    // ROI BFS scratch: per workgroup slot a dst-wave bitmap and two node lists
+    uint32_t *roi_b = nullptr;
+    int32_t *roi_la = nullptr, *roi_lb = nullptr;
+    int64_t roi_n = 0;
+    int32_t roi_slots = 0;
 };
 
 struct orgpu_ctx {

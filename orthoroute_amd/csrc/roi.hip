@@ -1,0 +1,248 @@
+// A2: per-net ROI extraction (bidirectional BFS + truncation), one
+// 1024-thread workgroup per net, many nets per launch.
+//
+// Reference: ROIExtractor.extract_roi_bfs (UPF:1636-1734) and the must-keep
+// re-add in _route_all (UPF:4683-4700).  The BFS is level-synchronous and the
+// set it marks does not depend on the order nodes are visited within a level,
+// so the workgroup expands each frontier in parallel.  Marks: the src wave
+// sets bit A (the output row), the dst wave sets bit B (scratch).  A node the
+// other wave already holds is not re-marked -- touching it only flags `met`,
+// which ends the search after the current step, so the reference's mark 3 is
+// never observed.
+//
+// Truncation keeps the max_nodes lowest node ids (the reference sorts the ROI
+// ascending), minus the nodes its must-keep swap pushes out (positions 0/1 when
+// src/dst fall beyond the cut), then src and dst are re-added.
+#include <algorithm>
+
+#include "common.h"
+#include "engine.h"
+
+#define RWG 1024
+
+struct RoiArgs {
+    int64_t N;
+    int64_t words;
+    const int32_t *indptr, *indices;
+    const orgpu_roi_req *reqs;
+    int n;
+    uint32_t *rows;       // [n_roi][words]
+    int32_t n_roi;
+    uint32_t *scratch_b;  // [slots][words]
+    int32_t *list_a, *list_b;  // [slots][N]
+    orgpu_roi_info *info;  // device [n]
+};
+
+__device__ __forceinline__ bool bit_get(const uint32_t *bm, int32_t v) { return (bm[v >> 5] >> (v & 31)) & 1u; }
+
+__global__ void __launch_bounds__(RWG)
+k_roi(RoiArgs a)
+{
+    __shared__ int s_na, s_nb, s_met, s_seen_i;
+    __shared__ int s_fa0, s_fa1, s_fb0, s_fb1, s_depth;
+    __shared__ uint32_t s_scan[RWG];
+    __shared__ long long s_acc;
+    __shared__ int s_cut, s_min0, s_min1, s_kept_delta;
+    const int tid = threadIdx.x;
+    uint32_t *B = a.scratch_b + (int64_t)blockIdx.x * a.words;
+    int32_t *LA = a.list_a + (int64_t)blockIdx.x * a.N;
+    int32_t *LB = a.list_b + (int64_t)blockIdx.x * a.N;
+    for (int r = blockIdx.x; r < a.n; r += gridDim.x) {
+        const orgpu_roi_req q = a.reqs[r];
+        uint32_t *A = a.rows + (int64_t)q.roi_index * a.words;
+        for (int64_t w = tid; w < a.words; w += RWG) A[w] = 0u;
+        __syncthreads();
+        if (tid == 0) {
+            LA[0] = q.src; LB[0] = q.dst;
+            A[q.src >> 5] |= 1u << (q.src & 31);
+            if (q.dst != q.src) B[q.dst >> 5] |= 1u << (q.dst & 31);
+            s_na = 1; s_nb = q.dst != q.src ? 1 : 0;
+            s_fa0 = 0; s_fa1 = 1; s_fb0 = 0; s_fb1 = s_nb;
+            s_met = 0; s_seen_i = q.dst != q.src ? 2 : 1; s_depth = 0;
+        }
+        __syncthreads();
+        const double stop = (double)q.max_nodes * 1.5;
+        while (true) {
+            const bool go = s_depth < q.max_depth && (s_fa1 > s_fa0 || s_fb1 > s_fb0) && !s_met;
+            __syncthreads();
+            if (!go) break;
+            for (int wave = 0; wave < 2; ++wave) {
+                if (wave == 1 && s_met) break;
+                const int f0 = wave == 0 ? s_fa0 : s_fb0;
+                const int f1 = wave == 0 ? s_fa1 : s_fb1;
+                uint32_t *mine = wave == 0 ? A : B;
+                const uint32_t *other = wave == 0 ? B : A;
+                int32_t *L = wave == 0 ? LA : LB;
+                int *cnt = wave == 0 ? &s_na : &s_nb;
+                for (int i = f0 + tid; i < f1; i += RWG) {
+                    const int32_t u = L[i];
+                    const int e1 = a.indptr[u + 1];
+                    for (int e = a.indptr[u]; e < e1; ++e) {
+                        const int32_t v = a.indices[e];
+                        if (bit_get(other, v)) { s_met = 1; continue; }
+                        const uint32_t m = 1u << (v & 31);
+                        const uint32_t old = atomicOr(&mine[v >> 5], m);
+                        if (!(old & m)) {
+                            const int pos = atomicAdd(cnt, 1);
+                            L[pos] = v;
+                            atomicAdd(&s_seen_i, 1);
+                        }
+                    }
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    if (wave == 0) { s_fa0 = f1; s_fa1 = s_na; }
+                    else { s_fb0 = f1; s_fb1 = s_nb; }
+                }
+                __syncthreads();
+            }
+            if (tid == 0) s_depth = s_depth + 1;
+            __syncthreads();
+            if ((double)s_seen_i > stop) break;
+        }
+        // seen = A | B: move the dst wave into the output row, clear scratch
+        const int nb_total = s_nb;
+        for (int i = tid; i < nb_total; i += RWG) {
+            const int32_t v = LB[i];
+            atomicOr(&A[v >> 5], 1u << (v & 31));
+            atomicAnd(&B[v >> 5], ~(1u << (v & 31)));
+        }
+        __syncthreads();
+        const long long seen = s_seen_i;
+        long long kept = seen;
+        if (seen > q.max_nodes) {
+            // cut = the max_nodes-th smallest seen node id (1-based rank)
+            if (tid == 0) { s_acc = 0; s_cut = -1; }
+            __syncthreads();
+            for (int64_t base = 0; base < a.words; base += RWG) {
+                const int64_t w = base + tid;
+                const uint32_t word = w < a.words ? A[w] : 0u;
+                s_scan[tid] = (uint32_t)__popc(word);
+                __syncthreads();
+                for (int o = 1; o < RWG; o <<= 1) {
+                    const uint32_t t = tid >= o ? s_scan[tid - o] : 0u;
+                    __syncthreads();
+                    s_scan[tid] += t;
+                    __syncthreads();
+                }
+                const long long acc = s_acc;
+                const long long incl = acc + s_scan[tid];
+                const long long excl = incl - __popc(word);
+                if (excl < q.max_nodes && incl >= q.max_nodes) {
+                    int need = (int)(q.max_nodes - excl);
+                    uint32_t x = word;
+                    int bit = 0;
+                    for (bit = 0; bit < 32; ++bit) {
+                        if ((x >> bit) & 1u) { if (--need == 0) break; }
+                    }
+                    s_cut = (int)(w * 32 + bit);
+                }
+                __syncthreads();
+                if (s_cut >= 0) break;
+                if (tid == 0) s_acc = acc + s_scan[RWG - 1];
+                __syncthreads();
+            }
+            const int cut = s_cut;
+            // the two lowest seen ids (array positions 0 and 1 of the sorted ROI)
+            if (tid == 0) { s_min0 = 0x7FFFFFFF; s_min1 = 0x7FFFFFFF; s_kept_delta = 0; }
+            __syncthreads();
+            const int na_total = s_na;
+            for (int i = tid; i < na_total + nb_total; i += RWG) {
+                const int32_t v = i < na_total ? LA[i] : LB[i - na_total];
+                atomicMin(&s_min0, v);
+                if (v > cut) atomicAnd(&A[v >> 5], ~(1u << (v & 31)));
+            }
+            __syncthreads();
+            for (int i = tid; i < na_total + nb_total; i += RWG) {
+                const int32_t v = i < na_total ? LA[i] : LB[i - na_total];
+                if (v != s_min0) atomicMin(&s_min1, v);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                kept = q.max_nodes;
+                const int32_t k0 = min(q.src, q.dst), k1 = max(q.src, q.dst);
+                // must-keep swaps push positions 0 / 1 out of the kept range
+                if (k0 > cut && (A[s_min0 >> 5] >> (s_min0 & 31) & 1u)) { A[s_min0 >> 5] &= ~(1u << (s_min0 & 31)); kept--; }
+                if (k1 > cut && (A[s_min1 >> 5] >> (s_min1 & 31) & 1u)) { A[s_min1 >> 5] &= ~(1u << (s_min1 & 31)); kept--; }
+                s_kept_delta = (int)(kept - q.max_nodes);
+            }
+            __syncthreads();
+            kept = q.max_nodes + s_kept_delta;
+        }
+        if (tid == 0) {
+            // _route_all re-adds src and dst when missing
+            const int32_t ks[2] = {q.src, q.dst};
+            if (q.readd) {
+                for (int j = 0; j < 2; ++j) {
+                    const int32_t v = ks[j];
+                    if (!((A[v >> 5] >> (v & 31)) & 1u)) { A[v >> 5] |= 1u << (v & 31); kept++; }
+                }
+            }
+            if (a.info) {
+                a.info[r].seen = seen;
+                a.info[r].kept = kept;
+                a.info[r].depth = s_depth;
+                a.info[r].met = s_met;
+                a.info[r].src_in = (A[q.src >> 5] >> (q.src & 31)) & 1u;
+                a.info[r].dst_in = (A[q.dst >> 5] >> (q.dst & 31)) & 1u;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+static int ensure_roi_scratch(Engine &en, int64_t N, int slots)
+{
+    const int64_t words = (N + 31) / 32;
+    if (en.roi_n >= N && en.roi_slots >= slots) return ORGPU_OK;
+    if (en.roi_b) { (void)hipFree(en.roi_b); (void)hipFree(en.roi_la); (void)hipFree(en.roi_lb); }
+    en.roi_b = nullptr; en.roi_la = nullptr; en.roi_lb = nullptr; en.roi_n = 0; en.roi_slots = 0;
+    HIP_CHECK_RET(hipMalloc((void **)&en.roi_b, (size_t)slots * words * sizeof(uint32_t)));
+    HIP_CHECK_RET(hipMemset(en.roi_b, 0, (size_t)slots * words * sizeof(uint32_t)));
+    HIP_CHECK_RET(hipMalloc((void **)&en.roi_la, (size_t)slots * N * sizeof(int32_t)));
+    HIP_CHECK_RET(hipMalloc((void **)&en.roi_lb, (size_t)slots * N * sizeof(int32_t)));
+    en.roi_n = N; en.roi_slots = slots;
+    return ORGPU_OK;
+}
+
+extern "C" int orgpu_roi_bfs(orgpu_t *h, const orgpu_graph *g, const orgpu_roi_req *reqs, int32_t n,
+                             uint32_t *roi_bitmaps, int32_t n_roi, orgpu_roi_info *info)
+{
+    if (!h || !g || n < 0 || (n > 0 && (!reqs || !roi_bitmaps))) return orgpu_fail(ORGPU_EINVAL, "roi_bfs: bad argument");
+    if (n == 0) return ORGPU_OK;
+    const int64_t N = g->n_nodes;
+    if (N <= 0 || N >= KEY_MAX_NODES || !g->indptr || !g->indices) return orgpu_fail(ORGPU_EINVAL, "roi_bfs: bad graph");
+    for (int i = 0; i < n; ++i) {
+        const orgpu_roi_req &q = reqs[i];
+        if (q.src < 0 || q.src >= N || q.dst < 0 || q.dst >= N || q.roi_index < 0 || q.roi_index >= n_roi ||
+            q.max_nodes < 1 || q.max_depth < 0)
+            return orgpu_fail(ORGPU_EINVAL, "roi_bfs: request %d invalid (src %d dst %d row %d of %d)", i, q.src, q.dst,
+                              q.roi_index, n_roi);
+        for (int j = 0; j < i; ++j)
+            if (reqs[j].roi_index == q.roi_index)
+                return orgpu_fail(ORGPU_EINVAL, "roi_bfs: requests %d and %d share bitmap row %d", j, i, q.roi_index);
+    }
+    HIP_CHECK_RET(hipSetDevice(h->device));
+    Engine &en = h->en;
+    // concurrent workgroups, each with ~9 bytes of scratch per node
+    const int slots = std::min<int>(n, (int)std::max<int64_t>(1, std::min<int64_t>(128, (int64_t)(8ll << 30) / (N * 9 + 1))));
+    int rc = ensure_roi_scratch(en, N, slots);
+    if (rc) return rc;
+    hipStream_t s = h->stream;
+    orgpu_roi_req *dreq = nullptr;
+    orgpu_roi_info *dinfo = nullptr;
+    HIP_CHECK_RET(hipMalloc((void **)&dreq, sizeof(orgpu_roi_req) * n));
+    HIP_CHECK_RET(hipMalloc((void **)&dinfo, sizeof(orgpu_roi_info) * n));
+    HIP_CHECK_RET(hipMemcpyAsync(dreq, reqs, sizeof(orgpu_roi_req) * n, hipMemcpyHostToDevice, s));
+    RoiArgs a;
+    a.N = N; a.words = (N + 31) / 32; a.indptr = g->indptr; a.indices = g->indices;
+    a.reqs = dreq; a.n = n; a.rows = roi_bitmaps; a.n_roi = n_roi;
+    a.scratch_b = en.roi_b; a.list_a = en.roi_la; a.list_b = en.roi_lb; a.info = dinfo;
+    hipLaunchKernelGGL(k_roi, dim3(slots), dim3(RWG), 0, s, a);
+    HIP_CHECK_RET(hipGetLastError());
+    if (info) HIP_CHECK_RET(hipMemcpyAsync(info, dinfo, sizeof(orgpu_roi_info) * n, hipMemcpyDeviceToHost, s));
+    HIP_CHECK_RET(hipStreamSynchronize(s));
+    HIP_CHECK_RET(hipFree(dreq));
+    HIP_CHECK_RET(hipFree(dinfo));
+    return ORGPU_OK;
+}

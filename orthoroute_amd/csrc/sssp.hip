@@ -549,6 +549,9 @@ void engine_release(Engine &en)
     if (en.sp_count) hipFree(en.sp_count);
     if (en.h_sp_count) hipHostFree(en.h_sp_count);
     if (en.up_buf) hipFree(en.up_buf);
+    if (en.roi_b) hipFree(en.roi_b);
+    if (en.roi_la) hipFree(en.roi_la);
+    if (en.roi_lb) hipFree(en.roi_lb);
     Engine fresh;
     fresh.stream = en.stream;
     fresh.device = en.device;

@@ -174,3 +174,111 @@ def route(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=None
              "minimal_nodes": st.minimal_nodes, "minimal_edges": st.minimal_edges,
              "kernel_ms": st.kernel_ms}
     return out, stats
+
+
+def roi_bfs(ctx, lat, requests, rows, readd=True):
+    """Batched ROI extraction.  requests: list of (src, dst, max_depth,
+    max_nodes, row); rows: device u32[n_roi][ceil(N/32)] written in place.
+    readd: re-add src/dst after truncation (the main search of _route_all) or
+    not (its wider-ROI fallback).  Returns a list of
+    (seen, kept, depth, met, src_in, dst_in)."""
+    n = len(requests)
+    if n == 0:
+        return []
+    reqs = (_abi.RoiReq * n)()
+    for i, (s, d, depth, mx, row) in enumerate(requests):
+        reqs[i].src, reqs[i].dst, reqs[i].max_depth = int(s), int(d), int(depth)
+        reqs[i].roi_index, reqs[i].max_nodes = int(row), int(mx)
+        reqs[i].readd = int(bool(readd))
+    info = (_abi.RoiInfo * n)()
+    g = lat.graph(lat.base_cost)
+    _abi.check(ctx.lib.orgpu_roi_bfs(ctx.handle, ctypes.byref(g), ctypes.cast(reqs, ctypes.c_void_p), n,
+                                     _ptr(rows), int(rows.shape[0]), ctypes.cast(info, ctypes.c_void_p)))
+    return [(info[i].seen, info[i].kept, info[i].depth, info[i].met, bool(info[i].src_in), bool(info[i].dst_in))
+            for i in range(n)]
+
+
+class Sparse:
+    """Host buffers receiving the nonzeros of a per-edge quantity."""
+
+    def __init__(self, capacity):
+        self.capacity = int(capacity)
+        self.idx = np.empty(max(self.capacity, 1), dtype=np.int32)
+        self.val = np.empty(max(self.capacity, 1), dtype=np.float32)
+
+    def c(self):
+        s = _abi.SparseOut()
+        s.idx = self.idx.ctypes.data_as(ctypes.c_void_p)
+        s.val = self.val.ctypes.data_as(ctypes.c_void_p)
+        s.capacity = self.capacity
+        s.count = 0
+        return s
+
+    def fetch(self, fn, *args):
+        """Call fn(*args, &out); returns (idx, val) sorted by edge index."""
+        s = self.c()
+        rc = fn(*args, ctypes.byref(s))
+        if rc == -4 and s.count > self.capacity:
+            self.__init__(int(s.count * 1.25) + 1024)
+            s = self.c()
+            rc = fn(*args, ctypes.byref(s))
+        _abi.check(rc)
+        n = int(s.count)
+        idx = self.idx[:n].astype(np.int64)
+        val = self.val[:n].copy()
+        order = np.argsort(idx, kind="stable")
+        return idx[order], val[order]
+
+
+class EdgeState:
+    """EdgeAccountant state in HBM (UPF:799-1043): canonical usage counts,
+    present, present_ema, history and the iteration's total cost."""
+
+    def __init__(self, ctx, lat):
+        torch = ctx.torch
+        self.ctx, self.lat = ctx, lat
+        E, dev = max(lat.E, 1), ctx.device
+        self.count = torch.zeros(E, dtype=torch.int32, device=dev)
+        self.present = torch.zeros(E, dtype=torch.float32, device=dev)
+        self.present_ema = torch.zeros(E, dtype=torch.float32, device=dev)
+        self.history = torch.zeros(E, dtype=torch.float32, device=dev)
+        self.total_cost = torch.zeros(E, dtype=torch.float32, device=dev)
+        self.sparse = Sparse(1 << 16)
+
+    def scatter(self, edge_ids, delta):
+        ids = np.ascontiguousarray(edge_ids, dtype=np.int32)
+        if len(ids) == 0:
+            return
+        _abi.check(self.ctx.lib.orgpu_usage_scatter(self.ctx.handle, _ptr(self.count), self.lat.E,
+                                                    ids.ctypes.data_as(ctypes.c_void_p), len(ids), int(delta)))
+
+    def build_present(self, penalty_edges=None, penalty=0.0):
+        pe = np.ascontiguousarray(penalty_edges if penalty_edges is not None else [], dtype=np.int32)
+        _abi.check(self.ctx.lib.orgpu_present_build(self.ctx.handle, self.lat.E, _ptr(self.count), _ptr(self.present),
+                                                    pe.ctypes.data_as(ctypes.c_void_p), len(pe),
+                                                    ctypes.c_float(np.float32(penalty))))
+
+    def update_ema(self, first, beta, one_minus_beta):
+        L = self.ctx.lib
+        return self.sparse.fetch(L.orgpu_present_ema, self.ctx.handle, self.lat.E, _ptr(self.count),
+                                 _ptr(self.present_ema), int(first), ctypes.c_float(beta),
+                                 ctypes.c_float(one_minus_beta))
+
+    def overuse_sparse(self, mode=0, hist_scale=0.1, threshold=0.5):
+        L = self.ctx.lib
+        return self.sparse.fetch(L.orgpu_overuse_sparse, self.ctx.handle, self.lat.E, _ptr(self.present),
+                                 _ptr(self.history), int(mode), ctypes.c_float(hist_scale),
+                                 ctypes.c_float(threshold))
+
+    def update_costs(self, params):
+        lat = self.lat
+        _abi.check(self.ctx.lib.orgpu_cost_update(self.ctx.handle, ctypes.byref(params), lat.N, lat.E,
+                                                  _ptr(lat.indptr), _ptr(lat.indices), _ptr(lat.edge_kind),
+                                                  _ptr(lat.base_cost), _ptr(self.present_ema), _ptr(self.history),
+                                                  _ptr(self.total_cost)))
+
+    def update_history(self, decay, gain, cap_mult, use_cap=True):
+        _abi.check(self.ctx.lib.orgpu_history_update(self.ctx.handle, self.lat.E, _ptr(self.present),
+                                                     _ptr(self.lat.base_cost), _ptr(self.history),
+                                                     ctypes.c_float(decay), ctypes.c_float(gain),
+                                                     ctypes.c_float(cap_mult), int(bool(use_cap))))
