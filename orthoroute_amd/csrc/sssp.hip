@@ -345,11 +345,16 @@ __device__ void control_step(const RoundArgs &a, uint32_t mode, uint64_t cur)
         SlotInfo *si = &a.slots[s];
         if (si->state != 1) continue;
         const uint64_t kd = atomic_load_u64(&a.keys[(uint64_t)s * (uint64_t)a.N + (uint64_t)si->dst]);
-        bool found = false;
-        if (kd != KEY_INF && si->base + key_bucket_rel(kd, a.inv_delta) < newcur) found = true;
-        if (found) { si->state = 2; atomicSub(&c->active, 1u); atomicAdd(&c->epoch, 1u); }
-        else if ((int32_t)ld32((const uint32_t *)&si->pending) <= 0) { si->state = 3; atomicSub(&c->active, 1u); atomicAdd(&c->epoch, 1u); }
-        else atomicOr(&s_any_active, 1u);
+        // no queued items left: the search is exhausted and every key final
+        const bool exhausted = (int32_t)ld32((const uint32_t *)&si->pending) <= 0;
+        const bool reached = kd != KEY_INF;
+        if (reached && (exhausted || si->base + key_bucket_rel(kd, a.inv_delta) < newcur)) {
+            si->state = 2; atomicSub(&c->active, 1u); atomicAdd(&c->epoch, 1u);
+        } else if (exhausted) {
+            si->state = 3; atomicSub(&c->active, 1u); atomicAdd(&c->epoch, 1u);
+        } else {
+            atomicOr(&s_any_active, 1u);
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -623,8 +628,10 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
 
     // slot_drained[k]: the slot's previous search has no queued items left
     std::vector<char> slot_drained(K, 1);
+    int last_seeded = 0;
     auto seed_free_slots = [&]() -> int {
         int ns = 0;
+        last_seeded = 0;
         for (int k = 0; k < K && next < n_nets; ++k) {
             if (slot_req[k] >= 0 || !slot_drained[k]) continue;
             const orgpu_net_req &r = nets[next];
@@ -640,6 +647,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
         HIP_CHECK_RET(hipMemcpyAsync(en.seed, en.h_seed, sizeof(SeedReq) * ns, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_seed, dim3((ns + 255) / 256), dim3(256), 0, s, a, en.seed, ns);
         HIP_CHECK_RET(hipGetLastError());
+        last_seeded = ns;
         return 0;
     };
     int rc = seed_free_slots();
@@ -675,7 +683,10 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
             if (searching == 0) {
                 rc = seed_free_slots();
                 if (rc) return rc;
-                if (en.h_ctrl->mode == MODE_IDLE && next >= n_nets)
+                // the status snapshot predates this seeding; with nothing
+                // seeded, nothing searching and the engine idle, the
+                // remaining requests can never finish
+                if (last_seeded == 0 && en.h_ctrl->mode == MODE_IDLE && next >= n_nets)
                     return orgpu_fail(ORGPU_EHIP, "route: engine idle with %d unfinished requests",
                                       n_nets - done_nets);
             }
