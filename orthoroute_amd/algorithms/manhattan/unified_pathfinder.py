@@ -357,6 +357,10 @@ class PathFinderRouter:
         self.locked_nets: Set[str] = set()
         self.net_clean_streak: Dict[str, int] = defaultdict(int)
         self._net_to_edges: Dict[str, np.ndarray] = {}
+        self._net_nodes: Dict[str, np.ndarray] = {}
+        self._net_hops: Dict[str, np.ndarray] = {}
+        self._ko_nodes = np.zeros(0, np.int64)      # via keepouts: node, first owner
+        self._ko_owner = np.zeros(0, np.int64)
         self.portals = {}
         self.net_pad_ids: Dict[str, Tuple[str, str]] = {}
         self.stagnation_counter = 0
@@ -624,48 +628,71 @@ class PathFinderRouter:
         self.accounting.state.update_costs(p)
         self.ctx.sync()
 
-    # ---------------- via usage / ownership (UPF:2678-2904)
-    def _rebuild_via_usage_from_committed(self):
-        self.via_col_use.fill(0)
-        if self._segZ > 0:
-            self.via_seg_use.fill(0)
-        L = self._Nz
-        keep = {}
-        if getattr(self, "_via_keepouts_map", None) is not None:
-            keep = {k: v for k, v in self._via_keepouts_map.items() if v not in self.net_paths}
-        km = dict(keep) if (getattr(self, "_via_keepouts_map", None) is not None) else None
-        for nid, path in self.net_paths.items():
-            if path and len(path) > 1:
-                if km is None:
-                    km = {}
-                hops = self._via_hops(path)
-                for x, y, zlo, zhi in hops:
-                    x, y, zlo, zhi = int(x), int(y), int(zlo), int(zhi)
-                    self.via_col_use[x, y] += 1
-                    if self._segZ > 0:
-                        a, b = max(1, min(zlo, L - 2)), max(1, min(zhi, L - 2))
-                        if b > a:
-                            self.via_seg_use[x, y, a - 1:b - 1] += 1
-                    for z in range(zlo, zhi + 1):
-                        km.setdefault((z, x, y), nid)
-        self._via_keepouts_map = km
-        self._rebuild_node_owner()
-
-    def _rebuild_node_owner(self):
-        self.node_owner.fill(-1)
+    # ---------------- committed-net arrays
+    def _flat(self):
+        """Flat arrays over the nets that hold a path, in net_paths order:
+        per edge (edge id, src node, dst node, net id) and per via hop
+        (x, y, z_lo, z_hi, net id)."""
+        E, S, D, NE, H, NH = [], [], [], [], [], []
         for name, path in self.net_paths.items():
             if not path or len(path) < 2:
                 continue
-            self._mark_barrels(path, self._get_net_id(name))
-        if self._via_keepouts_map:
-            X, plane = self._Nx, self._Nx * self._Ny
-            for (z, x, y), owner in self._via_keepouts_map.items():
-                if z <= 0:
-                    continue
-                n = z * plane + y * X + x
-                nid = self._get_net_id(owner)
-                if self.node_owner[n] == -1:
-                    self.node_owner[n] = nid
+            nid = self._get_net_id(name)
+            nodes = self._net_nodes.get(name)
+            if nodes is None:
+                nodes = np.asarray(path, dtype=np.int64)
+            e = self._net_to_edges.get(name)
+            if e is not None and len(e) == len(nodes) - 1:
+                E.append(e); S.append(nodes[:-1]); D.append(nodes[1:])
+                NE.append(np.full(len(e), nid, np.int64))
+            h = self._net_hops.get(name)
+            if h is None:
+                h = self._via_hops(path)
+            if len(h):
+                H.append(h); NH.append(np.full(len(h), nid, np.int64))
+        cat = lambda xs, w=None: (np.concatenate(xs) if xs else np.zeros((0,) if w is None else (0, w), np.int64))
+        return cat(E), cat(S), cat(D), cat(NE), cat(H, 4), cat(NH)
+
+    # ---------------- via usage / ownership (UPF:2678-2904)
+    def _rebuild_via_usage_from_committed(self, flat):
+        """Column/segment via usage, keepout owners (first owner wins) and the
+        node ownership map from every committed path."""
+        _, _, _, _, H, NH = flat
+        X, Y, L = self._Nx, self._Ny, self._Nz
+        plane = X * Y
+        self.via_col_use.fill(0)
+        if self._segZ > 0:
+            self.via_seg_use.fill(0)
+        if len(H):
+            x, y, zlo, zhi = H[:, 0], H[:, 1], H[:, 2], H[:, 3]
+            np.add.at(self.via_col_use, (x, y), 1)
+            if self._segZ > 0:
+                a = np.clip(zlo, 1, L - 2)
+                b = np.clip(zhi, 1, L - 2)
+                n = np.maximum(b - a, 0)
+                if n.sum():
+                    rep = np.repeat(np.arange(len(n)), n)
+                    off = np.arange(int(n.sum())) - np.repeat(np.cumsum(n) - n, n)
+                    np.add.at(self.via_seg_use, (x[rep], y[rep], (a[rep] - 1) + off), 1)
+            # barrel nodes (z_lo..z_hi inclusive), in committed order
+            span = zhi - zlo + 1
+            rep = np.repeat(np.arange(len(span)), span)
+            zz = np.repeat(zlo, span) + (np.arange(int(span.sum())) - np.repeat(np.cumsum(span) - span, span))
+            nodes = zz * plane + y[rep] * X + x[rep]
+            owner = NH[rep]
+            # keepouts: first committed owner of each (z, x, y)
+            u, first = np.unique(nodes, return_index=True)
+            self._ko_nodes, self._ko_owner = u, owner[first]
+            # node ownership: last committed owner of each barrel node wins;
+            # every keepout cell is a barrel node, so the reference's
+            # keepout fill of unowned cells (UPF:2890-2904) changes nothing
+            ur, last_rev = np.unique(nodes[::-1], return_index=True)
+            self.node_owner.fill(-1)
+            self.node_owner[ur] = owner[::-1][last_rev]
+        else:
+            self.node_owner.fill(-1)
+            self._ko_nodes = np.zeros(0, np.int64)
+            self._ko_owner = np.zeros(0, np.int64)
 
     def _mark_barrels(self, path, nid):
         X, plane = self._Nx, self._Nx * self._Ny
@@ -675,46 +702,47 @@ class PathFinderRouter:
 
     def _keepout_owner_array(self):
         own = np.full(self.N, -1, dtype=np.int32)
-        X, plane = self._Nx, self._Nx * self._Ny
-        for (z, x, y), owner in self._via_keepouts_map.items():
-            own[z * plane + y * X + x] = self._get_net_id(owner)
+        own[self._ko_nodes] = self._ko_owner
         return own
 
-    def _detect_barrel_conflicts(self):
+    def _detect_barrel_conflicts(self, flat):
         """UPF:5478-5569: committed edges touching a node owned by another net."""
-        edges, nets, srcs, dsts = [], [], [], []
-        for name, path in self.net_paths.items():
-            if not path or len(path) < 2:
-                continue
-            e = self._net_to_edges.get(name)
-            if e is None or len(e) != len(path) - 1:
-                continue
-            nid = self._get_net_id(name)
-            p = np.asarray(path, dtype=np.int64)
-            edges.append(e); srcs.append(p[:-1]); dsts.append(p[1:])
-            nets.append(np.full(len(e), nid, np.int32))
-        if not edges:
-            return np.zeros(0, np.int32), 0
-        e = np.concatenate(edges); n = np.concatenate(nets)
-        so = self.node_owner[np.concatenate(srcs)]
-        do = self.node_owner[np.concatenate(dsts)]
-        m = ((so != -1) & (so != n)) | ((do != -1) & (do != n))
-        ce = e[m]
+        E, S, D, NE, _, _ = flat
+        if len(E) == 0:
+            return np.zeros(0, np.int64), 0
+        so = self.node_owner[S]
+        do = self.node_owner[D]
+        m = ((so != -1) & (so != NE)) | ((do != -1) & (do != NE))
+        ce = E[m]
         return ce, int(len(ce))
 
     # ---------------- bookkeeping (UPF:4989-5007)
-    def _update_net_edge_tracking(self, nid, edges):
+    def _update_net_edge_tracking(self, nid, edges, path=None):
         self._net_to_edges.pop(nid, None)
         self._net_to_edges[nid] = np.asarray(edges, dtype=np.int64)
+        if path is not None:
+            self._net_nodes[nid] = np.asarray(path, dtype=np.int64)
+            self._net_hops[nid] = self._via_hops(path)
 
     def _clear_net_edge_tracking(self, nid):
         self._net_to_edges.pop(nid, None)
+        self._net_nodes.pop(nid, None)
+        self._net_hops.pop(nid, None)
 
     # ---------------- ordering / hotset / rip (UPF:4401-4439, 5009-5140, 5300-5335)
-    def _over_lookup(self, idx, val):
-        return dict(zip(idx.tolist(), val.tolist()))
+    def _dense(self, idx, val):
+        d = np.zeros(self.accounting.E, dtype=np.float32)
+        d[idx] = val
+        return d
 
-    def _order_nets_by_difficulty(self, tasks, over_map):
+    @staticmethod
+    def _seqsum(vals):
+        """Python's float sum (sequential float64), as the reference's sum()."""
+        if len(vals) == 0:
+            return 0.0
+        return float(np.cumsum(vals.astype(np.float64))[-1])
+
+    def _order_nets_by_difficulty(self, tasks, over_dense):
         scores = []
         for nid, (s, d) in tasks.items():
             sx, sy, sz = self._coords(s)
@@ -723,7 +751,7 @@ class PathFinderRouter:
             cong = 0.0
             if nid in self._net_to_edges:
                 edges = self._net_to_edges[nid]
-                cong = sum(over_map.get(int(e), 0.0) for e in edges) / max(1, len(edges))
+                cong = self._seqsum(over_dense[edges]) / max(1, len(edges))
             scores.append((dist * 3 * (cong + 1), nid))
         scores.sort(reverse=True)
         rot = self._rng.randint(0, min(5, len(scores) // 10))
@@ -732,34 +760,34 @@ class PathFinderRouter:
             ordered = ordered[rot:] + ordered[:rot]
         return ordered
 
-    def _build_hotset(self, tasks, hot_idx, hot_over, ripped=None):
+    def _build_hotset(self, tasks, hot_idx, hot_over, flat, ripped=None):
         ripped = ripped or set()
-        over_idx = set(hot_idx.tolist())
-        over_map = dict(zip(hot_idx.tolist(), hot_over.tolist()))
         if not hasattr(self, "_net_clean_iters"):
             self._net_clean_iters = {}
-        if len(over_idx) == 0:
+        if len(hot_idx) == 0:
             unrouted = {n for n in tasks if not self.net_paths.get(n)}
             return unrouted | ripped
-        offenders = set()
-        hot_sorted = np.asarray(sorted(over_idx), dtype=np.int64)
-        for n, e in self._net_to_edges.items():
-            if len(e) and np.isin(e, hot_sorted, assume_unique=False).any():
-                offenders.add(n)
+        E, _, _, NE, _, _ = flat
+        hot_mask = np.zeros(self.accounting.E, dtype=bool)
+        hot_mask[hot_idx] = True
+        hot_val = np.zeros(self.accounting.E, dtype=np.float32)
+        hot_val[hot_idx] = hot_over
+        id_to_name = {v: k for k, v in self.net_id_map.items()}
+        offenders = {id_to_name[int(i)] for i in np.unique(NE[hot_mask[E]])} if len(E) else set()
         for n in tasks:
             if n in offenders:
                 self._net_clean_iters[n] = 0
             else:
                 self._net_clean_iters[n] = self._net_clean_iters.get(n, 0) + 1
-        frozen = {n for n in offenders if self._net_clean_iters.get(n, 0) >= int(getattr(self.config, "freeze_after_clean", 3))}
-        offenders -= frozen
+        thr = int(getattr(self.config, "freeze_after_clean", 3))
+        offenders -= {n for n in offenders if self._net_clean_iters.get(n, 0) >= thr}
         offenders |= ripped
         unrouted = {n for n in tasks if not self.net_paths.get(n)}
         scores = []
         for n in offenders:
             if n in self._net_to_edges:
-                imp = sum(over_map[int(e)] for e in self._net_to_edges[n] if int(e) in over_idx)
-                scores.append((imp, n))
+                e = self._net_to_edges[n]
+                scores.append((self._seqsum(hot_val[e][hot_mask[e]]), n))
         for n in unrouted:
             if n not in offenders:
                 scores.append((0.0, n))
@@ -778,14 +806,17 @@ class PathFinderRouter:
             self._last_reroute_iter[n] = self.iteration
         return set(cool)
 
-    def _rip_top_k_offenders(self, over_map, k=20):
-        over_idx = set(over_map)
+    def _rip_top_k_offenders(self, o_idx, o_val, k=20):
+        over = self._dense(o_idx, o_val)
+        mask = np.zeros(self.accounting.E, dtype=bool)
+        mask[o_idx] = True
         scores = []
         for n, path in self.net_paths.items():
             if not path or n in self.locked_nets:
                 continue
             if n in self._net_to_edges:
-                imp = sum(over_map[int(e)] for e in self._net_to_edges[n] if int(e) in over_idx)
+                e = self._net_to_edges[n]
+                imp = self._seqsum(over[e][mask[e]])
                 if imp > 0:
                     scores.append((imp, n))
         scores.sort(reverse=True)
@@ -861,10 +892,11 @@ class PathFinderRouter:
             all_tasks = tasks
         self.full_graph_fallback_count = 0
         bonus = self.stagnation_counter * 0.6
-        ordered = self._order_nets_by_difficulty(tasks, over_map or {})
+        if over_map is None:
+            over_map = np.zeros(self.accounting.E, dtype=np.float32)
+        ordered = self._order_nets_by_difficulty(tasks, over_map)
         total = len(tasks)
-        km = getattr(self, "_via_keepouts_map", None) or {}
-        use_ko = total < 100 and len(km) > 0
+        use_ko = total < 100 and len(self._ko_nodes) > 0
         ko_owner = self._keepout_owner_array() if use_ko else None
         reqs = []
         for nid in ordered:
@@ -894,7 +926,7 @@ class PathFinderRouter:
                 self.accounting.commit_path(edges)
                 self.net_paths[nid] = path
                 self._mark_barrels(path, self._get_net_id(nid))
-                self._update_net_edge_tracking(nid, edges)
+                self._update_net_edge_tracking(nid, edges, path)
             else:
                 self.net_paths[nid] = []
                 self._clear_net_edge_tracking(nid)
@@ -935,7 +967,8 @@ class PathFinderRouter:
             self.iteration = it
             # STEP 0/1: usage counts are maintained incrementally in HBM
             self.accounting.refresh_from_canonical()
-            self._rebuild_via_usage_from_committed()
+            flat = self._flat()
+            self._rebuild_via_usage_from_committed(flat)
             over = np.maximum(0, self.via_col_use - self.via_col_cap).astype(np.float32)
             self.via_col_pres = 0.6 * over + 0.4 * self.via_col_pres
             if self._segZ > 0:
@@ -944,7 +977,7 @@ class PathFinderRouter:
                 np.cumsum(self.via_seg_pres, axis=2, out=self.via_seg_prefix)
             # over = max(0, present - 1) at iteration start (counts)
             p_idx, p_val = self.accounting.state.overuse_sparse(0)
-            over_map = self._over_lookup(p_idx, p_val)
+            over_map = self._dense(p_idx, p_val)
             ema_idx, ema_val = self._update_present_ema(first=(it == 1))
             lb = self._compute_layer_bias(ema_idx, ema_val)
             vcm = 1.0
@@ -977,7 +1010,7 @@ class PathFinderRouter:
                         self._excluded_nets.add(n)
             if cfg.reroute_only_offenders and it > 1:
                 h_idx, h_over = self.accounting.state.overuse_sparse(1, 0.1, 0.5)
-                offenders = self._build_hotset(tasks, h_idx, h_over, ripped=self._last_ripped)
+                offenders = self._build_hotset(tasks, h_idx, h_over, flat, ripped=self._last_ripped)
                 offenders -= self._excluded_nets
                 sub = {k: v for k, v in tasks.items() if k in offenders and k not in self._excluded_nets}
                 self._last_ripped = set()
@@ -993,7 +1026,7 @@ class PathFinderRouter:
             self.stats["route_s"] += time.perf_counter() - t_r
             # present = counts, then the barrel-conflict penalty (UPF:3833-3877)
             if it <= 50 or it % 10 == 0:
-                ce, cc = self._detect_barrel_conflicts()
+                ce, cc = self._detect_barrel_conflicts(self._flat())
             else:
                 ce, cc = np.zeros(0, np.int32), getattr(self, "_last_barrel_conflict_count", 0)
             self._last_barrel_conflict_count = cc
@@ -1081,7 +1114,7 @@ class PathFinderRouter:
                 pres_fac = min(pres_fac * 1.5, pres_fac_max)
             if stagnant >= cfg.stagnation_patience:
                 self.stagnation_counter += 1
-                self._last_ripped = self._rip_top_k_offenders(self._over_lookup(o_idx, o_val), k=20)
+                self._last_ripped = self._rip_top_k_offenders(o_idx, o_val, k=20)
                 self._freeze_pres_fac_until = it + 2
                 stagnant = 0
                 continue
@@ -1125,11 +1158,12 @@ class PathFinderRouter:
         """UPF:4306-4399.  The reference picks the first 60 of a Python set of
         net names (order depends on the string hash seed); sorted names are used
         here."""
-        conflict = set(o_idx.tolist())
+        conflict = np.zeros(self.accounting.E, dtype=bool)
+        conflict[o_idx] = True
         nets = set()
         for n, path in self.net_paths.items():
             e = self._net_to_edges.get(n)
-            if path and e is not None and any(int(x) in conflict for x in e):
+            if path and e is not None and conflict[e].any():
                 nets.add(n)
         if not nets:
             return {"success": False, "error_code": "NO-CONFLICT-NETS"}
@@ -1144,13 +1178,13 @@ class PathFinderRouter:
             self._update_costs(pres_fac, cfg.hist_cost_weight * 1.5, 0.3, cfg.base_cost_weight, ones, 0,
                                with_vias=False)
             h_idx, h_over = self.accounting.state.overuse_sparse(1, 0.1, 0.5)
-            hs = self._build_hotset(ctasks, h_idx, h_over)
+            hs = self._build_hotset(ctasks, h_idx, h_over, self._flat())
             hs = set(sorted(hs)[:60]) or nets
             sub = {k: v for k, v in ctasks.items() if k in hs}
             p_idx, p_val = self.accounting.state.overuse_sparse(0)
             old = self.stagnation_counter
             self.stagnation_counter += 3
-            self._route_all(sub, all_tasks=tasks, pres_fac=pres_fac, over_map=self._over_lookup(p_idx, p_val))
+            self._route_all(sub, all_tasks=tasks, pres_fac=pres_fac, over_map=self._dense(p_idx, p_val))
             self.stagnation_counter = old
             self.accounting.state.build_present()
             s, _ = self.accounting.compute_overuse(self)

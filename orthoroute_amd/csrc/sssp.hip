@@ -58,13 +58,47 @@ __device__ __forceinline__ uint32_t logical_block()
     return (b % 8) * (g / 8) + b / 8;
 }
 
-struct PushCtx {
-    uint32_t out_buf, ovf_buf;
-    uint32_t ring[MAX_RING];
+// ---------------------------------------------------------------------------
+// Queue appends are staged per wavefront in LDS, one segment per destination
+// queue: a full segment (or the end of the round) is written out with a
+// single atomicAdd on the queue's counter and a contiguous, coalesced store.
+// Without staging every (wave, queue, 64-edge step) pays a device-scope atomic
+// on one of a handful of counters, and those serialise.
+// ---------------------------------------------------------------------------
+#define SEG 128
+#define MAX_NB 16                    // ring buckets supported (LDS staging size)
+#define MAX_STAGED (MAX_NB + 3)
+
+// Lanes of a wave exchange the staged items and counts through LDS.  Every
+// access is volatile: one lane reads what another lane wrote, which the
+// compiler would otherwise treat as a value only this lane can change and
+// forward from an earlier load.  A wave's LDS operations execute in order.
+struct Stage {
+    volatile uint32_t (*seg)[SEG];   // [MAX_STAGED][SEG] of this wave
+    volatile uint32_t *n;            // [MAX_STAGED]
 };
 
-// Wave-aggregated append: one atomicAdd per (wave, destination queue).
-__device__ __forceinline__ void wave_push(const RoundArgs &a, bool want, uint32_t buf, uint32_t item)
+__device__ __forceinline__ void stage_flush(const RoundArgs &a, const Stage &st, uint32_t b)
+{
+    const uint32_t n = st.n[b];
+    if (n == 0) return;
+    const uint32_t lane = lane_id();
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&a.ctrl->count[b], n);
+    base = __shfl(base, 0);
+    for (uint32_t i = lane; i < n; i += 64) {
+        const uint64_t pos = (uint64_t)base + i;
+        if (pos < a.qcap) a.queues[(uint64_t)b * a.qcap + pos] = st.seg[b][i];
+        else a.ctrl->overflow = 1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) st.n[b] = 0;
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Wave-aggregated append of `item` to queue `buf` for the lanes with `want`.
+__device__ __forceinline__ void stage_push(const RoundArgs &a, const Stage &st, bool want, uint32_t buf,
+                                           uint32_t item)
 {
     uint64_t pending = __ballot(want);
     const uint32_t lane = lane_id();
@@ -73,17 +107,32 @@ __device__ __forceinline__ void wave_push(const RoundArgs &a, bool want, uint32_
         const uint32_t lb = __shfl(buf, leader);
         const bool mine = want && buf == lb;
         const uint64_t grp = __ballot(mine);
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&a.ctrl->count[lb], (uint32_t)__popcll(grp));
-        base = __shfl(base, leader);
+        const uint32_t cnt = (uint32_t)__popcll(grp);
+        uint32_t have = st.n[lb];
+        if (have + cnt > SEG) {
+            stage_flush(a, st, lb);
+            have = 0;
+        }
         if (mine) {
-            const uint64_t pos = (uint64_t)base + (uint64_t)__popcll(grp & ((1ull << lane) - 1ull));
-            if (pos < a.qcap) a.queues[(uint64_t)lb * a.qcap + pos] = item;
-            else a.ctrl->overflow = 1;
+            st.seg[lb][have + (uint32_t)__popcll(grp & ((1ull << lane) - 1ull))] = item;
             want = false;
         }
+        __builtin_amdgcn_wave_barrier();
+        if ((int)lane == leader) st.n[lb] = have + cnt;
+        __builtin_amdgcn_wave_barrier();
         pending &= ~grp;
     }
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)x, o, 64), hi = __shfl_xor((uint32_t)(x >> 32), o, 64);
+        const uint64_t y = ((uint64_t)hi << 32) | lo;
+        x = y < x ? y : x;
+    }
+    return x;
 }
 
 __device__ __forceinline__ bool test_and_set_bit(uint32_t *bm, uint32_t item)
@@ -120,7 +169,10 @@ k_round(RoundArgs a)
     __shared__ uint32_t s_out, s_ovf, s_ovfn, s_in;
     __shared__ unsigned long long s_items, s_edges;
     __shared__ int s_last;
+    __shared__ uint32_t s_seg[WAVES_PER_WG][MAX_STAGED][SEG];
+    __shared__ uint32_t s_seg_n[WAVES_PER_WG][MAX_STAGED];
     for (int i = threadIdx.x; i < a.K; i += WG) s_slot_delta[i] = 0;
+    for (int i = threadIdx.x; i < WAVES_PER_WG * MAX_STAGED; i += WG) (&s_seg_n[0][0])[i] = 0;
     if (threadIdx.x < nb) s_ring[threadIdx.x] = c->ring[threadIdx.x];
     if (threadIdx.x == 0) {
         s_out = c->out_buf; s_ovf = c->ovf_buf; s_ovfn = c->ovf_next_buf;
@@ -144,6 +196,12 @@ k_round(RoundArgs a)
     const uint64_t end = std::min<uint64_t>(start + chunk, n_in);
     const uint64_t N = (uint64_t)a.N;
     unsigned long long my_items = 0, my_edges = 0;
+    const Stage st{s_seg[wave], s_seg_n[wave]};
+    // lowest overflow bucket this lane saw a key move into.  It counts every
+    // such key, also one whose item already sits in the overflow queue (its
+    // dedupe bit was set): that item's bucket just dropped, and the bucket
+    // jump out of an empty ring must not pass it.
+    uint64_t my_ovf_min = ~0ull;
 
     for (uint64_t base = start; base < end; base += 64) {
         const uint64_t i = base + lane;
@@ -173,9 +231,10 @@ k_round(RoundArgs a)
                         // redistribution: move into the ring or keep in overflow
                         uint32_t dst;
                         if (b < cur + nb) dst = s_ring[b % nb];
-                        else { dst = s_ovfn; atomicMin((unsigned long long *)&c->ovf_min, (unsigned long long)b); }
+                        else dst = s_ovfn;
                         live = !test_and_set_bit(a.bits + (uint64_t)dst * a.bitmap_words, item);
                         if (live) atomicAdd(&s_slot_delta[slot], 1);
+                        if (dst == s_ovfn && b < my_ovf_min) my_ovf_min = b;
                         // reuse `row` to carry the destination queue id
                         row = (int)dst;
                     }
@@ -183,7 +242,7 @@ k_round(RoundArgs a)
             }
         }
         if (mode == MODE_REDIST) {
-            wave_push(a, live, (uint32_t)row, item);
+            stage_push(a, st, live, (uint32_t)row, item);
             continue;
         }
         // ---- relax: flatten the edges of the 64 items over the lanes ----
@@ -241,19 +300,23 @@ k_round(RoundArgs a)
                                 uint64_t b = si.base + key_bucket_rel(nk, a.inv_delta);
                                 if (b <= cur) buf = s_out;
                                 else if (b < cur + nb) buf = s_ring[b % nb];
-                                else {
-                                    buf = s_ovf;
-                                    atomicMin((unsigned long long *)&c->ovf_min, (unsigned long long)b);
-                                }
+                                else buf = s_ovf;
                                 want = !test_and_set_bit(a.bits + (uint64_t)buf * a.bitmap_words, vitem);
                                 if (want) atomicAdd(&s_slot_delta[slot_j], 1);
+                                if (buf == s_ovf && b < my_ovf_min) my_ovf_min = b;
                             }
                         }
                     }
                 }
             }
-            wave_push(a, want, buf, vitem);
+            stage_push(a, st, want, buf, vitem);
         }
+    }
+    // drain this wave's staged appends; one overflow-minimum update per wave
+    for (uint32_t b = 0; b < nb + 3; ++b) stage_flush(a, st, b);
+    {
+        const uint64_t m = wave_min_u64(my_ovf_min);
+        if (lane == 0 && m != ~0ull) atomicMin((unsigned long long *)&c->ovf_min, (unsigned long long)m);
     }
 
     // ---- per-workgroup flush, then the last workgroup runs the control step
@@ -382,6 +445,13 @@ struct SeedReq {
     int32_t slot, src, dst, net_id, roi_index, use_keepout, req, pad;
 };
 
+// Marks finished slots free (their queued items, if any, keep draining).
+__global__ void k_release(SlotInfo *slots, const int32_t *list, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) slots[list[i]].state = 0;
+}
+
 __global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n)
 {
     Ctrl *c = a.ctrl;
@@ -481,7 +551,7 @@ static int launch_round(Engine &en, const RoundArgs &a, int grid)
 int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets, int64_t qcap, int32_t max_path)
 {
     if (slots < 1 || slots > MAX_SLOTS) return orgpu_fail(ORGPU_EINVAL, "reserve: slots %d outside [1, %d]", slots, MAX_SLOTS);
-    if (n_buckets < 1 || n_buckets > MAX_RING) return orgpu_fail(ORGPU_EINVAL, "reserve: n_buckets %d outside [1, %d]", n_buckets, MAX_RING);
+    if (n_buckets < 1 || n_buckets > MAX_NB) return orgpu_fail(ORGPU_EINVAL, "reserve: n_buckets %d outside [1, %d]", n_buckets, MAX_NB);
     if (n_nodes < 1 || n_nodes >= KEY_MAX_NODES) return orgpu_fail(ORGPU_EINVAL, "reserve: bad node count");
     if ((uint64_t)n_nodes * (uint64_t)slots >= (1ull << 32))
         return orgpu_fail(ORGPU_EINVAL, "reserve: slots*nodes = %lld exceeds 2^32 work items",
@@ -514,7 +584,7 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
     HIP_CHECK_RET(hipHostMalloc((void **)&en.h_slot_list, sizeof(int32_t) * slots, hipHostMallocDefault));
     HIP_CHECK_RET(hipHostMalloc((void **)&en.h_lens, sizeof(int32_t) * slots, hipHostMallocDefault));
     HIP_CHECK_RET(hipHostMalloc((void **)&en.h_dists, sizeof(float) * slots, hipHostMallocDefault));
-    HIP_CHECK_RET(hipHostMalloc((void **)&en.h_stage, (size_t)2 * max_path * sizeof(int32_t), hipHostMallocDefault));
+    HIP_CHECK_RET(hipHostMalloc((void **)&en.h_stage, (size_t)2 * slots * max_path * sizeof(int32_t), hipHostMallocDefault));
     HIP_CHECK_RET(hipHostMalloc((void **)&en.h_counters, sizeof(unsigned long long) * 2, hipHostMallocDefault));
     HIP_CHECK_RET(hipEventCreate(&en.ev_poll));
     HIP_CHECK_RET(hipEventCreate(&en.ev_t0));
@@ -699,12 +769,28 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
         HIP_CHECK_RET(hipMemcpyAsync(en.h_lens, en.lens, sizeof(int32_t) * K, hipMemcpyDeviceToHost, s));
         HIP_CHECK_RET(hipMemcpyAsync(en.h_dists, en.dists, sizeof(float) * K, hipMemcpyDeviceToHost, s));
         HIP_CHECK_RET(hipStreamSynchronize(s));
+        // all finished paths come back in one synchronisation; finished slot
+        // i stages at h_stage + 2*i*cap_path (nodes, then edges)
+        const size_t cap = (size_t)en.cap_path;
+        for (int i = 0; i < nf; ++i) {
+            const int k = en.h_slot_list[i];
+            const int len = en.h_lens[k];
+            if (len < 0) return orgpu_fail(ORGPU_ECAPACITY, "route: request %d: predecessor chain broken or longer than %d",
+                                           slot_req[k], en.cap_path);
+            if (len == 0) continue;
+            int32_t *h = en.h_stage + 2 * (size_t)i * cap;
+            HIP_CHECK_RET(hipMemcpyAsync(h, en.stage_nodes + (size_t)k * cap, sizeof(int32_t) * len,
+                                         hipMemcpyDeviceToHost, s));
+            if (path_edges)
+                HIP_CHECK_RET(hipMemcpyAsync(h + cap, en.stage_edges + (size_t)k * cap, sizeof(int32_t) * len,
+                                             hipMemcpyDeviceToHost, s));
+        }
+        HIP_CHECK_RET(hipStreamSynchronize(s));
         for (int i = 0; i < nf; ++i) {
             const int k = en.h_slot_list[i];
             const int req = slot_req[k];
-            int len = en.h_lens[k];
-            if (len < 0) return orgpu_fail(ORGPU_ECAPACITY, "route: request %d: predecessor chain broken or longer than %d",
-                                           req, en.cap_path);
+            const int len = en.h_lens[k];
+            const int32_t *h = en.h_stage + 2 * (size_t)i * cap;
             if (p->count_minimal && en.h_slots[k].state == 2) {
                 HIP_CHECK_RET(hipMemsetAsync(en.counters, 0, sizeof(unsigned long long) * 2, s));
                 hipLaunchKernelGGL(k_count_minimal, dim3(512), dim3(256), 0, s, a, k, en.counters);
@@ -717,15 +803,10 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
             if (len > 0) {
                 if (out_pos + len > path_capacity)
                     return orgpu_fail(ORGPU_ECAPACITY, "route: path output capacity %lld exceeded", (long long)path_capacity);
-                HIP_CHECK_RET(hipMemcpyAsync(en.h_stage, en.stage_nodes + (int64_t)k * en.cap_path, sizeof(int32_t) * len,
-                                             hipMemcpyDeviceToHost, s));
-                HIP_CHECK_RET(hipMemcpyAsync(en.h_stage + en.cap_path, en.stage_edges + (int64_t)k * en.cap_path,
-                                             sizeof(int32_t) * len, hipMemcpyDeviceToHost, s));
-                HIP_CHECK_RET(hipStreamSynchronize(s));
                 // staging holds dst..src; the edge entering node i sits at i
-                for (int t = 0; t < len; ++t) path_nodes[out_pos + t] = en.h_stage[len - 1 - t];
+                for (int t = 0; t < len; ++t) path_nodes[out_pos + t] = h[len - 1 - t];
                 if (path_edges) {
-                    for (int t = 0; t + 1 < len; ++t) path_edges[out_pos + t] = en.h_stage[en.cap_path + len - 2 - t];
+                    for (int t = 0; t + 1 < len; ++t) path_edges[out_pos + t] = h[cap + len - 2 - t];
                     path_edges[out_pos + len - 1] = -1;
                 }
             }
@@ -738,9 +819,10 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
             ++done_nets;
         }
         // mark the finished slots free on the device before reseeding
+        hipLaunchKernelGGL(k_release, dim3((nf + 255) / 256), dim3(256), 0, s, en.slots, en.slot_list, nf);
+        HIP_CHECK_RET(hipGetLastError());
         for (int i = 0; i < nf; ++i) {
             const int k = en.h_slot_list[i];
-            HIP_CHECK_RET(hipMemsetAsync(&en.slots[k].state, 0, sizeof(uint32_t), s));
             slot_drained[k] = en.h_slots[k].pending <= 0;
         }
         rc = seed_free_slots();

@@ -1,0 +1,90 @@
+"""Synthetic routing workloads (seeded) for the benchmark configurations.
+
+Every generator returns (X, Y, L, requests) with requests as the tuple form
+the reference router accepts, (name, src_node, dst_node) (UPF:2660-2669);
+node = z*X*Y + y*X + x, pins on layer 0.
+
+  cfg1  96x96x6, 300 two-pin nets, endpoints uniform over distinct top-layer
+        cells (all pins unique).
+  cfg2  600x400x18 backplane: 20 connectors of 2 rows x 200 pins at 2-cell
+        pitch (400 cells wide), non-overlapping; 8,000 nets join pin k of
+        connector i to pin k of connector j, (i, j, k) drawn without repeats.
+  cfg3  300x300x12 BGA escape: 40x40 balls at 2-cell pitch, centred; 1,400
+        nets ball -> random perimeter cell (the multi-pin part of this config
+        is routed as its first two pins, as the reference's live path does).
+  cfg4  1000x800x24: 40,000 nets between opposite corners of boxes whose
+        centres are uniform and half-widths lognormal(ln 20, 1.0), clipped.
+  cfg5  serving stream: job sizes log-uniform in lattice nodes, generated like
+        cfg1 (see orthoroute_amd.serving).
+"""
+import numpy as np
+
+
+def _node(X, Y, x, y, z=0):
+    return int((z * Y + y) * X + x)
+
+
+def cfg1(seed=1, X=96, Y=96, L=6, n_nets=300):
+    rng = np.random.default_rng(seed)
+    cells = rng.permutation(X * Y)[: 2 * n_nets]
+    reqs = [(f"N{i:04d}", int(cells[2 * i]), int(cells[2 * i + 1])) for i in range(n_nets)]
+    return X, Y, L, reqs
+
+
+def cfg2(seed=2, X=600, Y=400, L=18, n_conn=20, pins_per_row=200, n_nets=8000):
+    rng = np.random.default_rng(seed)
+    width = 2 * pins_per_row               # cells spanned by one row
+    # rows y0 and y0+2 per connector; stack connectors without overlapping rows
+    slots = (Y - 3) // 3 + 1
+    if slots < n_conn:
+        raise ValueError("board too short for the connectors")
+    ys = np.sort(rng.choice(slots, n_conn, replace=False)) * 3
+    xs = rng.integers(0, max(1, X - width + 1), n_conn)
+    pins_per = 2 * pins_per_row
+
+    def pin(c, k):
+        row, col = divmod(int(k), pins_per_row)
+        return _node(X, Y, int(xs[c]) + 2 * col, int(ys[c]) + 2 * row)
+    seen = set()
+    reqs = []
+    while len(reqs) < n_nets:
+        i, j = (int(v) for v in rng.choice(n_conn, 2, replace=False))
+        k = int(rng.integers(0, pins_per))
+        if (i, j, k) in seen:
+            continue
+        seen.add((i, j, k))
+        reqs.append((f"B{len(reqs):05d}", pin(i, k), pin(j, k)))
+    return X, Y, L, reqs
+
+
+def cfg3(seed=3, X=300, Y=300, L=12, balls=40, n_nets=1400):
+    rng = np.random.default_rng(seed)
+    x0 = (X - 2 * (balls - 1)) // 2
+    y0 = (Y - 2 * (balls - 1)) // 2
+    ball = [(x0 + 2 * i, y0 + 2 * j) for j in range(balls) for i in range(balls)]
+    per = [(x, 0) for x in range(X)] + [(x, Y - 1) for x in range(X)] + \
+          [(0, y) for y in range(1, Y - 1)] + [(X - 1, y) for y in range(1, Y - 1)]
+    bsel = rng.choice(len(ball), n_nets, replace=False)
+    psel = rng.choice(len(per), n_nets, replace=False)
+    reqs = [(f"G{i:04d}", _node(X, Y, *ball[b]), _node(X, Y, *per[p])) for i, (b, p) in enumerate(zip(bsel, psel))]
+    return X, Y, L, reqs
+
+
+def cfg4(seed=4, X=1000, Y=800, L=24, n_nets=40000):
+    rng = np.random.default_rng(seed)
+    used = set()
+    reqs = []
+    while len(reqs) < n_nets:
+        cx, cy = rng.integers(0, X), rng.integers(0, Y)
+        hx, hy = rng.lognormal(np.log(20.0), 1.0, 2)
+        x1, x2 = int(np.clip(cx - hx, 0, X - 1)), int(np.clip(cx + hx, 0, X - 1))
+        y1, y2 = int(np.clip(cy - hy, 0, Y - 1)), int(np.clip(cy + hy, 0, Y - 1))
+        s, d = _node(X, Y, x1, y1), _node(X, Y, x2, y2)
+        if s == d or s in used or d in used:
+            continue
+        used.add(s); used.add(d)
+        reqs.append((f"S{len(reqs):05d}", s, d))
+    return X, Y, L, reqs
+
+
+CONFIGS = {"cfg1": cfg1, "cfg2": cfg2, "cfg3": cfg3, "cfg4": cfg4}

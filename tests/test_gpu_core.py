@@ -60,7 +60,7 @@ def make_costs(o, rng, kind="jitter"):
     return c
 
 
-def run_case(ctx, X, Y, L, policy, slots, n_nets, seed, cost_kind, roi_frac=0.5, keepout=False):
+def run_case(ctx, X, Y, L, policy, slots, n_nets, seed, cost_kind, roi_frac=0.5, keepout=False, dn=None):
     import torch
     from orthoroute_amd.device import SearchParams, route
     say(f"[route] {policy} {X}x{Y}x{L} slots={slots} nets={n_nets} seed={seed} costs={cost_kind} keepout={keepout}")
@@ -90,8 +90,10 @@ def run_case(ctx, X, Y, L, policy, slots, n_nets, seed, cost_kind, roi_frac=0.5,
         reqs.append((int(s), int(d), int(i % 6), roi_idx, int(keepout)))
     roi_d = torch.from_numpy(np.stack(rois)).to(ctx.device) if rois else None
     own_d = torch.from_numpy(owner).to(ctx.device) if owner is not None else None
-    p = SearchParams(slots=slots, delta=float(rng.choice([0.3, 1.0, 4.0])), n_buckets=int(rng.choice([1, 4, 8])),
-                     rounds_per_poll=16)
+    if dn is None:
+        dn = (float(rng.choice([0.3, 1.0, 4.0])), int(rng.choice([1, 4, 8])))
+    say(f"[route]   delta={dn[0]} n_buckets={dn[1]}")
+    p = SearchParams(slots=slots, delta=dn[0], n_buckets=dn[1], rounds_per_poll=16)
     res, stats = route(ctx, g, cost_d, reqs, p, roi_bitmaps=roi_d, keepout_owner=own_d)
     for i, (s, d, nid, roi_idx, ko) in enumerate(reqs):
         path, dist, pred, _, _ = oracle.sssp_canon(o, c, s, d, roi_mask=masks[i],
@@ -126,3 +128,12 @@ def test_route_unreachable_and_refill(ctx):
     # destinations must finish as "no path" without blocking refills
     stats = run_case(ctx, 16, 14, 6, "reference", 3, 30, 5, "jitter", roi_frac=0.0)
     assert stats["rounds"] > 0
+
+
+@pytest.mark.parametrize("nb", [1, 2])
+def test_route_overflow_heavy(ctx, nb):
+    # narrow buckets and a short ring: most keys land in the overflow queue and
+    # many improve again while queued there (lower overflow bucket, item
+    # already present); the bucket jump out of an empty ring must not pass them
+    for seed, kind in [(21, "random"), (22, "jitter")]:
+        run_case(ctx, 24, 20, 6, "adjacent", 4, 16, seed, kind, roi_frac=0.0, dn=(0.05, nb))
