@@ -1,0 +1,262 @@
+#!/usr/bin/env python3
+"""PathFinder routing benchmark on one MI355X.
+
+Workload: the 600x400x18 backplane (4,320,000 lattice nodes, alternating H/V
+layers with vias between adjacent layers, 16,782,000 directed edges, FP32
+costs, via cost 3.0); 20 connectors of 2 rows x 200 pins at 2-cell pitch at
+seeded positions; 8,000 two-pin nets joining pin k of connector i to pin k of
+connector j (orthoroute_amd.workloads.cfg2).
+
+One step = one full PathFinder routing pass over the board: every net is
+(re)routed on the iteration-1 cost snapshot -- ROI extraction for the short
+nets, the batched SSSP, backtrace, commit of the usage counts -- followed by
+the present / overuse accounting.  Inputs (lattice, costs, requests) are
+resident in HBM before the timed region.
+
+value = SSSP GTEPS of the whole step: Dijkstra-minimal edges (out-edges of the
+nodes a label-setting search must settle, i.e. every node whose final cost is
+at most the destination's) per second of step wall time.  The minimal counts
+come from an untimed counting pass over the same searches.
+
+Also reported: plain (one search at a time) vs batched on a sample of the same
+searches; the k_round roofline from HIP-event timing of the round launches;
+the CPU oracle's GTEPS on a bounded sample of the same searches (single core),
+with its paths compared against the GPU's.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+METRIC = "PathFinder time-to-zero-overuse (s) + SSSP GTEPS, batched vs plain, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes of a label-setting search (no re-expansions, no queue or
+# dedupe traffic): per settled node its queue entry (4), own key (8) and CSR
+# row bounds (8); per scanned edge its target id (4), cost (4) and target key (8)
+BYTES_PER_NODE = 20
+BYTES_PER_EDGE = 16
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--slots", type=int, default=64, help="concurrent searches of the batched form")
+    ap.add_argument("--plain-sample", type=int, default=48, help="searches timed in the plain vs batched comparison")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU oracle sample budget (0 = skip)")
+    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run 5 plain device copies of BYTES bytes (HBM counter calibration) and exit")
+    return ap.parse_args()
+
+
+def copy_probe(nbytes):
+    """Five plain device-to-device copies of a known size: the reference point
+    that scales the profiler's FETCH_SIZE / WRITE_SIZE readings to bytes."""
+    import torch
+    src = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "launches": 5}), flush=True)
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "this build has no multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+    if args.steps < 1:
+        sys.exit("--steps must be >= 1")
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
+
+    import numpy as np
+    import torch
+    from orthoroute_amd import workloads
+    from orthoroute_amd.algorithms.manhattan.unified_pathfinder import (PathFinderConfig, PathFinderRouter,
+                                                                        ROI_THRESHOLD_STEPS)
+
+    X, Y, L, reqs = workloads.CONFIGS[args.workload]()
+    cfg = PathFinderConfig()
+    cfg.lattice_policy = "adjacent"
+    cfg.batch_slots = args.slots
+    r = PathFinderRouter(cfg)
+    t0 = time.perf_counter()
+    r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
+    torch.cuda.synchronize()
+    log(f"[bench] lattice {X}x{Y}x{L} N={r.N} E={r.accounting.E} built in {time.perf_counter() - t0:.2f}s")
+    tasks = r._parse_requests(reqs)
+    pres_fac, _, _, _, hmult = r._negotiation_setup(tasks)
+    r.iteration = 1
+    _, over_map = r._begin_iteration(1, pres_fac, hmult)
+    torch.cuda.synchronize()
+    st = r.accounting.state
+
+    def step():
+        routed, failed = r._route_all(tasks, over_map=over_map)
+        st.build_present()
+        o_idx, _ = st.overuse_sparse(0)
+        return routed, failed, len(o_idx)
+
+    def snap():
+        return {k: float(v) for k, v in r.stats.items()}
+
+    # untimed counting pass: Dijkstra-minimal nodes/edges of this step's searches
+    cfg.count_minimal = True
+    s0 = snap()
+    t0 = time.perf_counter()
+    routed, failed, over_cnt = step()
+    s1 = snap()
+    cfg.count_minimal = False
+    min_nodes = s1.get("sssp_minimal_nodes", 0) - s0.get("sssp_minimal_nodes", 0)
+    min_edges = s1.get("sssp_minimal_edges", 0) - s0.get("sssp_minimal_edges", 0)
+    log(f"[bench] counting pass {time.perf_counter() - t0:.2f}s routed={routed} failed={failed} "
+        f"overused_edges={over_cnt} minimal nodes={min_nodes:.4g} edges={min_edges:.4g}")
+    for w in range(max(0, args.warmup - 1)):
+        t0 = time.perf_counter()
+        step()
+        log(f"[bench] warmup {w + 2} {time.perf_counter() - t0:.2f}s")
+
+    torch.cuda.synchronize()
+    s0 = snap()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        routed, failed, over_cnt = step()
+        log(f"[bench] step {k + 1}/{args.steps} {time.perf_counter() - ts:.2f}s routed={routed} failed={failed}")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    s1 = snap()
+    d = {k: s1.get(k, 0.0) - s0.get(k, 0.0) for k in s1}
+    K = args.steps
+    gteps = min_edges * K / wall / 1e9
+    launches = d.get("sssp_launches", 0.0)
+    round_ms = d.get("sssp_round_ms", 0.0)
+    avg_launch_us = 1e3 * round_ms / max(launches, 1)
+    alg_bytes = (BYTES_PER_NODE * min_nodes + BYTES_PER_EDGE * min_edges) * K
+    achieved = alg_bytes / max(round_ms * 1e-3, 1e-12) / 1e9
+    log(f"[bench] {K} steps in {wall:.2f}s; rounds kernel {round_ms / 1e3:.2f}s over {launches:.0f} launches "
+        f"({avg_launch_us:.1f} us avg); edges scanned {d.get('sssp_edges', 0):.4g}")
+
+    # plain (one search at a time) vs batched on the same sample of searches
+    sample = list(tasks.items())[:args.plain_sample]
+    sreqs = []
+    for name, (s, dd) in sample:
+        sx, sy, _ = r._coords(s)
+        dx, dy, _ = r._coords(dd)
+        sreqs.append((s, dd, -1, abs(dx - sx) + abs(dy - sy) < ROI_THRESHOLD_STEPS, 0.0))
+    timing = {}
+    paths = {}
+    for label, slots in (("plain", 1), ("batched", args.slots)):
+        cfg.batch_slots = slots
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = r._search_batch(sreqs)
+        torch.cuda.synchronize()
+        timing[label] = time.perf_counter() - t0
+        paths[label] = [None if o is None else o[0] for o in out]
+        log(f"[bench] {label} sample of {len(sreqs)} searches: {timing[label]:.2f}s")
+    cfg.batch_slots = args.slots
+    same = paths["plain"] == paths["batched"]
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        cpu = cpu_baseline(r, sreqs, paths["batched"], args.cpu_seconds, X, Y, L)
+
+    traffic = None
+    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_round_cfg2_traffic.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    line = {
+        "metric": METRIC,
+        "value": round(gteps, 4),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / K, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload} backplane {X}x{Y}x{L}, adjacent-layer vias, {r.N} nodes, "
+                        f"{r.accounting.E} edges, {len(tasks)} two-pin nets; one step = one full routing "
+                        f"pass on the iteration-1 cost snapshot",
+            "nodes": r.N, "edges": r.accounting.E, "nets": len(tasks), "slots": args.slots,
+            "delta": r._search_params(args.slots).delta, "n_buckets": r._search_params(args.slots).n_buckets,
+        },
+        "nets_per_s": round(len(tasks) * K / wall, 1),
+        "routed": routed, "failed": failed,
+        "minimal_edges_per_step": int(min_edges), "minimal_nodes_per_step": int(min_nodes),
+        "edges_scanned_per_step": int(d.get("sssp_edges", 0) / K),
+        "sssp_s_per_step": round(d.get("sssp_s", 0) / K, 3),
+        "roi_s_per_step": round(d.get("roi_s", 0) / K, 3),
+        "plain_vs_batched": {"searches": len(sreqs), "plain_s": round(timing["plain"], 3),
+                             "batched_s": round(timing["batched"], 3),
+                             "speedup": round(timing["plain"] / max(timing["batched"], 1e-9), 2),
+                             "identical_paths": same},
+        "roofline": {"kernel": "k_round", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "avg_launch_us": round(avg_launch_us, 2), "launches": int(launches),
+                     "alg_bytes_per_launch": round(alg_bytes / max(launches, 1), 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(r, sreqs, gpu_paths, budget_s, X, Y, L):
+    """The oracle's label-setting search (C, one thread) on the same searches
+    and the same cost snapshot, for about budget_s seconds."""
+    import numpy as np
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import oracle
+    t0 = time.perf_counter()
+    lat = oracle.Lattice(X, Y, L, "adjacent", pitch=1.0, via_cost=3.0)
+    costs = r.accounting.total_cost[:r.accounting.E].cpu().numpy()
+    log(f"[bench] cpu oracle lattice + costs in {time.perf_counter() - t0:.2f}s")
+    done = traversed = 0
+    equal = 0
+    t_search = 0.0
+    for (s, d, _, use_roi, bonus), gp in zip(sreqs, gpu_paths):
+        t1 = time.perf_counter()
+        mask = None
+        if use_roi:
+            roi = oracle.roi_bfs(lat, s, d, stagnation_bonus=bonus)
+            mask = np.zeros(lat.N, np.uint8)
+            mask[roi] = 1
+            mask[s] = mask[d] = 1          # the router re-adds both endpoints
+        path, _, _, _, tr = oracle.sssp_canon(lat, costs, s, d, roi_mask=mask)
+        t_search += time.perf_counter() - t1
+        done += 1
+        traversed += tr
+        equal += int((path is None and gp is None) or (path is not None and gp is not None and
+                                                         list(path) == list(gp)))
+        if t_search > budget_s:
+            break
+    log(f"[bench] cpu oracle {done} searches in {t_search:.2f}s, {equal}/{done} paths equal to the GPU's")
+    return {"value": round(traversed / max(t_search, 1e-9) / 1e9, 5), "unit": "GTEPS", "cores": 1,
+            "kind": "port",
+            "sample": f"the first {done} nets of the request list on the step's cost snapshot: oracle ROI BFS "
+                      f"(nets under 125 steps) + label-setting search in C on one thread, {t_search:.1f}s",
+            "searches_per_s": round(done / max(t_search, 1e-9), 3),
+            "paths_equal_to_gpu": f"{equal}/{done}"}
+
+
+if __name__ == "__main__":
+    main()

@@ -115,7 +115,10 @@ typedef struct orgpu_sssp_stats {
     int64_t edges;           /* edges relaxed (incl. re-relaxations) */
     int64_t minimal_nodes;   /* Dijkstra-minimal settled nodes (count_minimal) */
     int64_t minimal_edges;   /* their out-edges (count_minimal) */
-    double kernel_ms;        /* device time of the round kernels (HIP events) */
+    double kernel_ms;        /* device time from first seed to last path (HIP events) */
+    int64_t launches;        /* round kernels launched (idle ones return at once) */
+    double round_ms;         /* device time of the round kernels alone (HIP events
+                                around each back-to-back batch of launches) */
 } orgpu_sssp_stats;
 
 /* Reserves scratch for graphs of up to n_nodes nodes, `slots` concurrent

@@ -60,7 +60,7 @@ class SsspParams(ctypes.Structure):
 class SsspStats(ctypes.Structure):
     _fields_ = [("rounds", ctypes.c_int64), ("items", ctypes.c_int64), ("edges", ctypes.c_int64),
                 ("minimal_nodes", ctypes.c_int64), ("minimal_edges", ctypes.c_int64),
-                ("kernel_ms", ctypes.c_double)]
+                ("kernel_ms", ctypes.c_double), ("launches", ctypes.c_int64), ("round_ms", ctypes.c_double)]
 
 
 class RoiReq(ctypes.Structure):

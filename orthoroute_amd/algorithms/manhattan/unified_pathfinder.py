@@ -121,6 +121,7 @@ class PathFinderConfig:
     sssp_delta = None
     sssp_buckets = None
     max_iterations_cap = None
+    count_minimal = False          # also count Dijkstra-minimal nodes/edges per search
 
 
 # ----------------------------------------------------------------------------
@@ -838,7 +839,8 @@ class PathFinderRouter:
             d2, nb2 = choose_params(self, slots)
             d = d if d is not None else d2
             nb = nb if nb is not None else nb2
-        return SearchParams(slots=slots, delta=float(d), n_buckets=int(nb), rounds_per_poll=32)
+        return SearchParams(slots=slots, delta=float(d), n_buckets=int(nb), rounds_per_poll=32,
+                            count_minimal=bool(self.config.count_minimal))
 
     def _search_batch(self, reqs, keepout_owner=None, readd=True):
         """reqs: list of (src, dst, net_int, roi, stagnation_bonus).  roi
@@ -881,6 +883,11 @@ class PathFinderRouter:
         self.stats["sssp_kernel_ms"] += st["kernel_ms"]
         self.stats["sssp_rounds"] += st["rounds"]
         self.stats["sssp_edges"] += st["edges"]
+        self.stats["sssp_items"] += st["items"]
+        self.stats["sssp_launches"] += st["launches"]
+        self.stats["sssp_round_ms"] += st["round_ms"]
+        self.stats["sssp_minimal_nodes"] += st["minimal_nodes"]
+        self.stats["sssp_minimal_edges"] += st["minimal_edges"]
         self.stats["sssp_searches"] += len(sreqs)
         for j, i in enumerate(todo):
             r = res[j]
@@ -934,9 +941,11 @@ class PathFinderRouter:
         return routed, len(all_tasks) - routed
 
     # ---------------- negotiation (UPF:3548-4259)
-    def _pathfinder_negotiation(self, tasks, progress_cb=None, iteration_cb=None):
+    def _negotiation_setup(self, tasks):
+        """Board analysis, derived parameters and the layer-count schedule
+        (UPF:3548-3600): returns (pres_fac, pres_fac_mult, pres_fac_max,
+        hist_gain, hmult)."""
         cfg = self.config
-        t_start = time.perf_counter()
         ch = analyze_board_characteristics(self.lattice, tasks)
         apply_derived_parameters(cfg, derive_routing_parameters(ch))
         self.h_layers, self.v_layers = ch.h_layers, ch.v_layers
@@ -953,6 +962,50 @@ class PathFinderRouter:
         pres_fac_mult = float(os.getenv("ORTHO_PRES_FAC_MULT", pres_fac_mult))
         pres_fac_max = float(os.getenv("ORTHO_PRES_FAC_MAX", pres_fac_max))
         hist_gain = float(os.getenv("ORTHO_HIST_GAIN", hist_gain))
+        return pres_fac, pres_fac_mult, pres_fac_max, hist_gain, hmult
+
+    def _begin_iteration(self, it, pres_fac, hmult):
+        """STEPS 0-2 of an iteration (UPF:3640-3745): usage refresh, via
+        usage and pooling pressure, overuse snapshot, present EMA, layer bias
+        and the cost update.  Returns (flat committed arrays, dense
+        iteration-start overuse)."""
+        cfg = self.config
+        E = self.accounting.E
+        self.accounting.refresh_from_canonical()
+        flat = self._flat()
+        self._rebuild_via_usage_from_committed(flat)
+        over = np.maximum(0, self.via_col_use - self.via_col_cap).astype(np.float32)
+        self.via_col_pres = 0.6 * over + 0.4 * self.via_col_pres
+        if self._segZ > 0:
+            over = np.maximum(0, self.via_seg_use - self.via_seg_cap).astype(np.float32)
+            self.via_seg_pres = 0.6 * over + 0.4 * self.via_seg_pres
+            np.cumsum(self.via_seg_pres, axis=2, out=self.via_seg_prefix)
+        # over = max(0, present - 1) at iteration start (counts)
+        p_idx, p_val = self.accounting.state.overuse_sparse(0)
+        over_map = self._dense(p_idx, p_val)
+        ema_idx, ema_val = self._update_present_ema(first=(it == 1))
+        lb = self._compute_layer_bias(ema_idx, ema_val)
+        vcm = 1.0
+        if pres_fac >= 64:
+            if self._via_rank is None:
+                k = self.graph.edge_kind[:E].to(dtype=self.ctx.torch.int64)
+                self._via_rank = (self.ctx.torch.cumsum(k, 0) - k).cpu().numpy()
+                self._n_via = int(k.sum().item())
+            kind = self.graph.edge_kind[:E]
+            kk = kind[self.ctx.torch.from_numpy(p_idx).to(self.ctx.device)].cpu().numpy() if len(p_idx) else np.zeros(0, np.uint8)
+            vsel = kk == 1
+            via_over = float(sum_f32(self._n_via, self._via_rank[p_idx[vsel]], p_val[vsel]))
+            tot_over = float(sum_f32(E, p_idx, p_val))
+            if tot_over > 0:
+                vcm = 1.5 if via_over / tot_over > 0.7 else 0.5
+        self._update_costs(pres_fac, cfg.hist_cost_weight * hmult, vcm, cfg.base_cost_weight, lb, it)
+        return flat, over_map
+
+    def _pathfinder_negotiation(self, tasks, progress_cb=None, iteration_cb=None):
+        cfg = self.config
+        t_start = time.perf_counter()
+        pres_fac, pres_fac_mult, pres_fac_max, hist_gain, hmult = self._negotiation_setup(tasks)
+        n_sig = self._Nz - 2
         best = float("inf")
         stagnant = 0
         self._negotiation_ran = True
@@ -965,35 +1018,8 @@ class PathFinderRouter:
         for it in range(1, cfg.max_iterations + 1):
             t_it = time.perf_counter()
             self.iteration = it
-            # STEP 0/1: usage counts are maintained incrementally in HBM
-            self.accounting.refresh_from_canonical()
-            flat = self._flat()
-            self._rebuild_via_usage_from_committed(flat)
-            over = np.maximum(0, self.via_col_use - self.via_col_cap).astype(np.float32)
-            self.via_col_pres = 0.6 * over + 0.4 * self.via_col_pres
-            if self._segZ > 0:
-                over = np.maximum(0, self.via_seg_use - self.via_seg_cap).astype(np.float32)
-                self.via_seg_pres = 0.6 * over + 0.4 * self.via_seg_pres
-                np.cumsum(self.via_seg_pres, axis=2, out=self.via_seg_prefix)
-            # over = max(0, present - 1) at iteration start (counts)
-            p_idx, p_val = self.accounting.state.overuse_sparse(0)
-            over_map = self._dense(p_idx, p_val)
-            ema_idx, ema_val = self._update_present_ema(first=(it == 1))
-            lb = self._compute_layer_bias(ema_idx, ema_val)
-            vcm = 1.0
-            if pres_fac >= 64:
-                if self._via_rank is None:
-                    k = self.graph.edge_kind[:E].to(dtype=self.ctx.torch.int64)
-                    self._via_rank = (self.ctx.torch.cumsum(k, 0) - k).cpu().numpy()
-                    self._n_via = int(k.sum().item())
-                kind = self.graph.edge_kind[:E]
-                kk = kind[self.ctx.torch.from_numpy(p_idx).to(self.ctx.device)].cpu().numpy() if len(p_idx) else np.zeros(0, np.uint8)
-                vsel = kk == 1
-                via_over = float(sum_f32(self._n_via, self._via_rank[p_idx[vsel]], p_val[vsel]))
-                tot_over = float(sum_f32(E, p_idx, p_val))
-                if tot_over > 0:
-                    vcm = 1.5 if via_over / tot_over > 0.7 else 0.5
-            self._update_costs(pres_fac, cfg.hist_cost_weight * hmult, vcm, cfg.base_cost_weight, lb, it)
+            # STEP 0-2: usage counts are maintained incrementally in HBM
+            flat, over_map = self._begin_iteration(it, pres_fac, hmult)
             if it > 10 and it % 10 == 0 and self._excluded_nets:
                 self._excluded_nets.clear()
                 self._net_failure_count.clear()

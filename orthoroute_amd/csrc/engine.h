@@ -31,7 +31,7 @@ struct Engine {
     int32_t *h_slot_list = nullptr, *h_lens = nullptr, *h_stage = nullptr;
     float *h_dists = nullptr;
     unsigned long long *h_counters = nullptr;
-    hipEvent_t ev_poll = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
+    hipEvent_t ev_poll = nullptr, ev_t0 = nullptr, ev_t1 = nullptr, ev_r0 = nullptr, ev_r1 = nullptr;
     // congestion-state helpers: nonzero staging and host->device uploads
     int32_t *sp_idx = nullptr;
     float *sp_val = nullptr;

@@ -589,6 +589,8 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
     HIP_CHECK_RET(hipEventCreate(&en.ev_poll));
     HIP_CHECK_RET(hipEventCreate(&en.ev_t0));
     HIP_CHECK_RET(hipEventCreate(&en.ev_t1));
+    HIP_CHECK_RET(hipEventCreate(&en.ev_r0));
+    HIP_CHECK_RET(hipEventCreate(&en.ev_r1));
     en.cap_nodes = n_nodes; en.cap_slots = slots; en.cap_nb = n_buckets; en.cap_q = qcap; en.cap_path = max_path;
     en.words = words;
     return ORGPU_OK;
@@ -619,6 +621,8 @@ void engine_release(Engine &en)
     if (en.ev_poll) hipEventDestroy(en.ev_poll);
     if (en.ev_t0) hipEventDestroy(en.ev_t0);
     if (en.ev_t1) hipEventDestroy(en.ev_t1);
+    if (en.ev_r0) hipEventDestroy(en.ev_r0);
+    if (en.ev_r1) hipEventDestroy(en.ev_r1);
     if (en.sp_idx) hipFree(en.sp_idx);
     if (en.sp_val) hipFree(en.sp_val);
     if (en.sp_count) hipFree(en.sp_count);
@@ -723,13 +727,23 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     int rc = seed_free_slots();
     if (rc) return rc;
 
+    int64_t launches = 0;
+    double round_ms = 0.0;
     while (done_nets < n_nets) {
+        HIP_CHECK_RET(hipEventRecord(en.ev_r0, s));
         for (int r = 0; r < rpp; ++r) launch_round(en, a, grid);
         HIP_CHECK_RET(hipGetLastError());
+        HIP_CHECK_RET(hipEventRecord(en.ev_r1, s));
+        launches += rpp;
         HIP_CHECK_RET(hipMemcpyAsync(en.h_ctrl, en.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
         HIP_CHECK_RET(hipMemcpyAsync(en.h_slots, en.slots, sizeof(SlotInfo) * K, hipMemcpyDeviceToHost, s));
         HIP_CHECK_RET(hipEventRecord(en.ev_poll, s));
         HIP_CHECK_RET(hipEventSynchronize(en.ev_poll));
+        {
+            float bms = 0.0f;
+            HIP_CHECK_RET(hipEventElapsedTime(&bms, en.ev_r0, en.ev_r1));
+            round_ms += bms;
+        }
         if (en.h_ctrl->overflow)
             return orgpu_fail(ORGPU_ECAPACITY, "route: a bucket queue exceeded %lld items; reserve a larger queue_capacity",
                               (long long)en.cap_q);
@@ -854,6 +868,8 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
         stats->minimal_nodes = (int64_t)min_nodes;
         stats->minimal_edges = (int64_t)min_edges;
         stats->kernel_ms = ms;
+        stats->launches = launches;
+        stats->round_ms = round_ms;
     }
     return ORGPU_OK;
 }

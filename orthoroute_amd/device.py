@@ -172,7 +172,7 @@ def route(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=None
             out.append(None)
     stats = {"rounds": st.rounds, "items": st.items, "edges": st.edges,
              "minimal_nodes": st.minimal_nodes, "minimal_edges": st.minimal_edges,
-             "kernel_ms": st.kernel_ms}
+             "kernel_ms": st.kernel_ms, "launches": st.launches, "round_ms": st.round_ms}
     return out, stats
 
 
