@@ -4,7 +4,7 @@
  * Plain pointers and sizes only.  Every large array is caller-owned device
  * memory (allocated by the caller, e.g. a torch tensor on cuda:0); it must
  * stay alive until the handle's stream has been synchronised.  The handle owns
- * only its scratch (search keys, bucket queues, bitmaps, staging), sized by
+ * only its scratch (search keys, bucket bitmaps, path staging), sized by
  * orgpu_reserve.  Calls are asynchronous on the handle's stream unless the
  * comment says otherwise.  Return value: 0 on success, a negative ORGPU_E*
  * code on failure (detail in orgpu_last_error(), thread-local).  "No path" is
@@ -111,7 +111,7 @@ typedef struct orgpu_sssp_params {
 
 typedef struct orgpu_sssp_stats {
     int64_t rounds;          /* round launches executed */
-    int64_t items;           /* queue items expanded (incl. re-expansions) */
+    int64_t items;           /* frontier items expanded (incl. re-expansions) */
     int64_t edges;           /* edges relaxed (incl. re-relaxations) */
     int64_t minimal_nodes;   /* Dijkstra-minimal settled nodes (count_minimal) */
     int64_t minimal_edges;   /* their out-edges (count_minimal) */
@@ -122,8 +122,11 @@ typedef struct orgpu_sssp_stats {
 } orgpu_sssp_stats;
 
 /* Reserves scratch for graphs of up to n_nodes nodes, `slots` concurrent
- * searches and `n_buckets` ring buckets; queue_capacity items per queue
- * (0 = slots*n_nodes).  Blocks (allocation). */
+ * searches and `n_buckets` ring buckets (at most 32): keys slots*n_nodes*8
+ * bytes, (n_buckets+3) bucket bitmaps of slots*n_nodes bits, path staging
+ * 2*slots*max_path_len ints.  queue_capacity is ignored (the frontiers are
+ * bitmaps; the argument stays for callers of the earlier queue form).
+ * Blocks (allocation). */
 int orgpu_reserve(orgpu_t *h, int64_t n_nodes, int32_t slots, int32_t n_buckets,
                   int64_t queue_capacity, int32_t max_path_len);
 

@@ -55,7 +55,7 @@ __host__ __device__ inline uint64_t key_bucket_rel(uint64_t k, float inv_delta)
 // Search engine layout
 // ---------------------------------------------------------------------------
 #define MAX_RING 32
-// physical queue buffers: ring [0, nb), OUT, OVF, OVF_NEXT
+// physical bucket bitmaps: ring [0, nb), OUT, OVF (and one spare)
 #define MAX_BUFS (MAX_RING + 3)
 
 enum { MODE_IDLE = 0, MODE_RELAX = 1, MODE_REDIST = 2 };
@@ -67,7 +67,7 @@ struct Ctrl {
     uint32_t finished;         // last-workgroup counter
     uint32_t ring[MAX_RING];   // ring position -> physical buffer
     uint32_t out_buf, ovf_buf, ovf_next_buf;
-    uint32_t overflow;         // a queue exceeded its capacity
+    uint32_t unused0;
     uint32_t count[MAX_BUFS];  // items per physical buffer
     uint32_t active;           // slots seeded and not finished
     uint32_t epoch;            // bumps when any slot finishes
@@ -100,8 +100,6 @@ struct RoundArgs {
     int64_t roi_words;
     const int32_t *ko_owner;  // [N] or null
     uint64_t *keys;           // [K][N]
-    uint32_t *queues;         // [MAX_BUFS][qcap]
-    uint64_t qcap;
     uint32_t *bits;           // [MAX_BUFS][bitmap_words]
     uint64_t bitmap_words;
 };

@@ -9,12 +9,11 @@ struct Engine {
     int device = 0;
     hipStream_t stream = nullptr;
     // capacities reserved
-    int64_t cap_nodes = 0, cap_q = 0;
+    int64_t cap_nodes = 0;
     int32_t cap_slots = 0, cap_nb = 0, cap_path = 0;
     uint64_t words = 0;
     // device scratch
     uint64_t *keys = nullptr;
-    uint32_t *queues = nullptr;
     uint32_t *bits = nullptr;
     Ctrl *ctrl = nullptr;
     SlotInfo *slots = nullptr;

@@ -7,10 +7,13 @@
 // with the lowest-predecessor-index tie-break (see common.h for the key).
 //
 // Design (one HIP stream, no host sync inside a round):
-//  * state: keys u64[K][N]; work items are u32 (slot*N + node).
-//  * buckets: a ring of nb queues for buckets cur .. cur+nb-1, an OUT queue
-//    for re-relaxations inside the current bucket and an overflow queue for
-//    buckets >= cur+nb.  Each queue has a dedupe bitmap (one bit per item).
+//  * state: keys u64[K][N]; a work item is the u32 slot*N + node.
+//  * buckets are bitmaps over all items (one bit per (slot, node)): a ring of
+//    nb bitmaps for buckets cur .. cur+nb-1, an OUT bitmap for re-relaxations
+//    inside the current bucket and an overflow bitmap for buckets >= cur+nb.
+//    Setting a bit is the push (and its own de-duplication); a round scans
+//    its bucket's bitmap in item order, so a wavefront's items are neighbours
+//    on the lattice and their CSR rows and neighbour keys share cache lines.
 //  * one kernel launch = one round.  The last workgroup to finish (atomic
 //    counter) runs the control step: swap OUT in, advance the bucket, detect
 //    finished slots, schedule an overflow redistribution or go idle.  The host
@@ -18,11 +21,10 @@
 //  * every slot shares the global bucket index; a slot seeded at bucket c
 //    measures its distances from c (per-slot base), so finished slots are
 //    refilled with the next request while the others keep running.
-//  * a wavefront expands 64 items at once: it scans their degrees, then each
-//    lane takes one edge of the flattened edge list (consecutive lanes read
-//    consecutive CSR entries), relaxes it with a 64-bit atomicMin, and the
-//    improved targets are appended with one atomicAdd per wave and queue
-//    (ballot + popcount compaction).
+//  * a wavefront takes 64 bitmap words (2048 items) at a time, enumerates
+//    their set bits 64 per batch (ballot-free prefix scan + bit select), then
+//    flattens the batch's edges over the lanes (consecutive lanes read
+//    consecutive CSR entries) and relaxes each with a 64-bit atomicMin.
 #include <stdio.h>
 #include <string.h>
 
@@ -34,6 +36,7 @@
 #define WG 256
 #define WAVES_PER_WG (WG / 64)
 #define MAX_SLOTS 1024
+#define MAX_NB MAX_RING
 
 __device__ __forceinline__ uint64_t atomic_load_u64(const uint64_t *p)
 {
@@ -47,93 +50,6 @@ __device__ __forceinline__ uint32_t ld32(const uint32_t *p)
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-
-__device__ __forceinline__ uint32_t logical_block()
-{
-    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
-    // contiguous range of logical blocks (and so of queue items) so that
-    // neighbouring items share that XCD's L2.
-    const uint32_t g = gridDim.x, b = blockIdx.x;
-    if (g % 8 != 0) return b;
-    return (b % 8) * (g / 8) + b / 8;
-}
-
-// ---------------------------------------------------------------------------
-// Queue appends are staged per wavefront in LDS, one segment per destination
-// queue: a full segment (or the end of the round) is written out with a
-// single atomicAdd on the queue's counter and a contiguous, coalesced store.
-// Without staging every (wave, queue, 64-edge step) pays a device-scope atomic
-// on one of a handful of counters, and those serialise.
-// ---------------------------------------------------------------------------
-#define SEG 128
-#define MAX_NB 16                    // ring buckets supported (LDS staging size)
-#define MAX_STAGED (MAX_NB + 3)
-
-// Lanes of a wave exchange the staged items and counts through LDS.  Every
-// access is volatile: one lane reads what another lane wrote, which the
-// compiler would otherwise treat as a value only this lane can change and
-// forward from an earlier load.  A wave's LDS operations execute in order.
-struct Stage {
-    volatile uint32_t (*seg)[SEG];   // [MAX_STAGED][SEG] of this wave
-    volatile uint32_t *n;            // [MAX_STAGED]
-};
-
-__device__ __forceinline__ void stage_flush(const RoundArgs &a, const Stage &st, uint32_t b)
-{
-    const uint32_t n = st.n[b];
-    if (n == 0) return;
-    const uint32_t lane = lane_id();
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&a.ctrl->count[b], n);
-    base = __shfl(base, 0);
-    for (uint32_t i = lane; i < n; i += 64) {
-        const uint64_t pos = (uint64_t)base + i;
-        if (pos < a.qcap) a.queues[(uint64_t)b * a.qcap + pos] = st.seg[b][i];
-        else a.ctrl->overflow = 1;
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (lane == 0) st.n[b] = 0;
-    __builtin_amdgcn_wave_barrier();
-}
-
-// Wave-aggregated append of `item` to queue `buf` for the lanes with `want`.
-__device__ __forceinline__ void stage_push(const RoundArgs &a, const Stage &st, bool want, uint32_t buf,
-                                           uint32_t item)
-{
-    uint64_t pending = __ballot(want);
-    const uint32_t lane = lane_id();
-    while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const uint32_t lb = __shfl(buf, leader);
-        const bool mine = want && buf == lb;
-        const uint64_t grp = __ballot(mine);
-        const uint32_t cnt = (uint32_t)__popcll(grp);
-        uint32_t have = st.n[lb];
-        if (have + cnt > SEG) {
-            stage_flush(a, st, lb);
-            have = 0;
-        }
-        if (mine) {
-            st.seg[lb][have + (uint32_t)__popcll(grp & ((1ull << lane) - 1ull))] = item;
-            want = false;
-        }
-        __builtin_amdgcn_wave_barrier();
-        if ((int)lane == leader) st.n[lb] = have + cnt;
-        __builtin_amdgcn_wave_barrier();
-        pending &= ~grp;
-    }
-}
-
-__device__ __forceinline__ uint64_t wave_min_u64(uint64_t x)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const uint32_t lo = __shfl_xor((uint32_t)x, o, 64), hi = __shfl_xor((uint32_t)(x >> 32), o, 64);
-        const uint64_t y = ((uint64_t)hi << 32) | lo;
-        x = y < x ? y : x;
-    }
-    return x;
-}
 
 __device__ __forceinline__ bool test_and_set_bit(uint32_t *bm, uint32_t item)
 {
@@ -153,7 +69,71 @@ __device__ __forceinline__ int wave_scan(int x)
     return x;
 }
 
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)x, o, 64), hi = __shfl_xor((uint32_t)(x >> 32), o, 64);
+        const uint64_t y = ((uint64_t)hi << 32) | lo;
+        x = y < x ? y : x;
+    }
+    return x;
+}
+
+// Lane of the batch element t: the highest lane j whose exclusive prefix
+// excl_j is <= t (lanes with a zero count share the next lane's prefix, so
+// the highest such lane is the one that owns element t).
+__device__ __forceinline__ int owner_lane(int excl, int t)
+{
+    int j = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        const int cand = j + step;
+        const int ec = __shfl(excl, cand);
+        if (ec <= t) j = cand;
+    }
+    return j;
+}
+
+// position of the r-th (0-based) set bit of w; w has more than r bits set
+__device__ __forceinline__ uint32_t select_bit(uint32_t w, uint32_t r)
+{
+    uint32_t pos = 0, c;
+    c = __popc(w & 0xFFFFu); if (r >= c) { r -= c; w >>= 16; pos += 16; }
+    c = __popc(w & 0xFFu);   if (r >= c) { r -= c; w >>= 8;  pos += 8; }
+    c = __popc(w & 0xFu);    if (r >= c) { r -= c; w >>= 4;  pos += 4; }
+    c = __popc(w & 0x3u);    if (r >= c) { r -= c; w >>= 2;  pos += 2; }
+    if (r >= (w & 1u)) pos += 1;
+    return pos;
+}
+
 __device__ void control_step(const RoundArgs &a, uint32_t mode, uint64_t cur);
+
+// Workgroups are dispatched round-robin over the 8 XCDs.  Each XCD takes one
+// contiguous eighth of the bitmap's chunks (so a slot's keys and the lattice
+// rows its frontier touches stay in that XCD's L2), and inside it the waves
+// stride over the chunks (so a dense region is spread over many waves).
+struct ChunkWalk {
+    uint64_t first, end, stride;
+};
+
+__device__ __forceinline__ ChunkWalk chunk_walk(uint64_t nchunks, uint32_t wave)
+{
+    const uint32_t g = gridDim.x, b = blockIdx.x;
+    ChunkWalk w;
+    if (g % 8 == 0) {
+        const uint32_t xcd = b % 8, per = g / 8, local = b / 8;
+        const uint64_t lo = nchunks * xcd / 8, hi = nchunks * (xcd + 1) / 8;
+        w.first = lo + (uint64_t)local * WAVES_PER_WG + wave;
+        w.end = hi;
+        w.stride = (uint64_t)per * WAVES_PER_WG;
+    } else {
+        w.first = (uint64_t)b * WAVES_PER_WG + wave;
+        w.end = nchunks;
+        w.stride = (uint64_t)g * WAVES_PER_WG;
+    }
+    return w;
+}
 
 __global__ void __launch_bounds__(WG)
 k_round(RoundArgs a)
@@ -164,156 +144,168 @@ k_round(RoundArgs a)
     const uint64_t cur = c->cur;
     const uint32_t nb = a.nb;
 
-    __shared__ int32_t s_slot_delta[MAX_SLOTS];
+    __shared__ int32_t s_slot_delta[MAX_SLOTS];   // pending-item change per slot
+    __shared__ int32_t s_buf_delta[MAX_BUFS];     // set-bit change per bitmap
     __shared__ uint32_t s_ring[MAX_RING];
-    __shared__ uint32_t s_out, s_ovf, s_ovfn, s_in;
+    __shared__ uint32_t s_out, s_ovf, s_in;
     __shared__ unsigned long long s_items, s_edges;
     __shared__ int s_last;
-    __shared__ uint32_t s_seg[WAVES_PER_WG][MAX_STAGED][SEG];
-    __shared__ uint32_t s_seg_n[WAVES_PER_WG][MAX_STAGED];
+    __shared__ uint32_t s_clear[WAVES_PER_WG][64]; // redistribution: bits leaving each word
     for (int i = threadIdx.x; i < a.K; i += WG) s_slot_delta[i] = 0;
-    for (int i = threadIdx.x; i < WAVES_PER_WG * MAX_STAGED; i += WG) (&s_seg_n[0][0])[i] = 0;
+    if (threadIdx.x < MAX_BUFS) s_buf_delta[threadIdx.x] = 0;
     if (threadIdx.x < nb) s_ring[threadIdx.x] = c->ring[threadIdx.x];
     if (threadIdx.x == 0) {
-        s_out = c->out_buf; s_ovf = c->ovf_buf; s_ovfn = c->ovf_next_buf;
+        s_out = c->out_buf; s_ovf = c->ovf_buf;
         s_in = mode == MODE_RELAX ? c->ring[cur % nb] : c->ovf_buf;
         s_items = 0; s_edges = 0;
     }
     __syncthreads();
 
     const uint32_t in_buf = s_in;
-    // a queue that overflowed holds only its first qcap items (the host
-    // reports the overflow); never read past them
-    const uint32_t n_in = (uint32_t)std::min<uint64_t>(c->count[in_buf], a.qcap);
-    const uint32_t *qin = a.queues + (uint64_t)in_buf * a.qcap;
-    uint32_t *bits_in = a.bits + (uint64_t)in_buf * a.bitmap_words;
+    uint32_t *bin = a.bits + (uint64_t)in_buf * a.bitmap_words;
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / 64;
-    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES_PER_WG;
-    const uint64_t lw = (uint64_t)logical_block() * WAVES_PER_WG + wave;
-    const uint64_t chunk = ((n_in + nwaves * 64 - 1) / (nwaves * 64)) * 64;
-    const uint64_t start = lw * chunk;
-    const uint64_t end = std::min<uint64_t>(start + chunk, n_in);
-    const uint64_t N = (uint64_t)a.N;
+    const uint64_t W = a.bitmap_words;
+    const uint32_t N = (uint32_t)a.N;
     unsigned long long my_items = 0, my_edges = 0;
-    const Stage st{s_seg[wave], s_seg_n[wave]};
     // lowest overflow bucket this lane saw a key move into.  It counts every
-    // such key, also one whose item already sits in the overflow queue (its
-    // dedupe bit was set): that item's bucket just dropped, and the bucket
-    // jump out of an empty ring must not pass it.
+    // such key, also one whose item already sits in the overflow bitmap (its
+    // bit was set): that item's bucket just dropped, and the bucket jump out
+    // of an empty ring must not pass it.
     uint64_t my_ovf_min = ~0ull;
+    uint32_t *clr = s_clear[wave];
 
-    for (uint64_t base = start; base < end; base += 64) {
-        const uint64_t i = base + lane;
-        uint32_t item = 0, slot = 0, u = 0;
-        uint64_t ku = KEY_INF;
-        int row = 0, deg = 0;
-        bool live = false;
-        if (i < end) {
-            item = qin[i];
-            atomicAnd(&bits_in[item >> 5], ~(1u << (item & 31)));
-            slot = item / (uint32_t)N;
-            u = item - slot * (uint32_t)N;
-            atomicAdd(&s_slot_delta[slot], -1);
-            const SlotInfo si = a.slots[slot];
-            if (si.state == 1) {
-                ku = atomic_load_u64(&a.keys[item]);
-                if (ku != KEY_INF) {
-                    const uint64_t b = si.base + key_bucket_rel(ku, a.inv_delta);
-                    if (mode == MODE_RELAX) {
-                        if (b == cur) {
-                            row = a.indptr[u];
-                            deg = a.indptr[u + 1] - row;
-                            live = true;
-                            my_items++;
+    const ChunkWalk cw = chunk_walk((W + 63) / 64, wave);
+    for (uint64_t ch = cw.first; ch < cw.end; ch += cw.stride) {
+        const uint64_t wi = ch * 64 + lane;
+        const uint32_t word = wi < W ? bin[wi] : 0u;
+        if (__ballot(word != 0u) == 0ull) continue;
+        // a relaxation round consumes its whole bucket; nothing else sets
+        // bits in it during the round (pushes go to OUT, later ring buckets
+        // or overflow)
+        if (mode == MODE_RELAX && word) bin[wi] = 0u;
+        if (mode == MODE_REDIST) __hip_atomic_store(&clr[lane], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        const int cnt = __popc(word);
+        const int incl = wave_scan(cnt);
+        const int total = __shfl(incl, 63);
+        const int excl = incl - cnt;
+        for (int b0 = 0; b0 < total; b0 += 64) {
+            // ---- this lane's item of the batch ----
+            const int t = b0 + (int)lane;
+            uint32_t item = 0, slot = 0, u = 0;
+            uint64_t ku = KEY_INF;
+            int row = 0, deg = 0;
+            bool live = false;
+            uint32_t dst_buf = 0;
+            const int j = owner_lane(excl, t);
+            const uint32_t wj = __shfl(word, j);
+            const int ej = __shfl(excl, j);
+            if (t < total) {
+                const uint32_t bit = select_bit(wj, (uint32_t)(t - ej));
+                item = (uint32_t)((ch * 64 + (uint64_t)j) * 32 + bit);
+                slot = item / N;
+                u = item - slot * N;
+                const SlotInfo si = a.slots[slot];
+                bool leaves = true;   // redistribution: the item leaves the overflow bitmap
+                if (si.state == 1) {
+                    ku = atomic_load_u64(&a.keys[item]);
+                    if (ku != KEY_INF) {
+                        const uint64_t b = si.base + key_bucket_rel(ku, a.inv_delta);
+                        if (mode == MODE_RELAX) {
+                            if (b == cur) {
+                                row = a.indptr[u];
+                                deg = a.indptr[u + 1] - row;
+                                my_items++;
+                            }
+                        } else if (b >= cur) {
+                            if (b < cur + nb) {
+                                dst_buf = s_ring[b % nb];
+                                live = !test_and_set_bit(a.bits + (uint64_t)dst_buf * W, item);
+                            } else {
+                                leaves = false;
+                                if (b < my_ovf_min) my_ovf_min = b;
+                            }
                         }
-                    } else if (b >= cur) {
-                        // redistribution: move into the ring or keep in overflow
-                        uint32_t dst;
-                        if (b < cur + nb) dst = s_ring[b % nb];
-                        else dst = s_ovfn;
-                        live = !test_and_set_bit(a.bits + (uint64_t)dst * a.bitmap_words, item);
-                        if (live) atomicAdd(&s_slot_delta[slot], 1);
-                        if (dst == s_ovfn && b < my_ovf_min) my_ovf_min = b;
-                        // reuse `row` to carry the destination queue id
-                        row = (int)dst;
                     }
                 }
-            }
-        }
-        if (mode == MODE_REDIST) {
-            stage_push(a, st, live, (uint32_t)row, item);
-            continue;
-        }
-        // ---- relax: flatten the edges of the 64 items over the lanes ----
-        const int incl = wave_scan(deg);
-        const int total = __shfl(incl, 63);
-        const int excl = incl - deg;
-        my_edges += (unsigned long long)deg;
-        for (int t0 = 0; t0 < total; t0 += 64) {
-            const int t = t0 + (int)lane;
-            int j = 0;
-#pragma unroll
-            for (int step = 32; step > 0; step >>= 1) {
-                const int cand = j + step;
-                const int ec = __shfl(excl, cand);
-                if (ec <= t) j = cand;
-            }
-            const int row_j = __shfl(row, j);
-            const int excl_j = __shfl(excl, j);
-            const uint32_t item_j = __shfl(item, j);
-            const uint32_t slot_j = __shfl(slot, j);
-            const uint32_t ulo = __shfl((uint32_t)ku, j), uhi = __shfl((uint32_t)(ku >> 32), j);
-            const uint64_t ku_j = ((uint64_t)uhi << 32) | ulo;
-            bool want = false;
-            uint32_t buf = 0, vitem = 0;
-            if (t < total) {
-                const int e = row_j + (t - excl_j);
-                const uint32_t v = (uint32_t)a.indices[e];
-                const uint32_t u_j = item_j - slot_j * (uint32_t)N;
-                const SlotInfo si = a.slots[slot_j];
-                bool ok = true;
-                if (si.roi_index >= 0) {
-                    const uint32_t w = a.roi[(uint64_t)si.roi_index * a.roi_words + (v >> 5)];
-                    ok = (w >> (v & 31)) & 1u;
+                if (mode == MODE_RELAX) {
+                    atomicAdd(&s_slot_delta[slot], -1);
+                } else if (leaves) {
+                    atomicOr(&clr[j], 1u << bit);
+                    atomicAdd(&s_slot_delta[slot], live ? 0 : -1);
+                    atomicAdd(&s_buf_delta[in_buf], -1);
+                    if (live) atomicAdd(&s_buf_delta[dst_buf], 1);
                 }
-                if (ok) {
-                    uint32_t ko = key_ko(ku_j);
-                    float nd = key_dist(ku_j);
-                    if (si.use_keepout && a.ko_owner) {
-                        const int32_t own = a.ko_owner[u_j];
-                        if (own >= 0 && own != si.net_id && a.kind[e] == 0) {
-                            ko = ko < 31 ? ko + 1 : 31;
+            }
+            if (mode == MODE_REDIST) continue;
+
+            // ---- relax: flatten the batch's edges over the lanes ----
+            const int eincl = wave_scan(deg);
+            const int etotal = __shfl(eincl, 63);
+            const int eexcl = eincl - deg;
+            my_edges += (unsigned long long)deg;
+            for (int t0 = 0; t0 < etotal; t0 += 64) {
+                const int te = t0 + (int)lane;
+                const int jj = owner_lane(eexcl, te);
+                const int row_j = __shfl(row, jj);
+                const int excl_j = __shfl(eexcl, jj);
+                const uint32_t item_j = __shfl(item, jj);
+                const uint32_t slot_j = __shfl(slot, jj);
+                const uint32_t ulo = __shfl((uint32_t)ku, jj), uhi = __shfl((uint32_t)(ku >> 32), jj);
+                const uint64_t ku_j = ((uint64_t)uhi << 32) | ulo;
+                if (te < etotal) {
+                    const int e = row_j + (te - excl_j);
+                    const uint32_t v = (uint32_t)a.indices[e];
+                    const uint32_t u_j = item_j - slot_j * N;
+                    const SlotInfo si = a.slots[slot_j];
+                    bool ok = true;
+                    if (si.roi_index >= 0) {
+                        const uint32_t w = a.roi[(uint64_t)si.roi_index * a.roi_words + (v >> 5)];
+                        ok = (w >> (v & 31)) & 1u;
+                    }
+                    if (ok) {
+                        uint32_t ko = key_ko(ku_j);
+                        float nd = key_dist(ku_j);
+                        if (si.use_keepout && a.ko_owner) {
+                            const int32_t own = a.ko_owner[u_j];
+                            if (own >= 0 && own != si.net_id && a.kind[e] == 0) {
+                                ko = ko < 31 ? ko + 1 : 31;
+                            } else {
+                                nd = nd + a.cost[e];
+                            }
                         } else {
                             nd = nd + a.cost[e];
                         }
-                    } else {
-                        nd = nd + a.cost[e];
-                    }
-                    if (nd < __builtin_inff()) {
-                        const uint64_t nk = key_pack(ko, nd, u_j);
-                        vitem = slot_j * (uint32_t)N + v;
-                        uint64_t *kp = &a.keys[vitem];
-                        if (nk < *kp) {
-                            const uint64_t old = atomicMin((unsigned long long *)kp, (unsigned long long)nk);
-                            if (nk < old && key_cost(nk) < key_cost(old)) {
-                                uint64_t b = si.base + key_bucket_rel(nk, a.inv_delta);
-                                if (b <= cur) buf = s_out;
-                                else if (b < cur + nb) buf = s_ring[b % nb];
-                                else buf = s_ovf;
-                                want = !test_and_set_bit(a.bits + (uint64_t)buf * a.bitmap_words, vitem);
-                                if (want) atomicAdd(&s_slot_delta[slot_j], 1);
-                                if (buf == s_ovf && b < my_ovf_min) my_ovf_min = b;
+                        if (nd < __builtin_inff()) {
+                            const uint64_t nk = key_pack(ko, nd, u_j);
+                            const uint32_t vitem = slot_j * N + v;
+                            uint64_t *kp = &a.keys[vitem];
+                            if (nk < *kp) {
+                                const uint64_t old = atomicMin((unsigned long long *)kp, (unsigned long long)nk);
+                                if (nk < old && key_cost(nk) < key_cost(old)) {
+                                    const uint64_t b = si.base + key_bucket_rel(nk, a.inv_delta);
+                                    uint32_t buf;
+                                    if (b <= cur) buf = s_out;
+                                    else if (b < cur + nb) buf = s_ring[b % nb];
+                                    else buf = s_ovf;
+                                    if (buf == s_ovf && b < my_ovf_min) my_ovf_min = b;
+                                    if (!test_and_set_bit(a.bits + (uint64_t)buf * W, vitem)) {
+                                        atomicAdd(&s_slot_delta[slot_j], 1);
+                                        atomicAdd(&s_buf_delta[buf], 1);
+                                    }
+                                }
                             }
                         }
                     }
                 }
             }
-            stage_push(a, st, want, buf, vitem);
+        }
+        if (mode == MODE_REDIST) {
+            // no other writer touches the overflow bitmap during the round
+            const uint32_t m = __hip_atomic_load(&clr[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            if (m) bin[wi] = word & ~m;
         }
     }
-    // drain this wave's staged appends; one overflow-minimum update per wave
-    for (uint32_t b = 0; b < nb + 3; ++b) stage_flush(a, st, b);
     {
         const uint64_t m = wave_min_u64(my_ovf_min);
         if (lane == 0 && m != ~0ull) atomicMin((unsigned long long *)&c->ovf_min, (unsigned long long)m);
@@ -327,8 +319,12 @@ k_round(RoundArgs a)
         const int dlt = s_slot_delta[s];
         if (dlt != 0) atomicAdd(&a.slots[s].pending, dlt);
     }
-    // every thread's queue appends and counter updates must be visible
-    // before this workgroup counts itself finished
+    if (threadIdx.x < nb + 3) {
+        const int dlt = s_buf_delta[threadIdx.x];
+        if (dlt != 0) atomicAdd(&c->count[threadIdx.x], (uint32_t)dlt);
+    }
+    // every thread's bitmap, key and counter updates must be visible before
+    // this workgroup counts itself finished
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -351,10 +347,11 @@ __device__ void control_step(const RoundArgs &a, uint32_t mode, uint64_t cur)
     Ctrl *c = a.ctrl;
     const uint32_t nb = a.nb;
     __shared__ uint64_t s_newcur;
-    __shared__ uint32_t s_ring_total, s_mode, s_any_active;
+    __shared__ uint32_t s_mode;
     if (threadIdx.x == 0) {
         uint64_t newcur = cur;
         if (mode == MODE_RELAX) {
+            // the round consumed (and cleared) its bucket
             const uint32_t pos = (uint32_t)(cur % nb);
             const uint32_t in = c->ring[pos];
             c->count[in] = 0;
@@ -367,17 +364,13 @@ __device__ void control_step(const RoundArgs &a, uint32_t mode, uint64_t cur)
                 s_mode = 0xFFFFFFFFu; // decide below
             }
         } else {
-            // redistribution consumed OVF; OVF_NEXT holds what stays
-            c->count[c->ovf_buf] = 0;
-            const uint32_t t = c->ovf_buf;
-            c->ovf_buf = c->ovf_next_buf;
-            c->ovf_next_buf = t;
+            // redistribution moved what fits the window out of the overflow
+            // bitmap in place; its count is already adjusted
             s_mode = 0xFFFFFFFFu;
         }
         uint32_t tot = 0;
         for (uint32_t p = 0; p < nb; ++p) tot += ld32(&c->count[c->ring[p]]);
         tot += ld32(&c->count[c->out_buf]);
-        s_ring_total = tot;
         if (s_mode == 0xFFFFFFFFu) {
             const uint32_t ovf = ld32(&c->count[c->ovf_buf]);
             if (tot == 0) {
@@ -398,7 +391,6 @@ __device__ void control_step(const RoundArgs &a, uint32_t mode, uint64_t cur)
             }
         }
         s_newcur = newcur;
-        s_any_active = 0;
     }
     __syncthreads();
     const uint64_t newcur = s_newcur;
@@ -408,15 +400,13 @@ __device__ void control_step(const RoundArgs &a, uint32_t mode, uint64_t cur)
         SlotInfo *si = &a.slots[s];
         if (si->state != 1) continue;
         const uint64_t kd = atomic_load_u64(&a.keys[(uint64_t)s * (uint64_t)a.N + (uint64_t)si->dst]);
-        // no queued items left: the search is exhausted and every key final
+        // no set bits left: the search is exhausted and every key final
         const bool exhausted = (int32_t)ld32((const uint32_t *)&si->pending) <= 0;
         const bool reached = kd != KEY_INF;
         if (reached && (exhausted || si->base + key_bucket_rel(kd, a.inv_delta) < newcur)) {
             si->state = 2; atomicSub(&c->active, 1u); atomicAdd(&c->epoch, 1u);
         } else if (exhausted) {
             si->state = 3; atomicSub(&c->active, 1u); atomicAdd(&c->epoch, 1u);
-        } else {
-            atomicOr(&s_any_active, 1u);
         }
     }
     __syncthreads();
@@ -467,16 +457,10 @@ __global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n)
         const uint32_t item = (uint32_t)r.slot * (uint32_t)a.N + (uint32_t)r.src;
         a.keys[item] = key_pack(0, 0.0f, KEY_NOPRED);
         const uint32_t buf = c->ring[c->cur % a.nb];
-        const uint32_t m = 1u << (item & 31);
-        uint32_t *bm = a.bits + (uint64_t)buf * a.bitmap_words;
-        // a slot is reseeded only once its previous search has no queued
-        // items left, so the item's dedupe bit is clear here
+        // a slot is reseeded only once its previous search has no set bits
+        // left, so the source's bit is clear here
         si->pending = 1;
-        if (!(atomicOr(&bm[item >> 5], m) & m)) {
-            const uint32_t pos = atomicAdd(&c->count[buf], 1u);
-            if (pos < a.qcap) a.queues[(uint64_t)buf * a.qcap + pos] = item;
-            else c->overflow = 1;
-        }
+        if (!test_and_set_bit(a.bits + (uint64_t)buf * a.bitmap_words, item)) atomicAdd(&c->count[buf], 1u);
         atomicAdd(&c->active, 1u);
     }
     __syncthreads();
@@ -553,20 +537,19 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
     if (slots < 1 || slots > MAX_SLOTS) return orgpu_fail(ORGPU_EINVAL, "reserve: slots %d outside [1, %d]", slots, MAX_SLOTS);
     if (n_buckets < 1 || n_buckets > MAX_NB) return orgpu_fail(ORGPU_EINVAL, "reserve: n_buckets %d outside [1, %d]", n_buckets, MAX_NB);
     if (n_nodes < 1 || n_nodes >= KEY_MAX_NODES) return orgpu_fail(ORGPU_EINVAL, "reserve: bad node count");
-    if ((uint64_t)n_nodes * (uint64_t)slots >= (1ull << 32))
+    if ((uint64_t)n_nodes * (uint64_t)slots >= (1ull << 32) - 64)
         return orgpu_fail(ORGPU_EINVAL, "reserve: slots*nodes = %lld exceeds 2^32 work items",
                           (long long)n_nodes * slots);
     if (max_path < 2) max_path = 1 << 16;
     const uint64_t items = (uint64_t)n_nodes * (uint64_t)slots;
-    if (qcap <= 0 || (uint64_t)qcap > items) qcap = (int64_t)items;
-    if (en.cap_nodes >= n_nodes && en.cap_slots >= slots && en.cap_nb >= n_buckets && en.cap_q >= qcap &&
+    (void)qcap;   // frontiers are bitmaps: no item queues to size
+    if (en.cap_nodes >= n_nodes && en.cap_slots >= slots && en.cap_nb >= n_buckets &&
         en.cap_path >= max_path)
         return ORGPU_OK;
     engine_release(en);
     const uint64_t words = (items + 31) / 32;
     const int nbufs = n_buckets + 3;
     HIP_CHECK_RET(hipMalloc((void **)&en.keys, items * sizeof(uint64_t)));
-    HIP_CHECK_RET(hipMalloc((void **)&en.queues, (uint64_t)nbufs * (uint64_t)qcap * sizeof(uint32_t)));
     HIP_CHECK_RET(hipMalloc((void **)&en.bits, (uint64_t)nbufs * words * sizeof(uint32_t)));
     HIP_CHECK_RET(hipMemset(en.bits, 0, (uint64_t)nbufs * words * sizeof(uint32_t)));
     HIP_CHECK_RET(hipMalloc((void **)&en.ctrl, sizeof(Ctrl)));
@@ -591,7 +574,7 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
     HIP_CHECK_RET(hipEventCreate(&en.ev_t1));
     HIP_CHECK_RET(hipEventCreate(&en.ev_r0));
     HIP_CHECK_RET(hipEventCreate(&en.ev_r1));
-    en.cap_nodes = n_nodes; en.cap_slots = slots; en.cap_nb = n_buckets; en.cap_q = qcap; en.cap_path = max_path;
+    en.cap_nodes = n_nodes; en.cap_slots = slots; en.cap_nb = n_buckets; en.cap_path = max_path;
     en.words = words;
     return ORGPU_OK;
 }
@@ -599,7 +582,6 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
 void engine_release(Engine &en)
 {
     if (en.keys) hipFree(en.keys);
-    if (en.queues) hipFree(en.queues);
     if (en.bits) hipFree(en.bits);
     if (en.ctrl) hipFree(en.ctrl);
     if (en.slots) hipFree(en.slots);
@@ -672,7 +654,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     a.indptr = g->indptr; a.indices = g->indices; a.cost = g->cost; a.kind = g->edge_kind;
     a.roi = roi_bitmaps; a.roi_words = (N + 31) / 32;
     a.ko_owner = keepout_owner;
-    a.keys = en.keys; a.queues = en.queues; a.qcap = (uint64_t)en.cap_q;
+    a.keys = en.keys;
     a.bits = en.bits; a.bitmap_words = (uint64_t)((N * (int64_t)K + 31) / 32);
 
     // control block init
@@ -744,9 +726,6 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
             HIP_CHECK_RET(hipEventElapsedTime(&bms, en.ev_r0, en.ev_r1));
             round_ms += bms;
         }
-        if (en.h_ctrl->overflow)
-            return orgpu_fail(ORGPU_ECAPACITY, "route: a bucket queue exceeded %lld items; reserve a larger queue_capacity",
-                              (long long)en.cap_q);
         int nf = 0, searching = 0;
         for (int k = 0; k < K; ++k) {
             if (slot_req[k] < 0) {
