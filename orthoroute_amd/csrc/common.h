@@ -67,12 +67,19 @@ struct Ctrl {
     uint32_t finished;         // last-workgroup counter
     uint32_t ring[MAX_RING];   // ring position -> physical buffer
     uint32_t out_buf, ovf_buf, ovf_next_buf;
-    uint32_t unused0;
+    uint32_t spare_list;       // chunk-list storage not bound to a bitmap
     uint32_t count[MAX_BUFS];  // items per physical buffer
     uint32_t active;           // slots seeded and not finished
     uint32_t epoch;            // bumps when any slot finishes
     uint64_t rounds, items, edges;
+    uint32_t list_id[MAX_BUFS];     // bitmap -> chunk-list storage
 };
+
+// A bucket's chunk list is kept in LIST_BINS bins by item region (bin of
+// chunk ch = ch * LIST_BINS / chunks); the bins in order form the list a
+// round walks, so neighbouring chunks are visited by the same XCD without
+// sorting anything.
+#define LIST_BINS 64
 
 struct SlotInfo {
     uint64_t base;      // bucket of distance 0 for this slot's search
@@ -102,6 +109,21 @@ struct RoundArgs {
     uint64_t *keys;           // [K][N]
     uint32_t *bits;           // [MAX_BUFS][bitmap_words]
     uint64_t bitmap_words;
+    uint32_t *summ;           // [MAX_BUFS][summ_words]: one bit per 64-word chunk
+    uint64_t summ_words;
+    uint32_t *lists;          // [MAX_BUFS + 1][LIST_BINS][bin_cap] chunk lists
+    uint32_t *list_cnt;       // [MAX_BUFS + 1][LIST_BINS] entries per bin
+    uint64_t chunks;          // chunks per bitmap (64 words each)
+    uint64_t bin_cap;         // chunks per bin
+    struct RoundPart *parts;  // [grid] per-workgroup partials of a round
+};
+
+// What one workgroup of a round leaves for the control kernel.
+struct RoundPart {
+    int32_t buf_delta[MAX_BUFS];   // set-bit change per bucket bitmap
+    uint32_t pad;
+    unsigned long long items, edges;
+    unsigned long long ovf_min;    // lowest overflow bucket it saw
 };
 
 #define HIP_CHECK_RET(expr)                                                   \

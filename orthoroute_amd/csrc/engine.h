@@ -15,6 +15,10 @@ struct Engine {
     // device scratch
     uint64_t *keys = nullptr;
     uint32_t *bits = nullptr;
+    uint32_t *summ = nullptr;
+    uint32_t *lists = nullptr, *list_cnt = nullptr;
+    RoundPart *parts = nullptr;
+    uint64_t summ_words = 0, chunks = 0;
     Ctrl *ctrl = nullptr;
     SlotInfo *slots = nullptr;
     SeedReq *seed = nullptr;

@@ -107,39 +107,159 @@ __device__ __forceinline__ uint32_t select_bit(uint32_t w, uint32_t r)
     return pos;
 }
 
-__device__ void control_step(const RoundArgs &a, uint32_t mode, uint64_t cur);
+// Appends chunk ch to chunk-list storage sid (its region's bin).
+__device__ __forceinline__ void list_append(const RoundArgs &a, uint32_t sid, uint32_t ch)
+{
+    const uint32_t bin = (uint32_t)((uint64_t)ch * LIST_BINS / a.chunks);
+    const uint64_t cell = (uint64_t)sid * LIST_BINS + bin;
+    const uint32_t pos = atomicAdd(&a.list_cnt[cell], 1u);
+    a.lists[cell * a.bin_cap + pos] = ch;
+}
+
+// Marks bit `item` of bucket bitmap `buf`; returns true when it was clear.
+// The first bit of a word also marks the word's chunk (64 words) in the
+// bucket's summary bitmap, and the chunk whose summary bit this turns on is
+// appended to the bucket's chunk list: the chunks a round must visit, each
+// listed once.  list_sid: the bucket's chunk-list storage.
+__device__ __forceinline__ bool push_bit(const RoundArgs &a, uint32_t buf, uint32_t list_sid, uint32_t item)
+{
+    const uint32_t m = 1u << (item & 31);
+    const uint32_t old = atomicOr(&a.bits[(uint64_t)buf * a.bitmap_words + (item >> 5)], m);
+    if (old & m) return false;
+    if (old == 0u) {
+        const uint32_t ch = item >> 11;
+        const uint32_t cm = 1u << (ch & 31);
+        if (!(atomicOr(&a.summ[(uint64_t)buf * a.summ_words + (ch >> 5)], cm) & cm)) list_append(a, list_sid, ch);
+    }
+    return true;
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs.  Each XCD takes one
-// contiguous eighth of the bitmap's chunks (so a slot's keys and the lattice
-// rows its frontier touches stay in that XCD's L2), and inside it the waves
-// stride over the chunks (so a dense region is spread over many waves).
+// contiguous eighth of the round's chunk list (chunks are in item order, so
+// a slot's keys and the lattice rows its frontier touches stay in that XCD's
+// L2), and inside it the waves stride over the chunks.
 struct ChunkWalk {
-    uint64_t first, end, stride;
+    uint32_t first, end, stride;
 };
 
-__device__ __forceinline__ ChunkWalk chunk_walk(uint64_t nchunks, uint32_t wave)
+__device__ __forceinline__ ChunkWalk chunk_walk(uint32_t n, uint32_t wave)
 {
     const uint32_t g = gridDim.x, b = blockIdx.x;
     ChunkWalk w;
     if (g % 8 == 0) {
         const uint32_t xcd = b % 8, per = g / 8, local = b / 8;
-        const uint64_t lo = nchunks * xcd / 8, hi = nchunks * (xcd + 1) / 8;
-        w.first = lo + (uint64_t)local * WAVES_PER_WG + wave;
-        w.end = hi;
-        w.stride = (uint64_t)per * WAVES_PER_WG;
+        w.first = (uint32_t)((uint64_t)n * xcd / 8) + local * WAVES_PER_WG + wave;
+        w.end = (uint32_t)((uint64_t)n * (xcd + 1) / 8);
+        w.stride = per * WAVES_PER_WG;
     } else {
-        w.first = (uint64_t)b * WAVES_PER_WG + wave;
-        w.end = nchunks;
-        w.stride = (uint64_t)g * WAVES_PER_WG;
+        w.first = b * WAVES_PER_WG + wave;
+        w.end = n;
+        w.stride = g * WAVES_PER_WG;
     }
     return w;
 }
 
+// Expands one item per lane (lanes with has=false only take part in the
+// wave's collective steps): consumes it, and if its key still lies in the
+// current bucket relaxes its out-edges.  A lane walks its own CSR row four
+// edges at a time with the four edges' loads issued together; the slot's
+// search parameters are read once per item.
+__device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bool has, uint64_t cur, uint32_t nb,
+                                           const uint32_t *s_ring, const uint32_t *s_lid, uint32_t s_out, uint32_t s_ovf,
+                                           int32_t *s_slot_delta, int32_t *s_buf_delta,
+                                           unsigned long long &my_items, unsigned long long &my_edges,
+                                           uint64_t &my_ovf_min)
+{
+    if (!has) return;
+    const uint32_t N = (uint32_t)a.N;
+    const uint32_t slot = item / N;
+    const uint32_t u = item - slot * N;
+    atomicAdd(&s_slot_delta[slot], -1);
+    const SlotInfo si = a.slots[slot];
+    if (si.state != 1) return;
+    const uint64_t ku = atomic_load_u64(&a.keys[item]);
+    if (ku == KEY_INF || si.base + key_bucket_rel(ku, a.inv_delta) != cur) return;
+    const int row = a.indptr[u];
+    const int deg = a.indptr[u + 1] - row;
+    my_items++;
+    my_edges += (unsigned long long)deg;
+    const uint32_t ko0 = key_ko(ku);
+    const float d0 = key_dist(ku);
+    // keepout: edges of other nets' via barrels (planar edges leaving an
+    // owned node) count as keepout edges instead of adding their cost
+    const bool ko_node = si.use_keepout && a.ko_owner && a.ko_owner[u] >= 0 && a.ko_owner[u] != si.net_id;
+    const uint32_t *roi = si.roi_index >= 0 ? a.roi + (uint64_t)si.roi_index * a.roi_words : nullptr;
+    uint64_t *keys = a.keys + (uint64_t)slot * N;
+    for (int k0 = 0; k0 < deg; k0 += 4) {
+        uint32_t v[4];
+        float c[4];
+        bool ok[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            ok[q] = k0 + q < deg;
+            if (ok[q]) {
+                v[q] = (uint32_t)a.indices[row + k0 + q];
+                c[q] = a.cost[row + k0 + q];
+            }
+        }
+        if (roi) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (ok[q]) ok[q] = (roi[v[q] >> 5] >> (v[q] & 31)) & 1u;
+        }
+        uint64_t nk[4], kv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (!ok[q]) continue;
+            uint32_t ko = ko0;
+            float nd = d0;
+            if (ko_node && a.kind[row + k0 + q] == 0) ko = ko < 31 ? ko + 1 : 31;
+            else nd = nd + c[q];
+            ok[q] = nd < __builtin_inff();
+            nk[q] = key_pack(ko, nd, u);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (ok[q]) kv[q] = keys[v[q]];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (!ok[q] || !(nk[q] < kv[q])) continue;
+            const uint64_t old = atomicMin((unsigned long long *)&keys[v[q]], (unsigned long long)nk[q]);
+            if (nk[q] < old && key_cost(nk[q]) < key_cost(old)) {
+                const uint64_t b = si.base + key_bucket_rel(nk[q], a.inv_delta);
+                uint32_t buf;
+                if (b <= cur) buf = s_out;
+                else if (b < cur + nb) buf = s_ring[b % nb];
+                else buf = s_ovf;
+                if (buf == s_ovf && b < my_ovf_min) my_ovf_min = b;
+                if (push_bit(a, buf, s_lid[buf], slot * N + v[q])) {
+                    atomicAdd(&s_slot_delta[slot], 1);
+                    atomicAdd(&s_buf_delta[buf], 1);
+                }
+            }
+        }
+    }
+}
+
+// Entry li of a binned chunk list (binoff: exclusive prefix of the bins).
+__device__ __forceinline__ uint32_t list_at(const uint32_t *lst, const uint32_t *binoff, uint64_t bin_cap, uint32_t li)
+{
+    uint32_t b = 0;
+#pragma unroll
+    for (uint32_t step = LIST_BINS / 2; step > 0; step >>= 1)
+        if (binoff[b + step] <= li) b += step;
+    return lst[(uint64_t)b * bin_cap + (li - binoff[b])];
+}
+
+// One round: expand (RELAX) or redistribute (REDIST) every item of the
+// round's input bucket, visiting the chunks the control kernel listed.  Each
+// workgroup leaves its bucket-count changes and counters in its own partials
+// entry (plain stores); the control kernel that follows reduces them.
 __global__ void __launch_bounds__(WG)
 k_round(RoundArgs a)
 {
     Ctrl *c = a.ctrl;
-    const uint32_t mode = __hip_atomic_load(&c->mode, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t mode = c->mode;
     if (mode == MODE_IDLE) return;
     const uint64_t cur = c->cur;
     const uint32_t nb = a.nb;
@@ -147,19 +267,34 @@ k_round(RoundArgs a)
     __shared__ int32_t s_slot_delta[MAX_SLOTS];   // pending-item change per slot
     __shared__ int32_t s_buf_delta[MAX_BUFS];     // set-bit change per bitmap
     __shared__ uint32_t s_ring[MAX_RING];
-    __shared__ uint32_t s_out, s_ovf, s_in;
-    __shared__ unsigned long long s_items, s_edges;
-    __shared__ int s_last;
+    __shared__ uint32_t s_lid[MAX_BUFS];          // bitmap -> chunk-list storage
+    __shared__ uint32_t s_out, s_ovf, s_in, s_nlist, s_spare;
+    __shared__ uint32_t s_binoff[LIST_BINS + 1];  // exclusive prefix of the input list's bins
+    __shared__ unsigned long long s_items, s_edges, s_ovf_min;
     __shared__ uint32_t s_clear[WAVES_PER_WG][64]; // redistribution: bits leaving each word
+    __shared__ uint32_t s_ibuf[WAVES_PER_WG][128]; // relaxation: items gathered for the next batch
     for (int i = threadIdx.x; i < a.K; i += WG) s_slot_delta[i] = 0;
     if (threadIdx.x < MAX_BUFS) s_buf_delta[threadIdx.x] = 0;
     if (threadIdx.x < nb) s_ring[threadIdx.x] = c->ring[threadIdx.x];
+    if (threadIdx.x < nb + 3) s_lid[threadIdx.x] = c->list_id[threadIdx.x];
     if (threadIdx.x == 0) {
         s_out = c->out_buf; s_ovf = c->ovf_buf;
         s_in = mode == MODE_RELAX ? c->ring[cur % nb] : c->ovf_buf;
-        s_items = 0; s_edges = 0;
+        s_spare = c->spare_list;
+        s_items = 0; s_edges = 0; s_ovf_min = ~0ull;
     }
     __syncthreads();
+    // chunks of the input bucket (no chunk joins this list during the round):
+    // bin sizes -> exclusive prefix in LDS
+    const uint32_t in_sid = s_lid[s_in];
+    if (threadIdx.x < 64) {
+        const int cnt = threadIdx.x < LIST_BINS ? (int)a.list_cnt[(uint64_t)in_sid * LIST_BINS + threadIdx.x] : 0;
+        const int inc = wave_scan(cnt);
+        s_binoff[threadIdx.x] = (uint32_t)(inc - cnt);
+        if (threadIdx.x == 63) { s_binoff[LIST_BINS] = (uint32_t)inc; s_nlist = (uint32_t)inc; }
+    }
+    __syncthreads();
+    const uint32_t *clist = a.lists + (uint64_t)in_sid * LIST_BINS * a.bin_cap;
 
     const uint32_t in_buf = s_in;
     uint32_t *bin = a.bits + (uint64_t)in_buf * a.bitmap_words;
@@ -175,250 +310,290 @@ k_round(RoundArgs a)
     uint64_t my_ovf_min = ~0ull;
     uint32_t *clr = s_clear[wave];
 
-    const ChunkWalk cw = chunk_walk((W + 63) / 64, wave);
-    for (uint64_t ch = cw.first; ch < cw.end; ch += cw.stride) {
-        const uint64_t wi = ch * 64 + lane;
-        const uint32_t word = wi < W ? bin[wi] : 0u;
-        if (__ballot(word != 0u) == 0ull) continue;
-        // a relaxation round consumes its whole bucket; nothing else sets
-        // bits in it during the round (pushes go to OUT, later ring buckets
-        // or overflow)
-        if (mode == MODE_RELAX && word) bin[wi] = 0u;
-        if (mode == MODE_REDIST) __hip_atomic_store(&clr[lane], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-        const int cnt = __popc(word);
-        const int incl = wave_scan(cnt);
-        const int total = __shfl(incl, 63);
-        const int excl = incl - cnt;
-        for (int b0 = 0; b0 < total; b0 += 64) {
-            // ---- this lane's item of the batch ----
-            const int t = b0 + (int)lane;
-            uint32_t item = 0, slot = 0, u = 0;
-            uint64_t ku = KEY_INF;
-            int row = 0, deg = 0;
-            bool live = false;
-            uint32_t dst_buf = 0;
-            const int j = owner_lane(excl, t);
-            const uint32_t wj = __shfl(word, j);
-            const int ej = __shfl(excl, j);
-            if (t < total) {
-                const uint32_t bit = select_bit(wj, (uint32_t)(t - ej));
-                item = (uint32_t)((ch * 64 + (uint64_t)j) * 32 + bit);
-                slot = item / N;
-                u = item - slot * N;
-                const SlotInfo si = a.slots[slot];
-                bool leaves = true;   // redistribution: the item leaves the overflow bitmap
-                if (si.state == 1) {
-                    ku = atomic_load_u64(&a.keys[item]);
-                    if (ku != KEY_INF) {
-                        const uint64_t b = si.base + key_bucket_rel(ku, a.inv_delta);
-                        if (mode == MODE_RELAX) {
-                            if (b == cur) {
-                                row = a.indptr[u];
-                                deg = a.indptr[u + 1] - row;
-                                my_items++;
-                            }
-                        } else if (b >= cur) {
-                            if (b < cur + nb) {
-                                dst_buf = s_ring[b % nb];
-                                live = !test_and_set_bit(a.bits + (uint64_t)dst_buf * W, item);
-                            } else {
-                                leaves = false;
-                                if (b < my_ovf_min) my_ovf_min = b;
-                            }
-                        }
+    const ChunkWalk cw = chunk_walk(s_nlist, wave);
+    if (mode == MODE_RELAX) {
+        // Items of consecutive chunks are gathered into full 64-item batches
+        // (a chunk of a sparse bucket holds a few dozen), so each batch's
+        // dependent loads and atomics serve 64 items at once.
+        uint32_t *ibuf = s_ibuf[wave];
+        int nbuf = 0;
+        for (uint32_t li = cw.first;; li += cw.stride) {
+            const bool more = li < cw.end;
+            if (more) {
+                const uint64_t ch = list_at(clist, s_binoff, a.bin_cap, li);
+                const uint64_t wi = ch * 64 + lane;
+                const uint32_t word = wi < W ? bin[wi] : 0u;
+                if (__ballot(word != 0u) == 0ull) continue;
+                // a relaxation round consumes its whole bucket; nothing else
+                // sets bits in it during the round (pushes go to OUT, later
+                // ring buckets or overflow)
+                if (word) bin[wi] = 0u;
+                const int cnt = __popc(word);
+                const int incl = wave_scan(cnt);
+                const int total = __shfl(incl, 63);
+                const int excl = incl - cnt;
+                for (int b0 = 0; b0 < total; b0 += 64) {
+                    const int t = b0 + (int)lane;
+                    const int j = owner_lane(excl, t);
+                    const uint32_t wj = __shfl(word, j);
+                    const int ej = __shfl(excl, j);
+                    const int k = min(64, total - b0);
+                    if (t < total) {
+                        const uint32_t bit = select_bit(wj, (uint32_t)(t - ej));
+                        __hip_atomic_store(&ibuf[nbuf + (int)lane], (uint32_t)((ch * 64 + (uint64_t)j) * 32 + bit),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    }
+                    nbuf += k;
+                    if (nbuf >= 64) {
+                        const uint32_t it = __hip_atomic_load(&ibuf[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        const uint32_t rest = __hip_atomic_load(&ibuf[64 + lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        __hip_atomic_store(&ibuf[lane], rest, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        nbuf -= 64;
+                        relax_item(a, it, true, cur, nb, s_ring, s_lid, s_out, s_ovf, s_slot_delta, s_buf_delta,
+                                   my_items, my_edges, my_ovf_min);
                     }
                 }
-                if (mode == MODE_RELAX) {
-                    atomicAdd(&s_slot_delta[slot], -1);
-                } else if (leaves) {
-                    atomicOr(&clr[j], 1u << bit);
-                    atomicAdd(&s_slot_delta[slot], live ? 0 : -1);
-                    atomicAdd(&s_buf_delta[in_buf], -1);
-                    if (live) atomicAdd(&s_buf_delta[dst_buf], 1);
+            } else {
+                if (nbuf > 0) {
+                    const uint32_t it = __hip_atomic_load(&ibuf[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    relax_item(a, it, (int)lane < nbuf, cur, nb, s_ring, s_lid, s_out, s_ovf, s_slot_delta, s_buf_delta,
+                               my_items, my_edges, my_ovf_min);
                 }
+                break;
             }
-            if (mode == MODE_REDIST) continue;
-
-            // ---- relax: flatten the batch's edges over the lanes ----
-            const int eincl = wave_scan(deg);
-            const int etotal = __shfl(eincl, 63);
-            const int eexcl = eincl - deg;
-            my_edges += (unsigned long long)deg;
-            for (int t0 = 0; t0 < etotal; t0 += 64) {
-                const int te = t0 + (int)lane;
-                const int jj = owner_lane(eexcl, te);
-                const int row_j = __shfl(row, jj);
-                const int excl_j = __shfl(eexcl, jj);
-                const uint32_t item_j = __shfl(item, jj);
-                const uint32_t slot_j = __shfl(slot, jj);
-                const uint32_t ulo = __shfl((uint32_t)ku, jj), uhi = __shfl((uint32_t)(ku >> 32), jj);
-                const uint64_t ku_j = ((uint64_t)uhi << 32) | ulo;
-                if (te < etotal) {
-                    const int e = row_j + (te - excl_j);
-                    const uint32_t v = (uint32_t)a.indices[e];
-                    const uint32_t u_j = item_j - slot_j * N;
-                    const SlotInfo si = a.slots[slot_j];
-                    bool ok = true;
-                    if (si.roi_index >= 0) {
-                        const uint32_t w = a.roi[(uint64_t)si.roi_index * a.roi_words + (v >> 5)];
-                        ok = (w >> (v & 31)) & 1u;
-                    }
-                    if (ok) {
-                        uint32_t ko = key_ko(ku_j);
-                        float nd = key_dist(ku_j);
-                        if (si.use_keepout && a.ko_owner) {
-                            const int32_t own = a.ko_owner[u_j];
-                            if (own >= 0 && own != si.net_id && a.kind[e] == 0) {
-                                ko = ko < 31 ? ko + 1 : 31;
-                            } else {
-                                nd = nd + a.cost[e];
-                            }
-                        } else {
-                            nd = nd + a.cost[e];
-                        }
-                        if (nd < __builtin_inff()) {
-                            const uint64_t nk = key_pack(ko, nd, u_j);
-                            const uint32_t vitem = slot_j * N + v;
-                            uint64_t *kp = &a.keys[vitem];
-                            if (nk < *kp) {
-                                const uint64_t old = atomicMin((unsigned long long *)kp, (unsigned long long)nk);
-                                if (nk < old && key_cost(nk) < key_cost(old)) {
-                                    const uint64_t b = si.base + key_bucket_rel(nk, a.inv_delta);
-                                    uint32_t buf;
-                                    if (b <= cur) buf = s_out;
-                                    else if (b < cur + nb) buf = s_ring[b % nb];
-                                    else buf = s_ovf;
-                                    if (buf == s_ovf && b < my_ovf_min) my_ovf_min = b;
-                                    if (!test_and_set_bit(a.bits + (uint64_t)buf * W, vitem)) {
-                                        atomicAdd(&s_slot_delta[slot_j], 1);
-                                        atomicAdd(&s_buf_delta[buf], 1);
-                                    }
+        }
+    } else {
+        for (uint32_t li = cw.first; li < cw.end; li += cw.stride) {
+            const uint64_t ch = list_at(clist, s_binoff, a.bin_cap, li);
+            const uint64_t wi = ch * 64 + lane;
+            const uint32_t word = wi < W ? bin[wi] : 0u;
+            if (__ballot(word != 0u) == 0ull) continue;
+            __hip_atomic_store(&clr[lane], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            const int cnt = __popc(word);
+            const int incl = wave_scan(cnt);
+            const int total = __shfl(incl, 63);
+            const int excl = incl - cnt;
+            for (int b0 = 0; b0 < total; b0 += 64) {
+                const int t = b0 + (int)lane;
+                const int j = owner_lane(excl, t);
+                const uint32_t wj = __shfl(word, j);
+                const int ej = __shfl(excl, j);
+                if (t < total) {
+                    const uint32_t bit = select_bit(wj, (uint32_t)(t - ej));
+                    const uint32_t item = (uint32_t)((ch * 64 + (uint64_t)j) * 32 + bit);
+                    const uint32_t slot = item / N;
+                    const SlotInfo si = a.slots[slot];
+                    bool leaves = true;   // the item leaves the overflow bitmap
+                    bool live = false;
+                    uint32_t dst_buf = 0;
+                    if (si.state == 1) {
+                        const uint64_t ku = atomic_load_u64(&a.keys[item]);
+                        if (ku != KEY_INF) {
+                            const uint64_t b = si.base + key_bucket_rel(ku, a.inv_delta);
+                            if (b >= cur) {
+                                if (b < cur + nb) {
+                                    dst_buf = s_ring[b % nb];
+                                    live = push_bit(a, dst_buf, s_lid[dst_buf], item);
+                                } else {
+                                    leaves = false;
+                                    if (b < my_ovf_min) my_ovf_min = b;
                                 }
                             }
                         }
                     }
+                    if (leaves) {
+                        atomicOr(&clr[j], 1u << bit);
+                        if (!live) atomicAdd(&s_slot_delta[slot], -1);
+                        atomicAdd(&s_buf_delta[in_buf], -1);
+                        if (live) atomicAdd(&s_buf_delta[dst_buf], 1);
+                    }
                 }
             }
-        }
-        if (mode == MODE_REDIST) {
             // no other writer touches the overflow bitmap during the round
             const uint32_t m = __hip_atomic_load(&clr[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-            if (m) bin[wi] = word & ~m;
+            const uint32_t left = word & ~m;
+            if (m) bin[wi] = left;
+            // a chunk that keeps items moves to the spare list (the overflow
+            // bucket's list after the round); an emptied one leaves the summary
+            const bool any_left = __ballot(left != 0u) != 0ull;
+            if (lane == 0) {
+                if (!any_left) {
+                    atomicAnd(&a.summ[(uint64_t)in_buf * a.summ_words + (ch >> 5)], ~(1u << (ch & 31)));
+                } else {
+                    list_append(a, s_spare, (uint32_t)ch);
+                }
+            }
         }
     }
     {
         const uint64_t m = wave_min_u64(my_ovf_min);
-        if (lane == 0 && m != ~0ull) atomicMin((unsigned long long *)&c->ovf_min, (unsigned long long)m);
+        if (lane == 0 && m != ~0ull) atomicMin(&s_ovf_min, (unsigned long long)m);
     }
-
-    // ---- per-workgroup flush, then the last workgroup runs the control step
     atomicAdd(&s_items, my_items);
     atomicAdd(&s_edges, my_edges);
     __syncthreads();
+    // pending items per slot: a workgroup's chunks belong to few slots
     for (int s = threadIdx.x; s < a.K; s += WG) {
         const int dlt = s_slot_delta[s];
         if (dlt != 0) atomicAdd(&a.slots[s].pending, dlt);
     }
-    if (threadIdx.x < nb + 3) {
-        const int dlt = s_buf_delta[threadIdx.x];
-        if (dlt != 0) atomicAdd(&c->count[threadIdx.x], (uint32_t)dlt);
-    }
-    // every thread's bitmap, key and counter updates must be visible before
-    // this workgroup counts itself finished
-    __threadfence();
-    __syncthreads();
+    RoundPart *pp = &a.parts[blockIdx.x];
+    if (threadIdx.x < nb + 3) pp->buf_delta[threadIdx.x] = s_buf_delta[threadIdx.x];
     if (threadIdx.x == 0) {
-        if (s_items) atomicAdd((unsigned long long *)&c->items, s_items);
-        if (s_edges) atomicAdd((unsigned long long *)&c->edges, s_edges);
-        __threadfence();
-        const uint32_t t = atomicAdd(&c->finished, 1u);
-        s_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (s_last) {
-        __threadfence();
-        control_step(a, mode, cur);
+        pp->items = s_items;
+        pp->edges = s_edges;
+        pp->ovf_min = s_ovf_min;
     }
 }
 
-// Runs in one workgroup after every other workgroup of the round finished.
-__device__ void control_step(const RoundArgs &a, uint32_t mode, uint64_t cur)
+#define CTRL_WG 1024
+
+// Runs after each round (one workgroup): reduces the round's partials,
+// advances the bucket window, detects finished slots, schedules an overflow
+// redistribution or goes idle, and retires the consumed bucket's chunk list.
+// The control block is staged through LDS so that the serial decision reads
+// no global memory.
+__global__ void __launch_bounds__(CTRL_WG)
+k_control(RoundArgs a, int round_grid)
 {
     Ctrl *c = a.ctrl;
     const uint32_t nb = a.nb;
+    const uint32_t nbuf = nb + 3;
+    __shared__ int32_t s_cnt[MAX_BUFS];
+    __shared__ uint32_t s_ring[MAX_RING];
+    __shared__ uint32_t s_mode0, s_out, s_ovf;
+    __shared__ uint64_t s_cur, s_ovf_min0;
+    __shared__ unsigned long long s_items, s_edges, s_ovf_min;
     __shared__ uint64_t s_newcur;
-    __shared__ uint32_t s_mode;
+    __shared__ uint32_t s_mode, s_in_cleared, s_in_list, s_zero_list;
+    __shared__ uint32_t s_bincnt[LIST_BINS + 1];   // exclusive prefix of the consumed list's bins
+    if (threadIdx.x < nbuf) s_cnt[threadIdx.x] = (int32_t)c->count[threadIdx.x];
+    if (threadIdx.x < nb) s_ring[threadIdx.x] = c->ring[threadIdx.x];
     if (threadIdx.x == 0) {
+        s_mode0 = c->mode; s_out = c->out_buf; s_ovf = c->ovf_buf;
+        s_cur = c->cur; s_ovf_min0 = c->ovf_min;
+        s_items = 0; s_edges = 0; s_ovf_min = ~0ull;
+    }
+    __syncthreads();
+    const uint32_t mode = s_mode0;
+    if (mode == MODE_IDLE) return;
+    const uint64_t cur = s_cur;
+    {
+        unsigned long long it = 0, ed = 0, om = ~0ull;
+        for (int g = threadIdx.x; g < round_grid; g += CTRL_WG) {
+            const RoundPart *pp = &a.parts[g];
+            for (uint32_t b = 0; b < nbuf; ++b) {
+                const int32_t d = pp->buf_delta[b];
+                if (d) atomicAdd(&s_cnt[b], d);
+            }
+            it += pp->items; ed += pp->edges;
+            om = pp->ovf_min < om ? pp->ovf_min : om;
+        }
+        if (it) atomicAdd(&s_items, it);
+        if (ed) atomicAdd(&s_edges, ed);
+        if (om != ~0ull) atomicMin(&s_ovf_min, om);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t ovf_min = s_ovf_min < s_ovf_min0 ? s_ovf_min : s_ovf_min0;
         uint64_t newcur = cur;
+        uint32_t m = 0xFFFFFFFFu;   // decided below unless set here
+        s_in_cleared = 0xFFFFFFFFu;
+        s_zero_list = 0xFFFFFFFFu;
         if (mode == MODE_RELAX) {
             // the round consumed (and cleared) its bucket
             const uint32_t pos = (uint32_t)(cur % nb);
-            const uint32_t in = c->ring[pos];
-            c->count[in] = 0;
-            if (ld32(&c->count[c->out_buf]) > 0) {
-                c->ring[pos] = c->out_buf;
-                c->out_buf = in;
-                s_mode = MODE_RELAX;
+            const uint32_t in = s_ring[pos];
+            s_cnt[in] = 0;
+            s_in_cleared = in;
+            s_in_list = c->list_id[in];
+            if (s_cnt[s_out] > 0) {
+                s_ring[pos] = s_out;
+                s_out = in;
+                m = MODE_RELAX;
             } else {
                 newcur = cur + 1;
-                s_mode = 0xFFFFFFFFu; // decide below
             }
         } else {
-            // redistribution moved what fits the window out of the overflow
-            // bitmap in place; its count is already adjusted
-            s_mode = 0xFFFFFFFFu;
+            // the overflow bucket's surviving chunks were listed in the spare
+            // storage: it becomes the overflow list, the old one the spare
+            const uint32_t old = c->list_id[s_ovf];
+            c->list_id[s_ovf] = c->spare_list;
+            c->spare_list = old;
+            s_zero_list = old;
         }
-        uint32_t tot = 0;
-        for (uint32_t p = 0; p < nb; ++p) tot += ld32(&c->count[c->ring[p]]);
-        tot += ld32(&c->count[c->out_buf]);
-        if (s_mode == 0xFFFFFFFFu) {
-            const uint32_t ovf = ld32(&c->count[c->ovf_buf]);
+        // (a redistribution moved what fits the window out of the overflow
+        // bitmap in place; its count is already adjusted)
+        int32_t tot = s_cnt[s_out];
+        for (uint32_t p = 0; p < nb; ++p) tot += s_cnt[s_ring[p]];
+        if (m == 0xFFFFFFFFu) {
+            const int32_t ovf = s_cnt[s_ovf];
             if (tot == 0) {
                 if (ovf == 0) {
-                    s_mode = MODE_IDLE;
+                    m = MODE_IDLE;
                 } else {
                     // nothing inside the window: jump to the lowest overflow
                     // bucket and redistribute
-                    const uint64_t m = atomic_load_u64(&c->ovf_min);
-                    if (m > newcur) newcur = m;
-                    s_mode = MODE_REDIST;
+                    if (ovf_min > newcur) newcur = ovf_min;
+                    m = MODE_REDIST;
                 }
             } else if (mode == MODE_RELAX && ovf > 0 && newcur % nb == 0) {
                 // the window slid by a full ring: pull overflow items into it
-                s_mode = MODE_REDIST;
+                m = MODE_REDIST;
             } else {
-                s_mode = MODE_RELAX;
+                m = MODE_RELAX;
             }
         }
+        if (m == MODE_REDIST) ovf_min = ~0ull;
+        s_mode = m;
         s_newcur = newcur;
+        c->items += s_items;
+        c->edges += s_edges;
+        c->ovf_min = ovf_min;
+        c->out_buf = s_out;
+        c->cur = newcur;
+        c->rounds++;
     }
     __syncthreads();
     const uint64_t newcur = s_newcur;
+    const uint32_t m = s_mode;
+    if (threadIdx.x < nbuf) c->count[threadIdx.x] = (uint32_t)s_cnt[threadIdx.x];
+    if (threadIdx.x < nb) c->ring[threadIdx.x] = s_ring[threadIdx.x];
+    // the consumed bucket: clear the summary words of its listed chunks and
+    // empty its list
+    if (s_in_cleared != 0xFFFFFFFFu) {
+        uint32_t *sm = a.summ + (uint64_t)s_in_cleared * a.summ_words;
+        uint32_t *cnt = a.list_cnt + (uint64_t)s_in_list * LIST_BINS;
+        if (threadIdx.x < 64) {
+            const int n = threadIdx.x < LIST_BINS ? (int)cnt[threadIdx.x] : 0;
+            const int inc = wave_scan(n);
+            s_bincnt[threadIdx.x] = (uint32_t)(inc - n);
+            if (threadIdx.x == 63) s_bincnt[LIST_BINS] = (uint32_t)inc;
+        }
+        __syncthreads();
+        const uint32_t *lst = a.lists + (uint64_t)s_in_list * LIST_BINS * a.bin_cap;
+        for (uint32_t i = threadIdx.x; i < s_bincnt[LIST_BINS]; i += CTRL_WG)
+            sm[list_at(lst, s_bincnt, a.bin_cap, i) >> 5] = 0u;
+        if (threadIdx.x < LIST_BINS) cnt[threadIdx.x] = 0u;
+    }
+    if (s_zero_list != 0xFFFFFFFFu && threadIdx.x < LIST_BINS)
+        a.list_cnt[(uint64_t)s_zero_list * LIST_BINS + threadIdx.x] = 0u;
     // slot completion: dst settled (its bucket is below the new current
     // bucket), or no pending items left (dst unreachable)
-    for (int s = threadIdx.x; s < a.K; s += WG) {
+    for (int s = threadIdx.x; s < a.K; s += CTRL_WG) {
         SlotInfo *si = &a.slots[s];
         if (si->state != 1) continue;
-        const uint64_t kd = atomic_load_u64(&a.keys[(uint64_t)s * (uint64_t)a.N + (uint64_t)si->dst]);
+        const uint64_t kd = a.keys[(uint64_t)s * (uint64_t)a.N + (uint64_t)si->dst];
         // no set bits left: the search is exhausted and every key final
-        const bool exhausted = (int32_t)ld32((const uint32_t *)&si->pending) <= 0;
+        const bool exhausted = si->pending <= 0;
         const bool reached = kd != KEY_INF;
         if (reached && (exhausted || si->base + key_bucket_rel(kd, a.inv_delta) < newcur)) {
-            si->state = 2; atomicSub(&c->active, 1u); atomicAdd(&c->epoch, 1u);
+            si->state = 2; atomicSub(&c->active, 1u);
         } else if (exhausted) {
-            si->state = 3; atomicSub(&c->active, 1u); atomicAdd(&c->epoch, 1u);
+            si->state = 3; atomicSub(&c->active, 1u);
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        c->rounds++;
-        c->cur = newcur;
-        const uint32_t m = s_mode;
-        if (m == MODE_REDIST) c->ovf_min = ~0ull;
-        c->finished = 0;
-        __threadfence();
-        __hip_atomic_store(&c->mode, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (threadIdx.x == 0) c->mode = m;
 }
 
 // ---------------------------------------------------------------------------
@@ -460,7 +635,7 @@ __global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n)
         // a slot is reseeded only once its previous search has no set bits
         // left, so the source's bit is clear here
         si->pending = 1;
-        if (!test_and_set_bit(a.bits + (uint64_t)buf * a.bitmap_words, item)) atomicAdd(&c->count[buf], 1u);
+        if (push_bit(a, buf, c->list_id[buf], item)) atomicAdd(&c->count[buf], 1u);
         atomicAdd(&c->active, 1u);
     }
     __syncthreads();
@@ -526,9 +701,12 @@ __global__ void k_count_minimal(RoundArgs a, int slot, unsigned long long *out)
 // ---------------------------------------------------------------------------
 #include "engine.h"
 
+#define MAX_GRID 8192
+
 static int launch_round(Engine &en, const RoundArgs &a, int grid)
 {
     hipLaunchKernelGGL(k_round, dim3(grid), dim3(WG), 0, en.stream, a);
+    hipLaunchKernelGGL(k_control, dim3(1), dim3(CTRL_WG), 0, en.stream, a, grid);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -548,10 +726,17 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
         return ORGPU_OK;
     engine_release(en);
     const uint64_t words = (items + 31) / 32;
+    const uint64_t chunks = (words + 63) / 64;
+    const uint64_t summ_words = (chunks + 31) / 32;
     const int nbufs = n_buckets + 3;
     HIP_CHECK_RET(hipMalloc((void **)&en.keys, items * sizeof(uint64_t)));
     HIP_CHECK_RET(hipMalloc((void **)&en.bits, (uint64_t)nbufs * words * sizeof(uint32_t)));
     HIP_CHECK_RET(hipMemset(en.bits, 0, (uint64_t)nbufs * words * sizeof(uint32_t)));
+    HIP_CHECK_RET(hipMalloc((void **)&en.summ, (uint64_t)nbufs * summ_words * sizeof(uint32_t)));
+    const uint64_t bin_cap = (chunks + LIST_BINS - 1) / LIST_BINS + 1;
+    HIP_CHECK_RET(hipMalloc((void **)&en.lists, (uint64_t)(nbufs + 1) * LIST_BINS * bin_cap * sizeof(uint32_t)));
+    HIP_CHECK_RET(hipMalloc((void **)&en.list_cnt, (uint64_t)(nbufs + 1) * LIST_BINS * sizeof(uint32_t)));
+    HIP_CHECK_RET(hipMalloc((void **)&en.parts, (size_t)MAX_GRID * sizeof(RoundPart)));
     HIP_CHECK_RET(hipMalloc((void **)&en.ctrl, sizeof(Ctrl)));
     HIP_CHECK_RET(hipMalloc((void **)&en.slots, sizeof(SlotInfo) * slots));
     HIP_CHECK_RET(hipMalloc((void **)&en.seed, sizeof(SeedReq) * slots));
@@ -576,6 +761,8 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
     HIP_CHECK_RET(hipEventCreate(&en.ev_r1));
     en.cap_nodes = n_nodes; en.cap_slots = slots; en.cap_nb = n_buckets; en.cap_path = max_path;
     en.words = words;
+    en.summ_words = summ_words;
+    en.chunks = chunks;
     return ORGPU_OK;
 }
 
@@ -583,6 +770,10 @@ void engine_release(Engine &en)
 {
     if (en.keys) hipFree(en.keys);
     if (en.bits) hipFree(en.bits);
+    if (en.summ) hipFree(en.summ);
+    if (en.lists) hipFree(en.lists);
+    if (en.list_cnt) hipFree(en.list_cnt);
+    if (en.parts) hipFree(en.parts);
     if (en.ctrl) hipFree(en.ctrl);
     if (en.slots) hipFree(en.slots);
     if (en.seed) hipFree(en.seed);
@@ -656,6 +847,12 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     a.ko_owner = keepout_owner;
     a.keys = en.keys;
     a.bits = en.bits; a.bitmap_words = (uint64_t)((N * (int64_t)K + 31) / 32);
+    a.summ = en.summ; a.summ_words = ((a.bitmap_words + 63) / 64 + 31) / 32;
+    a.lists = en.lists;
+    a.list_cnt = en.list_cnt;
+    a.chunks = (a.bitmap_words + 63) / 64;
+    a.bin_cap = (a.chunks + LIST_BINS - 1) / LIST_BINS + 1;
+    a.parts = en.parts;
 
     // control block init
     Ctrl hc;
@@ -664,15 +861,19 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     hc.ovf_min = ~0ull;
     for (int i = 0; i < nb; ++i) hc.ring[i] = (uint32_t)i;
     hc.out_buf = (uint32_t)nb; hc.ovf_buf = (uint32_t)nb + 1; hc.ovf_next_buf = (uint32_t)nb + 2;
+    for (int i = 0; i < nb + 3; ++i) hc.list_id[i] = (uint32_t)i;
+    hc.spare_list = (uint32_t)nb + 3;
     *en.h_ctrl = hc;
     HIP_CHECK_RET(hipMemcpyAsync(en.ctrl, en.h_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, s));
     HIP_CHECK_RET(hipMemsetAsync(en.slots, 0, sizeof(SlotInfo) * K, s));
     // queue bitmaps must be clear: they are cleared as items are consumed, but
     // a previous call may have stopped with items left (finished slots)
     HIP_CHECK_RET(hipMemsetAsync(en.bits, 0, (size_t)(nb + 3) * a.bitmap_words * sizeof(uint32_t), s));
+    HIP_CHECK_RET(hipMemsetAsync(en.summ, 0, (size_t)(nb + 3) * a.summ_words * sizeof(uint32_t), s));
+    HIP_CHECK_RET(hipMemsetAsync(en.list_cnt, 0, (size_t)(nb + 4) * LIST_BINS * sizeof(uint32_t), s));
 
-    int grid = p->grid > 0 ? p->grid : 2048;
-    grid = (grid + 7) / 8 * 8;
+    int grid = p->grid > 0 ? p->grid : 1024;
+    grid = std::min((grid + 7) / 8 * 8, MAX_GRID);
     const int rpp = p->rounds_per_poll > 0 ? p->rounds_per_poll : 32;
     std::vector<int> slot_req(K, -1);
     int next = 0, done_nets = 0;
