@@ -31,10 +31,10 @@ import time
 
 METRIC = "PathFinder time-to-zero-overuse (s) + SSSP GTEPS, batched vs plain, 1 GPU"
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes of a label-setting search (no re-expansions, no queue or
-# dedupe traffic): per settled node its queue entry (4), own key (8) and CSR
-# row bounds (8); per scanned edge its target id (4), cost (4) and target key (8)
-BYTES_PER_NODE = 20
+# algorithmic bytes of a label-setting search (no re-expansions, no frontier
+# or de-duplication traffic): per settled node its CSR row bounds (8) and own
+# key (8); per scanned edge its target id (4), cost (4) and target key (8)
+BYTES_PER_NODE = 16
 BYTES_PER_EDGE = 16
 
 
@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--slots", type=int, default=64, help="concurrent searches of the batched form")
+    ap.add_argument("--slots", type=int, default=256, help="concurrent searches of the batched form")
     ap.add_argument("--plain-sample", type=int, default=48, help="searches timed in the plain vs batched comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU oracle sample budget (0 = skip)")
     ap.add_argument("--workload", default="cfg2")
@@ -206,6 +206,7 @@ def main():
         "minimal_edges_per_step": int(min_edges), "minimal_nodes_per_step": int(min_nodes),
         "edges_scanned_per_step": int(d.get("sssp_edges", 0) / K),
         "sssp_s_per_step": round(d.get("sssp_s", 0) / K, 3),
+        "sssp_kernel_gteps": round(min_edges * K / max(d.get("sssp_kernel_ms", 0) * 1e-3, 1e-9) / 1e9, 4),
         "roi_s_per_step": round(d.get("roi_s", 0) / K, 3),
         "plain_vs_batched": {"searches": len(sreqs), "plain_s": round(timing["plain"], 3),
                              "batched_s": round(timing["batched"], 3),

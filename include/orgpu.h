@@ -107,6 +107,14 @@ typedef struct orgpu_sssp_params {
     int32_t rounds_per_poll;/* kernel rounds queued between host status polls */
     int32_t grid;           /* workgroups per round launch (0 = auto) */
     int32_t count_minimal;  /* 1: count Dijkstra-minimal settled nodes/edges per net */
+    /* Goal-directed search (0, 0 = plain Dijkstra order).  Lower bounds of the
+     * cost of one planar step (h_xy) and of one layer of via span (h_z) under
+     * the call's costs: buckets are taken on g + h with
+     * h(v) = h_xy*(|dx|+|dy|) + h_z*|dz| to the destination.  With h a lower
+     * bound of the remaining cost, every node on any equal-cost shortest path
+     * is still expanded before the destination is declared settled, so paths
+     * and tie-breaks are those of the plain search. */
+    float h_xy, h_z;
 } orgpu_sssp_params;
 
 typedef struct orgpu_sssp_stats {
@@ -249,6 +257,13 @@ int orgpu_history_update(orgpu_t *h, int64_t n_edges, const float *present, cons
  * UPF:5021-5024). */
 int orgpu_overuse_sparse(orgpu_t *h, int64_t n_edges, const float *present, const float *history,
                          int mode, float hist_scale, float threshold, orgpu_sparse_out *out);
+
+/* Lower bounds for the goal-directed search (orgpu_sssp_params.h_xy/h_z):
+ * out[0] = min cost over planar edges, out[1] = min over via edges of
+ * cost / layer span, both over finite non-negative costs (device f32[2],
+ * caller-owned; 0x7F7F7F7F when a kind has no finite edge).  No reference
+ * counterpart: the reference's Dijkstra is not goal-directed. */
+int orgpu_cost_bounds(orgpu_t *h, const orgpu_graph *g, const uint8_t *edge_layer, float *out);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,7 @@ EXPORTED = [
     "orgpu_lattice_count", "orgpu_lattice_build",
     "orgpu_reserve", "orgpu_route", "orgpu_roi_bfs",
     "orgpu_usage_scatter", "orgpu_present_build", "orgpu_present_ema", "orgpu_cost_update",
-    "orgpu_history_update", "orgpu_overuse_sparse",
+    "orgpu_history_update", "orgpu_overuse_sparse", "orgpu_cost_bounds",
 ]
 
 
@@ -54,7 +54,8 @@ class NetReq(ctypes.Structure):
 class SsspParams(ctypes.Structure):
     _fields_ = [("delta", ctypes.c_float), ("n_buckets", ctypes.c_int32),
                 ("slots", ctypes.c_int32), ("rounds_per_poll", ctypes.c_int32),
-                ("grid", ctypes.c_int32), ("count_minimal", ctypes.c_int32)]
+                ("grid", ctypes.c_int32), ("count_minimal", ctypes.c_int32),
+                ("h_xy", ctypes.c_float), ("h_z", ctypes.c_float)]
 
 
 class SsspStats(ctypes.Structure):
@@ -124,6 +125,7 @@ def load():
         "orgpu_present_build": [P, i64, P, P, P, i64, f32],
         "orgpu_present_ema": [P, i64, P, P, ctypes.c_int, f32, f32, ctypes.POINTER(SparseOut)],
         "orgpu_cost_update": [P, ctypes.POINTER(CostParams), i64, i64, P, P, P, P, P, P, P],
+        "orgpu_cost_bounds": [P, ctypes.POINTER(Graph), P, P],
         "orgpu_history_update": [P, i64, P, P, P, f32, f32, f32, ctypes.c_int],
         "orgpu_overuse_sparse": [P, i64, P, P, ctypes.c_int, f32, f32, ctypes.POINTER(SparseOut)],
     }

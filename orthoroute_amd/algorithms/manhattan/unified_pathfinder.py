@@ -35,7 +35,8 @@ from typing import Dict, List, Optional, Set, Tuple
 import numpy as np
 
 from ... import _abi
-from ...device import (Context, DeviceLattice, EdgeState, SearchParams, roi_bfs, route as device_route)
+from ...device import (Context, DeviceLattice, EdgeState, SearchParams, cost_bounds, roi_bfs,
+                       route as device_route)
 from ...lattice import DIR_H, DIR_HV, DIR_V, layer_directions, reference_via_pairs
 from ...reductions import sum_f32
 from .board_analyzer import analyze_board_characteristics
@@ -69,7 +70,8 @@ class PathFinderConfig:
     """UPF:564-634 defaults, plus this build's execution knobs:
     lattice_policy ('reference' | 'adjacent'), batch_slots (concurrent
     searches per launch), sssp_delta / sssp_buckets (bucket width and ring
-    size; None = autotuned), max_iterations_cap."""
+    size; None = autotuned), max_iterations_cap, count_minimal,
+    goal_directed."""
     max_iterations: int = 40
     pres_fac_init: float = 1.0
     pres_fac_mult: float = 1.10
@@ -117,11 +119,12 @@ class PathFinderConfig:
     allowed_via_spans = None
     # execution knobs of this build
     lattice_policy: str = "reference"
-    batch_slots: int = 64
+    batch_slots: int = 256
     sssp_delta = None
     sssp_buckets = None
     max_iterations_cap = None
     count_minimal = False          # also count Dijkstra-minimal nodes/edges per search
+    goal_directed = True           # bucket on g + lower bound to the destination (same paths)
 
 
 # ----------------------------------------------------------------------------
@@ -628,6 +631,8 @@ class PathFinderRouter:
                 p.col_cap, p.seg_cap = 4, 2
         self.accounting.state.update_costs(p)
         self.ctx.sync()
+        # the iteration's cost lower bounds for the goal-directed search
+        self._hbounds = cost_bounds(self.ctx, self.graph.dlat, self.accounting.total_cost)
 
     # ---------------- committed-net arrays
     def _flat(self):
@@ -839,8 +844,13 @@ class PathFinderRouter:
             d2, nb2 = choose_params(self, slots)
             d = d if d is not None else d2
             nb = nb if nb is not None else nb2
+        # goal-directed order (same paths) unless the Dijkstra-minimal work is
+        # being counted, which needs the plain order's settled set
+        hx, hz = (0.0, 0.0)
+        if self.config.goal_directed and not self.config.count_minimal:
+            hx, hz = getattr(self, "_hbounds", (0.0, 0.0))
         return SearchParams(slots=slots, delta=float(d), n_buckets=int(nb), rounds_per_poll=32,
-                            count_minimal=bool(self.config.count_minimal))
+                            count_minimal=bool(self.config.count_minimal), h_xy=hx, h_z=hz)
 
     def _search_batch(self, reqs, keepout_owner=None, readd=True):
         """reqs: list of (src, dst, net_int, roi, stagnation_bonus).  roi
@@ -867,7 +877,8 @@ class PathFinderRouter:
         todo = [i for i in range(len(reqs)) if i not in skip]
         if not todo:
             return out
-        slots = max(1, min(int(self.config.batch_slots), len(todo)))
+        # a work item is the u32 slot*N + node: at most (2^32 - 64) / N slots
+        slots = max(1, min(int(self.config.batch_slots), len(todo), ((1 << 32) - 64) // self.N - 1))
         params = self._search_params(slots)
         sreqs = []
         for i in todo:

@@ -43,10 +43,11 @@ __host__ __device__ inline uint32_t key_pred(uint64_t k) { return (uint32_t)(k &
 // (keepout count, distance) part, comparable as an integer
 __host__ __device__ inline uint64_t key_cost(uint64_t k) { return k >> 28; }
 
-// bucket of a key relative to the slot's base bucket
-__host__ __device__ inline uint64_t key_bucket_rel(uint64_t k, float inv_delta)
+// bucket of a key relative to the slot's base bucket; h: lower bound of the
+// remaining cost to the destination (0 for plain Dijkstra order)
+__host__ __device__ inline uint64_t key_bucket_rel(uint64_t k, float inv_delta, float h = 0.0f)
 {
-    float q = key_dist(k) * inv_delta;
+    float q = (key_dist(k) + h) * inv_delta;
     uint32_t b = q >= 4.0e9f ? 4000000000u : (uint32_t)q;
     return ((uint64_t)key_ko(k) << 32) | b;
 }
@@ -90,6 +91,8 @@ struct SlotInfo {
     uint32_t state;     // 0 free/draining, 1 searching, 2 dst reached, 3 dst unreachable
     int32_t req;        // request index
     int32_t pending;    // items of this slot still queued (pushes - pops)
+    int32_t dx, dy, dz; // destination coordinates (goal-directed bucketing)
+    int32_t pad;
 };
 
 struct RoundArgs {
@@ -99,6 +102,8 @@ struct RoundArgs {
     int64_t N;
     uint32_t nb;
     float inv_delta;
+    float h_xy, h_z;          // goal-directed lower bounds (0, 0: Dijkstra order)
+    int32_t X, plane;         // node = z*plane + y*X + x
     const int32_t *indptr;
     const int32_t *indices;
     const float *cost;

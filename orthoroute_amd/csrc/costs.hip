@@ -337,3 +337,48 @@ extern "C" int orgpu_overuse_sparse(orgpu_t *h, int64_t n_edges, const float *pr
     HIP_CHECK_RET(hipGetLastError());
     return read_sparse(en, out);
 }
+
+// ---------------------------------------------------------------------------
+// Lower bounds of the iteration's costs for the goal-directed search: the
+// cheapest planar edge, and the cheapest via edge per layer of span
+// (cost / |z_dst - z_src|).  Non-negative floats order like their bit
+// patterns, so the minima are taken with integer atomicMin on the bits.
+// Infinite (blocked) and NaN costs are skipped.
+// ---------------------------------------------------------------------------
+__global__ void k_cost_bounds(int64_t n_edges, const int32_t *indices, const float *cost, const uint8_t *edge_kind,
+                              const uint8_t *edge_layer, int32_t plane, uint32_t *out_bits)
+{
+    float m_xy = __builtin_inff(), m_z = __builtin_inff();
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_edges; e += (int64_t)gridDim.x * blockDim.x) {
+        const float c = cost[e];
+        if (!(c >= 0.0f) || !(c < __builtin_inff())) continue;
+        if (edge_kind[e] == 0) {
+            m_xy = fminf(m_xy, c);
+        } else {
+            const int span = abs(indices[e] / plane - (int)edge_layer[e]);
+            if (span > 0) m_z = fminf(m_z, c / (float)span);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        m_xy = fminf(m_xy, __shfl_xor(m_xy, o, 64));
+        m_z = fminf(m_z, __shfl_xor(m_z, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&out_bits[0], __float_as_uint(m_xy));
+        atomicMin(&out_bits[1], __float_as_uint(m_z));
+    }
+}
+
+extern "C" int orgpu_cost_bounds(orgpu_t *h, const orgpu_graph *g, const uint8_t *edge_layer, float *out)
+{
+    if (!h || !g || !g->indices || !g->cost || !g->edge_kind || !edge_layer || !out || g->n_edges < 0)
+        return orgpu_fail(ORGPU_EINVAL, "cost_bounds: bad argument");
+    if (g->X <= 0 || g->Y <= 0) return orgpu_fail(ORGPU_EINVAL, "cost_bounds: lattice shape missing");
+    HIP_CHECK_RET(hipSetDevice(h->device));
+    hipStream_t s = h->stream;
+    HIP_CHECK_RET(hipMemsetAsync(out, 0x7F, 2 * sizeof(float), s));   // 0x7F7F7F7F: a large finite float
+    hipLaunchKernelGGL(k_cost_bounds, dim3(grid_for(std::max<int64_t>(g->n_edges, 1))), dim3(256), 0, s, g->n_edges,
+                       g->indices, g->cost, g->edge_kind, edge_layer, g->X * g->Y, (uint32_t *)out);
+    HIP_CHECK_RET(hipGetLastError());
+    return ORGPU_OK;
+}

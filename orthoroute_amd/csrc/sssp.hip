@@ -159,6 +159,18 @@ __device__ __forceinline__ ChunkWalk chunk_walk(uint32_t n, uint32_t wave)
     return w;
 }
 
+// Lower bound of the remaining cost from node v to the slot's destination
+// (goal-directed bucketing; 0 when the call runs in plain Dijkstra order).
+__device__ __forceinline__ float heur(const RoundArgs &a, const SlotInfo &si, uint32_t v)
+{
+    if (a.h_xy == 0.0f && a.h_z == 0.0f) return 0.0f;
+    const int z = (int)(v / (uint32_t)a.plane);
+    const int r = (int)(v - (uint32_t)z * (uint32_t)a.plane);
+    const int y = r / a.X;
+    const int x = r - y * a.X;
+    return a.h_xy * (float)(abs(x - si.dx) + abs(y - si.dy)) + a.h_z * (float)abs(z - si.dz);
+}
+
 // Expands one item per lane (lanes with has=false only take part in the
 // wave's collective steps): consumes it, and if its key still lies in the
 // current bucket relaxes its out-edges.  A lane walks its own CSR row four
@@ -178,7 +190,17 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bo
     const SlotInfo si = a.slots[slot];
     if (si.state != 1) return;
     const uint64_t ku = atomic_load_u64(&a.keys[item]);
-    if (ku == KEY_INF || si.base + key_bucket_rel(ku, a.inv_delta) != cur) return;
+    if (ku == KEY_INF) return;
+    {
+        // Plain order: an item whose key has dropped below the current bucket
+        // since its push was already expanded there (stale copy).  Goal-
+        // directed order: FP32 rounding of g + h can put a neighbour one
+        // bucket below the bucket that pushed it (into OUT), so everything at
+        // or below the current bucket is expanded.
+        const uint64_t b = si.base + key_bucket_rel(ku, a.inv_delta, heur(a, si, u));
+        const bool goal = a.h_xy != 0.0f || a.h_z != 0.0f;
+        if (b > cur || (b < cur && !goal)) return;
+    }
     const int row = a.indptr[u];
     const int deg = a.indptr[u + 1] - row;
     my_items++;
@@ -226,7 +248,7 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bo
             if (!ok[q] || !(nk[q] < kv[q])) continue;
             const uint64_t old = atomicMin((unsigned long long *)&keys[v[q]], (unsigned long long)nk[q]);
             if (nk[q] < old && key_cost(nk[q]) < key_cost(old)) {
-                const uint64_t b = si.base + key_bucket_rel(nk[q], a.inv_delta);
+                const uint64_t b = si.base + key_bucket_rel(nk[q], a.inv_delta, heur(a, si, v[q]));
                 uint32_t buf;
                 if (b <= cur) buf = s_out;
                 else if (b < cur + nb) buf = s_ring[b % nb];
@@ -389,7 +411,7 @@ k_round(RoundArgs a)
                     if (si.state == 1) {
                         const uint64_t ku = atomic_load_u64(&a.keys[item]);
                         if (ku != KEY_INF) {
-                            const uint64_t b = si.base + key_bucket_rel(ku, a.inv_delta);
+                            const uint64_t b = si.base + key_bucket_rel(ku, a.inv_delta, heur(a, si, item - slot * N));
                             if (b >= cur) {
                                 if (b < cur + nb) {
                                     dst_buf = s_ring[b % nb];
@@ -587,7 +609,10 @@ k_control(RoundArgs a, int round_grid)
         // no set bits left: the search is exhausted and every key final
         const bool exhausted = si->pending <= 0;
         const bool reached = kd != KEY_INF;
-        if (reached && (exhausted || si->base + key_bucket_rel(kd, a.inv_delta) < newcur)) {
+        // goal-directed order settles the destination one bucket later: the
+        // margin absorbs FP32 rounding of g + h on its equal-cost predecessors
+        const uint64_t margin = (a.h_xy != 0.0f || a.h_z != 0.0f) ? 1u : 0u;
+        if (reached && (exhausted || si->base + key_bucket_rel(kd, a.inv_delta) + margin < newcur)) {
             si->state = 2; atomicSub(&c->active, 1u);
         } else if (exhausted) {
             si->state = 3; atomicSub(&c->active, 1u);
@@ -624,10 +649,19 @@ __global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n)
     if (i < n) {
         const SeedReq r = reqs[i];
         SlotInfo *si = &a.slots[r.slot];
-        si->base = c->cur;
         si->src = r.src; si->dst = r.dst; si->net_id = r.net_id;
         si->roi_index = r.roi_index; si->use_keepout = r.use_keepout;
         si->req = r.req;
+        if (a.plane > 0 && a.X > 0) {
+            const int z = r.dst / a.plane, rr = r.dst - z * a.plane;
+            si->dz = z; si->dy = rr / a.X; si->dx = rr - (rr / a.X) * a.X;
+        } else {
+            si->dx = si->dy = si->dz = 0;
+        }
+        // the source (g = 0) lands in the current bucket: its bucket under
+        // the goal-directed order, floor(h(src) / delta), is taken off the
+        // slot's base (the bucket index starts at 2^32, so this never wraps)
+        si->base = c->cur - key_bucket_rel(key_pack(0, 0.0f, KEY_NOPRED), a.inv_delta, heur(a, *si, (uint32_t)r.src));
         si->state = 1;
         const uint32_t item = (uint32_t)r.slot * (uint32_t)a.N + (uint32_t)r.src;
         a.keys[item] = key_pack(0, 0.0f, KEY_NOPRED);
@@ -825,6 +859,11 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
                           "(N=%lld, K=%d, nb=%d)", (long long)N, K, p->n_buckets, (long long)en.cap_nodes,
                           en.cap_slots, en.cap_nb);
     if (!(p->delta > 0.0f)) return orgpu_fail(ORGPU_EINVAL, "route: delta must be > 0");
+    if (!(p->h_xy >= 0.0f && p->h_z >= 0.0f && p->h_xy < 3.0e38f && p->h_z < 3.0e38f))
+        return orgpu_fail(ORGPU_EINVAL, "route: h_xy %g / h_z %g must be finite and >= 0", (double)p->h_xy, (double)p->h_z);
+    if ((p->h_xy > 0.0f || p->h_z > 0.0f) && (g->X <= 0 || g->Y <= 0 || g->L <= 0 ||
+                                              (int64_t)g->X * g->Y * g->L != g->n_nodes))
+        return orgpu_fail(ORGPU_EINVAL, "route: goal-directed search needs the lattice shape (X*Y*L == n_nodes)");
     // keepout edges are ranked by count ahead of the remaining cost, which is
     // what a penalty far above every path cost (the reference's 1e9) does
     if (keepout_owner && !(keepout_cost >= 1.0e6f))
@@ -842,6 +881,8 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     RoundArgs a;
     a.ctrl = en.ctrl; a.slots = en.slots; a.K = K; a.N = N; a.nb = (uint32_t)nb;
     a.inv_delta = 1.0f / p->delta;
+    a.h_xy = p->h_xy; a.h_z = p->h_z;
+    a.X = g->X; a.plane = g->X * g->Y;
     a.indptr = g->indptr; a.indices = g->indices; a.cost = g->cost; a.kind = g->edge_kind;
     a.roi = roi_bitmaps; a.roi_words = (N + 31) / 32;
     a.ko_owner = keepout_owner;
@@ -858,6 +899,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     Ctrl hc;
     memset(&hc, 0, sizeof(hc));
     hc.mode = MODE_IDLE;
+    hc.cur = 1ull << 32;   // room below for goal-directed slot bases
     hc.ovf_min = ~0ull;
     for (int i = 0; i < nb; ++i) hc.ring[i] = (uint32_t)i;
     hc.out_buf = (uint32_t)nb; hc.ovf_buf = (uint32_t)nb + 1; hc.ovf_next_buf = (uint32_t)nb + 2;

@@ -124,15 +124,38 @@ class DeviceLattice:
 
 
 class SearchParams:
-    def __init__(self, slots=1, delta=1.0, n_buckets=8, rounds_per_poll=32, grid=0, count_minimal=False):
+    """Search launch parameters.  h_xy / h_z: lower bounds of one planar step
+    and of one layer of via span under the call's costs (cost_bounds); both 0
+    runs the plain Dijkstra bucket order.  Paths are the same either way."""
+
+    def __init__(self, slots=1, delta=1.0, n_buckets=8, rounds_per_poll=32, grid=0, count_minimal=False,
+                 h_xy=0.0, h_z=0.0):
         self.slots, self.delta, self.n_buckets = int(slots), float(delta), int(n_buckets)
         self.rounds_per_poll, self.grid, self.count_minimal = int(rounds_per_poll), int(grid), bool(count_minimal)
+        self.h_xy, self.h_z = float(h_xy), float(h_z)
 
     def c(self):
         p = _abi.SsspParams()
         p.delta, p.n_buckets, p.slots = self.delta, self.n_buckets, self.slots
         p.rounds_per_poll, p.grid, p.count_minimal = self.rounds_per_poll, self.grid, int(self.count_minimal)
+        p.h_xy, p.h_z = self.h_xy, self.h_z
         return p
+
+
+def cost_bounds(ctx, lat, cost):
+    """Lower bounds (h_xy, h_z) of the costs for the goal-directed search:
+    the cheapest planar edge and the cheapest via edge per layer of span,
+    shrunk by 2^-10 so that FP32 rounding of g + h can never exceed a true
+    remaining cost; 0 where no finite edge of the kind exists."""
+    torch = ctx.torch
+    out = torch.empty(2, dtype=torch.float32, device=ctx.device)
+    g = lat.graph(cost)
+    _abi.check(ctx.lib.orgpu_cost_bounds(ctx.handle, ctypes.byref(g), _ptr(lat.edge_layer), _ptr(out)))
+    b = out.cpu().numpy().astype(np.float64)
+    res = []
+    for v in b:
+        res.append(0.0 if not np.isfinite(v) or v > 1e30 or v <= 0.0 else float(np.float32(v * (1.0 - 2.0 ** -10))))
+    return res[0], res[1]
 
 
 def route(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=None, path_capacity=None):

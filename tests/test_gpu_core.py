@@ -60,9 +60,10 @@ def make_costs(o, rng, kind="jitter"):
     return c
 
 
-def run_case(ctx, X, Y, L, policy, slots, n_nets, seed, cost_kind, roi_frac=0.5, keepout=False, dn=None):
+def run_case(ctx, X, Y, L, policy, slots, n_nets, seed, cost_kind, roi_frac=0.5, keepout=False, dn=None,
+             goal=False):
     import torch
-    from orthoroute_amd.device import SearchParams, route
+    from orthoroute_amd.device import SearchParams, cost_bounds, route
     say(f"[route] {policy} {X}x{Y}x{L} slots={slots} nets={n_nets} seed={seed} costs={cost_kind} keepout={keepout}")
     rng = np.random.default_rng(seed)
     o = oracle.Lattice(X, Y, L, policy)
@@ -92,8 +93,9 @@ def run_case(ctx, X, Y, L, policy, slots, n_nets, seed, cost_kind, roi_frac=0.5,
     own_d = torch.from_numpy(owner).to(ctx.device) if owner is not None else None
     if dn is None:
         dn = (float(rng.choice([0.3, 1.0, 4.0])), int(rng.choice([1, 4, 8])))
-    say(f"[route]   delta={dn[0]} n_buckets={dn[1]}")
-    p = SearchParams(slots=slots, delta=dn[0], n_buckets=dn[1], rounds_per_poll=16)
+    hx, hz = cost_bounds(ctx, g, cost_d) if goal else (0.0, 0.0)
+    say(f"[route]   delta={dn[0]} n_buckets={dn[1]} h=({hx}, {hz})")
+    p = SearchParams(slots=slots, delta=dn[0], n_buckets=dn[1], rounds_per_poll=16, h_xy=hx, h_z=hz)
     res, stats = route(ctx, g, cost_d, reqs, p, roi_bitmaps=roi_d, keepout_owner=own_d)
     for i, (s, d, nid, roi_idx, ko) in enumerate(reqs):
         path, dist, pred, _, _ = oracle.sssp_canon(o, c, s, d, roi_mask=masks[i],
@@ -111,16 +113,41 @@ def run_case(ctx, X, Y, L, policy, slots, n_nets, seed, cost_kind, roi_frac=0.5,
     return stats
 
 
+@pytest.mark.parametrize("goal", [False, True], ids=["dijkstra", "goal"])
 @pytest.mark.parametrize("policy", ["reference", "adjacent"])
 @pytest.mark.parametrize("slots", [1, 7])
-def test_route_matches_oracle(ctx, policy, slots):
+def test_route_matches_oracle(ctx, policy, slots, goal):
     for seed, kind in [(1, "jitter"), (2, "ties"), (3, "random")]:
-        run_case(ctx, 24, 20, 6, policy, slots, 24, seed + 10 * slots, kind)
+        run_case(ctx, 24, 20, 6, policy, slots, 24, seed + 10 * slots, kind, goal=goal)
 
 
-def test_route_keepout_matches_oracle(ctx):
-    run_case(ctx, 20, 18, 6, "reference", 5, 20, 77, "jitter", keepout=True)
-    run_case(ctx, 20, 18, 6, "adjacent", 1, 10, 78, "random", keepout=True)
+@pytest.mark.parametrize("goal", [False, True], ids=["dijkstra", "goal"])
+def test_route_keepout_matches_oracle(ctx, goal):
+    run_case(ctx, 20, 18, 6, "reference", 5, 20, 77, "jitter", keepout=True, goal=goal)
+    run_case(ctx, 20, 18, 6, "adjacent", 1, 10, 78, "random", keepout=True, goal=goal)
+
+
+def test_cost_bounds(ctx):
+    # the cheapest planar edge and the cheapest via edge per layer of span,
+    # recomputed by brute force over the oracle lattice
+    import torch
+    from orthoroute_amd.device import cost_bounds
+    for policy, seed in [("reference", 31), ("adjacent", 32)]:
+        X, Y, L = 18, 15, 7
+        rng = np.random.default_rng(seed)
+        o = oracle.Lattice(X, Y, L, policy)
+        g = ctx.build_lattice(X, Y, L, policy)
+        c = make_costs(o, rng, "random")
+        hx, hz = cost_bounds(ctx, g, torch.from_numpy(c).to(ctx.device))
+        plane = X * Y
+        src = np.repeat(np.arange(o.N), np.diff(o.indptr))
+        span = np.abs(o.indices // plane - src // plane)
+        fin = np.isfinite(c)
+        exp_xy = c[(o.edge_kind == 0) & fin].min()
+        vs = (o.edge_kind == 1) & fin
+        exp_z = (c[vs] / span[vs].astype(np.float32)).min()
+        shrink = lambda v: float(np.float32(np.float64(v) * (1.0 - 2.0 ** -10)))
+        assert hx == shrink(exp_xy) and hz == shrink(exp_z), (policy, hx, hz, exp_xy, exp_z)
 
 
 def test_route_unreachable_and_refill(ctx):
@@ -137,3 +164,4 @@ def test_route_overflow_heavy(ctx, nb):
     # already present); the bucket jump out of an empty ring must not pass them
     for seed, kind in [(21, "random"), (22, "jitter")]:
         run_case(ctx, 24, 20, 6, "adjacent", 4, 16, seed, kind, roi_frac=0.0, dn=(0.05, nb))
+        run_case(ctx, 24, 20, 6, "adjacent", 4, 16, seed, kind, roi_frac=0.0, dn=(0.05, nb), goal=True)
