@@ -166,7 +166,7 @@ def main():
         out = r._search_batch(sreqs)
         torch.cuda.synchronize()
         timing[label] = time.perf_counter() - t0
-        paths[label] = [None if o is None else o[0] for o in out]
+        paths[label] = [None if o is None else o[0].tolist() for o in out]
         log(f"[bench] {label} sample of {len(sreqs)} searches: {timing[label]:.2f}s")
     cfg.batch_slots = args.slots
     same = paths["plain"] == paths["batched"]

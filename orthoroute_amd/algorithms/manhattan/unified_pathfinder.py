@@ -362,7 +362,6 @@ class PathFinderRouter:
         self.net_clean_streak: Dict[str, int] = defaultdict(int)
         self._net_to_edges: Dict[str, np.ndarray] = {}
         self._net_nodes: Dict[str, np.ndarray] = {}
-        self._net_hops: Dict[str, np.ndarray] = {}
         self._ko_nodes = np.zeros(0, np.int64)      # via keepouts: node, first owner
         self._ko_owner = np.zeros(0, np.int64)
         self.portals = {}
@@ -639,25 +638,38 @@ class PathFinderRouter:
         """Flat arrays over the nets that hold a path, in net_paths order:
         per edge (edge id, src node, dst node, net id) and per via hop
         (x, y, z_lo, z_hi, net id)."""
-        E, S, D, NE, H, NH = [], [], [], [], [], []
+        nodes_l, edges_l, ids = [], [], []
         for name, path in self.net_paths.items():
             if not path or len(path) < 2:
                 continue
-            nid = self._get_net_id(name)
             nodes = self._net_nodes.get(name)
             if nodes is None:
                 nodes = np.asarray(path, dtype=np.int64)
             e = self._net_to_edges.get(name)
-            if e is not None and len(e) == len(nodes) - 1:
-                E.append(e); S.append(nodes[:-1]); D.append(nodes[1:])
-                NE.append(np.full(len(e), nid, np.int64))
-            h = self._net_hops.get(name)
-            if h is None:
-                h = self._via_hops(path)
-            if len(h):
-                H.append(h); NH.append(np.full(len(h), nid, np.int64))
-        cat = lambda xs, w=None: (np.concatenate(xs) if xs else np.zeros((0,) if w is None else (0, w), np.int64))
-        return cat(E), cat(S), cat(D), cat(NE), cat(H, 4), cat(NH)
+            if e is None or len(e) != len(nodes) - 1:
+                e = np.full(len(nodes) - 1, -1, np.int64)
+            nodes_l.append(nodes); edges_l.append(e); ids.append(self._get_net_id(name))
+        z0 = np.zeros(0, np.int64)
+        if not nodes_l:
+            return z0, z0, z0, z0, np.zeros((0, 4), np.int64), z0
+        S = np.concatenate([n[:-1] for n in nodes_l]).astype(np.int64)
+        D = np.concatenate([n[1:] for n in nodes_l]).astype(np.int64)
+        E = np.concatenate(edges_l).astype(np.int64)
+        NS = np.repeat(np.asarray(ids, np.int64), [len(n) - 1 for n in nodes_l])
+        H, NH = self._hops_of(S, D, NS)
+        ok = E >= 0
+        return E[ok], S[ok], D[ok], NS[ok], H, NH
+
+    def _hops_of(self, S, D, NS):
+        """Via hops (x, y, z_lo, z_hi) among node pairs S->D, with their nets."""
+        plane = self._Nx * self._Ny
+        rs, rd = S % plane, D % plane
+        zs, zd = S // plane, D // plane
+        hop = (rs == rd) & (zs != zd)
+        r = rs[hop]
+        H = np.stack([r % self._Nx, r // self._Nx, np.minimum(zs[hop], zd[hop]), np.maximum(zs[hop], zd[hop])],
+                     axis=1) if hop.any() else np.zeros((0, 4), np.int64)
+        return H, NS[hop]
 
     # ---------------- via usage / ownership (UPF:2678-2904)
     def _rebuild_via_usage_from_committed(self, flat):
@@ -700,6 +712,26 @@ class PathFinderRouter:
             self._ko_nodes = np.zeros(0, np.int64)
             self._ko_owner = np.zeros(0, np.int64)
 
+    def _mark_barrels_many(self, paths, nids):
+        """_mark_barrels for several paths in order (a later path's barrel
+        overwrites an earlier one's owner, as the per-path calls would)."""
+        if not paths:
+            return
+        S = np.concatenate([np.asarray(p[:-1], np.int64) for p in paths])
+        D = np.concatenate([np.asarray(p[1:], np.int64) for p in paths])
+        NS = np.repeat(np.asarray(nids, np.int64), [len(p) - 1 for p in paths])
+        H, NH = self._hops_of(S, D, NS)
+        if len(H) == 0:
+            return
+        X, plane = self._Nx, self._Nx * self._Ny
+        span = H[:, 3] - H[:, 2] + 1
+        rep = np.repeat(np.arange(len(span)), span)
+        zz = np.repeat(H[:, 2], span) + (np.arange(int(span.sum())) - np.repeat(np.cumsum(span) - span, span))
+        nodes = zz * plane + H[rep, 1] * X + H[rep, 0]
+        owner = NH[rep]
+        ur, last_rev = np.unique(nodes[::-1], return_index=True)
+        self.node_owner[ur] = owner[::-1][last_rev]
+
     def _mark_barrels(self, path, nid):
         X, plane = self._Nx, self._Nx * self._Ny
         for x, y, zlo, zhi in self._via_hops(path):
@@ -728,12 +760,10 @@ class PathFinderRouter:
         self._net_to_edges[nid] = np.asarray(edges, dtype=np.int64)
         if path is not None:
             self._net_nodes[nid] = np.asarray(path, dtype=np.int64)
-            self._net_hops[nid] = self._via_hops(path)
 
     def _clear_net_edge_tracking(self, nid):
         self._net_to_edges.pop(nid, None)
         self._net_nodes.pop(nid, None)
-        self._net_hops.pop(nid, None)
 
     # ---------------- ordering / hotset / rip (UPF:4401-4439, 5009-5140, 5300-5335)
     def _dense(self, idx, val):
@@ -854,7 +884,8 @@ class PathFinderRouter:
 
     def _search_batch(self, reqs, keepout_owner=None, readd=True):
         """reqs: list of (src, dst, net_int, roi, stagnation_bonus).  roi
-        False = whole graph.  Returns per request (nodes, edges) or None."""
+        False = whole graph.  Returns per request (nodes, edges) as int32
+        arrays, or None."""
         torch = self.ctx.torch
         dlat = self.graph.dlat
         out = [None] * len(reqs)
@@ -902,7 +933,7 @@ class PathFinderRouter:
         self.stats["sssp_searches"] += len(sreqs)
         for j, i in enumerate(todo):
             r = res[j]
-            out[i] = None if r is None else (r[0].tolist(), r[1])
+            out[i] = None if r is None else (r[0], r[1])
         return out
 
     def _route_all(self, tasks, all_tasks=None, pres_fac=1.0, iteration=0, over_map=None):
@@ -932,22 +963,30 @@ class PathFinderRouter:
             fres = self._search_batch(freqs, None, readd=False)
             for i, r in zip(fb, fres):
                 results[i] = r
-        # commit in routing order (UPF:4783-4812)
+        # commit in routing order (UPF:4783-4812); usage counts are integer
+        # adds, so the per-net +1/-1 scatters run as two batched launches
+        dec, inc, marked, mark_ids = [], [], [], []
         for nid, r in zip(ordered, results):
             if nid in self.net_paths and self.net_paths[nid]:
                 old = self._net_to_edges.get(nid)
                 if old is not None:
-                    self.accounting.clear_path(old)
+                    dec.append(old)
                 self._clear_net_edge_tracking(nid)
             if r is not None and len(r[0]) > 1:
                 path, edges = r
-                self.accounting.commit_path(edges)
-                self.net_paths[nid] = path
-                self._mark_barrels(path, self._get_net_id(nid))
+                inc.append(edges)
+                self.net_paths[nid] = path.tolist()
+                marked.append(path)
+                mark_ids.append(self._get_net_id(nid))
                 self._update_net_edge_tracking(nid, edges, path)
             else:
                 self.net_paths[nid] = []
                 self._clear_net_edge_tracking(nid)
+        if dec:
+            self.accounting.clear_path(np.concatenate(dec))
+        if inc:
+            self.accounting.commit_path(np.concatenate(inc))
+        self._mark_barrels_many(marked, mark_ids)
         routed = sum(1 for p in self.net_paths.values() if p)
         return routed, len(all_tasks) - routed
 

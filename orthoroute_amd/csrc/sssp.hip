@@ -159,6 +159,27 @@ __device__ __forceinline__ ChunkWalk chunk_walk(uint32_t n, uint32_t wave)
     return w;
 }
 
+// A wave's queue of items it pushed into the current bucket: expanded later
+// in the same round (a round then covers several hops of a bucket instead of
+// one).  Ring of LQ_CAP entries in LDS; tail is advanced by the lanes with an
+// LDS atomic, head only at wave-uniform points.
+#define LQ_CAP 512
+#define LQ_BATCHES 16          // local batches a wave may run per round
+struct LocalQ {
+    uint32_t *buf;
+    uint32_t *tail;
+    uint32_t head;
+};
+
+// After a batch (wave-uniform): lanes that found the queue full pushed to OUT
+// instead, so the tail is pulled back to the last stored entry.
+__device__ __forceinline__ void lq_clamp(const LocalQ &lq)
+{
+    const uint32_t t = __hip_atomic_load(lq.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    if (t - lq.head > LQ_CAP && __lane_id() == 0)
+        __hip_atomic_store(lq.tail, lq.head + LQ_CAP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
 // Lower bound of the remaining cost from node v to the slot's destination
 // (goal-directed bucketing; 0 when the call runs in plain Dijkstra order).
 __device__ __forceinline__ float heur(const RoundArgs &a, const SlotInfo &si, uint32_t v)
@@ -176,7 +197,8 @@ __device__ __forceinline__ float heur(const RoundArgs &a, const SlotInfo &si, ui
 // current bucket relaxes its out-edges.  A lane walks its own CSR row four
 // edges at a time with the four edges' loads issued together; the slot's
 // search parameters are read once per item.
-__device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bool has, uint64_t cur, uint32_t nb,
+__device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bool has, bool from_bitmap,
+                                           const LocalQ &lq, uint64_t cur, uint32_t nb,
                                            const uint32_t *s_ring, const uint32_t *s_lid, uint32_t s_out, uint32_t s_ovf,
                                            int32_t *s_slot_delta, int32_t *s_buf_delta,
                                            unsigned long long &my_items, unsigned long long &my_edges,
@@ -186,7 +208,7 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bo
     const uint32_t N = (uint32_t)a.N;
     const uint32_t slot = item / N;
     const uint32_t u = item - slot * N;
-    atomicAdd(&s_slot_delta[slot], -1);
+    if (from_bitmap) atomicAdd(&s_slot_delta[slot], -1);
     const SlotInfo si = a.slots[slot];
     if (si.state != 1) return;
     const uint64_t ku = atomic_load_u64(&a.keys[item]);
@@ -249,6 +271,16 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bo
             const uint64_t old = atomicMin((unsigned long long *)&keys[v[q]], (unsigned long long)nk[q]);
             if (nk[q] < old && key_cost(nk[q]) < key_cost(old)) {
                 const uint64_t b = si.base + key_bucket_rel(nk[q], a.inv_delta, heur(a, si, v[q]));
+                if (b <= cur) {
+                    // same bucket: this wave expands it later in the round
+                    // (its local queue), unless the queue is full
+                    const uint32_t pos = atomicAdd(lq.tail, 1u);
+                    if (pos - lq.head < LQ_CAP) {
+                        __hip_atomic_store(&lq.buf[pos & (LQ_CAP - 1)], slot * N + v[q], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        continue;
+                    }
+                }
                 uint32_t buf;
                 if (b <= cur) buf = s_out;
                 else if (b < cur + nb) buf = s_ring[b % nb];
@@ -295,8 +327,11 @@ k_round(RoundArgs a)
     __shared__ unsigned long long s_items, s_edges, s_ovf_min;
     __shared__ uint32_t s_clear[WAVES_PER_WG][64]; // redistribution: bits leaving each word
     __shared__ uint32_t s_ibuf[WAVES_PER_WG][128]; // relaxation: items gathered for the next batch
+    __shared__ uint32_t s_lq[WAVES_PER_WG][LQ_CAP]; // relaxation: this wave's same-bucket items
+    __shared__ uint32_t s_lq_tail[WAVES_PER_WG];
     for (int i = threadIdx.x; i < a.K; i += WG) s_slot_delta[i] = 0;
     if (threadIdx.x < MAX_BUFS) s_buf_delta[threadIdx.x] = 0;
+    if (threadIdx.x < WAVES_PER_WG) s_lq_tail[threadIdx.x] = 0;
     if (threadIdx.x < nb) s_ring[threadIdx.x] = c->ring[threadIdx.x];
     if (threadIdx.x < nb + 3) s_lid[threadIdx.x] = c->list_id[threadIdx.x];
     if (threadIdx.x == 0) {
@@ -339,6 +374,7 @@ k_round(RoundArgs a)
         // dependent loads and atomics serve 64 items at once.
         uint32_t *ibuf = s_ibuf[wave];
         int nbuf = 0;
+        LocalQ lq{s_lq[wave], &s_lq_tail[wave], 0u};
         for (uint32_t li = cw.first;; li += cw.stride) {
             const bool more = li < cw.end;
             if (more) {
@@ -371,18 +407,46 @@ k_round(RoundArgs a)
                         const uint32_t rest = __hip_atomic_load(&ibuf[64 + lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                         __hip_atomic_store(&ibuf[lane], rest, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                         nbuf -= 64;
-                        relax_item(a, it, true, cur, nb, s_ring, s_lid, s_out, s_ovf, s_slot_delta, s_buf_delta,
-                                   my_items, my_edges, my_ovf_min);
+                        relax_item(a, it, true, true, lq, cur, nb, s_ring, s_lid, s_out, s_ovf, s_slot_delta,
+                                   s_buf_delta, my_items, my_edges, my_ovf_min);
+                        lq_clamp(lq);
                     }
                 }
             } else {
                 if (nbuf > 0) {
                     const uint32_t it = __hip_atomic_load(&ibuf[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-                    relax_item(a, it, (int)lane < nbuf, cur, nb, s_ring, s_lid, s_out, s_ovf, s_slot_delta, s_buf_delta,
-                               my_items, my_edges, my_ovf_min);
+                    relax_item(a, it, (int)lane < nbuf, true, lq, cur, nb, s_ring, s_lid, s_out, s_ovf, s_slot_delta,
+                               s_buf_delta, my_items, my_edges, my_ovf_min);
+                    lq_clamp(lq);
                 }
                 break;
             }
+        }
+        // expand the wave's own same-bucket items: several hops of the bucket
+        // in one round; what is left after LQ_BATCHES batches goes to OUT
+        for (int nbat = 0;; ++nbat) {
+            const uint32_t tail = __hip_atomic_load(lq.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            if (tail == lq.head) break;
+            if (nbat == LQ_BATCHES) {
+                for (uint32_t i = lq.head + lane; i < tail; i += 64) {
+                    const uint32_t it = __hip_atomic_load(&lq.buf[i & (LQ_CAP - 1)], __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    if (push_bit(a, s_out, s_lid[s_out], it)) {
+                        atomicAdd(&s_slot_delta[it / N], 1);
+                        atomicAdd(&s_buf_delta[s_out], 1);
+                    }
+                }
+                break;
+            }
+            const uint32_t n = min(64u, tail - lq.head);
+            uint32_t it = 0;
+            if (lane < n)
+                it = __hip_atomic_load(&lq.buf[(lq.head + lane) & (LQ_CAP - 1)], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
+            lq.head += n;
+            relax_item(a, it, lane < n, false, lq, cur, nb, s_ring, s_lid, s_out, s_ovf, s_slot_delta, s_buf_delta,
+                       my_items, my_edges, my_ovf_min);
+            lq_clamp(lq);
         }
     } else {
         for (uint32_t li = cw.first; li < cw.end; li += cw.stride) {
