@@ -15,13 +15,17 @@ resident in HBM before the timed region.
 
 value = SSSP GTEPS of the whole step: Dijkstra-minimal edges (out-edges of the
 nodes a label-setting search must settle, i.e. every node whose final cost is
-at most the destination's) per second of step wall time.  The minimal counts
-come from an untimed counting pass over the same searches.
+at most the destination's) per second of step wall time.  The searches run in
+goal-directed bucket order and scan far fewer edges than that (reported as
+edges_scanned_per_step); the minimal count is the fixed measure of the work a
+Dijkstra search of the same requests must do.  It comes from an untimed
+counting pass in plain Dijkstra order (or from --minimal-counts).
 
 Also reported: plain (one search at a time) vs batched on a sample of the same
-searches; the k_round roofline from HIP-event timing of the round launches;
-the CPU oracle's GTEPS on a bounded sample of the same searches (single core),
-with its paths compared against the GPU's.
+searches; the k_round roofline (bytes of the expanded items and scanned edges
+per launch over the HIP-event time of the round launches); the CPU oracle's
+GTEPS on a bounded sample of the same searches (single core), with its paths
+compared against the GPU's.
 """
 import argparse
 import json
@@ -51,6 +55,11 @@ def parse():
     ap.add_argument("--plain-sample", type=int, default=48, help="searches timed in the plain vs batched comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU oracle sample budget (0 = skip)")
     ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--minimal-counts", default="", metavar="JSON",
+                    help="take the Dijkstra-minimal node/edge counts of the step from this file (written by an "
+                         "earlier run with --save-minimal-counts) instead of an untimed counting pass; with "
+                         "--warmup 0 --plain-sample 0 the timed step's launches are then the only SSSP launches")
+    ap.add_argument("--save-minimal-counts", default="", metavar="JSON")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run 5 plain device copies of BYTES bytes (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -113,21 +122,33 @@ def main():
     def snap():
         return {k: float(v) for k, v in r.stats.items()}
 
-    # untimed counting pass: Dijkstra-minimal nodes/edges of this step's searches
-    cfg.count_minimal = True
-    s0 = snap()
-    t0 = time.perf_counter()
-    routed, failed, over_cnt = step()
-    s1 = snap()
-    cfg.count_minimal = False
-    min_nodes = s1.get("sssp_minimal_nodes", 0) - s0.get("sssp_minimal_nodes", 0)
-    min_edges = s1.get("sssp_minimal_edges", 0) - s0.get("sssp_minimal_edges", 0)
-    log(f"[bench] counting pass {time.perf_counter() - t0:.2f}s routed={routed} failed={failed} "
-        f"overused_edges={over_cnt} minimal nodes={min_nodes:.4g} edges={min_edges:.4g}")
-    for w in range(max(0, args.warmup - 1)):
+    n_warm = args.warmup
+    if args.minimal_counts:
+        with open(args.minimal_counts) as f:
+            mc = json.load(f)[args.workload]
+        min_nodes, min_edges = float(mc["minimal_nodes"]), float(mc["minimal_edges"])
+        log(f"[bench] minimal counts from {args.minimal_counts}: nodes={min_nodes:.4g} edges={min_edges:.4g}")
+    else:
+        # untimed counting pass (plain Dijkstra order): Dijkstra-minimal
+        # nodes/edges of this step's searches; it is the first warmup step
+        cfg.count_minimal = True
+        s0 = snap()
+        t0 = time.perf_counter()
+        routed, failed, over_cnt = step()
+        s1 = snap()
+        cfg.count_minimal = False
+        min_nodes = s1.get("sssp_minimal_nodes", 0) - s0.get("sssp_minimal_nodes", 0)
+        min_edges = s1.get("sssp_minimal_edges", 0) - s0.get("sssp_minimal_edges", 0)
+        log(f"[bench] counting pass {time.perf_counter() - t0:.2f}s routed={routed} failed={failed} "
+            f"overused_edges={over_cnt} minimal nodes={min_nodes:.4g} edges={min_edges:.4g}")
+        n_warm -= 1
+        if args.save_minimal_counts:
+            with open(args.save_minimal_counts, "w") as f:
+                json.dump({args.workload: {"minimal_nodes": int(min_nodes), "minimal_edges": int(min_edges)}}, f)
+    for w in range(max(0, n_warm)):
         t0 = time.perf_counter()
         step()
-        log(f"[bench] warmup {w + 2} {time.perf_counter() - t0:.2f}s")
+        log(f"[bench] warmup step {time.perf_counter() - t0:.2f}s")
 
     torch.cuda.synchronize()
     s0 = snap()
@@ -145,21 +166,28 @@ def main():
     launches = d.get("sssp_launches", 0.0)
     round_ms = d.get("sssp_round_ms", 0.0)
     avg_launch_us = 1e3 * round_ms / max(launches, 1)
-    alg_bytes = (BYTES_PER_NODE * min_nodes + BYTES_PER_EDGE * min_edges) * K
+    # algorithmic bytes of the work the round launches performed (expanded
+    # items and scanned edges, re-expansions included); the Dijkstra-minimal
+    # equivalent is reported beside it
+    alg_bytes = BYTES_PER_NODE * d.get("sssp_items", 0) + BYTES_PER_EDGE * d.get("sssp_edges", 0)
     achieved = alg_bytes / max(round_ms * 1e-3, 1e-12) / 1e9
+    dij_bytes = (BYTES_PER_NODE * min_nodes + BYTES_PER_EDGE * min_edges) * K
+    dij_equiv = dij_bytes / max(round_ms * 1e-3, 1e-12) / 1e9
     log(f"[bench] {K} steps in {wall:.2f}s; rounds kernel {round_ms / 1e3:.2f}s over {launches:.0f} launches "
         f"({avg_launch_us:.1f} us avg); edges scanned {d.get('sssp_edges', 0):.4g}")
 
     # plain (one search at a time) vs batched on the same sample of searches
-    sample = list(tasks.items())[:args.plain_sample]
-    sreqs = []
-    for name, (s, dd) in sample:
-        sx, sy, _ = r._coords(s)
-        dx, dy, _ = r._coords(dd)
-        sreqs.append((s, dd, -1, abs(dx - sx) + abs(dy - sy) < ROI_THRESHOLD_STEPS, 0.0))
-    timing = {}
-    paths = {}
-    for label, slots in (("plain", 1), ("batched", args.slots)):
+    def sample_reqs(n):
+        out = []
+        for name, (s, dd) in list(tasks.items())[:n]:
+            sx, sy, _ = r._coords(s)
+            dx, dy, _ = r._coords(dd)
+            out.append((s, dd, -1, abs(dx - sx) + abs(dy - sy) < ROI_THRESHOLD_STEPS, 0.0))
+        return out
+    sreqs = sample_reqs(args.plain_sample)
+    timing = {"plain": 0.0, "batched": 0.0}
+    paths = {"plain": [], "batched": []}
+    for label, slots in ((("plain", 1), ("batched", args.slots)) if sreqs else ()):
         cfg.batch_slots = slots
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -173,7 +201,9 @@ def main():
 
     cpu = None
     if args.cpu_seconds > 0:
-        cpu = cpu_baseline(r, sreqs, paths["batched"], args.cpu_seconds, X, Y, L)
+        creqs = sreqs or sample_reqs(48)
+        cpu = cpu_baseline(r, creqs, paths["batched"] if sreqs else [None] * len(creqs), args.cpu_seconds, X, Y, L,
+                           compare=bool(sreqs))
 
     traffic = None
     prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_round_cfg2_traffic.json")
@@ -215,13 +245,14 @@ def main():
         "roofline": {"kernel": "k_round", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "avg_launch_us": round(avg_launch_us, 2), "launches": int(launches),
-                     "alg_bytes_per_launch": round(alg_bytes / max(launches, 1), 1)},
+                     "alg_bytes_per_launch": round(alg_bytes / max(launches, 1), 1),
+                     "dijkstra_minimal_equiv_GBs": round(dij_equiv, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(r, sreqs, gpu_paths, budget_s, X, Y, L):
+def cpu_baseline(r, sreqs, gpu_paths, budget_s, X, Y, L, compare=True):
     """The oracle's label-setting search (C, one thread) on the same searches
     and the same cost snapshot, for about budget_s seconds."""
     import numpy as np
@@ -256,7 +287,7 @@ def cpu_baseline(r, sreqs, gpu_paths, budget_s, X, Y, L):
             "sample": f"the first {done} nets of the request list on the step's cost snapshot: oracle ROI BFS "
                       f"(nets under 125 steps) + label-setting search in C on one thread, {t_search:.1f}s",
             "searches_per_s": round(done / max(t_search, 1e-9), 3),
-            "paths_equal_to_gpu": f"{equal}/{done}"}
+            "paths_equal_to_gpu": f"{equal}/{done}" if compare else "not compared"}
 
 
 if __name__ == "__main__":
