@@ -51,10 +51,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--slots", type=int, default=256, help="concurrent searches of the batched form")
+    ap.add_argument("--slots", type=int, default=512, help="concurrent searches of the batched form")
     ap.add_argument("--plain-sample", type=int, default=48, help="searches timed in the plain vs batched comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU oracle sample budget (0 = skip)")
     ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--delta", type=float, default=0.0, help="bucket width (0 = the router's choice)")
+    ap.add_argument("--buckets", type=int, default=0, help="ring buckets (0 = the router's choice)")
+    ap.add_argument("--local-batches", type=int, default=0, help="same-bucket batches per wave per round")
+    ap.add_argument("--grid", type=int, default=0, help="workgroups per round launch")
     ap.add_argument("--minimal-counts", default="", metavar="JSON",
                     help="take the Dijkstra-minimal node/edge counts of the step from this file (written by an "
                          "earlier run with --save-minimal-counts) instead of an untimed counting pass; with "
@@ -101,6 +105,12 @@ def main():
     cfg = PathFinderConfig()
     cfg.lattice_policy = "adjacent"
     cfg.batch_slots = args.slots
+    if args.delta > 0:
+        cfg.sssp_delta = args.delta
+    if args.buckets > 0:
+        cfg.sssp_buckets = args.buckets
+    cfg.sssp_local_batches = args.local_batches
+    cfg.sssp_grid = args.grid
     r = PathFinderRouter(cfg)
     t0 = time.perf_counter()
     r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
@@ -230,6 +240,7 @@ def main():
                         f"pass on the iteration-1 cost snapshot",
             "nodes": r.N, "edges": r.accounting.E, "nets": len(tasks), "slots": args.slots,
             "delta": r._search_params(args.slots).delta, "n_buckets": r._search_params(args.slots).n_buckets,
+            "local_batches": args.local_batches or 64, "grid": args.grid or 1024,
         },
         "nets_per_s": round(len(tasks) * K / wall, 1),
         "routed": routed, "failed": failed,

@@ -115,6 +115,10 @@ typedef struct orgpu_sssp_params {
      * is still expanded before the destination is declared settled, so paths
      * and tie-breaks are those of the plain search. */
     float h_xy, h_z;
+    /* Same-bucket items a wave pushes are expanded by that wave within the
+     * round, up to local_batches batches of 64 (0 = 64); the rest wait for
+     * the next round. */
+    int32_t local_batches;
 } orgpu_sssp_params;
 
 typedef struct orgpu_sssp_stats {

@@ -119,9 +119,11 @@ class PathFinderConfig:
     allowed_via_spans = None
     # execution knobs of this build
     lattice_policy: str = "reference"
-    batch_slots: int = 256
+    batch_slots: int = 512
     sssp_delta = None
     sssp_buckets = None
+    sssp_local_batches = 0         # same-bucket batches per wave per round (0 = library default)
+    sssp_grid = 0                  # workgroups per round launch (0 = library default)
     max_iterations_cap = None
     count_minimal = False          # also count Dijkstra-minimal nodes/edges per search
     goal_directed = True           # bucket on g + lower bound to the destination (same paths)
@@ -880,7 +882,8 @@ class PathFinderRouter:
         if self.config.goal_directed and not self.config.count_minimal:
             hx, hz = getattr(self, "_hbounds", (0.0, 0.0))
         return SearchParams(slots=slots, delta=float(d), n_buckets=int(nb), rounds_per_poll=32,
-                            count_minimal=bool(self.config.count_minimal), h_xy=hx, h_z=hz)
+                            grid=int(self.config.sssp_grid or 0), count_minimal=bool(self.config.count_minimal),
+                            h_xy=hx, h_z=hz, local_batches=int(self.config.sssp_local_batches or 0))
 
     def _search_batch(self, reqs, keepout_owner=None, readd=True):
         """reqs: list of (src, dst, net_int, roi, stagnation_bonus).  roi

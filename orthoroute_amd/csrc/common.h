@@ -103,6 +103,7 @@ struct RoundArgs {
     uint32_t nb;
     float inv_delta;
     float h_xy, h_z;          // goal-directed lower bounds (0, 0: Dijkstra order)
+    int32_t local_batches;    // same-bucket batches a wave expands per round
     int32_t X, plane;         // node = z*plane + y*X + x
     const int32_t *indptr;
     const int32_t *indices;
@@ -116,6 +117,10 @@ struct RoundArgs {
     uint64_t bitmap_words;
     uint32_t *summ;           // [MAX_BUFS][summ_words]: one bit per 64-word chunk
     uint64_t summ_words;
+    uint32_t *touch_bits;     // [K][touch_words]: key chunks (2048 keys) a search set
+    uint32_t *touch_list;     // [K][touch_cap]: the same chunks as a list
+    uint32_t *touch_n;        // [K]
+    int64_t touch_words, touch_cap;
     uint32_t *lists;          // [MAX_BUFS + 1][LIST_BINS][bin_cap] chunk lists
     uint32_t *list_cnt;       // [MAX_BUFS + 1][LIST_BINS] entries per bin
     uint64_t chunks;          // chunks per bitmap (64 words each)

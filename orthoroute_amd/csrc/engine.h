@@ -17,6 +17,8 @@ struct Engine {
     uint32_t *bits = nullptr;
     uint32_t *summ = nullptr;
     uint32_t *lists = nullptr, *list_cnt = nullptr;
+    uint32_t *touch_bits = nullptr, *touch_list = nullptr, *touch_n = nullptr;
+    int64_t keys_n = -1;   // node count the keys' slot layout was last reset for
     RoundPart *parts = nullptr;
     uint64_t summ_words = 0, chunks = 0;
     Ctrl *ctrl = nullptr;

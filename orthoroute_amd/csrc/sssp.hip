@@ -134,6 +134,19 @@ __device__ __forceinline__ bool push_bit(const RoundArgs &a, uint32_t buf, uint3
     return true;
 }
 
+// A key of (slot, node) left KEY_INF: records the node's 2048-key chunk so
+// that reusing the slot resets only the chunks its search touched.
+__device__ __forceinline__ void touch_key(const RoundArgs &a, uint32_t slot, uint32_t node)
+{
+    const uint32_t ch = node >> 11;
+    const uint32_t m = 1u << (ch & 31);
+    uint32_t *w = a.touch_bits + (uint64_t)slot * a.touch_words + (ch >> 5);
+    if (*w & m) return;
+    if (atomicOr(w, m) & m) return;
+    const uint32_t pos = atomicAdd(&a.touch_n[slot], 1u);
+    a.touch_list[(uint64_t)slot * a.touch_cap + pos] = ch;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs.  Each XCD takes one
 // contiguous eighth of the round's chunk list (chunks are in item order, so
 // a slot's keys and the lattice rows its frontier touches stay in that XCD's
@@ -163,8 +176,7 @@ __device__ __forceinline__ ChunkWalk chunk_walk(uint32_t n, uint32_t wave)
 // in the same round (a round then covers several hops of a bucket instead of
 // one).  Ring of LQ_CAP entries in LDS; tail is advanced by the lanes with an
 // LDS atomic, head only at wave-uniform points.
-#define LQ_CAP 512
-#define LQ_BATCHES 16          // local batches a wave may run per round
+#define LQ_CAP 1024
 struct LocalQ {
     uint32_t *buf;
     uint32_t *tail;
@@ -269,6 +281,7 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bo
         for (int q = 0; q < 4; ++q) {
             if (!ok[q] || !(nk[q] < kv[q])) continue;
             const uint64_t old = atomicMin((unsigned long long *)&keys[v[q]], (unsigned long long)nk[q]);
+            if (old == KEY_INF) touch_key(a, slot, v[q]);
             if (nk[q] < old && key_cost(nk[q]) < key_cost(old)) {
                 const uint64_t b = si.base + key_bucket_rel(nk[q], a.inv_delta, heur(a, si, v[q]));
                 if (b <= cur) {
@@ -423,11 +436,11 @@ k_round(RoundArgs a)
             }
         }
         // expand the wave's own same-bucket items: several hops of the bucket
-        // in one round; what is left after LQ_BATCHES batches goes to OUT
+        // in one round; what is left after local_batches batches goes to OUT
         for (int nbat = 0;; ++nbat) {
             const uint32_t tail = __hip_atomic_load(lq.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
             if (tail == lq.head) break;
-            if (nbat == LQ_BATCHES) {
+            if (nbat == a.local_batches) {
                 for (uint32_t i = lq.head + lane; i < tail; i += 64) {
                     const uint32_t it = __hip_atomic_load(&lq.buf[i & (LQ_CAP - 1)], __ATOMIC_RELAXED,
                                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
@@ -699,6 +712,25 @@ struct SeedReq {
     int32_t slot, src, dst, net_id, roi_index, use_keepout, req, pad;
 };
 
+// Resets the keys of the slots about to be seeded: only the 2048-key chunks
+// their previous searches touched (each workgroup fills whole chunks).  The
+// touch records themselves are cleared by k_seed, which runs after this.
+__global__ void k_reset_keys(RoundArgs a, const SeedReq *reqs, int n)
+{
+    const uint64_t N = (uint64_t)a.N;
+    for (int r = 0; r < n; ++r) {
+        const uint32_t s = (uint32_t)reqs[r].slot;
+        const uint32_t cnt = a.touch_n[s];
+        const uint32_t *lst = a.touch_list + (uint64_t)s * a.touch_cap;
+        uint64_t *ks = a.keys + (uint64_t)s * N;
+        for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
+            const uint64_t lo = (uint64_t)lst[i] * 2048u;
+            const uint64_t hi = lo + 2048u < N ? lo + 2048u : N;
+            for (uint64_t v = lo + threadIdx.x; v < hi; v += blockDim.x) ks[v] = KEY_INF;
+        }
+    }
+}
+
 // Marks finished slots free (their queued items, if any, keep draining).
 __global__ void k_release(SlotInfo *slots, const int32_t *list, int n)
 {
@@ -733,6 +765,10 @@ __global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n)
         // a slot is reseeded only once its previous search has no set bits
         // left, so the source's bit is clear here
         si->pending = 1;
+        // the slot's keys were reset (k_reset_keys): start a new touch record
+        for (int64_t w = 0; w < a.touch_words; ++w) a.touch_bits[(uint64_t)r.slot * a.touch_words + w] = 0u;
+        a.touch_n[r.slot] = 0u;
+        touch_key(a, (uint32_t)r.slot, (uint32_t)r.src);
         if (push_bit(a, buf, c->list_id[buf], item)) atomicAdd(&c->count[buf], 1u);
         atomicAdd(&c->active, 1u);
     }
@@ -834,6 +870,12 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
     const uint64_t bin_cap = (chunks + LIST_BINS - 1) / LIST_BINS + 1;
     HIP_CHECK_RET(hipMalloc((void **)&en.lists, (uint64_t)(nbufs + 1) * LIST_BINS * bin_cap * sizeof(uint32_t)));
     HIP_CHECK_RET(hipMalloc((void **)&en.list_cnt, (uint64_t)(nbufs + 1) * LIST_BINS * sizeof(uint32_t)));
+    {
+        const uint64_t tcap = ((uint64_t)n_nodes + 2047) / 2048, twords = (tcap + 31) / 32;
+        HIP_CHECK_RET(hipMalloc((void **)&en.touch_bits, (uint64_t)slots * twords * sizeof(uint32_t)));
+        HIP_CHECK_RET(hipMalloc((void **)&en.touch_list, (uint64_t)slots * tcap * sizeof(uint32_t)));
+        HIP_CHECK_RET(hipMalloc((void **)&en.touch_n, (uint64_t)slots * sizeof(uint32_t)));
+    }
     HIP_CHECK_RET(hipMalloc((void **)&en.parts, (size_t)MAX_GRID * sizeof(RoundPart)));
     HIP_CHECK_RET(hipMalloc((void **)&en.ctrl, sizeof(Ctrl)));
     HIP_CHECK_RET(hipMalloc((void **)&en.slots, sizeof(SlotInfo) * slots));
@@ -871,6 +913,9 @@ void engine_release(Engine &en)
     if (en.summ) hipFree(en.summ);
     if (en.lists) hipFree(en.lists);
     if (en.list_cnt) hipFree(en.list_cnt);
+    if (en.touch_bits) hipFree(en.touch_bits);
+    if (en.touch_list) hipFree(en.touch_list);
+    if (en.touch_n) hipFree(en.touch_n);
     if (en.parts) hipFree(en.parts);
     if (en.ctrl) hipFree(en.ctrl);
     if (en.slots) hipFree(en.slots);
@@ -923,6 +968,8 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
                           "(N=%lld, K=%d, nb=%d)", (long long)N, K, p->n_buckets, (long long)en.cap_nodes,
                           en.cap_slots, en.cap_nb);
     if (!(p->delta > 0.0f)) return orgpu_fail(ORGPU_EINVAL, "route: delta must be > 0");
+    if (p->local_batches < 0 || p->local_batches > 4096)
+        return orgpu_fail(ORGPU_EINVAL, "route: local_batches %d outside [0, 4096]", p->local_batches);
     if (!(p->h_xy >= 0.0f && p->h_z >= 0.0f && p->h_xy < 3.0e38f && p->h_z < 3.0e38f))
         return orgpu_fail(ORGPU_EINVAL, "route: h_xy %g / h_z %g must be finite and >= 0", (double)p->h_xy, (double)p->h_z);
     if ((p->h_xy > 0.0f || p->h_z > 0.0f) && (g->X <= 0 || g->Y <= 0 || g->L <= 0 ||
@@ -946,6 +993,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     a.ctrl = en.ctrl; a.slots = en.slots; a.K = K; a.N = N; a.nb = (uint32_t)nb;
     a.inv_delta = 1.0f / p->delta;
     a.h_xy = p->h_xy; a.h_z = p->h_z;
+    a.local_batches = p->local_batches > 0 ? p->local_batches : 64;
     a.X = g->X; a.plane = g->X * g->Y;
     a.indptr = g->indptr; a.indices = g->indices; a.cost = g->cost; a.kind = g->edge_kind;
     a.roi = roi_bitmaps; a.roi_words = (N + 31) / 32;
@@ -953,6 +1001,9 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     a.keys = en.keys;
     a.bits = en.bits; a.bitmap_words = (uint64_t)((N * (int64_t)K + 31) / 32);
     a.summ = en.summ; a.summ_words = ((a.bitmap_words + 63) / 64 + 31) / 32;
+    a.touch_cap = (N + 2047) / 2048;
+    a.touch_words = (a.touch_cap + 31) / 32;
+    a.touch_bits = en.touch_bits; a.touch_list = en.touch_list; a.touch_n = en.touch_n;
     a.lists = en.lists;
     a.list_cnt = en.list_cnt;
     a.chunks = (a.bitmap_words + 63) / 64;
@@ -977,6 +1028,14 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     HIP_CHECK_RET(hipMemsetAsync(en.bits, 0, (size_t)(nb + 3) * a.bitmap_words * sizeof(uint32_t), s));
     HIP_CHECK_RET(hipMemsetAsync(en.summ, 0, (size_t)(nb + 3) * a.summ_words * sizeof(uint32_t), s));
     HIP_CHECK_RET(hipMemsetAsync(en.list_cnt, 0, (size_t)(nb + 4) * LIST_BINS * sizeof(uint32_t), s));
+    // Keys are reset per slot from the touch records of its last search; those
+    // records describe the layout slot*N of the node count they were made for.
+    if (en.keys_n != N) {
+        hipLaunchKernelGGL(k_fill_keys, dim3(4096), dim3(256), 0, s, en.keys, (int64_t)en.cap_slots * N);
+        HIP_CHECK_RET(hipGetLastError());
+        HIP_CHECK_RET(hipMemsetAsync(en.touch_n, 0, sizeof(uint32_t) * en.cap_slots, s));
+        en.keys_n = N;
+    }
 
     int grid = p->grid > 0 ? p->grid : 1024;
     grid = std::min((grid + 7) / 8 * 8, MAX_GRID);
@@ -1003,11 +1062,8 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
             ++next; ++ns;
         }
         if (ns == 0) return 0;
-        for (int i = 0; i < ns; ++i) {
-            const int k = en.h_seed[i].slot;
-            hipLaunchKernelGGL(k_fill_keys, dim3(1024), dim3(256), 0, s, en.keys + (uint64_t)k * (uint64_t)N, N);
-        }
         HIP_CHECK_RET(hipMemcpyAsync(en.seed, en.h_seed, sizeof(SeedReq) * ns, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_reset_keys, dim3(512), dim3(256), 0, s, a, en.seed, ns);
         hipLaunchKernelGGL(k_seed, dim3((ns + 255) / 256), dim3(256), 0, s, a, en.seed, ns);
         HIP_CHECK_RET(hipGetLastError());
         last_seeded = ns;
