@@ -781,16 +781,34 @@ class PathFinderRouter:
         return float(np.cumsum(vals.astype(np.float64))[-1])
 
     def _order_nets_by_difficulty(self, tasks, over_dense):
-        scores = []
-        for nid, (s, d) in tasks.items():
-            sx, sy, sz = self._coords(s)
-            dx, dy, dz = self._coords(d)
-            dist = abs(dx - sx) + abs(dy - sy) + abs(dz - sz)
-            cong = 0.0
-            if nid in self._net_to_edges:
-                edges = self._net_to_edges[nid]
-                cong = self._seqsum(over_dense[edges]) / max(1, len(edges))
-            scores.append((dist * 3 * (cong + 1), nid))
+        """UPF:4401-4439: score = manhattan*3*(1 + mean overuse on the net's
+        previous path), sorted descending with the net name as tie-break.
+        Vectorised; the congestion term is the sequential float64 sum the
+        reference's sum() takes, and it is only evaluated for nets whose
+        previous path crosses an overused edge (it is 0.0 otherwise)."""
+        names = list(tasks)
+        if not names:
+            return []
+        sd = np.asarray([tasks[n] for n in names], dtype=np.int64)
+        X, plane = self._Nx, self._Nx * self._Ny
+        zs, rs = np.divmod(sd[:, 0], plane)
+        zd, rd = np.divmod(sd[:, 1], plane)
+        dist = np.abs(rd % X - rs % X) + np.abs(rd // X - rs // X) + np.abs(zd - zs)
+        cong = np.zeros(len(names), dtype=np.float64)
+        with_edges = [i for i, n in enumerate(names) if n in self._net_to_edges]
+        if with_edges and over_dense is not None and len(over_dense):
+            segs = [self._net_to_edges[names[i]] for i in with_edges]
+            lens = np.fromiter((len(e) for e in segs), dtype=np.int64, count=len(segs))
+            flat = np.concatenate(segs) if segs else np.zeros(0, np.int64)
+            hit = (over_dense[flat] != 0).astype(np.int64)
+            starts = np.concatenate([[0], np.cumsum(lens)[:-1]])
+            nz = np.add.reduceat(hit, starts) if len(flat) else np.zeros(len(segs), np.int64)
+            nz[lens == 0] = 0
+            for k in np.nonzero(nz)[0]:
+                e = segs[k]
+                cong[with_edges[k]] = self._seqsum(over_dense[e]) / max(1, len(e))
+        score = (dist * 3).astype(np.float64) * (cong + 1.0)
+        scores = list(zip(score.tolist(), names))
         scores.sort(reverse=True)
         rot = self._rng.randint(0, min(5, len(scores) // 10))
         ordered = [n for _, n in scores]
