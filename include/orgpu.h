@@ -265,9 +265,12 @@ int orgpu_overuse_sparse(orgpu_t *h, int64_t n_edges, const float *present, cons
 /* Lower bounds for the goal-directed search (orgpu_sssp_params.h_xy/h_z):
  * out[0] = min cost over planar edges, out[1] = min over via edges of
  * cost / layer span, both over finite non-negative costs (device f32[2],
- * caller-owned; 0x7F7F7F7F when a kind has no finite edge).  No reference
- * counterpart: the reference's Dijkstra is not goal-directed. */
-int orgpu_cost_bounds(orgpu_t *h, const orgpu_graph *g, const uint8_t *edge_layer, float *out);
+ * caller-owned; 0x7F7F7F7F when a kind has no finite edge).  hist (device
+ * u32[64] or null): planar finite costs counted by binary exponent, bin =
+ * exponent + 32 clamped to [0, 63] -- the scale from which the caller picks
+ * the bucket width.  No reference counterpart: the reference's Dijkstra is
+ * neither goal-directed nor bucketed. */
+int orgpu_cost_bounds(orgpu_t *h, const orgpu_graph *g, const uint8_t *edge_layer, float *out, uint32_t *hist);
 
 #ifdef __cplusplus
 }

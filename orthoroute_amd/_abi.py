@@ -125,7 +125,7 @@ def load():
         "orgpu_present_build": [P, i64, P, P, P, i64, f32],
         "orgpu_present_ema": [P, i64, P, P, ctypes.c_int, f32, f32, ctypes.POINTER(SparseOut)],
         "orgpu_cost_update": [P, ctypes.POINTER(CostParams), i64, i64, P, P, P, P, P, P, P],
-        "orgpu_cost_bounds": [P, ctypes.POINTER(Graph), P, P],
+        "orgpu_cost_bounds": [P, ctypes.POINTER(Graph), P, P, P],
         "orgpu_history_update": [P, i64, P, P, P, f32, f32, f32, ctypes.c_int],
         "orgpu_overuse_sparse": [P, i64, P, P, ctypes.c_int, f32, f32, ctypes.POINTER(SparseOut)],
     }

@@ -632,8 +632,11 @@ class PathFinderRouter:
                 p.col_cap, p.seg_cap = 4, 2
         self.accounting.state.update_costs(p)
         self.ctx.sync()
-        # the iteration's cost lower bounds for the goal-directed search
-        self._hbounds = cost_bounds(self.ctx, self.graph.dlat, self.accounting.total_cost)
+        # the iteration's cost lower bounds for the goal-directed search, and
+        # its typical planar cost (bucket width scale)
+        hx, hz, med = cost_bounds(self.ctx, self.graph.dlat, self.accounting.total_cost, with_scale=True)
+        self._hbounds = (hx, hz)
+        self._cost_scale = med
 
     # ---------------- committed-net arrays
     def _flat(self):
@@ -891,7 +894,7 @@ class PathFinderRouter:
         nb = self.config.sssp_buckets
         if d is None or nb is None:
             from ...autotune import choose_params
-            d2, nb2 = choose_params(self, slots)
+            d2, nb2 = choose_params(self, slots, getattr(self, "_cost_scale", 0.0))
             d = d if d is not None else d2
             nb = nb if nb is not None else nb2
         # goal-directed order (same paths) unless the Dijkstra-minimal work is

@@ -30,14 +30,20 @@ def _load():
         pass
 
 
-def choose_params(router, slots):
+def choose_params(router, slots, cost_scale=0.0):
     """(delta, n_buckets) for this lattice; a measured entry when one exists,
-    otherwise a width of a few planar steps of the iteration's base cost."""
+    otherwise a width of a few typical planar steps.  cost_scale: the
+    iteration's median planar cost rounded up to a power of two (0 = unknown,
+    then the base cost of a step); congestion raises it in later iterations,
+    and the width follows so that a search keeps a similar number of
+    buckets."""
     _load()
     k = _key(router, slots)
     if k in _CACHE:
         d, nb = _CACHE[k]
         return float(d), int(nb)
+    if cost_scale > 0.0:
+        return max(2.4 * float(cost_scale), 1e-3), 8
     step = float(router.config.base_cost_weight) * float(router.lattice.pitch)
     return max(step * 4.0, 1e-3), 8
 

@@ -346,14 +346,22 @@ extern "C" int orgpu_overuse_sparse(orgpu_t *h, int64_t n_edges, const float *pr
 // Infinite (blocked) and NaN costs are skipped.
 // ---------------------------------------------------------------------------
 __global__ void k_cost_bounds(int64_t n_edges, const int32_t *indices, const float *cost, const uint8_t *edge_kind,
-                              const uint8_t *edge_layer, int32_t plane, uint32_t *out_bits)
+                              const uint8_t *edge_layer, int32_t plane, uint32_t *out_bits, uint32_t *hist)
 {
+    __shared__ uint32_t s_hist[64];
+    if (threadIdx.x < 64) s_hist[threadIdx.x] = 0;
+    __syncthreads();
     float m_xy = __builtin_inff(), m_z = __builtin_inff();
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_edges; e += (int64_t)gridDim.x * blockDim.x) {
         const float c = cost[e];
         if (!(c >= 0.0f) || !(c < __builtin_inff())) continue;
         if (edge_kind[e] == 0) {
             m_xy = fminf(m_xy, c);
+            if (hist) {
+                // bin = binary exponent + 32, clamped to [0, 63] (0 also takes zeros)
+                const int ex = c > 0.0f ? (int)((__float_as_uint(c) >> 23) & 0xFF) - 127 + 32 : 0;
+                atomicAdd(&s_hist[ex < 0 ? 0 : (ex > 63 ? 63 : ex)], 1u);
+            }
         } else {
             const int span = abs(indices[e] / plane - (int)edge_layer[e]);
             if (span > 0) m_z = fminf(m_z, c / (float)span);
@@ -367,9 +375,12 @@ __global__ void k_cost_bounds(int64_t n_edges, const int32_t *indices, const flo
         atomicMin(&out_bits[0], __float_as_uint(m_xy));
         atomicMin(&out_bits[1], __float_as_uint(m_z));
     }
+    __syncthreads();
+    if (hist && threadIdx.x < 64 && s_hist[threadIdx.x]) atomicAdd(&hist[threadIdx.x], s_hist[threadIdx.x]);
 }
 
-extern "C" int orgpu_cost_bounds(orgpu_t *h, const orgpu_graph *g, const uint8_t *edge_layer, float *out)
+extern "C" int orgpu_cost_bounds(orgpu_t *h, const orgpu_graph *g, const uint8_t *edge_layer, float *out,
+                                 uint32_t *hist)
 {
     if (!h || !g || !g->indices || !g->cost || !g->edge_kind || !edge_layer || !out || g->n_edges < 0)
         return orgpu_fail(ORGPU_EINVAL, "cost_bounds: bad argument");
@@ -377,8 +388,9 @@ extern "C" int orgpu_cost_bounds(orgpu_t *h, const orgpu_graph *g, const uint8_t
     HIP_CHECK_RET(hipSetDevice(h->device));
     hipStream_t s = h->stream;
     HIP_CHECK_RET(hipMemsetAsync(out, 0x7F, 2 * sizeof(float), s));   // 0x7F7F7F7F: a large finite float
+    if (hist) HIP_CHECK_RET(hipMemsetAsync(hist, 0, 64 * sizeof(uint32_t), s));
     hipLaunchKernelGGL(k_cost_bounds, dim3(grid_for(std::max<int64_t>(g->n_edges, 1))), dim3(256), 0, s, g->n_edges,
-                       g->indices, g->cost, g->edge_kind, edge_layer, g->X * g->Y, (uint32_t *)out);
+                       g->indices, g->cost, g->edge_kind, edge_layer, g->X * g->Y, (uint32_t *)out, hist);
     HIP_CHECK_RET(hipGetLastError());
     return ORGPU_OK;
 }

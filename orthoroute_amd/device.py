@@ -144,20 +144,30 @@ class SearchParams:
         return p
 
 
-def cost_bounds(ctx, lat, cost):
+def cost_bounds(ctx, lat, cost, with_scale=False):
     """Lower bounds (h_xy, h_z) of the costs for the goal-directed search:
     the cheapest planar edge and the cheapest via edge per layer of span,
     shrunk by 2^-10 so that FP32 rounding of g + h can never exceed a true
-    remaining cost; 0 where no finite edge of the kind exists."""
+    remaining cost; 0 where no finite edge of the kind exists.  with_scale:
+    also return the median planar cost to within a factor of 2 (the upper
+    edge of the binary-exponent bin that holds the median)."""
     torch = ctx.torch
     out = torch.empty(2, dtype=torch.float32, device=ctx.device)
+    hist = torch.empty(64, dtype=torch.int32, device=ctx.device) if with_scale else None
     g = lat.graph(cost)
-    _abi.check(ctx.lib.orgpu_cost_bounds(ctx.handle, ctypes.byref(g), _ptr(lat.edge_layer), _ptr(out)))
+    _abi.check(ctx.lib.orgpu_cost_bounds(ctx.handle, ctypes.byref(g), _ptr(lat.edge_layer), _ptr(out), _ptr(hist)))
     b = out.cpu().numpy().astype(np.float64)
     res = []
     for v in b:
         res.append(0.0 if not np.isfinite(v) or v > 1e30 or v <= 0.0 else float(np.float32(v * (1.0 - 2.0 ** -10))))
-    return res[0], res[1]
+    if not with_scale:
+        return res[0], res[1]
+    h = hist.cpu().numpy().astype(np.int64)
+    med = 0.0
+    if h.sum() > 0:
+        k = int(np.searchsorted(np.cumsum(h), (h.sum() + 1) // 2))
+        med = float(2.0 ** (k - 32 + 1))
+    return res[0], res[1], med
 
 
 def route(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=None, path_capacity=None):

@@ -46,6 +46,14 @@ def pack_bits(mask):
 
 
 def make_costs(o, rng, kind="jitter"):
+    if kind == "congested":   # late-iteration-like: heavy-tailed present/history terms
+        c = (o.costs * np.float32(0.3)).astype(np.float32)
+        c += (np.arange(o.E, dtype=np.float32) % 9973) * np.float32(1e-6)
+        hot = rng.random(o.E) < 0.2
+        c[hot] += (rng.pareto(1.2, hot.sum()) * 20.0).astype(np.float32)
+        blk = (o.edge_kind == 1) & (rng.random(o.E) < 0.05)
+        c[blk] = np.inf
+        return c
     if kind == "jitter":   # reference-like: 0.3*base + (e % 9973)*1e-6 + congestion
         c = (o.costs * np.float32(0.3)).astype(np.float32)
         c += (np.arange(o.E, dtype=np.float32) % 9973) * np.float32(1e-6)
@@ -148,6 +156,13 @@ def test_cost_bounds(ctx):
         exp_z = (c[vs] / span[vs].astype(np.float32)).min()
         shrink = lambda v: float(np.float32(np.float64(v) * (1.0 - 2.0 ** -10)))
         assert hx == shrink(exp_xy) and hz == shrink(exp_z), (policy, hx, hz, exp_xy, exp_z)
+        # planar cost scale: the binary-exponent bin holding the median
+        _, _, med = cost_bounds(ctx, g, torch.from_numpy(c).to(ctx.device), with_scale=True)
+        pl = c[(o.edge_kind == 0) & fin].astype(np.float64)
+        ex = np.where(pl > 0, np.frexp(pl)[1] - 1 + 32, 0).clip(0, 63)
+        hist = np.bincount(ex, minlength=64)
+        k = int(np.searchsorted(np.cumsum(hist), (hist.sum() + 1) // 2))
+        assert med == 2.0 ** (k - 31), (policy, med, k)
 
 
 def test_route_unreachable_and_refill(ctx):
@@ -165,3 +180,12 @@ def test_route_overflow_heavy(ctx, nb):
     for seed, kind in [(21, "random"), (22, "jitter")]:
         run_case(ctx, 24, 20, 6, "adjacent", 4, 16, seed, kind, roi_frac=0.0, dn=(0.05, nb))
         run_case(ctx, 24, 20, 6, "adjacent", 4, 16, seed, kind, roi_frac=0.0, dn=(0.05, nb), goal=True)
+
+
+@pytest.mark.parametrize("goal", [False, True], ids=["dijkstra", "goal"])
+@pytest.mark.parametrize("delta", [0.3, 6.0, 60.0, 600.0])
+def test_route_congested_costs_any_delta(ctx, delta, goal):
+    # paths must not depend on the bucket width, also when costs span four
+    # orders of magnitude (late negotiation iterations)
+    for policy, seed in [("adjacent", 41), ("reference", 42)]:
+        run_case(ctx, 30, 26, 6, policy, 16, 40, seed, "congested", roi_frac=0.3, dn=(delta, 8), goal=goal)
