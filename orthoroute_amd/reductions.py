@@ -43,11 +43,18 @@ def _pw_dense(a):
 
 
 def sum_f32(n, idx, val):
-    """np.sum of a length-n float32 array that is zero except a[idx] = val."""
+    """np.sum of a length-n float32 array that is zero except a[idx] = val
+    (idx without repeats)."""
     idx = np.asarray(idx, dtype=np.int64)
     val = np.asarray(val, dtype=np.float32)
     if len(idx) == 0:
         return _f32(0.0)
+    if len(idx) > 2048:
+        # many nonzeros: materialising the dense array and letting numpy sum
+        # it is the reference's own operation and cheaper than the block walk
+        a = np.zeros(n, dtype=np.float32)
+        a[idx] = val
+        return np.sum(a)
     order = np.argsort(idx, kind="stable")
     idx, val = idx[order], val[order]
     total = _f32(0.0)
