@@ -157,6 +157,41 @@ int orgpu_route(orgpu_t *h, const orgpu_graph *g, const orgpu_net_req *nets, int
                 int64_t path_capacity, int64_t *path_offset, float *path_dist,
                 orgpu_sssp_stats *stats);
 
+/* Multi-source / multi-sink search: SimpleDijkstra.find_path_multisource_
+ * multisink (UPF:1889-1989) and the GPU seam find_path_fullgraph_gpu_seeds
+ * (UPF:4558-4565).  Each request seeds seeds[seed_off .. +seed_n) at their
+ * entry costs and ends on the cheapest of targets[target_off .. +target_n)
+ * by (distance + target cost); targets are settled but never expanded, and
+ * the search runs until its frontier is empty (the reference does not stop
+ * at the first target).  Seeds and targets outside the request's ROI row
+ * are ignored.  Among equal totals the lower distance, then the lower node
+ * index, then the earlier entry wins. */
+typedef struct orgpu_ms_point {
+    int32_t node;
+    float cost;             /* finite, >= 0 */
+} orgpu_ms_point;
+
+typedef struct orgpu_ms_req {
+    int32_t seed_off, seed_n;
+    int32_t target_off, target_n;
+    int32_t net_id;
+    int32_t roi_index;      /* < 0: whole graph */
+    int32_t use_keepout;
+    int32_t reserved;
+} orgpu_ms_req;
+
+/* As orgpu_route, for multi-source / multi-sink requests.  Paths run from a
+ * seed to the chosen target; path_total[i] (host f64[n], optional) receives
+ * dist + target cost, path_target[i] (host i32[n], optional) the chosen
+ * target's index within the request's target list (-1: none reached).  A
+ * chosen target that is itself a seed gives a path of one node. */
+int orgpu_route_ms(orgpu_t *h, const orgpu_graph *g, const orgpu_ms_req *reqs, int32_t n_reqs,
+                   const orgpu_ms_point *seeds, int64_t n_seeds, const orgpu_ms_point *targets,
+                   int64_t n_targets, const uint32_t *roi_bitmaps, int32_t n_roi,
+                   const int32_t *keepout_owner, float keepout_cost, const orgpu_sssp_params *params,
+                   int32_t *path_nodes, int32_t *path_edges, int64_t path_capacity, int64_t *path_offset,
+                   double *path_total, int32_t *path_target, orgpu_sssp_stats *stats);
+
 /* ---- A2 ROI extraction -------------------------------------------------------- */
 
 /* ROIExtractor.extract_roi_bfs (UPF:1636-1734) plus the must-keep re-add of

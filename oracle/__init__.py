@@ -20,6 +20,11 @@ Parts and what pins them (each checked in tests/test_oracle_*.py):
         pinned by the reference's own paths (tests/golden/dijkstra.npz).
   roi_bfs                        ROIExtractor.extract_roi(_bfs) UPF:1619-1734
         pinned by the reference's own ROIs (tests/golden/roi_bfs.npz).
+  ms_ref                         find_path_multisource_multisink UPF:1889-1989
+        pinned by the reference's own paths (tests/golden/multisource.npz).
+  ms_canon                       multi-source search in the GPU contract's
+        semantics: pinned by brute force over all simple paths on tiny graphs
+        and by the total cost of ms_ref (within 1e-5 relative).
   oracle.costs, oracle.negotiation
         pinned by tests/golden/costs.npz and the reference's full negotiation
         trajectories (tests/golden/negotiation*.npz).
@@ -63,6 +68,10 @@ def lib():
         _lib.orc_backtrace.argtypes = [P, i32, i32, P, i64]
         _lib.orc_sssp_ref.restype = i64
         _lib.orc_sssp_ref.argtypes = [i64, P, P, P, P, i64, P, P, P, i32, f32, i32, i32, P, i64, P]
+        _lib.orc_ms_ref.restype = i64
+        _lib.orc_ms_ref.argtypes = [i64, P, P, P, i64, P, P, i64, P, P, i64, P, P, P, i64, P]
+        _lib.orc_ms_canon.restype = i64
+        _lib.orc_ms_canon.argtypes = [i64, P, P, P, P, i64, P, P, i64, P, P, P, P, P, P, P]
         _lib.orc_roi_bfs.restype = i64
         _lib.orc_roi_bfs.argtypes = [i64, P, P, i32, i32, ctypes.c_int, i64, P, P, P, P]
     return _lib
@@ -233,6 +242,59 @@ def roi_bfs(lat, src, dst, stagnation_bonus=0.0, max_nodes=50_000):
     if R < 0:
         raise MemoryError("oracle roi failed")
     return out[:R].copy()
+
+
+def ms_ref(lat, costs, seeds, targets, roi_nodes, global_to_roi=None):
+    """Exact emulation of find_path_multisource_multisink (UPF:1889-1989):
+    seeds / targets are lists of (node, cost).  Returns (path or None, best
+    total)."""
+    roi_nodes = np.ascontiguousarray(roi_nodes, dtype=np.int32)
+    if global_to_roi is None:
+        global_to_roi = np.full(lat.N, -1, dtype=np.int32)
+        global_to_roi[roi_nodes] = np.arange(len(roi_nodes), dtype=np.int32)
+    g2r = np.ascontiguousarray(global_to_roi, dtype=np.int32)
+    sn = np.ascontiguousarray([a for a, _ in seeds], dtype=np.int32)
+    sc = np.ascontiguousarray([b for _, b in seeds], dtype=np.float64)
+    tn = np.ascontiguousarray([a for a, _ in targets], dtype=np.int32)
+    tc = np.ascontiguousarray([b for _, b in targets], dtype=np.float64)
+    costs = np.ascontiguousarray(costs, dtype=np.float32)
+    out = np.empty(max(len(roi_nodes), 1), dtype=np.int32)
+    best = ctypes.c_double(np.inf)
+    n = lib().orc_ms_ref(lat.N, _p(lat.indptr), _p(lat.indices), _p(costs), len(roi_nodes), _p(roi_nodes), _p(g2r),
+                         len(sn), _p(sn), _p(sc), len(tn), _p(tn), _p(tc), _p(out), len(out), ctypes.byref(best))
+    if n < 0:
+        raise MemoryError("oracle ms_ref failed")
+    return (out[:n].tolist() if n > 1 else None), best.value
+
+
+def ms_canon(lat, costs, seeds, targets, roi_mask=None):
+    """Canonical multi-source / multi-sink search (the GPU contract): returns
+    (path or None, chosen target entry index or -1, best total (float64),
+    dist f32[N], pred i32[N], expanded, traversed)."""
+    sn = np.ascontiguousarray([a for a, _ in seeds], dtype=np.int32)
+    sc = np.ascontiguousarray([b for _, b in seeds], dtype=np.float32)
+    tn = np.ascontiguousarray([a for a, _ in targets], dtype=np.int32)
+    tc = np.ascontiguousarray([b for _, b in targets], dtype=np.float32)
+    costs = np.ascontiguousarray(costs, dtype=np.float32)
+    rm = None if roi_mask is None else np.ascontiguousarray(roi_mask, dtype=np.uint8)
+    dist = np.empty(lat.N, dtype=np.float32)
+    pred = np.empty(lat.N, dtype=np.int32)
+    best = ctypes.c_double(np.inf)
+    ex, tr = ctypes.c_int64(0), ctypes.c_int64(0)
+    k = lib().orc_ms_canon(lat.N, _p(lat.indptr), _p(lat.indices), _p(costs), _p(rm), len(sn), _p(sn), _p(sc),
+                           len(tn), _p(tn), _p(tc), _p(dist), _p(pred), ctypes.byref(best), ctypes.byref(ex),
+                           ctypes.byref(tr))
+    if k < -1:
+        raise MemoryError("oracle ms_canon failed")
+    path = None
+    if k >= 0:
+        cur, p = int(tn[k]), []
+        while cur != -1:
+            p.append(cur)
+            cur = int(pred[cur])
+        p.reverse()
+        path = p if len(p) > 1 else None
+    return path, int(k), best.value, dist, pred, ex.value, tr.value
 
 
 def path_cost(lat, costs, path):

@@ -12,6 +12,8 @@
  *                   (the GPU contract; same search space as SimpleDijkstra)
  *   orc_sssp_ref    SimpleDijkstra.find_path_roi, emulated exactly       UPF:1801-1887
  *   orc_roi_bfs     ROIExtractor.extract_roi_bfs                         UPF:1636-1734
+ *   orc_ms_ref      SimpleDijkstra.find_path_multisource_multisink, exact UPF:1889-1989
+ *   orc_ms_canon    the same search in the canonical FP32 semantics (GPU contract)
  *
  * Built with gcc -O2 (SSE float arithmetic, no excess precision, no FMA
  * contraction: -ffp-contract=off).
@@ -364,6 +366,157 @@ int64_t orc_sssp_ref(int64_t N, const int32_t *indptr, const int32_t *indices,
     }
     free(h.a); free(dist); free(parent); free(visited); free(alts);
     return n > 1 ? n : 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multi-source / multi-sink search (UPF:1889-1989)                          */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Exact emulation of find_path_multisource_multisink.  Seeds (node, cost)
+ * inside the ROI set dist = float32(cost) and push (cost, roi index) -- a
+ * later seed of the same node overwrites an earlier one, as the reference's
+ * loop does.  Targets inside the ROI form a dict: a later entry for the same
+ * node replaces the earlier one.  The heap pops (float64 key, roi index);
+ * a popped target is recorded when du + target_cost < best (strict, so the
+ * first found wins ties) and is not expanded; the search runs until the heap
+ * is empty.  Relaxation compares float32(alt) < dist like SimpleDijkstra.
+ * Writes the path (seed .. target, global ids) and returns its length, or 0
+ * for "None" (no target reached, or a path of one node).
+ */
+int64_t orc_ms_ref(int64_t N, const int32_t *indptr, const int32_t *indices, const float *costs,
+                   int64_t R, const int32_t *roi_nodes, const int32_t *global_to_roi,
+                   int64_t n_seed, const int32_t *seed_node, const double *seed_cost,
+                   int64_t n_tgt, const int32_t *tgt_node, const double *tgt_cost,
+                   int32_t *out, int64_t max_len, double *best_total)
+{
+    (void)N;
+    float *dist = (float *)malloc(sizeof(float) * (size_t)R);
+    int32_t *parent = (int32_t *)malloc(sizeof(int32_t) * (size_t)R);
+    uint8_t *visited = (uint8_t *)calloc((size_t)R, 1);
+    uint8_t *is_t = (uint8_t *)calloc((size_t)R, 1);
+    double *tcost = (double *)malloc(sizeof(double) * (size_t)R);
+    if (!dist || !parent || !visited || !is_t || !tcost) return -1;
+    for (int64_t i = 0; i < R; ++i) { dist[i] = INFINITY; parent[i] = -1; }
+    heapd_t h = {0};
+    for (int64_t i = 0; i < n_seed; ++i) {
+        int32_t r = global_to_roi[seed_node[i]];
+        if (r < 0) continue;
+        dist[r] = (float)seed_cost[i];
+        hd_push(&h, (hd_t){seed_cost[i], r});
+    }
+    int64_t nt = 0;
+    for (int64_t i = 0; i < n_tgt; ++i) {
+        int32_t r = global_to_roi[tgt_node[i]];
+        if (r < 0) continue;
+        is_t[r] = 1;
+        tcost[r] = tgt_cost[i];
+        nt++;
+    }
+    int32_t reached = -1;
+    double best = INFINITY;
+    if (h.n > 0 && nt > 0) {
+        while (h.n > 0) {
+            hd_t t = hd_pop(&h);
+            int32_t ur = t.n;
+            if (visited[ur]) continue;
+            visited[ur] = 1;
+            if (is_t[ur]) {
+                double total = t.d + tcost[ur];
+                if (total < best) { best = total; reached = ur; }
+                continue;
+            }
+            int32_t ug = roi_nodes[ur];
+            for (int32_t e = indptr[ug]; e < indptr[ug + 1]; ++e) {
+                int32_t vr = global_to_roi[indices[e]];
+                if (vr < 0 || visited[vr]) continue;
+                double alt = t.d + (double)costs[e];
+                if ((float)alt < dist[vr]) {
+                    dist[vr] = (float)alt;
+                    parent[vr] = ur;
+                    hd_push(&h, (hd_t){alt, vr});
+                }
+            }
+        }
+    }
+    int64_t n = 0;
+    if (reached >= 0) {
+        int32_t cur = reached;
+        while (cur != -1 && n < max_len) { out[n++] = roi_nodes[cur]; cur = parent[cur]; }
+        if (cur != -1) n = 0;
+        for (int64_t i = 0, j = n - 1; i < j; ++i, --j) { int32_t tt = out[i]; out[i] = out[j]; out[j] = tt; }
+    }
+    if (best_total) *best_total = best;
+    free(h.a); free(dist); free(parent); free(visited); free(is_t); free(tcost);
+    return n > 1 ? n : 0;
+}
+
+/*
+ * Canonical multi-source / multi-sink search (the GPU contract).  Search
+ * space: nodes with roi[n] != 0 (NULL: whole graph).  Seeds inside it start
+ * at key (cost, no predecessor), the lowest cost winning for a repeated node.
+ * Keys follow orc_sssp_canon (fp32 distance, lowest predecessor on ties);
+ * target nodes are settled but never expanded, and the search runs to
+ * exhaustion.  The chosen target entry minimises (double(dist) +
+ * double(target cost), dist, node, entry index).  Returns the entry index or
+ * -1; dist/pred describe the whole search.
+ */
+int64_t orc_ms_canon(int64_t N, const int32_t *indptr, const int32_t *indices, const float *costs,
+                     const uint8_t *roi, int64_t n_seed, const int32_t *seed_node, const float *seed_cost,
+                     int64_t n_tgt, const int32_t *tgt_node, const float *tgt_cost,
+                     float *dist, int32_t *pred, double *best_total, int64_t *expanded, int64_t *traversed)
+{
+    uint8_t *done = (uint8_t *)calloc((size_t)N, 1);
+    uint8_t *is_t = (uint8_t *)calloc((size_t)N, 1);
+    if (!done || !is_t) return -2;
+    for (int64_t i = 0; i < N; ++i) { dist[i] = INFINITY; pred[i] = -1; }
+    heapf_t h = {0};
+    for (int64_t i = 0; i < n_seed; ++i) {
+        int32_t v = seed_node[i];
+        if (roi && !roi[v]) continue;
+        if (seed_cost[i] < dist[v]) { dist[v] = seed_cost[i]; hf_push(&h, (hf_t){seed_cost[i], v}); }
+    }
+    for (int64_t i = 0; i < n_tgt; ++i)
+        if (!roi || roi[tgt_node[i]]) is_t[tgt_node[i]] = 1;
+    int64_t nexp = 0, ntrav = 0;
+    while (h.n > 0) {
+        hf_t t = hf_pop(&h);
+        int32_t u = t.n;
+        if (done[u] || t.d != dist[u]) continue;
+        done[u] = 1;
+        nexp++;
+        if (is_t[u]) continue;
+        float du = dist[u];
+        for (int32_t e = indptr[u]; e < indptr[u + 1]; ++e) {
+            int32_t v = indices[e];
+            ntrav++;
+            if (roi && !roi[v]) continue;
+            float nd = du + costs[e];
+            if (!(nd < INFINITY)) continue;
+            if (nd < dist[v]) {
+                dist[v] = nd; pred[v] = u;
+                hf_push(&h, (hf_t){nd, v});
+            } else if (nd == dist[v] && pred[v] != -1 && u < pred[v]) {
+                pred[v] = u;
+            }
+        }
+    }
+    int64_t best = -1;
+    double bt = INFINITY;
+    for (int64_t i = 0; i < n_tgt; ++i) {
+        int32_t v = tgt_node[i];
+        if ((roi && !roi[v]) || !(dist[v] < INFINITY)) continue;
+        double tot = (double)dist[v] + (double)tgt_cost[i];
+        int better = best < 0 || tot < bt ||
+                     (tot == bt && (dist[v] < dist[tgt_node[best]] ||
+                                    (dist[v] == dist[tgt_node[best]] && v < tgt_node[best])));
+        if (better) { best = i; bt = tot; }
+    }
+    if (best_total) *best_total = bt;
+    if (expanded) *expanded = nexp;
+    if (traversed) *traversed = ntrav;
+    free(h.a); free(done); free(is_t);
+    return best;
 }
 
 /* ------------------------------------------------------------------------ */

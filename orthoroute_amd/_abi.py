@@ -19,7 +19,7 @@ EXPORTED = [
     "orgpu_lattice_count", "orgpu_lattice_build",
     "orgpu_reserve", "orgpu_route", "orgpu_roi_bfs",
     "orgpu_usage_scatter", "orgpu_present_build", "orgpu_present_ema", "orgpu_cost_update",
-    "orgpu_history_update", "orgpu_overuse_sparse", "orgpu_cost_bounds",
+    "orgpu_history_update", "orgpu_overuse_sparse", "orgpu_cost_bounds", "orgpu_route_ms",
 ]
 
 
@@ -49,6 +49,17 @@ class NetReq(ctypes.Structure):
     _fields_ = [("src", ctypes.c_int32), ("dst", ctypes.c_int32), ("net_id", ctypes.c_int32),
                 ("roi_index", ctypes.c_int32), ("use_keepout", ctypes.c_int32),
                 ("reserved", ctypes.c_int32)]
+
+
+class MsPoint(ctypes.Structure):
+    _fields_ = [("node", ctypes.c_int32), ("cost", ctypes.c_float)]
+
+
+class MsReq(ctypes.Structure):
+    _fields_ = [("seed_off", ctypes.c_int32), ("seed_n", ctypes.c_int32),
+                ("target_off", ctypes.c_int32), ("target_n", ctypes.c_int32),
+                ("net_id", ctypes.c_int32), ("roi_index", ctypes.c_int32),
+                ("use_keepout", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class SsspParams(ctypes.Structure):
@@ -120,6 +131,8 @@ def load():
         "orgpu_reserve": [P, i64, i32, i32, i64, i32],
         "orgpu_route": [P, ctypes.POINTER(Graph), P, i32, P, i32, P, f32, ctypes.POINTER(SsspParams),
                         P, P, i64, P, P, ctypes.POINTER(SsspStats)],
+        "orgpu_route_ms": [P, ctypes.POINTER(Graph), P, i32, P, i64, P, i64, P, i32, P, f32,
+                           ctypes.POINTER(SsspParams), P, P, i64, P, P, P, ctypes.POINTER(SsspStats)],
         "orgpu_roi_bfs": [P, ctypes.POINTER(Graph), P, i32, P, i32, P],
         "orgpu_usage_scatter": [P, P, i64, P, i64, i32],
         "orgpu_present_build": [P, i64, P, P, P, i64, f32],

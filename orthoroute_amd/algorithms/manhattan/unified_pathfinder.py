@@ -36,7 +36,7 @@ import numpy as np
 
 from ... import _abi
 from ...device import (Context, DeviceLattice, EdgeState, SearchParams, cost_bounds, roi_bfs,
-                       route as device_route)
+                       route as device_route, route_ms as device_route_ms)
 from ...lattice import DIR_H, DIR_HV, DIR_V, layer_directions, reference_via_pairs
 from ...reductions import sum_f32
 from .board_analyzer import analyze_board_characteristics
@@ -333,6 +333,34 @@ class SimpleDijkstra:
         res, _ = device_route(self.ctx, self.graph.dlat, costs, [(src, dst, -1, 0, 0)], self.params, roi_bitmaps=rows)
         r = res[0]
         return None if r is None else [int(v) for v in r[0]]
+
+    def find_path_multisource_multisink(self, src_seeds, dst_targets, costs, roi_nodes, global_to_roi):
+        """UPF:1889-1989: cheapest path from any seed (node, entry cost) to
+        any target (node, exit cost) inside the ROI; targets are settled but
+        not expanded and the search runs until its frontier is empty.
+        Returns (path, entry_layer, exit_layer), or None when no target is
+        reached or the best path is a single node.  Ties between targets go
+        to the lower dist, then the lower node id."""
+        g2r = np.asarray(global_to_roi.get() if hasattr(global_to_roi, "get") else global_to_roi)
+        seeds = [(int(v), float(c)) for v, c in src_seeds if g2r[int(v)] >= 0]
+        targets = [(int(v), float(c)) for v, c in dst_targets if g2r[int(v)] >= 0]
+        if not seeds or not targets:
+            return None
+        torch = self.ctx.torch
+        bits = np.packbits((g2r >= 0).astype(np.uint8), bitorder="little")
+        bits = np.concatenate([bits, np.zeros((-len(bits)) % 4, np.uint8)]).view(np.int32)
+        rows = torch.from_numpy(bits.reshape(1, -1)).to(self.ctx.device)
+        if not isinstance(costs, torch.Tensor):
+            costs = torch.from_numpy(np.ascontiguousarray(costs, dtype=np.float32)).to(self.ctx.device)
+        res, _ = device_route_ms(self.ctx, self.graph.dlat, costs, [(seeds, targets, -1, 0, 0)], self.params,
+                                 roi_bitmaps=rows)
+        r = res[0]
+        if r is None or len(r[0]) <= 1:
+            return None
+        path = [int(v) for v in r[0]]
+        if self.plane_size:
+            return path, path[0] // self.plane_size, path[-1] // self.plane_size
+        return path, 0, 0
 
 
 # ----------------------------------------------------------------------------

@@ -100,3 +100,24 @@ extern "C" int orgpu_route(orgpu_t *h, const orgpu_graph *g, const orgpu_net_req
     return engine_route(h->en, g, nets, n_nets, roi_bitmaps, n_roi, keepout_owner, keepout_cost, params,
                         path_nodes, path_edges, path_capacity, path_offset, path_dist, stats);
 }
+
+extern "C" int orgpu_route_ms(orgpu_t *h, const orgpu_graph *g, const orgpu_ms_req *reqs, int32_t n_reqs,
+                              const orgpu_ms_point *seeds, int64_t n_seeds, const orgpu_ms_point *targets,
+                              int64_t n_targets, const uint32_t *roi_bitmaps, int32_t n_roi,
+                              const int32_t *keepout_owner, float keepout_cost, const orgpu_sssp_params *params,
+                              int32_t *path_nodes, int32_t *path_edges, int64_t path_capacity, int64_t *path_offset,
+                              double *path_total, int32_t *path_target, orgpu_sssp_stats *stats)
+{
+    if (!h) return orgpu_fail(ORGPU_EINVAL, "route_ms: null handle");
+    if (n_reqs < 0) return orgpu_fail(ORGPU_EINVAL, "route_ms: negative request count");
+    HIP_CHECK_RET(hipSetDevice(h->device));
+    if (h->en.cap_slots == 0) return orgpu_fail(ORGPU_EINVAL, "route_ms: call orgpu_reserve first");
+    if (n_reqs == 0) {
+        if (path_offset) path_offset[0] = 0;
+        return ORGPU_OK;
+    }
+    if (!path_nodes) return orgpu_fail(ORGPU_EINVAL, "route_ms: null path output");
+    MsCall ms{reqs, seeds, targets, n_seeds, n_targets, path_total, path_target};
+    return engine_route(h->en, g, nullptr, n_reqs, roi_bitmaps, n_roi, keepout_owner, keepout_cost, params,
+                        path_nodes, path_edges, path_capacity, path_offset, nullptr, stats, &ms);
+}

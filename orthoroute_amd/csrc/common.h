@@ -12,7 +12,9 @@
 //   bits 63..59  keepout edges on the path (saturating at 31)
 //   bits 58..28  FP32 bit pattern of the path cost without keepout edges
 //                (non-negative floats order like their bit patterns)
-//   bits 27..0   predecessor node index (lowest index wins ties)
+//   bits 27..0   predecessor field: predecessor node index + 1 (lowest
+//                index wins ties); 0 for a source, so that a source keeps
+//                "no predecessor" against an equal-cost path into it
 //
 // The reference replaces the cost of a keepout edge by 1e9
 // (UPF:3182, 3220) and sums in float64; with path costs far below 1e9 that
@@ -22,8 +24,8 @@
 #define KEY_INF 0xFFFFFFFFFFFFFFFFull
 #define KEY_PRED_BITS 28
 #define KEY_PRED_MASK 0x0FFFFFFFu
-#define KEY_NOPRED 0x0FFFFFFFu
-#define KEY_MAX_NODES (1ll << 28) - 1
+#define KEY_NOPRED 0u
+#define KEY_MAX_NODES ((1ll << 28) - 2)
 
 __host__ __device__ inline uint64_t key_pack(uint32_t ko, float d, uint32_t pred)
 {
@@ -39,6 +41,7 @@ __host__ __device__ inline float key_dist(uint64_t k)
     __builtin_memcpy(&d, &db, 4);
     return d;
 }
+// predecessor field (KEY_NOPRED or node + 1)
 __host__ __device__ inline uint32_t key_pred(uint64_t k) { return (uint32_t)(k & KEY_PRED_MASK); }
 // (keepout count, distance) part, comparable as an integer
 __host__ __device__ inline uint64_t key_cost(uint64_t k) { return k >> 28; }
@@ -92,7 +95,9 @@ struct SlotInfo {
     int32_t req;        // request index
     int32_t pending;    // items of this slot still queued (pushes - pops)
     int32_t dx, dy, dz; // destination coordinates (goal-directed bucketing)
-    int32_t pad;
+    int32_t ms;         // multi-source / multi-sink search (targets below)
+    int32_t t_off, t_n; // its targets in RoundArgs::ms_tgt
+    int32_t pad0, pad1;
 };
 
 struct RoundArgs {
@@ -117,6 +122,9 @@ struct RoundArgs {
     uint64_t bitmap_words;
     uint32_t *summ;           // [MAX_BUFS][summ_words]: one bit per 64-word chunk
     uint64_t summ_words;
+    const orgpu_ms_point *ms_tgt;  // targets of multi-source requests
+    uint32_t *tbits;          // [K][tbits_words]: target nodes of each multi-source slot
+    int64_t tbits_words;
     uint32_t *touch_bits;     // [K][touch_words]: key chunks (2048 keys) a search set
     uint32_t *touch_list;     // [K][touch_cap]: the same chunks as a list
     uint32_t *touch_n;        // [K]

@@ -19,6 +19,13 @@ struct Engine {
     uint32_t *lists = nullptr, *list_cnt = nullptr;
     uint32_t *touch_bits = nullptr, *touch_list = nullptr, *touch_n = nullptr;
     int64_t keys_n = -1;   // node count the keys' slot layout was last reset for
+    // multi-source / multi-sink calls
+    orgpu_ms_point *ms_seed = nullptr, *ms_tgt = nullptr;
+    int64_t ms_seed_cap = 0, ms_tgt_cap = 0;
+    uint32_t *tbits = nullptr;
+    int64_t tbits_cap = 0;
+    double *ms_total = nullptr, *h_ms_total = nullptr;
+    int32_t *ms_choice = nullptr, *h_ms_choice = nullptr;
     RoundPart *parts = nullptr;
     uint64_t summ_words = 0, chunks = 0;
     Ctrl *ctrl = nullptr;
@@ -59,7 +66,17 @@ struct orgpu_ctx {
 
 int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets, int64_t qcap, int32_t max_path);
 void engine_release(Engine &en);
+// the multi-source part of a route call (null for two-pin requests)
+struct MsCall {
+    const orgpu_ms_req *reqs;
+    const orgpu_ms_point *seeds, *targets;
+    int64_t n_seeds, n_targets;
+    double *path_total;
+    int32_t *path_target;
+};
+
 int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, int32_t n_nets,
                  const uint32_t *roi_bitmaps, int32_t n_roi, const int32_t *keepout_owner, float keepout_cost,
                  const orgpu_sssp_params *p, int32_t *path_nodes, int32_t *path_edges,
-                 int64_t path_capacity, int64_t *path_offset, float *path_dist, orgpu_sssp_stats *stats);
+                 int64_t path_capacity, int64_t *path_offset, float *path_dist, orgpu_sssp_stats *stats,
+                 const MsCall *ms = nullptr);

@@ -196,7 +196,7 @@ __device__ __forceinline__ void lq_clamp(const LocalQ &lq)
 // (goal-directed bucketing; 0 when the call runs in plain Dijkstra order).
 __device__ __forceinline__ float heur(const RoundArgs &a, const SlotInfo &si, uint32_t v)
 {
-    if (a.h_xy == 0.0f && a.h_z == 0.0f) return 0.0f;
+    if ((a.h_xy == 0.0f && a.h_z == 0.0f) || si.ms) return 0.0f;
     const int z = (int)(v / (uint32_t)a.plane);
     const int r = (int)(v - (uint32_t)z * (uint32_t)a.plane);
     const int y = r / a.X;
@@ -235,6 +235,8 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bo
         const bool goal = a.h_xy != 0.0f || a.h_z != 0.0f;
         if (b > cur || (b < cur && !goal)) return;
     }
+    // a target of a multi-source search is settled but not expanded
+    if (si.ms && ((a.tbits[(uint64_t)slot * a.tbits_words + (u >> 5)] >> (u & 31)) & 1u)) return;
     const int row = a.indptr[u];
     const int deg = a.indptr[u + 1] - row;
     my_items++;
@@ -272,7 +274,7 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bo
             if (ko_node && a.kind[row + k0 + q] == 0) ko = ko < 31 ? ko + 1 : 31;
             else nd = nd + c[q];
             ok[q] = nd < __builtin_inff();
-            nk[q] = key_pack(ko, nd, u);
+            nk[q] = key_pack(ko, nd, u + 1u);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -682,9 +684,15 @@ k_control(RoundArgs a, int round_grid)
     for (int s = threadIdx.x; s < a.K; s += CTRL_WG) {
         SlotInfo *si = &a.slots[s];
         if (si->state != 1) continue;
-        const uint64_t kd = a.keys[(uint64_t)s * (uint64_t)a.N + (uint64_t)si->dst];
         // no set bits left: the search is exhausted and every key final
         const bool exhausted = si->pending <= 0;
+        if (si->ms) {
+            // multi-source searches run until their frontier is empty; the
+            // backtrace picks the target
+            if (exhausted) { si->state = 2; atomicSub(&c->active, 1u); }
+            continue;
+        }
+        const uint64_t kd = a.keys[(uint64_t)s * (uint64_t)a.N + (uint64_t)si->dst];
         const bool reached = kd != KEY_INF;
         // goal-directed order settles the destination one bucket later: the
         // margin absorbs FP32 rounding of g + h on its equal-cost predecessors
@@ -709,7 +717,8 @@ __global__ void k_fill_keys(uint64_t *keys, int64_t n)
 }
 
 struct SeedReq {
-    int32_t slot, src, dst, net_id, roi_index, use_keepout, req, pad;
+    int32_t slot, src, dst, net_id, roi_index, use_keepout, req, ms;
+    int32_t s_off, s_n, t_off, t_n;   // multi-source: seeds / targets in the call's arrays
 };
 
 // Resets the keys of the slots about to be seeded: only the 2048-key chunks
@@ -732,13 +741,23 @@ __global__ void k_reset_keys(RoundArgs a, const SeedReq *reqs, int n)
 }
 
 // Marks finished slots free (their queued items, if any, keep draining).
-__global__ void k_release(SlotInfo *slots, const int32_t *list, int n)
+__global__ void k_release(RoundArgs a, const int32_t *list, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) slots[list[i]].state = 0;
+    if (i >= n) return;
+    SlotInfo *si = &a.slots[list[i]];
+    si->state = 0;
+    if (si->ms) {
+        uint32_t *row = a.tbits + (uint64_t)list[i] * a.tbits_words;
+        for (int t = 0; t < si->t_n; ++t) {
+            const uint32_t v = (uint32_t)a.ms_tgt[si->t_off + t].node;
+            atomicAnd(&row[v >> 5], ~(1u << (v & 31)));
+        }
+        si->ms = 0;
+    }
 }
 
-__global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n)
+__global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n, const orgpu_ms_point *ms_seed)
 {
     Ctrl *c = a.ctrl;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -748,28 +767,62 @@ __global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n)
         si->src = r.src; si->dst = r.dst; si->net_id = r.net_id;
         si->roi_index = r.roi_index; si->use_keepout = r.use_keepout;
         si->req = r.req;
-        if (a.plane > 0 && a.X > 0) {
+        si->ms = r.ms; si->t_off = r.t_off; si->t_n = r.t_n;
+        if (a.plane > 0 && a.X > 0 && !r.ms) {
             const int z = r.dst / a.plane, rr = r.dst - z * a.plane;
             si->dz = z; si->dy = rr / a.X; si->dx = rr - (rr / a.X) * a.X;
         } else {
             si->dx = si->dy = si->dz = 0;
         }
-        // the source (g = 0) lands in the current bucket: its bucket under
-        // the goal-directed order, floor(h(src) / delta), is taken off the
-        // slot's base (the bucket index starts at 2^32, so this never wraps)
-        si->base = c->cur - key_bucket_rel(key_pack(0, 0.0f, KEY_NOPRED), a.inv_delta, heur(a, *si, (uint32_t)r.src));
-        si->state = 1;
-        const uint32_t item = (uint32_t)r.slot * (uint32_t)a.N + (uint32_t)r.src;
-        a.keys[item] = key_pack(0, 0.0f, KEY_NOPRED);
-        const uint32_t buf = c->ring[c->cur % a.nb];
-        // a slot is reseeded only once its previous search has no set bits
-        // left, so the source's bit is clear here
-        si->pending = 1;
         // the slot's keys were reset (k_reset_keys): start a new touch record
         for (int64_t w = 0; w < a.touch_words; ++w) a.touch_bits[(uint64_t)r.slot * a.touch_words + w] = 0u;
         a.touch_n[r.slot] = 0u;
-        touch_key(a, (uint32_t)r.slot, (uint32_t)r.src);
-        if (push_bit(a, buf, c->list_id[buf], item)) atomicAdd(&c->count[buf], 1u);
+        // a slot is reseeded only once its previous search has no set bits
+        // left, so its items' bits are clear here
+        if (!r.ms) {
+            // the source (g = 0) lands in the current bucket: its bucket under
+            // the goal-directed order, floor(h(src) / delta), is taken off the
+            // slot's base (the bucket index starts at 2^32, so this never wraps)
+            si->base = c->cur - key_bucket_rel(key_pack(0, 0.0f, KEY_NOPRED), a.inv_delta, heur(a, *si, (uint32_t)r.src));
+            si->state = 1;
+            const uint32_t item = (uint32_t)r.slot * (uint32_t)a.N + (uint32_t)r.src;
+            a.keys[item] = key_pack(0, 0.0f, KEY_NOPRED);
+            const uint32_t buf = c->ring[c->cur % a.nb];
+            si->pending = 1;
+            touch_key(a, (uint32_t)r.slot, (uint32_t)r.src);
+            if (push_bit(a, buf, c->list_id[buf], item)) atomicAdd(&c->count[buf], 1u);
+        } else {
+            si->base = c->cur;
+            si->state = 1;
+            uint32_t *trow = a.tbits + (uint64_t)r.slot * a.tbits_words;
+            for (int t = 0; t < r.t_n; ++t) {
+                const uint32_t v = (uint32_t)a.ms_tgt[r.t_off + t].node;
+                atomicOr(&trow[v >> 5], 1u << (v & 31));
+            }
+            const uint32_t *roi = r.roi_index >= 0 ? a.roi + (uint64_t)r.roi_index * a.roi_words : nullptr;
+            int pend = 0;
+            for (int t = 0; t < r.s_n; ++t) {
+                const orgpu_ms_point sp = ms_seed[r.s_off + t];
+                const uint32_t v = (uint32_t)sp.node;
+                if (roi && !((roi[v >> 5] >> (v & 31)) & 1u)) continue;
+                const uint32_t item = (uint32_t)r.slot * (uint32_t)a.N + v;
+                const uint64_t k = key_pack(0, sp.cost, KEY_NOPRED);
+                const uint64_t old = atomicMin((unsigned long long *)&a.keys[item], (unsigned long long)k);
+                if (old == KEY_INF) touch_key(a, (uint32_t)r.slot, v);
+                if (k < old && key_cost(k) < key_cost(old)) {
+                    const uint64_t b = si->base + key_bucket_rel(k, a.inv_delta);
+                    uint32_t buf;
+                    if (b < c->cur + a.nb) {
+                        buf = c->ring[b % a.nb];
+                    } else {
+                        buf = c->ovf_buf;
+                        atomicMin((unsigned long long *)&c->ovf_min, (unsigned long long)b);
+                    }
+                    if (push_bit(a, buf, c->list_id[buf], item)) { atomicAdd(&c->count[buf], 1u); ++pend; }
+                }
+            }
+            si->pending = pend;
+        }
         atomicAdd(&c->active, 1u);
     }
     __syncthreads();
@@ -784,7 +837,8 @@ __global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n)
 // the sorted row, UPF:5463-5476).  len = 0: unreachable; -1: broken chain or
 // longer than the staging row.
 __global__ void k_backtrace(RoundArgs a, const int32_t *slot_list, int n, int32_t *stage_nodes,
-                            int32_t *stage_edges, int32_t *lens, float *dists, int32_t max_len)
+                            int32_t *stage_edges, int32_t *lens, float *dists, int32_t max_len,
+                            double *ms_total, int32_t *ms_choice)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -794,15 +848,44 @@ __global__ void k_backtrace(RoundArgs a, const int32_t *slot_list, int n, int32_
     int32_t *pe = stage_edges + (int64_t)s * max_len;
     const uint64_t *ks = a.keys + (uint64_t)s * (uint64_t)a.N;
     if (si.state != 2) { lens[s] = 0; dists[s] = __builtin_inff(); return; }
-    dists[s] = key_dist(ks[si.dst]);
+    int32_t end = si.dst;
+    if (si.ms) {
+        // the cheapest target by (keepout edges, dist + target cost), then
+        // dist, node index, entry order
+        int best = -1;
+        uint32_t bko = 0;
+        double bt = 0.0;
+        float bd = 0.0f;
+        int32_t bn = 0;
+        for (int t = 0; t < si.t_n; ++t) {
+            const orgpu_ms_point tp = a.ms_tgt[si.t_off + t];
+            const uint64_t k = ks[tp.node];
+            if (k == KEY_INF) continue;
+            const uint32_t ko = key_ko(k);
+            const float d = key_dist(k);
+            const double tot = (double)d + (double)tp.cost;
+            const bool better = best < 0 || ko < bko ||
+                                (ko == bko && (tot < bt || (tot == bt && (d < bd || (d == bd && tp.node < bn)))));
+            if (better) { best = t; bko = ko; bt = tot; bd = d; bn = tp.node; }
+        }
+        if (ms_choice) ms_choice[s] = best;
+        if (ms_total) ms_total[s] = best >= 0 ? bt : (double)__builtin_inff();
+        if (best < 0) { lens[s] = 0; dists[s] = __builtin_inff(); return; }
+        end = bn;
+    }
+    dists[s] = key_dist(ks[end]);
     int len = 0;
-    int32_t cur = si.dst;
+    int32_t cur = end;
     while (true) {
         if (len >= max_len) { len = -1; break; }
         pn[len++] = cur;
-        if (cur == si.src) break;
-        const uint32_t p = key_pred(ks[cur]);
-        if (p == KEY_NOPRED) { len = -1; break; }
+        if (!si.ms && cur == si.src) break;
+        const uint32_t pf = key_pred(ks[cur]);
+        if (pf == KEY_NOPRED) {
+            if (!si.ms) len = -1;   // a two-pin chain must end at its source
+            break;                  // a multi-source chain ends at a seed
+        }
+        const uint32_t p = pf - 1u;
         // first CSR edge p -> cur
         const int r0 = a.indptr[p], r1 = a.indptr[p + 1];
         int ee = -1;
@@ -916,6 +999,13 @@ void engine_release(Engine &en)
     if (en.touch_bits) hipFree(en.touch_bits);
     if (en.touch_list) hipFree(en.touch_list);
     if (en.touch_n) hipFree(en.touch_n);
+    if (en.ms_seed) hipFree(en.ms_seed);
+    if (en.ms_tgt) hipFree(en.ms_tgt);
+    if (en.tbits) hipFree(en.tbits);
+    if (en.ms_total) hipFree(en.ms_total);
+    if (en.ms_choice) hipFree(en.ms_choice);
+    if (en.h_ms_total) hipHostFree(en.h_ms_total);
+    if (en.h_ms_choice) hipHostFree(en.h_ms_choice);
     if (en.parts) hipFree(en.parts);
     if (en.ctrl) hipFree(en.ctrl);
     if (en.slots) hipFree(en.slots);
@@ -957,9 +1047,9 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
                  const uint32_t *roi_bitmaps, int32_t n_roi, const int32_t *keepout_owner, float keepout_cost,
                  const orgpu_sssp_params *p,
                  int32_t *path_nodes, int32_t *path_edges, int64_t path_capacity, int64_t *path_offset,
-                 float *path_dist, orgpu_sssp_stats *stats)
+                 float *path_dist, orgpu_sssp_stats *stats, const MsCall *ms)
 {
-    if (!g || !p || (n_nets > 0 && !nets) || !path_offset) return orgpu_fail(ORGPU_EINVAL, "route: null argument");
+    if (!g || !p || (n_nets > 0 && !nets && !ms) || !path_offset) return orgpu_fail(ORGPU_EINVAL, "route: null argument");
     if (!g->indptr || !g->indices || !g->cost || !g->edge_kind) return orgpu_fail(ORGPU_EINVAL, "route: null graph array");
     const int64_t N = g->n_nodes;
     const int K = p->slots;
@@ -979,7 +1069,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     // what a penalty far above every path cost (the reference's 1e9) does
     if (keepout_owner && !(keepout_cost >= 1.0e6f))
         return orgpu_fail(ORGPU_EINVAL, "route: keepout_cost %g must dominate path costs (>= 1e6)", (double)keepout_cost);
-    for (int i = 0; i < n_nets; ++i) {
+    for (int i = 0; i < n_nets && !ms; ++i) {
         const orgpu_net_req &r = nets[i];
         if (r.src < 0 || r.src >= N || r.dst < 0 || r.dst >= N)
             return orgpu_fail(ORGPU_EINVAL, "route: request %d endpoint outside [0, %lld)", i, (long long)N);
@@ -988,6 +1078,62 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
         if (r.use_keepout && !keepout_owner) return orgpu_fail(ORGPU_EINVAL, "route: request %d needs keepout_owner", i);
     }
     hipStream_t s = en.stream;
+    if (ms) {
+        if ((ms->n_seeds > 0 && !ms->seeds) || (ms->n_targets > 0 && !ms->targets) || (n_nets > 0 && !ms->reqs) ||
+            ms->n_seeds < 0 || ms->n_targets < 0)
+            return orgpu_fail(ORGPU_EINVAL, "route_ms: null argument");
+        for (int64_t i = 0; i < ms->n_seeds; ++i) {
+            const orgpu_ms_point &q = ms->seeds[i];
+            if (q.node < 0 || q.node >= N || !(q.cost >= 0.0f && q.cost < 3.0e38f))
+                return orgpu_fail(ORGPU_EINVAL, "route_ms: seed %lld (node %d, cost %g) invalid", (long long)i, q.node,
+                                  (double)q.cost);
+        }
+        for (int64_t i = 0; i < ms->n_targets; ++i) {
+            const orgpu_ms_point &q = ms->targets[i];
+            if (q.node < 0 || q.node >= N || !(q.cost >= 0.0f && q.cost < 3.0e38f))
+                return orgpu_fail(ORGPU_EINVAL, "route_ms: target %lld (node %d, cost %g) invalid", (long long)i, q.node,
+                                  (double)q.cost);
+        }
+        for (int i = 0; i < n_nets; ++i) {
+            const orgpu_ms_req &r = ms->reqs[i];
+            if (r.seed_off < 0 || r.seed_n < 0 || (int64_t)r.seed_off + r.seed_n > ms->n_seeds ||
+                r.target_off < 0 || r.target_n < 0 || (int64_t)r.target_off + r.target_n > ms->n_targets)
+                return orgpu_fail(ORGPU_EINVAL, "route_ms: request %d seed/target range outside the arrays", i);
+            if (r.roi_index >= 0 && (!roi_bitmaps || r.roi_index >= n_roi))
+                return orgpu_fail(ORGPU_EINVAL, "route_ms: request %d ROI row %d outside [0, %d)", i, r.roi_index, n_roi);
+            if (r.use_keepout && !keepout_owner) return orgpu_fail(ORGPU_EINVAL, "route_ms: request %d needs keepout_owner", i);
+        }
+        // device copies of the seeds / targets, per-slot target bitmaps and outputs
+        const int64_t tw = (N + 31) / 32;
+        if (en.ms_seed_cap < std::max<int64_t>(ms->n_seeds, 1)) {
+            if (en.ms_seed) hipFree(en.ms_seed);
+            en.ms_seed_cap = std::max<int64_t>(ms->n_seeds, 1024);
+            HIP_CHECK_RET(hipMalloc((void **)&en.ms_seed, sizeof(orgpu_ms_point) * en.ms_seed_cap));
+        }
+        if (en.ms_tgt_cap < std::max<int64_t>(ms->n_targets, 1)) {
+            if (en.ms_tgt) hipFree(en.ms_tgt);
+            en.ms_tgt_cap = std::max<int64_t>(ms->n_targets, 1024);
+            HIP_CHECK_RET(hipMalloc((void **)&en.ms_tgt, sizeof(orgpu_ms_point) * en.ms_tgt_cap));
+        }
+        if (en.tbits_cap < (int64_t)K * tw) {
+            if (en.tbits) hipFree(en.tbits);
+            en.tbits_cap = (int64_t)en.cap_slots * ((en.cap_nodes + 31) / 32);
+            HIP_CHECK_RET(hipMalloc((void **)&en.tbits, sizeof(uint32_t) * en.tbits_cap));
+        }
+        if (!en.ms_total) {
+            HIP_CHECK_RET(hipMalloc((void **)&en.ms_total, sizeof(double) * MAX_SLOTS));
+            HIP_CHECK_RET(hipMalloc((void **)&en.ms_choice, sizeof(int32_t) * MAX_SLOTS));
+            HIP_CHECK_RET(hipHostMalloc((void **)&en.h_ms_total, sizeof(double) * MAX_SLOTS, hipHostMallocDefault));
+            HIP_CHECK_RET(hipHostMalloc((void **)&en.h_ms_choice, sizeof(int32_t) * MAX_SLOTS, hipHostMallocDefault));
+        }
+        if (ms->n_seeds > 0)
+            HIP_CHECK_RET(hipMemcpyAsync(en.ms_seed, ms->seeds, sizeof(orgpu_ms_point) * ms->n_seeds,
+                                         hipMemcpyHostToDevice, s));
+        if (ms->n_targets > 0)
+            HIP_CHECK_RET(hipMemcpyAsync(en.ms_tgt, ms->targets, sizeof(orgpu_ms_point) * ms->n_targets,
+                                         hipMemcpyHostToDevice, s));
+        HIP_CHECK_RET(hipMemsetAsync(en.tbits, 0, sizeof(uint32_t) * (size_t)K * tw, s));
+    }
     const int nb = p->n_buckets;
     RoundArgs a;
     a.ctrl = en.ctrl; a.slots = en.slots; a.K = K; a.N = N; a.nb = (uint32_t)nb;
@@ -1001,6 +1147,9 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     a.keys = en.keys;
     a.bits = en.bits; a.bitmap_words = (uint64_t)((N * (int64_t)K + 31) / 32);
     a.summ = en.summ; a.summ_words = ((a.bitmap_words + 63) / 64 + 31) / 32;
+    a.ms_tgt = ms ? en.ms_tgt : nullptr;
+    a.tbits = ms ? en.tbits : nullptr;
+    a.tbits_words = (N + 31) / 32;
     a.touch_cap = (N + 2047) / 2048;
     a.touch_words = (a.touch_cap + 31) / 32;
     a.touch_bits = en.touch_bits; a.touch_list = en.touch_list; a.touch_n = en.touch_n;
@@ -1056,15 +1205,21 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
         last_seeded = 0;
         for (int k = 0; k < K && next < n_nets; ++k) {
             if (slot_req[k] >= 0 || !slot_drained[k]) continue;
-            const orgpu_net_req &r = nets[next];
-            en.h_seed[ns] = SeedReq{k, r.src, r.dst, r.net_id, r.roi_index, r.use_keepout, next, 0};
+            if (ms) {
+                const orgpu_ms_req &r = ms->reqs[next];
+                en.h_seed[ns] = SeedReq{k, -1, -1, r.net_id, r.roi_index, r.use_keepout, next, 1,
+                                        r.seed_off, r.seed_n, r.target_off, r.target_n};
+            } else {
+                const orgpu_net_req &r = nets[next];
+                en.h_seed[ns] = SeedReq{k, r.src, r.dst, r.net_id, r.roi_index, r.use_keepout, next, 0, 0, 0, 0, 0};
+            }
             slot_req[k] = next;
             ++next; ++ns;
         }
         if (ns == 0) return 0;
         HIP_CHECK_RET(hipMemcpyAsync(en.seed, en.h_seed, sizeof(SeedReq) * ns, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_reset_keys, dim3(512), dim3(256), 0, s, a, en.seed, ns);
-        hipLaunchKernelGGL(k_seed, dim3((ns + 255) / 256), dim3(256), 0, s, a, en.seed, ns);
+        hipLaunchKernelGGL(k_seed, dim3((ns + 255) / 256), dim3(256), 0, s, a, en.seed, ns, ms ? en.ms_seed : nullptr);
         HIP_CHECK_RET(hipGetLastError());
         last_seeded = ns;
         return 0;
@@ -1120,10 +1275,15 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
         }
         HIP_CHECK_RET(hipMemcpyAsync(en.slot_list, en.h_slot_list, sizeof(int32_t) * nf, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_backtrace, dim3((nf + 63) / 64), dim3(64), 0, s, a, en.slot_list, nf,
-                           en.stage_nodes, en.stage_edges, en.lens, en.dists, en.cap_path);
+                           en.stage_nodes, en.stage_edges, en.lens, en.dists, en.cap_path,
+                           ms ? en.ms_total : nullptr, ms ? en.ms_choice : nullptr);
         HIP_CHECK_RET(hipGetLastError());
         HIP_CHECK_RET(hipMemcpyAsync(en.h_lens, en.lens, sizeof(int32_t) * K, hipMemcpyDeviceToHost, s));
         HIP_CHECK_RET(hipMemcpyAsync(en.h_dists, en.dists, sizeof(float) * K, hipMemcpyDeviceToHost, s));
+        if (ms) {
+            HIP_CHECK_RET(hipMemcpyAsync(en.h_ms_total, en.ms_total, sizeof(double) * K, hipMemcpyDeviceToHost, s));
+            HIP_CHECK_RET(hipMemcpyAsync(en.h_ms_choice, en.ms_choice, sizeof(int32_t) * K, hipMemcpyDeviceToHost, s));
+        }
         HIP_CHECK_RET(hipStreamSynchronize(s));
         // all finished paths come back in one synchronisation; finished slot
         // i stages at h_stage + 2*i*cap_path (nodes, then edges)
@@ -1147,7 +1307,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
             const int req = slot_req[k];
             const int len = en.h_lens[k];
             const int32_t *h = en.h_stage + 2 * (size_t)i * cap;
-            if (p->count_minimal && en.h_slots[k].state == 2) {
+            if (p->count_minimal && en.h_slots[k].state == 2 && !ms) {
                 HIP_CHECK_RET(hipMemsetAsync(en.counters, 0, sizeof(unsigned long long) * 2, s));
                 hipLaunchKernelGGL(k_count_minimal, dim3(512), dim3(256), 0, s, a, k, en.counters);
                 HIP_CHECK_RET(hipMemcpyAsync(en.h_counters, en.counters, sizeof(unsigned long long) * 2,
@@ -1170,12 +1330,14 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
             offs[req] = out_pos;
             out_pos += len;
             if (path_dist) path_dist[req] = en.h_dists[k];
+            if (ms && ms->path_total) ms->path_total[req] = en.h_ms_total[k];
+            if (ms && ms->path_target) ms->path_target[req] = en.h_ms_choice[k];
             slot_req[k] = -1;
             en.h_slots[k].state = 0;
             ++done_nets;
         }
         // mark the finished slots free on the device before reseeding
-        hipLaunchKernelGGL(k_release, dim3((nf + 255) / 256), dim3(256), 0, s, en.slots, en.slot_list, nf);
+        hipLaunchKernelGGL(k_release, dim3((nf + 255) / 256), dim3(256), 0, s, a, en.slot_list, nf);
         HIP_CHECK_RET(hipGetLastError());
         for (int i = 0; i < nf; ++i) {
             const int k = en.h_slot_list[i];

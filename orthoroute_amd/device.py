@@ -123,6 +123,53 @@ class DeviceLattice:
         return g
 
 
+class DeviceGraph:
+    """A caller's CSR (indptr i32[N+1], indices i32[E], numpy or device
+    tensors) in HBM with the per-edge layer and kind the search reads.  With
+    the lattice shape (X, Y, L), node = z*X*Y + y*X + x gives each edge's
+    layers; without it the graph is one layer of N nodes (every edge planar,
+    no goal-directed bounds), as the reference's full-graph GPU search lays
+    it out (cuda_dijkstra.py:5552-5556)."""
+
+    def __init__(self, ctx, indptr, indices, shape=None):
+        torch = ctx.torch
+        self.ctx = ctx
+        ip = indptr.detach().cpu().numpy() if hasattr(indptr, "detach") else np.asarray(indptr)
+        ix = indices.detach().cpu().numpy() if hasattr(indices, "detach") else np.asarray(indices)
+        ip = np.ascontiguousarray(ip, dtype=np.int32)
+        ix = np.ascontiguousarray(ix, dtype=np.int32)
+        self.N, self.E = len(ip) - 1, int(ip[-1])
+        pad = int(self.E == 0)
+        if self.N < 1 or len(ix) < self.E or ip[0] != 0 or np.any(np.diff(ip) < 0):
+            raise ValueError("indptr/indices do not form a CSR graph")
+        if self.E and (ix[:self.E].min() < 0 or ix[:self.E].max() >= self.N):
+            raise ValueError("CSR column index outside [0, N)")
+        if shape is None:
+            self.X, self.Y, self.L = self.N, 1, 1
+        else:
+            self.X, self.Y, self.L = (int(v) for v in shape)
+            if self.X * self.Y * self.L != self.N:
+                raise ValueError(f"lattice shape {shape} does not hold {self.N} nodes")
+        plane = self.X * self.Y
+        src_z = np.repeat((np.arange(self.N, dtype=np.int64) // plane).astype(np.int32), np.diff(ip))
+        dst_z = ix[:self.E] // plane
+        dev = ctx.device
+        self.indptr = torch.from_numpy(ip).to(dev)
+        self.indices = torch.from_numpy(np.concatenate([ix[:self.E], np.zeros(pad, np.int32)])).to(dev)
+        self.edge_layer = torch.from_numpy(np.concatenate([src_z.astype(np.uint8),
+                                                           np.zeros(pad, np.uint8)])).to(dev)
+        self.edge_kind = torch.from_numpy(np.concatenate([(src_z != dst_z).astype(np.uint8),
+                                                          np.zeros(pad, np.uint8)])).to(dev)
+
+    def graph(self, cost):
+        g = _abi.Graph()
+        g.n_nodes, g.n_edges = self.N, self.E
+        g.indptr, g.indices = self.indptr.data_ptr(), self.indices.data_ptr()
+        g.cost, g.edge_kind = cost.data_ptr(), self.edge_kind.data_ptr()
+        g.X, g.Y, g.L = self.X, self.Y, self.L
+        return g
+
+
 class SearchParams:
     """Search launch parameters.  h_xy / h_z: lower bounds of one planar step
     and of one layer of via span under the call's costs (cost_bounds); both 0
@@ -207,6 +254,60 @@ def route(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=None
             out.append(None)
     stats = {"rounds": st.rounds, "items": st.items, "edges": st.edges,
              "minimal_nodes": st.minimal_nodes, "minimal_edges": st.minimal_edges,
+             "kernel_ms": st.kernel_ms, "launches": st.launches, "round_ms": st.round_ms}
+    return out, stats
+
+
+def route_ms(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=None, path_capacity=None):
+    """Multi-source / multi-sink searches.  requests = list of (seeds,
+    targets, net_id, roi_index, use_keepout) with seeds / targets lists of
+    (node, entry cost).  Returns (list of (nodes, edges, total, target_entry)
+    or None per request, stats dict); a chosen target that is itself a seed
+    gives a one-node path."""
+    n = len(requests)
+    ctx.reserve(lat.N, params.slots, params.n_buckets)
+    reqs = (_abi.MsReq * max(n, 1))()
+    seeds, targets = [], []
+    for i, (sd, tg, nid, roi, ko) in enumerate(requests):
+        reqs[i].seed_off, reqs[i].seed_n = len(seeds), len(sd)
+        reqs[i].target_off, reqs[i].target_n = len(targets), len(tg)
+        reqs[i].net_id, reqs[i].roi_index, reqs[i].use_keepout = int(nid), int(roi), int(ko)
+        seeds.extend(sd)
+        targets.extend(tg)
+    sarr = (_abi.MsPoint * max(len(seeds), 1))()
+    for i, (v, c) in enumerate(seeds):
+        sarr[i].node, sarr[i].cost = int(v), float(c)
+    tarr = (_abi.MsPoint * max(len(targets), 1))()
+    for i, (v, c) in enumerate(targets):
+        tarr[i].node, tarr[i].cost = int(v), float(c)
+    if path_capacity is None:
+        path_capacity = max(1024, n * 4 * (lat.X + lat.Y + 2 * lat.L))
+    nodes = np.empty(path_capacity, dtype=np.int32)
+    edges = np.empty(path_capacity, dtype=np.int32)
+    offs = np.zeros(n + 1, dtype=np.int64)
+    total = np.zeros(max(n, 1), dtype=np.float64)
+    choice = np.zeros(max(n, 1), dtype=np.int32)
+    st = _abi.SsspStats()
+    g = lat.graph(cost)
+    n_roi = 0 if roi_bitmaps is None else int(roi_bitmaps.shape[0])
+    if roi_bitmaps is not None and roi_bitmaps.shape[1] * 32 < lat.N:
+        raise ValueError("roi bitmap rows are shorter than the node count")
+    vp = ctypes.c_void_p
+    rc = ctx.lib.orgpu_route_ms(ctx.handle, ctypes.byref(g), ctypes.cast(reqs, vp), n,
+                                ctypes.cast(sarr, vp), len(seeds), ctypes.cast(tarr, vp), len(targets),
+                                _ptr(roi_bitmaps), n_roi, _ptr(keepout_owner), ctypes.c_float(KEEPOUT_COST),
+                                ctypes.byref(params.c()), nodes.ctypes.data_as(vp), edges.ctypes.data_as(vp),
+                                path_capacity, offs.ctypes.data_as(vp), total.ctypes.data_as(vp),
+                                choice.ctypes.data_as(vp), ctypes.byref(st))
+    _abi.check(rc)
+    out = []
+    for i in range(n):
+        a, b = offs[i], offs[i + 1]
+        if b > a:
+            out.append((nodes[a:b].copy(), edges[a:b - 1].copy(), float(total[i]), int(choice[i])))
+        else:
+            out.append(None)
+    stats = {"rounds": st.rounds, "items": st.items, "edges": st.edges,
              "kernel_ms": st.kernel_ms, "launches": st.launches, "round_ms": st.round_ms}
     return out, stats
 

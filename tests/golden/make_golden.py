@@ -135,6 +135,51 @@ def gen_dijkstra(upf):
         paths=np.concatenate(paths) if paths else np.zeros(0, np.int64))
 
 
+def gen_multisource(upf):
+    """SimpleDijkstra.find_path_multisource_multisink (UPF:1889-1989): seeds
+    and targets with entry costs, on the full graph and on BFS ROIs."""
+    lat, g = make_lattice(upf, 18, 14, 6)
+    E = len(g.indices)
+    N = 18 * 14 * 6
+    rng = np.random.default_rng(21)
+    solver = upf.SimpleDijkstra(g, lat)
+    ext = upf.ROIExtractor(g, lattice=lat)
+    base = np.asarray(g.base_costs, dtype=np.float32)
+    snaps, rows = [], []
+    seeds_all, targets_all, paths = [], [], []
+    for s in range(3):
+        if s == 0:
+            costs = (base * np.float32(0.3)).astype(np.float32)
+            costs += (np.arange(E, dtype=np.float32) % 9973) * np.float32(1e-6)
+        elif s == 1:
+            costs = (base * (1 + 4 * rng.random(E))).astype(np.float32)
+        else:   # many exact ties
+            costs = (base * np.float32(0.5) * rng.integers(1, 3, E).astype(np.float32)).astype(np.float32)
+        snaps.append(costs)
+        for k in range(14):
+            ns, nt = int(rng.integers(1, 5)), int(rng.integers(1, 5))
+            seeds = [(int(rng.integers(0, N)), float(rng.choice([0.0, 0.3, 1.7]))) for _ in range(ns)]
+            targets = [(int(rng.integers(0, N)), float(rng.choice([0.0, 0.45, 2.0]))) for _ in range(nt)]
+            if k % 2 == 0:
+                roi, g2r = ext.extract_roi(seeds[0][0], targets[0][0], initial_radius=30)
+                mode = 0
+            else:
+                roi = np.arange(N, dtype=np.int32); g2r = np.arange(N, dtype=np.int32)
+                mode = 1
+            r = solver.find_path_multisource_multisink(seeds, targets, costs, roi, g2r)
+            p = [] if r is None else list(r[0])
+            rows.append((s, mode, len(seeds_all), ns, len(targets_all), nt, len(p)))
+            seeds_all += seeds; targets_all += targets
+            paths.append(np.asarray(p, dtype=np.int64))
+    np.savez_compressed(
+        os.path.join(HERE, "multisource.npz"), X=18, Y=14, L=6, snaps=np.stack(snaps),
+        rows=np.array(rows, dtype=np.int64),
+        seed_node=np.array([a for a, _ in seeds_all], np.int64), seed_cost=np.array([b for _, b in seeds_all]),
+        target_node=np.array([a for a, _ in targets_all], np.int64),
+        target_cost=np.array([b for _, b in targets_all]),
+        paths=np.concatenate(paths) if paths else np.zeros(0, np.int64))
+
+
 def gen_costs(upf):
     lat, g = make_lattice(upf, 12, 10, 6)
     E = len(g.indices)
@@ -262,6 +307,8 @@ def main():
         gen_dijkstra(upf); print("dijkstra done")
     if "costs" in which:
         gen_costs(upf); print("costs done")
+    if "multisource" in which:
+        gen_multisource(upf); print("multisource done")
     if "negotiation" in which:
         r = gen_negotiation(upf, Board, Net, Pad, Coordinate)
         print("negotiation", {k: r.get(k) for k in ("success", "overuse_sum", "overuse_edges", "failed_nets")})

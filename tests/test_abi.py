@@ -55,3 +55,15 @@ def test_error_reporting():
     rc = lib.orgpu_lattice_count(ctypes.byref(d), ctypes.byref(n), ctypes.byref(e))
     assert rc == -1
     assert b"bad shape" in lib.orgpu_last_error()
+
+
+def test_solver_seam_fails_loudly_without_a_gpu():
+    """No silent CPU fallback: the solver seam raises where no GPU is visible
+    (the reference's caller then takes its own CPU path, UPF:1819-1821)."""
+    import numpy as np
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    from orthoroute_amd.solver import GpuSolver
+    with pytest.raises(RuntimeError):
+        GpuSolver(np.array([0, 1, 1], np.int32), np.array([1], np.int32))

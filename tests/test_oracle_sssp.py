@@ -156,3 +156,99 @@ def test_canon_brute_force_tiny():
             assert path is None
         else:
             assert dist[n - 1] == np.float32(best[n - 1][0])
+
+
+# ---------------------------------------------------------------------------
+# multi-source / multi-sink (UPF:1889-1989)
+# ---------------------------------------------------------------------------
+
+def _ms_cases(g):
+    rows = g["rows"]
+    off = 0
+    for row in rows:
+        s, mode, so, sn, to, tn, plen = (int(v) for v in row)
+        seeds = list(zip(g["seed_node"][so:so + sn].tolist(), g["seed_cost"][so:so + sn].tolist()))
+        targets = list(zip(g["target_node"][to:to + tn].tolist(), g["target_cost"][to:to + tn].tolist()))
+        ref = g["paths"][off:off + plen].tolist()
+        off += plen
+        yield s, mode, seeds, targets, ref
+
+
+def test_ms_ref_matches_reference_paths(golden_dir):
+    g = _golden(golden_dir, "multisource.npz")
+    lat = oracle.Lattice(int(g["X"]), int(g["Y"]), int(g["L"]))
+    n = 0
+    for i, (s, mode, seeds, targets, ref) in enumerate(_ms_cases(g)):
+        roi = _roi_for(lat, seeds[0][0], targets[0][0], mode) if mode == 1 else \
+            oracle.roi_bfs(lat, seeds[0][0], targets[0][0], max_nodes=50_000)
+        path, _ = oracle.ms_ref(lat, g["snaps"][s], seeds, targets, roi)
+        assert (path or []) == ref, f"case {i}"
+        n += 1
+    assert n == len(g["rows"])
+
+
+def test_ms_canon_total_within_1e5_of_reference(golden_dir):
+    g = _golden(golden_dir, "multisource.npz")
+    lat = oracle.Lattice(int(g["X"]), int(g["Y"]), int(g["L"]))
+    for i, (s, mode, seeds, targets, ref) in enumerate(_ms_cases(g)):
+        roi = np.arange(lat.N, dtype=np.int32) if mode == 1 else \
+            oracle.roi_bfs(lat, seeds[0][0], targets[0][0], max_nodes=50_000)
+        _, best_ref = oracle.ms_ref(lat, g["snaps"][s], seeds, targets, roi)
+        mask = np.zeros(lat.N, np.uint8)
+        mask[roi] = 1
+        path, k, best, dist, pred, _, _ = oracle.ms_canon(lat, g["snaps"][s], seeds, targets, roi_mask=mask)
+        assert (k >= 0) == np.isfinite(best_ref), f"case {i}"
+        if k >= 0:
+            assert abs(best - best_ref) <= 1e-5 * max(1.0, abs(best_ref)), (i, best, best_ref)
+            if ref and path is not None:
+                # the canonical path is a valid seed -> target walk of that total
+                assert path[0] in [a for a, _ in seeds] and path[-1] == targets[k][0]
+
+
+def test_ms_canon_brute_force_tiny():
+    """All simple paths from any seed to any target (targets are not passed
+    through: they are never expanded) on tiny random digraphs."""
+    rng = np.random.default_rng(17)
+
+    class G:
+        pass
+    for trial in range(60):
+        n = 8
+        adj = [sorted(set(rng.integers(0, n, 3).tolist()) - {u}) for u in range(n)]
+        g = G()
+        g.N = n
+        g.indptr = np.zeros(n + 1, np.int32)
+        g.indptr[1:] = np.cumsum([len(a) for a in adj])
+        g.indices = np.array([v for a in adj for v in a], dtype=np.int32)
+        w = rng.integers(1, 5, len(g.indices)).astype(np.float32) * np.float32(0.5)
+        perm = rng.permutation(n)
+        seeds = [(int(perm[0]), float(rng.integers(0, 3) * 0.5)), (int(perm[1]), float(rng.integers(0, 3) * 0.5))]
+        targets = [(int(perm[2]), float(rng.integers(0, 3) * 0.5)), (int(perm[3]), float(rng.integers(0, 3) * 0.5))]
+        tset = {t for t, _ in targets}
+        best = np.inf
+
+        def dfs(u, cost, seen):
+            nonlocal best
+            for t, tc in targets:
+                if u == t:
+                    best = min(best, cost + tc)
+            if u in tset:
+                return
+            for e in range(g.indptr[u], g.indptr[u + 1]):
+                v = int(g.indices[e])
+                if v not in seen:
+                    seen.add(v)
+                    dfs(v, cost + float(w[e]), seen)
+                    seen.discard(v)
+        for s, sc in seeds:
+            dfs(s, sc, {s})
+        path, k, tot, dist, pred, _, _ = oracle.ms_canon(g, w, seeds, targets)
+        if not np.isfinite(best):
+            assert k < 0
+        else:
+            assert k >= 0 and tot == best, (trial, tot, best)
+            # the path walks CSR edges from a seed, every hop tight
+            assert path[0] in [s for s, _ in seeds]
+            for a, b in zip(path, path[1:]):
+                row = range(g.indptr[a], g.indptr[a + 1])
+                assert any(int(g.indices[e]) == b and np.float32(dist[a] + w[e]) == dist[b] for e in row)
