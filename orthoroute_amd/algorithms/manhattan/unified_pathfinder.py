@@ -694,14 +694,17 @@ class PathFinderRouter:
         return E[ok], S[ok], D[ok], NS[ok], H, NH
 
     def _hops_of(self, S, D, NS):
-        """Via hops (x, y, z_lo, z_hi) among node pairs S->D, with their nets."""
+        """Via hops (x, y, z_lo, z_hi) among node pairs S->D, with their nets.
+        S->D are lattice edges: a planar step moves the index by 1 or X, a
+        via by a nonzero multiple of X*Y, so |D - S| >= X*Y marks the vias."""
         plane = self._Nx * self._Ny
-        rs, rd = S % plane, D % plane
-        zs, zd = S // plane, D // plane
-        hop = (rs == rd) & (zs != zd)
-        r = rs[hop]
-        H = np.stack([r % self._Nx, r // self._Nx, np.minimum(zs[hop], zd[hop]), np.maximum(zs[hop], zd[hop])],
-                     axis=1) if hop.any() else np.zeros((0, 4), np.int64)
+        hop = np.abs(D - S) >= plane
+        if not hop.any():
+            return np.zeros((0, 4), np.int64), NS[hop]
+        s, d = S[hop], D[hop]
+        zs, r = np.divmod(s, plane)
+        zd = d // plane
+        H = np.stack([r % self._Nx, r // self._Nx, np.minimum(zs, zd), np.maximum(zs, zd)], axis=1)
         return H, NS[hop]
 
     # ---------------- via usage / ownership (UPF:2678-2904)
@@ -750,9 +753,15 @@ class PathFinderRouter:
         overwrites an earlier one's owner, as the per-path calls would)."""
         if not paths:
             return
-        S = np.concatenate([np.asarray(p[:-1], np.int64) for p in paths])
-        D = np.concatenate([np.asarray(p[1:], np.int64) for p in paths])
-        NS = np.repeat(np.asarray(nids, np.int64), [len(p) - 1 for p in paths])
+        lens = np.fromiter((len(p) for p in paths), dtype=np.int64, count=len(paths))
+        allp = np.concatenate(paths)
+        if allp.dtype != np.int32:
+            allp = allp.astype(np.int64)
+        # consecutive pairs inside each path (drop the pairs across paths)
+        inner = np.ones(max(len(allp) - 1, 0), dtype=bool)
+        inner[np.cumsum(lens)[:-1] - 1] = False
+        S, D = allp[:-1][inner], allp[1:][inner]
+        NS = np.repeat(np.asarray(nids, np.int64), lens - 1)
         H, NH = self._hops_of(S, D, NS)
         if len(H) == 0:
             return
@@ -831,18 +840,34 @@ class PathFinderRouter:
             segs = [self._net_to_edges[names[i]] for i in with_edges]
             lens = np.fromiter((len(e) for e in segs), dtype=np.int64, count=len(segs))
             flat = np.concatenate(segs) if segs else np.zeros(0, np.int64)
-            hit = (over_dense[flat] != 0).astype(np.int64)
-            starts = np.concatenate([[0], np.cumsum(lens)[:-1]])
-            nz = np.add.reduceat(hit, starts) if len(flat) else np.zeros(len(segs), np.int64)
-            nz[lens == 0] = 0
-            for k in np.nonzero(nz)[0]:
-                e = segs[k]
-                cong[with_edges[k]] = self._seqsum(over_dense[e]) / max(1, len(e))
+            vals = over_dense[flat]
+            nzm = vals != 0
+            if nzm.any():
+                # The reference's sum() adds the values in path order in
+                # float64; adding +0.0 leaves a non-negative sum unchanged, so
+                # the sum over the nonzero values in order is the same number.
+                # Step j adds every net's j-th nonzero value at once.
+                v = vals[nzm].astype(np.float64)
+                sg = np.repeat(np.arange(len(segs)), lens)[nzm]
+                cnt = np.bincount(sg, minlength=len(segs))
+                pos = np.arange(len(v)) - (np.cumsum(cnt) - cnt)[sg]
+                order = np.argsort(pos, kind="stable")
+                bounds = np.searchsorted(pos[order], np.arange(int(cnt.max()) + 1))
+                acc = np.zeros(len(segs), dtype=np.float64)
+                for j in range(int(cnt.max())):
+                    sl = order[bounds[j]:bounds[j + 1]]
+                    acc[sg[sl]] += v[sl]
+                hitn = cnt > 0
+                cong[np.asarray(with_edges)[hitn]] = acc[hitn] / np.maximum(1, lens[hitn])
         score = (dist * 3).astype(np.float64) * (cong + 1.0)
-        scores = list(zip(score.tolist(), names))
-        scores.sort(reverse=True)
-        rot = self._rng.randint(0, min(5, len(scores) // 10))
-        ordered = [n for _, n in scores]
+        # descending (score, name): names are unique, so this is the order
+        # of the reference's reverse tuple sort
+        by_name = sorted(range(len(names)), key=names.__getitem__)
+        name_rank = np.empty(len(names), dtype=np.int64)
+        name_rank[by_name] = np.arange(len(names))
+        order = np.lexsort((name_rank, score))[::-1]
+        rot = self._rng.randint(0, min(5, len(names) // 10))
+        ordered = [names[i] for i in order.tolist()]
         if rot > 0:
             ordered = ordered[rot:] + ordered[:rot]
         return ordered
@@ -999,13 +1024,13 @@ class PathFinderRouter:
         total = len(tasks)
         use_ko = total < 100 and len(self._ko_nodes) > 0
         ko_owner = self._keepout_owner_array() if use_ko else None
-        reqs = []
-        for nid in ordered:
-            s, d = tasks[nid]
-            sx, sy, _ = self._coords(s)
-            dx, dy, _ = self._coords(d)
-            md = abs(dx - sx) + abs(dy - sy)
-            reqs.append((s, d, self._get_net_id(nid) if use_ko else -1, md < ROI_THRESHOLD_STEPS, bonus))
+        sd = np.asarray([tasks[nid] for nid in ordered], dtype=np.int64).reshape(-1, 2)
+        X, plane = self._Nx, self._Nx * self._Ny
+        rs, rd = sd[:, 0] % plane, sd[:, 1] % plane
+        md = np.abs(rd % X - rs % X) + np.abs(rd // X - rs // X)
+        nids = [self._get_net_id(nid) for nid in ordered] if use_ko else [-1] * len(ordered)
+        reqs = list(zip(sd[:, 0].tolist(), sd[:, 1].tolist(), nids, (md < ROI_THRESHOLD_STEPS).tolist(),
+                        [bonus] * len(ordered)))
         results = self._search_batch(reqs, ko_owner, readd=True)
         # wider-ROI fallback for the first five failures, in routing order
         fb = [i for i, r in enumerate(results) if r is None or len(r[0]) <= 1][:5]

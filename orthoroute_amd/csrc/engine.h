@@ -33,6 +33,11 @@ struct Engine {
     SeedReq *seed = nullptr;
     int32_t *slot_list = nullptr;
     int32_t *stage_nodes = nullptr, *stage_edges = nullptr;
+    // finished paths packed back to back in src -> dst order: nodes in
+    // [0, cap_slots*cap_path), edges in the second half; one copy per poll
+    int32_t *pack = nullptr;
+    uint32_t *pack_n = nullptr, *pack_off = nullptr;
+    uint32_t *h_pack_off = nullptr, *h_pack_n = nullptr;
     int32_t *lens = nullptr;
     float *dists = nullptr;
     unsigned long long *counters = nullptr;

@@ -838,7 +838,8 @@ __global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n, const orgpu_ms_p
 // longer than the staging row.
 __global__ void k_backtrace(RoundArgs a, const int32_t *slot_list, int n, int32_t *stage_nodes,
                             int32_t *stage_edges, int32_t *lens, float *dists, int32_t max_len,
-                            double *ms_total, int32_t *ms_choice)
+                            double *ms_total, int32_t *ms_choice,
+                            int32_t *pack, uint32_t *pack_n, uint32_t *pack_off, int64_t pack_half)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -895,6 +896,16 @@ __global__ void k_backtrace(RoundArgs a, const int32_t *slot_list, int n, int32_
         cur = (int32_t)p;
     }
     lens[s] = len;
+    if (len <= 0) return;
+    // pack src -> dst: node t is pn[len-1-t], the edge leaving it pe[len-2-t]
+    const uint32_t off = atomicAdd(pack_n, (uint32_t)len);
+    pack_off[s] = off;
+    int32_t *on = pack + off;
+    int32_t *oe = pack + pack_half + off;
+    for (int t = 0; t < len; ++t) {
+        on[t] = pn[len - 1 - t];
+        oe[t] = t + 1 < len ? pe[len - 2 - t] : -1;
+    }
 }
 
 // Optional Dijkstra-minimal work count: nodes whose (keepout, distance) is at
@@ -966,6 +977,11 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
     HIP_CHECK_RET(hipMalloc((void **)&en.slot_list, sizeof(int32_t) * slots));
     HIP_CHECK_RET(hipMalloc((void **)&en.stage_nodes, (size_t)slots * max_path * sizeof(int32_t)));
     HIP_CHECK_RET(hipMalloc((void **)&en.stage_edges, (size_t)slots * max_path * sizeof(int32_t)));
+    HIP_CHECK_RET(hipMalloc((void **)&en.pack, (size_t)2 * slots * max_path * sizeof(int32_t)));
+    HIP_CHECK_RET(hipMalloc((void **)&en.pack_n, sizeof(uint32_t)));
+    HIP_CHECK_RET(hipMalloc((void **)&en.pack_off, sizeof(uint32_t) * slots));
+    HIP_CHECK_RET(hipHostMalloc((void **)&en.h_pack_off, sizeof(uint32_t) * slots, hipHostMallocDefault));
+    HIP_CHECK_RET(hipHostMalloc((void **)&en.h_pack_n, sizeof(uint32_t), hipHostMallocDefault));
     HIP_CHECK_RET(hipMalloc((void **)&en.lens, sizeof(int32_t) * slots));
     HIP_CHECK_RET(hipMalloc((void **)&en.dists, sizeof(float) * slots));
     HIP_CHECK_RET(hipMalloc((void **)&en.counters, sizeof(unsigned long long) * 2));
@@ -1012,6 +1028,11 @@ void engine_release(Engine &en)
     if (en.seed) hipFree(en.seed);
     if (en.slot_list) hipFree(en.slot_list);
     if (en.stage_nodes) hipFree(en.stage_nodes);
+    if (en.pack) hipFree(en.pack);
+    if (en.pack_n) hipFree(en.pack_n);
+    if (en.pack_off) hipFree(en.pack_off);
+    if (en.h_pack_off) hipHostFree(en.h_pack_off);
+    if (en.h_pack_n) hipHostFree(en.h_pack_n);
     if (en.stage_edges) hipFree(en.stage_edges);
     if (en.lens) hipFree(en.lens);
     if (en.dists) hipFree(en.dists);
@@ -1274,39 +1295,43 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
             continue;
         }
         HIP_CHECK_RET(hipMemcpyAsync(en.slot_list, en.h_slot_list, sizeof(int32_t) * nf, hipMemcpyHostToDevice, s));
+        HIP_CHECK_RET(hipMemsetAsync(en.pack_n, 0, sizeof(uint32_t), s));
+        const int64_t half = (int64_t)en.cap_slots * en.cap_path;
         hipLaunchKernelGGL(k_backtrace, dim3((nf + 63) / 64), dim3(64), 0, s, a, en.slot_list, nf,
                            en.stage_nodes, en.stage_edges, en.lens, en.dists, en.cap_path,
-                           ms ? en.ms_total : nullptr, ms ? en.ms_choice : nullptr);
+                           ms ? en.ms_total : nullptr, ms ? en.ms_choice : nullptr,
+                           en.pack, en.pack_n, en.pack_off, half);
         HIP_CHECK_RET(hipGetLastError());
         HIP_CHECK_RET(hipMemcpyAsync(en.h_lens, en.lens, sizeof(int32_t) * K, hipMemcpyDeviceToHost, s));
         HIP_CHECK_RET(hipMemcpyAsync(en.h_dists, en.dists, sizeof(float) * K, hipMemcpyDeviceToHost, s));
+        HIP_CHECK_RET(hipMemcpyAsync(en.h_pack_off, en.pack_off, sizeof(uint32_t) * K, hipMemcpyDeviceToHost, s));
+        HIP_CHECK_RET(hipMemcpyAsync(en.h_pack_n, en.pack_n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         if (ms) {
             HIP_CHECK_RET(hipMemcpyAsync(en.h_ms_total, en.ms_total, sizeof(double) * K, hipMemcpyDeviceToHost, s));
             HIP_CHECK_RET(hipMemcpyAsync(en.h_ms_choice, en.ms_choice, sizeof(int32_t) * K, hipMemcpyDeviceToHost, s));
         }
         HIP_CHECK_RET(hipStreamSynchronize(s));
-        // all finished paths come back in one synchronisation; finished slot
-        // i stages at h_stage + 2*i*cap_path (nodes, then edges)
-        const size_t cap = (size_t)en.cap_path;
         for (int i = 0; i < nf; ++i) {
             const int k = en.h_slot_list[i];
-            const int len = en.h_lens[k];
-            if (len < 0) return orgpu_fail(ORGPU_ECAPACITY, "route: request %d: predecessor chain broken or longer than %d",
-                                           slot_req[k], en.cap_path);
-            if (len == 0) continue;
-            int32_t *h = en.h_stage + 2 * (size_t)i * cap;
-            HIP_CHECK_RET(hipMemcpyAsync(h, en.stage_nodes + (size_t)k * cap, sizeof(int32_t) * len,
-                                         hipMemcpyDeviceToHost, s));
-            if (path_edges)
-                HIP_CHECK_RET(hipMemcpyAsync(h + cap, en.stage_edges + (size_t)k * cap, sizeof(int32_t) * len,
-                                             hipMemcpyDeviceToHost, s));
+            if (en.h_lens[k] < 0)
+                return orgpu_fail(ORGPU_ECAPACITY, "route: request %d: predecessor chain broken or longer than %d",
+                                  slot_req[k], en.cap_path);
         }
-        HIP_CHECK_RET(hipStreamSynchronize(s));
+        // every finished path, packed src -> dst, comes back in one copy
+        // (nodes at h_stage, edges at h_stage + half)
+        const uint32_t packed = *en.h_pack_n;
+        if (packed > 0) {
+            HIP_CHECK_RET(hipMemcpyAsync(en.h_stage, en.pack, sizeof(int32_t) * packed, hipMemcpyDeviceToHost, s));
+            if (path_edges)
+                HIP_CHECK_RET(hipMemcpyAsync(en.h_stage + half, en.pack + half, sizeof(int32_t) * packed,
+                                             hipMemcpyDeviceToHost, s));
+            HIP_CHECK_RET(hipStreamSynchronize(s));
+        }
         for (int i = 0; i < nf; ++i) {
             const int k = en.h_slot_list[i];
             const int req = slot_req[k];
             const int len = en.h_lens[k];
-            const int32_t *h = en.h_stage + 2 * (size_t)i * cap;
+            const int32_t *h = en.h_stage + (len > 0 ? en.h_pack_off[k] : 0u);
             if (p->count_minimal && en.h_slots[k].state == 2 && !ms) {
                 HIP_CHECK_RET(hipMemsetAsync(en.counters, 0, sizeof(unsigned long long) * 2, s));
                 hipLaunchKernelGGL(k_count_minimal, dim3(512), dim3(256), 0, s, a, k, en.counters);
@@ -1319,12 +1344,8 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
             if (len > 0) {
                 if (out_pos + len > path_capacity)
                     return orgpu_fail(ORGPU_ECAPACITY, "route: path output capacity %lld exceeded", (long long)path_capacity);
-                // staging holds dst..src; the edge entering node i sits at i
-                for (int t = 0; t < len; ++t) path_nodes[out_pos + t] = h[len - 1 - t];
-                if (path_edges) {
-                    for (int t = 0; t + 1 < len; ++t) path_edges[out_pos + t] = h[cap + len - 2 - t];
-                    path_edges[out_pos + len - 1] = -1;
-                }
+                memcpy(path_nodes + out_pos, h, sizeof(int32_t) * len);
+                if (path_edges) memcpy(path_edges + out_pos, h + half, sizeof(int32_t) * len);
             }
             lens[req] = len;
             offs[req] = out_pos;

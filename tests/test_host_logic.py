@@ -136,8 +136,8 @@ def test_barrel_conflicts():
     assert n == len(exp) and got.tolist() == exp
 
 
-@pytest.mark.parametrize("seed", [6, 7])
-def test_order_nets_matches_loop_form(seed):
+@pytest.mark.parametrize("seed,n_over", [(6, 200), (7, 200), (8, 4000)])
+def test_order_nets_matches_loop_form(seed, n_over):
     rng = np.random.default_rng(seed)
     r = bare_router(37, 23, 5)
     plane = 37 * 23
@@ -148,7 +148,7 @@ def test_order_nets_matches_loop_form(seed):
         if rng.random() < 0.7:
             r._net_to_edges[f"N{i:03d}"] = rng.integers(0, 5000, rng.integers(0, 30)).astype(np.int64)
     over = np.zeros(5000, np.float32)
-    over[rng.integers(0, 5000, 200)] = (rng.random(200) * 3).astype(np.float32)
+    over[rng.integers(0, 5000, n_over)] = (rng.random(n_over) * 3).astype(np.float32)
     r._rng = random.Random(seed)
     got = r._order_nets_by_difficulty(tasks, over)
 
