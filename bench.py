@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--slots", type=int, default=512, help="concurrent searches of the batched form")
     ap.add_argument("--plain-sample", type=int, default=48, help="searches timed in the plain vs batched comparison")
+    ap.add_argument("--mt-nets", type=int, default=200,
+                    help="cfg3 multi-terminal nets routed as trees after the timed steps (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU oracle sample budget (0 = skip)")
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--delta", type=float, default=0.0, help="bucket width (0 = the router's choice)")
@@ -215,6 +217,8 @@ def main():
         cpu = cpu_baseline(r, creqs, paths["batched"] if sreqs else [None] * len(creqs), args.cpu_seconds, X, Y, L,
                            compare=bool(sreqs))
 
+    mt = multi_terminal(args) if args.mt_nets > 0 else None
+
     traffic = None
     prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_round_cfg2_traffic.json")
     if os.path.exists(prof):
@@ -259,8 +263,62 @@ def main():
                      "alg_bytes_per_launch": round(alg_bytes / max(launches, 1), 1),
                      "dijkstra_minimal_equiv_GBs": round(dij_equiv, 1)},
         "cpu_baseline": cpu,
+        "multi_terminal": mt,
     }
     print(json.dumps(line), flush=True)
+
+
+def multi_terminal(args):
+    """cfg3's multi-terminal nets (300x300x12, adjacent-layer vias, 3-8 pins
+    each) routed as trees by batched multi-source searches on the base
+    costs; the oracle's tree for the first nets, about 5 s of CPU, compared
+    with the GPU's."""
+    import numpy as np
+    import torch
+    from orthoroute_amd import workloads
+    from orthoroute_amd.algorithms.manhattan.multi_terminal import MultiTerminalNetRouter
+    from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig, PathFinderRouter
+    X, Y, L, nets = workloads.cfg3_multi()
+    nets = nets[:args.mt_nets]
+    cfg = PathFinderConfig()
+    cfg.lattice_policy = "adjacent"
+    cfg.batch_slots = args.slots
+    r = PathFinderRouter(cfg)
+    r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
+    MultiTerminalNetRouter(r).route_all_nets(nets[:8])      # warm-up
+    torch.cuda.synchronize()
+    mtr = MultiTerminalNetRouter(r)
+    t0 = time.perf_counter()
+    res = mtr.route_all_nets(nets)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    pins = sum(len(p) for _, p in nets)
+    attach = sum(len(mtr.trees[n]["order"]) - 1 for n, _ in nets)
+    log(f"[bench] multi-terminal: {len(nets)} nets, {pins} pins, {attach} attachments in {t:.3f}s")
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import oracle
+    lat = oracle.Lattice(X, Y, L, "adjacent", pitch=1.0, via_cost=3.0)
+    base = r.graph.dlat.base_cost[:lat.E].cpu().numpy()
+    t1 = time.perf_counter()
+    checked = equal = 0
+    for name, p in nets:
+        br, order, unreached, nodes = oracle.steiner_canon(lat, base, p)
+        tr = mtr.trees[name]
+        equal += int(tr["branches"] == br and tr["order"] == order and tr["unreached"] == unreached)
+        checked += 1
+        if time.perf_counter() - t1 > 5.0:
+            break
+    tc = time.perf_counter() - t1
+    log(f"[bench] multi-terminal oracle: {checked} nets in {tc:.2f}s, {equal}/{checked} trees equal")
+    return {"workload": f"cfg3 multi-terminal nets {X}x{Y}x{L}, adjacent-layer vias, {len(nets)} nets, {pins} pins, "
+                        f"base costs",
+            "trees_s": round(t, 4), "attachments": attach, "steps": mtr.stats.get("steps"),
+            "attachments_per_s": round(attach / max(t, 1e-9), 1),
+            "fully_connected": res["_validation"]["fully_connected"],
+            "cpu_oracle": {"nets": checked, "s": round(tc, 3), "cores": 1,
+                           "attachments_per_s": round(sum(len(mtr.trees[n]["order"]) - 1 for n, _ in nets[:checked])
+                                                      / max(tc, 1e-9), 2),
+                           "trees_equal_to_gpu": f"{equal}/{checked}"}}
 
 
 def cpu_baseline(r, sreqs, gpu_paths, budget_s, X, Y, L, compare=True):

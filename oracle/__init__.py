@@ -25,6 +25,12 @@ Parts and what pins them (each checked in tests/test_oracle_*.py):
   ms_canon                       multi-source search in the GPU contract's
         semantics: pinned by brute force over all simple paths on tiny graphs
         and by the total cost of ms_ref (within 1e-5 relative).
+  steiner_canon                  multi-terminal trees by repeated ms_canon
+        attachments (the operation the build defines for the reference's
+        unreachable multi-terminal router, MT:557-820): pinned by scipy
+        multi-source distances (each branch costs the tree-to-pin distance
+        of the pin it attaches, and that pin is the nearest one left) and by
+        the two-pin case reducing to sssp_canon.
   oracle.costs, oracle.negotiation
         pinned by tests/golden/costs.npz and the reference's full negotiation
         trajectories (tests/golden/negotiation*.npz).
@@ -295,6 +301,32 @@ def ms_canon(lat, costs, seeds, targets, roi_mask=None):
         p.reverse()
         path = p if len(p) > 1 else None
     return path, int(k), best.value, dist, pred, ex.value, tr.value
+
+
+def steiner_canon(lat, costs, pins, roi_mask=None):
+    """Grow a tree over `pins` (node ids): start at pins[0]; each step runs
+    ms_canon with every tree node seeded at cost 0 and every unconnected pin
+    as a target (cost 0), attaches the chosen pin by its path and adds the
+    path's nodes to the tree.  Returns (branches, order, unreached, nodes) as
+    orthoroute_amd.steiner.grow_trees does."""
+    pins = [int(v) for v in pins]
+    tree, in_tree = pins[:1], set(pins[:1])
+    left = list(range(1, len(pins)))
+    branches, order, unreached = [], [0] if pins else [], []
+    while left:
+        path, k, _, *_ = ms_canon(lat, costs, [(v, 0.0) for v in tree], [(pins[j], 0.0) for j in left], roi_mask)
+        if k < 0:
+            unreached = sorted(left)
+            break
+        j = left.pop(k)
+        order.append(j)
+        br = path if path is not None else [pins[j]]
+        branches.append(br)
+        for v in br:
+            if v not in in_tree:
+                in_tree.add(v)
+                tree.append(v)
+    return branches, order, unreached, tree
 
 
 def path_cost(lat, costs, path):

@@ -1,0 +1,95 @@
+"""Multi-terminal routing surface (orthoroute/algorithms/manhattan/
+multi_terminal.py, cited as MT): ``MultiTerminalNetRouter(pathfinder)`` with
+``route_all_nets(nets, progress_cb, iteration_cb)`` (MT:1005-1160).
+
+The reference grows each net's tree pad by pad with two-pin searches from
+chosen tree nodes (MT:557-820); that pair search calls a router method that
+does not exist, so the reference never produces a tree.  Here a net's tree is
+grown by multi-source searches from the whole tree to every unconnected pin
+(orthoroute_amd.steiner.grow_trees), all nets of a step batched into one GPU
+engine call on one cost snapshot.  Nets are independent of each other (the
+reference commits nothing between them either), so batching changes no
+result.
+"""
+from typing import Dict, List
+
+from ...steiner import grow_trees
+
+
+class MultiTerminalNetRouter:
+    """Routes nets with any number of pads as trees on the router's lattice."""
+
+    def __init__(self, pathfinder, costs=None):
+        self.pf = pathfinder
+        self.costs = costs
+        self.stats = {"total_nets": 0, "multi_terminal_nets": 0, "two_terminal_nets": 0,
+                      "fully_connected": 0, "partially_connected": 0}
+        self.trees = {}
+
+    def _pins(self, net):
+        """(name, pins): tuples (name, [node ids]) as given; Net-like objects
+        map their pads through the router's pad map, duplicates removed in
+        order (MT:187-250)."""
+        if isinstance(net, tuple) and len(net) == 2:
+            name, nodes = net
+            pins = [int(v) for v in nodes]
+        else:
+            name = net.name
+            pins = []
+            for pad in getattr(net, "pads", []):
+                k = self.pf._pad_key(pad)
+                if k in self.pf.pad_to_node:
+                    pins.append(int(self.pf.pad_to_node[k]))
+        seen, out = set(), []
+        for v in pins:
+            if v not in seen:
+                seen.add(v)
+                out.append(v)
+        return name, out
+
+    def _priority(self, name, pins):
+        """MT:1097-1114: more pads, longer spread and GND/VCC nets first."""
+        X = self.pf._Nx
+        plane = X * self.pf._Ny
+        xy = [((v % plane) % X, (v % plane) // X) for v in pins]
+        spread = max((abs(a[0] - b[0]) + abs(a[1] - b[1]) for i, a in enumerate(xy) for b in xy[i + 1:]), default=0)
+        bonus = 1000 if ("GND" in name.upper() or "VCC" in name.upper()) else 0
+        return -(len(pins) * 100 + spread + bonus)
+
+    def route_all_nets(self, nets: List, progress_cb=None, iteration_cb=None) -> Dict:
+        """Returns {net name: tree nodes (first-seen order)} plus
+        '_validation' and '_stats' entries, as MT:1029-1095 / 1135-1160."""
+        multi, two = [], []
+        for net in nets:
+            name, pins = self._pins(net)
+            if len(pins) < 2:
+                continue
+            (multi if len(pins) > 2 else two).append((name, pins))
+        multi.sort(key=lambda np_: self._priority(*np_))
+        order = multi + two
+        self.stats.update(total_nets=len(nets), multi_terminal_nets=len(multi), two_terminal_nets=len(two))
+        pf = self.pf
+        cost = self.costs
+        if cost is None:
+            cost = pf.accounting.total_cost if hasattr(pf, "_hbounds") else pf.graph.dlat.base_cost
+        params = pf._search_params(max(1, min(int(pf.config.batch_slots), len(order))))
+        trees, st = grow_trees(pf.ctx, pf.graph.dlat, cost, [p for _, p in order], params)
+        self.stats["searches"] = st["searches"]
+        self.stats["steps"] = st["steps"]
+        results, full, ratio_sum = {}, 0, 0.0
+        for i, ((name, pins), t) in enumerate(zip(order, trees)):
+            self.trees[name] = t
+            results[name] = t["nodes"]
+            connected = len(pins) - len(t["unreached"])
+            full += int(not t["unreached"])
+            ratio_sum += connected / len(pins)
+            if progress_cb:
+                progress_cb(i + 1, len(order), None)
+        n = len(order)
+        self.stats["fully_connected"] = full
+        self.stats["partially_connected"] = n - full
+        validation = {"total_nets": n, "fully_connected": full, "partially_connected": n - full,
+                      "success_rate": full / n if n else 0.0, "average_connectivity": ratio_sum / n if n else 0.0}
+        results["_validation"] = validation
+        results["_stats"] = dict(self.stats)
+        return results

@@ -548,6 +548,8 @@ k_round(RoundArgs a)
 }
 
 #define CTRL_WG 1024
+// targets a multi-source slot may have for the early completion test
+#define MS_EARLY_TARGETS 256
 
 // Runs after each round (one workgroup): reduces the round's partials,
 // advances the bucket window, detects finished slots, schedules an overflow
@@ -686,17 +688,37 @@ k_control(RoundArgs a, int round_grid)
         if (si->state != 1) continue;
         // no set bits left: the search is exhausted and every key final
         const bool exhausted = si->pending <= 0;
+        // goal-directed order settles the destination one bucket later: the
+        // margin absorbs FP32 rounding of g + h on its equal-cost predecessors
+        const uint64_t margin = (a.h_xy != 0.0f || a.h_z != 0.0f) ? 1u : 0u;
         if (si->ms) {
-            // multi-source searches run until their frontier is empty; the
-            // backtrace picks the target
-            if (exhausted) { si->state = 2; atomicSub(&c->active, 1u); }
+            // A multi-source search is done when its frontier is empty, or
+            // once the current bucket has passed the bucket of the best
+            // (keepout edges, dist + target cost) among its reached targets
+            // (at most MS_EARLY_TARGETS are scanned): every node with a dist
+            // up to that total is then settled, so no cheaper or equal target
+            // is left to find and the keys on the paths are final.  The
+            // backtrace picks the target.
+            bool done = exhausted;
+            if (!done && si->t_n <= MS_EARLY_TARGETS) {
+                uint64_t best = ~0ull;
+                const uint64_t *ks = a.keys + (uint64_t)s * (uint64_t)a.N;
+                for (int t = 0; t < si->t_n; ++t) {
+                    const orgpu_ms_point tp = a.ms_tgt[si->t_off + t];
+                    const uint64_t k = ks[tp.node];
+                    if (k == KEY_INF) continue;
+                    // float(dist + cost) >= every float dist <= that total
+                    const float tot = (float)((double)key_dist(k) + (double)tp.cost);
+                    const uint64_t b = key_bucket_rel(key_pack(key_ko(k), tot, 0u), a.inv_delta);
+                    best = b < best ? b : best;
+                }
+                done = best != ~0ull && si->base + best + margin < newcur;
+            }
+            if (done) { si->state = 2; atomicSub(&c->active, 1u); }
             continue;
         }
         const uint64_t kd = a.keys[(uint64_t)s * (uint64_t)a.N + (uint64_t)si->dst];
         const bool reached = kd != KEY_INF;
-        // goal-directed order settles the destination one bucket later: the
-        // margin absorbs FP32 rounding of g + h on its equal-cost predecessors
-        const uint64_t margin = (a.h_xy != 0.0f || a.h_z != 0.0f) ? 1u : 0u;
         if (reached && (exhausted || si->base + key_bucket_rel(kd, a.inv_delta) + margin < newcur)) {
             si->state = 2; atomicSub(&c->active, 1u);
         } else if (exhausted) {

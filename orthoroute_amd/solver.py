@@ -14,6 +14,7 @@ import numpy as np
 
 from .device import (Context, DeviceGraph, SearchParams, cost_bounds, route as device_route,
                      route_ms as device_route_ms)
+from .steiner import grow_trees
 
 
 def _bits_from_nodes(nodes, N):
@@ -164,3 +165,22 @@ class GpuSolver:
         res, self.last_stats = device_route_ms(self.ctx, self.graph, c, reqs, p,
                                                roi_bitmaps=self._rows(rows) if rows else None)
         return [None if r is None else ([int(v) for v in r[0]], r[2], r[3]) for r in res]
+
+    def route_trees(self, costs, nets, roi_spec=None):
+        """Multi-terminal nets as trees (orthoroute_amd.steiner.grow_trees):
+        nets is a list of pin lists; roi_spec as in find_paths_batch, one
+        entry per net.  Returns grow_trees' per-net dicts."""
+        c = self._costs(costs)
+        rows, row_of = [], []
+        for i in range(len(nets)):
+            r = None if roi_spec is None else roi_spec[i]
+            if r is None:
+                row_of.append(-1)
+                continue
+            r = np.asarray(r)
+            row_of.append(len(rows))
+            rows.append(r.astype(np.uint32) if r.dtype == np.uint32 else _bits_from_nodes(r, self.N))
+        p = self._params(c, len(nets))
+        trees, self.last_stats = grow_trees(self.ctx, self.graph, c, nets, p,
+                                            roi_rows=self._rows(rows) if rows else None, row_of=row_of)
+        return trees

@@ -57,17 +57,49 @@ def cfg2(seed=2, X=600, Y=400, L=18, n_conn=20, pins_per_row=200, n_nets=8000):
     return X, Y, L, reqs
 
 
+def _perimeter(X, Y, depth=3):
+    """Cells of the board's outer ring `depth` cells deep (the single outer
+    ring holds 2(X+Y)-4 = 1,196 cells at 300x300, fewer than cfg3's 1,400
+    two-pin nets plus the multi-terminal pads), row-major."""
+    return [(x, y) for y in range(Y) for x in range(X)
+            if x < depth or y < depth or x >= X - depth or y >= Y - depth]
+
+
 def cfg3(seed=3, X=300, Y=300, L=12, balls=40, n_nets=1400):
     rng = np.random.default_rng(seed)
     x0 = (X - 2 * (balls - 1)) // 2
     y0 = (Y - 2 * (balls - 1)) // 2
     ball = [(x0 + 2 * i, y0 + 2 * j) for j in range(balls) for i in range(balls)]
-    per = [(x, 0) for x in range(X)] + [(x, Y - 1) for x in range(X)] + \
-          [(0, y) for y in range(1, Y - 1)] + [(X - 1, y) for y in range(1, Y - 1)]
+    per = _perimeter(X, Y)
     bsel = rng.choice(len(ball), n_nets, replace=False)
     psel = rng.choice(len(per), n_nets, replace=False)
     reqs = [(f"G{i:04d}", _node(X, Y, *ball[b]), _node(X, Y, *per[p])) for i, (b, p) in enumerate(zip(bsel, psel))]
     return X, Y, L, reqs
+
+
+def cfg3_multi(seed=3, X=300, Y=300, L=12, balls=40, n_nets=1400, n_multi=200):
+    """cfg3's 200 multi-terminal nets: one ball and 2-7 perimeter pads each
+    (3-8 pins), drawn after cfg3's two-pin nets from the same generator,
+    among the balls and perimeter cells those nets left free.  Returns
+    (X, Y, L, [(name, [pin nodes])])."""
+    rng = np.random.default_rng(seed)
+    x0 = (X - 2 * (balls - 1)) // 2
+    y0 = (Y - 2 * (balls - 1)) // 2
+    ball = [(x0 + 2 * i, y0 + 2 * j) for j in range(balls) for i in range(balls)]
+    per = _perimeter(X, Y)
+    bsel = rng.choice(len(ball), n_nets, replace=False)
+    psel = rng.choice(len(per), n_nets, replace=False)
+    free_b = np.setdiff1d(np.arange(len(ball)), bsel)
+    free_p = np.setdiff1d(np.arange(len(per)), psel)
+    mb = rng.choice(free_b, n_multi, replace=False)
+    k = rng.integers(2, 8, n_multi)
+    mp = rng.choice(free_p, int(k.sum()), replace=False)
+    nets, off = [], 0
+    for i in range(n_multi):
+        pins = [_node(X, Y, *ball[mb[i]])] + [_node(X, Y, *per[q]) for q in mp[off:off + k[i]]]
+        off += int(k[i])
+        nets.append((f"M{i:04d}", pins))
+    return X, Y, L, nets
 
 
 def cfg4(seed=4, X=1000, Y=800, L=24, n_nets=40000):
