@@ -217,6 +217,7 @@ def main():
         cpu = cpu_baseline(r, creqs, paths["batched"] if sreqs else [None] * len(creqs), args.cpu_seconds, X, Y, L,
                            compare=bool(sreqs))
 
+    geo = geometry(r)
     mt = multi_terminal(args) if args.mt_nets > 0 else None
 
     traffic = None
@@ -263,9 +264,39 @@ def main():
                      "alg_bytes_per_launch": round(alg_bytes / max(launches, 1), 1),
                      "dijkstra_minimal_equiv_GBs": round(dij_equiv, 1)},
         "cpu_baseline": cpu,
+        "geometry": geo,
         "multi_terminal": mt,
     }
     print(json.dumps(line), flush=True)
+
+
+def geometry(r):
+    """Track / via emission (UPF:5749-5830) of the step's committed paths,
+    and the oracle's per-hop walk over the first 500 of them (compared)."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import geometry as G
+    t0 = time.perf_counter()
+    tracks, vias = r._generate_geometry_from_paths()
+    t = time.perf_counter() - t0
+    hops = sum(max(len(p) - 1, 0) for p in r.net_paths.values())
+    sample = dict(list((n, p) for n, p in r.net_paths.items() if p)[:500])
+    g = r.lattice.geom
+    t1 = time.perf_counter()
+    et, ev = G.generate(sample, r._Nx, r._Ny, r._Nz, r.config.layer_names or [], g.pitch, g.grid_min_x,
+                        g.grid_min_y, guard_outer=r.lattice.policy == "reference")
+    tc = time.perf_counter() - t1
+    width = r.config.grid_pitch * 0.6
+    for e in et:
+        e["width"] = width
+    names = set(sample)
+    same = [x for x in tracks if x["net"] in names] == et and [x for x in vias if x["net"] in names] == ev
+    shops = sum(len(p) - 1 for p in sample.values())
+    log(f"[bench] geometry: {len(tracks)} tracks, {len(vias)} vias from {hops} hops in {t:.3f}s; "
+        f"oracle sample {shops} hops in {tc:.2f}s, equal={same}")
+    return {"paths": sum(1 for p in r.net_paths.values() if p), "hops": hops, "tracks": len(tracks),
+            "vias": len(vias), "s": round(t, 4), "hops_per_s": round(hops / max(t, 1e-9), 1),
+            "cpu_oracle": {"paths": len(sample), "hops_per_s": round(shops / max(tc, 1e-9), 1), "cores": 1,
+                           "records_equal": bool(same)}}
 
 
 def multi_terminal(args):

@@ -1359,29 +1359,157 @@ class PathFinderRouter:
 
     # ---------------- geometry (UPF:5677-5914 surface)
     def emit_geometry(self, board):
+        """UPF:5677-5747: tracks and vias of the committed paths; the export
+        payload only when the routing has no overuse (edge or via), the
+        counts either way.  This build plans no pad escapes, so there is no
+        escape geometry to merge."""
         tracks, vias = self._generate_geometry_from_paths()
-        self._geometry_payload = GeometryPayload(tracks, vias)
+        self._provisional_geometry = GeometryPayload(tracks, vias)
+        over_sum, _ = self.accounting.compute_overuse(router_instance=self)
+        self._geometry_payload = GeometryPayload(tracks, vias) if over_sum == 0 else GeometryPayload([], [])
         return len(tracks), len(vias)
 
     def get_geometry_payload(self):
         return self._geometry_payload
 
+    def get_provisional_geometry(self):
+        return getattr(self, "_provisional_geometry", GeometryPayload([], []))
+
     def _generate_geometry_from_paths(self):
-        """Track and via records per committed path (the KiCad geometry
-        emission details of the reference are outside this build)."""
+        """UPF:5749-5830 with _segment_world / _via_world (UPF:5596-5617,
+        5652-5675): every path becomes its maximal straight track runs (a run
+        ends at a direction change, before a via and at the path's end) and
+        one via record per layer change, in path order.  Planar hops on the
+        outer layers are refused on the reference lattice, whose outer layers
+        carry no planar edges.  Vectorised over all paths; a path with a hop
+        that is not a lattice move (the reference skips such hops without
+        advancing its walk) goes through the per-hop walk instead."""
+        items = [(net, p) for net, p in self.net_paths.items() if p]
+        if not items:
+            return [], []
+        X, L = self._Nx, self._Nz
+        plane = X * self._Ny
+        g = self.lattice.geom
+        pitch, ox, oy = float(g.pitch), float(g.grid_min_x), float(g.grid_min_y)
+        width = self.config.grid_pitch * 0.6
+        names = list(self.config.layer_names or [])
+        lname = [names[z] if z < len(names) else f"L{z}" for z in range(L)]
+        guard = self.lattice.policy == "reference"
+        haxis = np.array([self.lattice.get_legal_axis(z) == "h" for z in range(L)])
+
+        lens = np.fromiter((len(p) for _, p in items), dtype=np.int64, count=len(items))
+        allp = np.concatenate([np.asarray(p, dtype=np.int64) for _, p in items])
+        inner = np.ones(max(len(allp) - 1, 0), dtype=bool)
+        inner[np.cumsum(lens)[:-1] - 1] = False
+        a, b = allp[:-1][inner], allp[1:][inner]
+        hp = np.repeat(np.arange(len(items)), lens - 1)
+        za, ra = np.divmod(a, plane)
+        zb, rb = np.divmod(b, plane)
+        xa, ya = ra % X, ra // X
+        xb, yb = rb % X, rb // X
+        planar = za == zb
+        outer = planar & ((za == 0) | (za == L - 1)) if guard else np.zeros(len(a), dtype=bool)
+        run = planar & ~outer
+        dx, dy = xb - xa, yb - ya
+        legal = (np.abs(dx) + np.abs(dy) == 1) & np.where(haxis[za], dy == 0, dx == 0)
+        bad_paths = np.unique(hp[run & ~legal])
+        ok = ~np.isin(hp, bad_paths) if len(bad_paths) else np.ones(len(a), dtype=bool)
+
+        def world(x, y):
+            wx = ox + np.asarray(x, dtype=np.float64) * pitch
+            wy = oy + np.asarray(y, dtype=np.float64) * pitch
+            return ox + np.round((wx - ox) / pitch) * pitch, oy + np.round((wy - oy) / pitch) * pitch
+
+        # runs: consecutive planar hops of one path in one direction
+        r_idx = np.nonzero(run & ok)[0]
+        dcode = (np.sign(dx[r_idx]) + 1) * 3 + (np.sign(dy[r_idx]) + 1)
+        start = np.ones(len(r_idx), dtype=bool)
+        if len(r_idx) > 1:
+            start[1:] = (r_idx[1:] != r_idx[:-1] + 1) | (dcode[1:] != dcode[:-1]) | (hp[r_idx[1:]] != hp[r_idx[:-1]])
+        first = r_idx[start]
+        last = r_idx[np.append(np.nonzero(start)[0][1:] - 1, len(r_idx) - 1)] if len(r_idx) else r_idx
+        x1, y1 = world(xa[first], ya[first])
+        x2, y2 = world(xb[last], yb[last])
+        v_idx = np.nonzero(~planar & ok)[0]
+        vx, vy = world(xa[v_idx], ya[v_idx])
+        vlo, vhi = np.minimum(za[v_idx], zb[v_idx]), np.maximum(za[v_idx], zb[v_idx])
+
+        net_names = [net for net, _ in items]
+        t_path, v_path = hp[first], hp[v_idx]
+        t_rec = [{"net": net_names[k], "layer": lname[z], "x1": p1, "y1": q1, "x2": p2, "y2": q2, "width": width}
+                 for k, z, p1, q1, p2, q2 in zip(t_path.tolist(), za[first].tolist(), x1.tolist(), y1.tolist(),
+                                                 x2.tolist(), y2.tolist())]
+        v_rec = [{"net": net_names[k], "x": p1, "y": q1, "from_layer": lname[lo], "to_layer": lname[hi],
+                  "diameter": 0.25, "drill": 0.15}
+                 for k, p1, q1, lo, hi in zip(v_path.tolist(), vx.tolist(), vy.tolist(), vlo.tolist(), vhi.tolist())]
+        if not len(bad_paths):
+            return t_rec, v_rec
+        # merge in path order with the per-hop walk of the irregular paths
+        bad = set(bad_paths.tolist())
         tracks, vias = [], []
-        names = self.config.layer_names or []
-        for net, path in self.net_paths.items():
-            for a, b in zip(path or [], (path or [])[1:]):
-                xa, ya, za = self._coords(a)
-                xb, yb, zb = self._coords(b)
-                wa = self.lattice.geom.lattice_to_world(xa, ya)
-                wb = self.lattice.geom.lattice_to_world(xb, yb)
-                if za == zb:
-                    tracks.append({"net": net, "layer": names[za] if za < len(names) else za,
-                                   "start": wa, "end": wb})
-                else:
-                    vias.append({"net": net, "at": wa, "from_layer": min(za, zb), "to_layer": max(za, zb)})
+        ti = vi = 0
+        for k, (net, p) in enumerate(items):
+            if k in bad:
+                t2, v2 = self._walk_geometry(net, p, lname, guard, width, world)
+                tracks += t2
+                vias += v2
+                continue
+            while ti < len(t_rec) and t_path[ti] == k:
+                tracks.append(t_rec[ti])
+                ti += 1
+            while vi < len(v_rec) and v_path[vi] == k:
+                vias.append(v_rec[vi])
+                vi += 1
+        return tracks, vias
+
+    def _walk_geometry(self, net, path, lname, guard, width, world):
+        """The per-hop form of UPF:5756-5830 for one path."""
+        tracks, vias = [], []
+        L = self._Nz
+
+        def seg(u, v, z):
+            (ux, uy, _), (vx_, vy_, _) = self._coords(u), self._coords(v)
+            p1, q1 = world(ux, uy)
+            p2, q2 = world(vx_, vy_)
+            return {"net": net, "layer": lname[z], "x1": float(p1), "y1": float(q1), "x2": float(p2),
+                    "y2": float(q2), "width": width}
+
+        run_start = prev = path[0]
+        prev_dir = None
+        prev_layer = self._coords(prev)[2]
+        for node in path[1:]:
+            x0, y0, z0 = self._coords(prev)
+            x1, y1, z1 = self._coords(node)
+            if guard and z0 == z1 and (z0 == 0 or z0 == L - 1):
+                prev = run_start = node
+                prev_layer = z1
+                continue
+            if z0 == z1:
+                if abs(x1 - x0) + abs(y1 - y0) != 1:
+                    continue
+                if self.lattice.get_legal_axis(z0) == "h":
+                    if y1 != y0:
+                        continue
+                elif x1 != x0:
+                    continue
+            if z1 != z0:
+                if prev != run_start:
+                    tracks.append(seg(run_start, prev, prev_layer))
+                wx, wy = world(x0, y0)
+                vias.append({"net": net, "x": float(wx), "y": float(wy), "from_layer": lname[min(z0, z1)],
+                             "to_layer": lname[max(z0, z1)], "diameter": 0.25, "drill": 0.15})
+                run_start = node
+                prev_dir = None
+            else:
+                d = ((x1 > x0) - (x1 < x0), (y1 > y0) - (y1 < y0))
+                if prev_dir is not None and d != prev_dir:
+                    tracks.append(seg(run_start, prev, prev_layer))
+                    run_start = prev
+                prev_dir = d
+            prev = node
+            prev_layer = z1
+        if prev != run_start:
+            tracks.append(seg(run_start, prev, prev_layer))
         return tracks, vias
 
 

@@ -171,3 +171,57 @@ def test_order_nets_matches_loop_form(seed, n_over):
     if rot > 0:
         exp = exp[rot:] + exp[:rot]
     assert got == exp
+
+
+def geometry_router(X, Y, L, policy, pitch, origin):
+    from orthoroute_amd.algorithms.manhattan.unified_pathfinder import Lattice3D, PathFinderConfig
+    r = bare_router(X, Y, L)
+    r.config = PathFinderConfig()
+    r.config.grid_pitch = pitch
+    r.config.layer_names = ["F.Cu"] + [f"In{i}.Cu" for i in range(1, L - 1)] + ["B.Cu"]
+    ox, oy = origin
+    r.lattice = Lattice3D((ox, oy, ox + (X - 1) * pitch, oy + (Y - 1) * pitch), pitch, L, policy)
+    assert (r.lattice.x_steps, r.lattice.y_steps) == (X, Y)
+    return r
+
+
+def lattice_walk(rng, X, Y, L, steps, irregular):
+    from oracle.geometry import axis
+    x, y, z = int(rng.integers(0, X)), int(rng.integers(0, Y)), int(rng.integers(0, L))
+    p = [z * X * Y + y * X + x]
+    for _ in range(steps):
+        u = rng.random()
+        if irregular and u < 0.04:
+            x = int(np.clip(x + rng.integers(-2, 3), 0, X - 1))      # not a lattice move
+            y = int(np.clip(y + rng.integers(-2, 3), 0, Y - 1))
+        elif u < 0.2:
+            z = int(np.clip(z + rng.choice([-2, -1, 1, 2]), 0, L - 1))
+        elif axis(L, z) == "h" or (axis(L, z) == "hv" and rng.random() < 0.5):
+            x = int(np.clip(x + rng.choice([-1, 1]), 0, X - 1))
+        else:
+            y = int(np.clip(y + rng.choice([-1, 1]), 0, Y - 1))
+        n = z * X * Y + y * X + x
+        if n != p[-1]:
+            p.append(n)
+    return p
+
+
+@pytest.mark.parametrize("policy,L,irregular", [("reference", 6, False), ("reference", 6, True),
+                                                ("adjacent", 5, True), ("reference", 2, False)])
+def test_geometry_matches_per_hop_oracle(policy, L, irregular):
+    """The vectorised emission equals the per-hop restatement of
+    UPF:5749-5830 (oracle.geometry) record for record."""
+    from oracle import geometry as G
+    rng = np.random.default_rng(L * 10 + irregular)
+    X, Y, pitch, origin = 17, 13, 0.5, (2.0, 3.0)
+    r = geometry_router(X, Y, L, policy, pitch, origin)
+    r.net_paths = {f"N{i:03d}": lattice_walk(rng, X, Y, L, int(rng.integers(0, 60)), irregular) for i in range(80)}
+    r.net_paths["EMPTY"] = []
+    got_t, got_v = r._generate_geometry_from_paths()
+    g = r.lattice.geom
+    exp_t, exp_v = G.generate(r.net_paths, X, Y, L, r.config.layer_names, g.pitch, g.grid_min_x, g.grid_min_y,
+                              guard_outer=(policy == "reference"))
+    for t in exp_t:
+        t["width"] = r.config.grid_pitch * 0.6
+    assert got_t == exp_t
+    assert got_v == exp_v
