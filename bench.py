@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--slots", type=int, default=512, help="concurrent searches of the batched form")
     ap.add_argument("--plain-sample", type=int, default=48, help="searches timed in the plain vs batched comparison")
+    ap.add_argument("--rounds-per-poll", type=int, default=32, help="SSSP rounds queued between slot refills")
     ap.add_argument("--mt-nets", type=int, default=200,
                     help="cfg3 multi-terminal nets routed as trees after the timed steps (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU oracle sample budget (0 = skip)")
@@ -113,6 +114,7 @@ def main():
         cfg.sssp_buckets = args.buckets
     cfg.sssp_local_batches = args.local_batches
     cfg.sssp_grid = args.grid
+    cfg.sssp_rounds_per_poll = args.rounds_per_poll
     r = PathFinderRouter(cfg)
     t0 = time.perf_counter()
     r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
@@ -246,6 +248,7 @@ def main():
             "nodes": r.N, "edges": r.accounting.E, "nets": len(tasks), "slots": args.slots,
             "delta": r._search_params(args.slots).delta, "n_buckets": r._search_params(args.slots).n_buckets,
             "local_batches": args.local_batches or 64, "grid": args.grid or 1024,
+            "rounds_per_poll": args.rounds_per_poll,
         },
         "nets_per_s": round(len(tasks) * K / wall, 1),
         "routed": routed, "failed": failed,

@@ -124,6 +124,7 @@ class PathFinderConfig:
     sssp_buckets = None
     sssp_local_batches = 0         # same-bucket batches per wave per round (0 = library default)
     sssp_grid = 0                  # workgroups per round launch (0 = library default)
+    sssp_rounds_per_poll = 32      # rounds queued between status polls / slot refills
     max_iterations_cap = None
     count_minimal = False          # also count Dijkstra-minimal nodes/edges per search
     goal_directed = True           # bucket on g + lower bound to the destination (same paths)
@@ -955,7 +956,8 @@ class PathFinderRouter:
         hx, hz = (0.0, 0.0)
         if self.config.goal_directed and not self.config.count_minimal:
             hx, hz = getattr(self, "_hbounds", (0.0, 0.0))
-        return SearchParams(slots=slots, delta=float(d), n_buckets=int(nb), rounds_per_poll=32,
+        return SearchParams(slots=slots, delta=float(d), n_buckets=int(nb),
+                            rounds_per_poll=int(self.config.sssp_rounds_per_poll or 32),
                             grid=int(self.config.sssp_grid or 0), count_minimal=bool(self.config.count_minimal),
                             h_xy=hx, h_z=hz, local_batches=int(self.config.sssp_local_batches or 0))
 

@@ -19,6 +19,7 @@
 #include "engine.h"
 
 #define RWG 1024
+#define ROI_CHUNK_WORDS 4096
 
 struct RoiArgs {
     int64_t N;
@@ -42,7 +43,7 @@ k_roi(RoiArgs a)
     __shared__ int s_fa0, s_fa1, s_fb0, s_fb1, s_depth;
     __shared__ uint32_t s_scan[RWG];
     __shared__ long long s_acc;
-    __shared__ int s_cut, s_min0, s_min1, s_kept_delta;
+    __shared__ int s_cut, s_min0, s_min1, s_kept_delta, s_chunk, s_chunk_cnt;
     const int tid = threadIdx.x;
     uint32_t *B = a.scratch_b + (int64_t)blockIdx.x * a.words;
     int32_t *LA = a.list_a + (int64_t)blockIdx.x * a.N;
@@ -74,18 +75,36 @@ k_roi(RoiArgs a)
                 const uint32_t *other = wave == 0 ? B : A;
                 int32_t *L = wave == 0 ? LA : LB;
                 int *cnt = wave == 0 ? &s_na : &s_nb;
+                // a thread's edges go four at a time with their loads and
+                // bitmap atomics issued together (the marks a level leaves do
+                // not depend on the order its edges are visited)
                 for (int i = f0 + tid; i < f1; i += RWG) {
                     const int32_t u = L[i];
-                    const int e1 = a.indptr[u + 1];
-                    for (int e = a.indptr[u]; e < e1; ++e) {
-                        const int32_t v = a.indices[e];
-                        if (bit_get(other, v)) { s_met = 1; continue; }
-                        const uint32_t m = 1u << (v & 31);
-                        const uint32_t old = atomicOr(&mine[v >> 5], m);
-                        if (!(old & m)) {
-                            const int pos = atomicAdd(cnt, 1);
-                            L[pos] = v;
-                            atomicAdd(&s_seen_i, 1);
+                    const int e0 = a.indptr[u], e1 = a.indptr[u + 1];
+                    for (int e = e0; e < e1; e += 4) {
+                        int32_t v[4];
+                        bool ok[4];
+                        uint32_t old[4], m[4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            ok[q] = e + q < e1;
+                            v[q] = ok[q] ? a.indices[e + q] : 0;
+                        }
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (ok[q] && bit_get(other, v[q])) { s_met = 1; ok[q] = false; }
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            m[q] = 1u << (v[q] & 31);
+                            old[q] = ok[q] ? atomicOr(&mine[v[q] >> 5], m[q]) : m[q];
+                        }
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            if (!(old[q] & m[q])) {
+                                const int pos = atomicAdd(cnt, 1);
+                                L[pos] = v[q];
+                                atomicAdd(&s_seen_i, 1);
+                            }
                         }
                     }
                 }
@@ -111,12 +130,33 @@ k_roi(RoiArgs a)
         const long long seen = s_seen_i;
         long long kept = seen;
         if (seen > q.max_nodes) {
-            // cut = the max_nodes-th smallest seen node id (1-based rank)
-            if (tid == 0) { s_acc = 0; s_cut = -1; }
+            // cut = the max_nodes-th smallest seen node id (1-based rank).
+            // Coarse pass: set bits per 4096-word chunk (wave sums, one
+            // barrier pair per chunk) up to the chunk that holds the cut;
+            // fine pass: prefix scan of that chunk's words.
+            if (tid == 0) { s_acc = 0; s_cut = -1; s_chunk = -1; s_chunk_cnt = 0; }
             __syncthreads();
-            for (int64_t base = 0; base < a.words; base += RWG) {
+            const int64_t nchunks = (a.words + ROI_CHUNK_WORDS - 1) / ROI_CHUNK_WORDS;
+            for (int64_t ch = 0; ch < nchunks; ++ch) {
+                const int64_t w_end = min((ch + 1) * ROI_CHUNK_WORDS, a.words);
+                int c = 0;
+                for (int64_t w = ch * ROI_CHUNK_WORDS + tid; w < w_end; w += RWG) c += __popc(A[w]);
+                for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+                if ((tid & 63) == 0 && c) atomicAdd(&s_chunk_cnt, c);
+                __syncthreads();
+                if (tid == 0) {
+                    if (s_acc + s_chunk_cnt >= q.max_nodes) s_chunk = (int)ch;
+                    else s_acc += s_chunk_cnt;
+                    s_chunk_cnt = 0;
+                }
+                __syncthreads();
+                if (s_chunk >= 0) break;
+            }
+            const int64_t w_lo = (int64_t)s_chunk * ROI_CHUNK_WORDS;
+            const int64_t w_hi = min(w_lo + ROI_CHUNK_WORDS, a.words);
+            for (int64_t base = w_lo; base < w_hi; base += RWG) {
                 const int64_t w = base + tid;
-                const uint32_t word = w < a.words ? A[w] : 0u;
+                const uint32_t word = w < w_hi ? A[w] : 0u;
                 s_scan[tid] = (uint32_t)__popc(word);
                 __syncthreads();
                 for (int o = 1; o < RWG; o <<= 1) {
@@ -224,8 +264,10 @@ extern "C" int orgpu_roi_bfs(orgpu_t *h, const orgpu_graph *g, const orgpu_roi_r
     }
     HIP_CHECK_RET(hipSetDevice(h->device));
     Engine &en = h->en;
-    // concurrent workgroups, each with ~9 bytes of scratch per node
-    const int slots = std::min<int>(n, (int)std::max<int64_t>(1, std::min<int64_t>(128, (int64_t)(8ll << 30) / (N * 9 + 1))));
+    // concurrent workgroups, each with ~9 bytes of scratch per node (up to
+    // 32 GB in all): a net's BFS is a chain of short levels, so throughput
+    // comes from many nets in flight (four workgroups per CU at 1024)
+    const int slots = std::min<int>(n, (int)std::max<int64_t>(1, std::min<int64_t>(1024, (int64_t)(32ll << 30) / (N * 9 + 1))));
     int rc = ensure_roi_scratch(en, N, slots);
     if (rc) return rc;
     hipStream_t s = h->stream;

@@ -58,7 +58,8 @@ def test_roi_random_truncation(ctx):
     import torch
     from orthoroute_amd.device import roi_bfs
     rng = np.random.default_rng(4)
-    for policy, (X, Y, L) in [("reference", (40, 30, 8)), ("adjacent", (50, 44, 6))]:
+    # the last shape spans several 4096-word chunks of the ROI bitmap
+    for policy, (X, Y, L) in [("reference", (40, 30, 8)), ("adjacent", (50, 44, 6)), ("adjacent", (180, 150, 8))]:
         lat = ctx.build_lattice(X, Y, L, policy)
         o = oracle.Lattice(X, Y, L, policy)
         n = 40
@@ -68,7 +69,7 @@ def test_roi_random_truncation(ctx):
         for i in range(n):
             s, d = (int(v) for v in rng.choice(lat.N, 2, replace=False))
             mx = int(rng.choice([50, 300, 2000, 50_000]))
-            depth = int(rng.integers(0, 80))
+            depth = int(rng.integers(0, 80 if X < 100 else 160))
             reqs.append((s, d, depth, mx, i))
         say(f"[roi] random {policy} {X}x{Y}x{L}")
         roi_bfs(ctx, lat, reqs, rows)
