@@ -38,17 +38,6 @@
 #define MAX_SLOTS 1024
 #define MAX_NB MAX_RING
 
-__device__ __forceinline__ uint64_t atomic_load_u64(const uint64_t *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// agent-scope loads for values other workgroups of the same launch updated
-__device__ __forceinline__ uint32_t ld32(const uint32_t *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 __device__ __forceinline__ bool test_and_set_bit(uint32_t *bm, uint32_t item)
@@ -132,6 +121,17 @@ __device__ __forceinline__ bool push_bit(const RoundArgs &a, uint32_t buf, uint3
         if (!(atomicOr(&a.summ[(uint64_t)buf * a.summ_words + (ch >> 5)], cm) & cm)) list_append(a, list_sid, ch);
     }
     return true;
+}
+
+// push_bit for a relaxation push: during a relaxation round the target
+// bitmaps (OUT, later ring buckets, overflow) only gain bits, so a set bit
+// seen by a plain load is set and the atomic can be skipped (a stale copy can
+// only miss bits, and then the atomic runs).
+__device__ __forceinline__ bool push_bit_relax(const RoundArgs &a, uint32_t buf, uint32_t list_sid, uint32_t item)
+{
+    const uint32_t m = 1u << (item & 31);
+    if (a.bits[(uint64_t)buf * a.bitmap_words + (item >> 5)] & m) return false;
+    return push_bit(a, buf, list_sid, item);
 }
 
 // A key of (slot, node) left KEY_INF: records the node's 2048-key chunk so
@@ -223,7 +223,7 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bo
     if (from_bitmap) atomicAdd(&s_slot_delta[slot], -1);
     const SlotInfo si = a.slots[slot];
     if (si.state != 1) return;
-    const uint64_t ku = atomic_load_u64(&a.keys[item]);
+    const uint64_t ku = __hip_atomic_load(&a.keys[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (ku == KEY_INF) return;
     {
         // Plain order: an item whose key has dropped below the current bucket
@@ -301,7 +301,7 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bo
                 else if (b < cur + nb) buf = s_ring[b % nb];
                 else buf = s_ovf;
                 if (buf == s_ovf && b < my_ovf_min) my_ovf_min = b;
-                if (push_bit(a, buf, s_lid[buf], slot * N + v[q])) {
+                if (push_bit_relax(a, buf, s_lid[buf], slot * N + v[q])) {
                     atomicAdd(&s_slot_delta[slot], 1);
                     atomicAdd(&s_buf_delta[buf], 1);
                 }
@@ -488,7 +488,7 @@ k_round(RoundArgs a)
                     bool live = false;
                     uint32_t dst_buf = 0;
                     if (si.state == 1) {
-                        const uint64_t ku = atomic_load_u64(&a.keys[item]);
+                        const uint64_t ku = __hip_atomic_load(&a.keys[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         if (ku != KEY_INF) {
                             const uint64_t b = si.base + key_bucket_rel(ku, a.inv_delta, heur(a, si, item - slot * N));
                             if (b >= cur) {
