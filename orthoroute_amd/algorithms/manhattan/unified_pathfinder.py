@@ -800,9 +800,13 @@ class PathFinderRouter:
     # ---------------- bookkeeping (UPF:4989-5007)
     def _update_net_edge_tracking(self, nid, edges, path=None):
         self._net_to_edges.pop(nid, None)
-        self._net_to_edges[nid] = np.asarray(edges, dtype=np.int64)
+        # int32 arrays from the engine are kept as they are (every consumer
+        # indexes or concatenates them); other inputs become int64
+        self._net_to_edges[nid] = edges if isinstance(edges, np.ndarray) and edges.dtype == np.int32 \
+            else np.asarray(edges, dtype=np.int64)
         if path is not None:
-            self._net_nodes[nid] = np.asarray(path, dtype=np.int64)
+            self._net_nodes[nid] = path if isinstance(path, np.ndarray) and path.dtype == np.int32 \
+                else np.asarray(path, dtype=np.int64)
 
     def _clear_net_edge_tracking(self, nid):
         self._net_to_edges.pop(nid, None)

@@ -223,10 +223,10 @@ def route(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=None
     (list of (nodes, edges, dist) or None per request, stats dict)."""
     n = len(requests)
     ctx.reserve(lat.N, params.slots, params.n_buckets)
-    reqs = (_abi.NetReq * max(n, 1))()
-    for i, (s, d, nid, roi, ko) in enumerate(requests):
-        reqs[i].src, reqs[i].dst, reqs[i].net_id = int(s), int(d), int(nid)
-        reqs[i].roi_index, reqs[i].use_keepout = int(roi), int(ko)
+    # orgpu_net_req rows (src, dst, net_id, roi_index, use_keepout, reserved)
+    reqs = np.zeros((max(n, 1), 6), dtype=np.int32)
+    if n:
+        reqs[:n, :5] = np.asarray(requests, dtype=np.int64)
     if path_capacity is None:
         path_capacity = max(1024, n * 4 * (lat.X + lat.Y + 2 * lat.L))
     nodes = np.empty(path_capacity, dtype=np.int32)
@@ -238,20 +238,23 @@ def route(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=None
     n_roi = 0 if roi_bitmaps is None else int(roi_bitmaps.shape[0])
     if roi_bitmaps is not None and roi_bitmaps.shape[1] * 32 < lat.N:
         raise ValueError("roi bitmap rows are shorter than the node count")
-    rc = ctx.lib.orgpu_route(ctx.handle, ctypes.byref(g), ctypes.cast(reqs, ctypes.c_void_p), n,
+    rc = ctx.lib.orgpu_route(ctx.handle, ctypes.byref(g), reqs.ctypes.data_as(ctypes.c_void_p), n,
                              _ptr(roi_bitmaps), n_roi, _ptr(keepout_owner), ctypes.c_float(KEEPOUT_COST),
                              ctypes.byref(params.c()), nodes.ctypes.data_as(ctypes.c_void_p),
                              edges.ctypes.data_as(ctypes.c_void_p), path_capacity,
                              offs.ctypes.data_as(ctypes.c_void_p), dist.ctypes.data_as(ctypes.c_void_p),
                              ctypes.byref(st))
     _abi.check(rc)
+    # views into compact copies of the used output (the capacity-sized
+    # buffers are released; a kept path holds only its call's paths)
+    used = int(offs[n])
+    nodes, edges = nodes[:used].copy(), edges[:used].copy()
     out = []
+    ol = offs.tolist()
+    dl = dist.tolist()
     for i in range(n):
-        a, b = offs[i], offs[i + 1]
-        if b - a > 1:
-            out.append((nodes[a:b].copy(), edges[a:b - 1].copy(), float(dist[i])))
-        else:
-            out.append(None)
+        a, b = ol[i], ol[i + 1]
+        out.append((nodes[a:b], edges[a:b - 1], dl[i]) if b - a > 1 else None)
     stats = {"rounds": st.rounds, "items": st.items, "edges": st.edges,
              "minimal_nodes": st.minimal_nodes, "minimal_edges": st.minimal_edges,
              "kernel_ms": st.kernel_ms, "launches": st.launches, "round_ms": st.round_ms}
