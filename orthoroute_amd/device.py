@@ -261,28 +261,47 @@ def route(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=None
     return out, stats
 
 
+MS_POINT = np.dtype([("node", "<i4"), ("cost", "<f4")])
+
+
+def _ms_points(pts):
+    """(node, cost) pairs -> orgpu_ms_point rows; pts is a sequence of pairs
+    or a (nodes, costs) pair of arrays."""
+    if isinstance(pts, tuple) and len(pts) == 2 and isinstance(pts[0], np.ndarray):
+        out = np.empty(len(pts[0]), dtype=MS_POINT)
+        out["node"], out["cost"] = pts[0], pts[1]
+        return out
+    out = np.empty(len(pts), dtype=MS_POINT)
+    if len(pts):
+        a = np.asarray(pts, dtype=np.float64).reshape(-1, 2)
+        out["node"], out["cost"] = a[:, 0], a[:, 1]
+    return out
+
+
 def route_ms(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=None, path_capacity=None):
     """Multi-source / multi-sink searches.  requests = list of (seeds,
     targets, net_id, roi_index, use_keepout) with seeds / targets lists of
-    (node, entry cost).  Returns (list of (nodes, edges, total, target_entry)
-    or None per request, stats dict); a chosen target that is itself a seed
-    gives a one-node path."""
+    (node, entry cost) pairs, or (nodes, costs) array pairs.  Returns (list
+    of (nodes, edges, total, target_entry) or None per request, stats
+    dict); a chosen target that is itself a seed gives a one-node path."""
     n = len(requests)
     ctx.reserve(lat.N, params.slots, params.n_buckets)
-    reqs = (_abi.MsReq * max(n, 1))()
-    seeds, targets = [], []
+    reqs = np.zeros((max(n, 1), 8), dtype=np.int32)
+    sl, tl = [], []
+    so = to = 0
     for i, (sd, tg, nid, roi, ko) in enumerate(requests):
-        reqs[i].seed_off, reqs[i].seed_n = len(seeds), len(sd)
-        reqs[i].target_off, reqs[i].target_n = len(targets), len(tg)
-        reqs[i].net_id, reqs[i].roi_index, reqs[i].use_keepout = int(nid), int(roi), int(ko)
-        seeds.extend(sd)
-        targets.extend(tg)
-    sarr = (_abi.MsPoint * max(len(seeds), 1))()
-    for i, (v, c) in enumerate(seeds):
-        sarr[i].node, sarr[i].cost = int(v), float(c)
-    tarr = (_abi.MsPoint * max(len(targets), 1))()
-    for i, (v, c) in enumerate(targets):
-        tarr[i].node, tarr[i].cost = int(v), float(c)
+        sp, tp = _ms_points(sd), _ms_points(tg)
+        reqs[i] = (so, len(sp), to, len(tp), int(nid), int(roi), int(ko), 0)
+        so += len(sp)
+        to += len(tp)
+        sl.append(sp)
+        tl.append(tp)
+    sarr = np.concatenate(sl) if sl else np.empty(0, MS_POINT)
+    tarr = np.concatenate(tl) if tl else np.empty(0, MS_POINT)
+    if len(sarr) == 0:
+        sarr = np.zeros(1, MS_POINT)
+    if len(tarr) == 0:
+        tarr = np.zeros(1, MS_POINT)
     if path_capacity is None:
         path_capacity = max(1024, n * 4 * (lat.X + lat.Y + 2 * lat.L))
     nodes = np.empty(path_capacity, dtype=np.int32)
@@ -296,20 +315,20 @@ def route_ms(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=N
     if roi_bitmaps is not None and roi_bitmaps.shape[1] * 32 < lat.N:
         raise ValueError("roi bitmap rows are shorter than the node count")
     vp = ctypes.c_void_p
-    rc = ctx.lib.orgpu_route_ms(ctx.handle, ctypes.byref(g), ctypes.cast(reqs, vp), n,
-                                ctypes.cast(sarr, vp), len(seeds), ctypes.cast(tarr, vp), len(targets),
+    rc = ctx.lib.orgpu_route_ms(ctx.handle, ctypes.byref(g), reqs.ctypes.data_as(vp), n,
+                                sarr.ctypes.data_as(vp), so, tarr.ctypes.data_as(vp), to,
                                 _ptr(roi_bitmaps), n_roi, _ptr(keepout_owner), ctypes.c_float(KEEPOUT_COST),
                                 ctypes.byref(params.c()), nodes.ctypes.data_as(vp), edges.ctypes.data_as(vp),
                                 path_capacity, offs.ctypes.data_as(vp), total.ctypes.data_as(vp),
                                 choice.ctypes.data_as(vp), ctypes.byref(st))
     _abi.check(rc)
+    used = int(offs[n])
+    nodes, edges = nodes[:used].copy(), edges[:used].copy()
     out = []
+    ol = offs.tolist()
     for i in range(n):
-        a, b = offs[i], offs[i + 1]
-        if b > a:
-            out.append((nodes[a:b].copy(), edges[a:b - 1].copy(), float(total[i]), int(choice[i])))
-        else:
-            out.append(None)
+        a, b = ol[i], ol[i + 1]
+        out.append((nodes[a:b], edges[a:b - 1], float(total[i]), int(choice[i])) if b > a else None)
     stats = {"rounds": st.rounds, "items": st.items, "edges": st.edges,
              "kernel_ms": st.kernel_ms, "launches": st.launches, "round_ms": st.round_ms}
     return out, stats

@@ -48,7 +48,9 @@ def grow_trees(ctx, graph, cost, nets, params, roi_rows=None, row_of=None):
         reqs = []
         for i in active:
             st = state[i]
-            reqs.append(([(v, 0.0) for v in st["tree"]], [(st["pins"][j], 0.0) for j in st["left"]], i,
+            tree = np.asarray(st["tree"], dtype=np.int32)
+            tgt = np.asarray([st["pins"][j] for j in st["left"]], dtype=np.int32)
+            reqs.append(((tree, np.zeros(len(tree), np.float32)), (tgt, np.zeros(len(tgt), np.float32)), i,
                          -1 if row_of is None else int(row_of[i]), 0))
         res, rs = route_ms(ctx, graph, cost, reqs, p, roi_bitmaps=roi_rows)
         stats["steps"] += 1
