@@ -413,7 +413,10 @@ class PathFinderRouter:
         self.config.layer_count = int(layers)
         names = self.config.layer_names
         if not names or len(names) < self.config.layer_count:
-            self.config.layer_names = (getattr(board, "layers", None) or
+            # layer names from the board (a list of names, or Layer objects)
+            board_names = getattr(board, "layer_names", None) or getattr(board, "layers", None) or []
+            board_names = [getattr(n, "name", n) for n in board_names]
+            self.config.layer_names = (board_names or
                                        (["F.Cu"] + [f"In{i}.Cu" for i in range(1, self.config.layer_count - 1)]
                                         + ["B.Cu"]))
         self._init_lattice(bounds, self.config.grid_pitch, self.config.layer_count)
@@ -1258,7 +1261,10 @@ class PathFinderRouter:
                     pass
             if iteration_cb:
                 try:
-                    iteration_cb(it, [], [], over_sum, over_cnt)
+                    # the current routing state as geometry (UPF:4066-4117;
+                    # no escape geometry to merge in this build)
+                    prov_tracks, prov_vias = self._generate_geometry_from_paths()
+                    iteration_cb(it, prov_tracks, prov_vias, over_sum, over_cnt)
                 except Exception as e:
                     logger.warning(f"[ITER {it}] iteration callback failed: {e}")
             if failed == 0 and over_sum == 0 and self._last_barrel_conflict_count == 0:

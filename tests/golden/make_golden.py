@@ -13,6 +13,8 @@ inputs and outputs of the functions on the PathFinder hot path:
   dijkstra.npz     SimpleDijkstra.find_path_roi             UPF:1801-1887
   costs.npz        EdgeAccountant update_costs / ema / history / overuse
   negotiation.npz  route_multiple_nets trajectory on a small board
+  serialization.json  .ORP writer / reader and .ORS writer on a synthetic
+                   board and geometry (orp_exporter.py, ors_exporter.py)
 
 The fixtures are data only: inputs and the outputs the reference produced.
 """
@@ -294,6 +296,108 @@ def gen_negotiation(upf, Board, Net, Pad, Coordinate, X=20, Y=20, L=6, n_nets=60
     return res
 
 
+def synthetic_board_spec(seed=11):
+    """A small board as plain data: components with pads (some without a
+    component id, one with a drill), nets over them, one unconnected pad."""
+    rng = np.random.default_rng(seed)
+    comps = []
+    for c in range(4):
+        pads = []
+        for k in range(3):
+            pads.append({"id": f"P{c}_{k}", "x": float(np.round(10 + 7 * c + rng.random() * 3, 3)),
+                         "y": float(np.round(20 + 4 * k + rng.random() * 2, 3)), "w": 0.6, "h": 0.4,
+                         "drill": 0.3 if (c, k) == (1, 2) else None, "layer": "F.Cu" if c % 2 == 0 else "B.Cu"})
+        comps.append({"id": f"C{c}", "ref": f"U{c + 1}", "x": 10.0 + 7 * c, "y": 24.0, "pads": pads})
+    nets = [("N1", "SIG1", ["P0_0", "P1_0", "P2_1"]), ("N2", "GND", ["P0_1", "P3_2"]),
+            ("N3", "VCC", ["P1_1", "P2_2", "P3_0", "P0_2"])]
+    return comps, nets
+
+
+def synthetic_geometry(seed=12):
+    rng = np.random.default_rng(seed)
+    tracks, vias = [], []
+    for i in range(30):
+        x, y = float(np.round(rng.random() * 50, 2)), float(np.round(rng.random() * 40, 2))
+        if i % 3 == 0:
+            tracks.append({"net": f"N{i % 4}", "layer": f"In{i % 3 + 1}.Cu", "x1": x, "y1": y,
+                           "x2": x + 1.2, "y2": y, "width": 0.24})
+        elif i % 3 == 1:
+            tracks.append({"net_id": f"N{i % 4}", "layer": "F.Cu", "start": [x, y], "end": [x, y + 2.0],
+                           "width": 0.2})
+        else:
+            tracks.append({"net": f"N{i % 4}", "layer": "B.Cu", "start_x": x, "start_y": y, "end_x": x + 3.0,
+                           "end_y": y + 4.0})
+    for i in range(12):
+        x, y = float(np.round(rng.random() * 50, 2)), float(np.round(rng.random() * 40, 2))
+        v = {"net": f"N{i % 5}", "x": x, "y": y, "from_layer": "F.Cu", "to_layer": f"In{i % 3 + 1}.Cu",
+             "diameter": 0.25, "drill": 0.15}
+        if i % 4 == 1:
+            v = {"net_id": f"N{i % 5}", "position": [x, y], "from_layer": "In1.Cu", "to_layer": "B.Cu",
+                 "size": 0.5, "drill": 0.25}
+        if i % 4 == 2:
+            v = {"net": f"N{i % 5}", "position": {"x": x, "y": y}, "from_layer": "In2.Cu", "to_layer": "In3.Cu"}
+        vias.append(v)
+    metrics = [{"iteration": 1, "overuse": 40, "routed": 3, "overflow": 12.5, "time": 0.5},
+               {"iteration": 2, "overuse_count": 0, "nets_routed": 3, "overflow_cost": 0.0, "wirelength": 88.0,
+                "via_count": 12, "iteration_time": 0.25}]
+    metadata = {"total_iterations": 2, "converged": True, "total_time": 0.75, "board_name": "synthetic",
+                "notes": "fixture"}
+    return tracks, vias, metrics, metadata
+
+
+def gen_serialization(root):
+    """.ORP / .ORS through the reference's own writers and reader."""
+    import tempfile
+    from orthoroute.domain.models.board import Board, Component, Coordinate, Layer, Net, Pad
+    from orthoroute.infrastructure.serialization import orp_exporter, ors_exporter
+    comps, nets = synthetic_board_spec()
+    pad_obj, components = {}, []
+    for c in comps:
+        pads = [Pad(id=p["id"], component_id=c["id"], net_id=None, position=Coordinate(p["x"], p["y"]),
+                    size=(p["w"], p["h"]), drill_size=p["drill"], layer=p["layer"]) for p in c["pads"]]
+        for p in pads:
+            pad_obj[p.id] = p
+        components.append(Component(id=c["id"], reference=c["ref"], value="", footprint="",
+                                    position=Coordinate(c["x"], c["y"]), pads=pads))
+    net_objs = [Net(id=nid, name=name, pads=[pad_obj[p] for p in pids]) for nid, name, pids in nets]
+    layers = [Layer(name=n, type="signal", stackup_position=i) for i, n in enumerate(["F.Cu", "In1.Cu", "In2.Cu", "B.Cu"])]
+    board = Board(id="B1", name="synthetic.kicad_pcb", components=components, nets=net_objs, layers=layers,
+                  layer_count=4)
+    tracks, vias, metrics, metadata = synthetic_geometry()
+
+    class Geo:
+        pass
+    geo = Geo()
+    geo.tracks, geo.vias = tracks, vias
+    with tempfile.TemporaryDirectory() as d:
+        orp_exporter.export_board_to_orp(board, os.path.join(d, "b.orp"))
+        orp = orp_exporter.import_board_from_orp(os.path.join(d, "b.orp"))
+        b2 = orp_exporter.convert_orp_to_board(orp)
+        ors_exporter.export_solution_to_ors(geo, metrics, metadata, os.path.join(d, "s.ors"))
+        with __import__("gzip").open(os.path.join(d, "s.ors"), "rt") as f:
+            ors = json.load(f)
+        g2, m2 = ors_exporter.import_solution_from_ors(os.path.join(d, "s.ors"))
+        payload = ors_exporter.convert_ors_to_geometry_payload(g2)
+    for sec in (orp["metadata"], ors["metadata"]):
+        sec.pop("export_timestamp", None)
+    board_view = {
+        "id": b2.id, "name": b2.name, "layer_count": b2.layer_count, "thickness": b2.thickness,
+        "layer_names": b2.layer_names, "kicad_bounds": list(b2._kicad_bounds),
+        "components": [{"id": c.id, "reference": c.reference, "x": c.position.x, "y": c.position.y,
+                        "layer": c.layer, "pads": [p.id for p in c.pads]} for c in b2.components],
+        "nets": [{"id": n.id, "name": n.name, "netclass": n.netclass, "pads": [p.id for p in n.pads]}
+                 for n in b2.nets],
+        "pads": {p.id: {"component_id": p.component_id, "net_id": p.net_id, "x": p.position.x, "y": p.position.y,
+                        "size": list(p.size), "drill": p.drill_size, "layer": p.layer}
+                 for c in b2.components for p in c.pads},
+    }
+    out = {"board_spec": {"components": comps, "nets": nets}, "orp": orp, "orp_board": board_view,
+           "geometry_in": {"tracks": tracks, "vias": vias, "metrics": metrics, "metadata": metadata},
+           "ors": ors, "payload": {"tracks": payload.tracks, "vias": payload.vias}}
+    with open(os.path.join(HERE, "serialization.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 def main():
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     which = sys.argv[2:] or ["lattice", "roi", "dijkstra", "costs", "negotiation"]
@@ -309,6 +413,8 @@ def main():
         gen_costs(upf); print("costs done")
     if "multisource" in which:
         gen_multisource(upf); print("multisource done")
+    if "serialization" in which:
+        gen_serialization(root); print("serialization done")
     if "negotiation" in which:
         r = gen_negotiation(upf, Board, Net, Pad, Coordinate)
         print("negotiation", {k: r.get(k) for k in ("success", "overuse_sum", "overuse_edges", "failed_nets")})

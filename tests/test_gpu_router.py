@@ -64,3 +64,42 @@ def test_negotiation_matches_oracle(case, slots):
     for nm, *_ in reqs:
         assert list(g_res["paths"].get(nm, [])) == list(o_res["paths"].get(nm, [])), nm
     assert bool(g_res.get("success")) == bool(o_res.get("success"))
+
+
+def test_headless_orp_to_ors(tmp_path):
+    """A board written as .ORP, routed headless, the solution written as .ORS:
+    the file holds the router's own geometry and one metrics row per
+    iteration."""
+    from orthoroute_amd import serialization as S
+    from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig
+    from orthoroute_amd.board import Board, Component, Coordinate, Layer, Net, Pad
+    from orthoroute_amd.headless import route_orp_file
+    print("[router] headless ORP -> ORS", flush=True)
+    rng = np.random.default_rng(8)
+    pads, comps = [], []
+    for c in range(6):
+        cp = [Pad(id=f"P{c}_{k}", component_id=f"C{c}", net_id=None,
+                  position=Coordinate(float(2 + 3 * c), float(2 + 2 * k)), size=(0.5, 0.5)) for k in range(4)]
+        pads += cp
+        comps.append(Component(id=f"C{c}", reference=f"U{c}", value="", footprint="",
+                               position=Coordinate(2.0 + 3 * c, 5.0), pads=cp))
+    order = rng.permutation(len(pads))
+    nets = [Net(id=f"N{i}", name=f"SIG{i}", pads=[pads[order[2 * i]], pads[order[2 * i + 1]]]) for i in range(8)]
+    layers = [Layer(name=n, type="signal", stackup_position=i) for i, n in enumerate(["F.Cu", "In1.Cu", "In2.Cu", "B.Cu"])]
+    board = Board(id="B", name="t.kicad_pcb", components=comps, nets=nets, layers=layers, layer_count=4)
+    S.export_board_to_orp(board, tmp_path / "t.orp")
+    cfg = PathFinderConfig()
+    cfg.grid_pitch = 0.5
+    cfg.max_iterations_cap = 20
+    router, result = route_orp_file(tmp_path / "t.orp", tmp_path / "t.ors", config=cfg)
+    geo, meta = S.import_solution_from_ors(tmp_path / "t.ors")
+    payload = router.get_geometry_payload()
+    assert meta["statistics"]["total_tracks"] == len(payload.tracks)
+    assert meta["statistics"]["total_vias"] == len(payload.vias)
+    assert len(meta["iteration_metrics"]) >= 1
+    assert meta["metadata"]["converged"] == bool(result["converged"])
+    if result["success"]:
+        # a converged routing exports its geometry; one with overuse exports none
+        assert len(payload.tracks) + len(payload.vias) > 0
+    routed = [n for n, p in result["paths"].items() if p]
+    assert len(routed) >= 1
