@@ -1,0 +1,76 @@
+"""Parity at the benchmark's full sizes: the 600x400x18 backplane (4.32 M
+nodes, 16.8 M edges) routed by the batched GPU engine on its iteration-1
+cost snapshot against the oracle's ROI + canonical search for a sample of the
+same nets, and cfg3's multi-terminal trees (300x300x12) against the oracle's
+trees."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_backplane_sample_matches_oracle():
+    from orthoroute_amd import workloads
+    from orthoroute_amd.algorithms.manhattan.unified_pathfinder import (PathFinderConfig, PathFinderRouter,
+                                                                        ROI_THRESHOLD_STEPS)
+    X, Y, L, reqs = workloads.cfg2()
+    say(f"[fullsize] backplane {X}x{Y}x{L}, {len(reqs)} nets")
+    cfg = PathFinderConfig()
+    cfg.lattice_policy = "adjacent"
+    cfg.batch_slots = 512
+    r = PathFinderRouter(cfg)
+    r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
+    tasks = r._parse_requests(reqs)
+    pres_fac, _, _, _, hmult = r._negotiation_setup(tasks)
+    r.iteration = 1
+    r._begin_iteration(1, pres_fac, hmult)
+    # the first 16 nets of each kind: ROI-limited and whole-graph searches
+    short, long_ = [], []
+    for name, (s, d) in tasks.items():
+        sx, sy, _ = r._coords(s)
+        dx, dy, _ = r._coords(d)
+        near = abs(dx - sx) + abs(dy - sy) < ROI_THRESHOLD_STEPS
+        (short if near else long_).append((s, d, -1, near, 0.0))
+    sample = short[:16] + long_[:6]
+    got = r._search_batch(sample)
+    lat = oracle.Lattice(X, Y, L, "adjacent", pitch=1.0, via_cost=3.0)
+    costs = r.accounting.total_cost[:r.accounting.E].cpu().numpy()
+    for i, ((s, d, _, near, bonus), g) in enumerate(zip(sample, got)):
+        mask = None
+        if near:
+            mask = np.zeros(lat.N, np.uint8)
+            mask[oracle.roi_bfs(lat, s, d, stagnation_bonus=bonus)] = 1
+            mask[s] = mask[d] = 1
+        path = oracle.sssp_canon(lat, costs, s, d, roi_mask=mask)[0]
+        assert (None if g is None else g[0].tolist()) == path, f"net {i} ({s}->{d}, roi={near})"
+
+
+def test_cfg3_multi_terminal_sample_matches_oracle():
+    from orthoroute_amd import workloads
+    from orthoroute_amd.algorithms.manhattan.multi_terminal import MultiTerminalNetRouter
+    from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig, PathFinderRouter
+    X, Y, L, nets = workloads.cfg3_multi()
+    say(f"[fullsize] cfg3 multi-terminal {X}x{Y}x{L}")
+    cfg = PathFinderConfig()
+    cfg.lattice_policy = "adjacent"
+    r = PathFinderRouter(cfg)
+    r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
+    mtr = MultiTerminalNetRouter(r)
+    res = mtr.route_all_nets(nets)
+    assert res["_validation"]["total_nets"] == len(nets)
+    lat = oracle.Lattice(X, Y, L, "adjacent", pitch=1.0, via_cost=3.0)
+    base = r.graph.dlat.base_cost[:lat.E].cpu().numpy()
+    for name, pins in nets[:3]:
+        branches, order, unreached, nodes = oracle.steiner_canon(lat, base, pins)
+        t = mtr.trees[name]
+        assert (t["branches"], t["order"], t["unreached"]) == (branches, order, unreached), name
+        assert res[name] == nodes
