@@ -93,11 +93,19 @@ k_roi(RoiArgs a)
 #pragma unroll
                         for (int q = 0; q < 4; ++q)
                             if (ok[q] && bit_get(other, v[q])) { s_met = 1; ok[q] = false; }
+                        // a node this wave already holds needs no atomic: bits
+                        // of `mine` are only set during a net's search, and
+                        // its clears (at the start of the net for A, at the
+                        // end of the previous net for B) are this workgroup's
+                        // own stores
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
                             m[q] = 1u << (v[q] & 31);
-                            old[q] = ok[q] ? atomicOr(&mine[v[q] >> 5], m[q]) : m[q];
+                            if (ok[q] && (mine[v[q] >> 5] & m[q])) ok[q] = false;
                         }
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            old[q] = ok[q] ? atomicOr(&mine[v[q] >> 5], m[q]) : m[q];
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
                             if (!(old[q] & m[q])) {
@@ -124,7 +132,13 @@ k_roi(RoiArgs a)
         for (int i = tid; i < nb_total; i += RWG) {
             const int32_t v = LB[i];
             atomicOr(&A[v >> 5], 1u << (v & 31));
-            atomicAnd(&B[v >> 5], ~(1u << (v & 31)));
+        }
+        __syncthreads();
+        // clear the scratch words the dst wave used (whole words: every bit
+        // of B belongs to this net)
+        for (int i = tid; i < nb_total; i += RWG) {
+            const int32_t v = LB[i];
+            B[v >> 5] = 0u;
         }
         __syncthreads();
         const long long seen = s_seen_i;

@@ -58,11 +58,13 @@ def test_roi_random_truncation(ctx):
     import torch
     from orthoroute_amd.device import roi_bfs
     rng = np.random.default_rng(4)
-    # the last shape spans several 4096-word chunks of the ROI bitmap
-    for policy, (X, Y, L) in [("reference", (40, 30, 8)), ("adjacent", (50, 44, 6)), ("adjacent", (180, 150, 8))]:
+    # the (180, 150, 8) shape spans several 4096-word chunks of the ROI
+    # bitmap; 1,100 nets on the last shape exceed the 1,024 concurrent
+    # workgroups, so workgroups run several nets in turn on their scratch
+    for policy, (X, Y, L), n in [("reference", (40, 30, 8), 40), ("adjacent", (50, 44, 6), 40),
+                                 ("adjacent", (180, 150, 8), 40), ("reference", (30, 24, 6), 1100)]:
         lat = ctx.build_lattice(X, Y, L, policy)
         o = oracle.Lattice(X, Y, L, policy)
-        n = 40
         words = (lat.N + 31) // 32
         rows = torch.zeros((n, words), dtype=torch.int32, device=ctx.device)
         reqs = []
