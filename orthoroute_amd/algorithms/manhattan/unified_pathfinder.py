@@ -296,6 +296,15 @@ class ROIExtractor:
         radius = max(60, int(md * 0.75 + stagnation_bonus * 2.0))
         return min(int(radius + stagnation_bonus * 2.0), 150)
 
+    def roi_depth_many(self, src, dst, bonus):
+        """roi_depth for arrays of requests (same float64 arithmetic)."""
+        X, plane = self.lattice.x_steps, self.lattice.x_steps * self.lattice.y_steps
+        rs, rd = np.asarray(src, np.int64) % plane, np.asarray(dst, np.int64) % plane
+        md = np.abs(rd % X - rs % X) + np.abs(rd // X - rs // X)
+        b2 = np.asarray(bonus, np.float64) * 2.0
+        radius = np.maximum(60, np.floor(md * 0.75 + b2).astype(np.int64))
+        return np.minimum(np.floor(radius + b2).astype(np.int64), 150)
+
     def extract_roi(self, src, dst, initial_radius=40, stagnation_bonus=0.0, portal_seeds=None):
         torch = self.ctx.torch
         words = (self.N + 31) // 32
@@ -969,38 +978,40 @@ class PathFinderRouter:
                             h_xy=hx, h_z=hz, local_batches=int(self.config.sssp_local_batches or 0))
 
     def _search_batch(self, reqs, keepout_owner=None, readd=True):
-        """reqs: list of (src, dst, net_int, roi, stagnation_bonus).  roi
-        False = whole graph.  Returns per request (nodes, edges) as int32
-        arrays, or None."""
+        """reqs: (src, dst, net_int, roi, stagnation_bonus) per request, as a
+        list of tuples or an (n, 5) array.  roi False = whole graph.  Returns
+        per request (nodes, edges) as int32 arrays, or None."""
         torch = self.ctx.torch
         dlat = self.graph.dlat
-        out = [None] * len(reqs)
-        roi_reqs, rows_of = [], {}
-        for i, (s, d, nid, roi, bonus) in enumerate(reqs):
-            if roi:
-                rows_of[i] = len(roi_reqs)
-                roi_reqs.append((s, d, self.roi_extractor.roi_depth(s, d, bonus), MAX_ROI_NODES, len(roi_reqs)))
+        R = np.asarray(reqs, dtype=np.float64).reshape(-1, 5)
+        n = len(R)
+        out = [None] * n
+        src, dst = R[:, 0].astype(np.int64), R[:, 1].astype(np.int64)
+        nid, use_roi, bonus = R[:, 2].astype(np.int64), R[:, 3] != 0, R[:, 4]
+        roi_at = np.nonzero(use_roi)[0]
+        row_of = np.full(n, -1, dtype=np.int64)
+        row_of[roi_at] = np.arange(len(roi_at))
         rows = None
-        skip = set()
+        keep = np.ones(n, dtype=bool)
         t0 = time.perf_counter()
-        if roi_reqs:
+        if len(roi_at):
             words = (self.N + 31) // 32
-            rows = torch.empty((len(roi_reqs), words), dtype=torch.int32, device=self.ctx.device)
-            info = roi_bfs(self.ctx, dlat, roi_reqs, rows, readd=readd)
-            for i, r in rows_of.items():
-                if not (info[r][4] and info[r][5]):
-                    skip.add(i)
+            rows = torch.empty((len(roi_at), words), dtype=torch.int32, device=self.ctx.device)
+            depth = self.roi_extractor.roi_depth_many(src[roi_at], dst[roi_at], bonus[roi_at])
+            rq = np.stack([src[roi_at], dst[roi_at], depth, np.full(len(roi_at), MAX_ROI_NODES, np.int64),
+                           np.arange(len(roi_at))], axis=1)
+            info = roi_bfs(self.ctx, dlat, rq, rows, readd=readd)
+            ok = np.fromiter((a[4] and a[5] for a in info), dtype=bool, count=len(info))
+            keep[roi_at[~ok]] = False
         self.stats["roi_s"] += time.perf_counter() - t0
-        todo = [i for i in range(len(reqs)) if i not in skip]
-        if not todo:
+        todo = np.nonzero(keep)[0]
+        if not len(todo):
             return out
         # a work item is the u32 slot*N + node: at most (2^32 - 64) / N slots
         slots = max(1, min(int(self.config.batch_slots), len(todo), ((1 << 32) - 64) // self.N - 1))
         params = self._search_params(slots)
-        sreqs = []
-        for i in todo:
-            s, d, nid, roi, _ = reqs[i]
-            sreqs.append((s, d, nid, rows_of[i] if roi else -1, int(keepout_owner is not None)))
+        sreqs = np.stack([src[todo], dst[todo], nid[todo], row_of[todo],
+                          np.full(len(todo), int(keepout_owner is not None), np.int64)], axis=1)
         ko = None
         if keepout_owner is not None:
             ko = torch.from_numpy(keepout_owner).to(self.ctx.device)
@@ -1017,7 +1028,7 @@ class PathFinderRouter:
         self.stats["sssp_minimal_nodes"] += st["minimal_nodes"]
         self.stats["sssp_minimal_edges"] += st["minimal_edges"]
         self.stats["sssp_searches"] += len(sreqs)
-        for j, i in enumerate(todo):
+        for j, i in enumerate(todo.tolist()):
             r = res[j]
             out[i] = None if r is None else (r[0], r[1])
         return out
@@ -1033,41 +1044,50 @@ class PathFinderRouter:
         total = len(tasks)
         use_ko = total < 100 and len(self._ko_nodes) > 0
         ko_owner = self._keepout_owner_array() if use_ko else None
-        sd = np.asarray([tasks[nid] for nid in ordered], dtype=np.int64).reshape(-1, 2)
+        n = len(ordered)
+        sd = np.fromiter((v for nid in ordered for v in tasks[nid]), dtype=np.int64, count=2 * n).reshape(n, 2)
         X, plane = self._Nx, self._Nx * self._Ny
         rs, rd = sd[:, 0] % plane, sd[:, 1] % plane
         md = np.abs(rd % X - rs % X) + np.abs(rd // X - rs // X)
-        nids = [self._get_net_id(nid) for nid in ordered] if use_ko else [-1] * len(ordered)
-        reqs = list(zip(sd[:, 0].tolist(), sd[:, 1].tolist(), nids, (md < ROI_THRESHOLD_STEPS).tolist(),
-                        [bonus] * len(ordered)))
+        reqs = np.empty((n, 5), dtype=np.float64)
+        reqs[:, 0], reqs[:, 1] = sd[:, 0], sd[:, 1]
+        reqs[:, 2] = [self._get_net_id(nid) for nid in ordered] if use_ko else -1
+        reqs[:, 3] = md < ROI_THRESHOLD_STEPS
+        reqs[:, 4] = bonus
         results = self._search_batch(reqs, ko_owner, readd=True)
         # wider-ROI fallback for the first five failures, in routing order
         fb = [i for i, r in enumerate(results) if r is None or len(r[0]) <= 1][:5]
         self.full_graph_fallback_count = len(fb)
         if fb:
-            freqs = [(reqs[i][0], reqs[i][1], -1, True, bonus * 2) for i in fb]
+            freqs = [(int(reqs[i, 0]), int(reqs[i, 1]), -1, True, bonus * 2) for i in fb]
             fres = self._search_batch(freqs, None, readd=False)
             for i, r in zip(fb, fres):
                 results[i] = r
         # commit in routing order (UPF:4783-4812); usage counts are integer
         # adds, so the per-net +1/-1 scatters run as two batched launches
+        # (the per-net bookkeeping of _clear_net_edge_tracking /
+        # _update_net_edge_tracking, inlined: this loop runs over every net)
         dec, inc, marked, mark_ids = [], [], [], []
+        paths, n2e, n2n, get_id = self.net_paths, self._net_to_edges, self._net_nodes, self._get_net_id
         for nid, r in zip(ordered, results):
-            if nid in self.net_paths and self.net_paths[nid]:
-                old = self._net_to_edges.get(nid)
+            if paths.get(nid):
+                old = n2e.pop(nid, None)
                 if old is not None:
                     dec.append(old)
-                self._clear_net_edge_tracking(nid)
+                n2n.pop(nid, None)
             if r is not None and len(r[0]) > 1:
                 path, edges = r
                 inc.append(edges)
-                self.net_paths[nid] = path.tolist()
+                paths[nid] = path.tolist()
                 marked.append(path)
-                mark_ids.append(self._get_net_id(nid))
-                self._update_net_edge_tracking(nid, edges, path)
+                mark_ids.append(get_id(nid))
+                n2e.pop(nid, None)
+                n2e[nid] = edges
+                n2n[nid] = path
             else:
-                self.net_paths[nid] = []
-                self._clear_net_edge_tracking(nid)
+                paths[nid] = []
+                n2e.pop(nid, None)
+                n2n.pop(nid, None)
         if dec:
             self.accounting.clear_path(np.concatenate(dec))
         if inc:

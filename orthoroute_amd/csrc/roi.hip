@@ -14,6 +14,7 @@
 // ascending), minus the nodes its must-keep swap pushes out (positions 0/1 when
 // src/dst fall beyond the cut), then src and dst are re-added.
 #include <algorithm>
+#include <vector>
 
 #include "common.h"
 #include "engine.h"
@@ -266,15 +267,17 @@ extern "C" int orgpu_roi_bfs(orgpu_t *h, const orgpu_graph *g, const orgpu_roi_r
     if (n == 0) return ORGPU_OK;
     const int64_t N = g->n_nodes;
     if (N <= 0 || N >= KEY_MAX_NODES || !g->indptr || !g->indices) return orgpu_fail(ORGPU_EINVAL, "roi_bfs: bad graph");
+    std::vector<int> row_user(n_roi > 0 ? n_roi : 0, -1);
     for (int i = 0; i < n; ++i) {
         const orgpu_roi_req &q = reqs[i];
         if (q.src < 0 || q.src >= N || q.dst < 0 || q.dst >= N || q.roi_index < 0 || q.roi_index >= n_roi ||
             q.max_nodes < 1 || q.max_depth < 0)
             return orgpu_fail(ORGPU_EINVAL, "roi_bfs: request %d invalid (src %d dst %d row %d of %d)", i, q.src, q.dst,
                               q.roi_index, n_roi);
-        for (int j = 0; j < i; ++j)
-            if (reqs[j].roi_index == q.roi_index)
-                return orgpu_fail(ORGPU_EINVAL, "roi_bfs: requests %d and %d share bitmap row %d", j, i, q.roi_index);
+        if (row_user[q.roi_index] >= 0)
+            return orgpu_fail(ORGPU_EINVAL, "roi_bfs: requests %d and %d share bitmap row %d", row_user[q.roi_index], i,
+                              q.roi_index);
+        row_user[q.roi_index] = i;
     }
     HIP_CHECK_RET(hipSetDevice(h->device));
     Engine &en = h->en;

@@ -262,6 +262,13 @@ def route(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=None
 
 
 MS_POINT = np.dtype([("node", "<i4"), ("cost", "<f4")])
+# orgpu_roi_req / orgpu_roi_info rows (the ctypes layouts of _abi.RoiReq / RoiInfo)
+ROI_REQ = np.dtype({"names": ["src", "dst", "max_depth", "roi_index", "max_nodes", "readd", "reserved"],
+                    "formats": ["<i4", "<i4", "<i4", "<i4", "<i8", "<i4", "<i4"],
+                    "offsets": [0, 4, 8, 12, 16, 24, 28], "itemsize": 32})
+ROI_INFO = np.dtype({"names": ["seen", "kept", "depth", "met", "src_in", "dst_in"],
+                     "formats": ["<i8", "<i8", "<i4", "<i4", "<i4", "<i4"],
+                     "offsets": [0, 8, 16, 20, 24, 28], "itemsize": 32})
 
 
 def _ms_points(pts):
@@ -343,17 +350,18 @@ def roi_bfs(ctx, lat, requests, rows, readd=True):
     n = len(requests)
     if n == 0:
         return []
-    reqs = (_abi.RoiReq * n)()
-    for i, (s, d, depth, mx, row) in enumerate(requests):
-        reqs[i].src, reqs[i].dst, reqs[i].max_depth = int(s), int(d), int(depth)
-        reqs[i].roi_index, reqs[i].max_nodes = int(row), int(mx)
-        reqs[i].readd = int(bool(readd))
-    info = (_abi.RoiInfo * n)()
+    q = np.asarray(requests, dtype=np.int64).reshape(n, 5)
+    reqs = np.zeros(n, dtype=ROI_REQ)
+    reqs["src"], reqs["dst"], reqs["max_depth"] = q[:, 0], q[:, 1], q[:, 2]
+    reqs["max_nodes"], reqs["roi_index"] = q[:, 3], q[:, 4]
+    reqs["readd"] = int(bool(readd))
+    info = np.zeros(n, dtype=ROI_INFO)
     g = lat.graph(lat.base_cost)
-    _abi.check(ctx.lib.orgpu_roi_bfs(ctx.handle, ctypes.byref(g), ctypes.cast(reqs, ctypes.c_void_p), n,
-                                     _ptr(rows), int(rows.shape[0]), ctypes.cast(info, ctypes.c_void_p)))
-    return [(info[i].seen, info[i].kept, info[i].depth, info[i].met, bool(info[i].src_in), bool(info[i].dst_in))
-            for i in range(n)]
+    vp = ctypes.c_void_p
+    _abi.check(ctx.lib.orgpu_roi_bfs(ctx.handle, ctypes.byref(g), reqs.ctypes.data_as(vp), n,
+                                     _ptr(rows), int(rows.shape[0]), info.ctypes.data_as(vp)))
+    return list(zip(info["seen"].tolist(), info["kept"].tolist(), info["depth"].tolist(), info["met"].tolist(),
+                    (info["src_in"] != 0).tolist(), (info["dst_in"] != 0).tolist()))
 
 
 class Sparse:
