@@ -134,11 +134,17 @@ __device__ __forceinline__ bool push_bit_relax(const RoundArgs &a, uint32_t buf,
     return push_bit(a, buf, list_sid, item);
 }
 
-// A key of (slot, node) left KEY_INF: records the node's 2048-key chunk so
-// that reusing the slot resets only the chunks its search touched.
+// A key of (slot, node) left KEY_INF: records the node's chunk of
+// 2^TOUCH_SHIFT keys so that reusing the slot resets only the chunks its
+// search touched.  1024-key chunks (8 KB) keep the reset close to the keys
+// a search set (it covers a compact region on each of many layers, a few
+// hundred nodes of every lattice row it crosses) while first touches, one
+// atomic each, stay rare.
+#define TOUCH_SHIFT 10
+#define TOUCH_KEYS (1u << TOUCH_SHIFT)
 __device__ __forceinline__ void touch_key(const RoundArgs &a, uint32_t slot, uint32_t node)
 {
-    const uint32_t ch = node >> 11;
+    const uint32_t ch = node >> TOUCH_SHIFT;
     const uint32_t m = 1u << (ch & 31);
     uint32_t *w = a.touch_bits + (uint64_t)slot * a.touch_words + (ch >> 5);
     if (*w & m) return;
@@ -743,9 +749,10 @@ struct SeedReq {
     int32_t s_off, s_n, t_off, t_n;   // multi-source: seeds / targets in the call's arrays
 };
 
-// Resets the keys of the slots about to be seeded: only the 2048-key chunks
-// their previous searches touched (each workgroup fills whole chunks).  The
-// touch records themselves are cleared by k_seed, which runs after this.
+// Resets the keys of the slots about to be seeded: only the chunks their
+// previous searches touched (each workgroup fills whole chunks), and clears
+// those chunks' touch bits (whole words: every set bit of the slot's record
+// is a listed chunk).  k_seed, which runs after this, restarts the lists.
 __global__ void k_reset_keys(RoundArgs a, const SeedReq *reqs, int n)
 {
     const uint64_t N = (uint64_t)a.N;
@@ -754,10 +761,13 @@ __global__ void k_reset_keys(RoundArgs a, const SeedReq *reqs, int n)
         const uint32_t cnt = a.touch_n[s];
         const uint32_t *lst = a.touch_list + (uint64_t)s * a.touch_cap;
         uint64_t *ks = a.keys + (uint64_t)s * N;
+        uint32_t *tb = a.touch_bits + (uint64_t)s * a.touch_words;
         for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
-            const uint64_t lo = (uint64_t)lst[i] * 2048u;
-            const uint64_t hi = lo + 2048u < N ? lo + 2048u : N;
+            const uint32_t ch = lst[i];
+            const uint64_t lo = (uint64_t)ch * TOUCH_KEYS;
+            const uint64_t hi = lo + TOUCH_KEYS < N ? lo + TOUCH_KEYS : N;
             for (uint64_t v = lo + threadIdx.x; v < hi; v += blockDim.x) ks[v] = KEY_INF;
+            if (threadIdx.x == 0) tb[ch >> 5] = 0u;
         }
     }
 }
@@ -796,8 +806,8 @@ __global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n, const orgpu_ms_p
         } else {
             si->dx = si->dy = si->dz = 0;
         }
-        // the slot's keys were reset (k_reset_keys): start a new touch record
-        for (int64_t w = 0; w < a.touch_words; ++w) a.touch_bits[(uint64_t)r.slot * a.touch_words + w] = 0u;
+        // the slot's keys and touch bits were reset (k_reset_keys): start a
+        // new touch list
         a.touch_n[r.slot] = 0u;
         // a slot is reseeded only once its previous search has no set bits
         // left, so its items' bits are clear here
@@ -854,77 +864,97 @@ __global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n, const orgpu_ms_p
     }
 }
 
-// One thread per finished slot: walk predecessors dst -> src into the
-// staging row (reversed), record the CSR edge id of each hop (first match in
-// the sorted row, UPF:5463-5476).  len = 0: unreachable; -1: broken chain or
-// longer than the staging row.
-__global__ void k_backtrace(RoundArgs a, const int32_t *slot_list, int n, int32_t *stage_nodes,
-                            int32_t *stage_edges, int32_t *lens, float *dists, int32_t max_len,
-                            double *ms_total, int32_t *ms_choice,
-                            int32_t *pack, uint32_t *pack_n, uint32_t *pack_off, int64_t pack_half)
+// One wave per finished slot.  Lane 0 picks the end node (the destination,
+// or the chosen target of a multi-source search) and walks the predecessor
+// fields back to the source into the staging row (one dependent key load per
+// hop); then the lanes find each hop's CSR edge id (first match in the sorted
+// row, UPF:5463-5476) and pack the path src -> dst into the shared output.
+// len = 0: unreachable; -1: broken chain or longer than the staging row.
+__global__ void __launch_bounds__(64)
+k_backtrace(RoundArgs a, const int32_t *slot_list, int n, int32_t *stage_nodes, int32_t *stage_edges,
+            int32_t *lens, float *dists, int32_t max_len, double *ms_total, int32_t *ms_choice,
+            int32_t *pack, uint32_t *pack_n, uint32_t *pack_off, int64_t pack_half)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int s_len;
+    __shared__ uint32_t s_off;
+    const int i = blockIdx.x;
     if (i >= n) return;
+    const int lane = (int)threadIdx.x;
     const int s = slot_list[i];
     const SlotInfo si = a.slots[s];
     int32_t *pn = stage_nodes + (int64_t)s * max_len;
     int32_t *pe = stage_edges + (int64_t)s * max_len;
     const uint64_t *ks = a.keys + (uint64_t)s * (uint64_t)a.N;
-    if (si.state != 2) { lens[s] = 0; dists[s] = __builtin_inff(); return; }
-    int32_t end = si.dst;
-    if (si.ms) {
-        // the cheapest target by (keepout edges, dist + target cost), then
-        // dist, node index, entry order
-        int best = -1;
-        uint32_t bko = 0;
-        double bt = 0.0;
-        float bd = 0.0f;
-        int32_t bn = 0;
-        for (int t = 0; t < si.t_n; ++t) {
-            const orgpu_ms_point tp = a.ms_tgt[si.t_off + t];
-            const uint64_t k = ks[tp.node];
-            if (k == KEY_INF) continue;
-            const uint32_t ko = key_ko(k);
-            const float d = key_dist(k);
-            const double tot = (double)d + (double)tp.cost;
-            const bool better = best < 0 || ko < bko ||
-                                (ko == bko && (tot < bt || (tot == bt && (d < bd || (d == bd && tp.node < bn)))));
-            if (better) { best = t; bko = ko; bt = tot; bd = d; bn = tp.node; }
+    if (lane == 0) {
+        int len = 0;
+        int32_t end = si.dst;
+        bool found = si.state == 2;
+        if (found && si.ms) {
+            // the cheapest target by (keepout edges, dist + target cost), then
+            // dist, node index, entry order
+            int best = -1;
+            uint32_t bko = 0;
+            double bt = 0.0;
+            float bd = 0.0f;
+            int32_t bn = 0;
+            for (int t = 0; t < si.t_n; ++t) {
+                const orgpu_ms_point tp = a.ms_tgt[si.t_off + t];
+                const uint64_t k = ks[tp.node];
+                if (k == KEY_INF) continue;
+                const uint32_t ko = key_ko(k);
+                const float d = key_dist(k);
+                const double tot = (double)d + (double)tp.cost;
+                const bool better = best < 0 || ko < bko ||
+                                    (ko == bko && (tot < bt || (tot == bt && (d < bd || (d == bd && tp.node < bn)))));
+                if (better) { best = t; bko = ko; bt = tot; bd = d; bn = tp.node; }
+            }
+            if (ms_choice) ms_choice[s] = best;
+            if (ms_total) ms_total[s] = best >= 0 ? bt : (double)__builtin_inff();
+            found = best >= 0;
+            end = bn;
         }
-        if (ms_choice) ms_choice[s] = best;
-        if (ms_total) ms_total[s] = best >= 0 ? bt : (double)__builtin_inff();
-        if (best < 0) { lens[s] = 0; dists[s] = __builtin_inff(); return; }
-        end = bn;
+        if (!found) {
+            dists[s] = __builtin_inff();
+        } else {
+            dists[s] = key_dist(ks[end]);
+            int32_t cur = end;
+            while (true) {
+                if (len >= max_len) { len = -1; break; }
+                pn[len++] = cur;
+                if (!si.ms && cur == si.src) break;
+                const uint32_t pf = key_pred(ks[cur]);
+                if (pf == KEY_NOPRED) {
+                    if (!si.ms) len = -1;   // a two-pin chain must end at its source
+                    break;                  // a multi-source chain ends at a seed
+                }
+                cur = (int32_t)(pf - 1u);
+            }
+        }
+        lens[s] = len;
+        s_len = len;
+        if (len > 0) {
+            const uint32_t off = atomicAdd(pack_n, (uint32_t)len);
+            pack_off[s] = off;
+            s_off = off;
+        }
     }
-    dists[s] = key_dist(ks[end]);
-    int len = 0;
-    int32_t cur = end;
-    while (true) {
-        if (len >= max_len) { len = -1; break; }
-        pn[len++] = cur;
-        if (!si.ms && cur == si.src) break;
-        const uint32_t pf = key_pred(ks[cur]);
-        if (pf == KEY_NOPRED) {
-            if (!si.ms) len = -1;   // a two-pin chain must end at its source
-            break;                  // a multi-source chain ends at a seed
-        }
-        const uint32_t p = pf - 1u;
-        // first CSR edge p -> cur
-        const int r0 = a.indptr[p], r1 = a.indptr[p + 1];
+    __syncthreads();
+    const int len = s_len;
+    if (len <= 0) return;
+    // edge entering pn[t] (from pn[t + 1]): first CSR edge pn[t+1] -> pn[t]
+    for (int t = lane; t + 1 < len; t += 64) {
+        const int32_t v = pn[t], u = pn[t + 1];
+        const int r0 = a.indptr[u], r1 = a.indptr[u + 1];
         int ee = -1;
         for (int e = r0; e < r1; ++e)
-            if (a.indices[e] == cur) { ee = e; break; }
-        pe[len - 1] = ee;   // edge entering pn[len-1]
-        cur = (int32_t)p;
+            if (a.indices[e] == v) { ee = e; break; }
+        pe[t] = ee;
     }
-    lens[s] = len;
-    if (len <= 0) return;
+    __syncthreads();
     // pack src -> dst: node t is pn[len-1-t], the edge leaving it pe[len-2-t]
-    const uint32_t off = atomicAdd(pack_n, (uint32_t)len);
-    pack_off[s] = off;
-    int32_t *on = pack + off;
-    int32_t *oe = pack + pack_half + off;
-    for (int t = 0; t < len; ++t) {
+    int32_t *on = pack + s_off;
+    int32_t *oe = pack + pack_half + s_off;
+    for (int t = lane; t < len; t += 64) {
         on[t] = pn[len - 1 - t];
         oe[t] = t + 1 < len ? pe[len - 2 - t] : -1;
     }
@@ -987,7 +1017,7 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
     HIP_CHECK_RET(hipMalloc((void **)&en.lists, (uint64_t)(nbufs + 1) * LIST_BINS * bin_cap * sizeof(uint32_t)));
     HIP_CHECK_RET(hipMalloc((void **)&en.list_cnt, (uint64_t)(nbufs + 1) * LIST_BINS * sizeof(uint32_t)));
     {
-        const uint64_t tcap = ((uint64_t)n_nodes + 2047) / 2048, twords = (tcap + 31) / 32;
+        const uint64_t tcap = ((uint64_t)n_nodes + TOUCH_KEYS - 1) / TOUCH_KEYS, twords = (tcap + 31) / 32;
         HIP_CHECK_RET(hipMalloc((void **)&en.touch_bits, (uint64_t)slots * twords * sizeof(uint32_t)));
         HIP_CHECK_RET(hipMalloc((void **)&en.touch_list, (uint64_t)slots * tcap * sizeof(uint32_t)));
         HIP_CHECK_RET(hipMalloc((void **)&en.touch_n, (uint64_t)slots * sizeof(uint32_t)));
@@ -1193,7 +1223,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     a.ms_tgt = ms ? en.ms_tgt : nullptr;
     a.tbits = ms ? en.tbits : nullptr;
     a.tbits_words = (N + 31) / 32;
-    a.touch_cap = (N + 2047) / 2048;
+    a.touch_cap = (N + TOUCH_KEYS - 1) / TOUCH_KEYS;
     a.touch_words = (a.touch_cap + 31) / 32;
     a.touch_bits = en.touch_bits; a.touch_list = en.touch_list; a.touch_n = en.touch_n;
     a.lists = en.lists;
@@ -1226,6 +1256,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
         hipLaunchKernelGGL(k_fill_keys, dim3(4096), dim3(256), 0, s, en.keys, (int64_t)en.cap_slots * N);
         HIP_CHECK_RET(hipGetLastError());
         HIP_CHECK_RET(hipMemsetAsync(en.touch_n, 0, sizeof(uint32_t) * en.cap_slots, s));
+        HIP_CHECK_RET(hipMemsetAsync(en.touch_bits, 0, sizeof(uint32_t) * (size_t)en.cap_slots * a.touch_words, s));
         en.keys_n = N;
     }
 
@@ -1319,7 +1350,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
         HIP_CHECK_RET(hipMemcpyAsync(en.slot_list, en.h_slot_list, sizeof(int32_t) * nf, hipMemcpyHostToDevice, s));
         HIP_CHECK_RET(hipMemsetAsync(en.pack_n, 0, sizeof(uint32_t), s));
         const int64_t half = (int64_t)en.cap_slots * en.cap_path;
-        hipLaunchKernelGGL(k_backtrace, dim3((nf + 63) / 64), dim3(64), 0, s, a, en.slot_list, nf,
+        hipLaunchKernelGGL(k_backtrace, dim3(nf), dim3(64), 0, s, a, en.slot_list, nf,
                            en.stage_nodes, en.stage_edges, en.lens, en.dists, en.cap_path,
                            ms ? en.ms_total : nullptr, ms ? en.ms_choice : nullptr,
                            en.pack, en.pack_n, en.pack_off, half);
