@@ -51,7 +51,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--slots", type=int, default=512, help="concurrent searches of the batched form")
+    ap.add_argument("--slots", type=int, default=1024,
+                    help="concurrent searches of the batched form (clamped so that slots x nodes < 2^32)")
     ap.add_argument("--plain-sample", type=int, default=48, help="searches timed in the plain vs batched comparison")
     ap.add_argument("--rounds-per-poll", type=int, default=32, help="SSSP rounds queued between slot refills")
     ap.add_argument("--mt-nets", type=int, default=200,
@@ -101,6 +102,7 @@ def main():
     import numpy as np
     import torch
     from orthoroute_amd import workloads
+    from orthoroute_amd.device import max_slots
     from orthoroute_amd.algorithms.manhattan.unified_pathfinder import (PathFinderConfig, PathFinderRouter,
                                                                         ROI_THRESHOLD_STEPS)
 
@@ -245,7 +247,8 @@ def main():
             "workload": f"{args.workload} backplane {X}x{Y}x{L}, adjacent-layer vias, {r.N} nodes, "
                         f"{r.accounting.E} edges, {len(tasks)} two-pin nets; one step = one full routing "
                         f"pass on the iteration-1 cost snapshot",
-            "nodes": r.N, "edges": r.accounting.E, "nets": len(tasks), "slots": args.slots,
+            "nodes": r.N, "edges": r.accounting.E, "nets": len(tasks),
+            "slots": min(args.slots, max_slots(r.N)),
             "delta": r._search_params(args.slots).delta, "n_buckets": r._search_params(args.slots).n_buckets,
             "local_batches": args.local_batches or 64, "grid": args.grid or 1024,
             "rounds_per_poll": args.rounds_per_poll,

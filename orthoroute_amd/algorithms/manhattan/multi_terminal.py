@@ -13,6 +13,7 @@ result.
 """
 from typing import Dict, List
 
+from ...device import max_slots
 from ...steiner import grow_trees
 
 
@@ -72,7 +73,7 @@ class MultiTerminalNetRouter:
         cost = self.costs
         if cost is None:
             cost = pf.accounting.total_cost if hasattr(pf, "_hbounds") else pf.graph.dlat.base_cost
-        params = pf._search_params(max(1, min(int(pf.config.batch_slots), len(order))))
+        params = pf._search_params(max(1, min(int(pf.config.batch_slots), len(order), max_slots(pf.N))))
         trees, st = grow_trees(pf.ctx, pf.graph.dlat, cost, [p for _, p in order], params)
         self.stats["searches"] = st["searches"]
         self.stats["steps"] = st["steps"]

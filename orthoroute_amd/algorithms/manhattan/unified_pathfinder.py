@@ -35,7 +35,7 @@ from typing import Dict, List, Optional, Set, Tuple
 import numpy as np
 
 from ... import _abi
-from ...device import (Context, DeviceLattice, EdgeState, SearchParams, cost_bounds, roi_bfs,
+from ...device import (Context, DeviceLattice, EdgeState, SearchParams, cost_bounds, max_slots, roi_bfs,
                        route as device_route, route_ms as device_route_ms)
 from ...lattice import DIR_H, DIR_HV, DIR_V, layer_directions, reference_via_pairs
 from ...reductions import sum_f32
@@ -119,7 +119,7 @@ class PathFinderConfig:
     allowed_via_spans = None
     # execution knobs of this build
     lattice_policy: str = "reference"
-    batch_slots: int = 512
+    batch_slots: int = 1024
     sssp_delta = None
     sssp_buckets = None
     sssp_local_batches = 0         # same-bucket batches per wave per round (0 = library default)
@@ -1008,7 +1008,7 @@ class PathFinderRouter:
         if not len(todo):
             return out
         # a work item is the u32 slot*N + node: at most (2^32 - 64) / N slots
-        slots = max(1, min(int(self.config.batch_slots), len(todo), ((1 << 32) - 64) // self.N - 1))
+        slots = max(1, min(int(self.config.batch_slots), len(todo), max_slots(self.N)))
         params = self._search_params(slots)
         sreqs = np.stack([src[todo], dst[todo], nid[todo], row_of[todo],
                           np.full(len(todo), int(keepout_owner is not None), np.int64)], axis=1)

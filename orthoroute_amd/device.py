@@ -26,6 +26,12 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
+def max_slots(n_nodes):
+    """Concurrent searches a graph of n_nodes allows: work items are the u32
+    slot*N + node, and the engine holds at most 1024 slots."""
+    return max(1, min(1024, ((1 << 32) - 64) // max(int(n_nodes), 1) - 1))
+
+
 class Context:
     """One orgpu handle bound to torch's current HIP stream on a device."""
 

@@ -12,7 +12,7 @@ raises, and the reference's caller then takes its own CPU path
 """
 import numpy as np
 
-from .device import (Context, DeviceGraph, SearchParams, cost_bounds, route as device_route,
+from .device import (Context, DeviceGraph, SearchParams, cost_bounds, max_slots, route as device_route,
                      route_ms as device_route_ms)
 from .steiner import grow_trees
 
@@ -68,7 +68,8 @@ class GpuSolver:
         if not self.goal_directed:
             hx = hz = 0.0
         delta = max(2.4 * med, 1e-3) if med > 0 else 1.0
-        return SearchParams(slots=max(1, min(slots, self.slots)), delta=delta, n_buckets=8, h_xy=hx, h_z=hz)
+        return SearchParams(slots=max(1, min(slots, self.slots, max_slots(self.N))), delta=delta, n_buckets=8,
+                            h_xy=hx, h_z=hz)
 
     def _rows(self, rows):
         torch = self.ctx.torch
