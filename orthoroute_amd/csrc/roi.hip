@@ -33,6 +33,7 @@ struct RoiArgs {
     uint32_t *scratch_b;  // [slots][words]
     int32_t *list_a, *list_b;  // [slots][N]
     orgpu_roi_info *info;  // device [n]
+    uint32_t *next;        // next request to take (zero at launch)
 };
 
 __device__ __forceinline__ bool bit_get(const uint32_t *bm, int32_t v) { return (bm[v >> 5] >> (v & 31)) & 1u; }
@@ -49,7 +50,15 @@ k_roi(RoiArgs a)
     uint32_t *B = a.scratch_b + (int64_t)blockIdx.x * a.words;
     int32_t *LA = a.list_a + (int64_t)blockIdx.x * a.N;
     int32_t *LB = a.list_b + (int64_t)blockIdx.x * a.N;
-    for (int r = blockIdx.x; r < a.n; r += gridDim.x) {
+    // nets are taken one at a time from a shared counter, so a workgroup
+    // that drew small ROIs takes more of them
+    __shared__ int s_next;
+    for (;;) {
+        if (tid == 0) s_next = (int)atomicAdd(a.next, 1u);
+        __syncthreads();
+        const int r = s_next;
+        __syncthreads();
+        if (r >= a.n) break;
         const orgpu_roi_req q = a.reqs[r];
         uint32_t *A = a.rows + (int64_t)q.roi_index * a.words;
         for (int64_t w = tid; w < a.words; w += RWG) A[w] = 0u;
@@ -290,18 +299,22 @@ extern "C" int orgpu_roi_bfs(orgpu_t *h, const orgpu_graph *g, const orgpu_roi_r
     hipStream_t s = h->stream;
     orgpu_roi_req *dreq = nullptr;
     orgpu_roi_info *dinfo = nullptr;
+    uint32_t *dnext = nullptr;
     HIP_CHECK_RET(hipMalloc((void **)&dreq, sizeof(orgpu_roi_req) * n));
     HIP_CHECK_RET(hipMalloc((void **)&dinfo, sizeof(orgpu_roi_info) * n));
+    HIP_CHECK_RET(hipMalloc((void **)&dnext, sizeof(uint32_t)));
     HIP_CHECK_RET(hipMemcpyAsync(dreq, reqs, sizeof(orgpu_roi_req) * n, hipMemcpyHostToDevice, s));
+    HIP_CHECK_RET(hipMemsetAsync(dnext, 0, sizeof(uint32_t), s));
     RoiArgs a;
     a.N = N; a.words = (N + 31) / 32; a.indptr = g->indptr; a.indices = g->indices;
     a.reqs = dreq; a.n = n; a.rows = roi_bitmaps; a.n_roi = n_roi;
-    a.scratch_b = en.roi_b; a.list_a = en.roi_la; a.list_b = en.roi_lb; a.info = dinfo;
+    a.scratch_b = en.roi_b; a.list_a = en.roi_la; a.list_b = en.roi_lb; a.info = dinfo; a.next = dnext;
     hipLaunchKernelGGL(k_roi, dim3(slots), dim3(RWG), 0, s, a);
     HIP_CHECK_RET(hipGetLastError());
     if (info) HIP_CHECK_RET(hipMemcpyAsync(info, dinfo, sizeof(orgpu_roi_info) * n, hipMemcpyDeviceToHost, s));
     HIP_CHECK_RET(hipStreamSynchronize(s));
     HIP_CHECK_RET(hipFree(dreq));
     HIP_CHECK_RET(hipFree(dinfo));
+    HIP_CHECK_RET(hipFree(dnext));
     return ORGPU_OK;
 }
