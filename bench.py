@@ -55,6 +55,8 @@ def parse():
                     help="concurrent searches of the batched form (clamped so that slots x nodes < 2^32)")
     ap.add_argument("--plain-sample", type=int, default=48, help="searches timed in the plain vs batched comparison")
     ap.add_argument("--rounds-per-poll", type=int, default=32, help="SSSP rounds queued between slot refills")
+    ap.add_argument("--ttz-nets", type=int, default=120,
+                    help="nets of the 96x96x6 board whose full negotiation is timed to zero overuse (0: skip)")
     ap.add_argument("--mt-nets", type=int, default=200,
                     help="cfg3 multi-terminal nets routed as trees after the timed steps (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU oracle sample budget (0 = skip)")
@@ -223,6 +225,7 @@ def main():
 
     geo = geometry(r)
     mt = multi_terminal(args) if args.mt_nets > 0 else None
+    ttz = time_to_zero_overuse(args) if args.ttz_nets > 0 else None
 
     traffic = None
     prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_round_cfg2_traffic.json")
@@ -272,8 +275,52 @@ def main():
         "cpu_baseline": cpu,
         "geometry": geo,
         "multi_terminal": mt,
+        "time_to_zero_overuse": ttz,
     }
     print(json.dumps(line), flush=True)
+
+
+def time_to_zero_overuse(args):
+    """The whole negotiation (route_multiple_nets from entry to its exit) on
+    a board it solves: the 96x96x6 lattice with the reference's via rules
+    and the first --ttz-nets nets of the 300-net generator (the full 300 do
+    not reach zero overuse under the reference's schedule).  The oracle runs
+    the same schedule with the same search semantics on the CPU; its
+    per-iteration overuse and its final paths are compared."""
+    import torch
+    from orthoroute_amd import workloads
+    from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig, PathFinderRouter
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle.negotiation import OracleRouter
+    X, Y, L, reqs = workloads.cfg1(n_nets=args.ttz_nets)
+
+    def gpu_run():
+        cfg = PathFinderConfig()
+        cfg.lattice_policy = "reference"
+        r = PathFinderRouter(cfg)
+        r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
+        torch.cuda.synchronize()
+        rec = []
+        t0 = time.perf_counter()
+        res = r.route_multiple_nets(reqs, iteration_cb=lambda it, t, v, s, c: rec.append((it, s, c)))
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, res, rec
+    gpu_run()                                     # warm-up (kernels, allocations)
+    t_gpu, res, rec = gpu_run()
+    log(f"[bench] time to zero overuse: {len(rec)} iterations, success={res.get('success')} in {t_gpu:.3f}s")
+    o = OracleRouter(X, Y, L, "reference", pitch=1.0, via_cost=3.0, sssp="canon")
+    orec = []
+    t0 = time.perf_counter()
+    ores = o.route_multiple_nets(reqs, max_iterations=len(rec) + 5,
+                                 iteration_cb=lambda it, t, v, s, c: orec.append((it, s, c)))
+    t_cpu = time.perf_counter() - t0
+    same_traj = orec == rec
+    same_paths = all(list(res["paths"].get(nm, [])) == list(ores["paths"].get(nm, [])) for nm, *_ in reqs)
+    log(f"[bench] oracle negotiation {t_cpu:.2f}s; trajectory equal={same_traj}, paths equal={same_paths}")
+    return {"workload": f"cfg1 lattice 96x96x6 (reference via rules), first {len(reqs)} of the 300 nets",
+            "converged": bool(res.get("success")), "iterations": len(rec), "s": round(t_gpu, 4),
+            "cpu_oracle": {"s": round(t_cpu, 3), "cores": 1, "kind": "port",
+                           "trajectory_equal": bool(same_traj), "paths_equal": bool(same_paths)}}
 
 
 def geometry(r):
