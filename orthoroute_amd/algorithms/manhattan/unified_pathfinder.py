@@ -125,6 +125,7 @@ class PathFinderConfig:
     sssp_local_batches = 0         # same-bucket batches per wave per round (0 = library default)
     sssp_grid = 0                  # workgroups per round launch (0 = library default)
     sssp_rounds_per_poll = 32      # rounds queued between status polls / slot refills
+    sssp_target_buckets = 128      # widen the bucket so a batch's deep searches span about this many
     max_iterations_cap = None
     count_minimal = False          # also count Dijkstra-minimal nodes/edges per search
     goal_directed = True           # bucket on g + lower bound to the destination (same paths)
@@ -402,6 +403,7 @@ class PathFinderRouter:
         self.net_clean_streak: Dict[str, int] = defaultdict(int)
         self._net_to_edges: Dict[str, np.ndarray] = {}
         self._net_nodes: Dict[str, np.ndarray] = {}
+        self._last_depth_hint = None               # bucket sizing (speed only)
         self._ko_nodes = np.zeros(0, np.int64)      # via keepouts: node, first owner
         self._ko_owner = np.zeros(0, np.int64)
         self.portals = {}
@@ -959,12 +961,18 @@ class PathFinderRouter:
         return victims
 
     # ---------------- batched search of one iteration (UPF:4441-4823)
-    def _search_params(self, slots):
+    def _search_params(self, slots, depth=None):
+        """depth: the batch's estimated search depth in cost units beyond
+        each source's lower bound (None = unknown); with no fixed delta
+        configured, the bucket is widened so that this depth spans about
+        config.sssp_target_buckets buckets."""
         d = self.config.sssp_delta
         nb = self.config.sssp_buckets
         if d is None or nb is None:
             from ...autotune import choose_params
             d2, nb2 = choose_params(self, slots, getattr(self, "_cost_scale", 0.0))
+            if depth is not None and self.config.sssp_target_buckets:
+                d2 = max(d2, float(depth) / float(self.config.sssp_target_buckets))
             d = d if d is not None else d2
             nb = nb if nb is not None else nb2
         # goal-directed order (same paths) unless the Dijkstra-minimal work is
@@ -977,10 +985,42 @@ class PathFinderRouter:
                             grid=int(self.config.sssp_grid or 0), count_minimal=bool(self.config.count_minimal),
                             h_xy=hx, h_z=hz, local_batches=int(self.config.sssp_local_batches or 0))
 
-    def _search_batch(self, reqs, keepout_owner=None, readd=True):
+    def _depth_hint(self, ordered, sd):
+        """How far past their first bucket the searches of a batch of nets
+        reach: the 90th percentile, over the nets that hold a path, of that
+        path's cost on this iteration's costs minus the goal-directed lower
+        bound at the net's source.  The current path is a feasible path, so
+        its cost bounds the search's final distance from above.  None when no
+        net holds a finite-cost path (e.g. iteration 1).  Paths are
+        identical for any bucket width; this only sets the width."""
+        n2e = self._net_to_edges
+        have = [i for i, nid in enumerate(ordered) if nid in n2e]
+        if not have:
+            return None
+        edges = [n2e[ordered[i]] for i in have]
+        lens = np.fromiter((len(e) for e in edges), dtype=np.int64, count=len(edges))
+        if not lens.sum():
+            return None
+        torch = self.ctx.torch
+        flat = torch.from_numpy(np.concatenate(edges).astype(np.int64)).to(self.ctx.device)
+        c = self.accounting.total_cost[flat].double().cpu().numpy()
+        starts = np.concatenate([[0], np.cumsum(lens)[:-1]])
+        ok = lens > 0
+        cost = np.add.reduceat(c, starts[ok]) if ok.any() else np.zeros(0)
+        hx, hz = getattr(self, "_hbounds", (0.0, 0.0))
+        X, plane = self._Nx, self._Nx * self._Ny
+        s, t = sd[np.asarray(have)[ok], 0], sd[np.asarray(have)[ok], 1]
+        rs, rt = s % plane, t % plane
+        h = hx * (np.abs(rt % X - rs % X) + np.abs(rt // X - rs // X)) + hz * np.abs(t // plane - s // plane)
+        gap = cost - h
+        gap = gap[np.isfinite(gap)]
+        return float(np.quantile(gap, 0.9)) if len(gap) else None
+
+    def _search_batch(self, reqs, keepout_owner=None, readd=True, depth_hint=None):
         """reqs: (src, dst, net_int, roi, stagnation_bonus) per request, as a
-        list of tuples or an (n, 5) array.  roi False = whole graph.  Returns
-        per request (nodes, edges) as int32 arrays, or None."""
+        list of tuples or an (n, 5) array.  roi False = whole graph.
+        depth_hint: the batch's search depth estimate (see _depth_hint).  Returns per
+        request (nodes, edges) as int32 arrays, or None."""
         torch = self.ctx.torch
         dlat = self.graph.dlat
         R = np.asarray(reqs, dtype=np.float64).reshape(-1, 5)
@@ -1009,7 +1049,7 @@ class PathFinderRouter:
             return out
         # a work item is the u32 slot*N + node: at most (2^32 - 64) / N slots
         slots = max(1, min(int(self.config.batch_slots), len(todo), max_slots(self.N)))
-        params = self._search_params(slots)
+        params = self._search_params(slots, depth_hint)
         sreqs = np.stack([src[todo], dst[todo], nid[todo], row_of[todo],
                           np.full(len(todo), int(keepout_owner is not None), np.int64)], axis=1)
         ko = None
@@ -1054,7 +1094,16 @@ class PathFinderRouter:
         reqs[:, 2] = [self._get_net_id(nid) for nid in ordered] if use_ko else -1
         reqs[:, 3] = md < ROI_THRESHOLD_STEPS
         reqs[:, 4] = bonus
-        results = self._search_batch(reqs, ko_owner, readd=True)
+        # a batch that the engine runs at once (no slot refills) is only as
+        # fast as its deepest searches: size its buckets by their depth (the
+        # last estimate when none of the batch's nets holds a path)
+        hint = None
+        if n <= 1024:
+            hint = self._depth_hint(ordered, sd)
+            if hint is None:
+                hint = self._last_depth_hint
+            self._last_depth_hint = hint
+        results = self._search_batch(reqs, ko_owner, readd=True, depth_hint=hint)
         # wider-ROI fallback for the first five failures, in routing order
         fb = [i for i, r in enumerate(results) if r is None or len(r[0]) <= 1][:5]
         self.full_graph_fallback_count = len(fb)
@@ -1165,6 +1214,7 @@ class PathFinderRouter:
         best = float("inf")
         stagnant = 0
         self._negotiation_ran = True
+        self._last_depth_hint = None
         self._net_failure_count = {}
         self._excluded_nets = set()
         routed = failed = 0

@@ -213,8 +213,12 @@ __device__ __forceinline__ float heur(const RoundArgs &a, const SlotInfo &si, ui
 // Expands one item per lane (lanes with has=false only take part in the
 // wave's collective steps): consumes it, and if its key still lies in the
 // current bucket relaxes its out-edges.  A lane walks its own CSR row four
-// edges at a time with the four edges' loads issued together; the slot's
-// search parameters are read once per item.
+// edges at a time.  The loads are arranged in as few dependent levels as the
+// data allows, because a sparse bucket is expanded hop by hop through the
+// wave's local queue and each hop costs the latency of its dependent chain:
+// (1) slot, own key and CSR row bounds; (2) the four edges' targets and
+// costs; (3) their ROI bits and target keys; (4) the four atomicMins; then
+// the pushes, and the touched-chunk records last.
 __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bool has, bool from_bitmap,
                                            const LocalQ &lq, uint64_t cur, uint32_t nb,
                                            const uint32_t *s_ring, const uint32_t *s_lid, uint32_t s_out, uint32_t s_ovf,
@@ -228,8 +232,10 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bo
     const uint32_t u = item - slot * N;
     if (from_bitmap) atomicAdd(&s_slot_delta[slot], -1);
     const SlotInfo si = a.slots[slot];
-    if (si.state != 1) return;
     const uint64_t ku = __hip_atomic_load(&a.keys[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int row = a.indptr[u];
+    const int row_end = a.indptr[u + 1];
+    if (si.state != 1) return;
     if (ku == KEY_INF) return;
     {
         // Plain order: an item whose key has dropped below the current bucket
@@ -243,8 +249,7 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bo
     }
     // a target of a multi-source search is settled but not expanded
     if (si.ms && ((a.tbits[(uint64_t)slot * a.tbits_words + (u >> 5)] >> (u & 31)) & 1u)) return;
-    const int row = a.indptr[u];
-    const int deg = a.indptr[u + 1] - row;
+    const int deg = row_end - row;
     my_items++;
     my_edges += (unsigned long long)deg;
     const uint32_t ko0 = key_ko(ku);
@@ -257,62 +262,71 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bo
     for (int k0 = 0; k0 < deg; k0 += 4) {
         uint32_t v[4];
         float c[4];
-        bool ok[4];
+        bool ok[4], planar_ko[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             ok[q] = k0 + q < deg;
+            planar_ko[q] = false;
             if (ok[q]) {
                 v[q] = (uint32_t)a.indices[row + k0 + q];
                 c[q] = a.cost[row + k0 + q];
+                if (ko_node) planar_ko[q] = a.kind[row + k0 + q] == 0;
             }
         }
-        if (roi) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (ok[q]) ok[q] = (roi[v[q] >> 5] >> (v[q] & 31)) & 1u;
-        }
-        uint64_t nk[4], kv[4];
+        // target keys are read with the ROI bits (every target id is a
+        // valid node); an edge leaving the ROI drops its key unused
+        uint32_t rw[4];
+        uint64_t kv[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             if (!ok[q]) continue;
-            uint32_t ko = ko0;
-            float nd = d0;
-            if (ko_node && a.kind[row + k0 + q] == 0) ko = ko < 31 ? ko + 1 : 31;
-            else nd = nd + c[q];
-            ok[q] = nd < __builtin_inff();
-            nk[q] = key_pack(ko, nd, u + 1u);
+            if (roi) rw[q] = roi[v[q] >> 5];
+            kv[q] = keys[v[q]];
         }
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (ok[q]) kv[q] = keys[v[q]];
+        uint64_t nk[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            if (!ok[q] || !(nk[q] < kv[q])) continue;
-            const uint64_t old = atomicMin((unsigned long long *)&keys[v[q]], (unsigned long long)nk[q]);
-            if (old == KEY_INF) touch_key(a, slot, v[q]);
-            if (nk[q] < old && key_cost(nk[q]) < key_cost(old)) {
-                const uint64_t b = si.base + key_bucket_rel(nk[q], a.inv_delta, heur(a, si, v[q]));
-                if (b <= cur) {
-                    // same bucket: this wave expands it later in the round
-                    // (its local queue), unless the queue is full
-                    const uint32_t pos = atomicAdd(lq.tail, 1u);
-                    if (pos - lq.head < LQ_CAP) {
-                        __hip_atomic_store(&lq.buf[pos & (LQ_CAP - 1)], slot * N + v[q], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
-                        continue;
-                    }
-                }
-                uint32_t buf;
-                if (b <= cur) buf = s_out;
-                else if (b < cur + nb) buf = s_ring[b % nb];
-                else buf = s_ovf;
-                if (buf == s_ovf && b < my_ovf_min) my_ovf_min = b;
-                if (push_bit_relax(a, buf, s_lid[buf], slot * N + v[q])) {
-                    atomicAdd(&s_slot_delta[slot], 1);
-                    atomicAdd(&s_buf_delta[buf], 1);
+            if (!ok[q]) continue;
+            if (roi && !((rw[q] >> (v[q] & 31)) & 1u)) { ok[q] = false; continue; }
+            uint32_t ko = ko0;
+            float nd = d0;
+            if (planar_ko[q]) ko = ko < 31 ? ko + 1 : 31;
+            else nd = nd + c[q];
+            nk[q] = key_pack(ko, nd, u + 1u);
+            ok[q] = nd < __builtin_inff() && nk[q] < kv[q];
+        }
+        uint64_t old[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (ok[q]) old[q] = atomicMin((unsigned long long *)&keys[v[q]], (unsigned long long)nk[q]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (!ok[q] || !(nk[q] < old[q] && key_cost(nk[q]) < key_cost(old[q]))) continue;
+            const uint64_t b = si.base + key_bucket_rel(nk[q], a.inv_delta, heur(a, si, v[q]));
+            if (b <= cur) {
+                // same bucket: this wave expands it later in the round (its
+                // local queue), unless the queue is full
+                const uint32_t pos = atomicAdd(lq.tail, 1u);
+                if (pos - lq.head < LQ_CAP) {
+                    __hip_atomic_store(&lq.buf[pos & (LQ_CAP - 1)], slot * N + v[q], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    continue;
                 }
             }
+            uint32_t buf;
+            if (b <= cur) buf = s_out;
+            else if (b < cur + nb) buf = s_ring[b % nb];
+            else buf = s_ovf;
+            if (buf == s_ovf && b < my_ovf_min) my_ovf_min = b;
+            if (push_bit_relax(a, buf, s_lid[buf], slot * N + v[q])) {
+                atomicAdd(&s_slot_delta[slot], 1);
+                atomicAdd(&s_buf_delta[buf], 1);
+            }
         }
+        // a key that left KEY_INF records its chunk for the slot's reset
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (ok[q] && old[q] == KEY_INF) touch_key(a, slot, v[q]);
     }
 }
 
