@@ -25,7 +25,10 @@ Also reported: plain (one search at a time) vs batched on a sample of the same
 searches; the k_round roofline (bytes of the expanded items and scanned edges
 per launch over the HIP-event time of the round launches); the CPU oracle's
 GTEPS on a bounded sample of the same searches (single core), with its paths
-compared against the GPU's.
+compared against the GPU's; geometry emission; cfg3's multi-terminal trees;
+the whole negotiation to zero overuse on a board the schedule solves, batched
+and plain, against the oracle; and one routing pass of the 1000x800x24
+stress board (cfg4), with its own plain vs batched sample.
 """
 import argparse
 import json
@@ -53,7 +56,13 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--slots", type=int, default=1024,
                     help="concurrent searches of the batched form (clamped so that slots x nodes < 2^32)")
-    ap.add_argument("--plain-sample", type=int, default=48, help="searches timed in the plain vs batched comparison")
+    ap.add_argument("--plain-sample", type=int, default=1024,
+                    help="searches timed in the plain vs batched comparison (the batched form holds up to "
+                         "--slots of them at once)")
+    ap.add_argument("--stress-nets", type=int, default=40000,
+                    help="nets of the 1000x800x24 stress board routed in one timed pass (0: skip)")
+    ap.add_argument("--stress-plain", type=int, default=256,
+                    help="searches of the stress board timed plain vs batched")
     ap.add_argument("--rounds-per-poll", type=int, default=32, help="SSSP rounds queued between slot refills")
     ap.add_argument("--ttz-nets", type=int, default=120,
                     help="nets of the 96x96x6 board whose full negotiation is timed to zero overuse (0: skip)")
@@ -105,8 +114,7 @@ def main():
     import torch
     from orthoroute_amd import workloads
     from orthoroute_amd.device import max_slots
-    from orthoroute_amd.algorithms.manhattan.unified_pathfinder import (PathFinderConfig, PathFinderRouter,
-                                                                        ROI_THRESHOLD_STEPS)
+    from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig, PathFinderRouter
 
     X, Y, L, reqs = workloads.CONFIGS[args.workload]()
     cfg = PathFinderConfig()
@@ -195,37 +203,19 @@ def main():
         f"({avg_launch_us:.1f} us avg); edges scanned {d.get('sssp_edges', 0):.4g}")
 
     # plain (one search at a time) vs batched on the same sample of searches
-    def sample_reqs(n):
-        out = []
-        for name, (s, dd) in list(tasks.items())[:n]:
-            sx, sy, _ = r._coords(s)
-            dx, dy, _ = r._coords(dd)
-            out.append((s, dd, -1, abs(dx - sx) + abs(dy - sy) < ROI_THRESHOLD_STEPS, 0.0))
-        return out
-    sreqs = sample_reqs(args.plain_sample)
-    timing = {"plain": 0.0, "batched": 0.0}
-    paths = {"plain": [], "batched": []}
-    for label, slots in ((("plain", 1), ("batched", args.slots)) if sreqs else ()):
-        cfg.batch_slots = slots
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        out = r._search_batch(sreqs)
-        torch.cuda.synchronize()
-        timing[label] = time.perf_counter() - t0
-        paths[label] = [None if o is None else o[0].tolist() for o in out]
-        log(f"[bench] {label} sample of {len(sreqs)} searches: {timing[label]:.2f}s")
-    cfg.batch_slots = args.slots
-    same = paths["plain"] == paths["batched"]
+    sreqs = sample_reqs(r, tasks, args.plain_sample)
+    timing, paths, same = plain_vs_batched(r, sreqs, args.slots)
 
     cpu = None
     if args.cpu_seconds > 0:
-        creqs = sreqs or sample_reqs(48)
-        cpu = cpu_baseline(r, creqs, paths["batched"] if sreqs else [None] * len(creqs), args.cpu_seconds, X, Y, L,
-                           compare=bool(sreqs))
+        creqs = sreqs[:48] or sample_reqs(r, tasks, 48)
+        cpu = cpu_baseline(r, creqs, paths["batched"][:len(creqs)] if sreqs else [None] * len(creqs),
+                           args.cpu_seconds, X, Y, L, compare=bool(sreqs))
 
     geo = geometry(r)
     mt = multi_terminal(args) if args.mt_nets > 0 else None
     ttz = time_to_zero_overuse(args) if args.ttz_nets > 0 else None
+    stress_line = stress(args) if args.stress_nets > 0 else None
 
     traffic = None
     prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_round_cfg2_traffic.json")
@@ -276,17 +266,104 @@ def main():
         "geometry": geo,
         "multi_terminal": mt,
         "time_to_zero_overuse": ttz,
+        "stress": stress_line,
     }
     print(json.dumps(line), flush=True)
+
+
+def sample_reqs(r, tasks, n):
+    """The first n two-pin nets of a router's task list as search requests
+    (src, dst, -1, use ROI, 0), with the router's ROI rule."""
+    from orthoroute_amd.algorithms.manhattan.unified_pathfinder import ROI_THRESHOLD_STEPS
+    out = []
+    for name, (s, dd) in list(tasks.items())[:n]:
+        sx, sy, _ = r._coords(s)
+        dx, dy, _ = r._coords(dd)
+        out.append((s, dd, -1, abs(dx - sx) + abs(dy - sy) < ROI_THRESHOLD_STEPS, 0.0))
+    return out
+
+
+def plain_vs_batched(r, sreqs, slots):
+    """The same searches on the router's current costs in the plain form
+    (one slot: one search after another) and the batched form; returns the
+    times, the paths and whether the paths are identical."""
+    import torch
+    timing = {"plain": 0.0, "batched": 0.0}
+    paths = {"plain": [], "batched": []}
+    for label, k in ((("plain", 1), ("batched", slots)) if sreqs else ()):
+        r.config.batch_slots = k
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = r._search_batch(sreqs)
+        torch.cuda.synchronize()
+        timing[label] = time.perf_counter() - t0
+        paths[label] = [None if o is None else o[0].tolist() for o in out]
+        log(f"[bench] {label} sample of {len(sreqs)} searches: {timing[label]:.2f}s")
+    r.config.batch_slots = slots
+    return timing, paths, paths["plain"] == paths["batched"]
+
+
+def stress(args):
+    """The 1000x800x24 stress board (adjacent-layer vias, 19.2 M nodes,
+    75.2 M edges) with 40,000 two-pin nets whose boxes have lognormal
+    half-widths: one full routing pass on the iteration-1 costs (ROI
+    extraction, batched SSSP, backtrace, commit, overuse accounting), timed
+    after an untimed warm-up pass, and the plain form against the batched
+    form on the first --stress-plain nets."""
+    import torch
+    from orthoroute_amd import workloads
+    from orthoroute_amd.device import max_slots
+    from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig, PathFinderRouter
+    X, Y, L, reqs = workloads.cfg4(n_nets=args.stress_nets)
+    cfg = PathFinderConfig()
+    cfg.lattice_policy = "adjacent"
+    cfg.batch_slots = args.slots
+    r = PathFinderRouter(cfg)
+    r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
+    tasks = r._parse_requests(reqs)
+    pres_fac, _, _, _, hmult = r._negotiation_setup(tasks)
+    r.iteration = 1
+    _, over_map = r._begin_iteration(1, pres_fac, hmult)
+    st = r.accounting.state
+
+    def step():
+        routed, failed = r._route_all(tasks, over_map=over_map)
+        st.build_present()
+        return routed, failed
+
+    step()
+    torch.cuda.synchronize()
+    s0 = {k: float(v) for k, v in r.stats.items()}
+    t0 = time.perf_counter()
+    routed, failed = step()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    d = {k: float(v) - s0.get(k, 0.0) for k, v in r.stats.items()}
+    log(f"[bench] stress {X}x{Y}x{L}: {len(tasks)} nets in {t:.2f}s, routed={routed} failed={failed}")
+    sreqs = sample_reqs(r, tasks, args.stress_plain)
+    timing, _, same = plain_vs_batched(r, sreqs, args.slots)
+    return {"workload": f"cfg4 stress board {X}x{Y}x{L}, adjacent-layer vias, {r.N} nodes, {r.accounting.E} edges, "
+                        f"{len(tasks)} two-pin nets; one routing pass on the iteration-1 cost snapshot",
+            "pass_s": round(t, 3), "nets_per_s": round(len(tasks) / max(t, 1e-9), 1), "routed": routed,
+            "failed": failed, "slots": min(args.slots, max_slots(r.N)),
+            "sssp_s": round(d.get("sssp_s", 0.0), 3), "roi_s": round(d.get("roi_s", 0.0), 3),
+            "edges_scanned": int(d.get("sssp_edges", 0.0)),
+            "plain_vs_batched": {"searches": len(sreqs), "plain_s": round(timing["plain"], 3),
+                                 "batched_s": round(timing["batched"], 3),
+                                 "speedup": round(timing["plain"] / max(timing["batched"], 1e-9), 2),
+                                 "identical_paths": same}}
 
 
 def time_to_zero_overuse(args):
     """The whole negotiation (route_multiple_nets from entry to its exit) on
     a board it solves: the 96x96x6 lattice with the reference's via rules
     and the first --ttz-nets nets of the 300-net generator (the full 300 do
-    not reach zero overuse under the reference's schedule).  The oracle runs
-    the same schedule with the same search semantics on the CPU; its
-    per-iteration overuse and its final paths are compared."""
+    not reach zero overuse under the reference's schedule), in the batched
+    form (all of an iteration's nets in one engine call) and the plain form
+    (one slot: the searches run one after another).  The oracle runs the
+    same schedule with the same search semantics on the CPU; its
+    per-iteration overuse and its final paths are compared, and so are the
+    plain form's."""
     import torch
     from orthoroute_amd import workloads
     from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig, PathFinderRouter
@@ -294,9 +371,10 @@ def time_to_zero_overuse(args):
     from oracle.negotiation import OracleRouter
     X, Y, L, reqs = workloads.cfg1(n_nets=args.ttz_nets)
 
-    def gpu_run():
+    def gpu_run(slots=1024):
         cfg = PathFinderConfig()
         cfg.lattice_policy = "reference"
+        cfg.batch_slots = slots
         r = PathFinderRouter(cfg)
         r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
         torch.cuda.synchronize()
@@ -308,6 +386,12 @@ def time_to_zero_overuse(args):
     gpu_run()                                     # warm-up (kernels, allocations)
     t_gpu, res, rec = gpu_run()
     log(f"[bench] time to zero overuse: {len(rec)} iterations, success={res.get('success')} in {t_gpu:.3f}s")
+    # the plain form: one search per engine launch sequence (one slot)
+    gpu_run(1)
+    t_plain, pres, prec = gpu_run(1)
+    plain_same = prec == rec and all(list(pres["paths"].get(nm, [])) == list(res["paths"].get(nm, []))
+                                     for nm, *_ in reqs)
+    log(f"[bench] plain form: {len(prec)} iterations in {t_plain:.3f}s; same trajectory and paths={plain_same}")
     o = OracleRouter(X, Y, L, "reference", pitch=1.0, via_cost=3.0, sssp="canon")
     orec = []
     t0 = time.perf_counter()
@@ -319,6 +403,8 @@ def time_to_zero_overuse(args):
     log(f"[bench] oracle negotiation {t_cpu:.2f}s; trajectory equal={same_traj}, paths equal={same_paths}")
     return {"workload": f"cfg1 lattice 96x96x6 (reference via rules), first {len(reqs)} of the 300 nets",
             "converged": bool(res.get("success")), "iterations": len(rec), "s": round(t_gpu, 4),
+            "plain": {"s": round(t_plain, 4), "slots": 1, "same_trajectory_and_paths": bool(plain_same),
+                      "speedup_batched": round(t_plain / max(t_gpu, 1e-9), 2)},
             "cpu_oracle": {"s": round(t_cpu, 3), "cores": 1, "kind": "port",
                            "trajectory_equal": bool(same_traj), "paths_equal": bool(same_paths)}}
 

@@ -14,17 +14,20 @@
 //    Setting a bit is the push (and its own de-duplication); a round scans
 //    its bucket's bitmap in item order, so a wavefront's items are neighbours
 //    on the lattice and their CSR rows and neighbour keys share cache lines.
-//  * one kernel launch = one round.  The last workgroup to finish (atomic
-//    counter) runs the control step: swap OUT in, advance the bucket, detect
-//    finished slots, schedule an overflow redistribution or go idle.  The host
-//    queues rounds blindly and polls a status block every few dozen rounds.
+//  * one round = a k_round launch (every workgroup expands a share of the
+//    bucket's listed chunks and leaves its counts in its own partials entry)
+//    followed by a one-workgroup k_control launch that reduces the partials,
+//    swaps OUT in, advances the bucket, detects finished slots, and schedules
+//    an overflow redistribution or goes idle.  The host queues rounds blindly
+//    and polls a status block every few dozen rounds.
 //  * every slot shares the global bucket index; a slot seeded at bucket c
 //    measures its distances from c (per-slot base), so finished slots are
 //    refilled with the next request while the others keep running.
-//  * a wavefront takes 64 bitmap words (2048 items) at a time, enumerates
-//    their set bits 64 per batch (ballot-free prefix scan + bit select), then
-//    flattens the batch's edges over the lanes (consecutive lanes read
-//    consecutive CSR entries) and relaxes each with a 64-bit atomicMin.
+//  * a wavefront takes the set bits of 64 bitmap words (2048 items) at a
+//    time (ballot-free prefix scan + bit select), gathers them into 64-item
+//    batches, and each lane relaxes its item's CSR row with 64-bit
+//    atomicMins; improvements inside the current bucket go to the wave's
+//    LDS queue and are expanded in the same round.
 #include <stdio.h>
 #include <string.h>
 

@@ -14,8 +14,8 @@ node = z*X*Y + y*X + x, pins on layer 0.
         is routed as its first two pins, as the reference's live path does).
   cfg4  1000x800x24: 40,000 nets between opposite corners of boxes whose
         centres are uniform and half-widths lognormal(ln 20, 1.0), clipped.
-  cfg5  serving stream: job sizes log-uniform in lattice nodes, generated like
-        cfg1 (see orthoroute_amd.serving).
+(The serving stream of jobs is not generated here: no serving loop is
+built.)
 """
 import numpy as np
 
