@@ -137,11 +137,15 @@ k_roi(RoiArgs a)
             __syncthreads();
             if ((double)s_seen_i > stop) break;
         }
-        // seen = A | B: move the dst wave into the output row, clear scratch
+        // seen = A | B: move the dst wave into the output row, clear scratch.
+        // Word by word with plain stores: neither row changes during this
+        // loop except through these stores, and every thread that writes a
+        // word writes the same value A | B, so the stores need no atomics
+        // (the workgroup's own stores and loads share one L1 and L2).
         const int nb_total = s_nb;
         for (int i = tid; i < nb_total; i += RWG) {
-            const int32_t v = LB[i];
-            atomicOr(&A[v >> 5], 1u << (v & 31));
+            const int32_t w = LB[i] >> 5;
+            A[w] = A[w] | B[w];
         }
         __syncthreads();
         // clear the scratch words the dst wave used (whole words: every bit
@@ -211,10 +215,18 @@ k_roi(RoiArgs a)
             if (tid == 0) { s_min0 = 0x7FFFFFFF; s_min1 = 0x7FFFFFFF; s_kept_delta = 0; }
             __syncthreads();
             const int na_total = s_na;
+            // the bits above the cut leave the row: every thread that writes
+            // word w writes A[w] with exactly those bits cleared (the same
+            // value), so plain stores suffice here too
             for (int i = tid; i < na_total + nb_total; i += RWG) {
                 const int32_t v = i < na_total ? LA[i] : LB[i - na_total];
                 atomicMin(&s_min0, v);
-                if (v > cut) atomicAnd(&A[v >> 5], ~(1u << (v & 31)));
+                if (v > cut) {
+                    const int32_t w = v >> 5;
+                    const int32_t keep_bits = cut - w * 32 + 1;   // ids w*32 .. cut stay
+                    const uint32_t keep = keep_bits <= 0 ? 0u : (keep_bits >= 32 ? ~0u : (1u << keep_bits) - 1u);
+                    A[w] = A[w] & keep;
+                }
             }
             __syncthreads();
             for (int i = tid; i < na_total + nb_total; i += RWG) {

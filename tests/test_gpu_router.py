@@ -33,13 +33,16 @@ def run_oracle(X, Y, L, policy, reqs, cap):
     return res, rec
 
 
-def run_gpu(X, Y, L, policy, reqs, cap, slots):
+def run_gpu(X, Y, L, policy, reqs, cap, slots, delta=1.0, target_buckets=None):
     from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig, PathFinderRouter
     cfg = PathFinderConfig()
     cfg.lattice_policy = policy
     cfg.batch_slots = slots
     cfg.max_iterations_cap = cap
-    cfg.sssp_delta, cfg.sssp_buckets = 1.0, 8
+    if delta is not None:
+        cfg.sssp_delta, cfg.sssp_buckets = delta, 8
+    if target_buckets is not None:
+        cfg.sssp_target_buckets = target_buckets
     r = PathFinderRouter(cfg)
     r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
     rec = []
@@ -64,6 +67,24 @@ def test_negotiation_matches_oracle(case, slots):
     for nm, *_ in reqs:
         assert list(g_res["paths"].get(nm, [])) == list(o_res["paths"].get(nm, [])), nm
     assert bool(g_res.get("success")) == bool(o_res.get("success"))
+
+
+@pytest.mark.parametrize("target", [0, 2, 128])
+def test_depth_sized_buckets_keep_results(target):
+    """The router's bucket width (the lattice's default, widened by the
+    batch's depth estimate towards `target` buckets; 0 = not widened) sets
+    only the speed: the per-iteration overuse, history and final paths equal
+    the oracle's for every choice."""
+    name, X, Y, L, policy, n, seed, cap = CASES[2]
+    print(f"[router] depth-sized buckets target={target} on {name}", flush=True)
+    reqs = requests(X, Y, n, seed)
+    o_res, o_rec = run_oracle(X, Y, L, policy, reqs, cap)
+    g_res, g_rec = run_gpu(X, Y, L, policy, reqs, cap, 64, delta=None, target_buckets=target)
+    assert [r[:3] for r in g_rec] == [r[:3] for r in o_rec]
+    for a, b in zip(g_rec, o_rec):
+        np.testing.assert_array_equal(a[3].view(np.uint32), b[3].view(np.uint32), err_msg=f"history it {a[0]}")
+    for nm, *_ in reqs:
+        assert list(g_res["paths"].get(nm, [])) == list(o_res["paths"].get(nm, [])), nm
 
 
 def test_headless_orp_to_ors(tmp_path):
