@@ -8,9 +8,9 @@ from orthoroute_amd.reductions import bincount_f64, sum_f32
 @pytest.mark.parametrize("seed", range(6))
 def test_sparse_sum_matches_numpy(seed):
     rng = np.random.default_rng(seed)
-    for n in [1, 7, 8, 127, 128, 129, 1000, 8191, 8192, 8193, 50_000, 300_000]:
+    for n in [1, 7, 8, 127, 128, 129, 1000, 8191, 8192, 8193, 16384, 16385, 24577, 50_000, 300_000, 441_600]:
         a = np.zeros(n, np.float32)
-        k = max(1, int(n * rng.choice([0.001, 0.05, 0.5])))
+        k = max(1, int(n * rng.choice([0.00001, 0.001, 0.05, 0.5])))
         idx = rng.choice(n, min(k, n), replace=False)
         a[idx] = (rng.random(len(idx)) * rng.choice([1, 37.5, 1e4], len(idx))).astype(np.float32)
         ref = np.sum(a)
