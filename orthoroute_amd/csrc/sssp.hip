@@ -61,6 +61,23 @@ __device__ __forceinline__ int wave_scan(int x)
     return x;
 }
 
+__device__ __forceinline__ int32_t wave_sum_i32(int32_t x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)x, o, 64), hi = __shfl_xor((uint32_t)(x >> 32), o, 64);
+        x += ((uint64_t)hi << 32) | lo;
+    }
+    return x;
+}
+
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t x)
 {
 #pragma unroll
@@ -419,10 +436,13 @@ k_round(RoundArgs a)
                 const uint64_t ch = list_at(clist, s_binoff, a.bin_cap, li);
                 const uint64_t wi = ch * 64 + lane;
                 const uint32_t word = wi < W ? bin[wi] : 0u;
-                if (__ballot(word != 0u) == 0ull) continue;
                 // a relaxation round consumes its whole bucket; nothing else
                 // sets bits in it during the round (pushes go to OUT, later
-                // ring buckets or overflow)
+                // ring buckets or overflow), and every chunk of a summary
+                // word that has a bit set is listed and consumed in this
+                // round, so the word is cleared whole (by each of them)
+                if (lane == 0) a.summ[(uint64_t)in_buf * a.summ_words + (ch >> 5)] = 0u;
+                if (__ballot(word != 0u) == 0ull) continue;
                 if (word) bin[wi] = 0u;
                 const int cnt = __popc(word);
                 const int incl = wave_scan(cnt);
@@ -592,7 +612,6 @@ k_control(RoundArgs a, int round_grid)
     __shared__ unsigned long long s_items, s_edges, s_ovf_min;
     __shared__ uint64_t s_newcur;
     __shared__ uint32_t s_mode, s_in_cleared, s_in_list, s_zero_list;
-    __shared__ uint32_t s_bincnt[LIST_BINS + 1];   // exclusive prefix of the consumed list's bins
     if (threadIdx.x < nbuf) s_cnt[threadIdx.x] = (int32_t)c->count[threadIdx.x];
     if (threadIdx.x < nb) s_ring[threadIdx.x] = c->ring[threadIdx.x];
     if (threadIdx.x == 0) {
@@ -605,19 +624,29 @@ k_control(RoundArgs a, int round_grid)
     if (mode == MODE_IDLE) return;
     const uint64_t cur = s_cur;
     {
+        // each wave sums its threads' partials with shuffles; one LDS atomic
+        // per wave and counter
+        const bool lead = (threadIdx.x & 63) == 0;
+        for (uint32_t b = 0; b < nbuf; ++b) {
+            int32_t d = 0;
+            for (int g = threadIdx.x; g < round_grid; g += CTRL_WG) d += a.parts[g].buf_delta[b];
+            d = wave_sum_i32(d);
+            if (lead && d) atomicAdd(&s_cnt[b], d);
+        }
         unsigned long long it = 0, ed = 0, om = ~0ull;
         for (int g = threadIdx.x; g < round_grid; g += CTRL_WG) {
             const RoundPart *pp = &a.parts[g];
-            for (uint32_t b = 0; b < nbuf; ++b) {
-                const int32_t d = pp->buf_delta[b];
-                if (d) atomicAdd(&s_cnt[b], d);
-            }
             it += pp->items; ed += pp->edges;
             om = pp->ovf_min < om ? pp->ovf_min : om;
         }
-        if (it) atomicAdd(&s_items, it);
-        if (ed) atomicAdd(&s_edges, ed);
-        if (om != ~0ull) atomicMin(&s_ovf_min, om);
+        it = wave_sum_u64(it);
+        ed = wave_sum_u64(ed);
+        om = wave_min_u64(om);
+        if (lead) {
+            if (it) atomicAdd(&s_items, it);
+            if (ed) atomicAdd(&s_edges, ed);
+            if (om != ~0ull) atomicMin(&s_ovf_min, om);
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -685,23 +714,10 @@ k_control(RoundArgs a, int round_grid)
     const uint32_t m = s_mode;
     if (threadIdx.x < nbuf) c->count[threadIdx.x] = (uint32_t)s_cnt[threadIdx.x];
     if (threadIdx.x < nb) c->ring[threadIdx.x] = s_ring[threadIdx.x];
-    // the consumed bucket: clear the summary words of its listed chunks and
-    // empty its list
-    if (s_in_cleared != 0xFFFFFFFFu) {
-        uint32_t *sm = a.summ + (uint64_t)s_in_cleared * a.summ_words;
-        uint32_t *cnt = a.list_cnt + (uint64_t)s_in_list * LIST_BINS;
-        if (threadIdx.x < 64) {
-            const int n = threadIdx.x < LIST_BINS ? (int)cnt[threadIdx.x] : 0;
-            const int inc = wave_scan(n);
-            s_bincnt[threadIdx.x] = (uint32_t)(inc - n);
-            if (threadIdx.x == 63) s_bincnt[LIST_BINS] = (uint32_t)inc;
-        }
-        __syncthreads();
-        const uint32_t *lst = a.lists + (uint64_t)s_in_list * LIST_BINS * a.bin_cap;
-        for (uint32_t i = threadIdx.x; i < s_bincnt[LIST_BINS]; i += CTRL_WG)
-            sm[list_at(lst, s_bincnt, a.bin_cap, i) >> 5] = 0u;
-        if (threadIdx.x < LIST_BINS) cnt[threadIdx.x] = 0u;
-    }
+    // the consumed bucket: its round cleared the summary words of its listed
+    // chunks; empty its list
+    if (s_in_cleared != 0xFFFFFFFFu && threadIdx.x < LIST_BINS)
+        a.list_cnt[(uint64_t)s_in_list * LIST_BINS + threadIdx.x] = 0u;
     if (s_zero_list != 0xFFFFFFFFu && threadIdx.x < LIST_BINS)
         a.list_cnt[(uint64_t)s_zero_list * LIST_BINS + threadIdx.x] = 0u;
     // slot completion: dst settled (its bucket is below the new current
