@@ -597,8 +597,8 @@ k_round(RoundArgs a)
 // Runs after each round (one workgroup): reduces the round's partials,
 // advances the bucket window, detects finished slots, schedules an overflow
 // redistribution or goes idle, and retires the consumed bucket's chunk list.
-// The control block is staged through LDS so that the serial decision reads
-// no global memory.
+// The control block's counts and ring are staged through LDS for the
+// serial decision.
 __global__ void __launch_bounds__(CTRL_WG)
 k_control(RoundArgs a, int round_grid)
 {
