@@ -20,7 +20,6 @@
 #include "engine.h"
 
 #define RWG 1024
-#define ROI_CHUNK_WORDS 4096
 
 struct RoiArgs {
     int64_t N;
@@ -38,14 +37,36 @@ struct RoiArgs {
 
 __device__ __forceinline__ bool bit_get(const uint32_t *bm, int32_t v) { return (bm[v >> 5] >> (v & 31)) & 1u; }
 
+__device__ __forceinline__ int wave_scan_incl(int x)
+{
+    const int lane = (int)__lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(x, o, 64);
+        if (lane >= o) x += t;
+    }
+    return x;
+}
+
+// position of the r-th (0-based) set bit of w; w has more than r bits set
+__device__ __forceinline__ int nth_set_bit(uint32_t w, int r)
+{
+    int pos = 0, c;
+    c = __popc(w & 0xFFFFu); if (r >= c) { r -= c; w >>= 16; pos += 16; }
+    c = __popc(w & 0xFFu);   if (r >= c) { r -= c; w >>= 8;  pos += 8; }
+    c = __popc(w & 0xFu);    if (r >= c) { r -= c; w >>= 4;  pos += 4; }
+    c = __popc(w & 0x3u);    if (r >= c) { r -= c; w >>= 2;  pos += 2; }
+    if (r >= (int)(w & 1u)) pos += 1;
+    return pos;
+}
+
 __global__ void __launch_bounds__(RWG)
 k_roi(RoiArgs a)
 {
     __shared__ int s_na, s_nb, s_met, s_seen_i;
     __shared__ int s_fa0, s_fa1, s_fb0, s_fb1, s_depth;
-    __shared__ uint32_t s_scan[RWG];
-    __shared__ long long s_acc;
-    __shared__ int s_cut, s_min0, s_min1, s_kept_delta, s_chunk, s_chunk_cnt;
+    __shared__ int s_wsum[RWG / 64];
+    __shared__ int s_cut, s_min0, s_min1, s_kept_delta;
     const int tid = threadIdx.x;
     uint32_t *B = a.scratch_b + (int64_t)blockIdx.x * a.words;
     int32_t *LA = a.list_a + (int64_t)blockIdx.x * a.N;
@@ -158,80 +179,47 @@ k_roi(RoiArgs a)
         const long long seen = s_seen_i;
         long long kept = seen;
         if (seen > q.max_nodes) {
-            // cut = the max_nodes-th smallest seen node id (1-based rank).
-            // Coarse pass: set bits per 4096-word chunk (wave sums, one
-            // barrier pair per chunk) up to the chunk that holds the cut;
-            // fine pass: prefix scan of that chunk's words.
-            if (tid == 0) { s_acc = 0; s_cut = -1; s_chunk = -1; s_chunk_cnt = 0; }
+            // Rank the seen nodes by id in one pass over the row: thread t
+            // counts the set bits of its contiguous word range, a block scan
+            // gives each range's first rank, and the ranges that hold ranks
+            // 1 and 2 (the two lowest ids: positions 0 and 1 of the sorted
+            // ROI) and max_nodes (the cut) locate those bits.
+            const int64_t per = (a.words + RWG - 1) / RWG;
+            const int64_t w0 = std::min<int64_t>((int64_t)tid * per, a.words);
+            const int64_t w1 = std::min<int64_t>(w0 + per, a.words);
+            int c = 0;
+            for (int64_t w = w0; w < w1; ++w) c += __popc(A[w]);
+            const int incl = wave_scan_incl(c);
+            if ((tid & 63) == 63) s_wsum[tid >> 6] = incl;
+            if (tid == 0) { s_cut = -1; s_min0 = 0x7FFFFFFF; s_min1 = 0x7FFFFFFF; s_kept_delta = 0; }
             __syncthreads();
-            const int64_t nchunks = (a.words + ROI_CHUNK_WORDS - 1) / ROI_CHUNK_WORDS;
-            for (int64_t ch = 0; ch < nchunks; ++ch) {
-                const int64_t w_end = min((ch + 1) * ROI_CHUNK_WORDS, a.words);
-                int c = 0;
-                for (int64_t w = ch * ROI_CHUNK_WORDS + tid; w < w_end; w += RWG) c += __popc(A[w]);
-                for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
-                if ((tid & 63) == 0 && c) atomicAdd(&s_chunk_cnt, c);
-                __syncthreads();
-                if (tid == 0) {
-                    if (s_acc + s_chunk_cnt >= q.max_nodes) s_chunk = (int)ch;
-                    else s_acc += s_chunk_cnt;
-                    s_chunk_cnt = 0;
+            long long excl = incl - c;
+            for (int j = 0; j < (tid >> 6); ++j) excl += s_wsum[j];
+            // id of the r-th (1-based) seen node when this range holds it
+            auto find = [&](long long r) -> int32_t {
+                if (!(excl < r && r <= excl + c)) return -1;
+                int need = (int)(r - excl);
+                for (int64_t w = w0; w < w1; ++w) {
+                    const uint32_t x = A[w];
+                    const int pc = __popc(x);
+                    if (need <= pc) return (int32_t)(w * 32 + nth_set_bit(x, need - 1));
+                    need -= pc;
                 }
-                __syncthreads();
-                if (s_chunk >= 0) break;
-            }
-            const int64_t w_lo = (int64_t)s_chunk * ROI_CHUNK_WORDS;
-            const int64_t w_hi = min(w_lo + ROI_CHUNK_WORDS, a.words);
-            for (int64_t base = w_lo; base < w_hi; base += RWG) {
-                const int64_t w = base + tid;
-                const uint32_t word = w < w_hi ? A[w] : 0u;
-                s_scan[tid] = (uint32_t)__popc(word);
-                __syncthreads();
-                for (int o = 1; o < RWG; o <<= 1) {
-                    const uint32_t t = tid >= o ? s_scan[tid - o] : 0u;
-                    __syncthreads();
-                    s_scan[tid] += t;
-                    __syncthreads();
-                }
-                const long long acc = s_acc;
-                const long long incl = acc + s_scan[tid];
-                const long long excl = incl - __popc(word);
-                if (excl < q.max_nodes && incl >= q.max_nodes) {
-                    int need = (int)(q.max_nodes - excl);
-                    uint32_t x = word;
-                    int bit = 0;
-                    for (bit = 0; bit < 32; ++bit) {
-                        if ((x >> bit) & 1u) { if (--need == 0) break; }
-                    }
-                    s_cut = (int)(w * 32 + bit);
-                }
-                __syncthreads();
-                if (s_cut >= 0) break;
-                if (tid == 0) s_acc = acc + s_scan[RWG - 1];
-                __syncthreads();
-            }
-            const int cut = s_cut;
-            // the two lowest seen ids (array positions 0 and 1 of the sorted ROI)
-            if (tid == 0) { s_min0 = 0x7FFFFFFF; s_min1 = 0x7FFFFFFF; s_kept_delta = 0; }
+                return -1;
+            };
+            const int32_t f_cut = find(q.max_nodes), f0 = find(1), f1 = find(2);
+            if (f_cut >= 0) s_cut = f_cut;
+            if (f0 >= 0) s_min0 = f0;
+            if (f1 >= 0) s_min1 = f1;
             __syncthreads();
-            const int na_total = s_na;
-            // the bits above the cut leave the row: every thread that writes
-            // word w writes A[w] with exactly those bits cleared (the same
-            // value), so plain stores suffice here too
-            for (int i = tid; i < na_total + nb_total; i += RWG) {
-                const int32_t v = i < na_total ? LA[i] : LB[i - na_total];
-                atomicMin(&s_min0, v);
-                if (v > cut) {
-                    const int32_t w = v >> 5;
-                    const int32_t keep_bits = cut - w * 32 + 1;   // ids w*32 .. cut stay
-                    const uint32_t keep = keep_bits <= 0 ? 0u : (keep_bits >= 32 ? ~0u : (1u << keep_bits) - 1u);
-                    A[w] = A[w] & keep;
-                }
-            }
-            __syncthreads();
-            for (int i = tid; i < na_total + nb_total; i += RWG) {
-                const int32_t v = i < na_total ? LA[i] : LB[i - na_total];
-                if (v != s_min0) atomicMin(&s_min1, v);
+            const int32_t cut = s_cut;
+            // the bits above the cut leave the row (this thread's words)
+            for (int64_t w = w0; w < w1; ++w) {
+                if (w * 32 + 31 <= cut) continue;
+                const int64_t keep_bits = (int64_t)cut - w * 32 + 1;   // ids w*32 .. cut stay
+                const uint32_t keep = keep_bits <= 0 ? 0u : (1u << keep_bits) - 1u;
+                const uint32_t x = A[w];
+                if (x & ~keep) A[w] = x & keep;
             }
             __syncthreads();
             if (tid == 0) {
