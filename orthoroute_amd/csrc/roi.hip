@@ -63,7 +63,7 @@ __device__ __forceinline__ int nth_set_bit(uint32_t w, int r)
 __global__ void __launch_bounds__(RWG)
 k_roi(RoiArgs a)
 {
-    __shared__ int s_na, s_nb, s_met, s_seen_i;
+    __shared__ int s_na, s_nb, s_met;   // s_na + s_nb = nodes seen
     __shared__ int s_fa0, s_fa1, s_fb0, s_fb1, s_depth;
     __shared__ int s_wsum[RWG / 64];
     __shared__ int s_cut, s_min0, s_min1, s_kept_delta;
@@ -90,7 +90,7 @@ k_roi(RoiArgs a)
             if (q.dst != q.src) B[q.dst >> 5] |= 1u << (q.dst & 31);
             s_na = 1; s_nb = q.dst != q.src ? 1 : 0;
             s_fa0 = 0; s_fa1 = 1; s_fb0 = 0; s_fb1 = s_nb;
-            s_met = 0; s_seen_i = q.dst != q.src ? 2 : 1; s_depth = 0;
+            s_met = 0; s_depth = 0;
         }
         __syncthreads();
         const double stop = (double)q.max_nodes * 1.5;
@@ -137,13 +137,20 @@ k_roi(RoiArgs a)
 #pragma unroll
                         for (int q = 0; q < 4; ++q)
                             old[q] = ok[q] ? atomicOr(&mine[v[q] >> 5], m[q]) : m[q];
+                        // newly marked nodes join the list: one LDS atomic
+                        // per wave and edge position (the active lanes that
+                        // marked a node take consecutive entries)
+                        const uint64_t below = (1ull << __lane_id()) - 1ull;
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
-                            if (!(old[q] & m[q])) {
-                                const int pos = atomicAdd(cnt, 1);
-                                L[pos] = v[q];
-                                atomicAdd(&s_seen_i, 1);
-                            }
+                            const bool fresh = !(old[q] & m[q]);
+                            const uint64_t bal = __ballot(fresh);
+                            if (bal == 0ull) continue;
+                            const int leader = __ffsll((unsigned long long)bal) - 1;
+                            int base = 0;
+                            if ((int)__lane_id() == leader) base = atomicAdd(cnt, __popcll(bal));
+                            base = __shfl(base, leader);
+                            if (fresh) L[base + __popcll(bal & below)] = v[q];
                         }
                     }
                 }
@@ -156,7 +163,7 @@ k_roi(RoiArgs a)
             }
             if (tid == 0) s_depth = s_depth + 1;
             __syncthreads();
-            if ((double)s_seen_i > stop) break;
+            if ((double)(s_na + s_nb) > stop) break;
         }
         // seen = A | B: move the dst wave into the output row, clear scratch.
         // Word by word with plain stores: neither row changes during this
@@ -176,7 +183,7 @@ k_roi(RoiArgs a)
             B[v >> 5] = 0u;
         }
         __syncthreads();
-        const long long seen = s_seen_i;
+        const long long seen = (long long)s_na + s_nb;
         long long kept = seen;
         if (seen > q.max_nodes) {
             // Rank the seen nodes by id in one pass over the row: thread t
