@@ -68,6 +68,9 @@ def parse():
                     help="nets of the 96x96x6 board whose full negotiation is timed to zero overuse (0: skip)")
     ap.add_argument("--mt-nets", type=int, default=200,
                     help="cfg3 multi-terminal nets routed as trees after the timed steps (0: skip)")
+    ap.add_argument("--serve-jobs", type=int, default=16, help="serving stream: jobs (0 = skip)")
+    ap.add_argument("--serve-max-nodes", type=float, default=4e6, help="serving stream: largest board (nodes)")
+    ap.add_argument("--serve-active", type=int, default=4, help="serving slots (jobs on the GPU at once)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU oracle sample budget (0 = skip)")
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--delta", type=float, default=0.0, help="bucket width (0 = the router's choice)")
@@ -216,6 +219,7 @@ def main():
     mt = multi_terminal(args) if args.mt_nets > 0 else None
     ttz = time_to_zero_overuse(args) if args.ttz_nets > 0 else None
     stress_line = stress(args) if args.stress_nets > 0 else None
+    serve = serving(args) if args.serve_jobs > 0 else None
 
     traffic = None
     prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_round_cfg2_traffic.json")
@@ -267,6 +271,7 @@ def main():
         "multi_terminal": mt,
         "time_to_zero_overuse": ttz,
         "stress": stress_line,
+        "serving": serve,
     }
     print(json.dumps(line), flush=True)
 
@@ -407,6 +412,55 @@ def time_to_zero_overuse(args):
                       "speedup_batched": round(t_plain / max(t_gpu, 1e-9), 2)},
             "cpu_oracle": {"s": round(t_cpu, 3), "cores": 1, "kind": "port",
                            "trajectory_equal": bool(same_traj), "paths_equal": bool(same_paths)}}
+
+
+def serving(args):
+    """The serving loop on a Poisson stream of routing jobs (cfg5's arrival
+    process, mean gap 0.5 s; boards log-uniform from 1e5 nodes to
+    --serve-max-nodes, 4-24 layers, adjacent-layer vias; each job's nets
+    drawn like cfg1 at a density the reference's schedule mostly solves,
+    0.75 * X * Y / (X + Y) nets; iteration cap 40).  Reports p50 / p99
+    time to zero overuse and latency, jobs per hour, how far the jobs'
+    device work overlapped (sum of per-job routing time / makespan), and
+    re-runs the three quickest jobs alone to compare their trajectories and
+    paths with the mix."""
+    import torch
+    from orthoroute_amd import workloads
+    from orthoroute_amd.serving import RoutingServer, replay, run_job, summarize
+    jobs = workloads.cfg5(n_jobs=args.serve_jobs, n_min=1e5, n_max=args.serve_max_nodes,
+                          net_density=0.75, max_iterations=40)
+    torch.cuda.synchronize()
+    srv = RoutingServer(hbm_budget=64 << 30, max_active=args.serve_active, roi_limit=1 << 30)
+    t0 = time.perf_counter()
+    res = replay(srv, jobs)
+    srv.close()
+    makespan = time.perf_counter() - t0
+    summ = summarize(res, makespan)
+    busy = sum(r.route_s for r in res if r.error is None)
+    log(f"[bench] serving: {summ['completed']}/{len(jobs)} jobs in {makespan:.2f}s, {summ['converged']} converged, "
+        f"p50/p99 time to zero overuse {summ['p50_time_to_zero_overuse_s']}/{summ['p99_time_to_zero_overuse_s']}s, "
+        f"peak {srv.peak_active} jobs on the GPU")
+    quick = sorted((r for r in res if r.error is None), key=lambda r: r.route_s)[:3]
+    same = []
+    for r in quick:
+        job = jobs[r.job_id]
+        ares, atraj, _, _ = run_job(job, roi_limit=1 << 30)
+        same.append(atraj == r.trajectory and all(
+            [int(v) for v in ares["paths"].get(nm, [])] == [int(v) for v in r.paths.get(nm, [])]
+            for nm, _, _ in job.nets))
+    summ.update({
+        "workload": f"cfg5-shaped Poisson stream: {len(jobs)} jobs, mean gap 0.5 s, boards "
+                    f"{min(j.nodes for j in jobs)}-{max(j.nodes for j in jobs)} nodes, 4-24 layers, "
+                    f"{min(len(j.nets) for j in jobs)}-{max(len(j.nets) for j in jobs)} nets per job, cap 40 iterations",
+        "serving_slots": args.serve_active, "peak_jobs_on_gpu": srv.peak_active,
+        "peak_admitted_gib": round(srv.peak_used / 2 ** 30, 2), "hbm_budget_gib": 64,
+        "sum_route_s": round(busy, 3), "overlap": round(busy / max(makespan, 1e-9), 2),
+        "alone_equals_mix": f"{sum(same)}/{len(same)}",
+        "jobs": [{"id": r.job_id, "nodes": r.nodes, "nets": r.nets, "iterations": r.iterations,
+                  "converged": r.converged, "route_s": round(r.route_s, 3), "latency_s": round(r.latency_s, 3)}
+                 for r in res if r.error is None],
+    })
+    return summ
 
 
 def geometry(r):

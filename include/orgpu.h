@@ -223,6 +223,13 @@ typedef struct orgpu_roi_info {
 int orgpu_roi_bfs(orgpu_t *h, const orgpu_graph *g, const orgpu_roi_req *reqs, int32_t n,
                   uint32_t *roi_bitmaps, int32_t n_roi, orgpu_roi_info *info);
 
+/* Upper bound in bytes on the handle's ROI BFS scratch (about 9 bytes per
+ * node per workgroup slot; default 32 GB).  No counterpart in the reference,
+ * whose GPU pools have fixed sizes (cuda_dijkstra.py:2322-2352); serving
+ * callers that keep several boards resident on one GPU lower it so each job's
+ * footprint stays bounded.  Scratch above a lowered limit is released. */
+int orgpu_set_roi_scratch(orgpu_t *h, int64_t max_bytes);
+
 /* ---- A5/A6 congestion state (EdgeAccountant in HBM) ------------------------- */
 
 /* Nonzero entries of a per-edge quantity, copied to host arrays (idx, val)

@@ -61,6 +61,12 @@ struct Engine {
     int32_t *roi_la = nullptr, *roi_lb = nullptr;
     int64_t roi_n = 0;
     int32_t roi_slots = 0;
+    int64_t roi_limit = 32ll << 30;   // bytes the ROI BFS scratch may take (orgpu_set_roi_scratch)
+    // ROI call staging, grown on demand and kept (no allocation per call)
+    orgpu_roi_req *roi_req = nullptr;
+    orgpu_roi_info *roi_info = nullptr;
+    uint32_t *roi_next = nullptr;
+    int64_t roi_req_cap = 0;
 };
 
 struct orgpu_ctx {

@@ -269,10 +269,23 @@ static int ensure_roi_scratch(Engine &en, int64_t N, int slots)
     if (en.roi_b) { (void)hipFree(en.roi_b); (void)hipFree(en.roi_la); (void)hipFree(en.roi_lb); }
     en.roi_b = nullptr; en.roi_la = nullptr; en.roi_lb = nullptr; en.roi_n = 0; en.roi_slots = 0;
     HIP_CHECK_RET(hipMalloc((void **)&en.roi_b, (size_t)slots * words * sizeof(uint32_t)));
-    HIP_CHECK_RET(hipMemset(en.roi_b, 0, (size_t)slots * words * sizeof(uint32_t)));
+    HIP_CHECK_RET(hipMemsetAsync(en.roi_b, 0, (size_t)slots * words * sizeof(uint32_t), en.stream));
     HIP_CHECK_RET(hipMalloc((void **)&en.roi_la, (size_t)slots * N * sizeof(int32_t)));
     HIP_CHECK_RET(hipMalloc((void **)&en.roi_lb, (size_t)slots * N * sizeof(int32_t)));
     en.roi_n = N; en.roi_slots = slots;
+    return ORGPU_OK;
+}
+
+static int ensure_roi_staging(Engine &en, int64_t n)
+{
+    if (en.roi_req_cap >= n) return ORGPU_OK;
+    if (en.roi_req) { (void)hipFree(en.roi_req); (void)hipFree(en.roi_info); }
+    en.roi_req = nullptr; en.roi_info = nullptr; en.roi_req_cap = 0;
+    const int64_t cap = std::max<int64_t>(n, 1024);
+    HIP_CHECK_RET(hipMalloc((void **)&en.roi_req, sizeof(orgpu_roi_req) * cap));
+    HIP_CHECK_RET(hipMalloc((void **)&en.roi_info, sizeof(orgpu_roi_info) * cap));
+    if (!en.roi_next) HIP_CHECK_RET(hipMalloc((void **)&en.roi_next, sizeof(uint32_t)));
+    en.roi_req_cap = cap;
     return ORGPU_OK;
 }
 
@@ -298,18 +311,17 @@ extern "C" int orgpu_roi_bfs(orgpu_t *h, const orgpu_graph *g, const orgpu_roi_r
     HIP_CHECK_RET(hipSetDevice(h->device));
     Engine &en = h->en;
     // concurrent workgroups, each with ~9 bytes of scratch per node (up to
-    // 32 GB in all): a net's BFS is a chain of short levels, so throughput
-    // comes from many nets in flight (four workgroups per CU at 1024)
-    const int slots = std::min<int>(n, (int)std::max<int64_t>(1, std::min<int64_t>(1024, (int64_t)(32ll << 30) / (N * 9 + 1))));
+    // the handle's ROI scratch limit, 32 GB by default): a net's BFS is a
+    // chain of short levels, so throughput comes from many nets in flight
+    // (four workgroups per CU at 1024)
+    const int slots = std::min<int>(n, (int)std::max<int64_t>(1, std::min<int64_t>(1024, en.roi_limit / (N * 9 + 1))));
     int rc = ensure_roi_scratch(en, N, slots);
+    if (!rc) rc = ensure_roi_staging(en, n);
     if (rc) return rc;
     hipStream_t s = h->stream;
-    orgpu_roi_req *dreq = nullptr;
-    orgpu_roi_info *dinfo = nullptr;
-    uint32_t *dnext = nullptr;
-    HIP_CHECK_RET(hipMalloc((void **)&dreq, sizeof(orgpu_roi_req) * n));
-    HIP_CHECK_RET(hipMalloc((void **)&dinfo, sizeof(orgpu_roi_info) * n));
-    HIP_CHECK_RET(hipMalloc((void **)&dnext, sizeof(uint32_t)));
+    orgpu_roi_req *dreq = en.roi_req;
+    orgpu_roi_info *dinfo = en.roi_info;
+    uint32_t *dnext = en.roi_next;
     HIP_CHECK_RET(hipMemcpyAsync(dreq, reqs, sizeof(orgpu_roi_req) * n, hipMemcpyHostToDevice, s));
     HIP_CHECK_RET(hipMemsetAsync(dnext, 0, sizeof(uint32_t), s));
     RoiArgs a;
@@ -320,8 +332,21 @@ extern "C" int orgpu_roi_bfs(orgpu_t *h, const orgpu_graph *g, const orgpu_roi_r
     HIP_CHECK_RET(hipGetLastError());
     if (info) HIP_CHECK_RET(hipMemcpyAsync(info, dinfo, sizeof(orgpu_roi_info) * n, hipMemcpyDeviceToHost, s));
     HIP_CHECK_RET(hipStreamSynchronize(s));
-    HIP_CHECK_RET(hipFree(dreq));
-    HIP_CHECK_RET(hipFree(dinfo));
-    HIP_CHECK_RET(hipFree(dnext));
+    return ORGPU_OK;
+}
+
+extern "C" int orgpu_set_roi_scratch(orgpu_t *h, int64_t max_bytes)
+{
+    if (!h) return orgpu_fail(ORGPU_EINVAL, "set_roi_scratch: null handle");
+    if (max_bytes < 1) return orgpu_fail(ORGPU_EINVAL, "set_roi_scratch: limit %lld must be positive", (long long)max_bytes);
+    Engine &en = h->en;
+    en.roi_limit = max_bytes;
+    if (en.roi_b && (int64_t)en.roi_slots * (en.roi_n * 9 + 1) > max_bytes) {
+        // drop scratch above the new limit; the next call sizes it again
+        HIP_CHECK_RET(hipSetDevice(h->device));
+        HIP_CHECK_RET(hipStreamSynchronize(h->stream));
+        (void)hipFree(en.roi_b); (void)hipFree(en.roi_la); (void)hipFree(en.roi_lb);
+        en.roi_b = nullptr; en.roi_la = nullptr; en.roi_lb = nullptr; en.roi_n = 0; en.roi_slots = 0;
+    }
     return ORGPU_OK;
 }

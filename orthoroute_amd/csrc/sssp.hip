@@ -1044,7 +1044,7 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
     const int nbufs = n_buckets + 3;
     HIP_CHECK_RET(hipMalloc((void **)&en.keys, items * sizeof(uint64_t)));
     HIP_CHECK_RET(hipMalloc((void **)&en.bits, (uint64_t)nbufs * words * sizeof(uint32_t)));
-    HIP_CHECK_RET(hipMemset(en.bits, 0, (uint64_t)nbufs * words * sizeof(uint32_t)));
+    HIP_CHECK_RET(hipMemsetAsync(en.bits, 0, (uint64_t)nbufs * words * sizeof(uint32_t), en.stream));
     HIP_CHECK_RET(hipMalloc((void **)&en.summ, (uint64_t)nbufs * summ_words * sizeof(uint32_t)));
     const uint64_t bin_cap = (chunks + LIST_BINS - 1) / LIST_BINS + 1;
     HIP_CHECK_RET(hipMalloc((void **)&en.lists, (uint64_t)(nbufs + 1) * LIST_BINS * bin_cap * sizeof(uint32_t)));
@@ -1143,9 +1143,13 @@ void engine_release(Engine &en)
     if (en.roi_b) hipFree(en.roi_b);
     if (en.roi_la) hipFree(en.roi_la);
     if (en.roi_lb) hipFree(en.roi_lb);
+    if (en.roi_req) hipFree(en.roi_req);
+    if (en.roi_info) hipFree(en.roi_info);
+    if (en.roi_next) hipFree(en.roi_next);
     Engine fresh;
     fresh.stream = en.stream;
     fresh.device = en.device;
+    fresh.roi_limit = en.roi_limit;
     en = fresh;
 }
 

@@ -33,15 +33,20 @@ def max_slots(n_nodes):
 
 
 class Context:
-    """One orgpu handle bound to torch's current HIP stream on a device."""
+    """One orgpu handle bound to a HIP stream on a device: `stream` (a
+    torch.cuda.Stream) or else torch's current stream there.  Torch work on
+    the handle's buffers must run on the same stream (the serving loop gives
+    each job its own stream and makes it current in the job's thread)."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, stream=None):
         torch = _torch()
         self.torch = torch
         self.device = torch.device("cuda", device)
         self.lib = _abi.load()
-        with torch.cuda.device(self.device):
-            self.stream = torch.cuda.current_stream(self.device)
+        if stream is None:
+            with torch.cuda.device(self.device):
+                stream = torch.cuda.current_stream(self.device)
+        self.stream = stream
         h = ctypes.c_void_p()
         _abi.check(self.lib.orgpu_create(device, ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(h)))
         self.handle = h
@@ -60,6 +65,10 @@ class Context:
 
     def sync(self):
         _abi.check(self.lib.orgpu_sync(self.handle))
+
+    def set_roi_scratch(self, max_bytes):
+        """Bound the handle's ROI BFS scratch (bytes)."""
+        _abi.check(self.lib.orgpu_set_roi_scratch(self.handle, int(max_bytes)))
 
     # ---- A1 ---------------------------------------------------------------
     def build_lattice(self, X, Y, L, policy="reference", pitch=1.0, via_cost=3.0):

@@ -14,8 +14,13 @@ node = z*X*Y + y*X + x, pins on layer 0.
         is routed as its first two pins, as the reference's live path does).
   cfg4  1000x800x24: 40,000 nets between opposite corners of boxes whose
         centres are uniform and half-widths lognormal(ln 20, 1.0), clipped.
-(The serving stream of jobs is not generated here: no serving loop is
-built.)
+  cfg5  serving stream: jobs with exponential inter-arrival times, node
+        counts log-uniform over a range, layer counts uniform, square boards
+        X = Y = round(sqrt(N / L)), each job's nets generated like cfg1.
+        `net_density` sets the net count from the board instead of the
+        log-uniform draw: nets = net_density * X * Y / (X + Y) (pins are on
+        the top layer, so the count follows the board's area over its
+        half-perimeter, the mean net length).
 """
 import numpy as np
 
@@ -120,3 +125,26 @@ def cfg4(seed=4, X=1000, Y=800, L=24, n_nets=40000):
 
 
 CONFIGS = {"cfg1": cfg1, "cfg2": cfg2, "cfg3": cfg3, "cfg4": cfg4}
+
+
+def cfg5(seed=5, n_jobs=64, mean_gap_s=0.5, n_min=1e5, n_max=2e7, l_min=4, l_max=24, nets_min=500,
+         nets_max=30000, net_density=None, max_iterations=40, policy="adjacent"):
+    """Serving jobs (serving.Job), arrival times in seconds from the start."""
+    from .serving import Job
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(mean_gap_s, n_jobs)
+    arrivals = np.concatenate([[0.0], np.cumsum(gaps[1:])])
+    jobs = []
+    for j in range(n_jobs):
+        n = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        L = int(rng.integers(l_min, l_max + 1))
+        X = Y = max(8, int(round(np.sqrt(n / L))))
+        if net_density is None:
+            k = int(round(np.exp(rng.uniform(np.log(nets_min), np.log(nets_max)))))
+        else:
+            k = int(round(net_density * X * Y / (X + Y)))
+        k = max(1, min(k, X * Y // 2))
+        _, _, _, reqs = cfg1(seed=seed * 1000 + j, X=X, Y=Y, L=L, n_nets=k)
+        reqs = [(f"J{j:02d}_{nm}", s_, d_) for nm, s_, d_ in reqs]
+        jobs.append(Job(j, X, Y, L, reqs, float(arrivals[j]), policy=policy, max_iterations=max_iterations))
+    return jobs

@@ -1,0 +1,77 @@
+"""Serving loop on the GPU: jobs for boards of different sizes routed
+together (admitted as they arrive, each on its own stream and handle) give
+the same per-iteration overuse and the same final paths as each job routed
+alone; one job is also checked against the oracle's negotiation."""
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _jobs():
+    from orthoroute_amd import workloads
+    from orthoroute_amd.serving import Job
+    spec = [(0, 48, 48, 4, 12, "adjacent"), (1, 64, 40, 6, 30, "adjacent"), (2, 32, 32, 4, 10, "reference"),
+            (3, 56, 56, 8, 40, "adjacent"), (4, 80, 72, 6, 36, "adjacent")]
+    jobs = []
+    for jid, X, Y, L, n, pol in spec:
+        _, _, _, reqs = workloads.cfg1(seed=100 + jid, X=X, Y=Y, L=L, n_nets=n)
+        jobs.append(Job(jid, X, Y, L, reqs, arrival_s=0.02 * jid, policy=pol, max_iterations=30))
+    return jobs
+
+
+@pytest.fixture(scope="module")
+def alone():
+    if not torch.cuda.is_available():
+        pytest.skip("needs a GPU")
+    from orthoroute_amd.serving import run_job
+    out = {}
+    for job in _jobs():
+        print(f"[serving] alone: job {job.job_id} {job.X}x{job.Y}x{job.L} {len(job.nets)} nets", flush=True)
+        res, traj, dt, _ = run_job(job)
+        out[job.job_id] = (res, traj)
+    return out
+
+
+def _paths(res, job):
+    return {nm: [int(v) for v in res["paths"].get(nm, [])] for nm, _, _ in job.nets}
+
+
+def test_mix_equals_alone(alone):
+    from orthoroute_amd.serving import RoutingServer, replay, summarize
+    jobs = _jobs()
+    srv = RoutingServer(hbm_budget=32 << 30, max_active=4, roi_limit=256 << 20)
+    print("[serving] mix of", len(jobs), "jobs", flush=True)
+    res = replay(srv, jobs)
+    srv.close()
+    assert srv.peak_active >= 2, "jobs never overlapped"
+    for job, r in zip(jobs, res):
+        assert r.error is None, r.error
+        ares, atraj = alone[job.job_id]
+        assert r.trajectory == atraj, f"job {job.job_id}: per-iteration overuse differs alone and in the mix"
+        assert r.paths.keys() == ares["paths"].keys()
+        assert _paths(r.__dict__, job) == _paths(ares, job), f"job {job.job_id}: paths differ alone and in the mix"
+        assert r.converged == bool(ares.get("converged"))
+    s = summarize(res, max(r.done_s for r in res))
+    assert s["completed"] == len(jobs)
+
+
+def test_mix_job_matches_oracle(alone):
+    """Job 2 (reference via rules) against the oracle's negotiation with the
+    same search semantics: identical per-iteration overuse and paths."""
+    from oracle.negotiation import OracleRouter
+    job = _jobs()[2]
+    res, traj = alone[2]
+    o = OracleRouter(job.X, job.Y, job.L, job.policy, pitch=1.0, via_cost=job.via_cost, sssp="canon")
+    otraj = []
+    ores = o.route_multiple_nets(job.nets, max_iterations=job.max_iterations,
+                                 iteration_cb=lambda it, t, v, s, c: otraj.append((it, s, c)))
+    assert otraj == traj
+    assert _paths(ores, job) == _paths(res, job)
