@@ -68,6 +68,7 @@ def parse():
                     help="nets of the 96x96x6 board whose full negotiation is timed to zero overuse (0: skip)")
     ap.add_argument("--mt-nets", type=int, default=200,
                     help="cfg3 multi-terminal nets routed as trees after the timed steps (0: skip)")
+    ap.add_argument("--graph-rounds", action="store_true", help="replay the rounds between polls as HIP graphs")
     ap.add_argument("--serve-jobs", type=int, default=16, help="serving stream: jobs (0 = skip)")
     ap.add_argument("--serve-max-nodes", type=float, default=4e6, help="serving stream: largest board (nodes)")
     ap.add_argument("--serve-active", type=int, default=4, help="serving slots (jobs on the GPU at once)")
@@ -130,6 +131,7 @@ def main():
     cfg.sssp_local_batches = args.local_batches
     cfg.sssp_grid = args.grid
     cfg.sssp_rounds_per_poll = args.rounds_per_poll
+    cfg.sssp_graph_rounds = args.graph_rounds
     r = PathFinderRouter(cfg)
     t0 = time.perf_counter()
     r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
@@ -323,6 +325,7 @@ def stress(args):
     cfg = PathFinderConfig()
     cfg.lattice_policy = "adjacent"
     cfg.batch_slots = args.slots
+    cfg.sssp_graph_rounds = args.graph_rounds
     r = PathFinderRouter(cfg)
     r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
     tasks = r._parse_requests(reqs)
@@ -380,6 +383,7 @@ def time_to_zero_overuse(args):
         cfg = PathFinderConfig()
         cfg.lattice_policy = "reference"
         cfg.batch_slots = slots
+        cfg.sssp_graph_rounds = args.graph_rounds
         r = PathFinderRouter(cfg)
         r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
         torch.cuda.synchronize()
@@ -437,7 +441,8 @@ def serving(args):
     makespan = time.perf_counter() - t0
     summ = summarize(res, makespan)
     busy = sum(r.route_s for r in res if r.error is None)
-    log(f"[bench] serving: {summ['completed']}/{len(jobs)} jobs in {makespan:.2f}s, {summ['converged']} converged, "
+    log(f"[bench] serving: {summ['completed']}/{len(jobs)} jobs in {makespan:.2f}s, "
+        f"{summ['reached_zero_overuse']} reached zero overuse, {summ['converged']} converged, "
         f"p50/p99 time to zero overuse {summ['p50_time_to_zero_overuse_s']}/{summ['p99_time_to_zero_overuse_s']}s, "
         f"peak {srv.peak_active} jobs on the GPU")
     quick = sorted((r for r in res if r.error is None), key=lambda r: r.route_s)[:3]
@@ -456,8 +461,10 @@ def serving(args):
         "peak_admitted_gib": round(srv.peak_used / 2 ** 30, 2), "hbm_budget_gib": 64,
         "sum_route_s": round(busy, 3), "overlap": round(busy / max(makespan, 1e-9), 2),
         "alone_equals_mix": f"{sum(same)}/{len(same)}",
-        "jobs": [{"id": r.job_id, "nodes": r.nodes, "nets": r.nets, "iterations": r.iterations,
-                  "converged": r.converged, "route_s": round(r.route_s, 3), "latency_s": round(r.latency_s, 3)}
+        "jobs": [{"id": r.job_id, "nodes": r.nodes, "layers": jobs[r.job_id].L, "nets": r.nets,
+                  "iterations": r.iterations, "converged": r.converged,
+                  "zero_overuse_s": None if r.zero_overuse_s is None else round(r.zero_overuse_s, 3),
+                  "route_s": round(r.route_s, 3), "latency_s": round(r.latency_s, 3)}
                  for r in res if r.error is None],
     })
     return summ

@@ -119,6 +119,10 @@ typedef struct orgpu_sssp_params {
      * round, up to local_batches batches of 64 (0 = 64); the rest wait for
      * the next round. */
     int32_t local_batches;
+    /* 0: the rounds between two polls are launched one by one; 1: they are
+     * replayed as one HIP graph (captured on the first call and again when
+     * the round arguments change).  Same kernels and results either way. */
+    int32_t launch_mode;
 } orgpu_sssp_params;
 
 typedef struct orgpu_sssp_stats {

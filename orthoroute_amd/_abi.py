@@ -67,7 +67,8 @@ class SsspParams(ctypes.Structure):
     _fields_ = [("delta", ctypes.c_float), ("n_buckets", ctypes.c_int32),
                 ("slots", ctypes.c_int32), ("rounds_per_poll", ctypes.c_int32),
                 ("grid", ctypes.c_int32), ("count_minimal", ctypes.c_int32),
-                ("h_xy", ctypes.c_float), ("h_z", ctypes.c_float), ("local_batches", ctypes.c_int32)]
+                ("h_xy", ctypes.c_float), ("h_z", ctypes.c_float), ("local_batches", ctypes.c_int32),
+                ("launch_mode", ctypes.c_int32)]
 
 
 class SsspStats(ctypes.Structure):

@@ -125,6 +125,7 @@ class PathFinderConfig:
     sssp_local_batches = 0         # same-bucket batches per wave per round (0 = library default)
     sssp_grid = 0                  # workgroups per round launch (0 = library default)
     sssp_rounds_per_poll = 32      # rounds queued between status polls / slot refills
+    sssp_graph_rounds = False      # replay the rounds between polls as one captured HIP graph
     sssp_target_buckets = 128      # widen the bucket so a batch's deep searches span about this many
     max_iterations_cap = None
     count_minimal = False          # also count Dijkstra-minimal nodes/edges per search
@@ -983,7 +984,8 @@ class PathFinderRouter:
         return SearchParams(slots=slots, delta=float(d), n_buckets=int(nb),
                             rounds_per_poll=int(self.config.sssp_rounds_per_poll or 32),
                             grid=int(self.config.sssp_grid or 0), count_minimal=bool(self.config.count_minimal),
-                            h_xy=hx, h_z=hz, local_batches=int(self.config.sssp_local_batches or 0))
+                            h_xy=hx, h_z=hz, local_batches=int(self.config.sssp_local_batches or 0),
+                            graph_rounds=bool(getattr(self.config, "sssp_graph_rounds", False)))
 
     def _depth_hint(self, ordered, sd):
         """How far past their first bucket the searches of a batch of nets

@@ -49,6 +49,12 @@ struct Engine {
     float *h_dists = nullptr;
     unsigned long long *h_counters = nullptr;
     hipEvent_t ev_poll = nullptr, ev_t0 = nullptr, ev_t1 = nullptr, ev_r0 = nullptr, ev_r1 = nullptr;
+    // a poll's rounds captured as one HIP graph (launch_mode 1): the
+    // capture stream only records; the executable is launched on `stream`
+    hipStream_t capture_stream = nullptr;
+    hipGraphExec_t round_exec = nullptr;
+    unsigned char round_exec_args[sizeof(RoundArgs)] = {};
+    int32_t round_exec_rpp = 0, round_exec_grid = 0;
     // congestion-state helpers: nonzero staging and host->device uploads
     int32_t *sp_idx = nullptr;
     float *sp_val = nullptr;

@@ -1016,11 +1016,43 @@ __global__ void k_count_minimal(RoundArgs a, int slot, unsigned long long *out)
 
 #define MAX_GRID 8192
 
-static int launch_round(Engine &en, const RoundArgs &a, int grid)
+static int launch_round(hipStream_t s, const RoundArgs &a, int grid)
 {
-    hipLaunchKernelGGL(k_round, dim3(grid), dim3(WG), 0, en.stream, a);
-    hipLaunchKernelGGL(k_control, dim3(1), dim3(CTRL_WG), 0, en.stream, a, grid);
+    hipLaunchKernelGGL(k_round, dim3(grid), dim3(WG), 0, s, a);
+    hipLaunchKernelGGL(k_control, dim3(1), dim3(CTRL_WG), 0, s, a, grid);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// rpp rounds on the engine stream: launch by launch (mode 0), or as one
+// HIP graph of the rpp round pairs (mode 1), captured on a private stream
+// the first time and again whenever the round arguments change.  The graph
+// holds kernel nodes only.
+static int launch_rounds(Engine &en, const RoundArgs &a, int grid, int rpp, int mode)
+{
+    if (mode != 1) {
+        for (int r = 0; r < rpp; ++r) launch_round(en.stream, a, grid);
+        HIP_CHECK_RET(hipGetLastError());
+        return ORGPU_OK;
+    }
+    if (!en.round_exec || en.round_exec_rpp != rpp || en.round_exec_grid != grid ||
+        memcmp(en.round_exec_args, &a, sizeof(a)) != 0) {
+        if (en.round_exec) { (void)hipGraphExecDestroy(en.round_exec); en.round_exec = nullptr; }
+        if (!en.capture_stream) HIP_CHECK_RET(hipStreamCreateWithFlags(&en.capture_stream, hipStreamNonBlocking));
+        hipGraph_t g = nullptr;
+        HIP_CHECK_RET(hipStreamBeginCapture(en.capture_stream, hipStreamCaptureModeThreadLocal));
+        for (int r = 0; r < rpp; ++r) launch_round(en.capture_stream, a, grid);
+        const hipError_t le = hipGetLastError();
+        HIP_CHECK_RET(hipStreamEndCapture(en.capture_stream, &g));
+        if (le != hipSuccess) { (void)hipGraphDestroy(g); return orgpu_fail_hip(le, "round capture", __FILE__, __LINE__); }
+        const hipError_t ie = hipGraphInstantiate(&en.round_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) { en.round_exec = nullptr; return orgpu_fail_hip(ie, "hipGraphInstantiate", __FILE__, __LINE__); }
+        memcpy(en.round_exec_args, &a, sizeof(a));
+        en.round_exec_rpp = rpp;
+        en.round_exec_grid = grid;
+    }
+    HIP_CHECK_RET(hipGraphLaunch(en.round_exec, en.stream));
+    return ORGPU_OK;
 }
 
 int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets, int64_t qcap, int32_t max_path)
@@ -1135,6 +1167,8 @@ void engine_release(Engine &en)
     if (en.ev_t1) hipEventDestroy(en.ev_t1);
     if (en.ev_r0) hipEventDestroy(en.ev_r0);
     if (en.ev_r1) hipEventDestroy(en.ev_r1);
+    if (en.round_exec) hipGraphExecDestroy(en.round_exec);
+    if (en.capture_stream) hipStreamDestroy(en.capture_stream);
     if (en.sp_idx) hipFree(en.sp_idx);
     if (en.sp_val) hipFree(en.sp_val);
     if (en.sp_count) hipFree(en.sp_count);
@@ -1246,6 +1280,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     }
     const int nb = p->n_buckets;
     RoundArgs a;
+    memset(&a, 0, sizeof(a));   // padding included: a captured round graph is reused when the bytes match
     a.ctrl = en.ctrl; a.slots = en.slots; a.K = K; a.N = N; a.nb = (uint32_t)nb;
     a.inv_delta = 1.0f / p->delta;
     a.h_xy = p->h_xy; a.h_z = p->h_z;
@@ -1342,8 +1377,8 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     double round_ms = 0.0;
     while (done_nets < n_nets) {
         HIP_CHECK_RET(hipEventRecord(en.ev_r0, s));
-        for (int r = 0; r < rpp; ++r) launch_round(en, a, grid);
-        HIP_CHECK_RET(hipGetLastError());
+        rc = launch_rounds(en, a, grid, rpp, p->launch_mode);
+        if (rc) return rc;
         HIP_CHECK_RET(hipEventRecord(en.ev_r1, s));
         launches += rpp;
         HIP_CHECK_RET(hipMemcpyAsync(en.h_ctrl, en.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));

@@ -191,11 +191,12 @@ class SearchParams:
     runs the plain Dijkstra bucket order.  Paths are the same either way."""
 
     def __init__(self, slots=1, delta=1.0, n_buckets=8, rounds_per_poll=32, grid=0, count_minimal=False,
-                 h_xy=0.0, h_z=0.0, local_batches=0):
+                 h_xy=0.0, h_z=0.0, local_batches=0, graph_rounds=False):
         self.slots, self.delta, self.n_buckets = int(slots), float(delta), int(n_buckets)
         self.rounds_per_poll, self.grid, self.count_minimal = int(rounds_per_poll), int(grid), bool(count_minimal)
         self.h_xy, self.h_z = float(h_xy), float(h_z)
         self.local_batches = int(local_batches)
+        self.graph_rounds = bool(graph_rounds)   # replay a poll's rounds as one captured HIP graph
 
     def c(self):
         p = _abi.SsspParams()
@@ -203,6 +204,7 @@ class SearchParams:
         p.rounds_per_poll, p.grid, p.count_minimal = self.rounds_per_poll, self.grid, int(self.count_minimal)
         p.h_xy, p.h_z = self.h_xy, self.h_z
         p.local_batches = self.local_batches
+        p.launch_mode = int(self.graph_rounds)
         return p
 
 

@@ -69,7 +69,7 @@ def make_costs(o, rng, kind="jitter"):
 
 
 def run_case(ctx, X, Y, L, policy, slots, n_nets, seed, cost_kind, roi_frac=0.5, keepout=False, dn=None,
-             goal=False):
+             goal=False, graph=False):
     import torch
     from orthoroute_amd.device import SearchParams, cost_bounds, route
     say(f"[route] {policy} {X}x{Y}x{L} slots={slots} nets={n_nets} seed={seed} costs={cost_kind} keepout={keepout}")
@@ -102,8 +102,9 @@ def run_case(ctx, X, Y, L, policy, slots, n_nets, seed, cost_kind, roi_frac=0.5,
     if dn is None:
         dn = (float(rng.choice([0.3, 1.0, 4.0])), int(rng.choice([1, 4, 8])))
     hx, hz = cost_bounds(ctx, g, cost_d) if goal else (0.0, 0.0)
-    say(f"[route]   delta={dn[0]} n_buckets={dn[1]} h=({hx}, {hz})")
-    p = SearchParams(slots=slots, delta=dn[0], n_buckets=dn[1], rounds_per_poll=16, h_xy=hx, h_z=hz)
+    say(f"[route]   delta={dn[0]} n_buckets={dn[1]} h=({hx}, {hz}) graph={graph}")
+    p = SearchParams(slots=slots, delta=dn[0], n_buckets=dn[1], rounds_per_poll=16, h_xy=hx, h_z=hz,
+                     graph_rounds=graph)
     res, stats = route(ctx, g, cost_d, reqs, p, roi_bitmaps=roi_d, keepout_owner=own_d)
     for i, (s, d, nid, roi_idx, ko) in enumerate(reqs):
         path, dist, pred, _, _ = oracle.sssp_canon(o, c, s, d, roi_mask=masks[i],
@@ -127,6 +128,16 @@ def run_case(ctx, X, Y, L, policy, slots, n_nets, seed, cost_kind, roi_frac=0.5,
 def test_route_matches_oracle(ctx, policy, slots, goal):
     for seed, kind in [(1, "jitter"), (2, "ties"), (3, "random")]:
         run_case(ctx, 24, 20, 6, policy, slots, 24, seed + 10 * slots, kind, goal=goal)
+
+
+@pytest.mark.parametrize("goal", [False, True], ids=["dijkstra", "goal"])
+@pytest.mark.parametrize("slots", [1, 7])
+def test_route_graph_rounds_match_oracle(ctx, slots, goal):
+    """Rounds replayed as a captured HIP graph (re-captured when the bucket
+    width or ring size changes between calls on the same handle): the same
+    paths as the oracle, call after call."""
+    for seed, kind, dn in [(4, "jitter", (1.0, 8)), (5, "ties", (0.3, 4)), (6, "random", (1.0, 8))]:
+        run_case(ctx, 24, 20, 6, "reference", slots, 24, seed + 10 * slots, kind, dn=dn, goal=goal, graph=True)
 
 
 @pytest.mark.parametrize("goal", [False, True], ids=["dijkstra", "goal"])

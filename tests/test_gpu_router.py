@@ -33,7 +33,7 @@ def run_oracle(X, Y, L, policy, reqs, cap):
     return res, rec
 
 
-def run_gpu(X, Y, L, policy, reqs, cap, slots, delta=1.0, target_buckets=None):
+def run_gpu(X, Y, L, policy, reqs, cap, slots, delta=1.0, target_buckets=None, graph=False):
     from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig, PathFinderRouter
     cfg = PathFinderConfig()
     cfg.lattice_policy = policy
@@ -43,6 +43,7 @@ def run_gpu(X, Y, L, policy, reqs, cap, slots, delta=1.0, target_buckets=None):
         cfg.sssp_delta, cfg.sssp_buckets = delta, 8
     if target_buckets is not None:
         cfg.sssp_target_buckets = target_buckets
+    cfg.sssp_graph_rounds = graph
     r = PathFinderRouter(cfg)
     r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
     rec = []
@@ -67,6 +68,21 @@ def test_negotiation_matches_oracle(case, slots):
     for nm, *_ in reqs:
         assert list(g_res["paths"].get(nm, [])) == list(o_res["paths"].get(nm, [])), nm
     assert bool(g_res.get("success")) == bool(o_res.get("success"))
+
+
+def test_negotiation_graph_rounds_matches_oracle():
+    """The whole negotiation with every poll's rounds replayed as a captured
+    HIP graph: the oracle's trajectory, history bits and paths."""
+    name, X, Y, L, policy, n, seed, cap = CASES[2]
+    print(f"[router] {name} graph rounds", flush=True)
+    reqs = requests(X, Y, n, seed)
+    o_res, o_rec = run_oracle(X, Y, L, policy, reqs, cap)
+    g_res, g_rec = run_gpu(X, Y, L, policy, reqs, cap, 64, delta=None, graph=True)
+    assert [r[:3] for r in g_rec] == [r[:3] for r in o_rec]
+    for a, b in zip(g_rec, o_rec):
+        np.testing.assert_array_equal(a[3].view(np.uint32), b[3].view(np.uint32), err_msg=f"history it {a[0]}")
+    for nm, *_ in reqs:
+        assert list(g_res["paths"].get(nm, [])) == list(o_res["paths"].get(nm, [])), nm
 
 
 @pytest.mark.parametrize("target", [0, 2, 128])

@@ -41,9 +41,10 @@ class Recorder:
         time.sleep(self.durations.get(job.job_id, 0.02))
         with self.lock:
             del self.running[job.job_id]
-        traj = [(1, 0, 0)]
-        return {"converged": True, "overuse_sum": 0, "paths": {nm: [s, d] for nm, s, d in job.nets}}, traj, \
-            self.durations.get(job.job_id, 0.02), (job.nodes, 4 * job.nodes)
+        dur = self.durations.get(job.job_id, 0.02)
+        traj = [(1, 5, 2), (2, 0, 0)]
+        return {"converged": True, "overuse_sum": 0, "paths": {nm: [s, d] for nm, s, d in job.nets},
+                "_iteration_s": [dur / 2, dur]}, traj, dur, (job.nodes, 4 * job.nodes)
 
 
 def test_footprint_grows_with_board_nets_and_slots():
@@ -116,6 +117,9 @@ def test_oversize_job_is_rejected_and_the_rest_run():
     assert res[1].error is not None and "exceeds the server budget" in res[1].error
     s = summarize(res, 1.0)
     assert s["completed"] == 2 and s["failed"] == 1 and s["converged"] == 2
+    # time to zero overuse: the end of the first iteration with none
+    assert s["reached_zero_overuse"] == 2 and res[0].zero_overuse_s == 0.02
+    assert s["p50_time_to_zero_overuse_s"] == 0.02
 
 
 def test_runner_error_frees_the_slot():
