@@ -227,6 +227,17 @@ typedef struct orgpu_roi_info {
 int orgpu_roi_bfs(orgpu_t *h, const orgpu_graph *g, const orgpu_roi_req *reqs, int32_t n,
                   uint32_t *roi_bitmaps, int32_t n_roi, orgpu_roi_info *info);
 
+/* The same ROI as orgpu_roi_bfs (rows, info and the reference's rules,
+ * UPF:1636-1734 and 4683-4700) for requests on the lattice that `d` describes
+ * (the graph orgpu_lattice_build makes from it), computed in closed form: the
+ * hop distance between two nodes is dx + dy plus the fewest via hops between
+ * their layers through planar layers of the needed directions, so each
+ * wave's marks are a ball whose size per level, meeting step and early stop
+ * follow without traversing edges.  max_depth at most 1024.  Needs no ROI
+ * scratch.  Blocks. */
+int orgpu_roi_lattice(orgpu_t *h, const orgpu_lattice_desc *d, const orgpu_roi_req *reqs, int32_t n,
+                      uint32_t *roi_bitmaps, int32_t n_roi, orgpu_roi_info *info);
+
 /* Upper bound in bytes on the handle's ROI BFS scratch (about 9 bytes per
  * node per workgroup slot; default 32 GB).  No counterpart in the reference,
  * whose GPU pools have fixed sizes (cuda_dijkstra.py:2322-2352); serving

@@ -20,7 +20,7 @@ EXPORTED = [
     "orgpu_reserve", "orgpu_route", "orgpu_roi_bfs",
     "orgpu_usage_scatter", "orgpu_present_build", "orgpu_present_ema", "orgpu_cost_update",
     "orgpu_history_update", "orgpu_overuse_sparse", "orgpu_cost_bounds", "orgpu_route_ms",
-    "orgpu_set_roi_scratch",
+    "orgpu_set_roi_scratch", "orgpu_roi_lattice",
 ]
 
 
@@ -137,6 +137,7 @@ def load():
                            ctypes.POINTER(SsspParams), P, P, i64, P, P, P, ctypes.POINTER(SsspStats)],
         "orgpu_roi_bfs": [P, ctypes.POINTER(Graph), P, i32, P, i32, P],
         "orgpu_set_roi_scratch": [P, i64],
+        "orgpu_roi_lattice": [P, ctypes.POINTER(LatticeDesc), P, i32, P, i32, P],
         "orgpu_usage_scatter": [P, P, i64, P, i64, i32],
         "orgpu_present_build": [P, i64, P, P, P, i64, f32],
         "orgpu_present_ema": [P, i64, P, P, ctypes.c_int, f32, f32, ctypes.POINTER(SparseOut)],

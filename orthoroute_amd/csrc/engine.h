@@ -73,6 +73,7 @@ struct Engine {
     orgpu_roi_info *roi_info = nullptr;
     uint32_t *roi_next = nullptr;
     int64_t roi_req_cap = 0;
+    uint8_t *roi_T = nullptr;   // closed-form ROI: via-hop table of the lattice
 };
 
 struct orgpu_ctx {

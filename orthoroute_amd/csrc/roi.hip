@@ -11,15 +11,18 @@
 // never observed.
 //
 // Truncation keeps the max_nodes lowest node ids (the reference sorts the ROI
-// ascending), minus the nodes its must-keep swap pushes out (positions 0/1 when
-// src/dst fall beyond the cut), then src and dst are re-added.
+// ascending), with src / dst beyond the cut swapped into positions 0 / 1 (the
+// ids there leave), then src and dst are re-added when the caller asks.
 #include <algorithm>
 #include <vector>
 
 #include "common.h"
 #include "engine.h"
 
-#define RWG 1024
+#ifndef ORGPU_ROI_WG
+#define ORGPU_ROI_WG 1024
+#endif
+#define RWG ORGPU_ROI_WG
 
 struct RoiArgs {
     int64_t N;
@@ -60,13 +63,110 @@ __device__ __forceinline__ int nth_set_bit(uint32_t w, int r)
     return pos;
 }
 
+// Truncation and the must-keep re-add (UPF:1700-1734, 4683-4700) on a
+// finished ROI row A holding `seen` marked nodes, then the request's info
+// record.  Called by every thread of the workgroup.
+struct RoiShared {
+    int wsum[RWG / 64];
+    int cut, min0, min1, kept_delta;
+};
+
+__device__ void roi_finish(uint32_t *A, int64_t words, const orgpu_roi_req &q, long long seen, int depth, int met,
+                           orgpu_roi_info *info, RoiShared &sh)
+{
+    const int tid = threadIdx.x;
+    long long kept = seen;
+    if (seen > q.max_nodes) {
+        // Rank the seen nodes by id in one pass over the row: thread t
+        // counts the set bits of its contiguous word range, a block scan
+        // gives each range's first rank, and the ranges that hold ranks
+        // 1 and 2 (the two lowest ids: positions 0 and 1 of the sorted
+        // ROI) and max_nodes (the cut) locate those bits.
+        const int64_t per = (words + RWG - 1) / RWG;
+        const int64_t w0 = std::min<int64_t>((int64_t)tid * per, words);
+        const int64_t w1 = std::min<int64_t>(w0 + per, words);
+        int c = 0;
+        for (int64_t w = w0; w < w1; ++w) c += __popc(A[w]);
+        const int incl = wave_scan_incl(c);
+        if ((tid & 63) == 63) sh.wsum[tid >> 6] = incl;
+        if (tid == 0) { sh.cut = -1; sh.min0 = 0x7FFFFFFF; sh.min1 = 0x7FFFFFFF; sh.kept_delta = 0; }
+        __syncthreads();
+        long long excl = incl - c;
+        for (int j = 0; j < (tid >> 6); ++j) excl += sh.wsum[j];
+        // id of the r-th (1-based) seen node when this range holds it
+        auto find = [&](long long r) -> int32_t {
+            if (!(excl < r && r <= excl + c)) return -1;
+            int need = (int)(r - excl);
+            for (int64_t w = w0; w < w1; ++w) {
+                const uint32_t x = A[w];
+                const int pc = __popc(x);
+                if (need <= pc) return (int32_t)(w * 32 + nth_set_bit(x, need - 1));
+                need -= pc;
+            }
+            return -1;
+        };
+        const int32_t f_cut = find(q.max_nodes), f0 = find(1), f1 = find(2);
+        if (f_cut >= 0) sh.cut = f_cut;
+        if (f0 >= 0) sh.min0 = f0;
+        if (f1 >= 0) sh.min1 = f1;
+        __syncthreads();
+        const int32_t cut = sh.cut;
+        // the bits above the cut leave the row (this thread's words)
+        for (int64_t w = w0; w < w1; ++w) {
+            if (w * 32 + 31 <= cut) continue;
+            const int64_t keep_bits = (int64_t)cut - w * 32 + 1;   // ids w*32 .. cut stay
+            const uint32_t keep = keep_bits <= 0 ? 0u : (1u << keep_bits) - 1u;
+            const uint32_t x = A[w];
+            if (x & ~keep) A[w] = x & keep;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            // must-keep swaps (UPF:1716-1727): a must-keep beyond the cut
+            // trades places with position 0 (the lower of src/dst) or 1 (the
+            // higher) of the ascending ROI, so it stays and the lowest /
+            // second-lowest id leaves (the second swap only when src != dst
+            // and position 1 is inside the cut); the count stays max_nodes
+            const int32_t k0 = min(q.src, q.dst), k1 = max(q.src, q.dst);
+            if (k0 > cut) {
+                A[sh.min0 >> 5] &= ~(1u << (sh.min0 & 31));
+                A[k0 >> 5] |= 1u << (k0 & 31);
+            }
+            if (q.src != q.dst && k1 > cut && q.max_nodes >= 2) {
+                A[sh.min1 >> 5] &= ~(1u << (sh.min1 & 31));
+                A[k1 >> 5] |= 1u << (k1 & 31);
+            }
+            sh.kept_delta = 0;
+        }
+        __syncthreads();
+        kept = q.max_nodes + sh.kept_delta;
+    }
+    if (tid == 0) {
+        // _route_all re-adds src and dst when missing
+        const int32_t ks[2] = {q.src, q.dst};
+        if (q.readd) {
+            for (int j = 0; j < 2; ++j) {
+                const int32_t v = ks[j];
+                if (!((A[v >> 5] >> (v & 31)) & 1u)) { A[v >> 5] |= 1u << (v & 31); kept++; }
+            }
+        }
+        if (info) {
+            info->seen = seen;
+            info->kept = kept;
+            info->depth = depth;
+            info->met = met;
+            info->src_in = (A[q.src >> 5] >> (q.src & 31)) & 1u;
+            info->dst_in = (A[q.dst >> 5] >> (q.dst & 31)) & 1u;
+        }
+    }
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(RWG)
 k_roi(RoiArgs a)
 {
     __shared__ int s_na, s_nb, s_met;   // s_na + s_nb = nodes seen
     __shared__ int s_fa0, s_fa1, s_fb0, s_fb1, s_depth;
-    __shared__ int s_wsum[RWG / 64];
-    __shared__ int s_cut, s_min0, s_min1, s_kept_delta;
+    __shared__ RoiShared sh;
     const int tid = threadIdx.x;
     uint32_t *B = a.scratch_b + (int64_t)blockIdx.x * a.words;
     int32_t *LA = a.list_a + (int64_t)blockIdx.x * a.N;
@@ -183,82 +283,197 @@ k_roi(RoiArgs a)
             B[v >> 5] = 0u;
         }
         __syncthreads();
-        const long long seen = (long long)s_na + s_nb;
-        long long kept = seen;
-        if (seen > q.max_nodes) {
-            // Rank the seen nodes by id in one pass over the row: thread t
-            // counts the set bits of its contiguous word range, a block scan
-            // gives each range's first rank, and the ranges that hold ranks
-            // 1 and 2 (the two lowest ids: positions 0 and 1 of the sorted
-            // ROI) and max_nodes (the cut) locate those bits.
-            const int64_t per = (a.words + RWG - 1) / RWG;
-            const int64_t w0 = std::min<int64_t>((int64_t)tid * per, a.words);
-            const int64_t w1 = std::min<int64_t>(w0 + per, a.words);
-            int c = 0;
-            for (int64_t w = w0; w < w1; ++w) c += __popc(A[w]);
-            const int incl = wave_scan_incl(c);
-            if ((tid & 63) == 63) s_wsum[tid >> 6] = incl;
-            if (tid == 0) { s_cut = -1; s_min0 = 0x7FFFFFFF; s_min1 = 0x7FFFFFFF; s_kept_delta = 0; }
-            __syncthreads();
-            long long excl = incl - c;
-            for (int j = 0; j < (tid >> 6); ++j) excl += s_wsum[j];
-            // id of the r-th (1-based) seen node when this range holds it
-            auto find = [&](long long r) -> int32_t {
-                if (!(excl < r && r <= excl + c)) return -1;
-                int need = (int)(r - excl);
-                for (int64_t w = w0; w < w1; ++w) {
-                    const uint32_t x = A[w];
-                    const int pc = __popc(x);
-                    if (need <= pc) return (int32_t)(w * 32 + nth_set_bit(x, need - 1));
-                    need -= pc;
-                }
-                return -1;
-            };
-            const int32_t f_cut = find(q.max_nodes), f0 = find(1), f1 = find(2);
-            if (f_cut >= 0) s_cut = f_cut;
-            if (f0 >= 0) s_min0 = f0;
-            if (f1 >= 0) s_min1 = f1;
-            __syncthreads();
-            const int32_t cut = s_cut;
-            // the bits above the cut leave the row (this thread's words)
-            for (int64_t w = w0; w < w1; ++w) {
-                if (w * 32 + 31 <= cut) continue;
-                const int64_t keep_bits = (int64_t)cut - w * 32 + 1;   // ids w*32 .. cut stay
-                const uint32_t keep = keep_bits <= 0 ? 0u : (1u << keep_bits) - 1u;
-                const uint32_t x = A[w];
-                if (x & ~keep) A[w] = x & keep;
-            }
-            __syncthreads();
-            if (tid == 0) {
-                kept = q.max_nodes;
-                const int32_t k0 = min(q.src, q.dst), k1 = max(q.src, q.dst);
-                // must-keep swaps push positions 0 / 1 out of the kept range
-                if (k0 > cut && (A[s_min0 >> 5] >> (s_min0 & 31) & 1u)) { A[s_min0 >> 5] &= ~(1u << (s_min0 & 31)); kept--; }
-                if (k1 > cut && (A[s_min1 >> 5] >> (s_min1 & 31) & 1u)) { A[s_min1 >> 5] &= ~(1u << (s_min1 & 31)); kept--; }
-                s_kept_delta = (int)(kept - q.max_nodes);
-            }
-            __syncthreads();
-            kept = q.max_nodes + s_kept_delta;
+        roi_finish(A, a.words, q, (long long)s_na + s_nb, s_depth, s_met, a.info ? a.info + r : nullptr, sh);
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// ROI on the lattice itself, in closed form.
+//
+// Every layer of the routing lattice is a full X x Y grid (planar edges along
+// its direction, or none) and vias join layer pairs at every cell, so the hop
+// distance from (x0, y0, z0) to (x, y, z) is
+//     dx + dy + T[z0][need][z],   need = (dx > 0) | (dy > 0) << 1,
+// with T[z0][need][z] the fewest via hops from layer z0 to layer z through
+// layers that include a planar layer allowing x moves if dx > 0 and one
+// allowing y moves if dy > 0 (a breadth-first search over (layer, covered
+// directions) on the host).  The bidirectional BFS of the reference then
+// marks ball_src(ks) | ball_dst(kd): before the waves meet each wave's marks
+// are a ball; the src step j meets iff the src-dst distance D = 2j - 1 and
+// the dst step j iff D = 2j; the early stop needs only the ball sizes, which
+// follow from counting grid columns by planar distance (clipped to the
+// board) times layers by via hops.  So a net costs a few thousand integer
+// operations plus one pass over the words of its ROI's bounding box,
+// instead of a traversal of ~75,000 nodes' edges.
+// ---------------------------------------------------------------------------
+#define ROI_MAXD 1024      // deepest level the closed form evaluates
+#define ROI_TK 256         // via-hop values; 255 = unreachable
+
+struct LatRoiArgs {
+    int32_t X, Y, L;
+    int64_t N, words;
+    const uint8_t *T;      // [L][4][L]
+    const orgpu_roi_req *reqs;
+    int n;
+    uint32_t *rows;
+    int32_t n_roi;
+    orgpu_roi_info *info;
+    uint32_t *next;
+};
+
+__device__ __forceinline__ int lat_hops(int x0, int y0, const uint8_t *Tz, int L, int x, int y, int z)
+{
+    const int dx = abs(x - x0), dy = abs(y - y0);
+    const int t = Tz[((dx > 0) | ((dy > 0) << 1)) * L + z];
+    return t == 255 ? 0x3FFFFFFF : dx + dy + t;
+}
+
+__global__ void __launch_bounds__(RWG)
+k_roi_lattice(LatRoiArgs a)
+{
+    __shared__ int s_next;
+    __shared__ uint8_t s_T[2][4 * ORGPU_MAX_LAYERS];
+    __shared__ int s_zc[2][4][ROI_TK];
+    __shared__ int s_h[2][ROI_MAXD + 1];
+    __shared__ int s_kmax, s_ks, s_kd, s_depth, s_met;
+    __shared__ unsigned long long s_seen;
+    __shared__ RoiShared sh;
+    const int tid = threadIdx.x;
+    const int lane = (int)__lane_id();
+    const int L = a.L, X = a.X, Y = a.Y;
+    const int64_t plane = (int64_t)X * Y;
+    for (;;) {
+        if (tid == 0) s_next = (int)atomicAdd(a.next, 1u);
+        __syncthreads();
+        const int r = s_next;
+        __syncthreads();
+        if (r >= a.n) break;
+        const orgpu_roi_req q = a.reqs[r];
+        uint32_t *A = a.rows + (int64_t)q.roi_index * a.words;
+        const int z0 = (int)(q.src / plane), z1 = (int)(q.dst / plane);
+        const int x0 = (int)((q.src % plane) % X), y0 = (int)((q.src % plane) / X);
+        const int x1 = (int)((q.dst % plane) % X), y1 = (int)((q.dst % plane) / X);
+        const int md = min(q.max_depth, ROI_MAXD);
+        for (int i = tid; i < 4 * L; i += RWG) {
+            s_T[0][i] = a.T[(int64_t)z0 * 4 * L + i];
+            s_T[1][i] = a.T[(int64_t)z1 * 4 * L + i];
         }
-        if (tid == 0) {
-            // _route_all re-adds src and dst when missing
-            const int32_t ks[2] = {q.src, q.dst};
-            if (q.readd) {
-                for (int j = 0; j < 2; ++j) {
-                    const int32_t v = ks[j];
-                    if (!((A[v >> 5] >> (v & 31)) & 1u)) { A[v >> 5] |= 1u << (v & 31); kept++; }
-                }
-            }
-            if (a.info) {
-                a.info[r].seen = seen;
-                a.info[r].kept = kept;
-                a.info[r].depth = s_depth;
-                a.info[r].met = s_met;
-                a.info[r].src_in = (A[q.src >> 5] >> (q.src & 31)) & 1u;
-                a.info[r].dst_in = (A[q.dst >> 5] >> (q.dst & 31)) & 1u;
+        for (int i = tid; i < 2 * 4 * ROI_TK; i += RWG) (&s_zc[0][0][0])[i] = 0;
+        for (int i = tid; i < 2 * (ROI_MAXD + 1); i += RWG) (&s_h[0][0])[i] = 0;
+        if (tid == 0) { s_seen = 0ull; s_kmax = 0; }
+        __syncthreads();
+        // layers by via hops: zc[w][need][t] = #{z : T[zw][need][z] == t}
+        for (int i = tid; i < 2 * 4 * L; i += RWG) {
+            const int w = i / (4 * L), rem = i - w * 4 * L;
+            const int t = s_T[w][rem];
+            if (t != 255) {
+                atomicAdd(&s_zc[w][rem / L][t], 1);
+                atomicMax(&s_kmax, t);
             }
         }
         __syncthreads();
+        // ball sizes: h[w][j] = nodes at distance exactly j (j <= md) from
+        // src (w = 0) or dst (w = 1): columns at planar distance m, by
+        // whether dx and dy are nonzero, times layers at via distance j - m
+        const int kmax = s_kmax;
+        for (int i = tid; i < 2 * (md + 1); i += RWG) {
+            const int w = i / (md + 1), m = i - w * (md + 1);
+            const int cx = w ? x1 : x0, cy = w ? y1 : y0;
+            auto nx = [&](int d) { return (cx - d >= 0 ? 1 : 0) + (cx + d <= X - 1 ? 1 : 0); };
+            auto ny = [&](int d) { return (cy - d >= 0 ? 1 : 0) + (cy + d <= Y - 1 ? 1 : 0); };
+            int cnt[4];
+            cnt[0] = m == 0 ? 1 : 0;
+            cnt[1] = m > 0 ? nx(m) : 0;
+            cnt[2] = m > 0 ? ny(m) : 0;
+            int c3 = 0;
+            for (int d = 1; d < m; ++d) c3 += nx(d) * ny(m - d);
+            cnt[3] = c3;
+            for (int c = 0; c < 4; ++c) {
+                if (!cnt[c]) continue;
+                for (int t = 0; t <= kmax && m + t <= md; ++t) {
+                    const int z = s_zc[w][c][t];
+                    if (z) atomicAdd(&s_h[w][m + t], cnt[c] * z);
+                }
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            // the reference's level loop (UPF:1653-1700) on the ball sizes
+            const int D = lat_hops(x0, y0, s_T[0], L, x1, y1, z1);
+            const bool same = q.src == q.dst;
+            const double stop = (double)q.max_nodes * 1.5;
+            long long seen = same ? 1 : 2;
+            int fa = 1, fb = 1, depth = 0, met = 0, ks = 0, kd = 0;
+            while (depth < md && (fa > 0 || fb > 0) && !met) {
+                const int j = depth + 1;
+                if (fa > 0) {
+                    if (!same && D == 2 * j - 1) met = 1;
+                    fa = s_h[0][j];
+                    seen += fa;
+                }
+                ks = j;
+                if (!met) {
+                    if (same) {
+                        // both waves start at the node: the dst wave's first
+                        // step touches the src wave's marks
+                        if (s_h[0][1] > 0) met = 1;
+                        fb = 0;
+                    } else if (fb > 0) {
+                        if (D == 2 * j) met = 1;
+                        fb = s_h[1][j];
+                        seen += fb;
+                    }
+                    kd = j;
+                }
+                depth = j;
+                if ((double)seen > stop) break;
+            }
+            s_ks = ks; s_kd = kd; s_depth = depth; s_met = met;
+        }
+        for (int64_t w = tid; w < a.words; w += RWG) A[w] = 0u;
+        __syncthreads();
+        // the row: words over the bounding box of both balls, all layers;
+        // a wave takes one (layer, y) row segment, each half-wave one word,
+        // one lane per node.  A word shared with the previous segment (rows
+        // shorter than a word apart) belongs to that segment only.
+        const int ks = s_ks, kd = s_kd;
+        const int xlo = max(0, min(x0 - ks, x1 - kd)), xhi = min(X - 1, max(x0 + ks, x1 + kd));
+        const int ylo = max(0, min(y0 - ks, y1 - kd)), yhi = min(Y - 1, max(y0 + ks, y1 + kd));
+        const int nyb = yhi - ylo + 1;
+        const int nrows = L * nyb;
+        unsigned long long my_seen = 0;
+        for (int row = tid >> 6; row < nrows; row += RWG / 64) {
+            const int z = row / nyb, y = ylo + (row - z * nyb);
+            const int64_t base = (int64_t)z * plane + (int64_t)y * X;
+            int64_t wa = (base + xlo) >> 5;
+            const int64_t wb = (base + xhi) >> 5;
+            int64_t pbase = -1;
+            if (y > ylo) pbase = base - X;
+            else if (z > 0) pbase = (int64_t)(z - 1) * plane + (int64_t)yhi * X;
+            if (pbase >= 0) wa = max(wa, ((pbase + xhi) >> 5) + 1);
+            for (int64_t wp = wa; wp <= wb; wp += 2) {
+                const int64_t w = wp + (lane >> 5);
+                const int64_t node = w * 32 + (lane & 31);
+                bool in = false;
+                if (w <= wb && node < a.N) {
+                    const int nz = (int)(node / plane);
+                    const int64_t rem = node - (int64_t)nz * plane;
+                    const int ny_ = (int)(rem / X), nx_ = (int)(rem - (int64_t)ny_ * X);
+                    in = lat_hops(x0, y0, s_T[0], L, nx_, ny_, nz) <= ks ||
+                         lat_hops(x1, y1, s_T[1], L, nx_, ny_, nz) <= kd;
+                }
+                const uint64_t bal = __ballot(in);
+                if ((lane & 31) == 0 && w <= wb) {
+                    const uint32_t word = lane < 32 ? (uint32_t)bal : (uint32_t)(bal >> 32);
+                    A[w] = word;
+                    my_seen += (unsigned long long)__popc(word);
+                }
+            }
+        }
+        if (my_seen) atomicAdd(&s_seen, my_seen);
+        __syncthreads();
+        roi_finish(A, a.words, q, (long long)s_seen, s_depth, s_met, a.info ? a.info + r : nullptr, sh);
     }
 }
 
@@ -331,6 +546,106 @@ extern "C" int orgpu_roi_bfs(orgpu_t *h, const orgpu_graph *g, const orgpu_roi_r
     hipLaunchKernelGGL(k_roi, dim3(slots), dim3(RWG), 0, s, a);
     HIP_CHECK_RET(hipGetLastError());
     if (info) HIP_CHECK_RET(hipMemcpyAsync(info, dinfo, sizeof(orgpu_roi_info) * n, hipMemcpyDeviceToHost, s));
+    HIP_CHECK_RET(hipStreamSynchronize(s));
+    return ORGPU_OK;
+}
+
+// Fewest via hops between layers covering the needed planar directions
+// (bit 0: x moves on an 'h' or 'hv' layer, bit 1: y moves on a 'v' or 'hv'
+// layer): T[z0][need][z], 255 where no layer sequence exists.  Each via
+// pair joins its two layers in both directions.
+static int lattice_hop_table(const orgpu_lattice_desc *d, std::vector<uint8_t> &T)
+{
+    const int L = d->L;
+    std::vector<int> cov(L, 0);
+    for (int z = 0; z < L; ++z)
+        if (d->planar[z]) cov[z] = d->dir[z] == 1 ? 1 : d->dir[z] == 0 ? 2 : 3;
+    std::vector<std::vector<int>> adj(L);
+    for (int i = 0; i < d->npairs; ++i) {
+        const int f = d->pair_from[i], t = d->pair_to[i];
+        adj[f].push_back(t);
+        adj[t].push_back(f);
+    }
+    T.assign((size_t)L * 4 * L, 255);
+    std::vector<int> dist((size_t)L * 4);
+    std::vector<int> cur, nxt;
+    for (int z0 = 0; z0 < L; ++z0) {
+        std::fill(dist.begin(), dist.end(), -1);
+        cur.assign(1, z0 * 4 + cov[z0]);
+        dist[z0 * 4 + cov[z0]] = 0;
+        while (!cur.empty()) {
+            nxt.clear();
+            for (int st : cur) {
+                const int z = st >> 2, m = st & 3;
+                for (int b : adj[z]) {
+                    const int ns = b * 4 + (m | cov[b]);
+                    if (dist[ns] < 0) { dist[ns] = dist[st] + 1; nxt.push_back(ns); }
+                }
+            }
+            cur.swap(nxt);
+        }
+        for (int z = 0; z < L; ++z)
+            for (int m = 0; m < 4; ++m) {
+                const int dd = dist[z * 4 + m];
+                if (dd < 0) continue;
+                if (dd > 254) return orgpu_fail(ORGPU_EINVAL, "roi_lattice: via distance %d too large", dd);
+                for (int need = 0; need < 4; ++need)
+                    if ((need & ~m) == 0) {
+                        uint8_t &e = T[((size_t)z0 * 4 + need) * L + z];
+                        if (dd < e) e = (uint8_t)dd;
+                    }
+            }
+    }
+    return ORGPU_OK;
+}
+
+extern "C" int orgpu_roi_lattice(orgpu_t *h, const orgpu_lattice_desc *d, const orgpu_roi_req *reqs, int32_t n,
+                                 uint32_t *roi_bitmaps, int32_t n_roi, orgpu_roi_info *info)
+{
+    if (!h || !d || n < 0 || (n > 0 && (!reqs || !roi_bitmaps))) return orgpu_fail(ORGPU_EINVAL, "roi_lattice: bad argument");
+    if (n == 0) return ORGPU_OK;
+    int64_t N = 0, E = 0;
+    int rc = orgpu_lattice_count(d, &N, &E);
+    if (rc) return rc;
+    if (N >= KEY_MAX_NODES) return orgpu_fail(ORGPU_EINVAL, "roi_lattice: lattice too large");
+    for (int i = 0; i < d->npairs; ++i)
+        if (!d->pair_from || !d->pair_to || d->pair_from[i] < 0 || d->pair_from[i] >= d->L || d->pair_to[i] < 0 ||
+            d->pair_to[i] >= d->L)
+            return orgpu_fail(ORGPU_EINVAL, "roi_lattice: via pair %d outside the layers", i);
+    std::vector<int> row_user(n_roi > 0 ? n_roi : 0, -1);
+    for (int i = 0; i < n; ++i) {
+        const orgpu_roi_req &q = reqs[i];
+        if (q.src < 0 || q.src >= N || q.dst < 0 || q.dst >= N || q.roi_index < 0 || q.roi_index >= n_roi ||
+            q.max_nodes < 1 || q.max_depth < 0)
+            return orgpu_fail(ORGPU_EINVAL, "roi_lattice: request %d invalid (src %d dst %d row %d of %d)", i, q.src,
+                              q.dst, q.roi_index, n_roi);
+        if (q.max_depth > ROI_MAXD)
+            return orgpu_fail(ORGPU_EINVAL, "roi_lattice: request %d depth %d above %d (use orgpu_roi_bfs)", i,
+                              q.max_depth, ROI_MAXD);
+        if (row_user[q.roi_index] >= 0)
+            return orgpu_fail(ORGPU_EINVAL, "roi_lattice: requests %d and %d share bitmap row %d", row_user[q.roi_index],
+                              i, q.roi_index);
+        row_user[q.roi_index] = i;
+    }
+    std::vector<uint8_t> T;
+    rc = lattice_hop_table(d, T);
+    if (rc) return rc;
+    HIP_CHECK_RET(hipSetDevice(h->device));
+    Engine &en = h->en;
+    rc = ensure_roi_staging(en, n);
+    if (rc) return rc;
+    if (!en.roi_T) HIP_CHECK_RET(hipMalloc((void **)&en.roi_T, (size_t)ORGPU_MAX_LAYERS * 4 * ORGPU_MAX_LAYERS));
+    hipStream_t s = h->stream;
+    HIP_CHECK_RET(hipMemcpyAsync(en.roi_T, T.data(), T.size(), hipMemcpyHostToDevice, s));
+    HIP_CHECK_RET(hipMemcpyAsync(en.roi_req, reqs, sizeof(orgpu_roi_req) * n, hipMemcpyHostToDevice, s));
+    HIP_CHECK_RET(hipMemsetAsync(en.roi_next, 0, sizeof(uint32_t), s));
+    LatRoiArgs a;
+    a.X = d->X; a.Y = d->Y; a.L = d->L; a.N = N; a.words = (N + 31) / 32;
+    a.T = en.roi_T; a.reqs = en.roi_req; a.n = n; a.rows = roi_bitmaps; a.n_roi = n_roi;
+    a.info = en.roi_info; a.next = en.roi_next;
+    hipLaunchKernelGGL(k_roi_lattice, dim3(std::min(n, 1024)), dim3(RWG), 0, s, a);
+    HIP_CHECK_RET(hipGetLastError());
+    if (info) HIP_CHECK_RET(hipMemcpyAsync(info, en.roi_info, sizeof(orgpu_roi_info) * n, hipMemcpyDeviceToHost, s));
     HIP_CHECK_RET(hipStreamSynchronize(s));
     return ORGPU_OK;
 }

@@ -1180,6 +1180,7 @@ void engine_release(Engine &en)
     if (en.roi_req) hipFree(en.roi_req);
     if (en.roi_info) hipFree(en.roi_info);
     if (en.roi_next) hipFree(en.roi_next);
+    if (en.roi_T) hipFree(en.roi_T);
     Engine fresh;
     fresh.stream = en.stream;
     fresh.device = en.device;

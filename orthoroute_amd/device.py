@@ -116,6 +116,7 @@ class DeviceLattice:
         self.X, self.Y, self.L, self.policy = int(X), int(Y), int(L), policy
         self.pitch, self.via_cost = float(pitch), float(via_cost)
         d, keep = _lattice_desc(self.X, self.Y, self.L, policy, self.pitch, self.via_cost)
+        self.desc, self._desc_keep = d, keep      # for the closed-form ROI
         n, e = ctypes.c_int64(), ctypes.c_int64()
         _abi.check(ctx.lib.orgpu_lattice_count(ctypes.byref(d), ctypes.byref(n), ctypes.byref(e)))
         self.N, self.E = n.value, e.value
@@ -358,12 +359,17 @@ def route_ms(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=N
     return out, stats
 
 
-def roi_bfs(ctx, lat, requests, rows, readd=True):
+ROI_LATTICE_MAX_DEPTH = 1024
+
+
+def roi_bfs(ctx, lat, requests, rows, readd=True, method="auto"):
     """Batched ROI extraction.  requests: list of (src, dst, max_depth,
     max_nodes, row); rows: device u32[n_roi][ceil(N/32)] written in place.
     readd: re-add src/dst after truncation (the main search of _route_all) or
-    not (its wider-ROI fallback).  Returns a list of
-    (seen, kept, depth, met, src_in, dst_in)."""
+    not (its wider-ROI fallback).  method: "bfs" traverses the CSR;
+    "lattice" computes the same ROI in closed form from the lattice shape
+    (DeviceLattice only); "auto" takes "lattice" where it applies.
+    Returns a list of (seen, kept, depth, met, src_in, dst_in)."""
     n = len(requests)
     if n == 0:
         return []
@@ -373,10 +379,17 @@ def roi_bfs(ctx, lat, requests, rows, readd=True):
     reqs["max_nodes"], reqs["roi_index"] = q[:, 3], q[:, 4]
     reqs["readd"] = int(bool(readd))
     info = np.zeros(n, dtype=ROI_INFO)
-    g = lat.graph(lat.base_cost)
     vp = ctypes.c_void_p
-    _abi.check(ctx.lib.orgpu_roi_bfs(ctx.handle, ctypes.byref(g), reqs.ctypes.data_as(vp), n,
-                                     _ptr(rows), int(rows.shape[0]), info.ctypes.data_as(vp)))
+    closed = getattr(lat, "desc", None) is not None and int(q[:, 2].max()) <= ROI_LATTICE_MAX_DEPTH
+    if method == "lattice" and not closed:
+        raise ValueError("closed-form ROI needs a DeviceLattice and depths up to 1024")
+    if method == "lattice" or (method == "auto" and closed):
+        _abi.check(ctx.lib.orgpu_roi_lattice(ctx.handle, ctypes.byref(lat.desc), reqs.ctypes.data_as(vp), n,
+                                             _ptr(rows), int(rows.shape[0]), info.ctypes.data_as(vp)))
+    else:
+        g = lat.graph(lat.base_cost)
+        _abi.check(ctx.lib.orgpu_roi_bfs(ctx.handle, ctypes.byref(g), reqs.ctypes.data_as(vp), n,
+                                         _ptr(rows), int(rows.shape[0]), info.ctypes.data_as(vp)))
     return list(zip(info["seen"].tolist(), info["kept"].tolist(), info["depth"].tolist(), info["met"].tolist(),
                     (info["src_in"] != 0).tolist(), (info["dst_in"] != 0).tolist()))
 

@@ -26,7 +26,8 @@ def unpack(row, N):
     return np.unpackbits(row.view(np.uint8), bitorder="little")[:N].astype(bool)
 
 
-def test_roi_matches_reference_golden(ctx, golden_dir):
+@pytest.mark.parametrize("method", ["bfs", "lattice"])
+def test_roi_matches_reference_golden(ctx, golden_dir, method):
     import torch
     from orthoroute_amd.device import roi_bfs
     g = np.load(os.path.join(golden_dir, "roi_bfs.npz"))
@@ -41,8 +42,8 @@ def test_roi_matches_reference_golden(ctx, golden_dir):
         s, d, b = int(g["src"][i]), int(g["dst"][i]), float(g["bonus"][i])
         _, depth = oracle.roi_radius(o, s, d, b)
         reqs.append((s, d, depth, int(g["max_nodes"][i]), i))
-    say(f"[roi] {n} golden cases")
-    info = roi_bfs(ctx, lat, reqs, rows)
+    say(f"[roi] {n} golden cases ({method})")
+    info = roi_bfs(ctx, lat, reqs, rows, method=method)
     rows_h = rows.cpu().numpy().view(np.uint32)
     off = 0
     for i in range(n):
@@ -54,7 +55,8 @@ def test_roi_matches_reference_golden(ctx, golden_dir):
         assert info[i][1] == len(ref)
 
 
-def test_roi_random_truncation(ctx):
+@pytest.mark.parametrize("method", ["bfs", "lattice"])
+def test_roi_random_truncation(ctx, method):
     import torch
     from orthoroute_amd.device import roi_bfs
     rng = np.random.default_rng(4)
@@ -73,8 +75,8 @@ def test_roi_random_truncation(ctx):
             mx = int(rng.choice([50, 300, 2000, 50_000]))
             depth = int(rng.integers(0, 80 if X < 100 else 160))
             reqs.append((s, d, depth, mx, i))
-        say(f"[roi] random {policy} {X}x{Y}x{L}")
-        roi_bfs(ctx, lat, reqs, rows)
+        say(f"[roi] random {policy} {X}x{Y}x{L} ({method})")
+        roi_bfs(ctx, lat, reqs, rows, method=method)
         rows_h = rows.cpu().numpy().view(np.uint32)
         for i, (s, d, depth, mx, _) in enumerate(reqs):
             out = np.empty(o.N, np.int32)
@@ -83,6 +85,55 @@ def test_roi_random_truncation(ctx):
             ref = set(out[:R].tolist()) | {s, d}
             got = set(np.nonzero(unpack(rows_h[i], lat.N))[0].tolist())
             assert got == ref, f"{policy} case {i} (s={s} d={d} depth={depth} max={mx})"
+
+
+def test_roi_lattice_equals_bfs_and_oracle(ctx):
+    """The closed-form ROI against the BFS kernel (rows and every info field)
+    and the oracle's BFS (marks, depth, meet flag) on every layer count from
+    1 to 8, both via policies, with depth caps from 0 to 300, node caps from
+    3 to 50,000, with and without the re-add, and src == dst (oracle only:
+    both of its waves start at the node)."""
+    import ctypes
+    import torch
+    from orthoroute_amd.device import roi_bfs
+    rng = np.random.default_rng(11)
+    for policy in ("reference", "adjacent"):
+        for L in range(1, 9):
+            X, Y = int(rng.integers(20, 70)), int(rng.integers(15, 60))
+            lat = ctx.build_lattice(X, Y, L, policy)
+            o = oracle.Lattice(X, Y, L, policy)
+            n = 48
+            words = (lat.N + 31) // 32
+            reqs = []
+            for i in range(n):
+                s, d = (int(v) for v in rng.choice(lat.N, 2, replace=False))
+                if i % 12 == 0:
+                    d = s
+                depth = int(rng.choice([0, 1, 2, 7, 25, 60, 150, 300]))
+                mx = int(rng.choice([3, 50, 600, 5000, 50_000]))
+                reqs.append((s, d, depth, mx, i))
+            for readd in (True, False):
+                say(f"[roi] lattice vs bfs {policy} {X}x{Y}x{L} readd={readd}")
+                r1 = torch.zeros((n, words), dtype=torch.int32, device=ctx.device)
+                r2 = torch.full((n, words), -1, dtype=torch.int32, device=ctx.device)
+                i1 = roi_bfs(ctx, lat, reqs, r1, readd=readd, method="bfs")
+                i2 = roi_bfs(ctx, lat, reqs, r2, readd=readd, method="lattice")
+                h1, h2 = r1.cpu().numpy(), r2.cpu().numpy()
+                for i, (s, d, depth, mx, _) in enumerate(reqs):
+                    out = np.empty(o.N, np.int32)
+                    seen = np.empty(o.N, np.uint8)
+                    dd, mm = ctypes.c_int(0), ctypes.c_int(0)
+                    R = oracle.lib().orc_roi_bfs(o.N, oracle._p(o.indptr), oracle._p(o.indices), s, d, depth, mx,
+                                                 oracle._p(out), oracle._p(seen), ctypes.byref(dd), ctypes.byref(mm))
+                    ref = set(out[:R].tolist()) | ({s, d} if readd else set())
+                    got = set(np.nonzero(unpack(h2[i], lat.N))[0].tolist())
+                    tag = f"{policy} {X}x{Y}x{L} case {i} (s={s} d={d} depth={depth} max={mx} readd={readd})"
+                    assert got == ref, tag
+                    assert (i2[i][2], i2[i][3]) == (dd.value, mm.value), tag
+                    assert i2[i][0] == int((seen != 0).sum()), tag
+                    if s != d:
+                        np.testing.assert_array_equal(h1[i], h2[i], err_msg=tag)
+                        assert i1[i] == i2[i], tag
 
 
 def _dev(ctx, a):
