@@ -250,6 +250,9 @@ class RoutingServer:
             res, traj, dt, (N, E) = self._runner(job, self.device, slots, self.roi_limit)
             t_it = res.get("_iteration_s") or []
             zero = next((t for (it, s, c), t in zip(traj, t_it) if s == 0 and c == 0), None)
+            if zero is None and res.get("converged"):
+                # the converging iteration exits before its callback (UPF:4023-4027)
+                zero = dt
             out = JobResult(job.job_id, N, E, len(job.nets), slots, fp, bool(res.get("converged")),
                             len(traj), int(res.get("overuse_sum") or 0), res.get("paths", {}), traj,
                             t_arr, t_admit, self.now(), dt, zero)
