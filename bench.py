@@ -479,7 +479,7 @@ def geometry(r):
     tracks, vias = r._generate_geometry_from_paths()
     t = time.perf_counter() - t0
     hops = sum(max(len(p) - 1, 0) for p in r.net_paths.values())
-    sample = dict(list((n, p) for n, p in r.net_paths.items() if p)[:500])
+    sample = dict(list((n, list(map(int, p))) for n, p in r.net_paths.items() if p)[:500])
     g = r.lattice.geom
     t1 = time.perf_counter()
     et, ev = G.generate(sample, r._Nx, r._Ny, r._Nz, r.config.layer_names or [], g.pitch, g.grid_min_x,

@@ -28,6 +28,7 @@ import logging
 import math
 import os
 import random
+import threading
 import time
 from collections import defaultdict
 from typing import Dict, List, Optional, Set, Tuple
@@ -48,6 +49,15 @@ f32 = np.float32
 ROI_THRESHOLD_STEPS = 125      # UPF:4617
 MAX_ROI_NODES = 50_000         # UPF:1659
 KEEPOUT_COST = 1e9             # UPF:3182
+
+
+class NodePath(np.ndarray):
+    """A routed net's node ids (int32) as the negotiation keeps them between
+    iterations: an array view of the search output, true when non-empty like
+    the reference's list.  route_multiple_nets hands back lists."""
+
+    def __bool__(self):
+        return self.size > 0
 
 
 class GeometryPayload:
@@ -576,7 +586,18 @@ class PathFinderRouter:
         if not tasks:
             self._negotiation_ran = True
             return {}
-        return self._pathfinder_negotiation(tasks, progress_cb, iteration_cb)
+        try:
+            return self._pathfinder_negotiation(tasks, progress_cb, iteration_cb)
+        finally:
+            self._paths_as_lists()
+
+    def _paths_as_lists(self):
+        """net_paths values as the reference's lists of node ids (the
+        negotiation keeps them as NodePath arrays between iterations); in
+        place, so a result dict that holds net_paths sees the lists."""
+        for n, p in self.net_paths.items():
+            if isinstance(p, np.ndarray):
+                self.net_paths[n] = p.tolist()
 
     def _parse_requests(self, requests):
         tasks = {}
@@ -1082,18 +1103,48 @@ class PathFinderRouter:
         bonus = self.stagnation_counter * 0.6
         if over_map is None:
             over_map = np.zeros(self.accounting.E, dtype=np.float32)
-        ordered = self._order_nets_by_difficulty(tasks, over_map)
         total = len(tasks)
         use_ko = total < 100 and len(self._ko_nodes) > 0
         ko_owner = self._keepout_owner_array() if use_ko else None
-        n = len(ordered)
-        sd = np.fromiter((v for nid in ordered for v in tasks[nid]), dtype=np.int64, count=2 * n).reshape(n, 2)
+        # The searches of an iteration read one cost snapshot, so their
+        # results do not depend on the routing order.  For a large batch they
+        # start while a helper thread computes that order (the library calls
+        # release the GIL), longest first by Manhattan distance, the order's
+        # leading term: slots refill in request order, and long searches
+        # started last would hold a refill batch open.  Results and requests
+        # are then put in routing order, which the fallback and the commit
+        # follow.
+        helper = None
+        if total > 1024 and not use_ko:
+            box = {}
+
+            def order_job():
+                try:
+                    box["ordered"] = self._order_nets_by_difficulty(tasks, over_map)
+                except BaseException as exc:     # re-raised below
+                    box["error"] = exc
+            helper = threading.Thread(target=order_job, name="net-order")
+            helper.start()
+            names = list(tasks)
+            st = np.fromiter((v for nid in names for v in tasks[nid]), dtype=np.int64,
+                             count=2 * len(names)).reshape(-1, 2)
+            X_, plane_ = self._Nx, self._Nx * self._Ny
+            zs_, rs_ = np.divmod(st[:, 0], plane_)
+            zd_, rd_ = np.divmod(st[:, 1], plane_)
+            dist_ = np.abs(rd_ % X_ - rs_ % X_) + np.abs(rd_ // X_ - rs_ // X_) + np.abs(zd_ - zs_)
+            search_order = [names[i] for i in np.argsort(-dist_, kind="stable").tolist()]
+        else:
+            ordered = self._order_nets_by_difficulty(tasks, over_map)
+            search_order = ordered
+        n = len(search_order)
+        sd = np.fromiter((v for nid in search_order for v in tasks[nid]), dtype=np.int64,
+                         count=2 * n).reshape(n, 2)
         X, plane = self._Nx, self._Nx * self._Ny
         rs, rd = sd[:, 0] % plane, sd[:, 1] % plane
         md = np.abs(rd % X - rs % X) + np.abs(rd // X - rs // X)
         reqs = np.empty((n, 5), dtype=np.float64)
         reqs[:, 0], reqs[:, 1] = sd[:, 0], sd[:, 1]
-        reqs[:, 2] = [self._get_net_id(nid) for nid in ordered] if use_ko else -1
+        reqs[:, 2] = [self._get_net_id(nid) for nid in search_order] if use_ko else -1
         reqs[:, 3] = md < ROI_THRESHOLD_STEPS
         reqs[:, 4] = bonus
         # a batch that the engine runs at once (no slot refills) is only as
@@ -1101,11 +1152,23 @@ class PathFinderRouter:
         # last estimate when none of the batch's nets holds a path)
         hint = None
         if n <= 1024:
-            hint = self._depth_hint(ordered, sd)
+            hint = self._depth_hint(search_order, sd)
             if hint is None:
                 hint = self._last_depth_hint
             self._last_depth_hint = hint
-        results = self._search_batch(reqs, ko_owner, readd=True, depth_hint=hint)
+        try:
+            results = self._search_batch(reqs, ko_owner, readd=True, depth_hint=hint)
+        finally:
+            if helper is not None:
+                helper.join()
+        if helper is not None:
+            if "error" in box:
+                raise box["error"]
+            ordered = box["ordered"]
+            pos = {nm: i for i, nm in enumerate(search_order)}
+            perm = np.fromiter((pos[nm] for nm in ordered), dtype=np.int64, count=n)
+            results = [results[i] for i in perm.tolist()]
+            reqs = reqs[perm]
         # wider-ROI fallback for the first five failures, in routing order
         fb = [i for i, r in enumerate(results) if r is None or len(r[0]) <= 1][:5]
         self.full_graph_fallback_count = len(fb)
@@ -1129,7 +1192,7 @@ class PathFinderRouter:
             if r is not None and len(r[0]) > 1:
                 path, edges = r
                 inc.append(edges)
-                paths[nid] = path.tolist()
+                paths[nid] = path.view(NodePath)
                 marked.append(path)
                 mark_ids.append(get_id(nid))
                 n2e.pop(nid, None)

@@ -85,6 +85,22 @@ def test_negotiation_graph_rounds_matches_oracle():
         assert list(g_res["paths"].get(nm, [])) == list(o_res["paths"].get(nm, [])), nm
 
 
+def test_negotiation_large_batch_matches_oracle():
+    """An iteration of more than 1,024 nets takes the path where the
+    searches start in task order while a helper thread computes the routing
+    order: the oracle's trajectory, history bits and final paths."""
+    X, Y, L, n, cap = 80, 72, 4, 1100, 3
+    print(f"[router] large batch {X}x{Y}x{L} nets={n}", flush=True)
+    reqs = requests(X, Y, n, 31)
+    o_res, o_rec = run_oracle(X, Y, L, "adjacent", reqs, cap)
+    g_res, g_rec = run_gpu(X, Y, L, "adjacent", reqs, cap, 1024, delta=None)
+    assert [r[:3] for r in g_rec] == [r[:3] for r in o_rec]
+    for a, b in zip(g_rec, o_rec):
+        np.testing.assert_array_equal(a[3].view(np.uint32), b[3].view(np.uint32), err_msg=f"history it {a[0]}")
+    for nm, *_ in reqs:
+        assert list(g_res["paths"].get(nm, [])) == list(o_res["paths"].get(nm, [])), nm
+
+
 @pytest.mark.parametrize("target", [0, 2, 128])
 def test_depth_sized_buckets_keep_results(target):
     """The router's bucket width (the lattice's default, widened by the
