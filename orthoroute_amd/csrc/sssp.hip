@@ -786,22 +786,25 @@ struct SeedReq {
 // previous searches touched (each workgroup fills whole chunks), and clears
 // those chunks' touch bits (whole words: every set bit of the slot's record
 // is a listed chunk).  k_seed, which runs after this, restarts the lists.
+// Grid (G, n): request blockIdx.y gets G workgroups that stride over the
+// chunks its slot's previous search touched (a few slots with many chunks
+// and many slots with few both spread over the chip).
 __global__ void k_reset_keys(RoundArgs a, const SeedReq *reqs, int n)
 {
     const uint64_t N = (uint64_t)a.N;
-    for (int r = 0; r < n; ++r) {
-        const uint32_t s = (uint32_t)reqs[r].slot;
-        const uint32_t cnt = a.touch_n[s];
-        const uint32_t *lst = a.touch_list + (uint64_t)s * a.touch_cap;
-        uint64_t *ks = a.keys + (uint64_t)s * N;
-        uint32_t *tb = a.touch_bits + (uint64_t)s * a.touch_words;
-        for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
-            const uint32_t ch = lst[i];
-            const uint64_t lo = (uint64_t)ch * TOUCH_KEYS;
-            const uint64_t hi = lo + TOUCH_KEYS < N ? lo + TOUCH_KEYS : N;
-            for (uint64_t v = lo + threadIdx.x; v < hi; v += blockDim.x) ks[v] = KEY_INF;
-            if (threadIdx.x == 0) tb[ch >> 5] = 0u;
-        }
+    const int r = (int)blockIdx.y;
+    if (r >= n) return;
+    const uint32_t s = (uint32_t)reqs[r].slot;
+    const uint32_t cnt = a.touch_n[s];
+    const uint32_t *lst = a.touch_list + (uint64_t)s * a.touch_cap;
+    uint64_t *ks = a.keys + (uint64_t)s * N;
+    uint32_t *tb = a.touch_bits + (uint64_t)s * a.touch_words;
+    for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
+        const uint32_t ch = lst[i];
+        const uint64_t lo = (uint64_t)ch * TOUCH_KEYS;
+        const uint64_t hi = lo + TOUCH_KEYS < N ? lo + TOUCH_KEYS : N;
+        for (uint64_t v = lo + threadIdx.x; v < hi; v += blockDim.x) ks[v] = KEY_INF;
+        if (threadIdx.x == 0) tb[ch >> 5] = 0u;
     }
 }
 
@@ -1365,7 +1368,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
         }
         if (ns == 0) return 0;
         HIP_CHECK_RET(hipMemcpyAsync(en.seed, en.h_seed, sizeof(SeedReq) * ns, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_reset_keys, dim3(512), dim3(256), 0, s, a, en.seed, ns);
+        hipLaunchKernelGGL(k_reset_keys, dim3(std::max(1, std::min(512, 2048 / ns)), ns), dim3(256), 0, s, a, en.seed, ns);
         hipLaunchKernelGGL(k_seed, dim3((ns + 255) / 256), dim3(256), 0, s, a, en.seed, ns, ms ? en.ms_seed : nullptr);
         HIP_CHECK_RET(hipGetLastError());
         last_seeded = ns;
