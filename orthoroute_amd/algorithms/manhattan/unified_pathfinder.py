@@ -791,15 +791,16 @@ class PathFinderRouter:
         if not paths:
             return
         lens = np.fromiter((len(p) for p in paths), dtype=np.int64, count=len(paths))
-        allp = np.concatenate(paths)
-        if allp.dtype != np.int32:
-            allp = allp.astype(np.int64)
-        # consecutive pairs inside each path (drop the pairs across paths)
-        inner = np.ones(max(len(allp) - 1, 0), dtype=bool)
-        inner[np.cumsum(lens)[:-1] - 1] = False
-        S, D = allp[:-1][inner], allp[1:][inner]
-        NS = np.repeat(np.asarray(nids, np.int64), lens - 1)
-        H, NH = self._hops_of(S, D, NS)
+        allp = np.concatenate(paths).astype(np.int64)
+        ends = np.cumsum(lens)
+        # via hops among the consecutive pairs inside each path (a planar
+        # step moves the index by 1 or X, a via by a multiple of X*Y; the
+        # pairs across two paths are dropped), with the net of each
+        hop = np.abs(allp[1:] - allp[:-1]) >= self._Nx * self._Ny
+        hop[ends[:-1] - 1] = False
+        i = np.nonzero(hop)[0]
+        NS = np.asarray(nids, np.int64)[np.searchsorted(ends, i, side="right")]
+        H, NH = self._hops_of(allp[i], allp[i + 1], NS)
         if len(H) == 0:
             return
         X, plane = self._Nx, self._Nx * self._Ny
