@@ -77,45 +77,53 @@ __device__ void roi_finish(uint32_t *A, int64_t words, const orgpu_roi_req &q, l
     const int tid = threadIdx.x;
     long long kept = seen;
     if (seen > q.max_nodes) {
-        // Rank the seen nodes by id in one pass over the row: thread t
-        // counts the set bits of its contiguous word range, a block scan
-        // gives each range's first rank, and the ranges that hold ranks
-        // 1 and 2 (the two lowest ids: positions 0 and 1 of the sorted
-        // ROI) and max_nodes (the cut) locate those bits.
-        const int64_t per = (words + RWG - 1) / RWG;
-        const int64_t w0 = std::min<int64_t>((int64_t)tid * per, words);
-        const int64_t w1 = std::min<int64_t>(w0 + per, words);
+        // Rank the seen nodes by id.  Each wave owns a contiguous chunk of
+        // the row and its lanes read consecutive words.  Pass 1 counts the
+        // chunk's bits; the prefix of the chunk totals gives each chunk's
+        // first rank; pass 2, in the chunks that hold rank 1, 2 or max_nodes
+        // (the two lowest ids: positions 0 and 1 of the sorted ROI, and the
+        // cut), scans again with a running prefix to locate those bits.
+        const int wid = tid >> 6, lane = tid & 63;
+        const int64_t per = (words + RWG / 64 - 1) / (RWG / 64);
+        const int64_t c0 = std::min<int64_t>((int64_t)wid * per, words);
+        const int64_t c1 = std::min<int64_t>(c0 + per, words);
         int c = 0;
-        for (int64_t w = w0; w < w1; ++w) c += __popc(A[w]);
-        const int incl = wave_scan_incl(c);
-        if ((tid & 63) == 63) sh.wsum[tid >> 6] = incl;
+        for (int64_t w = c0 + lane; w < c1; w += 64) c += __popc(A[w]);
+        const int tot = __shfl(wave_scan_incl(c), 63, 64);
+        if (lane == 0) sh.wsum[wid] = tot;
         if (tid == 0) { sh.cut = -1; sh.min0 = 0x7FFFFFFF; sh.min1 = 0x7FFFFFFF; sh.kept_delta = 0; }
         __syncthreads();
-        long long excl = incl - c;
-        for (int j = 0; j < (tid >> 6); ++j) excl += sh.wsum[j];
-        // id of the r-th (1-based) seen node when this range holds it
-        auto find = [&](long long r) -> int32_t {
-            if (!(excl < r && r <= excl + c)) return -1;
-            int need = (int)(r - excl);
-            for (int64_t w = w0; w < w1; ++w) {
-                const uint32_t x = A[w];
+        long long base = 0;
+        for (int j = 0; j < wid; ++j) base += sh.wsum[j];
+        const long long targets[3] = {(long long)q.max_nodes, 1, 2};
+        bool mine = false;
+        for (int k = 0; k < 3; ++k) mine |= base < targets[k] && targets[k] <= base + tot;
+        if (mine) {
+            long long run = base;
+            for (int64_t s0 = c0; s0 < c1 && run < base + tot; s0 += 64) {
+                const int64_t w = s0 + lane;
+                const uint32_t x = w < c1 ? A[w] : 0u;
                 const int pc = __popc(x);
-                if (need <= pc) return (int32_t)(w * 32 + nth_set_bit(x, need - 1));
-                need -= pc;
+                const int incl = wave_scan_incl(pc);
+                const long long lo = run + incl - pc;   // seen nodes before this lane's word
+                for (int k = 0; k < 3; ++k) {
+                    const long long r = targets[k];
+                    if (lo < r && r <= lo + pc) {
+                        const int32_t id = (int32_t)(w * 32 + nth_set_bit(x, (int)(r - lo) - 1));
+                        if (k == 0) sh.cut = id;
+                        else if (k == 1) sh.min0 = id;
+                        else sh.min1 = id;
+                    }
+                }
+                run += __shfl(incl, 63, 64);
             }
-            return -1;
-        };
-        const int32_t f_cut = find(q.max_nodes), f0 = find(1), f1 = find(2);
-        if (f_cut >= 0) sh.cut = f_cut;
-        if (f0 >= 0) sh.min0 = f0;
-        if (f1 >= 0) sh.min1 = f1;
+        }
         __syncthreads();
         const int32_t cut = sh.cut;
-        // the bits above the cut leave the row (this thread's words)
-        for (int64_t w = w0; w < w1; ++w) {
-            if (w * 32 + 31 <= cut) continue;
+        // the bits above the cut leave the row (the words from the cut's on)
+        for (int64_t w = std::max<int64_t>(0, (int64_t)cut >> 5) + tid; w < words; w += RWG) {
             const int64_t keep_bits = (int64_t)cut - w * 32 + 1;   // ids w*32 .. cut stay
-            const uint32_t keep = keep_bits <= 0 ? 0u : (1u << keep_bits) - 1u;
+            const uint32_t keep = keep_bits <= 0 ? 0u : keep_bits >= 32 ? ~0u : (1u << keep_bits) - 1u;
             const uint32_t x = A[w];
             if (x & ~keep) A[w] = x & keep;
         }
