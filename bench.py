@@ -425,9 +425,10 @@ def serving(args):
     drawn like cfg1 at a density the reference's schedule mostly solves,
     0.75 * X * Y / (X + Y) nets; iteration cap 40).  Reports p50 / p99
     time to zero overuse and latency, jobs per hour, how far the jobs'
-    device work overlapped (sum of per-job routing time / makespan), and
-    re-runs the three quickest jobs alone to compare their trajectories and
-    paths with the mix."""
+    device work overlapped (sum of per-job routing time / makespan), the
+    capacity when all jobs arrive at once (with the serving slots and with
+    one), and re-runs the three quickest jobs alone to compare their
+    trajectories and paths with the mix."""
     import torch
     from orthoroute_amd import workloads
     from orthoroute_amd.serving import RoutingServer, replay, run_job, summarize
@@ -445,6 +446,23 @@ def serving(args):
         f"{summ['reached_zero_overuse']} reached zero overuse, {summ['converged']} converged, "
         f"p50/p99 time to zero overuse {summ['p50_time_to_zero_overuse_s']}/{summ['p99_time_to_zero_overuse_s']}s, "
         f"peak {srv.peak_active} jobs on the GPU")
+    # capacity: the same jobs all at once, with the serving slots and with one
+    # (jobs strictly one after another)
+    import dataclasses
+    burst = [dataclasses.replace(j, arrival_s=0.0) for j in jobs]
+    cap = {}
+    for active in (args.serve_active, 1):
+        bsrv = RoutingServer(hbm_budget=64 << 30, max_active=active, roi_limit=1 << 30)
+        tb = time.perf_counter()
+        bres = replay(bsrv, burst)
+        bsrv.close()
+        mk = time.perf_counter() - tb
+        same = all(b.trajectory == r.trajectory for b, r in zip(bres, res))
+        cap[active] = {"slots": active, "makespan_s": round(mk, 3),
+                       "jobs_per_hour": round(3600.0 * len(jobs) / max(mk, 1e-9), 1),
+                       "same_trajectories_as_stream": bool(same)}
+    log(f"[bench] serving burst: {len(jobs)} jobs in {cap[args.serve_active]['makespan_s']}s with "
+        f"{args.serve_active} slots, {cap[1]['makespan_s']}s with one")
     quick = sorted((r for r in res if r.error is None), key=lambda r: r.route_s)[:3]
     same = []
     for r in quick:
@@ -461,6 +479,8 @@ def serving(args):
         "peak_admitted_gib": round(srv.peak_used / 2 ** 30, 2), "hbm_budget_gib": 64,
         "sum_route_s": round(busy, 3), "overlap": round(busy / max(makespan, 1e-9), 2),
         "alone_equals_mix": f"{sum(same)}/{len(same)}",
+        "burst": cap[args.serve_active], "burst_one_slot": cap[1],
+        "burst_speedup": round(cap[1]["makespan_s"] / max(cap[args.serve_active]["makespan_s"], 1e-9), 2),
         "jobs": [{"id": r.job_id, "nodes": r.nodes, "layers": jobs[r.job_id].L, "nets": r.nets,
                   "iterations": r.iterations, "converged": r.converged,
                   "zero_overuse_s": None if r.zero_overuse_s is None else round(r.zero_overuse_s, 3),
