@@ -1208,7 +1208,7 @@ class PathFinderRouter:
         if inc:
             self.accounting.commit_path(np.concatenate(inc))
         self._mark_barrels_many(marked, mark_ids)
-        routed = sum(1 for p in self.net_paths.values() if p)
+        routed = sum(1 for p in self.net_paths.values() if len(p))
         return routed, len(all_tasks) - routed
 
     # ---------------- negotiation (UPF:3548-4259)
