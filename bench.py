@@ -602,12 +602,32 @@ def cpu_baseline(r, sreqs, gpu_paths, budget_s, X, Y, L, compare=True):
         if t_search > budget_s:
             break
     log(f"[bench] cpu oracle {done} searches in {t_search:.2f}s, {equal}/{done} paths equal to the GPU's")
+    # the same searches over a pool of host threads (the oracle's C calls
+    # release the GIL): nets are independent within an iteration
+    from concurrent.futures import ThreadPoolExecutor
+    workers = max(1, min(16, os.cpu_count() or 1))
+
+    def one(req):
+        s, d, _, use_roi, bonus = req
+        mask = None
+        if use_roi:
+            mask = np.zeros(lat.N, np.uint8)
+            mask[oracle.roi_bfs(lat, s, d, stagnation_bonus=bonus)] = 1
+            mask[s] = mask[d] = 1
+        return oracle.sssp_canon(lat, costs, s, d, roi_mask=mask)[4]
+    t1 = time.perf_counter()
+    with ThreadPoolExecutor(workers) as ex:
+        tr_par = sum(ex.map(one, list(sreqs)[:done]))
+    t_par = time.perf_counter() - t1
+    log(f"[bench] cpu oracle, {workers} threads: {done} searches in {t_par:.2f}s")
     return {"value": round(traversed / max(t_search, 1e-9) / 1e9, 5), "unit": "GTEPS", "cores": 1,
             "kind": "port",
             "sample": f"the first {done} nets of the request list on the step's cost snapshot: oracle ROI BFS "
                       f"(nets under 125 steps) + label-setting search in C on one thread, {t_search:.1f}s",
             "searches_per_s": round(done / max(t_search, 1e-9), 3),
-            "paths_equal_to_gpu": f"{equal}/{done}" if compare else "not compared"}
+            "paths_equal_to_gpu": f"{equal}/{done}" if compare else "not compared",
+            "threads": {"cores": workers, "value": round(tr_par / max(t_par, 1e-9) / 1e9, 5),
+                        "searches_per_s": round(done / max(t_par, 1e-9), 3), "s": round(t_par, 2)}}
 
 
 if __name__ == "__main__":
