@@ -136,7 +136,8 @@ class PathFinderConfig:
     sssp_grid = 0                  # workgroups per round launch (0 = library default)
     sssp_rounds_per_poll = 32      # rounds queued between status polls / slot refills
     sssp_graph_rounds = False      # replay the rounds between polls as one captured HIP graph
-    sssp_target_buckets = 128      # widen the bucket so a batch's deep searches span about this many
+    sssp_target_buckets = None     # widen the bucket so a batch's deep searches span about this many
+                                   # (None: 128 from 1 M nodes up, 32 below)
     max_iterations_cap = None
     count_minimal = False          # also count Dijkstra-minimal nodes/edges per search
     goal_directed = True           # bucket on g + lower bound to the destination (same paths)
@@ -994,8 +995,15 @@ class PathFinderRouter:
         if d is None or nb is None:
             from ...autotune import choose_params
             d2, nb2 = choose_params(self, slots, getattr(self, "_cost_scale", 0.0))
-            if depth is not None and self.config.sssp_target_buckets:
-                d2 = max(d2, float(depth) / float(self.config.sssp_target_buckets))
+            target = self.config.sssp_target_buckets
+            if target is None:
+                # measured: the 12-iteration negotiation on the 96x96x6
+                # lattice (55 k nodes) takes 0.140 s at 32 and 0.153 s at
+                # 128; 30 iterations on the 600x400x18 backplane (4.3 M
+                # nodes) take 5.65 s at 128 and 7.95 s at 32
+                target = 128 if self.N >= 1_000_000 else 32
+            if depth is not None and target:
+                d2 = max(d2, float(depth) / float(target))
             d = d if d is not None else d2
             nb = nb if nb is not None else nb2
         # goal-directed order (same paths) unless the Dijkstra-minimal work is
