@@ -37,7 +37,8 @@ import numpy as np
 
 from ... import _abi
 from ...device import (Context, DeviceLattice, EdgeState, SearchParams, cost_bounds, max_slots, roi_bfs,
-                       route as device_route, route_ms as device_route_ms)
+                       route as device_route, route_flat as device_route_flat,
+                       route_ms as device_route_ms)
 from ...lattice import DIR_H, DIR_HV, DIR_V, layer_directions, reference_via_pairs
 from ...reductions import sum_f32
 from .board_analyzer import analyze_board_characteristics
@@ -1088,8 +1089,8 @@ class PathFinderRouter:
         if keepout_owner is not None:
             ko = torch.from_numpy(keepout_owner).to(self.ctx.device)
         t0 = time.perf_counter()
-        res, st = device_route(self.ctx, dlat, self.accounting.total_cost, sreqs, params, roi_bitmaps=rows,
-                               keepout_owner=ko)
+        nodes, edges, offs, _, st = device_route_flat(self.ctx, dlat, self.accounting.total_cost, sreqs, params,
+                                                      roi_bitmaps=rows, keepout_owner=ko)
         self.stats["sssp_s"] += time.perf_counter() - t0
         self.stats["sssp_kernel_ms"] += st["kernel_ms"]
         self.stats["sssp_rounds"] += st["rounds"]
@@ -1100,9 +1101,11 @@ class PathFinderRouter:
         self.stats["sssp_minimal_nodes"] += st["minimal_nodes"]
         self.stats["sssp_minimal_edges"] += st["minimal_edges"]
         self.stats["sssp_searches"] += len(sreqs)
+        ol = offs.tolist()
         for j, i in enumerate(todo.tolist()):
-            r = res[j]
-            out[i] = None if r is None else (r[0], r[1])
+            a, b = ol[j], ol[j + 1]
+            if b - a > 1:
+                out[i] = (nodes[a:b], edges[a:b - 1])
         return out
 
     def _route_all(self, tasks, all_tasks=None, pres_fac=1.0, iteration=0, over_map=None):

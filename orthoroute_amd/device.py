@@ -239,6 +239,22 @@ def route(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=None
     """Route requests = list of (src, dst, net_id, roi_index, use_keepout) on
     the lattice with edge costs `cost` (device f32[E]).  Returns
     (list of (nodes, edges, dist) or None per request, stats dict)."""
+    nodes, edges, offs, dist, stats = route_flat(ctx, lat, cost, requests, params, roi_bitmaps, keepout_owner,
+                                                 path_capacity)
+    out = []
+    ol = offs.tolist()
+    dl = dist.tolist()
+    for i in range(len(requests)):
+        a, b = ol[i], ol[i + 1]
+        out.append((nodes[a:b], edges[a:b - 1], dl[i]) if b - a > 1 else None)
+    return out, stats
+
+
+def route_flat(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=None, path_capacity=None):
+    """route() without the per-request list: (nodes, edges, offsets,
+    dist, stats), request i's path is nodes[offsets[i]:offsets[i+1]] (a
+    path of one node or none means no path) and its edges
+    edges[offsets[i]:offsets[i+1]-1]."""
     n = len(requests)
     ctx.reserve(lat.N, params.slots, params.n_buckets)
     # orgpu_net_req rows (src, dst, net_id, roi_index, use_keepout, reserved)
@@ -267,16 +283,10 @@ def route(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=None
     # buffers are released; a kept path holds only its call's paths)
     used = int(offs[n])
     nodes, edges = nodes[:used].copy(), edges[:used].copy()
-    out = []
-    ol = offs.tolist()
-    dl = dist.tolist()
-    for i in range(n):
-        a, b = ol[i], ol[i + 1]
-        out.append((nodes[a:b], edges[a:b - 1], dl[i]) if b - a > 1 else None)
     stats = {"rounds": st.rounds, "items": st.items, "edges": st.edges,
              "minimal_nodes": st.minimal_nodes, "minimal_edges": st.minimal_edges,
              "kernel_ms": st.kernel_ms, "launches": st.launches, "round_ms": st.round_ms}
-    return out, stats
+    return nodes, edges, offs, dist[:n], stats
 
 
 MS_POINT = np.dtype([("node", "<i4"), ("cost", "<f4")])
