@@ -196,6 +196,55 @@ int orgpu_route_ms(orgpu_t *h, const orgpu_graph *g, const orgpu_ms_req *reqs, i
                    int32_t *path_nodes, int32_t *path_edges, int64_t path_capacity, int64_t *path_offset,
                    double *path_total, int32_t *path_target, orgpu_sssp_stats *stats);
 
+/* ---- A3 reference-order search ------------------------------------------------ */
+
+/* One request of the reference-order search: SimpleDijkstra.find_path_roi
+ * (UPF:1801-1887) with the reference's own arithmetic and pop order, for
+ * callers that need its exact node sequences (and so its negotiation
+ * trajectory) rather than the canonical tie-break of orgpu_route.  The heap
+ * holds (float64 alt, position of the node in the reference's roi_nodes
+ * list); the position order is given as a key: 2*v + 2 for a node v of an
+ * ascending list, with up to four exceptions (rank_node[i] has key
+ * rank_key[i]) for nodes the list holds out of order (orgpu_roi_info.front /
+ * appended: keys 2*left[p] + 2 and 2*N + 2 + j).  roi_index < 0: whole graph
+ * (roi_nodes = arange(N)).  use_keepout as in orgpu_net_req (a keepout edge
+ * costs keepout_cost, summed in float64 like the reference's 1e9). */
+typedef struct orgpu_ref_req {
+    int32_t src, dst;
+    int32_t net_id;
+    int32_t roi_index;
+    int32_t use_keepout;
+    int32_t n_rank;           /* 0..4 */
+    int32_t rank_node[4];
+    uint32_t rank_key[4];
+} orgpu_ref_req;
+
+/* Reserves the reference-order search scratch: per concurrent search
+ * (`slots`, at most 4096) 16 bytes per node of a graph of up to n_nodes
+ * nodes and a heap of heap_cap 16-byte entries (the heap also stages the
+ * path, so paths up to 4*heap_cap nodes).  Blocks (allocation). */
+int orgpu_reserve_ref(orgpu_t *h, int64_t n_nodes, int32_t slots, int64_t heap_cap);
+
+/* Runs n reference-order searches (reqs: DEVICE array), `slots` at a time
+ * (0 = the reserved count), one wavefront per search.  Asynchronous on the
+ * handle's stream; every output is a DEVICE array:
+ *   path_offset i64[n], path_len i32[n]: request i's path (src..dst, global
+ *   ids) is path_nodes[path_offset[i] .. +path_len[i]) and the CSR edge id of
+ *   each hop (first match in the row, UPF:5463-5476) path_edges[same];
+ *   path_len 0 = None (no path, or a path of one node as the reference
+ *   returns None for), < 0 = error for that request: -1 path output
+ *   capacity, -2 heap capacity, -3 path longer than the staging row,
+ *   -4 invalid request (endpoint, ROI row or keepout array);
+ *   path_dist f32[n]: dist[dst] (float32, as the reference stores it);
+ *   counters u64[4] (may be NULL): += pops, += edges scanned, += path
+ *   entries written, and word 3 = OR of the error bits (1 << -len).
+ * The caller zeroes counters[2] (the output cursor) before the call. */
+int orgpu_route_ref(orgpu_t *h, const orgpu_graph *g, const orgpu_ref_req *reqs, int32_t n,
+                    const uint32_t *roi_bitmaps, int32_t n_roi, const int32_t *keepout_owner,
+                    float keepout_cost, int32_t slots, int32_t *path_nodes, int32_t *path_edges,
+                    int64_t path_capacity, int64_t *path_offset, int32_t *path_len, float *path_dist,
+                    uint64_t *counters);
+
 /* ---- A2 ROI extraction -------------------------------------------------------- */
 
 /* ROIExtractor.extract_roi_bfs (UPF:1636-1734) plus the must-keep re-add of
@@ -220,6 +269,15 @@ typedef struct orgpu_roi_info {
     int32_t met;      /* waves met */
     int32_t src_in;   /* src in the final ROI */
     int32_t dst_in;   /* dst in the final ROI */
+    /* Where the reference's roi_nodes list holds nodes out of ascending order
+     * (its position order breaks heap ties, UPF:1841-1875): front[p] is the
+     * must-keep that the truncation swapped into position p (p = 0, 1), which
+     * node left[p] held before (-1, -1: no swap at p, UPF:1716-1727);
+     * appended[j] is the j-th node the re-add appended after the sorted
+     * list (-1: none, UPF:4689-4696). */
+    int32_t front[2];
+    int32_t left[2];
+    int32_t appended[2];
 } orgpu_roi_info;
 
 /* reqs: host array; roi_bitmaps: device u32[n_roi][ceil(N/32)], rows written

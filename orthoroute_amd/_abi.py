@@ -12,6 +12,7 @@ from .build import LIB_PATH
 ORGPU_MAX_LAYERS = 64
 
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ECAPACITY"}
+ORGPU_EINVAL, ORGPU_ECAPACITY = -1, -4
 
 # symbols include/orgpu.h declares (checked by tests/test_abi.py)
 EXPORTED = [
@@ -20,7 +21,7 @@ EXPORTED = [
     "orgpu_reserve", "orgpu_route", "orgpu_roi_bfs",
     "orgpu_usage_scatter", "orgpu_present_build", "orgpu_present_ema", "orgpu_cost_update",
     "orgpu_history_update", "orgpu_overuse_sparse", "orgpu_cost_bounds", "orgpu_route_ms",
-    "orgpu_set_roi_scratch", "orgpu_roi_lattice",
+    "orgpu_set_roi_scratch", "orgpu_roi_lattice", "orgpu_reserve_ref", "orgpu_route_ref",
 ]
 
 
@@ -85,7 +86,14 @@ class RoiReq(ctypes.Structure):
 
 class RoiInfo(ctypes.Structure):
     _fields_ = [("seen", ctypes.c_int64), ("kept", ctypes.c_int64), ("depth", ctypes.c_int32),
-                ("met", ctypes.c_int32), ("src_in", ctypes.c_int32), ("dst_in", ctypes.c_int32)]
+                ("met", ctypes.c_int32), ("src_in", ctypes.c_int32), ("dst_in", ctypes.c_int32),
+                ("front", ctypes.c_int32 * 2), ("left", ctypes.c_int32 * 2), ("appended", ctypes.c_int32 * 2)]
+
+
+class RefReq(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_int32), ("dst", ctypes.c_int32), ("net_id", ctypes.c_int32),
+                ("roi_index", ctypes.c_int32), ("use_keepout", ctypes.c_int32), ("n_rank", ctypes.c_int32),
+                ("rank_node", ctypes.c_int32 * 4), ("rank_key", ctypes.c_uint32 * 4)]
 
 
 class SparseOut(ctypes.Structure):
@@ -136,6 +144,8 @@ def load():
         "orgpu_route_ms": [P, ctypes.POINTER(Graph), P, i32, P, i64, P, i64, P, i32, P, f32,
                            ctypes.POINTER(SsspParams), P, P, i64, P, P, P, ctypes.POINTER(SsspStats)],
         "orgpu_roi_bfs": [P, ctypes.POINTER(Graph), P, i32, P, i32, P],
+        "orgpu_reserve_ref": [P, i64, i32, i64],
+        "orgpu_route_ref": [P, ctypes.POINTER(Graph), P, i32, P, i32, P, f32, i32, P, P, i64, P, P, P, P],
         "orgpu_set_roi_scratch": [P, i64],
         "orgpu_roi_lattice": [P, ctypes.POINTER(LatticeDesc), P, i32, P, i32, P],
         "orgpu_usage_scatter": [P, P, i64, P, i64, i32],

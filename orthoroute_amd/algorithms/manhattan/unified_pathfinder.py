@@ -36,9 +36,9 @@ from typing import Dict, List, Optional, Set, Tuple
 import numpy as np
 
 from ... import _abi
-from ...device import (Context, DeviceLattice, EdgeState, SearchParams, cost_bounds, max_slots, roi_bfs,
-                       route as device_route, route_flat as device_route_flat,
-                       route_ms as device_route_ms)
+from ...device import (Context, DeviceLattice, EdgeState, SearchParams, cost_bounds, max_slots, rank_exceptions,
+                       roi_bfs, route as device_route, route_flat as device_route_flat,
+                       route_ms as device_route_ms, route_ref as device_route_ref)
 from ...lattice import DIR_H, DIR_HV, DIR_V, layer_directions, reference_via_pairs
 from ...reductions import sum_f32
 from .board_analyzer import analyze_board_characteristics
@@ -142,6 +142,10 @@ class PathFinderConfig:
     max_iterations_cap = None
     count_minimal = False          # also count Dijkstra-minimal nodes/edges per search
     goal_directed = True           # bucket on g + lower bound to the destination (same paths)
+    sssp_order = "canonical"       # "canonical": FP32 costs, lowest-predecessor tie-break (batched
+                                   # delta-stepping); "reference": the reference heap's float64 sums,
+                                   # float32 acceptance and (alt, roi position) pop order, one
+                                   # wavefront per search -- the reference's own trajectories
 
 
 # ----------------------------------------------------------------------------
@@ -1066,6 +1070,8 @@ class PathFinderRouter:
         row_of[roi_at] = np.arange(len(roi_at))
         rows = None
         keep = np.ones(n, dtype=bool)
+        ref_order = self.config.sssp_order == "reference"
+        info = None
         t0 = time.perf_counter()
         if len(roi_at):
             words = (self.N + 31) // 32
@@ -1073,8 +1079,8 @@ class PathFinderRouter:
             depth = self.roi_extractor.roi_depth_many(src[roi_at], dst[roi_at], bonus[roi_at])
             rq = np.stack([src[roi_at], dst[roi_at], depth, np.full(len(roi_at), MAX_ROI_NODES, np.int64),
                            np.arange(len(roi_at))], axis=1)
-            info = roi_bfs(self.ctx, dlat, rq, rows, readd=readd)
-            ok = np.fromiter((a[4] and a[5] for a in info), dtype=bool, count=len(info))
+            info = roi_bfs(self.ctx, dlat, rq, rows, readd=readd, raw=True)
+            ok = (info["src_in"] != 0) & (info["dst_in"] != 0)
             keep[roi_at[~ok]] = False
         self.stats["roi_s"] += time.perf_counter() - t0
         todo = np.nonzero(keep)[0]
@@ -1089,6 +1095,24 @@ class PathFinderRouter:
         if keepout_owner is not None:
             ko = torch.from_numpy(keepout_owner).to(self.ctx.device)
         t0 = time.perf_counter()
+        if ref_order:
+            # the reference's pop order needs each ROI's list order: the
+            # ascending node order, with the exceptions its ROI record names
+            ranks = [rank_exceptions(info[row_of[i]], self.N) if row_of[i] >= 0 else None for i in todo.tolist()]
+            nodes, edges, offs, _, rst = device_route_ref(self.ctx, dlat, self.accounting.total_cost, sreqs[:, :5],
+                                                          roi_bitmaps=rows, keepout_owner=ko, ranks=ranks,
+                                                          slots=int(self.config.batch_slots))
+            self.stats["sssp_s"] += time.perf_counter() - t0
+            self.stats["sssp_kernel_ms"] += rst["kernel_ms"]
+            self.stats["sssp_edges"] += rst["edges"]
+            self.stats["sssp_items"] += rst["pops"]
+            self.stats["sssp_searches"] += len(sreqs)
+            ol = offs.tolist()
+            for j, i in enumerate(todo.tolist()):
+                a, b = ol[j], ol[j + 1]
+                if b - a > 1:
+                    out[i] = (nodes[a:b], edges[a:b - 1])
+            return out
         nodes, edges, offs, _, st = device_route_flat(self.ctx, dlat, self.accounting.total_cost, sreqs, params,
                                                       roi_bitmaps=rows, keepout_owner=ko)
         self.stats["sssp_s"] += time.perf_counter() - t0

@@ -60,6 +60,7 @@ extern "C" int orgpu_destroy(orgpu_t *h)
     if (h->stream) hipStreamSynchronize(h->stream);
     else hipDeviceSynchronize();
     engine_release(h->en);
+    ref_release(h->en);
     delete h;
     return ORGPU_OK;
 }

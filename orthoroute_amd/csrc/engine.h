@@ -74,6 +74,11 @@ struct Engine {
     uint32_t *roi_next = nullptr;
     int64_t roi_req_cap = 0;
     uint8_t *roi_T = nullptr;   // closed-form ROI: via-hop table of the lattice
+    // reference-order search (sssp_ref.hip): per slot 16 B per node and a heap
+    void *ref_state = nullptr, *ref_heap = nullptr, *ref_ctl = nullptr;
+    int64_t ref_nodes = 0, ref_hcap = 0;
+    int32_t ref_slots = 0;
+    uint32_t ref_sid = 0;       // search ids below this one are used
 };
 
 struct orgpu_ctx {
@@ -84,6 +89,7 @@ struct orgpu_ctx {
 
 int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets, int64_t qcap, int32_t max_path);
 void engine_release(Engine &en);
+void ref_release(Engine &en);
 // the multi-source part of a route call (null for two-pin requests)
 struct MsCall {
     const orgpu_ms_req *reqs;

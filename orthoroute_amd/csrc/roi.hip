@@ -76,6 +76,7 @@ __device__ void roi_finish(uint32_t *A, int64_t words, const orgpu_roi_req &q, l
 {
     const int tid = threadIdx.x;
     long long kept = seen;
+    int32_t front[2] = {-1, -1}, left[2] = {-1, -1};
     if (seen > q.max_nodes) {
         // Rank the seen nodes by id.  Each wave owns a contiguous chunk of
         // the row and its lanes read consecutive words.  Pass 1 counts the
@@ -138,10 +139,12 @@ __device__ void roi_finish(uint32_t *A, int64_t words, const orgpu_roi_req &q, l
             if (k0 > cut) {
                 A[sh.min0 >> 5] &= ~(1u << (sh.min0 & 31));
                 A[k0 >> 5] |= 1u << (k0 & 31);
+                front[0] = k0; left[0] = sh.min0;
             }
             if (q.src != q.dst && k1 > cut && q.max_nodes >= 2) {
                 A[sh.min1 >> 5] &= ~(1u << (sh.min1 & 31));
                 A[k1 >> 5] |= 1u << (k1 & 31);
+                front[1] = k1; left[1] = sh.min1;
             }
             sh.kept_delta = 0;
         }
@@ -149,12 +152,14 @@ __device__ void roi_finish(uint32_t *A, int64_t words, const orgpu_roi_req &q, l
         kept = q.max_nodes + sh.kept_delta;
     }
     if (tid == 0) {
-        // _route_all re-adds src and dst when missing
+        // _route_all re-adds src and dst when missing, appended in that order
         const int32_t ks[2] = {q.src, q.dst};
+        int32_t app[2] = {-1, -1};
+        int na = 0;
         if (q.readd) {
             for (int j = 0; j < 2; ++j) {
                 const int32_t v = ks[j];
-                if (!((A[v >> 5] >> (v & 31)) & 1u)) { A[v >> 5] |= 1u << (v & 31); kept++; }
+                if (!((A[v >> 5] >> (v & 31)) & 1u)) { A[v >> 5] |= 1u << (v & 31); kept++; app[na++] = v; }
             }
         }
         if (info) {
@@ -164,6 +169,11 @@ __device__ void roi_finish(uint32_t *A, int64_t words, const orgpu_roi_req &q, l
             info->met = met;
             info->src_in = (A[q.src >> 5] >> (q.src & 31)) & 1u;
             info->dst_in = (A[q.dst >> 5] >> (q.dst & 31)) & 1u;
+            for (int j = 0; j < 2; ++j) {
+                info->front[j] = front[j];
+                info->left[j] = left[j];
+                info->appended[j] = app[j];
+            }
         }
     }
     __syncthreads();

@@ -1188,6 +1188,10 @@ void engine_release(Engine &en)
     fresh.stream = en.stream;
     fresh.device = en.device;
     fresh.roi_limit = en.roi_limit;
+    // the reference-order search scratch has its own reserve / release
+    fresh.ref_state = en.ref_state; fresh.ref_heap = en.ref_heap; fresh.ref_ctl = en.ref_ctl;
+    fresh.ref_nodes = en.ref_nodes; fresh.ref_hcap = en.ref_hcap; fresh.ref_slots = en.ref_slots;
+    fresh.ref_sid = en.ref_sid;
     en = fresh;
 }
 

@@ -294,9 +294,13 @@ MS_POINT = np.dtype([("node", "<i4"), ("cost", "<f4")])
 ROI_REQ = np.dtype({"names": ["src", "dst", "max_depth", "roi_index", "max_nodes", "readd", "reserved"],
                     "formats": ["<i4", "<i4", "<i4", "<i4", "<i8", "<i4", "<i4"],
                     "offsets": [0, 4, 8, 12, 16, 24, 28], "itemsize": 32})
-ROI_INFO = np.dtype({"names": ["seen", "kept", "depth", "met", "src_in", "dst_in"],
-                     "formats": ["<i8", "<i8", "<i4", "<i4", "<i4", "<i4"],
-                     "offsets": [0, 8, 16, 20, 24, 28], "itemsize": 32})
+ROI_INFO = np.dtype({"names": ["seen", "kept", "depth", "met", "src_in", "dst_in", "front", "left", "appended"],
+                     "formats": ["<i8", "<i8", "<i4", "<i4", "<i4", "<i4", ("<i4", 2), ("<i4", 2), ("<i4", 2)],
+                     "offsets": [0, 8, 16, 20, 24, 28, 32, 40, 48], "itemsize": 56})
+# orgpu_ref_req rows
+REF_REQ = np.dtype({"names": ["src", "dst", "net_id", "roi_index", "use_keepout", "n_rank", "rank_node", "rank_key"],
+                    "formats": ["<i4", "<i4", "<i4", "<i4", "<i4", "<i4", ("<i4", 4), ("<u4", 4)],
+                    "offsets": [0, 4, 8, 12, 16, 20, 24, 40], "itemsize": 56})
 
 
 def _ms_points(pts):
@@ -372,14 +376,16 @@ def route_ms(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=N
 ROI_LATTICE_MAX_DEPTH = 1024
 
 
-def roi_bfs(ctx, lat, requests, rows, readd=True, method="auto"):
+def roi_bfs(ctx, lat, requests, rows, readd=True, method="auto", raw=False):
     """Batched ROI extraction.  requests: list of (src, dst, max_depth,
     max_nodes, row); rows: device u32[n_roi][ceil(N/32)] written in place.
     readd: re-add src/dst after truncation (the main search of _route_all) or
     not (its wider-ROI fallback).  method: "bfs" traverses the CSR;
     "lattice" computes the same ROI in closed form from the lattice shape
     (DeviceLattice only); "auto" takes "lattice" where it applies.
-    Returns a list of (seen, kept, depth, met, src_in, dst_in)."""
+    Returns a list of (seen, kept, depth, met, src_in, dst_in), or with
+    raw=True the ROI_INFO records themselves (which also say where the
+    reference's roi_nodes list holds nodes out of ascending order)."""
     n = len(requests)
     if n == 0:
         return []
@@ -400,8 +406,129 @@ def roi_bfs(ctx, lat, requests, rows, readd=True, method="auto"):
         g = lat.graph(lat.base_cost)
         _abi.check(ctx.lib.orgpu_roi_bfs(ctx.handle, ctypes.byref(g), reqs.ctypes.data_as(vp), n,
                                          _ptr(rows), int(rows.shape[0]), info.ctypes.data_as(vp)))
+    if raw:
+        return info
     return list(zip(info["seen"].tolist(), info["kept"].tolist(), info["depth"].tolist(), info["met"].tolist(),
                     (info["src_in"] != 0).tolist(), (info["dst_in"] != 0).tolist()))
+
+
+def rank_exceptions(info, n_nodes):
+    """(node, key) pairs of the reference-order search for an ROI whose
+    ROI_INFO record is `info`: the keys order nodes like their positions in
+    the reference's roi_nodes list (UPF:1716-1727, 4689-4696), 2*v + 2 for
+    the ascending part; a must-keep swapped into position p takes the key of
+    the node that left it, and the j-th appended node comes after every
+    other (2*N + 2 + j)."""
+    out = []
+    for p in range(2):
+        if info["front"][p] >= 0:
+            out.append((int(info["front"][p]), 2 * int(info["left"][p]) + 2))
+    for j in range(2):
+        if info["appended"][j] >= 0:
+            out.append((int(info["appended"][j]), 2 * int(n_nodes) + 2 + j))
+    return out
+
+
+def ref_requests(requests, ranks=None):
+    """orgpu_ref_req rows from (src, dst, net_id, roi_index, use_keepout)
+    rows and, per request, a list of (node, key) rank exceptions."""
+    q = np.asarray(requests, dtype=np.int64).reshape(-1, 5)
+    rows = np.zeros(len(q), dtype=REF_REQ)
+    rows["src"], rows["dst"], rows["net_id"] = q[:, 0], q[:, 1], q[:, 2]
+    rows["roi_index"], rows["use_keepout"] = q[:, 3], q[:, 4]
+    if ranks is not None:
+        for i, ex in enumerate(ranks):
+            if not ex:
+                continue
+            if len(ex) > 4:
+                raise ValueError("at most four rank exceptions per request")
+            rows["n_rank"][i] = len(ex)
+            for k, (v, key) in enumerate(ex):
+                rows["rank_node"][i, k] = v
+                rows["rank_key"][i, k] = key
+    return rows
+
+
+REF_ERRORS = {-1: "path output capacity", -2: "heap capacity", -3: "path longer than the staging row",
+              -4: "invalid request"}
+
+
+def ref_slots_for(n_nodes, heap_cap, budget=32 << 30):
+    """Concurrent reference-order searches whose scratch (16 B per node and
+    16 B per heap entry each) fits `budget` bytes, at most 1024."""
+    per = 16 * int(n_nodes) + 16 * int(heap_cap)
+    return max(1, min(1024, int(budget) // per))
+
+
+def route_ref(ctx, lat, cost, requests, roi_bitmaps=None, keepout_owner=None, ranks=None, slots=None,
+              heap_cap=None, path_capacity=None):
+    """Reference-order searches (SimpleDijkstra.find_path_roi's own pop
+    order and float64 / float32 arithmetic).  requests: (src, dst, net_id,
+    roi_index, use_keepout) rows; ranks: per request the (node, key) rank
+    exceptions of its ROI (rank_exceptions), or None.  Returns (nodes,
+    edges, offsets, dist, stats) like route_flat: request i's path is
+    nodes[offsets[i]:offsets[i+1]] (fewer than two nodes = None)."""
+    torch = ctx.torch
+    rows = ref_requests(requests, ranks)
+    n = len(rows)
+    dev = ctx.device
+    if heap_cap is None:
+        heap_cap = int(min(max(4096, lat.N), 1 << 20))
+    cap = ref_slots_for(lat.N, heap_cap)
+    slots = cap if slots is None else min(int(slots), cap)
+    slots = max(1, min(slots, max(n, 1)))
+    key = (int(lat.N), slots, int(heap_cap))
+    have = getattr(ctx, "_ref_reserved", None)
+    if have is None or not all(a >= b for a, b in zip(have, key)):
+        _abi.check(ctx.lib.orgpu_reserve_ref(ctx.handle, key[0], key[1], key[2]))
+        ctx._ref_reserved = key
+    if n == 0:
+        return (np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(1, np.int64), np.zeros(0, np.float32),
+                {"pops": 0, "edges": 0, "kernel_ms": 0.0})
+    if path_capacity is None:
+        path_capacity = max(1024, n * 4 * (lat.X + lat.Y + 2 * lat.L))
+    n_roi = 0 if roi_bitmaps is None else int(roi_bitmaps.shape[0])
+    if roi_bitmaps is not None and roi_bitmaps.shape[1] * 32 < lat.N:
+        raise ValueError("roi bitmap rows are shorter than the node count")
+    g = lat.graph(cost)
+    d_req = torch.from_numpy(rows.view(np.uint8)).to(dev)
+    for _ in range(4):
+        nodes = torch.empty(max(path_capacity, 1), dtype=torch.int32, device=dev)
+        edges = torch.empty(max(path_capacity, 1), dtype=torch.int32, device=dev)
+        off = torch.zeros(n, dtype=torch.int64, device=dev)
+        ln = torch.zeros(n, dtype=torch.int32, device=dev)
+        dist = torch.zeros(n, dtype=torch.float32, device=dev)
+        cnt = torch.zeros(4, dtype=torch.int64, device=dev)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(ctx.stream)
+        _abi.check(ctx.lib.orgpu_route_ref(ctx.handle, ctypes.byref(g), _ptr(d_req), n, _ptr(roi_bitmaps), n_roi,
+                                           _ptr(keepout_owner), ctypes.c_float(KEEPOUT_COST), slots, _ptr(nodes),
+                                           _ptr(edges), int(path_capacity), _ptr(off), _ptr(ln), _ptr(dist),
+                                           _ptr(cnt)))
+        e1.record(ctx.stream)
+        ctx.sync()
+        lens = ln.cpu().numpy()
+        if (lens == -1).any():
+            path_capacity *= 2        # the output ran out: again with room for every path
+            continue
+        break
+    bad = lens[lens < 0]
+    if len(bad):
+        raise _abi.OrgpuError(_abi.ORGPU_ECAPACITY if bad[0] != -4 else _abi.ORGPU_EINVAL,
+                              f"route_ref: {REF_ERRORS.get(int(bad[0]), int(bad[0]))} "
+                              f"({len(bad)} of {n} requests)")
+    offs_dev = off.cpu().numpy()
+    c = cnt.cpu().numpy()
+    used = int(c[2])
+    allnodes = nodes[:used].cpu().numpy()
+    alledges = edges[:used].cpu().numpy()
+    # compact, in request order
+    offs = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(lens, out=offs[1:])
+    idx = np.repeat(offs_dev - offs[:-1], lens) + np.arange(int(offs[-1]), dtype=np.int64)
+    stats = {"pops": int(c[0]), "edges": int(c[1]), "kernel_ms": float(e0.elapsed_time(e1)), "slots": slots}
+    return allnodes[idx], alledges[idx], offs, dist.cpu().numpy(), stats
 
 
 class Sparse:

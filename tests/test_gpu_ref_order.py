@@ -1,0 +1,262 @@
+"""GPU reference-order search (orgpu_route_ref) against the reference's own
+Dijkstra: the golden paths the reference wrote (dijkstra.npz), the oracle's
+exact emulation of SimpleDijkstra.find_path_roi (UPF:1801-1887) on random
+cases with truncated / re-added ROIs, keepouts and tie-heavy costs, and the
+GPU router in reference order reproducing the reference's recorded
+negotiation trajectories (negotiation*.npz) bit for bit."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from orthoroute_amd.device import Context
+    return Context(0)
+
+
+def first_match_edges(o, path):
+    out = []
+    for u, v in zip(path, path[1:]):
+        row = o.indices[o.indptr[u]:o.indptr[u + 1]]
+        out.append(int(o.indptr[u] + np.nonzero(row == v)[0][0]))
+    return out
+
+
+def gpu_rois(ctx, g, o, cases, readd):
+    """cases: (src, dst, max_nodes, bonus) -> device rows, ROI_INFO records."""
+    import torch
+    from orthoroute_amd.device import roi_bfs
+    rq = []
+    for j, (s, d, m, b) in enumerate(cases):
+        _, depth = oracle.roi_radius(o, s, d, b)
+        rq.append((s, d, depth, m, j))
+    rows = torch.zeros((len(cases), (o.N + 31) // 32), dtype=torch.int32, device=ctx.device)
+    info = roi_bfs(ctx, g, rq, rows, readd=readd, raw=True)
+    return rows, info
+
+
+def oracle_list(o, s, d, m, b, readd):
+    roi = oracle.roi_bfs(o, s, d, stagnation_bonus=b, max_nodes=m)
+    if readd:
+        have = set(roi.tolist())
+        extra = [v for v in (s, d) if v not in have]
+        if extra:
+            roi = np.concatenate([roi, np.asarray(extra, np.int32)])
+    return roi.astype(np.int32)
+
+
+def test_ref_order_matches_reference_golden_paths(ctx, golden_dir):
+    """The reference's own find_path_roi outputs on its cost snapshots."""
+    import torch
+    from orthoroute_amd.device import rank_exceptions, route_ref
+    gd = np.load(os.path.join(golden_dir, "dijkstra.npz"))
+    X, Y, L = int(gd["X"]), int(gd["Y"]), int(gd["L"])
+    o = oracle.Lattice(X, Y, L)
+    g = ctx.build_lattice(X, Y, L, "reference")
+    off = 0
+    n = len(gd["src"])
+    exp = []
+    for i in range(n):
+        k = int(gd["path_len"][i])
+        exp.append(gd["paths"][off:off + k].tolist())
+        off += k
+    for snap in range(gd["snaps"].shape[0]):
+        idx = [i for i in range(n) if int(gd["src"][i]) // 1_000_000 == snap]
+        if not idx:
+            continue
+        say(f"[ref golden] snapshot {snap}: {len(idx)} searches")
+        cost_d = torch.from_numpy(np.ascontiguousarray(gd["snaps"][snap], np.float32)).to(ctx.device)
+        srcs = [int(gd["src"][i]) % 1_000_000 for i in idx]
+        dsts = [int(gd["dst"][i]) for i in idx]
+        roi_cases = [(s, d, 50_000, 0.0) for s, d, i in zip(srcs, dsts, idx) if int(gd["mode"][i]) == 0]
+        rows, info = gpu_rois(ctx, g, o, roi_cases, readd=False) if roi_cases else (None, None)
+        reqs, ranks, r = [], [], 0
+        for s, d, i in zip(srcs, dsts, idx):
+            if int(gd["mode"][i]) == 0:
+                reqs.append((s, d, -1, r, 0))
+                ranks.append(rank_exceptions(info[r], o.N))
+                r += 1
+            else:
+                reqs.append((s, d, -1, -1, 0))
+                ranks.append(None)
+        nodes, edges, offs, dist, st = route_ref(ctx, g, cost_d, reqs, roi_bitmaps=rows, ranks=ranks, slots=7)
+        for j, i in enumerate(idx):
+            got = nodes[offs[j]:offs[j + 1]].tolist()
+            assert got == exp[i], f"case {i}: gpu {got} reference {exp[i]}"
+            if got:
+                assert edges[offs[j]:offs[j + 1] - 1].tolist() == first_match_edges(o, got)
+
+
+def make_costs(o, rng, kind):
+    c = (o.costs * np.float32(0.3)).astype(np.float32)
+    if kind == "jitter":
+        c += (np.arange(o.E, dtype=np.float32) % 9973) * np.float32(1e-6)
+        hot = rng.random(o.E) < 0.05
+        c[hot] += rng.integers(1, 20, hot.sum()).astype(np.float32) * np.float32(0.7)
+    elif kind == "ties":      # many exact float64 ties: the roi-position order decides
+        c = (o.costs * np.float32(0.5) * rng.integers(1, 3, o.E).astype(np.float32)).astype(np.float32)
+    else:                     # round-off heavy: float32(alt) acceptance differs from float64 order
+        c = (o.costs * (0.1 + 3 * rng.random(o.E))).astype(np.float32)
+        c += np.float32(1e-7) * rng.integers(0, 3, o.E).astype(np.float32)
+    blk = (o.edge_kind == 1) & (rng.random(o.E) < 0.1)
+    c[blk] = np.inf
+    return c
+
+
+CASES = [
+    # policy, X, Y, L, seed, cost kind
+    ("reference", 20, 16, 6, 1, "jitter"),
+    ("reference", 24, 20, 6, 2, "ties"),
+    ("reference", 18, 14, 8, 3, "roundoff"),
+    ("adjacent", 30, 24, 6, 4, "ties"),
+    ("adjacent", 40, 30, 4, 5, "jitter"),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{c[1]}x{c[2]}x{c[3]}-{c[5]}" for c in CASES])
+def test_ref_order_matches_oracle_random(ctx, case):
+    """Random requests: full-graph searches, BFS ROIs cut to 300 / 2,000 /
+    50,000 nodes (must-keeps swapped to the front and, for the main search,
+    re-added at the end), keepouts; paths, first-match edges and float32
+    dist equal to the oracle's emulation of the reference heap."""
+    import torch
+    from orthoroute_amd.device import rank_exceptions, route_ref
+    policy, X, Y, L, seed, kind = case
+    rng = np.random.default_rng(seed)
+    o = oracle.Lattice(X, Y, L, policy)
+    g = ctx.build_lattice(X, Y, L, policy)
+    c = make_costs(o, rng, kind)
+    cost_d = torch.from_numpy(c).to(ctx.device)
+    owner = np.full(o.N, -1, np.int32)
+    sel = rng.random(o.N) < 0.06
+    owner[sel] = rng.integers(0, 5, sel.sum())
+    own_d = torch.from_numpy(owner).to(ctx.device)
+    for readd in (True, False):
+        n = 96
+        spec = []
+        for i in range(n):
+            s, d = (int(v) for v in rng.integers(0, o.N, 2))
+            if i % 17 == 0:
+                d = s                                  # a path of one node is None
+            use_roi = i % 4 != 0
+            m = int(rng.choice([300, 2000, 50_000]))
+            b = float(rng.choice([0.0, 0.6, 3.0]))
+            spec.append((s, d, use_roi, m, b, i % 5, int(i % 3 == 0)))
+        roi_cases = [(s, d, m, b) for s, d, u, m, b, _, _ in spec if u]
+        rows, info = gpu_rois(ctx, g, o, roi_cases, readd=readd)
+        say(f"[ref random] {case} readd={readd}: {n} searches, {len(roi_cases)} ROIs, "
+            f"{int((info['front'] >= 0).any(axis=1).sum())} with a front swap, "
+            f"{int((info['appended'] >= 0).any(axis=1).sum())} with a re-add")
+        reqs, ranks, lists, r = [], [], [], 0
+        for s, d, u, m, b, nid, ko in spec:
+            if u:
+                lst = oracle_list(o, s, d, m, b, readd)
+                mask = np.zeros(o.N, bool)
+                mask[lst] = True
+                row = rows[r].cpu().numpy().view(np.uint32)
+                got_mask = ((row[np.arange(o.N) >> 5] >> (np.arange(o.N) & 31)) & 1).astype(bool)
+                assert np.array_equal(got_mask, mask), f"ROI {s}->{d} max_nodes={m}: bitmap differs from the list"
+                reqs.append((s, d, nid, r, ko))
+                ranks.append(rank_exceptions(info[r], o.N))
+                lists.append(lst)
+                r += 1
+            else:
+                reqs.append((s, d, nid, -1, ko))
+                ranks.append(None)
+                lists.append(np.arange(o.N, dtype=np.int32))
+        nodes, edges, offs, dist, st = route_ref(ctx, g, cost_d, reqs, roi_bitmaps=rows, keepout_owner=own_d,
+                                                 ranks=ranks, slots=int(rng.choice([1, 5, 64])))
+        for j, ((s, d, nid, _, ko), lst) in enumerate(zip(reqs, lists)):
+            path, alt = oracle.sssp_ref(o, c, s, d, lst, keepout_owner=owner if ko else None, net_id=nid)
+            got = nodes[offs[j]:offs[j + 1]].tolist()
+            assert got == (path or []), f"request {j} ({s}->{d}, roi={reqs[j][3]}, ko={ko}): gpu {got} oracle {path}"
+            if path:
+                assert edges[offs[j]:offs[j + 1] - 1].tolist() == first_match_edges(o, path)
+                assert dist[j] == np.float32(alt), (j, dist[j], alt)
+
+
+@pytest.mark.parametrize("policy", ["reference", "adjacent"])
+def test_ref_order_follows_roi_position_on_ties(ctx, policy):
+    """Unit costs: many nodes pop at equal float64 alt, and the roi-position
+    order decides which predecessor relaxes a node first.  The GPU equals
+    the oracle on the reference's list order, and a planted mistake in the
+    oracle (the list reversed, so ties pop highest position first) changes
+    some paths, so the test sees the order."""
+    import torch
+    from orthoroute_amd.device import rank_exceptions, route_ref
+    X, Y, L = 30, 30, 6
+    o = oracle.Lattice(X, Y, L, policy)
+    g = ctx.build_lattice(X, Y, L, policy)
+    c = np.ones(o.E, np.float32)
+    cost_d = torch.from_numpy(c).to(ctx.device)
+    rng = np.random.default_rng(11)
+    spec = [(int(rng.integers(0, o.N)), int(rng.integers(0, o.N)), 400, 0.0) for _ in range(64)]
+    rows, info = gpu_rois(ctx, g, o, spec, readd=True)
+    reqs = [(s, d, -1, j, 0) for j, (s, d, _, _) in enumerate(spec)]
+    ranks = [rank_exceptions(info[j], o.N) for j in range(len(spec))]
+    nodes, edges, offs, dist, st = route_ref(ctx, g, cost_d, reqs, roi_bitmaps=rows, ranks=ranks)
+    differs = 0
+    for j, (s, d, m, b) in enumerate(spec):
+        lst = oracle_list(o, s, d, m, b, True)
+        path, _ = oracle.sssp_ref(o, c, s, d, lst)
+        assert nodes[offs[j]:offs[j + 1]].tolist() == (path or []), f"request {j}"
+        p2, _ = oracle.sssp_ref(o, c, s, d, lst[::-1].copy())
+        differs += (p2 or []) != (path or [])
+    say(f"[ref ties] {policy}: {differs} of {len(spec)} paths change when ties pop in reversed order")
+    assert differs > 0
+
+
+def run_router_ref(X, Y, L, reqs, cap, slots=1024):
+    from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig, PathFinderRouter
+    cfg = PathFinderConfig()
+    cfg.lattice_policy = "reference"
+    cfg.max_iterations_cap = cap
+    cfg.sssp_order = "reference"
+    cfg.batch_slots = slots
+    r = PathFinderRouter(cfg)
+    r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
+    rec = []
+
+    def cb(it, t, v, s, cnt):
+        E = r.accounting.E
+        rec.append((it, s, cnt, r.accounting.history.cpu().numpy()[:E].copy(),
+                    r.accounting.present_ema.cpu().numpy()[:E].copy()))
+    res = r.route_multiple_nets(reqs, iteration_cb=cb)
+    return res, rec
+
+
+@pytest.mark.parametrize("name", ["negotiation_easy.npz", "negotiation_l4.npz", "negotiation.npz"])
+@pytest.mark.parametrize("slots", [1, 1024])
+def test_router_reference_order_reproduces_reference_trajectory(golden_dir, name, slots):
+    """The reference's recorded route_multiple_nets runs: per-iteration
+    overuse sum and count, history and present_ema bits, final paths."""
+    gd = np.load(os.path.join(golden_dir, name))
+    X, Y, L = int(gd["X"]), int(gd["Y"]), int(gd["L"])
+    reqs = [(f"N{i:03d}", int(s), int(d)) for i, (s, d) in enumerate(zip(gd["src"], gd["dst"]))]
+    say(f"[ref router] {name} {X}x{Y}x{L} nets={len(reqs)} cap={int(gd['cap'])} slots={slots}")
+    res, rec = run_router_ref(X, Y, L, reqs, int(gd["cap"]), slots)
+    assert [tuple(x) for x in gd["over"].tolist()] == [r[:3] for r in rec]
+    for i, r in enumerate(rec):
+        np.testing.assert_array_equal(r[3].view(np.uint32), gd["history"][i].view(np.uint32),
+                                      err_msg=f"history, iteration {r[0]}")
+        np.testing.assert_array_equal(r[4].view(np.uint32), gd["ema"][i].view(np.uint32),
+                                      err_msg=f"present_ema, iteration {r[0]}")
+    off = 0
+    names = [q[0] for q in reqs]
+    for i, n in enumerate(gd["final_len"]):
+        exp = gd["final_nodes"][off:off + n].tolist()
+        off += n
+        assert list(res["paths"].get(names[i], [])) == exp, f"final path of {names[i]}"
+    assert bool(res.get("success", False)) == bool(gd["success"])
