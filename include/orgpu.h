@@ -245,6 +245,25 @@ int orgpu_route_ref(orgpu_t *h, const orgpu_graph *g, const orgpu_ref_req *reqs,
                     int64_t path_capacity, int64_t *path_offset, int32_t *path_len, float *path_dist,
                     uint64_t *counters);
 
+/* orgpu_route_ref for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  No handle: the search scratch is a
+ * fixed static device array of ORGPU_REF_NOALLOC_BYTES per GPU (256 bytes of
+ * control, then slots * n_nodes * 16 bytes of node state, then slots *
+ * heap_cap * 16 bytes of heaps), held by each GPU the library runs on until
+ * the process ends; a request that does not fit returns ORGPU_EINVAL.
+ * Calls nothing that allocates.  Runs on `hip_stream` of `device`; inputs,
+ * outputs and their meaning as orgpu_route_ref.  Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU (other streams or other
+ * host threads before the first has finished on the device) race. */
+#define ORGPU_REF_NOALLOC_BYTES (64ull << 20)
+int orgpu_route_ref_noalloc(int device, void *hip_stream, const orgpu_graph *g, const orgpu_ref_req *reqs,
+                            int32_t n, const uint32_t *roi_bitmaps, int32_t n_roi,
+                            const int32_t *keepout_owner, float keepout_cost, int32_t slots,
+                            int64_t heap_cap, int32_t *path_nodes, int32_t *path_edges,
+                            int64_t path_capacity, int64_t *path_offset, int32_t *path_len,
+                            float *path_dist, uint64_t *counters);
+
 /* ---- A2 ROI extraction -------------------------------------------------------- */
 
 /* ROIExtractor.extract_roi_bfs (UPF:1636-1734) plus the must-keep re-add of

@@ -13,6 +13,7 @@ ORGPU_MAX_LAYERS = 64
 
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ECAPACITY"}
 ORGPU_EINVAL, ORGPU_ECAPACITY = -1, -4
+REF_NOALLOC_BYTES = 64 << 20   # ORGPU_REF_NOALLOC_BYTES
 
 # symbols include/orgpu.h declares (checked by tests/test_abi.py)
 EXPORTED = [
@@ -22,6 +23,7 @@ EXPORTED = [
     "orgpu_usage_scatter", "orgpu_present_build", "orgpu_present_ema", "orgpu_cost_update",
     "orgpu_history_update", "orgpu_overuse_sparse", "orgpu_cost_bounds", "orgpu_route_ms",
     "orgpu_set_roi_scratch", "orgpu_roi_lattice", "orgpu_reserve_ref", "orgpu_route_ref",
+    "orgpu_route_ref_noalloc",
 ]
 
 
@@ -146,6 +148,8 @@ def load():
         "orgpu_roi_bfs": [P, ctypes.POINTER(Graph), P, i32, P, i32, P],
         "orgpu_reserve_ref": [P, i64, i32, i64],
         "orgpu_route_ref": [P, ctypes.POINTER(Graph), P, i32, P, i32, P, f32, i32, P, P, i64, P, P, P, P],
+        "orgpu_route_ref_noalloc": [ctypes.c_int, P, ctypes.POINTER(Graph), P, i32, P, i32, P, f32, i32, i64,
+                                    P, P, i64, P, P, P, P],
         "orgpu_set_roi_scratch": [P, i64],
         "orgpu_roi_lattice": [P, ctypes.POINTER(LatticeDesc), P, i32, P, i32, P],
         "orgpu_usage_scatter": [P, P, i64, P, i64, i32],

@@ -28,6 +28,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 #include "engine.h"
@@ -470,4 +471,78 @@ extern "C" int orgpu_route_ref(orgpu_t *h, const orgpu_graph *g, const orgpu_ref
     sc.sid = &en.ref_sid;
     return ref_launch(h->stream, sc, g, reqs, n, roi_bitmaps, n_roi, keepout_owner, keepout_cost, slots, path_nodes,
                       path_edges, path_capacity, path_offset, path_len, path_dist, counters);
+}
+
+// ---------------------------------------------------------------------------
+// The same search for callers that cannot allocate: its scratch is the
+// file-scope array below (one copy per GPU, held until the process ends).
+// Layout: [control 256 B][node state: slots * N * 16 B][heaps: slots *
+// heap_cap * 16 B].  In host code `scratch` names a host-side stand-in, so
+// its device address is looked up once per device (hipGetSymbolAddress) and
+// only that address is used.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[ORGPU_REF_NOALLOC_BYTES];
+
+#define NOALLOC_MAX_DEVICES 64
+static std::mutex g_noalloc_mu;
+static void *g_noalloc_ptr[NOALLOC_MAX_DEVICES];
+static uint32_t g_noalloc_sid[NOALLOC_MAX_DEVICES];
+static int64_t g_noalloc_state_bytes[NOALLOC_MAX_DEVICES];   // layout the stamps were last cleared for
+
+extern "C" int orgpu_route_ref_noalloc(int device, void *hip_stream, const orgpu_graph *g, const orgpu_ref_req *reqs,
+                                       int32_t n, const uint32_t *roi_bitmaps, int32_t n_roi,
+                                       const int32_t *keepout_owner, float keepout_cost, int32_t slots,
+                                       int64_t heap_cap, int32_t *path_nodes, int32_t *path_edges,
+                                       int64_t path_capacity, int64_t *path_offset, int32_t *path_len,
+                                       float *path_dist, uint64_t *counters)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return orgpu_fail(ORGPU_EINVAL, "route_ref_noalloc: bad device %d", device);
+    if (!g || g->n_nodes < 1) return orgpu_fail(ORGPU_EINVAL, "route_ref_noalloc: null or empty graph");
+    if (slots < 1 || heap_cap < 64) return orgpu_fail(ORGPU_EINVAL, "route_ref_noalloc: slots >= 1 and heap_cap >= 64 required");
+    const uint64_t state_bytes = (uint64_t)slots * (uint64_t)g->n_nodes * sizeof(RefNode);
+    const uint64_t heap_bytes = (uint64_t)slots * (uint64_t)heap_cap * sizeof(HeapEnt);
+    if (256ull + state_bytes + heap_bytes > (uint64_t)ORGPU_REF_NOALLOC_BYTES)
+        return orgpu_fail(ORGPU_EINVAL, "route_ref_noalloc: %d slots x (%lld nodes + %lld heap entries) x 16 B "
+                          "exceed the static scratch of %llu bytes", slots, (long long)g->n_nodes,
+                          (long long)heap_cap, (unsigned long long)ORGPU_REF_NOALLOC_BYTES);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    int prev = 0;
+    HIP_CHECK_RET(hipGetDevice(&prev));
+    if (prev != device) HIP_CHECK_RET(hipSetDevice(device));
+    int rc = ORGPU_OK;
+    hipStream_t s = (hipStream_t)hip_stream;
+    if (!g_noalloc_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) rc = orgpu_fail_hip(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                                                       "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+        else { g_noalloc_ptr[device] = p; g_noalloc_state_bytes[device] = -1; }
+    }
+    if (rc == ORGPU_OK) {
+        unsigned char *base = static_cast<unsigned char *>(g_noalloc_ptr[device]);
+        // node records must hold stamps of earlier searches only: when the
+        // layout changes, bytes that held heap entries may become records,
+        // so the records are cleared (a fill kernel) and the ids restart
+        if (g_noalloc_state_bytes[device] != (int64_t)state_bytes) {
+            hipLaunchKernelGGL(k_ref_clear, dim3(1024), dim3(256), 0, s, reinterpret_cast<uint4 *>(base + 256),
+                               (int64_t)(state_bytes / 16));
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) rc = orgpu_fail_hip(e, "k_ref_clear", __FILE__, __LINE__);
+            g_noalloc_state_bytes[device] = (int64_t)state_bytes;
+            g_noalloc_sid[device] = 0;
+        }
+        if (rc == ORGPU_OK) {
+            RefScratch sc;
+            sc.ctl = reinterpret_cast<RefCtl *>(base);
+            sc.cursor = reinterpret_cast<unsigned long long *>(base + 64);
+            sc.state = reinterpret_cast<RefNode *>(base + 256);
+            sc.heap = reinterpret_cast<HeapEnt *>(base + 256 + state_bytes);
+            sc.nodes = g->n_nodes; sc.hcap = heap_cap; sc.slots = slots;
+            sc.sid = &g_noalloc_sid[device];
+            rc = ref_launch(s, sc, g, reqs, n, roi_bitmaps, n_roi, keepout_owner, keepout_cost, slots, path_nodes,
+                            path_edges, path_capacity, path_offset, path_len, path_dist, counters);
+        }
+    }
+    if (prev != device) (void)hipSetDevice(prev);
+    return rc;
 }
