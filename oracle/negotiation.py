@@ -28,6 +28,7 @@ random.Random: seeded 42 where PadEscapePlanner seeds at init (its only effect
 on tuple requests), reseeded 42+it per iteration (UPF:3813-3818).
 """
 import math
+import os
 import random
 from collections import defaultdict
 
@@ -656,6 +657,10 @@ class OracleRouter:
             pres_fac_max, hmult = 64.0, 1.0
         else:
             pres_fac_max, hmult = 128.0, 0.8
+        # the reference's environment overrides for testing (UPF:3587-3590)
+        pres_fac_mult = float(os.getenv("ORTHO_PRES_FAC_MULT", pres_fac_mult))
+        pres_fac_max = float(os.getenv("ORTHO_PRES_FAC_MAX", pres_fac_max))
+        hist_gain = float(os.getenv("ORTHO_HIST_GAIN", hist_gain))
         best = float("inf")
         stagnant = 0
         self.net_failure = {}

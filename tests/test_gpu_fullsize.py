@@ -1,8 +1,8 @@
 """Parity at the benchmark's full sizes: the 600x400x18 backplane (4.32 M
-nodes, 16.8 M edges) routed by the batched GPU engine on its iteration-1
-cost snapshot against the oracle's ROI + canonical search for a sample of the
-same nets, and cfg3's multi-terminal trees (300x300x12) against the oracle's
-trees."""
+nodes, 16.8 M edges) and the 1000x800x24 stress board (19.2 M nodes, 75.2 M
+edges) routed by the batched GPU engine on their iteration-1 cost snapshots
+against the oracle's ROI + canonical search for a sample of the same nets,
+and cfg3's multi-terminal trees (300x300x12) against the oracle's trees."""
 import sys
 
 import numpy as np
@@ -18,12 +18,13 @@ def say(*a):
     sys.stdout.flush()
 
 
-def test_backplane_sample_matches_oracle():
+@pytest.mark.parametrize("which", ["cfg2", "cfg4"])
+def test_fullsize_sample_matches_oracle(which):
     from orthoroute_amd import workloads
     from orthoroute_amd.algorithms.manhattan.unified_pathfinder import (PathFinderConfig, PathFinderRouter,
                                                                         ROI_THRESHOLD_STEPS)
-    X, Y, L, reqs = workloads.cfg2()
-    say(f"[fullsize] backplane {X}x{Y}x{L}, {len(reqs)} nets")
+    X, Y, L, reqs = workloads.CONFIGS[which]()
+    say(f"[fullsize] {which} {X}x{Y}x{L}, {len(reqs)} nets")
     cfg = PathFinderConfig()
     cfg.lattice_policy = "adjacent"
     cfg.batch_slots = 512
@@ -40,7 +41,8 @@ def test_backplane_sample_matches_oracle():
         dx, dy, _ = r._coords(d)
         near = abs(dx - sx) + abs(dy - sy) < ROI_THRESHOLD_STEPS
         (short if near else long_).append((s, d, -1, near, 0.0))
-    sample = short[:16] + long_[:6]
+    sample = short[:16] + long_[:6] if which == "cfg2" else short[:24] + long_[:2]
+    say(f"[fullsize] {len(sample)} sampled searches ({sum(1 for q in sample if q[3])} ROI-limited)")
     got = r._search_batch(sample)
     lat = oracle.Lattice(X, Y, L, "adjacent", pitch=1.0, via_cost=3.0)
     costs = r.accounting.total_cost[:r.accounting.E].cpu().numpy()

@@ -260,3 +260,32 @@ def test_router_reference_order_reproduces_reference_trajectory(golden_dir, name
         off += n
         assert list(res["paths"].get(names[i], [])) == exp, f"final path of {names[i]}"
     assert bool(res.get("success", False)) == bool(gd["success"])
+
+
+def _digest(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float32).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("name", ["negotiation_l16.npz", "negotiation_l24.npz"])
+def test_router_reference_order_many_layers(golden_dir, name, monkeypatch):
+    """Boards with 14 and 22 signal layers (the pres_fac_max 64 / 128
+    branches, via annealing once pres_fac reaches 64): per-iteration overuse,
+    history / present_ema digests, final history bits and paths equal the
+    reference's recorded run."""
+    import json
+    gd = np.load(os.path.join(golden_dir, name))
+    for k, v in json.loads(str(gd["env"])).items():
+        monkeypatch.setenv(k, v)
+    X, Y, L = int(gd["X"]), int(gd["Y"]), int(gd["L"])
+    reqs = [(f"N{i:03d}", int(s), int(d)) for i, (s, d) in enumerate(zip(gd["src"], gd["dst"]))]
+    say(f"[ref router] {name} {X}x{Y}x{L} nets={len(reqs)} cap={int(gd['cap'])}")
+    res, rec = run_router_ref(X, Y, L, reqs, int(gd["cap"]))
+    assert [tuple(x) for x in gd["over"].tolist()] == [r[:3] for r in rec]
+    assert [_digest(r[3]) for r in rec] == gd["hist_digest"].tolist()
+    assert [_digest(r[4]) for r in rec] == gd["ema_digest"].tolist()
+    np.testing.assert_array_equal(rec[-1][3].view(np.uint32), gd["final_history"].view(np.uint32))
+    off = 0
+    for i, n in enumerate(gd["final_len"]):
+        assert list(res["paths"].get(reqs[i][0], [])) == gd["final_nodes"][off:off + n].tolist(), reqs[i][0]
+        off += n

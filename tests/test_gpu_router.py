@@ -70,6 +70,43 @@ def test_negotiation_matches_oracle(case, slots):
     assert bool(g_res.get("success")) == bool(o_res.get("success"))
 
 
+@pytest.mark.parametrize("shape", [(12, 10, 16, 30, 16), (10, 8, 24, 24, 24)], ids=["l16", "l24"])
+def test_negotiation_many_layers_matches_oracle(shape, monkeypatch):
+    """14 and 22 signal layers with a fast pres_fac ramp (the reference's
+    ORTHO_PRES_FAC_MULT override): the pres_fac_max 64 / 128 and history
+    weight branches and via annealing; batched GPU run vs the oracle with
+    the same search semantics."""
+    X, Y, L, n, seed = shape
+    monkeypatch.setenv("ORTHO_PRES_FAC_MULT", "1.6")
+    print(f"[router] {X}x{Y}x{L} nets={n}", flush=True)
+    reqs = requests(X, Y, n, seed)
+    o_res, o_rec = run_oracle(X, Y, L, "reference", reqs, 30)
+    g_res, g_rec = run_gpu(X, Y, L, "reference", reqs, 30, 1024, delta=None)
+    assert [r[:3] for r in g_rec] == [r[:3] for r in o_rec]
+    for a, b in zip(g_rec, o_rec):
+        np.testing.assert_array_equal(a[3].view(np.uint32), b[3].view(np.uint32), err_msg=f"history it {a[0]}")
+    for nm, *_ in reqs:
+        assert list(g_res["paths"].get(nm, [])) == list(o_res["paths"].get(nm, [])), nm
+
+
+@pytest.mark.parametrize("policy", ["reference", "adjacent"])
+def test_cfg1_full_negotiation_matches_oracle(policy):
+    """The whole cfg1 board: 96x96x6, its 300 seeded top-layer nets, 30
+    iterations, batched on the GPU vs the oracle with the same search
+    semantics (per-iteration overuse, history bits, final paths)."""
+    from orthoroute_amd import workloads
+    X, Y, L, reqs = workloads.cfg1()
+    print(f"[router] cfg1 {X}x{Y}x{L} {policy} nets={len(reqs)} cap=30", flush=True)
+    g_res, g_rec = run_gpu(X, Y, L, policy, reqs, 30, 1024, delta=None)
+    print(f"[router] cfg1 gpu done: final overuse {g_rec[-1][1]}", flush=True)
+    o_res, o_rec = run_oracle(X, Y, L, policy, reqs, 30)
+    assert [r[:3] for r in g_rec] == [r[:3] for r in o_rec]
+    for a, b in zip(g_rec, o_rec):
+        np.testing.assert_array_equal(a[3].view(np.uint32), b[3].view(np.uint32), err_msg=f"history it {a[0]}")
+    for nm, *_ in reqs:
+        assert list(g_res["paths"].get(nm, [])) == list(o_res["paths"].get(nm, [])), nm
+
+
 def test_negotiation_graph_rounds_matches_oracle():
     """The whole negotiation with every poll's rounds replayed as a captured
     HIP graph: the oracle's trajectory, history bits and paths."""

@@ -75,3 +75,27 @@ def test_mix_job_matches_oracle(alone):
                                  iteration_cb=lambda it, t, v, s, c: otraj.append((it, s, c)))
     assert otraj == traj
     assert _paths(ores, job) == _paths(res, job)
+
+
+def test_large_job_alone_equals_mix():
+    """A job at cfg5's lower end (160x160x4 = 102,400 nodes, 500 nets, 20
+    iterations) routed alone and in a mix with two smaller jobs that arrive
+    while it runs: the same per-iteration overuse and final paths."""
+    from orthoroute_amd import workloads
+    from orthoroute_amd.serving import Job, RoutingServer, replay, run_job
+    _, _, _, big_nets = workloads.cfg1(seed=500, X=160, Y=160, L=4, n_nets=500)
+    big = Job(10, 160, 160, 4, big_nets, arrival_s=0.0, policy="adjacent", max_iterations=20)
+    print(f"[serving] large job alone: {big.X}x{big.Y}x{big.L} {len(big.nets)} nets", flush=True)
+    ares, atraj, dt, _ = run_job(big)
+    print(f"[serving] alone: {len(atraj)} iterations in {dt:.2f} s", flush=True)
+    others = [_jobs()[1], _jobs()[3]]
+    for k, j in enumerate(others):
+        j.arrival_s = 0.05 * (k + 1)
+    srv = RoutingServer(hbm_budget=32 << 30, max_active=3, roi_limit=256 << 20)
+    res = replay(srv, [big] + others)
+    srv.close()
+    r = res[0]
+    assert r.error is None, r.error
+    assert srv.peak_active >= 2, "jobs never overlapped"
+    assert r.trajectory == atraj
+    assert _paths(r.__dict__, big) == _paths(ares, big)
