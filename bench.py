@@ -68,6 +68,15 @@ def parse():
                     help="nets of the 96x96x6 board whose full negotiation is timed to zero overuse (0: skip)")
     ap.add_argument("--mt-nets", type=int, default=200,
                     help="cfg3 multi-terminal nets routed as trees after the timed steps (0: skip)")
+    ap.add_argument("--cfg2-iterations", type=int, default=8,
+                    help="iteration cap of the full backplane negotiation timed batched and plain (0: skip)")
+    ap.add_argument("--cfg2-plain", type=int, default=1, help="also run the backplane negotiation plain (0: no)")
+    ap.add_argument("--cfg3-iterations", type=int, default=5,
+                    help="iteration cap of cfg3's 1,400 two-pin nets negotiated batched and plain (0: skip)")
+    ap.add_argument("--ref-iterations", type=int, default=30,
+                    help="iteration cap of the reference-order negotiation of the full 96x96x6 board, checked "
+                         "against the reference's recorded trajectory (0: skip)")
+    ap.add_argument("--cost-reps", type=int, default=5, help="timed repetitions of each cost-update kernel (0: skip)")
     ap.add_argument("--graph-rounds", action="store_true", help="replay the rounds between polls as HIP graphs")
     ap.add_argument("--serve-jobs", type=int, default=16, help="serving stream: jobs (0 = skip)")
     ap.add_argument("--serve-max-nodes", type=float, default=4e6, help="serving stream: largest board (nodes)")
@@ -218,16 +227,33 @@ def main():
                            args.cpu_seconds, X, Y, L, compare=bool(sreqs))
 
     geo = geometry(r)
+    costk = cost_kernels(r, pres_fac, hmult, args.cost_reps) if args.cost_reps > 0 else None
+    neg2 = negotiation(args, "cfg2", args.cfg2_iterations, bool(args.cfg2_plain)) if args.cfg2_iterations > 0 else None
+    neg3 = negotiation(args, "cfg3", args.cfg3_iterations, True) if args.cfg3_iterations > 0 else None
+    refo = reference_order(args) if args.ref_iterations > 0 else None
     mt = multi_terminal(args) if args.mt_nets > 0 else None
     ttz = time_to_zero_overuse(args) if args.ttz_nets > 0 else None
     stress_line = stress(args) if args.stress_nets > 0 else None
     serve = serving(args) if args.serve_jobs > 0 else None
 
-    traffic = None
-    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_round_cfg2_traffic.json")
+    # HBM traffic per launch comes from a committed rocprofv3 --pmc profile;
+    # it is reported only when that profile measured this build's kernel
+    # source (the digest it records equals the digest of sssp.hip now)
+    traffic, traffic_note = None, "no counter profile"
+    here = os.path.dirname(os.path.abspath(__file__))
+    prof = os.path.join(here, "profiles", "k_round_cfg2_traffic.json")
     if os.path.exists(prof):
+        import hashlib
         with open(prof) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        with open(os.path.join(here, "orthoroute_amd", "csrc", "sssp.hip"), "rb") as f:
+            src_digest = hashlib.sha256(f.read()).hexdigest()
+        if tj.get("kernel_source_sha256") == src_digest:
+            traffic = tj.get("hbm_bytes_per_launch")
+            traffic_note = f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel source ({src_digest[:12]}), " \
+                           f"copy-calibrated: profiles/k_round_cfg2_traffic.json"
+        else:
+            traffic_note = "the counter profile measured another version of the kernel source: not reported"
 
     line = {
         "metric": METRIC,
@@ -255,7 +281,13 @@ def main():
         "nets_per_s": round(len(tasks) * K / wall, 1),
         "routed": routed, "failed": failed,
         "minimal_edges_per_step": int(min_edges), "minimal_nodes_per_step": int(min_nodes),
+        "value_kind": "Dijkstra-minimal equivalent: out-edges of the nodes a label-setting search of the same "
+                      "requests settles, per second of step wall time (the searches scan fewer, see "
+                      "edges_scanned_per_step)",
         "edges_scanned_per_step": int(d.get("sssp_edges", 0) / K),
+        "scanned_edges_per_s_wall": round(d.get("sssp_edges", 0) / wall / 1e9, 3),
+        "scanned_edges_per_s_rounds": round(d.get("sssp_edges", 0) / max(round_ms * 1e-3, 1e-12) / 1e9, 3),
+        "scanned_unit": "G edges/s",
         "sssp_s_per_step": round(d.get("sssp_s", 0) / K, 3),
         "sssp_kernel_gteps": round(min_edges * K / max(d.get("sssp_kernel_ms", 0) * 1e-3, 1e-9) / 1e9, 4),
         "roi_s_per_step": round(d.get("roi_s", 0) / K, 3),
@@ -265,17 +297,200 @@ def main():
                              "identical_paths": same},
         "roofline": {"kernel": "k_round", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "traffic_note": traffic_note,
                      "avg_launch_us": round(avg_launch_us, 2), "launches": int(launches),
                      "alg_bytes_per_launch": round(alg_bytes / max(launches, 1), 1),
                      "dijkstra_minimal_equiv_GBs": round(dij_equiv, 1)},
         "cpu_baseline": cpu,
         "geometry": geo,
+        "cost_kernels": costk,
+        "negotiation_cfg2": neg2,
+        "negotiation_cfg3": neg3,
+        "reference_order": refo,
         "multi_terminal": mt,
         "time_to_zero_overuse": ttz,
         "stress": stress_line,
         "serving": serve,
     }
     print(json.dumps(line), flush=True)
+
+
+def cost_kernels(r, pres_fac, hmult, reps):
+    """HIP-event times of the per-iteration congestion kernels on the
+    backplane (16.8 M edges): the cost update (k_cost, 22 algorithmic bytes
+    per edge: base, EMA, history, total f32 and layer / kind u8), the
+    history update (k_history, 20 B: present, base, history in and out plus
+    the decay term) and the present EMA (k_ema, 12 B: count, EMA in and out;
+    its call also compacts the EMA's nonzeros and copies them to the host,
+    which the time includes).  Run after the timed steps; the cost update
+    rewrites the same total_cost from unchanged inputs."""
+    import numpy as np
+    import torch
+    st = r.accounting.state
+    E = r.accounting.E
+    out = {}
+
+    def timed(label, fn, nbytes):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        us = 1e3 * e0.elapsed_time(e1) / reps
+        gbs = nbytes * E / (us * 1e-6) / 1e9
+        out[label] = {"us": round(us, 1), "bytes_per_edge": nbytes, "GBs": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+    orig = st.update_costs
+
+    def update_timed(p):
+        # the cost parameters point at buffers that live only during this call
+        timed("k_cost", lambda: orig(p), 22)
+    st.update_costs = update_timed
+    try:
+        r._update_costs(pres_fac, r.config.hist_cost_weight * hmult, 1.0, r.config.base_cost_weight,
+                        r._layer_bias_ema, 1)
+    finally:
+        del st.update_costs
+    timed("k_history", lambda: st.update_history(1.0, 0.2, 15.0), 20)
+    timed("k_ema_with_compaction", lambda: st.update_ema(False, np.float32(0.4), np.float32(0.6)), 12)
+    log(f"[bench] cost kernels: " + ", ".join(f"{k} {v['us']} us ({v['GBs']} GB/s)" for k, v in out.items()))
+    out["edges"] = E
+    return out
+
+
+def negotiation(args, which, cap, with_plain):
+    """route_multiple_nets on a whole board from entry to exit (lattice and
+    cost state resident), batched and, if asked, plain (one slot: searches one
+    after another), with the iteration cap `cap`: time to zero overuse when
+    the schedule gets there, else the time to the cap; both forms must give
+    the same per-iteration overuse and final paths.  cfg2: the backplane's
+    8,000 nets; cfg3: the BGA board's 1,400 two-pin ball-to-perimeter nets."""
+    import torch
+    from orthoroute_amd import workloads
+    from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig, PathFinderRouter
+    X, Y, L, reqs = workloads.CONFIGS[which]()
+
+    def run(slots):
+        cfg = PathFinderConfig()
+        cfg.lattice_policy = "adjacent"
+        cfg.batch_slots = slots
+        cfg.max_iterations_cap = cap
+        cfg.sssp_graph_rounds = args.graph_rounds
+        r = PathFinderRouter(cfg)
+        r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
+        torch.cuda.synchronize()
+        rec, t_it = [], []
+        t0 = time.perf_counter()
+
+        def cb(it, t, v, s, c):
+            rec.append((it, s, c))
+            t_it.append(time.perf_counter() - t0)
+        res = r.route_multiple_nets(reqs, iteration_cb=cb)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        paths = {nm: list(map(int, res["paths"].get(nm, []))) for nm, *_ in reqs}
+        stats = dict(r.stats)
+        del r
+        torch.cuda.empty_cache()
+        return dt, res, rec, t_it, paths, stats
+    dt, res, rec, t_it, paths, stats = run(args.slots)
+    zero = next((t for (it, s, c), t in zip(rec, t_it) if s == 0), None)
+    log(f"[bench] {which} negotiation batched: {len(rec)} iterations in {dt:.2f}s, overuse at exit {rec[-1][1]}, "
+        f"zero overuse at {zero}")
+    out = {"workload": f"{which} {X}x{Y}x{L}, adjacent-layer vias, {len(reqs)} two-pin nets, iteration cap {cap}",
+           "iterations": len(rec), "time_to_zero_overuse_s": None if zero is None else round(zero, 3),
+           "reached_zero_overuse": zero is not None, "converged": bool(res.get("converged")),
+           "time_to_exit_s": round(dt, 3), "overuse_at_exit": int(rec[-1][1]),
+           "overuse_per_iteration": [int(x[1]) for x in rec],
+           "iteration_s": [round(b - a, 3) for a, b in zip([0.0] + t_it[:-1], t_it)],
+           "sssp_s": round(stats.get("sssp_s", 0.0), 3), "roi_s": round(stats.get("roi_s", 0.0), 3),
+           "searches": int(stats.get("sssp_searches", 0))}
+    if with_plain:
+        pdt, pres, prec, pt_it, ppaths, _ = run(1)
+        same = prec == rec and ppaths == paths
+        log(f"[bench] {which} negotiation plain: {len(prec)} iterations in {pdt:.2f}s; same trajectory={same}")
+        out["plain"] = {"slots": 1, "time_to_exit_s": round(pdt, 3), "same_trajectory_and_paths": bool(same),
+                        "speedup_batched": round(pdt / max(dt, 1e-9), 2)}
+    return out
+
+
+def reference_order(args):
+    """The reference-order search (float64 sums, float32 acceptance,
+    (alt, roi position) pops: SimpleDijkstra's own choices) driving the whole
+    negotiation of the 96x96x6 board with its 300 nets, against the
+    reference's recorded run of the same board (tests/golden/
+    trajectory_cfg1.npz: per-iteration overuse and sha256 digests of the
+    history / present_ema arrays, final paths).  The reference's own time for
+    that run (CPython, one core of the build container) is in the fixture and
+    reported beside it, labelled as such; the reference does not run here."""
+    import hashlib
+    import numpy as np
+    import torch
+    from orthoroute_amd import workloads
+    from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig, PathFinderRouter
+    fx = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "trajectory_cfg1.npz")
+    if not os.path.exists(fx):
+        return None
+    g = np.load(fx)
+    X, Y, L, reqs = workloads.cfg1()
+    cap = min(args.ref_iterations, int(g["cap"]))
+
+    def dig(t):
+        return hashlib.sha256(np.ascontiguousarray(t.cpu().numpy()[:E], np.float32).tobytes()).hexdigest()
+
+    def run(order):
+        nonlocal E
+        cfg = PathFinderConfig()
+        cfg.lattice_policy = "reference"
+        cfg.max_iterations_cap = cap
+        cfg.sssp_order = order
+        r = PathFinderRouter(cfg)
+        r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
+        E = r.accounting.E
+        torch.cuda.synchronize()
+        rec = []
+        t0 = time.perf_counter()
+        cbt = [0.0]
+
+        def cb(it, t, v, s, c):
+            t1 = time.perf_counter()
+            rec.append(((it, s, c), dig(r.accounting.history), dig(r.accounting.present_ema)))
+            cbt[0] += time.perf_counter() - t1
+        res = r.route_multiple_nets(reqs, iteration_cb=cb)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0 - cbt[0]
+        return dt, res, rec, dict(r.stats)
+    E = 0
+    dt, res, rec, st = run("reference")
+    n = min(len(rec), len(g["over"]))
+    same_over = [x[0] for x in rec[:n]] == [tuple(v) for v in g["over"].tolist()[:n]]
+    same_hist = [x[1] for x in rec[:n]] == g["hist_digest"].tolist()[:n]
+    same_ema = [x[2] for x in rec[:n]] == g["ema_digest"].tolist()[:n]
+    same_paths = None
+    if cap == int(g["cap"]):
+        off, ok = 0, True
+        for i, k in enumerate(g["final_len"]):
+            ok &= list(map(int, res["paths"].get(reqs[i][0], []))) == g["final_nodes"][off:off + k].tolist()
+            off += k
+        same_paths = bool(ok)
+    cdt, cres, crec, _ = run("canonical")
+    log(f"[bench] reference order cfg1: {len(rec)} iterations in {dt:.2f}s (canonical {cdt:.2f}s); equal to the "
+        f"reference: overuse {same_over}, history {same_hist}, ema {same_ema}, final paths {same_paths}")
+    first_div = next((a[0][0] for a, b in zip(crec, rec) if a[0] != b[0] or a[1] != b[1]), None)
+    return {"workload": f"cfg1 96x96x6 (reference via rules), {len(reqs)} nets, iteration cap {cap}",
+            "iterations": len(rec), "s": round(dt, 3), "searches": int(st.get("sssp_searches", 0)),
+            "equal_to_reference": {"overuse_per_iteration": bool(same_over), "history_digests": bool(same_hist),
+                                   "present_ema_digests": bool(same_ema), "final_paths": same_paths},
+            "reference_cpu": {"s": round(float(g["t_route_s"]), 1), "cores": 1, "kind": "reference",
+                              "where": "the reference's route_multiple_nets (CPython) recorded by "
+                                       "tests/golden/make_golden.py under taskset -c 0 in the build container",
+                              "cpu": str(g["cpu"]), "speedup_gpu": round(float(g["t_route_s"]) / max(dt, 1e-9), 1)},
+            "canonical_order": {"s": round(cdt, 3), "first_iteration_differing_from_reference_order": first_div}}
 
 
 def sample_reqs(r, tasks, n):

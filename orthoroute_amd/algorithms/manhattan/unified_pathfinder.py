@@ -39,6 +39,7 @@ from ... import _abi
 from ...device import (Context, DeviceLattice, EdgeState, SearchParams, cost_bounds, max_slots, rank_exceptions,
                        roi_bfs, route as device_route, route_flat as device_route_flat,
                        route_ms as device_route_ms, route_ref as device_route_ref)
+from ...autotune import BucketTuner, base_width, default_target
 from ...lattice import DIR_H, DIR_HV, DIR_V, layer_directions, reference_via_pairs
 from ...reductions import sum_f32
 from .board_analyzer import analyze_board_characteristics
@@ -713,28 +714,42 @@ class PathFinderRouter:
     def _flat(self):
         """Flat arrays over the nets that hold a path, in net_paths order:
         per edge (edge id, src node, dst node, net id) and per via hop
-        (x, y, z_lo, z_hi, net id)."""
-        nodes_l, edges_l, ids = [], [], []
+        (x, y, z_lo, z_hi, net id).  The result is kept until a committed
+        path changes: the cache holds the (name, path, edges) objects it was
+        built from and is reused only when every one is the same object."""
+        items = []
         for name, path in self.net_paths.items():
             if not path or len(path) < 2:
                 continue
-            nodes = self._net_nodes.get(name)
+            items.append((name, path, self._net_nodes.get(name), self._net_to_edges.get(name)))
+        cache = getattr(self, "_flat_cache", None)
+        if cache is not None and len(cache[0]) == len(items) and all(
+                a[0] == b[0] and a[1] is b[1] and a[2] is b[2] and a[3] is b[3] for a, b in zip(cache[0], items)):
+            return cache[1]
+        nodes_l, edges_l, ids = [], [], []
+        for name, path, nodes, e in items:
             if nodes is None:
                 nodes = np.asarray(path, dtype=np.int64)
-            e = self._net_to_edges.get(name)
             if e is None or len(e) != len(nodes) - 1:
                 e = np.full(len(nodes) - 1, -1, np.int64)
             nodes_l.append(nodes); edges_l.append(e); ids.append(self._get_net_id(name))
         z0 = np.zeros(0, np.int64)
         if not nodes_l:
-            return z0, z0, z0, z0, np.zeros((0, 4), np.int64), z0
-        S = np.concatenate([n[:-1] for n in nodes_l]).astype(np.int64)
-        D = np.concatenate([n[1:] for n in nodes_l]).astype(np.int64)
-        E = np.concatenate(edges_l).astype(np.int64)
-        NS = np.repeat(np.asarray(ids, np.int64), [len(n) - 1 for n in nodes_l])
-        H, NH = self._hops_of(S, D, NS)
-        ok = E >= 0
-        return E[ok], S[ok], D[ok], NS[ok], H, NH
+            out = (z0, z0, z0, z0, np.zeros((0, 4), np.int64), z0)
+        else:
+            lens = np.fromiter((len(n) for n in nodes_l), dtype=np.int64, count=len(nodes_l))
+            alln = np.concatenate(nodes_l).astype(np.int64)
+            inner = np.ones(len(alln) - 1, dtype=bool)
+            inner[np.cumsum(lens)[:-1] - 1] = False      # pairs that straddle two paths
+            S = alln[:-1][inner]
+            D = alln[1:][inner]
+            E = np.concatenate(edges_l).astype(np.int64)
+            NS = np.repeat(np.asarray(ids, np.int64), lens - 1)
+            H, NH = self._hops_of(S, D, NS)
+            ok = E >= 0
+            out = (E[ok], S[ok], D[ok], NS[ok], H, NH)
+        self._flat_cache = (items, out)
+        return out
 
     def _hops_of(self, S, D, NS):
         """Via hops (x, y, z_lo, z_hi) among node pairs S->D, with their nets.
@@ -856,17 +871,64 @@ class PathFinderRouter:
         self._net_nodes.pop(nid, None)
 
     # ---------------- ordering / hotset / rip (UPF:4401-4439, 5009-5140, 5300-5335)
-    def _dense(self, idx, val):
-        d = np.zeros(self.accounting.E, dtype=np.float32)
+    def _dense(self, idx, val, slot=None):
+        """val scattered into a dense float32[E] (zero elsewhere).  With a
+        slot name the array is that slot's persistent buffer: only the
+        entries its previous contents set are cleared, instead of allocating
+        and zero-filling E floats on every call; the previous contents of the
+        slot are gone."""
+        if slot is None:
+            d = np.zeros(self.accounting.E, dtype=np.float32)
+            d[idx] = val
+            return d
+        bufs = self.__dict__.setdefault("_dense_bufs", {})
+        ent = bufs.get(slot)
+        if ent is None or len(ent[0]) != self.accounting.E:
+            ent = [np.zeros(self.accounting.E, dtype=np.float32), np.zeros(0, np.int64)]
+            bufs[slot] = ent
+        d, prev = ent
+        d[prev] = 0.0
+        idx = np.asarray(idx, dtype=np.int64)
         d[idx] = val
+        ent[1] = idx
         return d
 
     @staticmethod
-    def _seqsum(vals):
-        """Python's float sum (sequential float64), as the reference's sum()."""
-        if len(vals) == 0:
-            return 0.0
-        return float(np.cumsum(vals.astype(np.float64))[-1])
+    def _path_sums(segs, dense):
+        """Per edge list, Python's sum() of dense[e] over its edges in path
+        order (sequential float64), for non-negative values: adding +0.0
+        leaves such a sum unchanged, so only the nonzero values are added,
+        every list's j-th one at step j."""
+        out = np.zeros(len(segs), dtype=np.float64)
+        if not segs:
+            return out
+        lens = np.fromiter((len(e) for e in segs), dtype=np.int64, count=len(segs))
+        flat = np.concatenate(segs).astype(np.int64) if lens.sum() else np.zeros(0, np.int64)
+        vals = dense[flat]
+        nzm = vals != 0
+        if not nzm.any():
+            return out
+        v = vals[nzm].astype(np.float64)
+        sg = np.repeat(np.arange(len(segs)), lens)[nzm]
+        cnt = np.bincount(sg, minlength=len(segs))
+        pos = np.arange(len(v)) - (np.cumsum(cnt) - cnt)[sg]
+        # lists with about as many nonzeros share a zero-padded matrix whose
+        # rows are summed left to right by cumsum (an accumulate is
+        # sequential; the padding adds +0.0)
+        cls = np.zeros(len(segs), dtype=np.int64)
+        has = cnt > 0
+        cls[has] = np.ceil(np.log2(cnt[has])).astype(np.int64)
+        by = np.argsort(cls[sg], kind="stable")          # nonzeros grouped by class, list order kept
+        kk = cls[sg][by]
+        ks = np.unique(kk)
+        bounds = np.searchsorted(kk, np.append(ks, ks[-1] + 1))
+        for j, k in enumerate(ks.tolist()):
+            idx = by[bounds[j]:bounds[j + 1]]
+            members, row = np.unique(sg[idx], return_inverse=True)
+            m = np.zeros((len(members), 1 << k), dtype=np.float64)
+            m[row, pos[idx]] = v[idx]
+            out[members] = np.cumsum(m, axis=1)[:, -1]
+        return out
 
     def _order_nets_by_difficulty(self, tasks, over_dense):
         """UPF:4401-4439: score = manhattan*3*(1 + mean overuse on the net's
@@ -885,28 +947,12 @@ class PathFinderRouter:
         cong = np.zeros(len(names), dtype=np.float64)
         with_edges = [i for i, n in enumerate(names) if n in self._net_to_edges]
         if with_edges and over_dense is not None and len(over_dense):
+            # the reference's sum() over the net's previous path, in path order
             segs = [self._net_to_edges[names[i]] for i in with_edges]
             lens = np.fromiter((len(e) for e in segs), dtype=np.int64, count=len(segs))
-            flat = np.concatenate(segs) if segs else np.zeros(0, np.int64)
-            vals = over_dense[flat]
-            nzm = vals != 0
-            if nzm.any():
-                # The reference's sum() adds the values in path order in
-                # float64; adding +0.0 leaves a non-negative sum unchanged, so
-                # the sum over the nonzero values in order is the same number.
-                # Step j adds every net's j-th nonzero value at once.
-                v = vals[nzm].astype(np.float64)
-                sg = np.repeat(np.arange(len(segs)), lens)[nzm]
-                cnt = np.bincount(sg, minlength=len(segs))
-                pos = np.arange(len(v)) - (np.cumsum(cnt) - cnt)[sg]
-                order = np.argsort(pos, kind="stable")
-                bounds = np.searchsorted(pos[order], np.arange(int(cnt.max()) + 1))
-                acc = np.zeros(len(segs), dtype=np.float64)
-                for j in range(int(cnt.max())):
-                    sl = order[bounds[j]:bounds[j + 1]]
-                    acc[sg[sl]] += v[sl]
-                hitn = cnt > 0
-                cong[np.asarray(with_edges)[hitn]] = acc[hitn] / np.maximum(1, lens[hitn])
+            acc = self._path_sums(segs, over_dense)
+            hitn = acc > 0
+            cong[np.asarray(with_edges)[hitn]] = acc[hitn] / np.maximum(1, lens[hitn])
         score = (dist * 3).astype(np.float64) * (cong + 1.0)
         # descending (score, name): names are unique, so this is the order
         # of the reference's reverse tuple sort
@@ -928,12 +974,12 @@ class PathFinderRouter:
             unrouted = {n for n in tasks if not self.net_paths.get(n)}
             return unrouted | ripped
         E, _, _, NE, _, _ = flat
-        hot_mask = np.zeros(self.accounting.E, dtype=bool)
-        hot_mask[hot_idx] = True
-        hot_val = np.zeros(self.accounting.E, dtype=np.float32)
-        hot_val[hot_idx] = hot_over
+        # hot edges: a 1 marker and their overuse in two slot buffers (an
+        # edge can be hot with zero overuse)
+        hot_mask = self._dense(hot_idx, 1.0, slot="hot_mask")
+        hot_val = self._dense(hot_idx, hot_over, slot="hot_val")
         id_to_name = {v: k for k, v in self.net_id_map.items()}
-        offenders = {id_to_name[int(i)] for i in np.unique(NE[hot_mask[E]])} if len(E) else set()
+        offenders = {id_to_name[int(i)] for i in np.unique(NE[hot_mask[E] != 0])} if len(E) else set()
         for n in tasks:
             if n in offenders:
                 self._net_clean_iters[n] = 0
@@ -943,11 +989,9 @@ class PathFinderRouter:
         offenders -= {n for n in offenders if self._net_clean_iters.get(n, 0) >= thr}
         offenders |= ripped
         unrouted = {n for n in tasks if not self.net_paths.get(n)}
-        scores = []
-        for n in offenders:
-            if n in self._net_to_edges:
-                e = self._net_to_edges[n]
-                scores.append((self._seqsum(hot_val[e][hot_mask[e]]), n))
+        scored = [n for n in offenders if n in self._net_to_edges]
+        sums = self._path_sums([self._net_to_edges[n] for n in scored], hot_val)
+        scores = list(zip(sums.tolist(), scored))
         for n in unrouted:
             if n not in offenders:
                 scores.append((0.0, n))
@@ -967,18 +1011,11 @@ class PathFinderRouter:
         return set(cool)
 
     def _rip_top_k_offenders(self, o_idx, o_val, k=20):
-        over = self._dense(o_idx, o_val)
-        mask = np.zeros(self.accounting.E, dtype=bool)
-        mask[o_idx] = True
-        scores = []
-        for n, path in self.net_paths.items():
-            if not path or n in self.locked_nets:
-                continue
-            if n in self._net_to_edges:
-                e = self._net_to_edges[n]
-                imp = self._seqsum(over[e][mask[e]])
-                if imp > 0:
-                    scores.append((imp, n))
+        over = self._dense(o_idx, o_val, slot="rip")
+        cand = [n for n, path in self.net_paths.items()
+                if path and n not in self.locked_nets and n in self._net_to_edges]
+        imps = self._path_sums([self._net_to_edges[n] for n in cand], over)
+        scores = [(v, n) for v, n in zip(imps.tolist(), cand) if v > 0]
         scores.sort(reverse=True)
         victims = {n for _, n in scores[:k]}
         for n in victims:
@@ -998,15 +1035,13 @@ class PathFinderRouter:
         d = self.config.sssp_delta
         nb = self.config.sssp_buckets
         if d is None or nb is None:
-            from ...autotune import choose_params
-            d2, nb2 = choose_params(self, slots, getattr(self, "_cost_scale", 0.0))
+            d2, nb2 = base_width(self, getattr(self, "_cost_scale", 0.0)), 8
             target = self.config.sssp_target_buckets
             if target is None:
-                # measured: the 12-iteration negotiation on the 96x96x6
-                # lattice (55 k nodes) takes 0.140 s at 32 and 0.153 s at
-                # 128; 30 iterations on the 600x400x18 backplane (4.3 M
-                # nodes) take 5.65 s at 128 and 7.95 s at 32
-                target = 128 if self.N >= 1_000_000 else 32
+                # a negotiation's tuner picks the target of batches with a
+                # depth estimate; otherwise the starting default
+                tuner = getattr(self, "_tuner", None)
+                target = tuner.target() if (tuner is not None and depth is not None) else default_target(self.N)
             if depth is not None and target:
                 d2 = max(d2, float(depth) / float(target))
             d = d if d is not None else d2
@@ -1115,7 +1150,12 @@ class PathFinderRouter:
             return out
         nodes, edges, offs, _, st = device_route_flat(self.ctx, dlat, self.accounting.total_cost, sreqs, params,
                                                       roi_bitmaps=rows, keepout_owner=ko)
-        self.stats["sssp_s"] += time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        self.stats["sssp_s"] += dt
+        tuner = getattr(self, "_tuner", None)
+        if (tuner is not None and depth_hint is not None and self.config.sssp_target_buckets is None
+                and self.config.sssp_delta is None):
+            tuner.record(dt)
         self.stats["sssp_kernel_ms"] += st["kernel_ms"]
         self.stats["sssp_rounds"] += st["rounds"]
         self.stats["sssp_edges"] += st["edges"]
@@ -1288,7 +1328,7 @@ class PathFinderRouter:
             np.cumsum(self.via_seg_pres, axis=2, out=self.via_seg_prefix)
         # over = max(0, present - 1) at iteration start (counts)
         p_idx, p_val = self.accounting.state.overuse_sparse(0)
-        over_map = self._dense(p_idx, p_val)
+        over_map = self._dense(p_idx, p_val, slot="over_map")
         ema_idx, ema_val = self._update_present_ema(first=(it == 1))
         lb = self._compute_layer_bias(ema_idx, ema_val)
         vcm = 1.0
@@ -1316,6 +1356,10 @@ class PathFinderRouter:
         stagnant = 0
         self._negotiation_ran = True
         self._last_depth_hint = None
+        # the bucket target of the batches with a depth estimate, tuned on
+        # this negotiation's own search times (speed only: paths are the same)
+        slots_key = int(min(int(cfg.batch_slots), max_slots(self.N)) > 1)
+        self._tuner = BucketTuner(f"{self._Nx}x{self._Ny}x{self._Nz}:{self.lattice.policy}:{slots_key}", self.N)
         self._net_failure_count = {}
         self._excluded_nets = set()
         routed = failed = 0
@@ -1562,14 +1606,39 @@ class PathFinderRouter:
         """UPF:5749-5830 with _segment_world / _via_world (UPF:5596-5617,
         5652-5675): every path becomes its maximal straight track runs (a run
         ends at a direction change, before a via and at the path's end) and
-        one via record per layer change, in path order.  Planar hops on the
-        outer layers are refused on the reference lattice, whose outer layers
-        carry no planar edges.  Vectorised over all paths; a path with a hop
-        that is not a lattice move (the reference skips such hops without
-        advancing its walk) goes through the per-hop walk instead."""
+        one via record per layer change, in path order, nets in net_paths
+        order.  A net's records depend on its path alone, so they are kept
+        per net and recomputed only for nets whose path object changed since
+        the last call (a late negotiation iteration reroutes a hundred of
+        thousands of nets); the records of an unchanged net are the same
+        objects as before."""
         items = [(net, p) for net, p in self.net_paths.items() if p]
         if not items:
             return [], []
+        g = self.lattice.geom
+        names = list(self.config.layer_names or [])
+        sig = (float(g.pitch), float(g.grid_min_x), float(g.grid_min_y), self.config.grid_pitch * 0.6,
+               tuple(names), self.lattice.policy, self._Nx, self._Ny, self._Nz)
+        cache = getattr(self, "_geo_cache", None)
+        if cache is None or cache[0] != sig:
+            cache = (sig, {})
+        per = cache[1]
+        todo = [(net, p) for net, p in items if net not in per or per[net][0] is not p]
+        if todo:
+            for (net, p), rec in zip(todo, self._geometry_records(todo)):
+                per[net] = (p, rec[0], rec[1])
+        keep = {net: per[net] for net, _ in items}
+        self._geo_cache = (sig, keep)
+        tracks = [t for net, _ in items for t in keep[net][1]]
+        vias = [v for net, _ in items for v in keep[net][2]]
+        return tracks, vias
+
+    def _geometry_records(self, items):
+        """(tracks, vias) per (net, path) item, vectorised over all items.
+        Planar hops on the outer layers are refused on the reference lattice,
+        whose outer layers carry no planar edges.  A path with a hop that is
+        not a lattice move (the reference skips such hops without advancing
+        its walk) goes through the per-hop walk instead."""
         X, L = self._Nx, self._Nz
         plane = X * self._Ny
         g = self.lattice.geom
@@ -1625,25 +1694,15 @@ class PathFinderRouter:
         v_rec = [{"net": net_names[k], "x": p1, "y": q1, "from_layer": lname[lo], "to_layer": lname[hi],
                   "diameter": 0.25, "drill": 0.15}
                  for k, p1, q1, lo, hi in zip(v_path.tolist(), vx.tolist(), vy.tolist(), vlo.tolist(), vhi.tolist())]
-        if not len(bad_paths):
-            return t_rec, v_rec
-        # merge in path order with the per-hop walk of the irregular paths
-        bad = set(bad_paths.tolist())
-        tracks, vias = [], []
-        ti = vi = 0
-        for k, (net, p) in enumerate(items):
-            if k in bad:
-                t2, v2 = self._walk_geometry(net, p, lname, guard, width, world)
-                tracks += t2
-                vias += v2
-                continue
-            while ti < len(t_rec) and t_path[ti] == k:
-                tracks.append(t_rec[ti])
-                ti += 1
-            while vi < len(v_rec) and v_path[vi] == k:
-                vias.append(v_rec[vi])
-                vi += 1
-        return tracks, vias
+        # records are in item order: split them per item
+        n = len(items)
+        tb = np.searchsorted(t_path, np.arange(n + 1)).tolist()
+        vb = np.searchsorted(v_path, np.arange(n + 1)).tolist()
+        out = [(t_rec[tb[k]:tb[k + 1]], v_rec[vb[k]:vb[k + 1]]) for k in range(n)]
+        for k in bad_paths.tolist():
+            net, p = items[k]
+            out[k] = self._walk_geometry(net, p, lname, guard, width, world)
+        return out
 
     def _walk_geometry(self, net, path, lname, guard, width, world):
         """The per-hop form of UPF:5756-5830 for one path."""

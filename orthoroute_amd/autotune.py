@@ -1,77 +1,122 @@
-"""Search-parameter selection (bucket width delta, ring size) for the SSSP.
+"""Bucket-width selection for the batched SSSP.
 
-The bucket width trades re-relaxation (wide buckets: more Bellman-Ford rounds
-inside a bucket) against round count (narrow buckets: more kernel launches).
-`choose_params` returns a cached choice per lattice shape; `tune` measures
-candidate (delta, n_buckets) pairs on a sample of the caller's searches and
-records the fastest.  Results are identical for every choice: only speed
-changes.
+The bucket width trades re-expansion (wide buckets: a node's key can drop
+several times inside its bucket, and each drop re-expands its row) against
+round count (narrow buckets: more round launches, each following its
+bucket's chains hop by hop).  Which side dominates depends on the board: on
+the full 8,000-net backplane the late iterations' congested costs make a
+search's cost range wide, and spreading it over 128 buckets re-expands every
+node about eight times over (8,192 buckets are 2.2x faster there), while
+500 nets on the same lattice are fastest at 128 and 2x slower at 8,192.
+Paths are identical for every width; only the time changes.
+
+`base_width` is the width of a batch with no depth estimate (iteration 1):
+2.4 typical planar steps.  A batch with a depth estimate (the spread of its
+nets' current path costs above their lower bounds) is spread over a target
+number of buckets, which `BucketTuner` chooses online during a negotiation:
+it measures each batch's search time, probes the neighbouring targets (x4
+up, then down), keeps the fastest, re-probes every few batches because the
+costs drift as congestion builds, and remembers the settled target per
+lattice shape for the next negotiation in the process.
 """
-import json
-import os
-import time
+import math
 
-_CACHE = {}
-_CACHE_FILE = os.environ.get("ORGPU_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "orgpu_tune.json"))
-
-
-def _key(router, slots):
-    lat = router.lattice
-    return f"{lat.x_steps}x{lat.y_steps}x{lat.layers}:{lat.policy}:{int(slots > 1)}"
+# targets a tuner moves between (x4 apart)
+TARGETS = (32, 128, 512, 2048, 8192, 32768)
+_SETTLED = {}   # lattice key -> settled target, for later negotiations in this process
 
 
-def _load():
-    if _CACHE or not os.path.exists(_CACHE_FILE):
-        return
-    try:
-        with open(_CACHE_FILE) as f:
-            _CACHE.update(json.load(f))
-    except (OSError, ValueError):
-        pass
-
-
-def choose_params(router, slots, cost_scale=0.0):
-    """(delta, n_buckets) for this lattice; a measured entry when one exists,
-    otherwise a width of a few typical planar steps.  cost_scale: the
-    iteration's median planar cost rounded up to a power of two (0 = unknown,
-    then the base cost of a step); congestion raises it in later iterations,
-    and the width follows so that a search keeps a similar number of
-    buckets."""
-    _load()
-    k = _key(router, slots)
-    if k in _CACHE:
-        d, nb = _CACHE[k]
-        return float(d), int(nb)
+def base_width(router, cost_scale=0.0):
+    """Bucket width with no depth estimate: 2.4 x the iteration's median
+    planar cost (cost_scale, rounded up to a power of two), or 4 base steps
+    when the costs are not known yet."""
     if cost_scale > 0.0:
-        return max(2.4 * float(cost_scale), 1e-3), 8
+        return max(2.4 * float(cost_scale), 1e-3)
     step = float(router.config.base_cost_weight) * float(router.lattice.pitch)
-    return max(step * 4.0, 1e-3), 8
+    return max(step * 4.0, 1e-3)
 
 
-def tune(router, sample_reqs, candidates=None, slots=None):
-    """Time candidate (delta, n_buckets) on sample_reqs (list of
-    (src, dst, net_id, use_roi, bonus)) with the router's current costs and
-    keep the fastest for this lattice shape."""
-    slots = slots or router.config.batch_slots
-    step = float(router.config.base_cost_weight) * float(router.lattice.pitch)
-    candidates = candidates or [(step * m, nb) for m in (1, 2, 4, 8, 16) for nb in (4, 8, 16)]
-    best, best_t = None, float("inf")
-    saved = (router.config.sssp_delta, router.config.sssp_buckets)
-    try:
-        for d, nb in candidates:
-            router.config.sssp_delta, router.config.sssp_buckets = d, nb
-            t0 = time.perf_counter()
-            router._search_batch(sample_reqs)
-            t = time.perf_counter() - t0
-            if t < best_t:
-                best, best_t = (d, nb), t
-    finally:
-        router.config.sssp_delta, router.config.sssp_buckets = saved
-    _CACHE[_key(router, slots)] = list(best)
-    try:
-        os.makedirs(os.path.dirname(_CACHE_FILE), exist_ok=True)
-        with open(_CACHE_FILE, "w") as f:
-            json.dump(_CACHE, f)
-    except OSError:
-        pass
-    return best, best_t
+def default_target(n_nodes):
+    """Starting target: 128 buckets from 1 M nodes up, 32 below (measured on
+    the 600x400x18 backplane and the 96x96x6 board)."""
+    return 128 if n_nodes >= 1_000_000 else 32
+
+
+class BucketTuner:
+    """Online choice of the target bucket count for one negotiation.
+
+    The target in use (the incumbent) is measured every batch.  Now and then
+    one batch runs at a neighbouring target (x4 up or down, a probe); the
+    probe's search time is compared with the incumbent's on the batches just
+    before and just after it, so that the drift of the costs as congestion
+    builds cancels, and the probe replaces the incumbent when it is at least
+    `gain` faster (at once when it takes under `clear` of the incumbent's
+    last time; the next probe in the same direction then runs on the very
+    next batch).  A failed probe turns the direction; probes follow every
+    other batch while they win, then back off (`backoff` batches after two
+    failures in a row)."""
+
+    def __init__(self, key, n_nodes, gain=0.25, clear=0.6, backoff=8):
+        start = _SETTLED.get(key, default_target(n_nodes))
+        self.key = key
+        self.inc = TARGETS.index(start) if start in TARGETS else 1
+        self.cur = self.inc
+        self.gain, self.clear, self.backoff = float(gain), float(clear), int(backoff)
+        self.dir = 1
+        self.fails = 0
+        self.since = 0
+        self.last = None          # incumbent's last time
+        self.pending = None       # (probe index, probe time, incumbent time before it)
+        self.history = []
+
+    @property
+    def best(self):
+        return self.inc
+
+    def target(self):
+        return TARGETS[self.cur]
+
+    def _accept(self, i, sec, climb):
+        self.inc = i
+        self.cur = i
+        self.fails = 0
+        self.since = 0
+        self.last = sec
+        _SETTLED[self.key] = TARGETS[i]
+        nxt = i + self.dir
+        if climb and 0 <= nxt < len(TARGETS):
+            self.cur = nxt
+
+    def record(self, seconds):
+        """Records the search time of a batch run at target(); picks the
+        target of the next batch."""
+        sec = float(seconds)
+        self.history.append((TARGETS[self.cur], round(sec, 4)))
+        if self.cur != self.inc:                       # a probe just ran
+            if self.last is not None and sec < self.clear * self.last:
+                self._accept(self.cur, sec, climb=True)
+                return
+            self.pending = (self.cur, sec, self.last)
+            self.cur = self.inc
+            return
+        if self.pending is not None:                   # the batch after a probe
+            p, p_sec, before = self.pending
+            self.pending = None
+            ref = sec if before is None else 0.5 * (before + sec)
+            if p_sec < (1.0 - self.gain) * ref:
+                self._accept(p, p_sec, climb=False)
+                return
+            self.fails += 1
+            self.dir = -self.dir
+        self.last = sec
+        self.since += 1
+        _SETTLED[self.key] = TARGETS[self.inc]
+        wait = 1 if self.fails < 2 else self.backoff
+        if self.since >= wait:
+            nxt = self.inc + self.dir
+            if not 0 <= nxt < len(TARGETS):
+                self.dir = -self.dir
+                nxt = self.inc + self.dir
+            if 0 <= nxt < len(TARGETS):
+                self.cur = nxt
+                self.since = 0

@@ -289,3 +289,21 @@ def test_router_reference_order_many_layers(golden_dir, name, monkeypatch):
     for i, n in enumerate(gd["final_len"]):
         assert list(res["paths"].get(reqs[i][0], [])) == gd["final_nodes"][off:off + n].tolist(), reqs[i][0]
         off += n
+
+
+def test_router_reference_order_cfg1_reproduces_reference(golden_dir):
+    """The whole cfg1 board (96x96x6, 300 nets, 30 iterations) in reference
+    order: the reference's recorded run, digest for digest."""
+    from orthoroute_amd import workloads
+    gd = np.load(os.path.join(golden_dir, "trajectory_cfg1.npz"))
+    X, Y, L, reqs = workloads.cfg1()
+    assert [q[1] for q in reqs] == gd["src"].tolist() and [q[2] for q in reqs] == gd["dst"].tolist()
+    say(f"[ref router] cfg1 {X}x{Y}x{L} nets={len(reqs)} cap={int(gd['cap'])}")
+    res, rec = run_router_ref(X, Y, L, reqs, int(gd["cap"]))
+    assert [tuple(x) for x in gd["over"].tolist()] == [r[:3] for r in rec]
+    assert [_digest(r[3]) for r in rec] == gd["hist_digest"].tolist()
+    assert [_digest(r[4]) for r in rec] == gd["ema_digest"].tolist()
+    off = 0
+    for i, n in enumerate(gd["final_len"]):
+        assert list(res["paths"].get(reqs[i][0], [])) == gd["final_nodes"][off:off + n].tolist(), reqs[i][0]
+        off += n

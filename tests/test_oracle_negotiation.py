@@ -153,3 +153,24 @@ def test_canonical_order_divergence_is_recorded(golden_dir, name):
         equal += list(res["paths"].get(reqs[i][0], [])) == g["final_nodes"][off:off + n].tolist()
         off += n
     assert (first, equal, len(reqs)) == CANON_DIVERGENCE[name]
+
+
+def test_cfg1_trajectory_matches_reference(golden_dir):
+    """The full 96x96x6 board with its 300 nets, 30 iterations: the
+    reference's recorded run (overuse, routing orders, total_cost / history /
+    present_ema digests, final arrays and paths)."""
+    g = load(golden_dir, "trajectory_cfg1.npz")
+    reqs, r, res, rec, orders, cost_digests = run_oracle_compact(g)
+    assert [tuple(x) for x in g["over"].tolist()] == [x[0] for x in rec]
+    assert [x[1] for x in rec] == g["hist_digest"].tolist()
+    assert [x[2] for x in rec] == g["ema_digest"].tolist()
+    assert cost_digests == g["cost_digest"].tolist()
+    off = 0
+    for i, n in enumerate(g["order_len"]):
+        assert orders[i] == g["order"][off:off + n].tolist(), f"order differs at call {i}"
+        off += n
+    np.testing.assert_array_equal(r.history.view(np.uint32), g["final_history"].view(np.uint32))
+    off = 0
+    for i, n in enumerate(g["final_len"]):
+        assert list(res["paths"].get(reqs[i][0], [])) == g["final_nodes"][off:off + n].tolist()
+        off += n
