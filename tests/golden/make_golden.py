@@ -396,6 +396,61 @@ def gen_trajectory(upf, Board, Net, Pad, Coordinate, X, Y, L, reqs, cap, name, e
     return res, t_init, t_route
 
 
+def gen_geometry(upf, Board, Net, Pad, Coordinate):
+    """_generate_geometry_from_paths (UPF:5749-5830, with _segment_world /
+    _via_world UPF:5596-5675) on the 24x22x6 board of negotiation_easy: the
+    reference's own final paths of that run, and crafted paths that reach the
+    walk's special cases (direction changes, U-turns, stacked and long vias,
+    planar hops on the outer layers, non-adjacent and off-axis hops)."""
+    X, Y, L = 24, 22, 6
+    corner = Net(id="corner", name="corner", pads=[
+        Pad(id="c0", component_id="C", net_id="corner", position=Coordinate(3.0, 3.0), size=(0.2, 0.2)),
+        Pad(id="c1", component_id="C", net_id="corner", position=Coordinate(float(X - 4), float(Y - 4)),
+            size=(0.2, 0.2)),
+    ])
+    board = Board(id="b", name="b", nets=[corner], layer_count=L)
+    cfg = upf.PathFinderConfig()
+    cfg.grid_pitch = 1.0
+    cfg.via_cost = 3.0
+    router = upf.PathFinderRouter(cfg, use_gpu=False)
+    router.initialize_graph(board)
+    lat = router.lattice
+
+    def n(x, y, z):
+        return int(lat.node_idx(x, y, z))
+    g = np.load(os.path.join(HERE, "negotiation_easy.npz"))
+    paths = {}
+    off = 0
+    for i, k in enumerate(g["final_len"]):
+        paths[f"R{i:03d}"] = [int(v) for v in g["final_nodes"][off:off + k]]
+        off += int(k)
+    # layer 1 is 'h', layer 2 'v' (UPF:1091-1106)
+    crafted = {
+        "C_run": [n(2, 5, 1), n(3, 5, 1), n(4, 5, 1), n(5, 5, 1)],
+        "C_corner": [n(2, 5, 1), n(3, 5, 1), n(3, 5, 2), n(3, 6, 2), n(3, 7, 2)],
+        "C_uturn": [n(2, 5, 1), n(3, 5, 1), n(4, 5, 1), n(3, 5, 1), n(2, 5, 1)],
+        "C_stack": [n(6, 6, 1), n(6, 6, 2), n(6, 6, 3), n(6, 6, 4), n(6, 7, 4), n(6, 8, 4)],
+        "C_long_via": [n(7, 3, 0), n(7, 3, 4), n(8, 3, 4), n(8, 3, 1)],
+        "C_outer": [n(9, 9, 0), n(9, 10, 0), n(9, 10, 1), n(10, 10, 1), n(10, 10, 5), n(10, 11, 5), n(10, 12, 5)],
+        "C_jump": [n(1, 1, 1), n(2, 1, 1), n(5, 1, 1), n(6, 1, 1), n(7, 1, 1)],
+        "C_offaxis": [n(1, 9, 1), n(2, 9, 1), n(2, 10, 1), n(3, 10, 1), n(4, 10, 1)],
+        "C_turns": [n(4, 4, 1), n(5, 4, 1), n(5, 4, 2), n(5, 5, 2), n(5, 5, 3), n(6, 5, 3), n(7, 5, 3),
+                    n(7, 5, 2), n(7, 4, 2), n(7, 3, 2)],
+        "C_single_hop": [n(12, 12, 1), n(13, 12, 1)],
+        "C_via_only": [n(12, 14, 0), n(12, 14, 5)],
+    }
+    paths.update(crafted)
+    router.net_paths = dict(paths)
+    tracks, vias = router._generate_geometry_from_paths()
+    geom = lat.geom
+    out = {"X": X, "Y": Y, "L": L, "pitch": float(geom.pitch), "grid_min_x": float(geom.grid_min_x),
+           "grid_min_y": float(geom.grid_min_y), "layer_names": list(router.config.layer_names),
+           "grid_pitch": float(router.config.grid_pitch), "paths": paths, "tracks": tracks, "vias": vias}
+    with open(os.path.join(HERE, "geometry.json"), "w") as f:
+        json.dump(out, f, indent=0)
+    return len(tracks), len(vias)
+
+
 def synthetic_board_spec(seed=11):
     """A small board as plain data: components with pads (some without a
     component id, one with a drill), nets over them, one unconnected pad."""
@@ -546,6 +601,8 @@ def main():
         r, ti, tr = gen_trajectory(upf, Board, Net, Pad, Coordinate, X, Y, L, reqs, 30, "trajectory_cfg1")
         print("cfg1", {k: r.get(k) for k in ("success", "overuse_sum", "failed_nets")},
               f"init {ti:.1f} s, route {tr:.1f} s")
+    if "geometry" in which:
+        print("geometry", gen_geometry(upf, Board, Net, Pad, Coordinate))
     if "negotiation2" in which:
         r = gen_negotiation(upf, Board, Net, Pad, Coordinate, X=16, Y=14, L=4, n_nets=45,
                             seed=9, cap=25, name="negotiation_l4")

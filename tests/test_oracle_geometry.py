@@ -1,5 +1,7 @@
 """Geometry emission oracle (oracle.geometry, UPF:5596-5675, 5749-5830)
 against hand-worked answers and invariants of the track/via records."""
+import os
+
 import numpy as np
 
 from oracle import geometry as G
@@ -89,3 +91,18 @@ def test_invariants_on_lattice_walks():
                 da = (np.sign(a["x2"] - a["x1"]), np.sign(a["y2"] - a["y1"]))
                 db = (np.sign(b["x2"] - b["x1"]), np.sign(b["y2"] - b["y1"]))
                 assert da != db
+
+
+def test_geometry_matches_reference_records(golden_dir):
+    """The reference's own _generate_geometry_from_paths output
+    (tests/golden/geometry.json): the reference-routed final paths of a
+    24x22x6 board and crafted paths through every special case of the walk."""
+    import json
+    with open(os.path.join(golden_dir, "geometry.json")) as f:
+        d = json.load(f)
+    t, v = G.generate(d["paths"], d["X"], d["Y"], d["L"], d["layer_names"], d["pitch"], d["grid_min_x"],
+                      d["grid_min_y"], guard_outer=True)
+    for e in t:
+        e["width"] = d["grid_pitch"] * 0.6
+    assert t == d["tracks"]
+    assert v == d["vias"]
