@@ -267,18 +267,23 @@ int orgpu_route_ref_noalloc(int device, void *hip_stream, const orgpu_graph *g, 
 /* ---- A2 ROI extraction -------------------------------------------------------- */
 
 /* ROIExtractor.extract_roi_bfs (UPF:1636-1734) plus the must-keep re-add of
- * _route_all (UPF:4683-4700): bidirectional level-synchronous BFS from src
+ * _route_all (UPF:4659-4700): bidirectional level-synchronous BFS from src
  * (wave 1) and dst (wave 2) until max_depth, both frontiers empty, the waves
- * meeting, or more than 1.5*max_nodes nodes seen; then the reference's
- * truncation to max_nodes (lowest node ids first, must-keep swaps) and src/dst
- * re-added.  The ROI is written as bitmap row roi_index (bit set = in ROI). */
+ * meeting, or more than 1.5*max_nodes nodes seen; then the reference's list
+ * rules: roi_nodes = the seen nodes ascending, must-keeps it lacks appended
+ * (src, dst, then keep[0], keep[1] -- the portal seeds of a portal net,
+ * UPF:1700-1708), truncation to max_nodes with the must-keeps beyond the cut
+ * swapped to the front (UPF:1711-1727); with readd, the keeps and then src
+ * and dst appended when missing (UPF:4659-4696).  The ROI is written as
+ * bitmap row roi_index (bit set = in ROI). */
 typedef struct orgpu_roi_req {
     int32_t src, dst;
     int32_t max_depth;
     int32_t roi_index;
-    int64_t max_nodes;
-    int32_t readd;    /* 1: re-add src/dst (main search); 0: as extract_roi returns it (fallback) */
+    int64_t max_nodes;  /* >= 1 */
+    int32_t readd;      /* 1: re-add keeps, src, dst (main search); 0: as extract_roi returns it (fallback) */
     int32_t reserved;
+    int32_t keep[2];    /* further must-keep nodes (-1: none) */
 } orgpu_roi_req;
 
 typedef struct orgpu_roi_info {
@@ -290,13 +295,13 @@ typedef struct orgpu_roi_info {
     int32_t dst_in;   /* dst in the final ROI */
     /* Where the reference's roi_nodes list holds nodes out of ascending order
      * (its position order breaks heap ties, UPF:1841-1875): front[p] is the
-     * must-keep that the truncation swapped into position p (p = 0, 1), which
-     * node left[p] held before (-1, -1: no swap at p, UPF:1716-1727);
-     * appended[j] is the j-th node the re-add appended after the sorted
-     * list (-1: none, UPF:4689-4696). */
-    int32_t front[2];
-    int32_t left[2];
-    int32_t appended[2];
+     * must-keep that the truncation swapped into position p (p = 0..3),
+     * which node left[p] held before (-1, -1: no swap at p, UPF:1716-1727);
+     * appended[j] is the j-th node kept after the sorted part (must-keeps
+     * appended before the truncation, then the re-adds; -1: none). */
+    int32_t front[4];
+    int32_t left[4];
+    int32_t appended[4];
 } orgpu_roi_info;
 
 /* reqs: host array; roi_bitmaps: device u32[n_roi][ceil(N/32)], rows written

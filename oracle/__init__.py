@@ -79,7 +79,8 @@ def lib():
         _lib.orc_ms_canon.restype = i64
         _lib.orc_ms_canon.argtypes = [i64, P, P, P, P, i64, P, P, i64, P, P, P, P, P, P, P]
         _lib.orc_roi_bfs.restype = i64
-        _lib.orc_roi_bfs.argtypes = [i64, P, P, i32, i32, ctypes.c_int, i64, P, P, P, P]
+        _lib.orc_roi_bfs.argtypes = [i64, P, P, i32, i32, ctypes.c_int, i64, P, ctypes.c_int, ctypes.c_int,
+                                     P, P, P, P]
     return _lib
 
 
@@ -237,14 +238,19 @@ def roi_radius(lat, src, dst, stagnation_bonus=0.0):
     return radius, min(int(radius + stagnation_bonus * 2.0), 150)
 
 
-def roi_bfs(lat, src, dst, stagnation_bonus=0.0, max_nodes=50_000):
-    """ROI node list in the reference's order (extract_roi -> extract_roi_bfs)."""
+def roi_bfs(lat, src, dst, stagnation_bonus=0.0, max_nodes=50_000, keep=(), readd=False):
+    """ROI node list in the reference's order (extract_roi -> extract_roi_bfs,
+    UPF:1619-1734): keep = further must-keep nodes (a portal net's seeds);
+    readd: then the keeps, src and dst appended when missing, as _route_all
+    does for the main search (UPF:4659-4696)."""
     _, depth = roi_radius(lat, src, dst, stagnation_bonus)
     out = np.empty(lat.N, dtype=np.int32)
     seen = np.empty(lat.N, dtype=np.uint8)
     d, m = ctypes.c_int(0), ctypes.c_int(0)
+    kp = np.ascontiguousarray([int(k) for k in keep], dtype=np.int32) if len(keep) else np.zeros(1, np.int32)
     R = lib().orc_roi_bfs(lat.N, _p(lat.indptr), _p(lat.indices), int(src), int(dst), int(depth),
-                          int(max_nodes), _p(out), _p(seen), ctypes.byref(d), ctypes.byref(m))
+                          int(max_nodes), _p(kp), len(keep), int(bool(readd)), _p(out), _p(seen), ctypes.byref(d),
+                          ctypes.byref(m))
     if R < 0:
         raise MemoryError("oracle roi failed")
     return out[:R].copy()

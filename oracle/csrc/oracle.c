@@ -536,6 +536,7 @@ int64_t orc_ms_canon(int64_t N, const int32_t *indptr, const int32_t *indices, c
  */
 int64_t orc_roi_bfs(int64_t N, const int32_t *indptr, const int32_t *indices,
                     int32_t src, int32_t dst, int max_depth, int64_t max_nodes,
+                    const int32_t *keep, int nkeep, int readd,
                     int32_t *roi_out, uint8_t *seen_out, int *depth_out,
                     int *met_out)
 {
@@ -579,28 +580,56 @@ int64_t orc_roi_bfs(int64_t N, const int32_t *indptr, const int32_t *indices,
         depth++;
         if ((double)nseen > (double)max_nodes * 1.5) break;
     }
+    /* roi_nodes = the seen nodes ascending; must-keeps it lacks appended
+     * (src, dst, then the portal seeds not yet listed, UPF:1700-1708) */
     int64_t R = 0;
     for (int64_t n = 0; n < N; ++n)
         if (seen[n]) roi_out[R++] = (int32_t)n;
-    /* must-keeps are always seen (marked at start), so nothing is appended */
-    if (R > max_nodes) {
-        int32_t keep[2] = {src, dst};
-        int nkeep = (src == dst) ? 1 : 2;
-        int64_t kept_idx[2];
-        int nk = 0;
-        for (int64_t i = 0; i < R; ++i) {
-            int32_t node = roi_out[i];
-            for (int j = 0; j < nkeep; ++j)
-                if (node == keep[j]) { kept_idx[nk++] = i; break; }
+    int32_t mk[8];
+    int nmk = 0;
+    {
+        int32_t cand[8];
+        int nc = 0;
+        cand[nc++] = src; cand[nc++] = dst;
+        for (int i = 0; i < nkeep && nc < 8; ++i) cand[nc++] = keep[i];
+        for (int i = 0; i < nc; ++i) {
+            int dup = cand[i] < 0;
+            for (int j = 0; j < nmk; ++j) dup |= mk[j] == cand[i];
+            if (!dup) mk[nmk++] = cand[i];
         }
-        for (int s = 0; s < nk; ++s) {
-            if (kept_idx[s] >= max_nodes) {
-                int32_t t = roi_out[kept_idx[s]];
-                roi_out[kept_idx[s]] = roi_out[s];
-                roi_out[s] = t;
+    }
+    for (int j = 0; j < nmk; ++j)
+        if (!seen[mk[j]]) roi_out[R++] = mk[j];
+    /* truncation: the must-keeps, in list order, each beyond the cut trades
+     * places with position (its rank among them) (UPF:1711-1727) */
+    if (R > max_nodes) {
+        int64_t kept_idx[8];
+        int nk = 0;
+        for (int64_t i = 0; i < R && nk < 8; ++i)
+            for (int j = 0; j < nmk; ++j)
+                if (roi_out[i] == mk[j]) { kept_idx[nk++] = i; break; }
+        for (int t = 0; t < nk; ++t) {
+            if (kept_idx[t] >= max_nodes) {
+                int32_t x = roi_out[kept_idx[t]];
+                roi_out[kept_idx[t]] = roi_out[t];
+                roi_out[t] = x;
             }
         }
         R = max_nodes;
+    }
+    /* _route_all: portal seeds, then src and dst, appended when missing
+     * (UPF:4659-4696) */
+    if (readd) {
+        int32_t ra[8];
+        int nr = 0;
+        for (int i = 0; i < nkeep && nr < 6; ++i) ra[nr++] = keep[i];
+        ra[nr++] = src; ra[nr++] = dst;
+        for (int i = 0; i < nr; ++i) {
+            if (ra[i] < 0) continue;
+            int have = 0;
+            for (int64_t k = 0; k < R && !have; ++k) have = roi_out[k] == ra[i];
+            if (!have) roi_out[R++] = ra[i];
+        }
     }
     if (seen_out) memcpy(seen_out, seen, (size_t)N);
     if (depth_out) *depth_out = depth;

@@ -83,13 +83,13 @@ class SsspStats(ctypes.Structure):
 class RoiReq(ctypes.Structure):
     _fields_ = [("src", ctypes.c_int32), ("dst", ctypes.c_int32), ("max_depth", ctypes.c_int32),
                 ("roi_index", ctypes.c_int32), ("max_nodes", ctypes.c_int64), ("readd", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("reserved", ctypes.c_int32), ("keep", ctypes.c_int32 * 2)]
 
 
 class RoiInfo(ctypes.Structure):
     _fields_ = [("seen", ctypes.c_int64), ("kept", ctypes.c_int64), ("depth", ctypes.c_int32),
                 ("met", ctypes.c_int32), ("src_in", ctypes.c_int32), ("dst_in", ctypes.c_int32),
-                ("front", ctypes.c_int32 * 2), ("left", ctypes.c_int32 * 2), ("appended", ctypes.c_int32 * 2)]
+                ("front", ctypes.c_int32 * 4), ("left", ctypes.c_int32 * 4), ("appended", ctypes.c_int32 * 4)]
 
 
 class RefReq(ctypes.Structure):

@@ -11,8 +11,9 @@
 // never observed.
 //
 // Truncation keeps the max_nodes lowest node ids (the reference sorts the ROI
-// ascending), with src / dst beyond the cut swapped into positions 0 / 1 (the
-// ids there leave), then src and dst are re-added when the caller asks.
+// ascending), with the must-keeps (src, dst and up to two portal seeds) beyond
+// the cut swapped into the list's first positions (the ids there leave); the
+// caller's re-add then puts back the seeds, src and dst when asked.
 #include <algorithm>
 #include <vector>
 
@@ -63,27 +64,56 @@ __device__ __forceinline__ int nth_set_bit(uint32_t w, int r)
     return pos;
 }
 
-// Truncation and the must-keep re-add (UPF:1700-1734, 4683-4700) on a
-// finished ROI row A holding `seen` marked nodes, then the request's info
-// record.  Called by every thread of the workgroup.
+// The reference's list rules on a finished ROI row A holding `seen` marked
+// nodes (UPF:1700-1734, 4659-4700), then the request's info record.  The
+// list is the seen nodes ascending, the must-keeps it lacks appended; when it
+// is longer than max_nodes, each must-keep beyond the cut trades places with
+// the list's position p (p = its rank among the must-keeps by position), so
+// it stays and the node there leaves, and the list is cut.  With readd the
+// keeps, then src and dst, are appended when missing.  Called by every
+// thread of the workgroup.
 struct RoiShared {
     int wsum[RWG / 64];
-    int cut, min0, min1, kept_delta;
+    int cut;        // node at sorted position max_nodes - 1 (the last one kept)
+    int mins[4];    // nodes at sorted positions 0..3
 };
+
+__device__ __forceinline__ bool row_has(const uint32_t *A, int32_t v) { return (A[v >> 5] >> (v & 31)) & 1u; }
+__device__ __forceinline__ void row_set(uint32_t *A, int32_t v) { A[v >> 5] |= 1u << (v & 31); }
+__device__ __forceinline__ void row_clear(uint32_t *A, int32_t v) { A[v >> 5] &= ~(1u << (v & 31)); }
 
 __device__ void roi_finish(uint32_t *A, int64_t words, const orgpu_roi_req &q, long long seen, int depth, int met,
                            orgpu_roi_info *info, RoiShared &sh)
 {
     const int tid = threadIdx.x;
-    long long kept = seen;
-    int32_t front[2] = {-1, -1}, left[2] = {-1, -1};
-    if (seen > q.max_nodes) {
+    // must-keeps in the reference's order: src, dst, then the keeps not yet listed
+    int32_t mk[4];
+    int nmk = 0;
+    {
+        const int32_t cand[4] = {q.src, q.dst, q.keep[0], q.keep[1]};
+        for (int i = 0; i < 4; ++i) {
+            const int32_t v = cand[i];
+            bool dup = v < 0;
+            for (int j = 0; j < nmk; ++j) dup |= mk[j] == v;
+            if (!dup) mk[nmk++] = v;
+        }
+    }
+    bool mseen[4] = {false, false, false, false};
+    int nmiss = 0;
+    for (int j = 0; j < nmk; ++j) {
+        mseen[j] = row_has(A, mk[j]);
+        nmiss += !mseen[j];
+    }
+    __syncthreads();   // every thread has read the must-keep bits before any change
+    const long long R = seen, total = seen + nmiss;
+    const bool cut_seen = R > q.max_nodes;
+    if (total > q.max_nodes) {
         // Rank the seen nodes by id.  Each wave owns a contiguous chunk of
         // the row and its lanes read consecutive words.  Pass 1 counts the
         // chunk's bits; the prefix of the chunk totals gives each chunk's
-        // first rank; pass 2, in the chunks that hold rank 1, 2 or max_nodes
-        // (the two lowest ids: positions 0 and 1 of the sorted ROI, and the
-        // cut), scans again with a running prefix to locate those bits.
+        // first rank; pass 2, in the chunks that hold a wanted rank (1-4:
+        // the list's first positions, and max_nodes: the cut), scans again
+        // with a running prefix to locate those bits.
         const int wid = tid >> 6, lane = tid & 63;
         const int64_t per = (words + RWG / 64 - 1) / (RWG / 64);
         const int64_t c0 = std::min<int64_t>((int64_t)wid * per, words);
@@ -92,13 +122,13 @@ __device__ void roi_finish(uint32_t *A, int64_t words, const orgpu_roi_req &q, l
         for (int64_t w = c0 + lane; w < c1; w += 64) c += __popc(A[w]);
         const int tot = __shfl(wave_scan_incl(c), 63, 64);
         if (lane == 0) sh.wsum[wid] = tot;
-        if (tid == 0) { sh.cut = -1; sh.min0 = 0x7FFFFFFF; sh.min1 = 0x7FFFFFFF; sh.kept_delta = 0; }
+        if (tid == 0) { sh.cut = -1; for (int k = 0; k < 4; ++k) sh.mins[k] = -1; }
         __syncthreads();
         long long base = 0;
         for (int j = 0; j < wid; ++j) base += sh.wsum[j];
-        const long long targets[3] = {(long long)q.max_nodes, 1, 2};
+        const long long targets[5] = {cut_seen ? (long long)q.max_nodes : -1, 1, 2, 3, 4};
         bool mine = false;
-        for (int k = 0; k < 3; ++k) mine |= base < targets[k] && targets[k] <= base + tot;
+        for (int k = 0; k < 5; ++k) mine |= base < targets[k] && targets[k] <= base + tot;
         if (mine) {
             long long run = base;
             for (int64_t s0 = c0; s0 < c1 && run < base + tot; s0 += 64) {
@@ -107,59 +137,79 @@ __device__ void roi_finish(uint32_t *A, int64_t words, const orgpu_roi_req &q, l
                 const int pc = __popc(x);
                 const int incl = wave_scan_incl(pc);
                 const long long lo = run + incl - pc;   // seen nodes before this lane's word
-                for (int k = 0; k < 3; ++k) {
+                for (int k = 0; k < 5; ++k) {
                     const long long r = targets[k];
                     if (lo < r && r <= lo + pc) {
                         const int32_t id = (int32_t)(w * 32 + nth_set_bit(x, (int)(r - lo) - 1));
                         if (k == 0) sh.cut = id;
-                        else if (k == 1) sh.min0 = id;
-                        else sh.min1 = id;
+                        else sh.mins[k - 1] = id;
                     }
                 }
                 run += __shfl(incl, 63, 64);
             }
         }
         __syncthreads();
-        const int32_t cut = sh.cut;
-        // the bits above the cut leave the row (the words from the cut's on)
-        for (int64_t w = std::max<int64_t>(0, (int64_t)cut >> 5) + tid; w < words; w += RWG) {
-            const int64_t keep_bits = (int64_t)cut - w * 32 + 1;   // ids w*32 .. cut stay
-            const uint32_t keep = keep_bits <= 0 ? 0u : keep_bits >= 32 ? ~0u : (1u << keep_bits) - 1u;
-            const uint32_t x = A[w];
-            if (x & ~keep) A[w] = x & keep;
+        if (cut_seen) {
+            // the bits above the cut leave the row (the words from the cut's on)
+            const int32_t cut = sh.cut;
+            for (int64_t w = std::max<int64_t>(0, (int64_t)cut >> 5) + tid; w < words; w += RWG) {
+                const int64_t keep_bits = (int64_t)cut - w * 32 + 1;   // ids w*32 .. cut stay
+                const uint32_t keep = keep_bits <= 0 ? 0u : keep_bits >= 32 ? ~0u : (1u << keep_bits) - 1u;
+                const uint32_t x = A[w];
+                if (x & ~keep) A[w] = x & keep;
+            }
         }
         __syncthreads();
-        if (tid == 0) {
-            // must-keep swaps (UPF:1716-1727): a must-keep beyond the cut
-            // trades places with position 0 (the lower of src/dst) or 1 (the
-            // higher) of the ascending ROI, so it stays and the lowest /
-            // second-lowest id leaves (the second swap only when src != dst
-            // and position 1 is inside the cut); the count stays max_nodes
-            const int32_t k0 = min(q.src, q.dst), k1 = max(q.src, q.dst);
-            if (k0 > cut) {
-                A[sh.min0 >> 5] &= ~(1u << (sh.min0 & 31));
-                A[k0 >> 5] |= 1u << (k0 & 31);
-                front[0] = k0; left[0] = sh.min0;
-            }
-            if (q.src != q.dst && k1 > cut && q.max_nodes >= 2) {
-                A[sh.min1 >> 5] &= ~(1u << (sh.min1 & 31));
-                A[k1 >> 5] |= 1u << (k1 & 31);
-                front[1] = k1; left[1] = sh.min1;
-            }
-            sh.kept_delta = 0;
-        }
-        __syncthreads();
-        kept = q.max_nodes + sh.kept_delta;
     }
     if (tid == 0) {
-        // _route_all re-adds src and dst when missing, appended in that order
-        const int32_t ks[2] = {q.src, q.dst};
-        int32_t app[2] = {-1, -1};
+        int32_t front[4] = {-1, -1, -1, -1}, left[4] = {-1, -1, -1, -1}, app[4] = {-1, -1, -1, -1};
         int na = 0;
+        // the must-keeps in list order: the seen ones by id, then the missing
+        // ones at positions R, R+1, ...; `beyond` = past the cut
+        int32_t kn[4], miss[4];
+        bool beyond[4];
+        int nk = 0, nm = 0;
+        for (int j = 0; j < nmk; ++j) {
+            if (!mseen[j]) continue;
+            int at = nk++;
+            while (at > 0 && kn[at - 1] > mk[j]) { kn[at] = kn[at - 1]; beyond[at] = beyond[at - 1]; --at; }
+            kn[at] = mk[j];
+            beyond[at] = cut_seen && mk[j] > sh.cut;
+        }
+        long long pos = R;
+        for (int j = 0; j < nmk; ++j) {
+            if (mseen[j]) continue;
+            kn[nk] = mk[j];
+            miss[nm++] = mk[j];
+            beyond[nk] = pos >= q.max_nodes;
+            if (!beyond[nk]) { row_set(A, mk[j]); if (na < 4) app[na++] = mk[j]; }
+            ++nk; ++pos;
+        }
+        long long kept = total;
+        if (total > q.max_nodes) {
+            for (int s2 = 0; s2 < nk; ++s2) {
+                // a must-keep past the cut trades places with position s2; when
+                // s2 itself is past the cut both stay out
+                if (!beyond[s2] || s2 >= q.max_nodes) continue;
+                // position s2 holds the s2-th lowest seen id, or an appended
+                // must-keep when the seen part is shorter
+                const int32_t out = s2 < R ? sh.mins[s2] : miss[s2 - R];
+                if (out >= 0) row_clear(A, out);
+                row_set(A, kn[s2]);
+                front[s2] = kn[s2];
+                left[s2] = out;
+            }
+            kept = q.max_nodes;
+        }
         if (q.readd) {
-            for (int j = 0; j < 2; ++j) {
-                const int32_t v = ks[j];
-                if (!((A[v >> 5] >> (v & 31)) & 1u)) { A[v >> 5] |= 1u << (v & 31); kept++; app[na++] = v; }
+            // _route_all appends the portal seeds, then src and dst, when missing
+            const int32_t ra[4] = {q.keep[0], q.keep[1], q.src, q.dst};
+            for (int j = 0; j < 4; ++j) {
+                const int32_t v = ra[j];
+                if (v < 0 || row_has(A, v)) continue;
+                row_set(A, v);
+                kept++;
+                if (na < 4) app[na++] = v;
             }
         }
         if (info) {
@@ -167,9 +217,9 @@ __device__ void roi_finish(uint32_t *A, int64_t words, const orgpu_roi_req &q, l
             info->kept = kept;
             info->depth = depth;
             info->met = met;
-            info->src_in = (A[q.src >> 5] >> (q.src & 31)) & 1u;
-            info->dst_in = (A[q.dst >> 5] >> (q.dst & 31)) & 1u;
-            for (int j = 0; j < 2; ++j) {
+            info->src_in = row_has(A, q.src);
+            info->dst_in = row_has(A, q.dst);
+            for (int j = 0; j < 4; ++j) {
                 info->front[j] = front[j];
                 info->left[j] = left[j];
                 info->appended[j] = app[j];
@@ -533,9 +583,10 @@ extern "C" int orgpu_roi_bfs(orgpu_t *h, const orgpu_graph *g, const orgpu_roi_r
     for (int i = 0; i < n; ++i) {
         const orgpu_roi_req &q = reqs[i];
         if (q.src < 0 || q.src >= N || q.dst < 0 || q.dst >= N || q.roi_index < 0 || q.roi_index >= n_roi ||
-            q.max_nodes < 1 || q.max_depth < 0)
-            return orgpu_fail(ORGPU_EINVAL, "roi_bfs: request %d invalid (src %d dst %d row %d of %d)", i, q.src, q.dst,
-                              q.roi_index, n_roi);
+            q.max_nodes < 1 || q.max_depth < 0 || q.keep[0] < -1 || q.keep[0] >= N || q.keep[1] < -1 || q.keep[1] >= N)
+            return orgpu_fail(ORGPU_EINVAL, "roi_bfs: request %d invalid (src %d dst %d keeps %d %d row %d of %d, "
+                              "max_nodes %lld)", i, q.src, q.dst, q.keep[0], q.keep[1], q.roi_index, n_roi,
+                              (long long)q.max_nodes);
         if (row_user[q.roi_index] >= 0)
             return orgpu_fail(ORGPU_EINVAL, "roi_bfs: requests %d and %d share bitmap row %d", row_user[q.roi_index], i,
                               q.roi_index);
@@ -634,9 +685,10 @@ extern "C" int orgpu_roi_lattice(orgpu_t *h, const orgpu_lattice_desc *d, const 
     for (int i = 0; i < n; ++i) {
         const orgpu_roi_req &q = reqs[i];
         if (q.src < 0 || q.src >= N || q.dst < 0 || q.dst >= N || q.roi_index < 0 || q.roi_index >= n_roi ||
-            q.max_nodes < 1 || q.max_depth < 0)
-            return orgpu_fail(ORGPU_EINVAL, "roi_lattice: request %d invalid (src %d dst %d row %d of %d)", i, q.src,
-                              q.dst, q.roi_index, n_roi);
+            q.max_nodes < 1 || q.max_depth < 0 || q.keep[0] < -1 || q.keep[0] >= N || q.keep[1] < -1 || q.keep[1] >= N)
+            return orgpu_fail(ORGPU_EINVAL, "roi_lattice: request %d invalid (src %d dst %d keeps %d %d row %d of %d, "
+                              "max_nodes %lld)", i, q.src, q.dst, q.keep[0], q.keep[1], q.roi_index, n_roi,
+                              (long long)q.max_nodes);
         if (q.max_depth > ROI_MAXD)
             return orgpu_fail(ORGPU_EINVAL, "roi_lattice: request %d depth %d above %d (use orgpu_roi_bfs)", i,
                               q.max_depth, ROI_MAXD);

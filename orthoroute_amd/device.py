@@ -291,12 +291,12 @@ def route_flat(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner
 
 MS_POINT = np.dtype([("node", "<i4"), ("cost", "<f4")])
 # orgpu_roi_req / orgpu_roi_info rows (the ctypes layouts of _abi.RoiReq / RoiInfo)
-ROI_REQ = np.dtype({"names": ["src", "dst", "max_depth", "roi_index", "max_nodes", "readd", "reserved"],
-                    "formats": ["<i4", "<i4", "<i4", "<i4", "<i8", "<i4", "<i4"],
-                    "offsets": [0, 4, 8, 12, 16, 24, 28], "itemsize": 32})
+ROI_REQ = np.dtype({"names": ["src", "dst", "max_depth", "roi_index", "max_nodes", "readd", "reserved", "keep"],
+                    "formats": ["<i4", "<i4", "<i4", "<i4", "<i8", "<i4", "<i4", ("<i4", 2)],
+                    "offsets": [0, 4, 8, 12, 16, 24, 28, 32], "itemsize": 40})
 ROI_INFO = np.dtype({"names": ["seen", "kept", "depth", "met", "src_in", "dst_in", "front", "left", "appended"],
-                     "formats": ["<i8", "<i8", "<i4", "<i4", "<i4", "<i4", ("<i4", 2), ("<i4", 2), ("<i4", 2)],
-                     "offsets": [0, 8, 16, 20, 24, 28, 32, 40, 48], "itemsize": 56})
+                     "formats": ["<i8", "<i8", "<i4", "<i4", "<i4", "<i4", ("<i4", 4), ("<i4", 4), ("<i4", 4)],
+                     "offsets": [0, 8, 16, 20, 24, 28, 32, 48, 64], "itemsize": 80})
 # orgpu_ref_req rows
 REF_REQ = np.dtype({"names": ["src", "dst", "net_id", "roi_index", "use_keepout", "n_rank", "rank_node", "rank_key"],
                     "formats": ["<i4", "<i4", "<i4", "<i4", "<i4", "<i4", ("<i4", 4), ("<u4", 4)],
@@ -376,9 +376,11 @@ def route_ms(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=N
 ROI_LATTICE_MAX_DEPTH = 1024
 
 
-def roi_bfs(ctx, lat, requests, rows, readd=True, method="auto", raw=False):
+def roi_bfs(ctx, lat, requests, rows, readd=True, method="auto", raw=False, keeps=None):
     """Batched ROI extraction.  requests: list of (src, dst, max_depth,
     max_nodes, row); rows: device u32[n_roi][ceil(N/32)] written in place.
+    keeps: per request up to two further must-keep nodes (a portal net's
+    seeds), an (n, 2) array with -1 for none.
     readd: re-add src/dst after truncation (the main search of _route_all) or
     not (its wider-ROI fallback).  method: "bfs" traverses the CSR;
     "lattice" computes the same ROI in closed form from the lattice shape
@@ -394,6 +396,7 @@ def roi_bfs(ctx, lat, requests, rows, readd=True, method="auto", raw=False):
     reqs["src"], reqs["dst"], reqs["max_depth"] = q[:, 0], q[:, 1], q[:, 2]
     reqs["max_nodes"], reqs["roi_index"] = q[:, 3], q[:, 4]
     reqs["readd"] = int(bool(readd))
+    reqs["keep"] = -1 if keeps is None else np.asarray(keeps, dtype=np.int64).reshape(n, 2)
     info = np.zeros(n, dtype=ROI_INFO)
     vp = ctypes.c_void_p
     closed = getattr(lat, "desc", None) is not None and int(q[:, 2].max()) <= ROI_LATTICE_MAX_DEPTH
@@ -420,10 +423,10 @@ def rank_exceptions(info, n_nodes):
     the node that left it, and the j-th appended node comes after every
     other (2*N + 2 + j)."""
     out = []
-    for p in range(2):
+    for p in range(4):
         if info["front"][p] >= 0:
             out.append((int(info["front"][p]), 2 * int(info["left"][p]) + 2))
-    for j in range(2):
+    for j in range(4):
         if info["appended"][j] >= 0:
             out.append((int(info["appended"][j]), 2 * int(n_nodes) + 2 + j))
     return out
@@ -441,7 +444,8 @@ def ref_requests(requests, ranks=None):
             if not ex:
                 continue
             if len(ex) > 4:
-                raise ValueError("at most four rank exceptions per request")
+                raise ValueError("at most four rank exceptions per request (a two-pin search's ROI has at most "
+                                 "src and dst out of order)")
             rows["n_rank"][i] = len(ex)
             for k, (v, key) in enumerate(ex):
                 rows["rank_node"][i, k] = v

@@ -93,6 +93,46 @@ def gen_roi(upf):
         roi=np.concatenate([c[4] for c in cases]))
 
 
+def gen_roi_keep(upf):
+    """extract_roi with portal seeds as must-keeps (UPF:1619-1734): seeds next
+    to the endpoints, far outside the BFS ball, equal to an endpoint; cuts
+    small enough that the must-keeps beyond them swap to the front."""
+    lat, g = make_lattice(upf, 30, 24, 6)
+    rng = np.random.default_rng(13)
+    plane = 30 * 24
+    cases = []
+    for i in range(24):
+        sx, sy, dx, dy = (int(v) for v in rng.integers(0, 24, 4))
+        src = int(int(rng.integers(1, 5)) * plane + sy * 30 + sx)
+        dst = int(int(rng.integers(1, 5)) * plane + dy * 30 + dx)
+        if src == dst:
+            continue
+        kind = i % 4
+        if kind == 0:      # portal seeds on F.Cu under the endpoints
+            seeds = [src % plane, dst % plane]
+        elif kind == 1:    # one far away (outside any small ball), one at the far corner
+            seeds = [int(rng.integers(0, 30 * 24 * 6)), 5 * plane + 23 * 30 + 29]
+        elif kind == 2:    # a seed equal to an endpoint, one duplicate
+            seeds = [src, int(rng.integers(0, plane))]
+            seeds.append(seeds[1])
+        else:
+            seeds = [int(rng.integers(0, 30 * 24 * 6)) for _ in range(2)]
+        bonus = [0.0, 0.6, 3.0][i % 3]
+        max_nodes = [300, 1000, 50_000, 150][i % 4]
+        ext = upf.ROIExtractor(g, lattice=lat)
+        ext.max_roi_nodes = max_nodes
+        roi, g2r = ext.extract_roi(src, dst, initial_radius=40, stagnation_bonus=bonus,
+                                   portal_seeds=[(int(n), 0.0) for n in seeds])
+        cases.append((src, dst, bonus, max_nodes, seeds, np.asarray(roi, dtype=np.int64)))
+    np.savez_compressed(
+        os.path.join(HERE, "roi_keep.npz"), X=30, Y=24, L=6,
+        src=np.array([c[0] for c in cases]), dst=np.array([c[1] for c in cases]),
+        bonus=np.array([c[2] for c in cases]), max_nodes=np.array([c[3] for c in cases]),
+        seed_len=np.array([len(c[4]) for c in cases]), seeds=np.concatenate([c[4] for c in cases]),
+        roi_len=np.array([len(c[5]) for c in cases]), roi=np.concatenate([c[5] for c in cases]))
+    return len(cases)
+
+
 def gen_dijkstra(upf):
     lat, g = make_lattice(upf, 20, 16, 6)
     E = len(g.indices)
@@ -601,6 +641,8 @@ def main():
         r, ti, tr = gen_trajectory(upf, Board, Net, Pad, Coordinate, X, Y, L, reqs, 30, "trajectory_cfg1")
         print("cfg1", {k: r.get(k) for k in ("success", "overuse_sum", "failed_nets")},
               f"init {ti:.1f} s, route {tr:.1f} s")
+    if "roi_keep" in which:
+        print("roi_keep", gen_roi_keep(upf))
     if "geometry" in which:
         print("geometry", gen_geometry(upf, Board, Net, Pad, Coordinate))
     if "negotiation2" in which:

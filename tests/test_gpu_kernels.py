@@ -80,7 +80,7 @@ def test_roi_random_truncation(ctx, method):
         rows_h = rows.cpu().numpy().view(np.uint32)
         for i, (s, d, depth, mx, _) in enumerate(reqs):
             out = np.empty(o.N, np.int32)
-            R = oracle.lib().orc_roi_bfs(o.N, oracle._p(o.indptr), oracle._p(o.indices), s, d, depth, mx,
+            R = oracle.lib().orc_roi_bfs(o.N, oracle._p(o.indptr), oracle._p(o.indices), s, d, depth, mx, None, 0, 0,
                                          oracle._p(out), None, None, None)
             ref = set(out[:R].tolist()) | {s, d}
             got = set(np.nonzero(unpack(rows_h[i], lat.N))[0].tolist())
@@ -123,7 +123,7 @@ def test_roi_lattice_equals_bfs_and_oracle(ctx):
                     out = np.empty(o.N, np.int32)
                     seen = np.empty(o.N, np.uint8)
                     dd, mm = ctypes.c_int(0), ctypes.c_int(0)
-                    R = oracle.lib().orc_roi_bfs(o.N, oracle._p(o.indptr), oracle._p(o.indices), s, d, depth, mx,
+                    R = oracle.lib().orc_roi_bfs(o.N, oracle._p(o.indptr), oracle._p(o.indices), s, d, depth, mx, None, 0, 0,
                                                  oracle._p(out), oracle._p(seen), ctypes.byref(dd), ctypes.byref(mm))
                     ref = set(out[:R].tolist()) | ({s, d} if readd else set())
                     got = set(np.nonzero(unpack(h2[i], lat.N))[0].tolist())
@@ -239,3 +239,50 @@ def test_via_pooling_and_hard_block_match_oracle(ctx):
             st.update_costs(p)
             got = st.total_cost.cpu().numpy()[:E]
             np.testing.assert_array_equal(got.view(np.uint32), exp.view(np.uint32))
+
+
+@pytest.mark.parametrize("method", ["lattice", "bfs"])
+@pytest.mark.parametrize("readd", [False, True])
+def test_roi_must_keeps_match_oracle(method, readd):
+    """Portal seeds as must-keeps on both ROI kernels: the row equals the
+    oracle list's set, and the front swaps and appended nodes it reports are
+    the oracle list's own (truncations at 150-1,000 nodes, seeds outside the
+    ball, seeds equal to an endpoint)."""
+    import torch
+    from orthoroute_amd.device import Context, roi_bfs
+    ctx = Context(0)
+    X, Y, L = 30, 24, 6
+    lat = ctx.build_lattice(X, Y, L, "reference")
+    o = oracle.Lattice(X, Y, L, "reference")
+    rng = np.random.default_rng(17)
+    plane = X * Y
+    reqs, keeps, lists = [], [], []
+    for j in range(60):
+        s, d = (int(v) for v in rng.integers(plane, o.N, 2))
+        kind = j % 3
+        if kind == 0:
+            kp = [s % plane, d % plane]
+        elif kind == 1:
+            kp = [int(rng.integers(0, o.N)), -1]
+        else:
+            kp = [s, int(rng.integers(0, o.N))]
+        mx = int(rng.choice([3, 5, 150, 300, 1000, 50_000]))
+        b = float(rng.choice([0.0, 0.6]))
+        _, depth = oracle.roi_radius(o, s, d, b)
+        reqs.append((s, d, depth, mx, j))
+        keeps.append(kp)
+        lists.append(oracle.roi_bfs(o, s, d, stagnation_bonus=b, max_nodes=mx, keep=[k for k in kp if k >= 0],
+                                    readd=readd))
+    rows = torch.zeros((len(reqs), (o.N + 31) // 32), dtype=torch.int32, device=ctx.device)
+    info = roi_bfs(ctx, lat, reqs, rows, readd=readd, method=method, raw=True, keeps=np.asarray(keeps))
+    h = rows.cpu().numpy()
+    for j, lst in enumerate(lists):
+        got = set(np.nonzero(unpack(h[j], o.N))[0].tolist())
+        assert got == set(lst.tolist()), f"request {j}: keeps {keeps[j]}"
+        assert int(info["kept"][j]) == len(lst)
+        for p in range(4):
+            if info["front"][j][p] >= 0:
+                assert lst[p] == info["front"][j][p], f"request {j}: front {p}"
+        app = [int(v) for v in info["appended"][j] if v >= 0]
+        # appended nodes sit after every node that stayed in sorted order
+        assert lst[len(lst) - len(app):].tolist() == app, f"request {j}: appended {app}"

@@ -252,3 +252,22 @@ def test_ms_canon_brute_force_tiny():
             for a, b in zip(path, path[1:]):
                 row = range(g.indptr[a], g.indptr[a + 1])
                 assert any(int(g.indices[e]) == b and np.float32(dist[a] + w[e]) == dist[b] for e in row)
+
+
+def test_roi_with_portal_seeds_matches_reference(golden_dir):
+    """extract_roi with portal seeds as must-keeps: the reference's lists
+    (roi_keep.npz), order included, with truncations that swap must-keeps to
+    the front and seeds the BFS never reaches."""
+    g = _golden(golden_dir, "roi_keep.npz")
+    lat = oracle.Lattice(int(g["X"]), int(g["Y"]), int(g["L"]))
+    so = ro = 0
+    for i in range(len(g["src"])):
+        k = int(g["seed_len"][i])
+        seeds = g["seeds"][so:so + k].tolist()
+        so += k
+        n = int(g["roi_len"][i])
+        ref = g["roi"][ro:ro + n]
+        ro += n
+        got = oracle.roi_bfs(lat, int(g["src"][i]), int(g["dst"][i]), stagnation_bonus=float(g["bonus"][i]),
+                             max_nodes=int(g["max_nodes"][i]), keep=seeds)
+        np.testing.assert_array_equal(got, ref, err_msg=f"case {i}")

@@ -140,7 +140,8 @@ def _oracle_bfs(lat, s, d, depth, max_nodes):
     seen = np.empty(lat.N, dtype=np.uint8)
     dd, mm = ctypes.c_int(0), ctypes.c_int(0)
     R = oracle.lib().orc_roi_bfs(lat.N, oracle._p(lat.indptr), oracle._p(lat.indices), int(s), int(d), int(depth),
-                                 int(max_nodes), oracle._p(out), oracle._p(seen), ctypes.byref(dd), ctypes.byref(mm))
+                                 int(max_nodes), None, 0, 0, oracle._p(out), oracle._p(seen), ctypes.byref(dd),
+                                 ctypes.byref(mm))
     assert R >= 0
     return seen != 0, dd.value, mm.value
 
