@@ -204,9 +204,10 @@ int orgpu_route_ms(orgpu_t *h, const orgpu_graph *g, const orgpu_ms_req *reqs, i
  * trajectory) rather than the canonical tie-break of orgpu_route.  The heap
  * holds (float64 alt, position of the node in the reference's roi_nodes
  * list); the position order is given as a key: 2*v + 2 for a node v of an
- * ascending list, with up to four exceptions (rank_node[i] has key
+ * ascending list, with up to eight exceptions (rank_node[i] has key
  * rank_key[i]) for nodes the list holds out of order (orgpu_roi_info.front /
- * appended: keys 2*left[p] + 2 and 2*N + 2 + j).  roi_index < 0: whole graph
+ * appended: keys 2*left[p] + 2 and 2*N + 2 + j; a portal net's re-added
+ * src/dst follow its appended nodes).  roi_index < 0: whole graph
  * (roi_nodes = arange(N)).  use_keepout as in orgpu_net_req (a keepout edge
  * costs keepout_cost, summed in float64 like the reference's 1e9). */
 typedef struct orgpu_ref_req {
@@ -214,9 +215,9 @@ typedef struct orgpu_ref_req {
     int32_t net_id;
     int32_t roi_index;
     int32_t use_keepout;
-    int32_t n_rank;           /* 0..4 */
-    int32_t rank_node[4];
-    uint32_t rank_key[4];
+    int32_t n_rank;           /* 0..8 */
+    int32_t rank_node[8];
+    uint32_t rank_key[8];
 } orgpu_ref_req;
 
 /* Reserves the reference-order search scratch: per concurrent search

@@ -53,6 +53,8 @@ class Config:
         self.grid_pitch = 0.4
         self.via_cost = 0.7
         self.portal_enabled = True
+        self.portal_delta_pref = 6
+        self.portal_retarget_patience = 3
         self.stagnation_patience = 5
         self.hotset_cap = 512
         self.hist_cost_weight = 10.0
@@ -171,6 +173,15 @@ def apply(cfg, d, max_iterations_cap=None):
 # router
 # --------------------------------------------------------------------------
 
+class OraclePortal:
+    """The Portal fields the router reads and retargets (PEP:55-84)."""
+
+    def __init__(self, x_idx, y_idx, entry_layer, direction, delta_steps, retarget_count=0):
+        self.x_idx, self.y_idx, self.entry_layer = int(x_idx), int(y_idx), int(entry_layer)
+        self.direction, self.delta_steps = int(direction), int(delta_steps)
+        self.retarget_count = int(retarget_count)
+
+
 class OracleRouter:
     def __init__(self, X, Y, L, policy="reference", pitch=1.0, via_cost=3.0, sssp="ref", config=None,
                  bounds=None):
@@ -227,6 +238,10 @@ class OracleRouter:
                          "z_lo": np.clip(np.minimum(zu, zv), 1, L - 2).astype(np.int32),
                          "z_hi": np.clip(np.maximum(zu, zv), 1, L - 2).astype(np.int32)}
         self.trace = []   # per-iteration records for tests
+        # portals (UPF:2054-2057): pad id -> OraclePortal, net -> (pad, pad)
+        self.portals = {}
+        self.net_pad_ids = {}
+        self.net_portal_failures = defaultdict(int)
 
     # ---------------- helpers
     def coords(self, n):
@@ -499,7 +514,44 @@ class OracleRouter:
                 self.edge_to_nets[e].discard(nid)
             del self.net_to_edges[nid]
 
+    def portal_nodes(self, nid):
+        """(source seed, target seed, source entry node, target entry node)
+        of a net with portals on both pads: a seed is the portal's F.Cu node
+        (UPF:2495-2513), an entry node the portal on its entry layer
+        (UPF:4624-4626); None without portals."""
+        if not self.cfg.portal_enabled or nid not in self.net_pad_ids:
+            return None
+        a, b = self.net_pad_ids[nid]
+        sp, dp = self.portals.get(a), self.portals.get(b)
+        if sp is None or dp is None:
+            return None
+        return (self.node(sp.x_idx, sp.y_idx, 0), self.node(dp.x_idx, dp.y_idx, 0),
+                self.node(sp.x_idx, sp.y_idx, sp.entry_layer), self.node(dp.x_idx, dp.y_idx, dp.entry_layer))
+
+    def retarget_portals(self, nid):
+        """_retarget_portals_for_net (UPF:4929-4969)."""
+        for pid in self.net_pad_ids.get(nid, ()):
+            p = self.portals.get(pid)
+            if p is None:
+                continue
+            if p.retarget_count == 0:
+                p.direction = -p.direction
+                p.y_idx = p.y_idx - 2 * p.direction * p.delta_steps
+                p.retarget_count += 1
+            elif p.retarget_count == 1:
+                new = self.cfg.portal_delta_pref
+                if new != p.delta_steps:
+                    p.y_idx += p.direction * (new - p.delta_steps)
+                    p.delta_steps = new
+                    p.retarget_count += 1
+
     def route_all(self, tasks, all_tasks, pres_fac, iteration=0):
+        """_route_all (UPF:4441-4823), one net at a time in routing order.  A
+        portal net's search is find_path_multisource_multisink with one seed
+        and one target (UPF:4732-4738); with one of each it settles the same
+        nodes in the same order as find_path_roi between them and returns the
+        same path (a target is never expanded and the path is fixed when it
+        is popped), so it runs as that search here."""
         self.fallback_count = 0
         bonus = self.stagnation_counter * 0.6
         ordered = self.order_nets(tasks)
@@ -519,10 +571,19 @@ class OracleRouter:
             sx, sy, _ = self.coords(src)
             dx, dy, _ = self.coords(dst)
             md = abs(dx - sx) + abs(dy - sy)
+            pn = self.portal_nodes(nid)
             if md < 125:
-                roi, g2r = self.extract_roi(src, dst, bonus)
+                if pn is not None:
+                    roi = oracle.roi_bfs(self.lat, pn[2], pn[3], stagnation_bonus=bonus, keep=(pn[0], pn[1]))
+                else:
+                    roi, g2r = self.extract_roi(src, dst, bonus)
             else:
                 roi = np.arange(self.N, dtype=np.int32)
+            if pn is not None:
+                have = set(roi.tolist())
+                add = [v for v in (pn[0], pn[1]) if v not in have]
+                if add:
+                    roi = np.append(roi, add)
             roi_set = set(roi.tolist())
             if src not in roi_set:
                 roi = np.append(roi, src)
@@ -531,11 +592,24 @@ class OracleRouter:
             g2r = np.full(self.N, -1, np.int32)
             g2r[roi] = np.arange(len(roi), dtype=np.int32)
             net_int = self.get_net_id_for_keepout(nid) if use_ko else -1
-            path = self.find_path(src, dst, costs, roi, g2r, ko_owner if use_ko else None, net_int)
+            path = None
+            if pn is not None:
+                path = self.find_path(pn[0], pn[1], costs, roi, g2r, ko_owner if use_ko else None, net_int)
+                if path is not None and len(path) <= 1:
+                    path = None
+            if pn is None or not path:
+                path = self.find_path(src, dst, costs, roi, g2r, ko_owner if use_ko else None, net_int)
             used_fallback = False
             if (not path or len(path) <= 1) and self.fallback_count < 5:
-                roi2, g2r2 = self.extract_roi(src, dst, bonus * 2)
-                path = self.find_path(src, dst, costs, roi2, g2r2)
+                if pn is not None:
+                    roi2, g2r2 = self.extract_roi(pn[2], pn[3], bonus * 2)
+                    path = self.find_path(pn[0], pn[1], costs, roi2, g2r2) \
+                        if g2r2[pn[0]] >= 0 and g2r2[pn[1]] >= 0 else None
+                    if path is not None and len(path) <= 1:
+                        path = None
+                else:
+                    roi2, g2r2 = self.extract_roi(src, dst, bonus * 2)
+                    path = self.find_path(src, dst, costs, roi2, g2r2)
                 self.fallback_count += 1
                 used_fallback = True
             if path and len(path) > 1:
@@ -547,6 +621,11 @@ class OracleRouter:
             else:
                 self.net_paths[nid] = []
                 self.clear_tracking(nid)
+                if self.cfg.portal_enabled and nid in self.net_pad_ids:
+                    self.net_portal_failures[nid] += 1
+                    if self.net_portal_failures[nid] >= self.cfg.portal_retarget_patience:
+                        self.retarget_portals(nid)
+                        self.net_portal_failures[nid] = 0
             rec.append((nid, used_fallback))
         total_routed = sum(1 for p in self.net_paths.values() if p)
         return total_routed, len(all_tasks) - total_routed, rec
@@ -632,6 +711,27 @@ class OracleRouter:
                     self.present[e] += 1
 
     # ---------------- negotiation (UPF:3548-4259)
+    def route_board_nets(self, nets, pad_to_node, max_iterations=None, iteration_cb=None):
+        """route_multiple_nets on Board nets given as (name, [pad ids]) with
+        the board's pad -> node map: a two-pad net's task is its pads' nodes,
+        or their portals' entry-layer nodes when both pads have portals
+        (_parse_requests, UPF:2600-2676)."""
+        reqs = []
+        for name, pads in nets:
+            # nets with more than two pads are multi-terminal nets, which the
+            # reference does not route (UPF:2548-2585)
+            if len(pads) != 2 or pads[0] not in pad_to_node or pads[1] not in pad_to_node:
+                continue
+            a, b = pads[0], pads[1]
+            s, d = pad_to_node[a], pad_to_node[b]
+            self.net_pad_ids[name] = (a, b)
+            if self.cfg.portal_enabled and a in self.portals and b in self.portals:
+                pa, pb = self.portals[a], self.portals[b]
+                s = self.node(pa.x_idx, pa.y_idx, pa.entry_layer)
+                d = self.node(pb.x_idx, pb.y_idx, pb.entry_layer)
+            reqs.append((name, int(s), int(d)))
+        return self.route_multiple_nets(reqs, max_iterations, iteration_cb)
+
     def route_multiple_nets(self, requests, max_iterations=None, iteration_cb=None):
         tasks = {}
         for name, s, d in requests:

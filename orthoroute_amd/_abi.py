@@ -95,7 +95,7 @@ class RoiInfo(ctypes.Structure):
 class RefReq(ctypes.Structure):
     _fields_ = [("src", ctypes.c_int32), ("dst", ctypes.c_int32), ("net_id", ctypes.c_int32),
                 ("roi_index", ctypes.c_int32), ("use_keepout", ctypes.c_int32), ("n_rank", ctypes.c_int32),
-                ("rank_node", ctypes.c_int32 * 4), ("rank_key", ctypes.c_uint32 * 4)]
+                ("rank_node", ctypes.c_int32 * 8), ("rank_key", ctypes.c_uint32 * 8)]
 
 
 class SparseOut(ctypes.Structure):

@@ -43,6 +43,7 @@ from ...autotune import BucketTuner, base_width, default_target
 from ...lattice import DIR_H, DIR_HV, DIR_V, layer_directions, reference_via_pairs
 from ...reductions import sum_f32
 from .board_analyzer import analyze_board_characteristics
+from .pad_escape_planner import PadEscapePlanner, Portal  # noqa: F401  (Portal: the reference's export)
 from .parameter_derivation import apply_derived_parameters, derive_routing_parameters
 
 logger = logging.getLogger(__name__)
@@ -66,6 +67,17 @@ class GeometryPayload:
     def __init__(self, tracks, vias):
         self.tracks = tracks
         self.vias = vias
+
+
+def _dedupe(items, key):
+    """items in order, later ones with an earlier one's key dropped."""
+    seen, out = set(), []
+    for it in items:
+        k = key(it)
+        if k not in seen:
+            seen.add(k)
+            out.append(it)
+    return out
 
 
 class GPUConfig:
@@ -126,7 +138,7 @@ class PathFinderConfig:
     phase_block_after: int = 2
     congestion_multiplier: float = 1.0
     max_search_nodes: int = 2000000
-    layer_names = None
+    layer_names = ("F.Cu", "In1.Cu", "In2.Cu", "In3.Cu", "In4.Cu", "B.Cu")   # UPF:632
     hotset_cap: int = 512
     allowed_via_spans = None
     # execution knobs of this build
@@ -426,6 +438,10 @@ class PathFinderRouter:
         self._ko_owner = np.zeros(0, np.int64)
         self.portals = {}
         self.net_pad_ids: Dict[str, Tuple[str, str]] = {}
+        self.net_portal_failures: Dict[str, int] = defaultdict(int)
+        self.net_portal_layers: Dict[str, Tuple[int, int]] = {}
+        self.escape_planner: Optional[PadEscapePlanner] = None
+        self._escape_tracks, self._escape_vias = [], []
         self.stagnation_counter = 0
         self.full_graph_fallback_count = 0
         self._last_ripped: Set[str] = set()
@@ -450,6 +466,11 @@ class PathFinderRouter:
                                         + ["B.Cu"]))
         self._init_lattice(bounds, self.config.grid_pitch, self.config.layer_count)
         self._map_pads(board)
+        # the escape planner is built once the pads are mapped; it seeds the
+        # router's generator (UPF:2237-2240)
+        self.escape_planner = PadEscapePlanner(self.lattice, self.config, self.pad_to_node,
+                                               random_seed=getattr(self.config, "escape_random_seed", 42),
+                                               rng=self._rng)
         return True
 
     def initialize_lattice(self, x_steps, y_steps, layers, pitch=1.0, via_cost=None):
@@ -557,9 +578,17 @@ class PathFinderRouter:
         return None
 
     def precompute_all_pad_escapes(self, board, nets_to_route=None):
-        """Pad escape planning (portals) is outside this build; nets route
-        pad node to pad node as the reference does when no portals exist."""
-        return [], []
+        """UPF:5619-5650: plans the pads' portals (PadEscapePlanner), routes
+        nets between them from now on and keeps the escape stubs for
+        emit_geometry.  Returns (escape tracks, escape vias); nothing before
+        initialize_graph(board) has built the planner."""
+        if self.escape_planner is None:
+            logger.error("precompute_all_pad_escapes: call initialize_graph(board) first")
+            return [], []
+        result = self.escape_planner.precompute_all_pad_escapes(board, nets_to_route)
+        self.portals = dict(self.escape_planner.portals)
+        self._escape_tracks, self._escape_vias = result
+        return result
 
     # ---------------- small helpers
     def _get_net_id(self, name):
@@ -586,9 +615,14 @@ class PathFinderRouter:
 
     # ---------------- requests
     def route_multiple_nets(self, requests: List, progress_cb=None, iteration_cb=None) -> Dict:
-        """UPF:2530-2598.  Multi-terminal nets route as their first two pads
-        (the reference's live path does the same: multi_terminal_enabled is
-        not a config attribute, UPF:2558)."""
+        """UPF:2530-2598.  Nets with more than two pads are set aside as
+        multi-terminal nets; they are routed (as their first two pads, the
+        reference's fallback once its multi-terminal router fails) only when
+        config.multi_terminal_enabled is set, and otherwise not at all, as in
+        the reference, whose PathFinderConfig has no such attribute."""
+        multi = [q for q in requests if hasattr(q, "pads") and len(q.pads) > 2]
+        if multi and not getattr(self.config, "multi_terminal_enabled", False):
+            requests = [q for q in requests if not (hasattr(q, "pads") and len(q.pads) > 2)]
         tasks = self._parse_requests(requests)
         if not tasks:
             self._negotiation_ran = True
@@ -607,6 +641,8 @@ class PathFinderRouter:
                 self.net_paths[n] = p.tolist()
 
     def _parse_requests(self, requests):
+        """UPF:2600-2676: a Board net's task is its first two pads' nodes, or
+        with portals on both pads the portals' entry-layer nodes."""
         tasks = {}
         for req in requests:
             if hasattr(req, "name") and hasattr(req, "pads"):
@@ -616,6 +652,11 @@ class PathFinderRouter:
                 if k1 in self.pad_to_node and k2 in self.pad_to_node:
                     s, d = self.pad_to_node[k1], self.pad_to_node[k2]
                     self.net_pad_ids[req.name] = (k1, k2)
+                    if self.config.portal_enabled and k1 in self.portals and k2 in self.portals:
+                        p1, p2 = self.portals[k1], self.portals[k2]
+                        s = self.lattice.node_idx(p1.x_idx, p1.y_idx, p1.entry_layer)
+                        d = self.lattice.node_idx(p2.x_idx, p2.y_idx, p2.entry_layer)
+                        self.net_portal_layers[req.name] = (p1.entry_layer, p2.entry_layer)
                     if s != d:
                         tasks[req.name] = (s, d)
                     else:
@@ -1089,17 +1130,29 @@ class PathFinderRouter:
         return float(np.quantile(gap, 0.9)) if len(gap) else None
 
     def _search_batch(self, reqs, keepout_owner=None, readd=True, depth_hint=None):
-        """reqs: (src, dst, net_int, roi, stagnation_bonus) per request, as a
-        list of tuples or an (n, 5) array.  roi False = whole graph.
+        """reqs: per request (src, dst, net_int, roi, stagnation_bonus), as a
+        list of tuples or an (n, 5) array, optionally followed by (roi_src,
+        roi_dst, keep0, keep1, add0, add1) for a portal net (n, 11): its ROI
+        grows from roi_src/roi_dst (the portals' entry-layer nodes) and
+        keeps keep0/keep1 (its seeds, -1 none), and with readd add0/add1 (the
+        task's nodes; -1 none) are appended after the other re-added nodes
+        when missing (UPF:4619-4700).  roi False = whole graph.  A request
+        whose src or dst is not in its ROI finds no path.
         depth_hint: the batch's search depth estimate (see _depth_hint).  Returns per
         request (nodes, edges) as int32 arrays, or None."""
         torch = self.ctx.torch
         dlat = self.graph.dlat
-        R = np.asarray(reqs, dtype=np.float64).reshape(-1, 5)
+        R = np.asarray(reqs, dtype=np.float64)
+        R = R.reshape(len(R), -1) if R.size else np.zeros((0, 5))
         n = len(R)
         out = [None] * n
         src, dst = R[:, 0].astype(np.int64), R[:, 1].astype(np.int64)
         nid, use_roi, bonus = R[:, 2].astype(np.int64), R[:, 3] != 0, R[:, 4]
+        if R.shape[1] >= 11:
+            rsrc, rdst = R[:, 5].astype(np.int64), R[:, 6].astype(np.int64)
+            keeps, adds = R[:, 7:9].astype(np.int64), R[:, 9:11].astype(np.int64)
+        else:
+            rsrc, rdst, keeps, adds = src, dst, None, None
         roi_at = np.nonzero(use_roi)[0]
         row_of = np.full(n, -1, dtype=np.int64)
         row_of[roi_at] = np.arange(len(roi_at))
@@ -1107,15 +1160,23 @@ class PathFinderRouter:
         keep = np.ones(n, dtype=bool)
         ref_order = self.config.sssp_order == "reference"
         info = None
+        added = {}
         t0 = time.perf_counter()
         if len(roi_at):
             words = (self.N + 31) // 32
             rows = torch.empty((len(roi_at), words), dtype=torch.int32, device=self.ctx.device)
-            depth = self.roi_extractor.roi_depth_many(src[roi_at], dst[roi_at], bonus[roi_at])
-            rq = np.stack([src[roi_at], dst[roi_at], depth, np.full(len(roi_at), MAX_ROI_NODES, np.int64),
+            depth = self.roi_extractor.roi_depth_many(rsrc[roi_at], rdst[roi_at], bonus[roi_at])
+            rq = np.stack([rsrc[roi_at], rdst[roi_at], depth, np.full(len(roi_at), MAX_ROI_NODES, np.int64),
                            np.arange(len(roi_at))], axis=1)
-            info = roi_bfs(self.ctx, dlat, rq, rows, readd=readd, raw=True)
+            kp = keeps[roi_at] if keeps is not None and (keeps[roi_at] >= 0).any() else None
+            info = roi_bfs(self.ctx, dlat, rq, rows, readd=readd, raw=True, keeps=kp)
+            if readd and adds is not None:
+                added = self._append_roi_nodes(rows, roi_at, adds[roi_at], rsrc[roi_at], rdst[roi_at])
+            same = (src[roi_at] == rsrc[roi_at]) & (dst[roi_at] == rdst[roi_at])
             ok = (info["src_in"] != 0) & (info["dst_in"] != 0)
+            if not same.all():
+                at = np.nonzero(~same)[0]
+                ok[at] = self._roi_has(rows, at, src[roi_at[at]]) & self._roi_has(rows, at, dst[roi_at[at]])
             keep[roi_at[~ok]] = False
         self.stats["roi_s"] += time.perf_counter() - t0
         todo = np.nonzero(keep)[0]
@@ -1133,7 +1194,15 @@ class PathFinderRouter:
         if ref_order:
             # the reference's pop order needs each ROI's list order: the
             # ascending node order, with the exceptions its ROI record names
-            ranks = [rank_exceptions(info[row_of[i]], self.N) if row_of[i] >= 0 else None for i in todo.tolist()]
+            ranks = []
+            for i in todo.tolist():
+                if row_of[i] < 0:
+                    ranks.append(None)
+                    continue
+                ex = rank_exceptions(info[row_of[i]], self.N)
+                na = int((info["appended"][row_of[i]] >= 0).sum())
+                ex += [(v, 2 * self.N + 2 + na + k) for k, v in enumerate(added.get(int(row_of[i]), ()))]
+                ranks.append(ex)
             nodes, edges, offs, _, rst = device_route_ref(self.ctx, dlat, self.accounting.total_cost, sreqs[:, :5],
                                                           roi_bitmaps=rows, keepout_owner=ko, ranks=ranks,
                                                           slots=int(self.config.batch_slots))
@@ -1171,6 +1240,76 @@ class PathFinderRouter:
             if b - a > 1:
                 out[i] = (nodes[a:b], edges[a:b - 1])
         return out
+
+    def _roi_has(self, rows, row_idx, nodes):
+        """Membership bits of nodes[k] in ROI row row_idx[k] (bool array)."""
+        torch = self.ctx.torch
+        nodes = np.asarray(nodes, dtype=np.int64)
+        r = torch.from_numpy(np.asarray(row_idx, dtype=np.int64)).to(self.ctx.device)
+        w = torch.from_numpy(nodes >> 5).to(self.ctx.device)
+        words = rows[r, w].cpu().numpy().view(np.uint32).astype(np.int64)
+        return ((words >> (nodes & 31)) & 1).astype(bool)
+
+    def _append_roi_nodes(self, rows, roi_at, adds, rsrc, rdst):
+        """A portal net's task nodes that its ROI lacks, appended after the
+        ROI's own re-added nodes (UPF:4683-4696: the reference appends src,
+        then dst, when missing).  They differ from the ROI's endpoints only
+        after a portal retarget, so this runs for few requests.  Returns
+        {row: [appended nodes in order]}."""
+        out = {}
+        cand = np.nonzero(((adds[:, 0] >= 0) & (adds[:, 0] != rsrc)) | ((adds[:, 1] >= 0) & (adds[:, 1] != rdst)))[0]
+        torch = self.ctx.torch
+        for j in cand.tolist():
+            for v in adds[j].tolist():
+                if v < 0 or self._roi_has(rows, [j], [v])[0]:
+                    continue
+                w = int(rows[j, v >> 5].item()) & 0xFFFFFFFF
+                w |= 1 << (v & 31)
+                rows[j, v >> 5] = torch.tensor(np.array([w], np.uint32).view(np.int32)[0], dtype=torch.int32)
+                out.setdefault(j, []).append(v)
+        return out
+
+    def _portal_nodes(self, nid):
+        """(seed at the source portal, seed at the target portal, source
+        entry node, target entry node) of a net routed between its pads'
+        portals (UPF:4509-4524, 4624-4639; a portal's seed is its F.Cu node,
+        UPF:2495-2513), or None.  A portal whose row a retarget has moved
+        off the lattice is not used."""
+        if not self.config.portal_enabled or not self.portals:
+            return None
+        ids = self.net_pad_ids.get(nid)
+        if ids is None:
+            return None
+        sp, dp = self.portals.get(ids[0]), self.portals.get(ids[1])
+        if sp is None or dp is None:
+            return None
+        Y = self.lattice.y_steps
+        if not (0 <= sp.y_idx < Y and 0 <= dp.y_idx < Y):
+            return None
+        ni = self.lattice.node_idx
+        return (ni(sp.x_idx, sp.y_idx, 0), ni(dp.x_idx, dp.y_idx, 0),
+                ni(sp.x_idx, sp.y_idx, sp.entry_layer), ni(dp.x_idx, dp.y_idx, dp.entry_layer))
+
+    def _retarget_portals_for_net(self, nid):
+        """UPF:4929-4969: after repeated failures a net's portals first flip
+        to the pad's other side, then move to the preferred escape length."""
+        ids = self.net_pad_ids.get(nid)
+        if ids is None:
+            return
+        for pid in ids:
+            p = self.portals.get(pid)
+            if p is None:
+                continue
+            if p.retarget_count == 0:
+                p.direction = -p.direction
+                p.y_idx = p.y_idx - 2 * p.direction * p.delta_steps
+                p.retarget_count += 1
+            elif p.retarget_count == 1:
+                new = self.config.portal_delta_pref
+                if new != p.delta_steps:
+                    p.y_idx += p.direction * (new - p.delta_steps)
+                    p.delta_steps = new
+                    p.retarget_count += 1
 
     def _route_all(self, tasks, all_tasks=None, pres_fac=1.0, iteration=0, over_map=None):
         if all_tasks is None:
@@ -1218,11 +1357,26 @@ class PathFinderRouter:
         X, plane = self._Nx, self._Nx * self._Ny
         rs, rd = sd[:, 0] % plane, sd[:, 1] % plane
         md = np.abs(rd % X - rs % X) + np.abs(rd // X - rs // X)
-        reqs = np.empty((n, 5), dtype=np.float64)
+        reqs = np.empty((n, 11), dtype=np.float64)
         reqs[:, 0], reqs[:, 1] = sd[:, 0], sd[:, 1]
         reqs[:, 2] = [self._get_net_id(nid) for nid in search_order] if use_ko else -1
         reqs[:, 3] = md < ROI_THRESHOLD_STEPS
         reqs[:, 4] = bonus
+        reqs[:, 5], reqs[:, 6] = sd[:, 0], sd[:, 1]
+        reqs[:, 7:11] = -1
+        # a net with portals on both pads searches from its source portal's
+        # seed to its target portal's (a multi-source search with one seed
+        # each, UPF:4732-4738, is this two-pin search) in an ROI grown from
+        # the portals' entry nodes that keeps both seeds
+        portal = {}
+        if self.config.portal_enabled and self.portals:
+            for i, nid in enumerate(search_order):
+                pn = self._portal_nodes(nid)
+                if pn is not None:
+                    portal[nid] = pn
+                    reqs[i, 0], reqs[i, 1], reqs[i, 5], reqs[i, 6] = pn
+                    reqs[i, 7], reqs[i, 8] = pn[0], pn[1]
+                    reqs[i, 9], reqs[i, 10] = sd[i, 0], sd[i, 1]
         # a batch that the engine runs at once (no slot refills) is only as
         # fast as its deepest searches: size its buckets by their depth (the
         # last estimate when none of the batch's nets holds a path)
@@ -1245,11 +1399,24 @@ class PathFinderRouter:
             perm = np.fromiter((pos[nm] for nm in ordered), dtype=np.int64, count=n)
             results = [results[i] for i in perm.tolist()]
             reqs = reqs[perm]
-        # wider-ROI fallback for the first five failures, in routing order
+        # a portal net whose portal search failed searches its task's nodes
+        # in the same ROI (UPF:4740-4748)
+        if portal:
+            pf = [i for i, (nid, r) in enumerate(zip(ordered, results))
+                  if nid in portal and (r is None or len(r[0]) <= 1)]
+            if pf:
+                preqs = reqs[pf].copy()
+                preqs[:, 0], preqs[:, 1] = preqs[:, 9], preqs[:, 10]
+                for i, r in zip(pf, self._search_batch(preqs, ko_owner, readd=True)):
+                    results[i] = r
+        # wider-ROI fallback for the first five failures, in routing order;
+        # a portal net's grows from its entry nodes without its seeds and
+        # searches seed to seed (UPF:4753-4781)
         fb = [i for i, r in enumerate(results) if r is None or len(r[0]) <= 1][:5]
         self.full_graph_fallback_count = len(fb)
         if fb:
-            freqs = [(int(reqs[i, 0]), int(reqs[i, 1]), -1, True, bonus * 2) for i in fb]
+            freqs = np.array([(reqs[i, 0], reqs[i, 1], -1, 1, bonus * 2, reqs[i, 5], reqs[i, 6], -1, -1, -1, -1)
+                              for i in fb], dtype=np.float64)
             fres = self._search_batch(freqs, None, readd=False)
             for i, r in zip(fb, fres):
                 results[i] = r
@@ -1278,6 +1445,21 @@ class PathFinderRouter:
                 paths[nid] = []
                 n2e.pop(nid, None)
                 n2n.pop(nid, None)
+        if self.net_pad_ids and self.config.portal_enabled:
+            # portal failures and retargeting (UPF:4800-4812); a net's portals
+            # are its own pads', so retargeting after the batch is the same as
+            # between nets
+            plane = self._Nx * self._Ny
+            patience = int(self.config.portal_retarget_patience)
+            for nid, r in zip(ordered, results):
+                if r is not None and len(r[0]) > 1:
+                    if nid in portal:
+                        self.net_portal_layers[nid] = (int(r[0][0]) // plane, int(r[0][-1]) // plane)
+                elif nid in self.net_pad_ids:
+                    self.net_portal_failures[nid] += 1
+                    if self.net_portal_failures[nid] >= patience:
+                        self._retarget_portals_for_net(nid)
+                        self.net_portal_failures[nid] = 0
         if dec:
             self.accounting.clear_path(np.concatenate(dec))
         if inc:
@@ -1586,17 +1768,30 @@ class PathFinderRouter:
 
     # ---------------- geometry (UPF:5677-5914 surface)
     def emit_geometry(self, board):
-        """UPF:5677-5747: tracks and vias of the committed paths; the export
-        payload only when the routing has no overuse (edge or via), the
-        counts either way.  This build plans no pad escapes, so there is no
-        escape geometry to merge."""
+        """UPF:5677-5747: tracks and vias of the committed paths, after the
+        pad escape stubs when there are any (escapes first, then routes,
+        duplicates by rounded geometry dropped); the export payload only when
+        the routing has no overuse (edge or via), the counts either way."""
         tracks, vias = self._generate_geometry_from_paths()
+        if self._escape_tracks:
+            tracks = _dedupe(list(self._escape_tracks) + tracks,
+                             lambda t: (t["net"], t["layer"], round(t["x1"], 3), round(t["y1"], 3),
+                                        round(t["x2"], 3), round(t["y2"], 3), round(t["width"], 3)))
+            vias = _dedupe(list(self._escape_vias) + vias,
+                           lambda v: (v["net"], round(v["x"], 3), round(v["y"], 3), v.get("from_layer"),
+                                      v.get("to_layer"), round(v.get("drill", 0), 3), round(v.get("diameter", 0), 3)))
         self._provisional_geometry = GeometryPayload(tracks, vias)
         over_sum, _ = self.accounting.compute_overuse(router_instance=self)
         self._geometry_payload = GeometryPayload(tracks, vias) if over_sum == 0 else GeometryPayload([], [])
         return len(tracks), len(vias)
 
     def get_geometry_payload(self):
+        """UPF:5898-5910: the export payload, or the provisional geometry
+        when there is no export payload (overuse remains)."""
+        prov = getattr(self, "_provisional_geometry", None)
+        if not self._geometry_payload.tracks and not self._geometry_payload.vias and prov is not None \
+                and (prov.tracks or prov.vias):
+            return prov
         return self._geometry_payload
 
     def get_provisional_geometry(self):

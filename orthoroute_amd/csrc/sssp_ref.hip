@@ -103,7 +103,7 @@ __device__ __forceinline__ uint32_t node_key(const orgpu_ref_req &q, int32_t v)
 {
     uint32_t k = 2u * (uint32_t)v + 2u;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 8; ++i)
         if (i < q.n_rank && q.rank_node[i] == v) k = q.rank_key[i];
     return k;
 }
@@ -204,7 +204,7 @@ k_sssp_ref(RefArgs a)
         float dd = __builtin_inff();
         int64_t off = 0;
         const bool valid = q.src >= 0 && q.src < a.N && q.dst >= 0 && q.dst < a.N && q.n_rank >= 0 &&
-                           q.n_rank <= 4 && q.roi_index < a.n_roi && (q.roi_index < 0 || a.roi) &&
+                           q.n_rank <= 8 && q.roi_index < a.n_roi && (q.roi_index < 0 || a.roi) &&
                            (!q.use_keepout || a.ko_owner);
         const uint32_t *roi = valid && q.roi_index >= 0 ? a.roi + (int64_t)q.roi_index * a.roi_words : nullptr;
         if (!valid) {

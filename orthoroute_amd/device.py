@@ -299,8 +299,8 @@ ROI_INFO = np.dtype({"names": ["seen", "kept", "depth", "met", "src_in", "dst_in
                      "offsets": [0, 8, 16, 20, 24, 28, 32, 48, 64], "itemsize": 80})
 # orgpu_ref_req rows
 REF_REQ = np.dtype({"names": ["src", "dst", "net_id", "roi_index", "use_keepout", "n_rank", "rank_node", "rank_key"],
-                    "formats": ["<i4", "<i4", "<i4", "<i4", "<i4", "<i4", ("<i4", 4), ("<u4", 4)],
-                    "offsets": [0, 4, 8, 12, 16, 20, 24, 40], "itemsize": 56})
+                    "formats": ["<i4", "<i4", "<i4", "<i4", "<i4", "<i4", ("<i4", 8), ("<u4", 8)],
+                    "offsets": [0, 4, 8, 12, 16, 20, 24, 56], "itemsize": 88})
 
 
 def _ms_points(pts):
@@ -443,9 +443,10 @@ def ref_requests(requests, ranks=None):
         for i, ex in enumerate(ranks):
             if not ex:
                 continue
-            if len(ex) > 4:
-                raise ValueError("at most four rank exceptions per request (a two-pin search's ROI has at most "
-                                 "src and dst out of order)")
+            if len(ex) > 8:
+                raise ValueError(f"request {i}: {len(ex)} rank exceptions, at most eight are supported (an ROI "
+                                 "holds at most four swapped-in and six appended must-keeps, and only a "
+                                 "truncated ROI swaps)")
             rows["n_rank"][i] = len(ex)
             for k, (v, key) in enumerate(ex):
                 rows["rank_node"][i, k] = v

@@ -20,6 +20,11 @@ def route_orp_file(orp_path, ors_path, config: PathFinderConfig = None, compress
         cfg.grid_pitch = float(pitch)
     router = PathFinderRouter(cfg)
     router.initialize_graph(board)
+    # the escape planner reads pad outlines from GUI-style pad records
+    # (main.py:494-500: x, y, width, height in mm)
+    board._gui_pads = [{"x": p["position"]["x"], "y": p["position"]["y"], "width": p["size"]["width"],
+                        "height": p["size"]["height"]} for p in orp.get("pads", [])
+                       if p.get("position") and p.get("size")]
     router.precompute_all_pad_escapes(board)
     router.prepare_routing_runtime()
     metrics = []

@@ -491,6 +491,107 @@ def gen_geometry(upf, Board, Net, Pad, Coordinate):
     return len(tracks), len(vias)
 
 
+def portal_board_spec(seed=31):
+    """A routing board as plain data (mm): two 2x10 connectors at 2 mm pitch,
+    a dense 1 mm row (escapes collide), a part with pads off the lattice
+    columns (45-degree stubs, one past the snap tolerance), a through-hole
+    pad; two-pin nets across them."""
+    rng = np.random.default_rng(seed)
+    comps = []
+
+    def part(cid, pads):
+        comps.append({"id": cid, "pads": [{"id": f"{cid}_{k}", "x": float(x), "y": float(y), "w": 0.6, "h": 0.6,
+                                           "drill": d} for k, (x, y, d) in enumerate(pads)]})
+    part("J1", [(6 + 2 * k, 6 + 2 * r, None) for r in range(2) for k in range(10)])
+    part("J2", [(10 + 2 * k, 30 + 2 * r, None) for r in range(2) for k in range(10)])
+    part("J3", [(30 + k, 18, None) for k in range(8)])
+    part("U1", [(20.3, 20.0, None), (22.6, 20.0, None), (24.5, 21.0, None), (26.0, 22.0, 0.8), (27.0, 14.0, None)])
+    nets = []
+    for k in range(10):
+        nets.append((f"A{k}", [f"J1_{k}", f"J2_{k}"]))
+    for k in range(6):
+        nets.append((f"B{k}", [f"J1_{10 + k}", f"J3_{k}"]))
+    nets.append(("C0", ["U1_0", "J2_12"]))
+    nets.append(("C1", ["U1_1", "J2_14"]))
+    nets.append(("C2", ["U1_2", "J3_6"]))
+    nets.append(("C3", ["U1_3", "J2_16"]))
+    nets.append(("C4", ["U1_4", "J3_7"]))
+    nets.append(("C5", ["J2_18", "J1_19", "J2_19"]))
+    return comps, nets
+
+
+def gen_portals(upf, root, cap=12):
+    """PadEscapePlanner.precompute_all_pad_escapes (PEP:127-486) through the
+    router's entry point (UPF:5619-5650) on portal_board_spec, then
+    route_multiple_nets on the board's nets with portals (UPF:4498-4812) for
+    `cap` iterations, and emit_geometry's merge of escapes and routes
+    (UPF:5677-5740)."""
+    from orthoroute.domain.models.board import Board, Component, Coordinate, Net, Pad
+    comps, nets = portal_board_spec()
+    pad_obj, components = {}, []
+    for c in comps:
+        pads = [Pad(id=p["id"], component_id=c["id"], net_id=None, position=Coordinate(p["x"], p["y"]),
+                    size=(p["w"], p["h"]), drill_size=p["drill"]) for p in c["pads"]]
+        for q in pads:
+            pad_obj[q.id] = q
+        components.append(Component(id=c["id"], reference=c["id"], value="", footprint="",
+                                    position=Coordinate(pads[0].position.x, pads[0].position.y), pads=pads))
+    net_objs = [Net(id=name, name=name, pads=[pad_obj[p] for p in pids]) for name, pids in nets]
+    board = Board(id="P", name="portals", components=components, nets=net_objs, layer_count=6)
+    cfg = upf.PathFinderConfig()
+    cfg.grid_pitch = 1.0
+    cfg.via_cost = 3.0
+    router = upf.PathFinderRouter(cfg, use_gpu=False)
+    router.initialize_graph(board)
+    tracks, vias = router.precompute_all_pad_escapes(board)
+    st = random.getstate()     # the planner's draws come from the global generator
+    lat = router.lattice
+    portals = {pid: [p.x_idx, p.y_idx, p.pad_layer, p.delta_steps, p.direction, p.pad_x, p.pad_y, p.entry_layer]
+               for pid, p in router.portals.items()}
+    orig_apply = upf.apply_derived_parameters
+
+    def capped_apply(config, derived):
+        orig_apply(config, derived)
+        config.max_iterations = cap
+    upf.apply_derived_parameters = capped_apply
+    rec = {"over": [], "hist": [], "ema": [], "order": []}
+    orig_order = router._order_nets_by_difficulty
+
+    def order_wrap(tasks):
+        o = orig_order(tasks)
+        rec["order"].append(list(o))
+        return o
+    router._order_nets_by_difficulty = order_wrap
+
+    def it_cb(it, tr, vi, over_sum, over_cnt):
+        rec["over"].append([it, int(over_sum), int(over_cnt)])
+        rec["hist"].append(_digest(np.asarray(router.accounting.history, dtype=np.float32)))
+        rec["ema"].append(_digest(np.asarray(router.accounting.present_ema, dtype=np.float32)))
+        rec["paths"].append({k: [int(v) for v in p] for k, p in router.net_paths.items()})
+    rec["paths"] = []
+    res = router.route_multiple_nets(net_objs, iteration_cb=it_cb)
+    upf.apply_derived_parameters = orig_apply
+    n_tracks, n_vias = router.emit_geometry(board)
+    payload = router.get_geometry_payload()
+    out = {"comps": comps, "nets": nets, "X": lat.x_steps, "Y": lat.y_steps, "L": lat.layers,
+           "grid_min_x": float(lat.geom.grid_min_x), "grid_min_y": float(lat.geom.grid_min_y),
+           "pad_to_node": {k: int(v) for k, v in router.pad_to_node.items()},
+           "portals": portals, "escape_tracks": tracks, "escape_vias": vias,
+           "retargeted": {pid: [p.x_idx, p.y_idx, p.delta_steps, p.direction, p.retarget_count]
+                          for pid, p in router.portals.items()},
+           "rng_state": [st[0], list(st[1]), st[2]],
+           "net_pad_ids": {k: list(v) for k, v in router.net_pad_ids.items()},
+           "tasks_cap": cap, "over": rec["over"], "hist_digest": rec["hist"], "ema_digest": rec["ema"],
+           "order": rec["order"], "paths_iter": rec["paths"],
+           "final_paths": {k: [int(v) for v in p] for k, p in res.get("paths", {}).items()},
+           "success": bool(res.get("success", False)),
+           "geometry_counts": [n_tracks, n_vias],
+           "payload_tracks": payload.tracks, "payload_vias": payload.vias}
+    with open(os.path.join(HERE, "portals.json"), "w") as f:
+        json.dump(out, f, indent=0)
+    return len(portals), len(tracks), rec["over"][-1] if rec["over"] else None
+
+
 def synthetic_board_spec(seed=11):
     """A small board as plain data: components with pads (some without a
     component id, one with a drill), nets over them, one unconnected pad."""
@@ -641,6 +742,8 @@ def main():
         r, ti, tr = gen_trajectory(upf, Board, Net, Pad, Coordinate, X, Y, L, reqs, 30, "trajectory_cfg1")
         print("cfg1", {k: r.get(k) for k in ("success", "overuse_sum", "failed_nets")},
               f"init {ti:.1f} s, route {tr:.1f} s")
+    if "portals" in which:
+        print("portals", gen_portals(upf, root))
     if "roi_keep" in which:
         print("roi_keep", gen_roi_keep(upf))
     if "geometry" in which:
