@@ -54,14 +54,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--slots", type=int, default=1024,
-                    help="concurrent searches of the batched form (clamped so that slots x nodes < 2^32)")
+    ap.add_argument("--slots", type=int, default=2048,
+                    help="concurrent searches of the batched form (clamped to what 70 %% of the device memory holds)")
     ap.add_argument("--plain-sample", type=int, default=1024,
                     help="searches timed in the plain vs batched comparison (the batched form holds up to "
                          "--slots of them at once)")
     ap.add_argument("--stress-nets", type=int, default=40000,
                     help="nets of the 1000x800x24 stress board routed in one timed pass (0: skip)")
-    ap.add_argument("--stress-plain", type=int, default=256,
+    ap.add_argument("--stress-plain", type=int, default=1024,
                     help="searches of the stress board timed plain vs batched")
     ap.add_argument("--rounds-per-poll", type=int, default=32, help="SSSP rounds queued between slot refills")
     ap.add_argument("--ttz-nets", type=int, default=120,
@@ -228,6 +228,13 @@ def main():
 
     geo = geometry(r)
     costk = cost_kernels(r, pres_fac, hmult, args.cost_reps) if args.cost_reps > 0 else None
+    # the sections below build boards of their own: the backplane's search
+    # scratch (each slot's keys and bucket bitmaps) is released first
+    sp_line, slots_line = r._search_params(args.slots), min(args.slots, max_slots(r.N))
+    r.ctx.close()
+    import gc
+    gc.collect()
+    torch.cuda.empty_cache()
     neg2 = negotiation(args, "cfg2", args.cfg2_iterations, bool(args.cfg2_plain)) if args.cfg2_iterations > 0 else None
     neg3 = negotiation(args, "cfg3", args.cfg3_iterations, True) if args.cfg3_iterations > 0 else None
     refo = reference_order(args) if args.ref_iterations > 0 else None
@@ -273,8 +280,7 @@ def main():
                         f"{r.accounting.E} edges, {len(tasks)} two-pin nets; one step = one full routing "
                         f"pass on the iteration-1 cost snapshot",
             "nodes": r.N, "edges": r.accounting.E, "nets": len(tasks),
-            "slots": min(args.slots, max_slots(r.N)),
-            "delta": r._search_params(args.slots).delta, "n_buckets": r._search_params(args.slots).n_buckets,
+            "slots": slots_line, "delta": sp_line.delta, "n_buckets": sp_line.n_buckets,
             "local_batches": args.local_batches or 64, "grid": args.grid or 1024,
             "rounds_per_poll": args.rounds_per_poll,
         },
@@ -537,9 +543,13 @@ def stress(args):
     from orthoroute_amd.device import max_slots
     from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig, PathFinderRouter
     X, Y, L, reqs = workloads.cfg4(n_nets=args.stress_nets)
+    import gc
+    gc.collect()
+    torch.cuda.empty_cache()
+    slots = min(args.slots, 1024)      # 1,024 searches of 19.2 M nodes: 184 GB of keys and bitmaps
     cfg = PathFinderConfig()
     cfg.lattice_policy = "adjacent"
-    cfg.batch_slots = args.slots
+    cfg.batch_slots = slots
     cfg.sssp_graph_rounds = args.graph_rounds
     r = PathFinderRouter(cfg)
     r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
@@ -564,11 +574,12 @@ def stress(args):
     d = {k: float(v) - s0.get(k, 0.0) for k, v in r.stats.items()}
     log(f"[bench] stress {X}x{Y}x{L}: {len(tasks)} nets in {t:.2f}s, routed={routed} failed={failed}")
     sreqs = sample_reqs(r, tasks, args.stress_plain)
-    timing, _, same = plain_vs_batched(r, sreqs, args.slots)
+    timing, _, same = plain_vs_batched(r, sreqs, slots)
+    r.ctx.close()
     return {"workload": f"cfg4 stress board {X}x{Y}x{L}, adjacent-layer vias, {r.N} nodes, {r.accounting.E} edges, "
                         f"{len(tasks)} two-pin nets; one routing pass on the iteration-1 cost snapshot",
             "pass_s": round(t, 3), "nets_per_s": round(len(tasks) / max(t, 1e-9), 1), "routed": routed,
-            "failed": failed, "slots": min(args.slots, max_slots(r.N)),
+            "failed": failed, "slots": min(slots, max_slots(r.N)),
             "sssp_s": round(d.get("sssp_s", 0.0), 3), "roi_s": round(d.get("roi_s", 0.0), 3),
             "edges_scanned": int(d.get("sssp_edges", 0.0)),
             "plain_vs_batched": {"searches": len(sreqs), "plain_s": round(timing["plain"], 3),

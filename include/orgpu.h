@@ -138,8 +138,9 @@ typedef struct orgpu_sssp_stats {
 } orgpu_sssp_stats;
 
 /* Reserves scratch for graphs of up to n_nodes nodes, `slots` concurrent
- * searches and `n_buckets` ring buckets (at most 32): keys slots*n_nodes*8
- * bytes, (n_buckets+3) bucket bitmaps of slots*n_nodes bits, path staging
+ * searches (at most 2048) and `n_buckets` ring buckets (at most 32): with
+ * NP = n_nodes rounded up to a multiple of 2048, keys slots*NP*8 bytes,
+ * (n_buckets+3) bucket bitmaps of slots*NP bits, path staging
  * 2*slots*max_path_len ints.  queue_capacity is ignored (the frontiers are
  * bitmaps; the argument stays for callers of the earlier queue form).
  * Blocks (allocation). */

@@ -105,6 +105,8 @@ struct RoundArgs {
     SlotInfo *slots;
     int32_t K;
     int64_t N;
+    int64_t NP;               // per-slot item stride: N rounded up to whole chunks
+    uint32_t cps;             // chunks per slot (NP / 2048)
     uint32_t nb;
     float inv_delta;
     float h_xy, h_z;          // goal-directed lower bounds (0, 0: Dijkstra order)
@@ -117,7 +119,7 @@ struct RoundArgs {
     const uint32_t *roi;      // [n_roi][words]
     int64_t roi_words;
     const int32_t *ko_owner;  // [N] or null
-    uint64_t *keys;           // [K][N]
+    uint64_t *keys;           // [K][NP]
     uint32_t *bits;           // [MAX_BUFS][bitmap_words]
     uint64_t bitmap_words;
     uint32_t *summ;           // [MAX_BUFS][summ_words]: one bit per 64-word chunk

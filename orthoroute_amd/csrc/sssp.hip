@@ -7,7 +7,10 @@
 // with the lowest-predecessor-index tie-break (see common.h for the key).
 //
 // Design (one HIP stream, no host sync inside a round):
-//  * state: keys u64[K][N]; a work item is the u32 slot*N + node.
+//  * state: keys u64[K][NP]; a work item is the u64 slot*NP + node, where the
+//    per-slot stride NP is N rounded up to a whole chunk (2048 items), so a
+//    chunk belongs to one slot and a batch gets its slot by one division per
+//    chunk.
 //  * buckets are bitmaps over all items (one bit per (slot, node)): a ring of
 //    nb bitmaps for buckets cur .. cur+nb-1, an OUT bitmap for re-relaxations
 //    inside the current bucket and an overflow bitmap for buckets >= cur+nb.
@@ -38,7 +41,8 @@
 
 #define WG 256
 #define WAVES_PER_WG (WG / 64)
-#define MAX_SLOTS 1024
+#define MAX_SLOTS 2048
+#define CHUNK_ITEMS 2048
 #define MAX_NB MAX_RING
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -130,13 +134,13 @@ __device__ __forceinline__ void list_append(const RoundArgs &a, uint32_t sid, ui
 // bucket's summary bitmap, and the chunk whose summary bit this turns on is
 // appended to the bucket's chunk list: the chunks a round must visit, each
 // listed once.  list_sid: the bucket's chunk-list storage.
-__device__ __forceinline__ bool push_bit(const RoundArgs &a, uint32_t buf, uint32_t list_sid, uint32_t item)
+__device__ __forceinline__ bool push_bit(const RoundArgs &a, uint32_t buf, uint32_t list_sid, uint64_t item)
 {
     const uint32_t m = 1u << (item & 31);
     const uint32_t old = atomicOr(&a.bits[(uint64_t)buf * a.bitmap_words + (item >> 5)], m);
     if (old & m) return false;
     if (old == 0u) {
-        const uint32_t ch = item >> 11;
+        const uint32_t ch = (uint32_t)(item >> 11);
         const uint32_t cm = 1u << (ch & 31);
         if (!(atomicOr(&a.summ[(uint64_t)buf * a.summ_words + (ch >> 5)], cm) & cm)) list_append(a, list_sid, ch);
     }
@@ -147,7 +151,7 @@ __device__ __forceinline__ bool push_bit(const RoundArgs &a, uint32_t buf, uint3
 // bitmaps (OUT, later ring buckets, overflow) only gain bits, so a set bit
 // seen by a plain load is set and the atomic can be skipped (a stale copy can
 // only miss bits, and then the atomic runs).
-__device__ __forceinline__ bool push_bit_relax(const RoundArgs &a, uint32_t buf, uint32_t list_sid, uint32_t item)
+__device__ __forceinline__ bool push_bit_relax(const RoundArgs &a, uint32_t buf, uint32_t list_sid, uint64_t item)
 {
     const uint32_t m = 1u << (item & 31);
     if (a.bits[(uint64_t)buf * a.bitmap_words + (item >> 5)] & m) return false;
@@ -204,7 +208,8 @@ __device__ __forceinline__ ChunkWalk chunk_walk(uint32_t n, uint32_t wave)
 // LDS atomic, head only at wave-uniform points.
 #define LQ_CAP 1024
 struct LocalQ {
-    uint32_t *buf;
+    uint32_t *buf;      // node
+    uint16_t *sbuf;     // slot
     uint32_t *tail;
     uint32_t head;
 };
@@ -239,7 +244,7 @@ __device__ __forceinline__ float heur(const RoundArgs &a, const SlotInfo &si, ui
 // (1) slot, own key and CSR row bounds; (2) the four edges' targets and
 // costs; (3) their ROI bits and target keys; (4) the four atomicMins; then
 // the pushes, and the touched-chunk records last.
-__device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bool has, bool from_bitmap,
+__device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t slot, uint32_t u, bool has, bool from_bitmap,
                                            const LocalQ &lq, uint64_t cur, uint32_t nb,
                                            const uint32_t *s_ring, const uint32_t *s_lid, uint32_t s_out, uint32_t s_ovf,
                                            int32_t *s_slot_delta, int32_t *s_buf_delta,
@@ -247,12 +252,11 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bo
                                            uint64_t &my_ovf_min)
 {
     if (!has) return;
-    const uint32_t N = (uint32_t)a.N;
-    const uint32_t slot = item / N;
-    const uint32_t u = item - slot * N;
+    const uint64_t NP = (uint64_t)a.NP;
     if (from_bitmap) atomicAdd(&s_slot_delta[slot], -1);
     const SlotInfo si = a.slots[slot];
-    const uint64_t ku = __hip_atomic_load(&a.keys[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t *keys = a.keys + (uint64_t)slot * NP;
+    const uint64_t ku = __hip_atomic_load(&keys[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int row = a.indptr[u];
     const int row_end = a.indptr[u + 1];
     if (si.state != 1) return;
@@ -278,7 +282,6 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bo
     // owned node) count as keepout edges instead of adding their cost
     const bool ko_node = si.use_keepout && a.ko_owner && a.ko_owner[u] >= 0 && a.ko_owner[u] != si.net_id;
     const uint32_t *roi = si.roi_index >= 0 ? a.roi + (uint64_t)si.roi_index * a.roi_words : nullptr;
-    uint64_t *keys = a.keys + (uint64_t)slot * N;
     for (int k0 = 0; k0 < deg; k0 += 4) {
         uint32_t v[4];
         float c[4];
@@ -328,7 +331,8 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bo
                 // local queue), unless the queue is full
                 const uint32_t pos = atomicAdd(lq.tail, 1u);
                 if (pos - lq.head < LQ_CAP) {
-                    __hip_atomic_store(&lq.buf[pos & (LQ_CAP - 1)], slot * N + v[q], __ATOMIC_RELAXED,
+                    __hip_atomic_store(&lq.buf[pos & (LQ_CAP - 1)], v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    __hip_atomic_store(&lq.sbuf[pos & (LQ_CAP - 1)], (uint16_t)slot, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WAVEFRONT);
                     continue;
                 }
@@ -338,7 +342,7 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t item, bo
             else if (b < cur + nb) buf = s_ring[b % nb];
             else buf = s_ovf;
             if (buf == s_ovf && b < my_ovf_min) my_ovf_min = b;
-            if (push_bit_relax(a, buf, s_lid[buf], slot * N + v[q])) {
+            if (push_bit_relax(a, buf, s_lid[buf], (uint64_t)slot * NP + v[q])) {
                 atomicAdd(&s_slot_delta[slot], 1);
                 atomicAdd(&s_buf_delta[buf], 1);
             }
@@ -381,8 +385,10 @@ k_round(RoundArgs a)
     __shared__ uint32_t s_binoff[LIST_BINS + 1];  // exclusive prefix of the input list's bins
     __shared__ unsigned long long s_items, s_edges, s_ovf_min;
     __shared__ uint32_t s_clear[WAVES_PER_WG][64]; // redistribution: bits leaving each word
-    __shared__ uint32_t s_ibuf[WAVES_PER_WG][128]; // relaxation: items gathered for the next batch
-    __shared__ uint32_t s_lq[WAVES_PER_WG][LQ_CAP]; // relaxation: this wave's same-bucket items
+    __shared__ uint32_t s_ibuf[WAVES_PER_WG][128]; // relaxation: items gathered for the next batch (node,
+    __shared__ uint16_t s_ibuf_s[WAVES_PER_WG][128]; //  slot)
+    __shared__ uint32_t s_lq[WAVES_PER_WG][LQ_CAP]; // relaxation: this wave's same-bucket items (node,
+    __shared__ uint16_t s_lq_s[WAVES_PER_WG][LQ_CAP]; //  slot)
     __shared__ uint32_t s_lq_tail[WAVES_PER_WG];
     for (int i = threadIdx.x; i < a.K; i += WG) s_slot_delta[i] = 0;
     if (threadIdx.x < MAX_BUFS) s_buf_delta[threadIdx.x] = 0;
@@ -413,7 +419,8 @@ k_round(RoundArgs a)
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / 64;
     const uint64_t W = a.bitmap_words;
-    const uint32_t N = (uint32_t)a.N;
+    const uint64_t NP = (uint64_t)a.NP;
+    const uint32_t cps = a.cps;
     unsigned long long my_items = 0, my_edges = 0;
     // lowest overflow bucket this lane saw a key move into.  It counts every
     // such key, also one whose item already sits in the overflow bitmap (its
@@ -428,8 +435,9 @@ k_round(RoundArgs a)
         // (a chunk of a sparse bucket holds a few dozen), so each batch's
         // dependent loads and atomics serve 64 items at once.
         uint32_t *ibuf = s_ibuf[wave];
+        uint16_t *ibuf_s = s_ibuf_s[wave];
         int nbuf = 0;
-        LocalQ lq{s_lq[wave], &s_lq_tail[wave], 0u};
+        LocalQ lq{s_lq[wave], s_lq_s[wave], &s_lq_tail[wave], 0u};
         for (uint32_t li = cw.first;; li += cw.stride) {
             const bool more = li < cw.end;
             if (more) {
@@ -448,6 +456,9 @@ k_round(RoundArgs a)
                 const int incl = wave_scan(cnt);
                 const int total = __shfl(incl, 63);
                 const int excl = incl - cnt;
+                // the chunk's slot and its first node (chunks never straddle slots)
+                const uint32_t cslot = (uint32_t)(ch / cps);
+                const uint32_t cnode = (uint32_t)(ch - (uint64_t)cslot * cps) * CHUNK_ITEMS;
                 for (int b0 = 0; b0 < total; b0 += 64) {
                     const int t = b0 + (int)lane;
                     const int j = owner_lane(excl, t);
@@ -456,24 +467,31 @@ k_round(RoundArgs a)
                     const int k = min(64, total - b0);
                     if (t < total) {
                         const uint32_t bit = select_bit(wj, (uint32_t)(t - ej));
-                        __hip_atomic_store(&ibuf[nbuf + (int)lane], (uint32_t)((ch * 64 + (uint64_t)j) * 32 + bit),
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        __hip_atomic_store(&ibuf[nbuf + (int)lane], cnode + (uint32_t)j * 32u + bit, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        __hip_atomic_store(&ibuf_s[nbuf + (int)lane], (uint16_t)cslot, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
                     }
                     nbuf += k;
                     if (nbuf >= 64) {
-                        const uint32_t it = __hip_atomic_load(&ibuf[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        const uint32_t nd = __hip_atomic_load(&ibuf[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        const uint32_t sl = __hip_atomic_load(&ibuf_s[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                         const uint32_t rest = __hip_atomic_load(&ibuf[64 + lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        const uint16_t rest_s = __hip_atomic_load(&ibuf_s[64 + lane], __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_WAVEFRONT);
                         __hip_atomic_store(&ibuf[lane], rest, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        __hip_atomic_store(&ibuf_s[lane], rest_s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                         nbuf -= 64;
-                        relax_item(a, it, true, true, lq, cur, nb, s_ring, s_lid, s_out, s_ovf, s_slot_delta,
+                        relax_item(a, sl, nd, true, true, lq, cur, nb, s_ring, s_lid, s_out, s_ovf, s_slot_delta,
                                    s_buf_delta, my_items, my_edges, my_ovf_min);
                         lq_clamp(lq);
                     }
                 }
             } else {
                 if (nbuf > 0) {
-                    const uint32_t it = __hip_atomic_load(&ibuf[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-                    relax_item(a, it, (int)lane < nbuf, true, lq, cur, nb, s_ring, s_lid, s_out, s_ovf, s_slot_delta,
+                    const uint32_t nd = __hip_atomic_load(&ibuf[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    const uint32_t sl = __hip_atomic_load(&ibuf_s[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    relax_item(a, sl, nd, (int)lane < nbuf, true, lq, cur, nb, s_ring, s_lid, s_out, s_ovf, s_slot_delta,
                                s_buf_delta, my_items, my_edges, my_ovf_min);
                     lq_clamp(lq);
                 }
@@ -487,22 +505,27 @@ k_round(RoundArgs a)
             if (tail == lq.head) break;
             if (nbat == a.local_batches) {
                 for (uint32_t i = lq.head + lane; i < tail; i += 64) {
-                    const uint32_t it = __hip_atomic_load(&lq.buf[i & (LQ_CAP - 1)], __ATOMIC_RELAXED,
+                    const uint32_t nd = __hip_atomic_load(&lq.buf[i & (LQ_CAP - 1)], __ATOMIC_RELAXED,
                                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
-                    if (push_bit(a, s_out, s_lid[s_out], it)) {
-                        atomicAdd(&s_slot_delta[it / N], 1);
+                    const uint32_t sl = __hip_atomic_load(&lq.sbuf[i & (LQ_CAP - 1)], __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    if (push_bit(a, s_out, s_lid[s_out], (uint64_t)sl * NP + nd)) {
+                        atomicAdd(&s_slot_delta[sl], 1);
                         atomicAdd(&s_buf_delta[s_out], 1);
                     }
                 }
                 break;
             }
             const uint32_t n = min(64u, tail - lq.head);
-            uint32_t it = 0;
-            if (lane < n)
-                it = __hip_atomic_load(&lq.buf[(lq.head + lane) & (LQ_CAP - 1)], __ATOMIC_RELAXED,
+            uint32_t nd = 0, sl = 0;
+            if (lane < n) {
+                nd = __hip_atomic_load(&lq.buf[(lq.head + lane) & (LQ_CAP - 1)], __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WAVEFRONT);
+                sl = __hip_atomic_load(&lq.sbuf[(lq.head + lane) & (LQ_CAP - 1)], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
+            }
             lq.head += n;
-            relax_item(a, it, lane < n, false, lq, cur, nb, s_ring, s_lid, s_out, s_ovf, s_slot_delta, s_buf_delta,
+            relax_item(a, sl, nd, lane < n, false, lq, cur, nb, s_ring, s_lid, s_out, s_ovf, s_slot_delta, s_buf_delta,
                        my_items, my_edges, my_ovf_min);
             lq_clamp(lq);
         }
@@ -517,6 +540,8 @@ k_round(RoundArgs a)
             const int incl = wave_scan(cnt);
             const int total = __shfl(incl, 63);
             const int excl = incl - cnt;
+            const uint32_t slot = (uint32_t)(ch / cps);
+            const uint32_t cnode = (uint32_t)(ch - (uint64_t)slot * cps) * CHUNK_ITEMS;
             for (int b0 = 0; b0 < total; b0 += 64) {
                 const int t = b0 + (int)lane;
                 const int j = owner_lane(excl, t);
@@ -524,8 +549,8 @@ k_round(RoundArgs a)
                 const int ej = __shfl(excl, j);
                 if (t < total) {
                     const uint32_t bit = select_bit(wj, (uint32_t)(t - ej));
-                    const uint32_t item = (uint32_t)((ch * 64 + (uint64_t)j) * 32 + bit);
-                    const uint32_t slot = item / N;
+                    const uint32_t node = cnode + (uint32_t)j * 32u + bit;
+                    const uint64_t item = (uint64_t)slot * NP + node;
                     const SlotInfo si = a.slots[slot];
                     bool leaves = true;   // the item leaves the overflow bitmap
                     bool live = false;
@@ -533,7 +558,7 @@ k_round(RoundArgs a)
                     if (si.state == 1) {
                         const uint64_t ku = __hip_atomic_load(&a.keys[item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         if (ku != KEY_INF) {
-                            const uint64_t b = si.base + key_bucket_rel(ku, a.inv_delta, heur(a, si, item - slot * N));
+                            const uint64_t b = si.base + key_bucket_rel(ku, a.inv_delta, heur(a, si, node));
                             if (b >= cur) {
                                 if (b < cur + nb) {
                                     dst_buf = s_ring[b % nb];
@@ -741,7 +766,7 @@ k_control(RoundArgs a, int round_grid)
             bool done = exhausted;
             if (!done && si->t_n <= MS_EARLY_TARGETS) {
                 uint64_t best = ~0ull;
-                const uint64_t *ks = a.keys + (uint64_t)s * (uint64_t)a.N;
+                const uint64_t *ks = a.keys + (uint64_t)s * (uint64_t)a.NP;
                 for (int t = 0; t < si->t_n; ++t) {
                     const orgpu_ms_point tp = a.ms_tgt[si->t_off + t];
                     const uint64_t k = ks[tp.node];
@@ -756,7 +781,7 @@ k_control(RoundArgs a, int round_grid)
             if (done) { si->state = 2; atomicSub(&c->active, 1u); }
             continue;
         }
-        const uint64_t kd = a.keys[(uint64_t)s * (uint64_t)a.N + (uint64_t)si->dst];
+        const uint64_t kd = a.keys[(uint64_t)s * (uint64_t)a.NP + (uint64_t)si->dst];
         const bool reached = kd != KEY_INF;
         if (reached && (exhausted || si->base + key_bucket_rel(kd, a.inv_delta) + margin < newcur)) {
             si->state = 2; atomicSub(&c->active, 1u);
@@ -792,12 +817,13 @@ struct SeedReq {
 __global__ void k_reset_keys(RoundArgs a, const SeedReq *reqs, int n)
 {
     const uint64_t N = (uint64_t)a.N;
+    const uint64_t NP = (uint64_t)a.NP;
     const int r = (int)blockIdx.y;
     if (r >= n) return;
     const uint32_t s = (uint32_t)reqs[r].slot;
     const uint32_t cnt = a.touch_n[s];
     const uint32_t *lst = a.touch_list + (uint64_t)s * a.touch_cap;
-    uint64_t *ks = a.keys + (uint64_t)s * N;
+    uint64_t *ks = a.keys + (uint64_t)s * NP;
     uint32_t *tb = a.touch_bits + (uint64_t)s * a.touch_words;
     for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
         const uint32_t ch = lst[i];
@@ -853,7 +879,7 @@ __global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n, const orgpu_ms_p
             // slot's base (the bucket index starts at 2^32, so this never wraps)
             si->base = c->cur - key_bucket_rel(key_pack(0, 0.0f, KEY_NOPRED), a.inv_delta, heur(a, *si, (uint32_t)r.src));
             si->state = 1;
-            const uint32_t item = (uint32_t)r.slot * (uint32_t)a.N + (uint32_t)r.src;
+            const uint64_t item = (uint64_t)r.slot * (uint64_t)a.NP + (uint64_t)r.src;
             a.keys[item] = key_pack(0, 0.0f, KEY_NOPRED);
             const uint32_t buf = c->ring[c->cur % a.nb];
             si->pending = 1;
@@ -873,7 +899,7 @@ __global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n, const orgpu_ms_p
                 const orgpu_ms_point sp = ms_seed[r.s_off + t];
                 const uint32_t v = (uint32_t)sp.node;
                 if (roi && !((roi[v >> 5] >> (v & 31)) & 1u)) continue;
-                const uint32_t item = (uint32_t)r.slot * (uint32_t)a.N + v;
+                const uint64_t item = (uint64_t)r.slot * (uint64_t)a.NP + v;
                 const uint64_t k = key_pack(0, sp.cost, KEY_NOPRED);
                 const uint64_t old = atomicMin((unsigned long long *)&a.keys[item], (unsigned long long)k);
                 if (old == KEY_INF) touch_key(a, (uint32_t)r.slot, v);
@@ -920,7 +946,7 @@ k_backtrace(RoundArgs a, const int32_t *slot_list, int n, int32_t *stage_nodes, 
     const SlotInfo si = a.slots[s];
     int32_t *pn = stage_nodes + (int64_t)s * max_len;
     int32_t *pe = stage_edges + (int64_t)s * max_len;
-    const uint64_t *ks = a.keys + (uint64_t)s * (uint64_t)a.N;
+    const uint64_t *ks = a.keys + (uint64_t)s * (uint64_t)a.NP;
     if (lane == 0) {
         int len = 0;
         int32_t end = si.dst;
@@ -1001,7 +1027,7 @@ k_backtrace(RoundArgs a, const int32_t *slot_list, int n, int32_t *stage_nodes, 
 __global__ void k_count_minimal(RoundArgs a, int slot, unsigned long long *out)
 {
     const SlotInfo si = a.slots[slot];
-    const uint64_t *ks = a.keys + (uint64_t)slot * (uint64_t)a.N;
+    const uint64_t *ks = a.keys + (uint64_t)slot * (uint64_t)a.NP;
     const uint64_t lim = key_cost(ks[si.dst]);
     unsigned long long nn = 0, ne = 0;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < a.N; v += (int64_t)gridDim.x * blockDim.x) {
@@ -1063,11 +1089,12 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
     if (slots < 1 || slots > MAX_SLOTS) return orgpu_fail(ORGPU_EINVAL, "reserve: slots %d outside [1, %d]", slots, MAX_SLOTS);
     if (n_buckets < 1 || n_buckets > MAX_NB) return orgpu_fail(ORGPU_EINVAL, "reserve: n_buckets %d outside [1, %d]", n_buckets, MAX_NB);
     if (n_nodes < 1 || n_nodes >= KEY_MAX_NODES) return orgpu_fail(ORGPU_EINVAL, "reserve: bad node count");
-    if ((uint64_t)n_nodes * (uint64_t)slots >= (1ull << 32) - 64)
-        return orgpu_fail(ORGPU_EINVAL, "reserve: slots*nodes = %lld exceeds 2^32 work items",
-                          (long long)n_nodes * slots);
     if (max_path < 2) max_path = 1 << 16;
-    const uint64_t items = (uint64_t)n_nodes * (uint64_t)slots;
+    // work items slot * NP + node: NP is n_nodes rounded up to whole chunks
+    const uint64_t np = ((uint64_t)n_nodes + CHUNK_ITEMS - 1) / CHUNK_ITEMS * CHUNK_ITEMS;
+    const uint64_t items = np * (uint64_t)slots;
+    if (items / CHUNK_ITEMS >= (1ull << 32))
+        return orgpu_fail(ORGPU_EINVAL, "reserve: slots*nodes = %llu exceeds 2^43 work items", (unsigned long long)items);
     (void)qcap;   // frontiers are bitmaps: no item queues to size
     if (en.cap_nodes >= n_nodes && en.cap_slots >= slots && en.cap_nb >= n_buckets &&
         en.cap_path >= max_path)
@@ -1290,6 +1317,8 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     RoundArgs a;
     memset(&a, 0, sizeof(a));   // padding included: a captured round graph is reused when the bytes match
     a.ctrl = en.ctrl; a.slots = en.slots; a.K = K; a.N = N; a.nb = (uint32_t)nb;
+    a.NP = (N + CHUNK_ITEMS - 1) / CHUNK_ITEMS * CHUNK_ITEMS;
+    a.cps = (uint32_t)(a.NP / CHUNK_ITEMS);
     a.inv_delta = 1.0f / p->delta;
     a.h_xy = p->h_xy; a.h_z = p->h_z;
     a.local_batches = p->local_batches > 0 ? p->local_batches : 64;
@@ -1298,7 +1327,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     a.roi = roi_bitmaps; a.roi_words = (N + 31) / 32;
     a.ko_owner = keepout_owner;
     a.keys = en.keys;
-    a.bits = en.bits; a.bitmap_words = (uint64_t)((N * (int64_t)K + 31) / 32);
+    a.bits = en.bits; a.bitmap_words = (uint64_t)a.NP * (uint64_t)K / 32;
     a.summ = en.summ; a.summ_words = ((a.bitmap_words + 63) / 64 + 31) / 32;
     a.ms_tgt = ms ? en.ms_tgt : nullptr;
     a.tbits = ms ? en.tbits : nullptr;
@@ -1331,9 +1360,9 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     HIP_CHECK_RET(hipMemsetAsync(en.summ, 0, (size_t)(nb + 3) * a.summ_words * sizeof(uint32_t), s));
     HIP_CHECK_RET(hipMemsetAsync(en.list_cnt, 0, (size_t)(nb + 4) * LIST_BINS * sizeof(uint32_t), s));
     // Keys are reset per slot from the touch records of its last search; those
-    // records describe the layout slot*N of the node count they were made for.
+    // records describe the layout slot*NP of the node count they were made for.
     if (en.keys_n != N) {
-        hipLaunchKernelGGL(k_fill_keys, dim3(4096), dim3(256), 0, s, en.keys, (int64_t)en.cap_slots * N);
+        hipLaunchKernelGGL(k_fill_keys, dim3(4096), dim3(256), 0, s, en.keys, (int64_t)en.cap_slots * a.NP);
         HIP_CHECK_RET(hipGetLastError());
         HIP_CHECK_RET(hipMemsetAsync(en.touch_n, 0, sizeof(uint32_t) * en.cap_slots, s));
         HIP_CHECK_RET(hipMemsetAsync(en.touch_bits, 0, sizeof(uint32_t) * (size_t)en.cap_slots * a.touch_words, s));

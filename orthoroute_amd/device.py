@@ -26,10 +26,36 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
-def max_slots(n_nodes):
-    """Concurrent searches a graph of n_nodes allows: work items are the u32
-    slot*N + node, and the engine holds at most 1024 slots."""
-    return max(1, min(1024, ((1 << 32) - 64) // max(int(n_nodes), 1) - 1))
+MAX_SLOTS = 2048       # concurrent searches one engine holds
+CHUNK_ITEMS = 2048     # a search's keys and bitmap bits are laid out per whole chunk
+_DEVICE_BYTES = None
+
+
+def device_memory():
+    """Total memory of the current device in bytes (0 without a GPU)."""
+    global _DEVICE_BYTES
+    if _DEVICE_BYTES is None:
+        import torch
+        _DEVICE_BYTES = int(torch.cuda.mem_get_info()[1]) if torch.cuda.is_available() else 0
+    return _DEVICE_BYTES
+
+
+def slot_bytes(n_nodes, n_buckets=8):
+    """Engine memory one concurrent search takes on a graph of n_nodes: an
+    8-byte key and n_buckets + 3 bucket-bitmap bits per node (the per-slot
+    stride rounds n_nodes up to a whole chunk), and its touched-chunk
+    records."""
+    np_ = -(-int(n_nodes) // CHUNK_ITEMS) * CHUNK_ITEMS
+    return 8 * np_ + (int(n_buckets) + 3) * np_ // 8 + 8 * (np_ // 1024 + 1)
+
+
+def max_slots(n_nodes, budget_bytes=None, n_buckets=8):
+    """Concurrent searches a graph of n_nodes allows: as many as the
+    engine's slot memory (slot_bytes each) fits in budget_bytes (default 70 %
+    of the device's memory), at most MAX_SLOTS."""
+    if budget_bytes is None:
+        budget_bytes = 0.7 * device_memory()
+    return max(1, min(MAX_SLOTS, int(budget_bytes // slot_bytes(n_nodes, n_buckets))))
 
 
 class Context:

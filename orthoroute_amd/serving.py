@@ -110,10 +110,11 @@ def footprint(X, Y, L, policy, n_nets, slots, roi_limit):
     return int(1.05 * (csr + state + search + roi)) + (64 << 20)
 
 
-def choose_slots(job, cap=1024):
-    """Concurrent searches for a job: all its nets at once when the graph
-    allows it (u32 work items), at most `cap`."""
-    return max(1, min(int(cap), len(job.nets), max_slots(job.nodes)))
+def choose_slots(job, cap=1024, budget_bytes=None):
+    """Concurrent searches for a job: all its nets at once when the slot
+    memory (device.slot_bytes per search) fits budget_bytes (default 70 % of
+    the device's memory), at most `cap`."""
+    return max(1, min(int(cap), len(job.nets), max_slots(job.nodes, budget_bytes)))
 
 
 def run_job(job, device=0, slots=None, roi_limit=1 << 30, stream=None):

@@ -56,11 +56,17 @@ def test_footprint_grows_with_board_nets_and_slots():
     assert footprint(64, 64, 6, "adjacent", 100, 64, 1 << 30) - base >= 48 * 8 * 64 * 64 * 6
 
 
-def test_choose_slots_respects_work_item_limit():
+def test_choose_slots_respects_memory_budget():
+    from orthoroute_amd.device import max_slots, slot_bytes
     small = _job(0, n=30)
-    assert choose_slots(small) == 30
-    big = Job(1, 2000, 2000, 24, [("a", 0, 1)] * 2000)   # 96 M nodes: 44 slots of u32 work items
-    assert choose_slots(big) == (2 ** 32 - 64) // big.nodes - 1
+    assert choose_slots(small, budget_bytes=16 << 30) == 30
+    big = Job(1, 2000, 2000, 24, [("a", 0, 1)] * 2000)   # 96 M nodes
+    k = choose_slots(big, budget_bytes=64 << 30)
+    assert k == max_slots(big.nodes, 64 << 30) and 1 < k < 2000
+    # keys (8 B) and 11 bucket-bitmap bits per node and search, per whole chunk of 2048 nodes
+    assert k * slot_bytes(big.nodes) <= 64 << 30 < (k + 1) * slot_bytes(big.nodes)
+    assert slot_bytes(1) == slot_bytes(2048) and slot_bytes(2049) > slot_bytes(2048)
+    assert max_slots(10, budget_bytes=1 << 50) == 2048
 
 
 def test_admission_budget_slots_and_fifo():
