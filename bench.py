@@ -92,6 +92,8 @@ def parse():
                          "earlier run with --save-minimal-counts) instead of an untimed counting pass; with "
                          "--warmup 0 --plain-sample 0 the timed step's launches are then the only SSSP launches")
     ap.add_argument("--save-minimal-counts", default="", metavar="JSON")
+    ap.add_argument("--access-probe", type=int, default=0, metavar="N",
+                    help="print the device's random 8-byte load and 64-bit atomicMin rates over N accesses and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run 5 plain device copies of BYTES bytes (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -110,6 +112,32 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": nbytes, "launches": 5}), flush=True)
 
 
+def access_probe(n):
+    """Random-access rates of the device: n 8-byte loads (gather) and n
+    64-bit atomic minimums (scatter_reduce amin) at uniformly random indices
+    of arrays from 1 MiB to 1 GiB; prints one JSON line per array size."""
+    import torch
+    dev = "cuda"
+    g = torch.Generator(device=dev)
+    g.manual_seed(1)
+    for nbytes in (1 << 20, 8 << 20, 64 << 20, 1 << 30):
+        m = nbytes // 8
+        keys = torch.randint(0, 1 << 62, (m,), device=dev, dtype=torch.int64, generator=g)
+        idx = torch.randint(0, m, (n,), device=dev, dtype=torch.int64, generator=g)
+        vals = torch.randint(0, 1 << 62, (n,), device=dev, dtype=torch.int64, generator=g)
+        out = {"array_bytes": nbytes, "accesses": n}
+        for label, fn in (("gather", lambda: keys[idx]),
+                          ("atomic_min", lambda: keys.scatter_reduce_(0, idx, vals, "amin"))):
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                fn()
+            torch.cuda.synchronize()
+            out[label + "_G_per_s"] = round(3 * n / (time.perf_counter() - t0) / 1e9, 2)
+        print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -121,6 +149,9 @@ def main():
         sys.exit("--steps must be >= 1")
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
+        return
+    if args.access_probe > 0:
+        access_probe(args.access_probe)
         return
 
     import numpy as np
