@@ -53,6 +53,7 @@ class Config:
         self.grid_pitch = 0.4
         self.via_cost = 0.7
         self.portal_enabled = True
+        self.multi_terminal_enabled = False
         self.portal_delta_pref = 6
         self.portal_retarget_patience = 3
         self.stagnation_patience = 5
@@ -242,6 +243,10 @@ class OracleRouter:
         self.portals = {}
         self.net_pad_ids = {}
         self.net_portal_failures = defaultdict(int)
+        # tree nets (cfg.multi_terminal_enabled): pins, and per routed tree
+        # its branches (node lists)
+        self.tree_pins = {}
+        self.tree_branches = {}
 
     # ---------------- helpers
     def coords(self, n):
@@ -370,6 +375,11 @@ class OracleRouter:
                         if key not in self.keepouts:
                             self.keepouts[key] = net_id
 
+    def segments(self, name, path):
+        """The node paths of a committed net: a tree's branches, else its
+        path."""
+        return self.tree_branches.get(name, [path])
+
     def rebuild_via_usage(self):
         self.via_col_use.fill(0)
         if self.has_seg:
@@ -380,7 +390,8 @@ class OracleRouter:
             self.keepouts.update(keep)
         for nid, path in self.net_paths.items():
             if path and len(path) > 1:
-                self.accumulate_via_usage(path, nid)
+                for seg in self.segments(nid, path):
+                    self.accumulate_via_usage(seg, nid)
         self.rebuild_node_owner()
 
     def rebuild_node_owner(self):
@@ -389,7 +400,8 @@ class OracleRouter:
             if not path or len(path) < 2:
                 continue
             nid = self.get_net_id(name)
-            self._mark_barrels(path, nid)
+            for seg in self.segments(name, path):
+                self._mark_barrels(seg, nid)
         if self.keepouts:
             for (z, x, y), owner in self.keepouts.items():
                 if z <= 0:
@@ -420,7 +432,7 @@ class OracleRouter:
             if not path or len(path) < 2:
                 continue
             nid = self.get_net_id(name)
-            e = self.path_to_edges(path)
+            e = self.net_to_edges[name] if name in self.tree_branches else self.path_to_edges(path)
             all_e.extend(e); all_n.extend([nid] * len(e))
         if not all_e:
             return np.array([], np.int32), 0
@@ -509,6 +521,7 @@ class OracleRouter:
             self.edge_to_nets[e].add(nid)
 
     def clear_tracking(self, nid):
+        self.tree_branches.pop(nid, None)
         if nid in self.net_to_edges:
             for e in self.net_to_edges[nid]:
                 self.edge_to_nets[e].discard(nid)
@@ -568,6 +581,26 @@ class OracleRouter:
                 old = self.net_to_edges[nid] if nid in self.net_to_edges else self.path_to_edges(self.net_paths[nid])
                 self.clear_path(old)
                 self.clear_tracking(nid)
+            if nid in self.tree_pins:
+                # a tree net: grown from its first pin by canonical
+                # multi-source searches on the iteration's costs (whole
+                # graph, no keepouts, no fallback); committed only when every
+                # pin is reached
+                branches, _, unreached, nodes = oracle.steiner_canon(self.lat, costs, self.tree_pins[nid])
+                if unreached:
+                    self.net_paths[nid] = []
+                    self.clear_tracking(nid)
+                else:
+                    edges = [e for b in branches for e in self.path_to_edges(b)]
+                    self.commit_path(edges)
+                    self.net_paths[nid] = list(nodes)
+                    gid = self.get_net_id(nid)
+                    for b in branches:
+                        self._mark_barrels(b, gid)
+                    self.update_tracking(nid, edges)
+                    self.tree_branches[nid] = [list(b) for b in branches]
+                rec.append((nid, False))
+                continue
             sx, sy, _ = self.coords(src)
             dx, dy, _ = self.coords(dst)
             md = abs(dx - sx) + abs(dy - sy)
@@ -733,8 +766,28 @@ class OracleRouter:
         return self.route_multiple_nets(reqs, max_iterations, iteration_cb)
 
     def route_multiple_nets(self, requests, max_iterations=None, iteration_cb=None):
+        """requests: (name, src, dst); with cfg.multi_terminal_enabled also
+        (name, [pins]), a tree net of its distinct pins (a task of its first
+        two pins for the schedule's ordering and analysis)."""
         tasks = {}
-        for name, s, d in requests:
+        for req in requests:
+            if len(req) == 2:
+                if not self.cfg.multi_terminal_enabled:
+                    continue
+                name, nodes = req
+                pins = []
+                for v in nodes:
+                    if int(v) not in pins:
+                        pins.append(int(v))
+                if len(pins) > 2:
+                    self.tree_pins[name] = pins
+                    tasks[name] = (pins[0], pins[1])
+                elif len(pins) == 2:
+                    tasks[name] = (pins[0], pins[1])
+                elif pins:
+                    self.net_paths[name] = [pins[0]]
+                continue
+            name, s, d = req
             if s != d:
                 tasks[name] = (int(s), int(d))
             else:
@@ -933,7 +986,8 @@ class OracleRouter:
         conflict = set(int(e) for e in np.nonzero(over > 0)[0])
         nets = set()
         for n, path in self.net_paths.items():
-            if path and any(e in conflict for e in self.path_to_edges(path)):
+            e = self.net_to_edges.get(n, []) if n in self.tree_branches else (self.path_to_edges(path) if path else [])
+            if path and any(x in conflict for x in e):
                 nets.add(n)
         if not nets:
             return {"success": False, "error_code": "NO-CONFLICT-NETS"}

@@ -44,6 +44,7 @@ from ...lattice import DIR_H, DIR_HV, DIR_V, layer_directions, reference_via_pai
 from ...reductions import sum_f32
 from .board_analyzer import analyze_board_characteristics
 from .pad_escape_planner import PadEscapePlanner, Portal  # noqa: F401  (Portal: the reference's export)
+from ...steiner import grow_trees
 from .parameter_derivation import apply_derived_parameters, derive_routing_parameters
 
 logger = logging.getLogger(__name__)
@@ -433,6 +434,11 @@ class PathFinderRouter:
         self.net_clean_streak: Dict[str, int] = defaultdict(int)
         self._net_to_edges: Dict[str, np.ndarray] = {}
         self._net_nodes: Dict[str, np.ndarray] = {}
+        # multi-terminal nets routed as trees inside the negotiation
+        # (multi_terminal_enabled): pins per net, and per routed tree its
+        # branches (node arrays) and their edge arrays
+        self._tree_pins: Dict[str, List[int]] = {}
+        self._tree_branches: Dict[str, Tuple[list, list]] = {}
         self._last_depth_hint = None               # bucket sizing (speed only)
         self._ko_nodes = np.zeros(0, np.int64)      # via keepouts: node, first owner
         self._ko_owner = np.zeros(0, np.int64)
@@ -616,10 +622,18 @@ class PathFinderRouter:
     # ---------------- requests
     def route_multiple_nets(self, requests: List, progress_cb=None, iteration_cb=None) -> Dict:
         """UPF:2530-2598.  Nets with more than two pads are set aside as
-        multi-terminal nets; they are routed (as their first two pads, the
-        reference's fallback once its multi-terminal router fails) only when
-        config.multi_terminal_enabled is set, and otherwise not at all, as in
-        the reference, whose PathFinderConfig has no such attribute."""
+        multi-terminal nets and not routed, as in the reference, whose
+        PathFinderConfig has no multi_terminal_enabled attribute.  With
+        config.multi_terminal_enabled set they are routed inside the
+        negotiation as trees (as are requests (name, [three or more pin
+        nodes])): each iteration a scheduled tree net is ripped up and regrown
+        on the iteration's costs by multi-source searches from the tree to
+        its unconnected pins (orthoroute_amd.steiner), its edges are committed
+        like a path's, each branch marks its via barrels, and net_paths holds
+        the tree's nodes in first-seen order.  A tree that cannot reach every
+        pin counts as a failed net.  Trees search the whole graph without
+        keepouts and take no wider-ROI fallback (the reference defines no
+        multi-terminal search that runs; this is this build's definition)."""
         multi = [q for q in requests if hasattr(q, "pads") and len(q.pads) > 2]
         if multi and not getattr(self.config, "multi_terminal_enabled", False):
             requests = [q for q in requests if not (hasattr(q, "pads") and len(q.pads) > 2)]
@@ -642,9 +656,37 @@ class PathFinderRouter:
 
     def _parse_requests(self, requests):
         """UPF:2600-2676: a Board net's task is its first two pads' nodes, or
-        with portals on both pads the portals' entry-layer nodes."""
+        with portals on both pads the portals' entry-layer nodes.  A tree net
+        (multi_terminal_enabled: more than two distinct pad nodes, or a
+        request (name, [pins])) is a task of its first two pins, with its
+        pins kept for the tree search."""
         tasks = {}
+        trees_on = bool(getattr(self.config, "multi_terminal_enabled", False))
         for req in requests:
+            if trees_on and hasattr(req, "name") and hasattr(req, "pads") and len(req.pads) > 2:
+                pins = []
+                for pad in req.pads:
+                    k = self._pad_key(pad)
+                    if k in self.pad_to_node and self.pad_to_node[k] not in pins:
+                        pins.append(int(self.pad_to_node[k]))
+                if len(pins) > 2:
+                    self._tree_pins[req.name] = pins
+                    tasks[req.name] = (pins[0], pins[1])
+                    continue
+            if trees_on and isinstance(req, tuple) and len(req) == 2 and not isinstance(req[1], (int, np.integer)):
+                name, nodes = req
+                pins = []
+                for v in nodes:
+                    if int(v) not in pins:
+                        pins.append(int(v))
+                if len(pins) > 2:
+                    self._tree_pins[name] = pins
+                    tasks[name] = (pins[0], pins[1])
+                elif len(pins) == 2:
+                    tasks[name] = (pins[0], pins[1])
+                elif pins:
+                    self.net_paths[name] = [pins[0]]
+                continue
             if hasattr(req, "name") and hasattr(req, "pads"):
                 if len(req.pads) < 2:
                     continue
@@ -759,8 +801,14 @@ class PathFinderRouter:
         path changes: the cache holds the (name, path, edges) objects it was
         built from and is reused only when every one is the same object."""
         items = []
+        trees = getattr(self, "_tree_branches", {})
         for name, path in self.net_paths.items():
             if not path or len(path) < 2:
+                continue
+            tb = trees.get(name)
+            if tb is not None:
+                # a tree: each branch is a path of the net
+                items.extend((name, b, b, e) for b, e in zip(*tb) if len(b) > 1)
                 continue
             items.append((name, path, self._net_nodes.get(name), self._net_to_edges.get(name)))
         cache = getattr(self, "_flat_cache", None)
@@ -910,6 +958,7 @@ class PathFinderRouter:
     def _clear_net_edge_tracking(self, nid):
         self._net_to_edges.pop(nid, None)
         self._net_nodes.pop(nid, None)
+        getattr(self, "_tree_branches", {}).pop(nid, None)
 
     # ---------------- ordering / hotset / rip (UPF:4401-4439, 5009-5140, 5300-5335)
     def _dense(self, idx, val, slot=None):
@@ -1241,6 +1290,36 @@ class PathFinderRouter:
                 out[i] = (nodes[a:b], edges[a:b - 1])
         return out
 
+    def _route_trees(self, names):
+        """Grows the trees of the tree nets `names` on this iteration's costs
+        (orthoroute_amd.steiner.grow_trees: every net still growing takes one
+        multi-source step per engine call, whole graph, no keepouts).
+        Returns {name: (branches, branch edge arrays, tree nodes) or None when
+        a pin stays unreached}."""
+        if not names:
+            return {}
+        slots = max(1, min(int(self.config.batch_slots), len(names), max_slots(self.N)))
+        t0 = time.perf_counter()
+        grown, st = grow_trees(self.ctx, self.graph.dlat, self.accounting.total_cost,
+                               [self._tree_pins[n] for n in names], self._search_params(slots))
+        self.stats["tree_s"] += time.perf_counter() - t0
+        self.stats["sssp_searches"] += st["searches"]
+        self.stats["sssp_edges"] += st["edges"]
+        self.stats["sssp_items"] += st["items"]
+        out = {}
+        for n, t in zip(names, grown):
+            if t["unreached"]:
+                out[n] = None
+                continue
+            branches = [np.asarray(b, dtype=np.int32) for b in t["branches"]]
+            bedges, off = [], 0
+            for b in branches:
+                k = len(b) - 1
+                bedges.append(np.asarray(t["edges"][off:off + k], dtype=np.int64))
+                off += k
+            out[n] = (branches, bedges, np.asarray(t["nodes"], dtype=np.int32))
+        return out
+
     def _roi_has(self, rows, row_idx, nodes):
         """Membership bits of nodes[k] in ROI row row_idx[k] (bool array)."""
         torch = self.ctx.torch
@@ -1319,6 +1398,10 @@ class PathFinderRouter:
         if over_map is None:
             over_map = np.zeros(self.accounting.E, dtype=np.float32)
         total = len(tasks)
+        # tree nets of this call (multi_terminal_enabled): grown below, not
+        # searched as two-pin nets
+        tree_pins = getattr(self, "_tree_pins", {})
+        trees = {nm for nm in tasks if nm in tree_pins}
         use_ko = total < 100 and len(self._ko_nodes) > 0
         ko_owner = self._keepout_owner_array() if use_ko else None
         # The searches of an iteration read one cost snapshot, so their
@@ -1340,7 +1423,7 @@ class PathFinderRouter:
                     box["error"] = exc
             helper = threading.Thread(target=order_job, name="net-order")
             helper.start()
-            names = list(tasks)
+            names = [nm for nm in tasks if nm not in trees]
             st = np.fromiter((v for nid in names for v in tasks[nid]), dtype=np.int64,
                              count=2 * len(names)).reshape(-1, 2)
             X_, plane_ = self._Nx, self._Nx * self._Ny
@@ -1350,7 +1433,7 @@ class PathFinderRouter:
             search_order = [names[i] for i in np.argsort(-dist_, kind="stable").tolist()]
         else:
             ordered = self._order_nets_by_difficulty(tasks, over_map)
-            search_order = ordered
+            search_order = [nm for nm in ordered if nm not in trees] if trees else ordered
         n = len(search_order)
         sd = np.fromiter((v for nid in search_order for v in tasks[nid]), dtype=np.int64,
                          count=2 * n).reshape(n, 2)
@@ -1395,10 +1478,25 @@ class PathFinderRouter:
             if "error" in box:
                 raise box["error"]
             ordered = box["ordered"]
+        if helper is not None or trees:
+            # results and requests in routing order (a tree net's entry is
+            # filled by the tree search)
             pos = {nm: i for i, nm in enumerate(search_order)}
-            perm = np.fromiter((pos[nm] for nm in ordered), dtype=np.int64, count=n)
-            results = [results[i] for i in perm.tolist()]
-            reqs = reqs[perm]
+            rows = np.full((len(ordered), reqs.shape[1]), -1.0)
+            full = []
+            for j, nm in enumerate(ordered):
+                i = pos.get(nm)
+                if i is None:
+                    full.append(None)
+                else:
+                    full.append(results[i])
+                    rows[j] = reqs[i]
+            results, reqs = full, rows
+        if trees:
+            grown = self._route_trees([nm for nm in ordered if nm in trees])
+            for j, nm in enumerate(ordered):
+                if nm in trees:
+                    results[j] = grown[nm]
         # a portal net whose portal search failed searches its task's nodes
         # in the same ROI (UPF:4740-4748)
         if portal:
@@ -1412,7 +1510,7 @@ class PathFinderRouter:
         # wider-ROI fallback for the first five failures, in routing order;
         # a portal net's grows from its entry nodes without its seeds and
         # searches seed to seed (UPF:4753-4781)
-        fb = [i for i, r in enumerate(results) if r is None or len(r[0]) <= 1][:5]
+        fb = [i for i, r in enumerate(results) if (r is None or len(r[0]) <= 1) and ordered[i] not in trees][:5]
         self.full_graph_fallback_count = len(fb)
         if fb:
             freqs = np.array([(reqs[i, 0], reqs[i, 1], -1, 1, bonus * 2, reqs[i, 5], reqs[i, 6], -1, -1, -1, -1)
@@ -1426,12 +1524,29 @@ class PathFinderRouter:
         # _update_net_edge_tracking, inlined: this loop runs over every net)
         dec, inc, marked, mark_ids = [], [], [], []
         paths, n2e, n2n, get_id = self.net_paths, self._net_to_edges, self._net_nodes, self._get_net_id
+        tb = self.__dict__.setdefault("_tree_branches", {})
         for nid, r in zip(ordered, results):
             if paths.get(nid):
                 old = n2e.pop(nid, None)
                 if old is not None:
                     dec.append(old)
                 n2n.pop(nid, None)
+                tb.pop(nid, None)
+            if nid in trees:
+                if r is None:
+                    paths[nid] = []
+                    continue
+                branches, bedges, nodes = r
+                edges = np.concatenate(bedges) if bedges else np.zeros(0, np.int64)
+                inc.append(edges)
+                paths[nid] = nodes.view(NodePath)
+                nid_int = get_id(nid)
+                for b in branches:
+                    marked.append(b)
+                    mark_ids.append(nid_int)
+                n2e[nid] = edges
+                tb[nid] = (branches, bedges)
+                continue
             if r is not None and len(r[0]) > 1:
                 path, edges = r
                 inc.append(edges)
@@ -1452,6 +1567,8 @@ class PathFinderRouter:
             plane = self._Nx * self._Ny
             patience = int(self.config.portal_retarget_patience)
             for nid, r in zip(ordered, results):
+                if nid in trees:
+                    continue
                 if r is not None and len(r[0]) > 1:
                     if nid in portal:
                         self.net_portal_layers[nid] = (int(r[0][0]) // plane, int(r[0][-1]) // plane)
@@ -1806,8 +1923,18 @@ class PathFinderRouter:
         per net and recomputed only for nets whose path object changed since
         the last call (a late negotiation iteration reroutes a hundred of
         thousands of nets); the records of an unchanged net are the same
-        objects as before."""
-        items = [(net, p) for net, p in self.net_paths.items() if p]
+        objects as before.  A tree net's records are its branches' records,
+        branch after branch."""
+        items = []      # (cache key, net, path)
+        trees = getattr(self, "_tree_branches", {})
+        for net, p in self.net_paths.items():
+            if not p:
+                continue
+            tb = trees.get(net)
+            if tb is None:
+                items.append((net, net, p))
+            else:
+                items.extend((("branch", net, i), net, b) for i, b in enumerate(tb[0]))
         if not items:
             return [], []
         g = self.lattice.geom
@@ -1818,14 +1945,15 @@ class PathFinderRouter:
         if cache is None or cache[0] != sig:
             cache = (sig, {})
         per = cache[1]
-        todo = [(net, p) for net, p in items if net not in per or per[net][0] is not p]
+        todo = [(key, net, p) for key, net, p in items if key not in per or per[key][0] is not p]
         if todo:
-            for (net, p), rec in zip(todo, self._geometry_records(todo)):
-                per[net] = (p, rec[0], rec[1])
-        keep = {net: per[net] for net, _ in items}
+            recs = self._geometry_records([(net, p) for _, net, p in todo])
+            for (key, _, p), rec in zip(todo, recs):
+                per[key] = (p, rec[0], rec[1])
+        keep = {key: per[key] for key, _, _ in items}
         self._geo_cache = (sig, keep)
-        tracks = [t for net, _ in items for t in keep[net][1]]
-        vias = [v for net, _ in items for v in keep[net][2]]
+        tracks = [t for key, _, _ in items for t in keep[key][1]]
+        vias = [v for key, _, _ in items for v in keep[key][2]]
         return tracks, vias
 
     def _geometry_records(self, items):
