@@ -71,6 +71,9 @@ def parse():
     ap.add_argument("--cfg2-iterations", type=int, default=8,
                     help="iteration cap of the full backplane negotiation timed batched and plain (0: skip)")
     ap.add_argument("--cfg2-plain", type=int, default=1, help="also run the backplane negotiation plain (0: no)")
+    ap.add_argument("--cfg3-tree-iterations", type=int, default=5,
+                    help="iteration cap of the cfg3 negotiation with its multi-terminal nets as trees (0: skip)")
+    ap.add_argument("--cfg3-tree-plain", type=int, default=1, help="also run that negotiation plain (0: no)")
     ap.add_argument("--cfg3-iterations", type=int, default=5,
                     help="iteration cap of cfg3's 1,400 two-pin nets negotiated batched and plain (0: skip)")
     ap.add_argument("--ref-iterations", type=int, default=30,
@@ -268,6 +271,8 @@ def main():
     torch.cuda.empty_cache()
     neg2 = negotiation(args, "cfg2", args.cfg2_iterations, bool(args.cfg2_plain)) if args.cfg2_iterations > 0 else None
     neg3 = negotiation(args, "cfg3", args.cfg3_iterations, True) if args.cfg3_iterations > 0 else None
+    neg3t = negotiation(args, "cfg3", args.cfg3_tree_iterations, bool(args.cfg3_tree_plain), trees=True) \
+        if args.cfg3_tree_iterations > 0 else None
     refo = reference_order(args) if args.ref_iterations > 0 else None
     mt = multi_terminal(args) if args.mt_nets > 0 else None
     ttz = time_to_zero_overuse(args) if args.ttz_nets > 0 else None
@@ -343,6 +348,7 @@ def main():
         "cost_kernels": costk,
         "negotiation_cfg2": neg2,
         "negotiation_cfg3": neg3,
+        "negotiation_cfg3_trees": neg3t,
         "reference_order": refo,
         "multi_terminal": mt,
         "time_to_zero_overuse": ttz,
@@ -404,17 +410,24 @@ def cost_kernels(r, pres_fac, hmult, reps):
     return out
 
 
-def negotiation(args, which, cap, with_plain):
+def negotiation(args, which, cap, with_plain, trees=False):
     """route_multiple_nets on a whole board from entry to exit (lattice and
     cost state resident), batched and, if asked, plain (one slot: searches one
     after another), with the iteration cap `cap`: time to zero overuse when
     the schedule gets there, else the time to the cap; both forms must give
     the same per-iteration overuse and final paths.  cfg2: the backplane's
-    8,000 nets; cfg3: the BGA board's 1,400 two-pin ball-to-perimeter nets."""
+    8,000 nets; cfg3: the BGA board's 1,400 two-pin ball-to-perimeter nets,
+    with trees=True also its 200 multi-terminal nets as trees inside the
+    negotiation (one negotiated board)."""
     import torch
     from orthoroute_amd import workloads
     from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig, PathFinderRouter
     X, Y, L, reqs = workloads.CONFIGS[which]()
+    n_two, n_tree = len(reqs), 0
+    if trees:
+        mt = workloads.cfg3_multi()[3]
+        reqs = list(reqs) + list(mt)
+        n_tree = len(mt)
 
     def run(slots):
         cfg = PathFinderConfig()
@@ -422,6 +435,7 @@ def negotiation(args, which, cap, with_plain):
         cfg.batch_slots = slots
         cfg.max_iterations_cap = cap
         cfg.sssp_graph_rounds = args.graph_rounds
+        cfg.multi_terminal_enabled = trees
         r = PathFinderRouter(cfg)
         r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
         torch.cuda.synchronize()
@@ -436,6 +450,7 @@ def negotiation(args, which, cap, with_plain):
         dt = time.perf_counter() - t0
         paths = {nm: list(map(int, res["paths"].get(nm, []))) for nm, *_ in reqs}
         stats = dict(r.stats)
+        r.ctx.close()
         del r
         torch.cuda.empty_cache()
         return dt, res, rec, t_it, paths, stats
@@ -443,13 +458,16 @@ def negotiation(args, which, cap, with_plain):
     zero = next((t for (it, s, c), t in zip(rec, t_it) if s == 0), None)
     log(f"[bench] {which} negotiation batched: {len(rec)} iterations in {dt:.2f}s, overuse at exit {rec[-1][1]}, "
         f"zero overuse at {zero}")
-    out = {"workload": f"{which} {X}x{Y}x{L}, adjacent-layer vias, {len(reqs)} two-pin nets, iteration cap {cap}",
+    kind = f"{n_two} two-pin nets" + (f" and {n_tree} multi-terminal nets as trees" if n_tree else "")
+    out = {"workload": f"{which} {X}x{Y}x{L}, adjacent-layer vias, {kind}, iteration cap {cap}",
            "iterations": len(rec), "time_to_zero_overuse_s": None if zero is None else round(zero, 3),
            "reached_zero_overuse": zero is not None, "converged": bool(res.get("converged")),
            "time_to_exit_s": round(dt, 3), "overuse_at_exit": int(rec[-1][1]),
            "overuse_per_iteration": [int(x[1]) for x in rec],
            "iteration_s": [round(b - a, 3) for a, b in zip([0.0] + t_it[:-1], t_it)],
            "sssp_s": round(stats.get("sssp_s", 0.0), 3), "roi_s": round(stats.get("roi_s", 0.0), 3),
+           "tree_s": round(stats.get("tree_s", 0.0), 3),
+           "routed_trees": sum(1 for q in reqs[n_two:] if res["paths"].get(q[0])),
            "searches": int(stats.get("sssp_searches", 0))}
     if with_plain:
         pdt, pres, prec, pt_it, ppaths, _ = run(1)
