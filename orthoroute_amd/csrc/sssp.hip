@@ -931,7 +931,8 @@ __global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n, const orgpu_ms_p
 // fields back to the source into the staging row (one dependent key load per
 // hop); then the lanes find each hop's CSR edge id (first match in the sorted
 // row, UPF:5463-5476) and pack the path src -> dst into the shared output.
-// len = 0: unreachable; -1: broken chain or longer than the staging row.
+// len = 0: unreachable; -1: broken chain or longer than the staging row;
+// -2: the path crosses 31 or more keepout edges (the key saturates there).
 __global__ void __launch_bounds__(64)
 k_backtrace(RoundArgs a, const int32_t *slot_list, int n, int32_t *stage_nodes, int32_t *stage_edges,
             int32_t *lens, float *dists, int32_t max_len, double *ms_total, int32_t *ms_choice,
@@ -977,6 +978,11 @@ k_backtrace(RoundArgs a, const int32_t *slot_list, int n, int32_t *stage_nodes, 
         }
         if (!found) {
             dists[s] = __builtin_inff();
+        } else if (key_ko(ks[end]) >= 31u) {
+            // the key's keepout count saturates at 31: past it, paths would
+            // no longer be ordered by their number of keepout edges
+            dists[s] = key_dist(ks[end]);
+            len = -2;
         } else {
             dists[s] = key_dist(ks[end]);
             int32_t cur = end;
@@ -1475,6 +1481,9 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
         HIP_CHECK_RET(hipStreamSynchronize(s));
         for (int i = 0; i < nf; ++i) {
             const int k = en.h_slot_list[i];
+            if (en.h_lens[k] == -2)
+                return orgpu_fail(ORGPU_ECAPACITY, "route: request %d: its path crosses 31 or more keepout edges, "
+                                  "which the search key cannot count", slot_req[k]);
             if (en.h_lens[k] < 0)
                 return orgpu_fail(ORGPU_ECAPACITY, "route: request %d: predecessor chain broken or longer than %d",
                                   slot_req[k], en.cap_path);
