@@ -403,6 +403,31 @@ int orgpu_history_update(orgpu_t *h, int64_t n_edges, const float *present, cons
 int orgpu_overuse_sparse(orgpu_t *h, int64_t n_edges, const float *present, const float *history,
                          int mode, float hist_scale, float threshold, orgpu_sparse_out *out);
 
+/* Via pooling pressure (UPF:3626-3637), per via column c of n_cols and per
+ * segment z of its seg_z: col_pres[c] = 0.6*max(0, col_use[c] - col_cap) +
+ * 0.4*col_pres[c], the same for seg_pres[c*seg_z + z] with seg_cap, and
+ * seg_prefix = the running sum of seg_pres along z (float32, in the order
+ * numpy's float32 expressions and cumsum take).  *over (device u64) is set
+ * to the sum of max(0, use - cap) over columns and segments (the via part of
+ * the overuse sum, UPF:838-876).  All arrays are device memory in
+ * [column][segment] order.  Asynchronous on the handle's stream. */
+int orgpu_via_pressure(orgpu_t *h, int64_t n_cols, int32_t seg_z, const int16_t *col_use, int32_t col_cap,
+                       const int16_t *seg_use, int32_t seg_cap, float *col_pres, float *seg_pres,
+                       float *seg_prefix, unsigned long long *over);
+
+/* Per committed path, the sum of a per-edge float32 quantity over the path's
+ * edge list in path order, in float64, one addition after another from 0.0:
+ * the value of Python's sum(float(v[e]) for e in edges) in the impact scores
+ * of _build_hotset (UPF:5060-5072: the values outside the hot set are zero
+ * here, and adding +0.0 leaves such a sum unchanged) and
+ * _rip_top_k_offenders (UPF:5300-5335).  Path s covers
+ * edges[offs[s] .. offs[s+1]); an edge id outside [0, n_edges) makes its
+ * path's sum NaN (it is not read).  dense: device f32[n_edges]; edges:
+ * device i32; offs: device i64[n_seg+1], non-decreasing; out: device
+ * f64[n_seg].  Asynchronous on the handle's stream. */
+int orgpu_segment_sums(orgpu_t *h, int64_t n_edges, const float *dense, const int32_t *edges,
+                       const int64_t *offs, int64_t n_seg, double *out);
+
 /* Lower bounds for the goal-directed search (orgpu_sssp_params.h_xy/h_z):
  * out[0] = min cost over planar edges, out[1] = min over via edges of
  * cost / layer span, both over finite non-negative costs (device f32[2],

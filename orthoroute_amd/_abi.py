@@ -23,7 +23,7 @@ EXPORTED = [
     "orgpu_usage_scatter", "orgpu_present_build", "orgpu_present_ema", "orgpu_cost_update",
     "orgpu_history_update", "orgpu_overuse_sparse", "orgpu_cost_bounds", "orgpu_route_ms",
     "orgpu_set_roi_scratch", "orgpu_roi_lattice", "orgpu_reserve_ref", "orgpu_route_ref",
-    "orgpu_route_ref_noalloc",
+    "orgpu_route_ref_noalloc", "orgpu_segment_sums", "orgpu_via_pressure",
 ]
 
 
@@ -159,6 +159,8 @@ def load():
         "orgpu_cost_bounds": [P, ctypes.POINTER(Graph), P, P, P],
         "orgpu_history_update": [P, i64, P, P, P, f32, f32, f32, ctypes.c_int],
         "orgpu_overuse_sparse": [P, i64, P, P, ctypes.c_int, f32, f32, ctypes.POINTER(SparseOut)],
+        "orgpu_segment_sums": [P, i64, P, P, P, i64, P],
+        "orgpu_via_pressure": [P, i64, i32, P, i32, P, i32, P, P, P, P],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -24,6 +24,7 @@ order-dependent step, the wider-ROI fallback for the first five failures
 gives the plain one-net-at-a-time form.
 """
 import ctypes
+import gc
 import logging
 import math
 import os
@@ -36,8 +37,8 @@ from typing import Dict, List, Optional, Set, Tuple
 import numpy as np
 
 from ... import _abi
-from ...device import (Context, DeviceLattice, EdgeState, SearchParams, cost_bounds, max_slots, rank_exceptions,
-                       roi_bfs, route as device_route, route_flat as device_route_flat,
+from ...device import (Context, DeviceLattice, EdgeState, SearchParams, cost_bounds, edge_list_sums, max_slots,
+                       rank_exceptions, roi_bfs, route as device_route, route_flat as device_route_flat,
                        route_ms as device_route_ms, route_ref as device_route_ref)
 from ...autotune import BucketTuner, base_width, default_target
 from ...lattice import DIR_H, DIR_HV, DIR_V, layer_directions, reference_via_pairs
@@ -503,14 +504,24 @@ class PathFinderRouter:
         self.layer_bias = np.ones(L, dtype=np.float32)
         self.via_col_cap = np.full((X, Y), int(getattr(self.config, "via_column_capacity", 4)), dtype=np.int16)
         self.via_col_use = np.zeros((X, Y), dtype=np.int16)
-        self.via_col_pres = np.zeros((X, Y), dtype=np.float32)
         self._segZ = max(L - 2, 0)
         if self._segZ > 0:
             cap = int(getattr(self.config, "via_segment_capacity", 2))
             self.via_seg_cap = np.full((X, Y, self._segZ), cap, dtype=np.int8)
             self.via_seg_use = np.zeros((X, Y, self._segZ), dtype=np.int16)
-            self.via_seg_pres = np.zeros((X, Y, self._segZ), dtype=np.float32)
-            self.via_seg_prefix = np.zeros((X, Y, self._segZ), dtype=np.float32)
+        # via pooling state in HBM ([x][y] and [x][y][segment] like the host
+        # arrays): usage copied from the host's recount, pressure EMAs and
+        # the segment prefix updated by orgpu_via_pressure, read by the cost
+        # update
+        torch, dev = self.ctx.torch, self.ctx.device
+        self._via_dev = {
+            "col_use": torch.zeros(X * Y, dtype=torch.int16, device=dev),
+            "col_pres": torch.zeros(X * Y, dtype=torch.float32, device=dev),
+            "seg_use": torch.zeros(max(X * Y * self._segZ, 1), dtype=torch.int16, device=dev),
+            "seg_pres": torch.zeros(max(X * Y * self._segZ, 1), dtype=torch.float32, device=dev),
+            "seg_prefix": torch.zeros(max(X * Y * self._segZ, 1), dtype=torch.float32, device=dev),
+            "over": torch.zeros(1, dtype=torch.int64, device=dev),
+        }
         self.node_owner = np.full(self.N, -1, dtype=np.int32)
         self.net_id_map = {}
         self.next_net_id = 0
@@ -641,9 +652,18 @@ class PathFinderRouter:
         if not tasks:
             self._negotiation_ran = True
             return {}
+        # Python's cyclic collector stays off for the negotiation: the loop
+        # makes no reference cycles, and a full collection walks every
+        # object the process holds (on the 8,000-net backplane two or three
+        # of them cost 50-100 ms each per negotiation).  It is switched back
+        # on at the end if it was on.
+        gc_on = gc.isenabled()
+        gc.disable()
         try:
             return self._pathfinder_negotiation(tasks, progress_cb, iteration_cb)
         finally:
+            if gc_on:
+                gc.enable()
             self._paths_as_lists()
 
     def _paths_as_lists(self):
@@ -771,9 +791,10 @@ class PathFinderRouter:
         if with_vias:
             p.via_pooling = 1
             p.seg_z = self._segZ
-            p.col_pres = dptr(self.via_col_pres)
+            vd = self._via_dev
+            p.col_pres = ctypes.c_void_p(vd["col_pres"].data_ptr())
             if self._segZ > 0:
-                p.seg_prefix = dptr(self.via_seg_prefix)
+                p.seg_prefix = ctypes.c_void_p(vd["seg_prefix"].data_ptr())
             p.col_weight = f32(float(getattr(self.config, "via_column_weight", 1.0)))
             p.seg_weight = f32(float(getattr(self.config, "via_segment_weight", 1.0)))
             disc, is64 = self._via_discount_table()
@@ -781,9 +802,9 @@ class PathFinderRouter:
             p.discount_f64 = int(is64)
             if it > 1:
                 p.hard_block = 1
-                p.col_use = dptr(self.via_col_use)
+                p.col_use = ctypes.c_void_p(vd["col_use"].data_ptr())
                 if self._segZ > 0:
-                    p.seg_use = dptr(self.via_seg_use)
+                    p.seg_use = ctypes.c_void_p(vd["seg_use"].data_ptr())
                 p.col_cap, p.seg_cap = 4, 2
         self.accounting.state.update_costs(p)
         self.ctx.sync()
@@ -802,18 +823,19 @@ class PathFinderRouter:
         built from and is reused only when every one is the same object."""
         items = []
         trees = getattr(self, "_tree_branches", {})
+        n2n, n2e = self._net_nodes, self._net_to_edges
         for name, path in self.net_paths.items():
-            if not path or len(path) < 2:
+            if len(path) < 2:
                 continue
             tb = trees.get(name)
             if tb is not None:
                 # a tree: each branch is a path of the net
                 items.extend((name, b, b, e) for b, e in zip(*tb) if len(b) > 1)
                 continue
-            items.append((name, path, self._net_nodes.get(name), self._net_to_edges.get(name)))
+            items.append((name, path, n2n.get(name), n2e.get(name)))
         cache = getattr(self, "_flat_cache", None)
         if cache is not None and len(cache[0]) == len(items) and all(
-                a[0] == b[0] and a[1] is b[1] and a[2] is b[2] and a[3] is b[3] for a, b in zip(cache[0], items)):
+                a[1] is b[1] and a[2] is b[2] and a[3] is b[3] and a[0] == b[0] for a, b in zip(cache[0], items)):
             return cache[1]
         nodes_l, edges_l, ids = [], [], []
         for name, path, nodes, e in items:
@@ -826,17 +848,22 @@ class PathFinderRouter:
         if not nodes_l:
             out = (z0, z0, z0, z0, np.zeros((0, 4), np.int64), z0)
         else:
+            # node and edge ids keep the engine's int32 (int64 when a path
+            # held int64); every value is below N or E
             lens = np.fromiter((len(n) for n in nodes_l), dtype=np.int64, count=len(nodes_l))
-            alln = np.concatenate(nodes_l).astype(np.int64)
+            alln = np.concatenate(nodes_l)
             inner = np.ones(len(alln) - 1, dtype=bool)
             inner[np.cumsum(lens)[:-1] - 1] = False      # pairs that straddle two paths
             S = alln[:-1][inner]
             D = alln[1:][inner]
-            E = np.concatenate(edges_l).astype(np.int64)
+            E = np.concatenate(edges_l)
             NS = np.repeat(np.asarray(ids, np.int64), lens - 1)
             H, NH = self._hops_of(S, D, NS)
-            ok = E >= 0
-            out = (E[ok], S[ok], D[ok], NS[ok], H, NH)
+            if len(E) == 0 or int(E.min()) >= 0:
+                out = (E, S, D, NS, H, NH)
+            else:
+                ok = E >= 0
+                out = (E[ok], S[ok], D[ok], NS[ok], H, NH)
         self._flat_cache = (items, out)
         return out
 
@@ -848,7 +875,7 @@ class PathFinderRouter:
         hop = np.abs(D - S) >= plane
         if not hop.any():
             return np.zeros((0, 4), np.int64), NS[hop]
-        s, d = S[hop], D[hop]
+        s, d = S[hop].astype(np.int64), D[hop].astype(np.int64)
         zs, r = np.divmod(s, plane)
         zd = d // plane
         H = np.stack([r % self._Nx, r // self._Nx, np.minimum(zs, zd), np.maximum(zs, zd)], axis=1)
@@ -894,6 +921,23 @@ class PathFinderRouter:
             self.node_owner.fill(-1)
             self._ko_nodes = np.zeros(0, np.int64)
             self._ko_owner = np.zeros(0, np.int64)
+
+    def _via_pressure(self):
+        """Via pooling pressure of the iteration (UPF:3626-3637) on the
+        device: the usage just recounted on the host goes to HBM, where
+        orgpu_via_pressure updates the column and segment pressure EMAs and
+        the segment prefix the cost update reads, and sums the via overuse."""
+        torch = self.ctx.torch
+        vd = self._via_dev
+        with torch.cuda.stream(self.ctx.stream):
+            vd["col_use"].copy_(torch.from_numpy(np.ascontiguousarray(self.via_col_use).reshape(-1)))
+            if self._segZ > 0:
+                vd["seg_use"].copy_(torch.from_numpy(np.ascontiguousarray(self.via_seg_use).reshape(-1)))
+        seg_cap = int(self.via_seg_cap.flat[0]) if self._segZ > 0 else 0
+        _abi.check(self.ctx.lib.orgpu_via_pressure(
+            self.ctx.handle, self._Nx * self._Ny, self._segZ, vd["col_use"].data_ptr(), int(self.via_col_cap.flat[0]),
+            vd["seg_use"].data_ptr(), seg_cap, vd["col_pres"].data_ptr(), vd["seg_pres"].data_ptr(),
+            vd["seg_prefix"].data_ptr(), vd["over"].data_ptr()))
 
     def _mark_barrels_many(self, paths, nids):
         """_mark_barrels for several paths in order (a later path's barrel
@@ -1064,10 +1108,9 @@ class PathFinderRouter:
             unrouted = {n for n in tasks if not self.net_paths.get(n)}
             return unrouted | ripped
         E, _, _, NE, _, _ = flat
-        # hot edges: a 1 marker and their overuse in two slot buffers (an
-        # edge can be hot with zero overuse)
+        # hot edges: a 1 marker in a slot buffer (an edge can be hot with
+        # zero overuse)
         hot_mask = self._dense(hot_idx, 1.0, slot="hot_mask")
-        hot_val = self._dense(hot_idx, hot_over, slot="hot_val")
         id_to_name = {v: k for k, v in self.net_id_map.items()}
         offenders = {id_to_name[int(i)] for i in np.unique(NE[hot_mask[E] != 0])} if len(E) else set()
         for n in tasks:
@@ -1080,7 +1123,10 @@ class PathFinderRouter:
         offenders |= ripped
         unrouted = {n for n in tasks if not self.net_paths.get(n)}
         scored = [n for n in offenders if n in self._net_to_edges]
-        sums = self._path_sums([self._net_to_edges[n] for n in scored], hot_val)
+        # impact: the sum of the hot edges' overuse along the net's path
+        # (UPF:5071), on the device for every offender at once
+        sums = edge_list_sums(self.ctx, self.accounting.E, [self._net_to_edges[n] for n in scored],
+                              hot_idx, hot_over)
         scores = list(zip(sums.tolist(), scored))
         for n in unrouted:
             if n not in offenders:
@@ -1101,10 +1147,9 @@ class PathFinderRouter:
         return set(cool)
 
     def _rip_top_k_offenders(self, o_idx, o_val, k=20):
-        over = self._dense(o_idx, o_val, slot="rip")
         cand = [n for n, path in self.net_paths.items()
                 if path and n not in self.locked_nets and n in self._net_to_edges]
-        imps = self._path_sums([self._net_to_edges[n] for n in cand], over)
+        imps = edge_list_sums(self.ctx, self.accounting.E, [self._net_to_edges[n] for n in cand], o_idx, o_val)
         scores = [(v, n) for v, n in zip(imps.tolist(), cand) if v > 0]
         scores.sort(reverse=True)
         victims = {n for _, n in scores[:k]}
@@ -1131,7 +1176,10 @@ class PathFinderRouter:
                 # a negotiation's tuner picks the target of batches with a
                 # depth estimate; otherwise the starting default
                 tuner = getattr(self, "_tuner", None)
-                target = tuner.target() if (tuner is not None and depth is not None) else default_target(self.N)
+                if tuner is not None and depth is not None:
+                    target = tuner.target(ratio=float(depth) / d2)
+                else:
+                    target = default_target(self.N)
             if depth is not None and target:
                 d2 = max(d2, float(depth) / float(target))
             d = d if d is not None else d2
@@ -1619,12 +1667,7 @@ class PathFinderRouter:
         self.accounting.refresh_from_canonical()
         flat = self._flat()
         self._rebuild_via_usage_from_committed(flat)
-        over = np.maximum(0, self.via_col_use - self.via_col_cap).astype(np.float32)
-        self.via_col_pres = 0.6 * over + 0.4 * self.via_col_pres
-        if self._segZ > 0:
-            over = np.maximum(0, self.via_seg_use - self.via_seg_cap).astype(np.float32)
-            self.via_seg_pres = 0.6 * over + 0.4 * self.via_seg_pres
-            np.cumsum(self.via_seg_pres, axis=2, out=self.via_seg_prefix)
+        self._via_pressure()
         # over = max(0, present - 1) at iteration start (counts)
         p_idx, p_val = self.accounting.state.overuse_sparse(0)
         over_map = self._dense(p_idx, p_val, slot="over_map")
@@ -1713,9 +1756,9 @@ class PathFinderRouter:
             o_idx, o_val = self.accounting.state.overuse_sparse(0)
             over_sum = int(sum_f32(E, o_idx, o_val))
             over_cnt = int(len(o_idx))
-            over_sum += int(np.maximum(0, self.via_col_use - self.via_col_cap).sum())
-            if self._segZ > 0:
-                over_sum += int(np.maximum(0, self.via_seg_use - self.via_seg_cap).sum())
+            # the via part: column and segment overuse of the usage recounted
+            # at the iteration's start (orgpu_via_pressure's sum)
+            over_sum += int(self._via_dev["over"].item())
             by_layer = None
             if over_sum > 0 and len(o_idx):
                 lay = self._edge_layers(o_idx)

@@ -3,21 +3,29 @@
 The bucket width trades re-expansion (wide buckets: a node's key can drop
 several times inside its bucket, and each drop re-expands its row) against
 round count (narrow buckets: more round launches, each following its
-bucket's chains hop by hop).  Which side dominates depends on the board: on
-the full 8,000-net backplane the late iterations' congested costs make a
-search's cost range wide, and spreading it over 128 buckets re-expands every
-node about eight times over (8,192 buckets are 2.2x faster there), while
-500 nets on the same lattice are fastest at 128 and 2x slower at 8,192.
-Paths are identical for every width; only the time changes.
+bucket's chains hop by hop).  Which side dominates depends on the board and
+the iteration: on the full 8,000-net backplane the congested iterations'
+searches are 5-9x faster at 8,192 buckets than at 128 (128 re-expands every
+node many times over), while the late iterations of the 300x300x12 BGA
+board are 1.5-2x faster at 128 than at 32,768 (a hundred searches on a
+1 M-node lattice: round count dominates).  Paths are identical for every
+width; only the time changes.
 
 `base_width` is the width of a batch with no depth estimate (iteration 1):
 2.4 typical planar steps.  A batch with a depth estimate (the spread of its
 nets' current path costs above their lower bounds) is spread over a target
-number of buckets, which `BucketTuner` chooses online during a negotiation:
-it measures each batch's search time, probes the neighbouring targets (x4
-up, then down), keeps the fastest, re-probes every few batches because the
-costs drift as congestion builds, and remembers the settled target per
-lattice shape for the next negotiation in the process.
+number of buckets, never narrower than the base width, which `BucketTuner`
+chooses online during a negotiation: it measures each batch's search time,
+probes the neighbouring targets (x4 up, then down), keeps the fastest,
+re-probes every few batches because the costs drift as congestion builds,
+and remembers the settled target per lattice shape for the next negotiation
+in the process.  A negotiation with nothing settled yet starts at the target
+that makes the first batch's buckets one base width wide (the depth over
+the base width, to the nearest x4 step): the congestion penalties of a few
+edges stretch the depth estimate far beyond the spread of the costs a
+search mostly meets, so that a fixed target count starts far too wide on
+congested boards (on the 8,000-net backplane 128 buckets took 8x longer
+than 8,192 in the first congested iterations).
 """
 import math
 
@@ -57,10 +65,12 @@ class BucketTuner:
     failures in a row)."""
 
     def __init__(self, key, n_nodes, gain=0.25, clear=0.6, backoff=8):
+        self.settled = key in _SETTLED
         start = _SETTLED.get(key, default_target(n_nodes))
         self.key = key
         self.inc = TARGETS.index(start) if start in TARGETS else 1
         self.cur = self.inc
+        self.started = False
         self.gain, self.clear, self.backoff = float(gain), float(clear), int(backoff)
         self.dir = 1
         self.fails = 0
@@ -73,7 +83,16 @@ class BucketTuner:
     def best(self):
         return self.inc
 
-    def target(self):
+    def target(self, ratio=None):
+        """The target of the next batch.  ratio: that batch's depth estimate
+        over the base width; on the first call of a tuner with no settled
+        target for its key, the start moves to the target nearest that ratio
+        (log scale), where the buckets are one base width wide."""
+        if not self.started:
+            self.started = True
+            if not self.settled and ratio is not None and ratio > 0.0 and math.isfinite(ratio):
+                i = min(range(len(TARGETS)), key=lambda k: abs(math.log(TARGETS[k] / ratio)))
+                self.inc = self.cur = i
         return TARGETS[self.cur]
 
     def _accept(self, i, sec, climb):

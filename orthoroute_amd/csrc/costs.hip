@@ -339,6 +339,115 @@ extern "C" int orgpu_overuse_sparse(orgpu_t *h, int64_t n_edges, const float *pr
 }
 
 // ---------------------------------------------------------------------------
+// Via pooling pressure (UPF:3626-3637): per via column, and per segment of
+// the column, pres = 0.6 * max(0, use - cap) + 0.4 * pres in float32 (each
+// product rounded, then the sum: numpy's float32 expression with the Python
+// constants taken as float32), then the running sum of the column's segment
+// pressures along z in float32 (numpy's cumsum along the last axis).  One
+// thread per column.  Also adds sum max(0, use - cap) over columns and
+// segments to *over (the via part of the iteration's overuse sum).
+// Columns are taken grid-stride, so any column count fits the grid.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_via_pressure(int64_t n_cols, int32_t seg_z, const int16_t *col_use, int32_t col_cap, const int16_t *seg_use,
+               int32_t seg_cap, float *col_pres, float *seg_pres, float *seg_prefix, unsigned long long *over)
+{
+    long long tot = 0;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n_cols; c += (int64_t)gridDim.x * blockDim.x) {
+        const float k6 = 0.6f, k4 = 0.4f;
+        const int co = max(0, (int)col_use[c] - col_cap);
+        tot += co;
+        const float a = k6 * (float)co;
+        const float b = k4 * col_pres[c];
+        col_pres[c] = a + b;
+        float acc = 0.0f;
+        for (int32_t z = 0; z < seg_z; ++z) {
+            const int64_t i = c * seg_z + z;
+            const int so = max(0, (int)seg_use[i] - seg_cap);
+            tot += so;
+            const float pa = k6 * (float)so;
+            const float pb = k4 * seg_pres[i];
+            const float pv = pa + pb;
+            seg_pres[i] = pv;
+            acc = acc + pv;
+            seg_prefix[i] = acc;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)tot, o, 64), hi = __shfl_xor((uint32_t)((unsigned long long)tot >> 32), o, 64);
+        tot += (long long)(((unsigned long long)hi << 32) | lo);
+    }
+    if ((threadIdx.x & 63) == 0 && tot) atomicAdd(over, (unsigned long long)tot);
+}
+
+extern "C" int orgpu_via_pressure(orgpu_t *h, int64_t n_cols, int32_t seg_z, const int16_t *col_use, int32_t col_cap,
+                                  const int16_t *seg_use, int32_t seg_cap, float *col_pres, float *seg_pres,
+                                  float *seg_prefix, unsigned long long *over)
+{
+    if (!h || n_cols < 0 || seg_z < 0 || seg_z > ORGPU_MAX_LAYERS || !over ||
+        (n_cols > 0 && (!col_use || !col_pres)) || (n_cols > 0 && seg_z > 0 && (!seg_use || !seg_pres || !seg_prefix)))
+        return orgpu_fail(ORGPU_EINVAL, "via_pressure: bad argument");
+    HIP_CHECK_RET(hipSetDevice(h->device));
+    hipStream_t s = h->stream;
+    HIP_CHECK_RET(hipMemsetAsync(over, 0, sizeof(unsigned long long), s));
+    if (n_cols == 0) return ORGPU_OK;
+    hipLaunchKernelGGL(k_via_pressure, dim3(grid_for(n_cols)), dim3(256), 0, s, n_cols, seg_z, col_use, col_cap,
+                       seg_use, seg_cap, col_pres, seg_pres, seg_prefix, over);
+    HIP_CHECK_RET(hipGetLastError());
+    return ORGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Per committed path, the sum of a per-edge quantity over its edge list in
+// path order: float64, one addition after another from 0.0, each term the
+// float32 value widened exactly -- the value of Python's
+// sum(float(v[e]) for e in edges).  One wave per path: its lanes load 64
+// edge ids and values at a time (coalesced ids, gathered values), then every
+// lane runs the same serial chain of additions over the 64 values taken
+// lane by lane, so the order of the additions is the path order.  An edge id
+// outside [0, n_edges) is not read; its path's sum becomes NaN.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_segment_sums(int64_t n_seg, const int64_t *offs, const int32_t *edges, int64_t n_edges, const float *dense,
+               double *out)
+{
+    const int64_t seg = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;   // uniform per wave
+    const int lane = threadIdx.x & 63;
+    if (seg >= n_seg) return;
+    const int64_t a = offs[seg], b = offs[seg + 1];
+    double acc = 0.0;
+    int bad = 0;
+    for (int64_t base = a; base < b; base += 64) {
+        const int64_t k = base + lane;
+        float v = 0.0f;
+        if (k < b) {
+            const int32_t e = edges[k];
+            if (e >= 0 && (int64_t)e < n_edges) v = dense[e];
+            else bad = 1;
+        }
+        const int m = (int)((b - base) < 64 ? (b - base) : 64);
+        for (int j = 0; j < m; ++j) acc += (double)__shfl(v, j, 64);
+    }
+    if (__ballot(bad)) acc = __builtin_nan("");
+    if (lane == 0) out[seg] = acc;
+}
+
+extern "C" int orgpu_segment_sums(orgpu_t *h, int64_t n_edges, const float *dense, const int32_t *edges,
+                                  const int64_t *offs, int64_t n_seg, double *out)
+{
+    if (!h || n_seg < 0 || n_edges < 0 || (n_seg > 0 && (!dense || !offs || !out)))
+        return orgpu_fail(ORGPU_EINVAL, "segment_sums: bad argument");
+    if (n_seg == 0) return ORGPU_OK;
+    if (n_seg > ((int64_t)1 << 40)) return orgpu_fail(ORGPU_EINVAL, "segment_sums: %lld segments", (long long)n_seg);
+    HIP_CHECK_RET(hipSetDevice(h->device));
+    const int64_t blocks = (n_seg + 3) / 4;   // four waves (paths) per 256-thread block
+    hipLaunchKernelGGL(k_segment_sums, dim3((unsigned)blocks), dim3(256), 0, h->stream, n_seg, offs, edges, n_edges,
+                       dense, out);
+    HIP_CHECK_RET(hipGetLastError());
+    return ORGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Lower bounds of the iteration's costs for the goal-directed search: the
 // cheapest planar edge, and the cheapest via edge per layer of span
 // (cost / |z_dst - z_src|).  Non-negative floats order like their bit

@@ -261,6 +261,42 @@ def cost_bounds(ctx, lat, cost, with_scale=False):
     return res[0], res[1], med
 
 
+def edge_list_sums(ctx, n_edges, lists, idx, val):
+    """Per edge list (host integer arrays), the sum over its edges, in list
+    order, of the per-edge quantity that is val at the edge ids idx and zero
+    elsewhere: float64, one addition after another from 0.0, each term the
+    float32 value widened -- the value of Python's sum(float(v[e]) for e in
+    edges) -- computed on the device (orgpu_segment_sums, one wave per list).
+    Returns float64[len(lists)]; raises on an edge id outside [0, n_edges)."""
+    torch = ctx.torch
+    n = len(lists)
+    out = np.zeros(n, dtype=np.float64)
+    if n == 0:
+        return out
+    lens = np.fromiter((len(e) for e in lists), dtype=np.int64, count=n)
+    offs = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(lens, out=offs[1:])
+    if offs[-1] == 0 or len(idx) == 0:
+        return out
+    flat = np.concatenate(lists)
+    lo, hi = int(flat.min()), int(flat.max())
+    if lo < 0 or hi >= n_edges:
+        raise ValueError(f"edge_list_sums: edge id {lo if lo < 0 else hi} outside [0, {n_edges})")
+    idx = np.asarray(idx, dtype=np.int64)
+    if len(idx) and (int(idx.min()) < 0 or int(idx.max()) >= n_edges):
+        raise ValueError("edge_list_sums: value index outside the edge range")
+    dev = ctx.device
+    with torch.cuda.stream(ctx.stream):
+        dense = torch.zeros(max(int(n_edges), 1), dtype=torch.float32, device=dev)
+        dense[torch.from_numpy(idx).to(dev)] = torch.from_numpy(np.asarray(val, dtype=np.float32)).to(dev)
+        e_d = torch.from_numpy(flat.astype(np.int32, copy=False)).to(dev)
+        o_d = torch.from_numpy(offs).to(dev)
+        res = torch.empty(n, dtype=torch.float64, device=dev)
+        _abi.check(ctx.lib.orgpu_segment_sums(ctx.handle, int(n_edges), _ptr(dense), _ptr(e_d), _ptr(o_d), n,
+                                              _ptr(res)))
+        return res.cpu().numpy()
+
+
 def route(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner=None, path_capacity=None):
     """Route requests = list of (src, dst, net_id, roi_index, use_keepout) on
     the lattice with edge costs `cost` (device f32[E]).  Returns

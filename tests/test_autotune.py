@@ -51,3 +51,20 @@ def test_tuner_remembers_the_settled_target_per_key():
     assert BucketTuner("board", 2_000_000).target() == 8192
     assert BucketTuner("other", 2_000_000).target() == 128
     assert BucketTuner("small", 50_000).target() == 32
+
+
+def test_tuner_starts_at_the_base_width_target_unless_settled():
+    """With nothing settled for its key, the first target is the one nearest
+    (log scale) to the depth / base-width ratio it is given; a settled
+    target, or no ratio, keeps the start."""
+    autotune._SETTLED.clear()
+    assert BucketTuner("a", 2_000_000).target(ratio=31_627.0) == 32768
+    assert BucketTuner("b", 2_000_000).target(ratio=2_300.0) == 2048
+    assert BucketTuner("c", 2_000_000).target(ratio=3.0) == 32
+    assert BucketTuner("d", 2_000_000).target() == 128
+    assert BucketTuner("e", 2_000_000).target(ratio=float("inf")) == 128
+    t = BucketTuner("f", 2_000_000)
+    assert t.target(ratio=9_000.0) == 8192
+    assert t.target(ratio=40.0) == 8192          # only the first call moves the start
+    autotune._SETTLED["g"] = 512
+    assert BucketTuner("g", 2_000_000).target(ratio=31_627.0) == 512
