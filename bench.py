@@ -81,6 +81,9 @@ def parse():
                          "against the reference's recorded trajectory (0: skip)")
     ap.add_argument("--cost-reps", type=int, default=5, help="timed repetitions of each cost-update kernel (0: skip)")
     ap.add_argument("--graph-rounds", action="store_true", help="replay the rounds between polls as HIP graphs")
+    ap.add_argument("--engine", type=int, default=0,
+                    help="two-pin search engine: 0 = bucket rounds over all slots, 1 = one wavefront per search")
+    ap.add_argument("--list-cap", type=int, default=0, help="engine 1: entries per bucket list (0 = default)")
     ap.add_argument("--serve-jobs", type=int, default=16, help="serving stream: jobs (0 = skip)")
     ap.add_argument("--serve-max-nodes", type=float, default=4e6, help="serving stream: largest board (nodes)")
     ap.add_argument("--serve-active", type=int, default=4, help="serving slots (jobs on the GPU at once)")
@@ -175,6 +178,7 @@ def main():
     cfg.sssp_grid = args.grid
     cfg.sssp_rounds_per_poll = args.rounds_per_poll
     cfg.sssp_graph_rounds = args.graph_rounds
+    cfg.sssp_engine, cfg.sssp_list_cap = args.engine, args.list_cap
     r = PathFinderRouter(cfg)
     t0 = time.perf_counter()
     r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
@@ -435,6 +439,7 @@ def negotiation(args, which, cap, with_plain, trees=False):
         cfg.batch_slots = slots
         cfg.max_iterations_cap = cap
         cfg.sssp_graph_rounds = args.graph_rounds
+        cfg.sssp_engine, cfg.sssp_list_cap = args.engine, args.list_cap
         cfg.multi_terminal_enabled = trees
         r = PathFinderRouter(cfg)
         r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
@@ -604,6 +609,7 @@ def stress(args):
     cfg.lattice_policy = "adjacent"
     cfg.batch_slots = slots
     cfg.sssp_graph_rounds = args.graph_rounds
+    cfg.sssp_engine, cfg.sssp_list_cap = args.engine, args.list_cap
     r = PathFinderRouter(cfg)
     r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
     tasks = r._parse_requests(reqs)
@@ -663,6 +669,7 @@ def time_to_zero_overuse(args):
         cfg.lattice_policy = "reference"
         cfg.batch_slots = slots
         cfg.sssp_graph_rounds = args.graph_rounds
+        cfg.sssp_engine, cfg.sssp_list_cap = args.engine, args.list_cap
         r = PathFinderRouter(cfg)
         r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
         torch.cuda.synchronize()

@@ -123,6 +123,16 @@ typedef struct orgpu_sssp_params {
      * replayed as one HIP graph (captured on the first call and again when
      * the round arguments change).  Same kernels and results either way. */
     int32_t launch_mode;
+    /* Search engine for two-pin requests: 0 = bucket rounds over all slots
+     * (every request kind); 1 = one wavefront per search, a persistent grid
+     * of min(slots, n_nets) one-wave workgroups that take requests in order
+     * from a device counter, each owning one key slot (same paths, dists and
+     * edge ids).  Multi-source requests and count_minimal use engine 0. */
+    int32_t engine;
+    /* engine 1: bucket-list entries one search may hold at once (8 bytes
+     * each, per slot; 0 = 2^20).  A search that would exceed it is handed to
+     * engine 0 (counted in orgpu_sssp_stats.retried). */
+    int32_t list_cap;
 } orgpu_sssp_params;
 
 typedef struct orgpu_sssp_stats {
@@ -134,7 +144,9 @@ typedef struct orgpu_sssp_stats {
     double kernel_ms;        /* device time from first seed to last path (HIP events) */
     int64_t launches;        /* round kernels launched (idle ones return at once) */
     double round_ms;         /* device time of the round kernels alone (HIP events
-                                around each back-to-back batch of launches) */
+                                around each back-to-back batch of launches; engine 1:
+                                the search kernel) */
+    int64_t retried;         /* engine 1: requests handed to engine 0 for capacity */
 } orgpu_sssp_stats;
 
 /* Reserves scratch for graphs of up to n_nodes nodes, `slots` concurrent

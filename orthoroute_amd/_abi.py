@@ -71,13 +71,14 @@ class SsspParams(ctypes.Structure):
                 ("slots", ctypes.c_int32), ("rounds_per_poll", ctypes.c_int32),
                 ("grid", ctypes.c_int32), ("count_minimal", ctypes.c_int32),
                 ("h_xy", ctypes.c_float), ("h_z", ctypes.c_float), ("local_batches", ctypes.c_int32),
-                ("launch_mode", ctypes.c_int32)]
+                ("launch_mode", ctypes.c_int32), ("engine", ctypes.c_int32), ("list_cap", ctypes.c_int32)]
 
 
 class SsspStats(ctypes.Structure):
     _fields_ = [("rounds", ctypes.c_int64), ("items", ctypes.c_int64), ("edges", ctypes.c_int64),
                 ("minimal_nodes", ctypes.c_int64), ("minimal_edges", ctypes.c_int64),
-                ("kernel_ms", ctypes.c_double), ("launches", ctypes.c_int64), ("round_ms", ctypes.c_double)]
+                ("kernel_ms", ctypes.c_double), ("launches", ctypes.c_int64), ("round_ms", ctypes.c_double),
+                ("retried", ctypes.c_int64)]
 
 
 class RoiReq(ctypes.Structure):

@@ -152,6 +152,9 @@ class PathFinderConfig:
     sssp_grid = 0                  # workgroups per round launch (0 = library default)
     sssp_rounds_per_poll = 32      # rounds queued between status polls / slot refills
     sssp_graph_rounds = False      # replay the rounds between polls as one captured HIP graph
+    sssp_engine = 0                # two-pin searches: 0 = bucket rounds over all slots, 1 = one
+                                   # wavefront per search (same paths)
+    sssp_list_cap = 0              # engine 1: entries per bucket list (0 = library default)
     sssp_target_buckets = None     # widen the bucket so a batch's deep searches span about this many
                                    # (None: 128 from 1 M nodes up, 32 below)
     max_iterations_cap = None
@@ -1193,7 +1196,9 @@ class PathFinderRouter:
                             rounds_per_poll=int(self.config.sssp_rounds_per_poll or 32),
                             grid=int(self.config.sssp_grid or 0), count_minimal=bool(self.config.count_minimal),
                             h_xy=hx, h_z=hz, local_batches=int(self.config.sssp_local_batches or 0),
-                            graph_rounds=bool(getattr(self.config, "sssp_graph_rounds", False)))
+                            graph_rounds=bool(getattr(self.config, "sssp_graph_rounds", False)),
+                            engine=int(getattr(self.config, "sssp_engine", 0) or 0),
+                            list_cap=int(getattr(self.config, "sssp_list_cap", 0) or 0))
 
     def _depth_hint(self, ordered, sd):
         """How far past their first bucket the searches of a batch of nets
@@ -1330,6 +1335,7 @@ class PathFinderRouter:
         self.stats["sssp_round_ms"] += st["round_ms"]
         self.stats["sssp_minimal_nodes"] += st["minimal_nodes"]
         self.stats["sssp_minimal_edges"] += st["minimal_edges"]
+        self.stats["sssp_retried"] += st["retried"]
         self.stats["sssp_searches"] += len(sreqs)
         ol = offs.tolist()
         for j, i in enumerate(todo.tolist()):

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "liborgpu.so")
-SOURCES = ["api.hip", "lattice.hip", "sssp.hip", "sssp_ref.hip", "costs.hip", "roi.hip"]
+SOURCES = ["api.hip", "lattice.hip", "sssp.hip", "sssp_solo.hip", "sssp_ref.hip", "costs.hip", "roi.hip"]
 ARCH = os.environ.get("ORGPU_ARCH", "gfx950")
 
 

@@ -61,6 +61,7 @@ extern "C" int orgpu_destroy(orgpu_t *h)
     else hipDeviceSynchronize();
     engine_release(h->en);
     ref_release(h->en);
+    solo_release(h->en);
     delete h;
     return ORGPU_OK;
 }

@@ -79,6 +79,19 @@ struct Engine {
     int64_t ref_nodes = 0, ref_hcap = 0;
     int32_t ref_slots = 0;
     uint32_t ref_sid = 0;       // search ids below this one are used
+    // bucket-round searches leave touch records for the next seeding of a
+    // slot; the solo engine resets them first (engine_prepare_keys)
+    bool touch_dirty = false;
+    // one-wave-per-search engine (sssp_solo.hip): per-slot bucket-list arena,
+    // request and result staging, control words
+    uint64_t *solo_arena = nullptr;
+    uint32_t *solo_links = nullptr;
+    uint32_t solo_nch = 0;
+    int32_t solo_slots = 0;
+    orgpu_net_req *solo_req = nullptr;
+    int32_t *solo_out = nullptr, *h_solo_out = nullptr;
+    int32_t solo_req_cap = 0;
+    uint32_t *solo_ctl = nullptr, *h_solo_ctl = nullptr;
 };
 
 struct orgpu_ctx {
@@ -90,6 +103,11 @@ struct orgpu_ctx {
 int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets, int64_t qcap, int32_t max_path);
 void engine_release(Engine &en);
 void ref_release(Engine &en);
+void solo_release(Engine &en);
+// Makes every key of the reserved slots KEY_INF for a graph of n_nodes nodes
+// (whole fill when the node count changed, else the chunks pending in the
+// bucket engine's touch records) and clears those records.
+int engine_prepare_keys(Engine &en, int64_t n_nodes);
 // the multi-source part of a route call (null for two-pin requests)
 struct MsCall {
     const orgpu_ms_req *reqs;
@@ -104,3 +122,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
                  const orgpu_sssp_params *p, int32_t *path_nodes, int32_t *path_edges,
                  int64_t path_capacity, int64_t *path_offset, float *path_dist, orgpu_sssp_stats *stats,
                  const MsCall *ms = nullptr);
+int engine_route_solo(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, int32_t n_nets,
+                      const uint32_t *roi_bitmaps, int32_t n_roi, const int32_t *keepout_owner, float keepout_cost,
+                      const orgpu_sssp_params *p, int32_t *path_nodes, int32_t *path_edges, int64_t path_capacity,
+                      int64_t *path_offset, float *path_dist, orgpu_sssp_stats *stats);

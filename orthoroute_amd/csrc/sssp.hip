@@ -1160,6 +1160,7 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
 
 void engine_release(Engine &en)
 {
+    solo_release(en);
     if (en.keys) hipFree(en.keys);
     if (en.bits) hipFree(en.bits);
     if (en.summ) hipFree(en.summ);
@@ -1228,6 +1229,53 @@ void engine_release(Engine &en)
     en = fresh;
 }
 
+// Resets the chunks listed in the touch records of every slot (grid (G,
+// slots): slot blockIdx.y) and clears their touch bits.
+__global__ void k_flush_touched(RoundArgs a)
+{
+    const uint64_t N = (uint64_t)a.N;
+    const uint32_t s = blockIdx.y;
+    const uint32_t cnt = a.touch_n[s];
+    const uint32_t *lst = a.touch_list + (uint64_t)s * a.touch_cap;
+    uint64_t *ks = a.keys + (uint64_t)s * (uint64_t)a.NP;
+    uint32_t *tb = a.touch_bits + (uint64_t)s * a.touch_words;
+    for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
+        const uint32_t ch = lst[i];
+        const uint64_t lo = (uint64_t)ch * TOUCH_KEYS;
+        const uint64_t hi = lo + TOUCH_KEYS < N ? lo + TOUCH_KEYS : N;
+        for (uint64_t v = lo + threadIdx.x; v < hi; v += blockDim.x) ks[v] = KEY_INF;
+        if (threadIdx.x == 0) tb[ch >> 5] = 0u;
+    }
+}
+
+int engine_prepare_keys(Engine &en, int64_t N)
+{
+    hipStream_t s = en.stream;
+    if (N <= 0 || N > en.cap_nodes) return orgpu_fail(ORGPU_EINVAL, "prepare_keys: node count outside the reserve");
+    RoundArgs a;
+    memset(&a, 0, sizeof(a));
+    a.N = N;
+    a.NP = (N + CHUNK_ITEMS - 1) / CHUNK_ITEMS * CHUNK_ITEMS;
+    a.keys = en.keys;
+    a.touch_cap = (N + TOUCH_KEYS - 1) / TOUCH_KEYS;
+    a.touch_words = (a.touch_cap + 31) / 32;
+    a.touch_bits = en.touch_bits; a.touch_list = en.touch_list; a.touch_n = en.touch_n;
+    if (en.keys_n != N) {
+        hipLaunchKernelGGL(k_fill_keys, dim3(4096), dim3(256), 0, s, en.keys, (int64_t)en.cap_slots * a.NP);
+        HIP_CHECK_RET(hipGetLastError());
+        en.keys_n = N;
+    } else if (en.touch_dirty) {
+        hipLaunchKernelGGL(k_flush_touched, dim3(64, en.cap_slots), dim3(256), 0, s, a);
+        HIP_CHECK_RET(hipGetLastError());
+    } else {
+        return ORGPU_OK;
+    }
+    HIP_CHECK_RET(hipMemsetAsync(en.touch_n, 0, sizeof(uint32_t) * en.cap_slots, s));
+    HIP_CHECK_RET(hipMemsetAsync(en.touch_bits, 0, sizeof(uint32_t) * (size_t)en.cap_slots * a.touch_words, s));
+    en.touch_dirty = false;
+    return ORGPU_OK;
+}
+
 int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, int32_t n_nets,
                  const uint32_t *roi_bitmaps, int32_t n_roi, const int32_t *keepout_owner, float keepout_cost,
                  const orgpu_sssp_params *p,
@@ -1262,6 +1310,11 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
             return orgpu_fail(ORGPU_EINVAL, "route: request %d ROI row %d outside [0, %d)", i, r.roi_index, n_roi);
         if (r.use_keepout && !keepout_owner) return orgpu_fail(ORGPU_EINVAL, "route: request %d needs keepout_owner", i);
     }
+    if (p->engine != 0 && p->engine != 1) return orgpu_fail(ORGPU_EINVAL, "route: engine %d is not 0 or 1", p->engine);
+    if (p->list_cap < 0) return orgpu_fail(ORGPU_EINVAL, "route: list_cap %d < 0", p->list_cap);
+    if (p->engine == 1 && !ms && !p->count_minimal)
+        return engine_route_solo(en, g, nets, n_nets, roi_bitmaps, n_roi, keepout_owner, keepout_cost, p, path_nodes,
+                                 path_edges, path_capacity, path_offset, path_dist, stats);
     hipStream_t s = en.stream;
     if (ms) {
         if ((ms->n_seeds > 0 && !ms->seeds) || (ms->n_targets > 0 && !ms->targets) || (n_nets > 0 && !ms->reqs) ||
@@ -1406,6 +1459,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
             ++next; ++ns;
         }
         if (ns == 0) return 0;
+        en.touch_dirty = true;
         HIP_CHECK_RET(hipMemcpyAsync(en.seed, en.h_seed, sizeof(SeedReq) * ns, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_reset_keys, dim3(std::max(1, std::min(512, 2048 / ns)), ns), dim3(256), 0, s, a, en.seed, ns);
         hipLaunchKernelGGL(k_seed, dim3((ns + 255) / 256), dim3(256), 0, s, a, en.seed, ns, ms ? en.ms_seed : nullptr);
@@ -1566,6 +1620,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
         stats->kernel_ms = ms;
         stats->launches = launches;
         stats->round_ms = round_ms;
+        stats->retried = 0;
     }
     return ORGPU_OK;
 }

@@ -1,0 +1,737 @@
+// A3/A4, second engine: one wavefront per search ("solo" searches).
+//
+// Same contract as the bucket-round engine of sssp.hip (the FP32 shortest
+// path with the lowest-predecessor tie-break, keys packed as in common.h,
+// goal-directed buckets with the one-bucket margin at the destination), for
+// two-pin requests.  Reference behaviour replaced: SimpleDijkstra.
+// find_path_roi (UPF:1801-1887), the owner-aware keepout (UPF:3150-3228),
+// the parent walk (UPF:1880-1887) and _path_to_edges (UPF:5463-5476).
+//
+// Design:
+//  * a persistent grid of one-wave workgroups; workgroup w owns key slot w
+//    (keys u64[NP] of its own) and takes requests from a global counter
+//    until none are left.  No workgroup ever waits for another one.
+//  * the wave alone reads and writes its slot's keys, so a relaxation is a
+//    plain store followed by a reload: lanes of the wave that raced on one
+//    node see the winner and the better candidates store again until the
+//    minimum stands (no global atomics: every device-scope atomic on this
+//    GPU is performed at the memory side).
+//  * the frontier is a set of per-wave lists in device memory (entries
+//    (key cost, node) in the key's own bit layout): a ring of SOLO_NBR
+//    bucket lists for the window [W, W + SOLO_NBR) and two overflow lists
+//    used in turn, all chains of chunks of the slot's arena.  An entry whose cost no longer equals its node's key is
+//    stale and skipped (every strict improvement pushes a new entry).
+//    Appends are wave-aggregated (one ballot per target list), so list
+//    traffic is coalesced and costs no atomics.
+//  * keys a search set are recorded per chunk of 2^touch_shift keys in an
+//    LDS bitmap and reset to KEY_INF by the wave when its search ends, so
+//    every search starts on a clean slot.
+//  * when the destination is settled (its bucket plus the margin lies below
+//    the next bucket to expand) or the frontier is empty, lane 0 walks the
+//    predecessor fields back to the source, the lanes find each hop's first
+//    CSR edge and the path is packed src -> dst into the shared output.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+#include "engine.h"
+
+#define SOLO_NBR 16                 // ring bucket lists
+#define SOLO_NL (SOLO_NBR + 2)      // plus two overflow lists
+#define SOLO_TOUCH_WORDS 1024       // LDS touch bitmap: 32768 key chunks
+#define SOLO_MAX_STEPS (1u << 22)  // expansion steps of 64 entries per search
+
+// request outcomes (out_len) besides a path length >= 0
+#define SOLO_BROKEN (-1)            // predecessor chain broken or longer than the staging row
+#define SOLO_KO_SATURATED (-2)      // path crosses 31 or more keepout edges
+#define SOLO_RETRY (-3)             // list or output capacity exceeded: route it on the other engine
+
+struct SoloArgs {
+    const int32_t *indptr;
+    const int32_t *indices;
+    const float *cost;
+    const uint8_t *kind;
+    const uint32_t *roi;           // [n_roi][roi_words]
+    int64_t roi_words;
+    const int32_t *ko_owner;       // [N] or null
+    int64_t N, NP;
+    int32_t X, plane;              // node = z*plane + y*X + x (goal-directed order)
+    float inv_delta, h_xy, h_z;
+    const orgpu_net_req *reqs;
+    int32_t n_reqs;
+    uint32_t *ctl;                 // [0] next request, [1] packed path entries used
+    uint64_t *keys;                // [slots][NP]
+    uint64_t *arena;               // [slots][nch][SOLO_CH] bucket-list entries
+    uint32_t *links;               // [slots][2][nch]: chunk links, free stack
+    uint32_t nch;                  // chunks per slot
+    uint32_t touch_shift;          // keys per touch chunk = 2^touch_shift
+    int32_t *stage;                // [slots][max_path] predecessor walk
+    int32_t max_path;
+    int32_t *pack;                 // nodes [0, pack_half), edges [pack_half, 2 pack_half)
+    uint64_t pack_half;
+    int32_t *out_len;
+    uint32_t *out_off;
+    float *out_dist;
+    unsigned long long *counters;  // [0] items expanded, [1] edges scanned
+};
+
+__device__ __forceinline__ uint64_t kload(const uint64_t *p)
+{
+    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void kstore(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// orders this wave's earlier key stores before its later key loads
+__device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint64_t wave_min64(uint64_t x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)x, o, 64), hi = __shfl_xor((uint32_t)(x >> 32), o, 64);
+        const uint64_t y = ((uint64_t)hi << 32) | lo;
+        x = y < x ? y : x;
+    }
+    return x;
+}
+
+__device__ __forceinline__ int wave_max_i32(int x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int y = __shfl_xor(x, o, 64);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
+// frontier entry: the key's (keepout, cost) fields with the node in the
+// predecessor field
+__device__ __forceinline__ uint64_t entry_pack(uint64_t key, uint32_t node)
+{
+    return (key & ~(uint64_t)KEY_PRED_MASK) | (uint64_t)node;
+}
+
+// Bucket lists of one search: chains of SOLO_CH-entry chunks in the slot's
+// arena.  Chunks come from a free stack (chunks a list has consumed) or
+// from a bump pointer, so any list may hold up to the whole arena.  The
+// chain links and the free stack live in device memory; the per-list state
+// (head / tail chunk and position, entry count) and the allocator counters
+// in LDS.  Every operation is wave-uniform.
+#define SOLO_CH 256
+#define SOLO_NONE 0xFFFFFFFFu
+struct SoloLists {
+    uint64_t *arena;       // [nch][SOLO_CH]
+    uint32_t *link;        // [nch] next chunk of the chain
+    uint32_t *freestk;     // [nch]
+    uint32_t nch;
+    uint32_t *hc, *hp, *tc, *tp, *cnt;   // LDS [SOLO_NL]
+    uint32_t *al;                        // LDS [2]: free-stack top, bump pointer
+
+    __device__ __forceinline__ bool empty(uint32_t t) const { return cnt[t] == 0u; }
+    __device__ __forceinline__ uint32_t alloc() const
+    {
+        const uint32_t top = al[0];
+        if (top > 0u) { al[0] = top - 1u; return freestk[top - 1u]; }
+        const uint32_t b = al[1];
+        if (b < nch) { al[1] = b + 1u; return b; }
+        return SOLO_NONE;
+    }
+    __device__ __forceinline__ void release(uint32_t c) const
+    {
+        const uint32_t top = al[0];
+        freestk[top] = c;
+        al[0] = top + 1u;
+    }
+    __device__ __forceinline__ void reset(uint32_t lane) const
+    {
+        if (lane < SOLO_NL) { hc[lane] = SOLO_NONE; tc[lane] = SOLO_NONE; hp[lane] = 0u; tp[lane] = 0u; cnt[lane] = 0u; }
+        if (lane < 2) al[lane] = 0u;
+    }
+    // Takes up to 64 entries from the head of list t: lane k < n gets
+    // entry k.  The chunks it empties go back to the free stack; the
+    // entries are read before any later append can reuse them.
+    __device__ __forceinline__ uint64_t pop(uint32_t t, uint32_t lane, uint32_t &n_out) const
+    {
+        const uint32_t c = cnt[t];
+        const uint32_t n = c < 64u ? c : 64u;
+        n_out = n;
+        const uint32_t h = hc[t], p = hp[t];
+        const bool cross = p + n > SOLO_CH;
+        const uint32_t nx = cross ? link[h] : SOLO_NONE;
+        uint64_t e = 0ull;
+        if (lane < n) {
+            const uint32_t pos = p + lane;
+            e = pos < SOLO_CH ? arena[(uint64_t)h * SOLO_CH + pos] : arena[(uint64_t)nx * SOLO_CH + (pos - SOLO_CH)];
+        }
+        const uint32_t left = c - n;
+        cnt[t] = left;
+        if (left == 0u) {
+            release(h);
+            if (cross) release(nx);
+            hc[t] = SOLO_NONE; tc[t] = SOLO_NONE; hp[t] = 0u; tp[t] = 0u;
+        } else if (p + n >= SOLO_CH) {
+            // (a consumed head chunk always has a successor here: entries remain)
+            const uint32_t nh = cross ? nx : link[h];
+            release(h);
+            hc[t] = nh;
+            hp[t] = p + n - SOLO_CH;
+        } else {
+            hp[t] = p + n;
+        }
+        return e;
+    }
+};
+
+// Appends each active lane's entry e to its list t, one ballot per distinct
+// list.  Returns false (wave-uniform) when the arena is exhausted; the
+// entries that did not fit are dropped and the caller abandons the search.
+__device__ __forceinline__ bool lists_append(const SoloLists &L, bool act, uint32_t t, uint64_t e)
+{
+    bool ok = true;
+    uint64_t m;
+    while ((m = __ballot(act)) != 0ull) {
+        const int first = __builtin_ctzll(m);
+        const uint32_t t0 = (uint32_t)__shfl((int)t, first);
+        const bool mine = act && t == t0;
+        const uint64_t mm = __ballot(mine);
+        const uint32_t cnt = (uint32_t)__popcll(mm);
+        if (mine) act = false;
+        uint32_t tc = L.tc[t0], tp = L.tp[t0];
+        if (tc == SOLO_NONE) {
+            tc = L.alloc();
+            if (tc == SOLO_NONE) { ok = false; continue; }
+            L.hc[t0] = tc; L.hp[t0] = 0u; tp = 0u;
+        }
+        uint32_t c2 = SOLO_NONE;
+        if (tp + cnt > SOLO_CH) {
+            c2 = L.alloc();
+            if (c2 == SOLO_NONE) { ok = false; L.tc[t0] = tc; L.tp[t0] = tp; continue; }
+            L.link[tc] = c2;
+        }
+        if (mine) {
+            const uint32_t pos = tp + lanes_below(mm);
+            const uint64_t at = pos < SOLO_CH ? (uint64_t)tc * SOLO_CH + pos : (uint64_t)c2 * SOLO_CH + (pos - SOLO_CH);
+            L.arena[at] = e;
+        }
+        if (c2 != SOLO_NONE) { L.tc[t0] = c2; L.tp[t0] = tp + cnt - SOLO_CH; }
+        else { L.tc[t0] = tc; L.tp[t0] = tp + cnt; }
+        L.cnt[t0] += cnt;
+    }
+    return ok;
+}
+
+__device__ __forceinline__ float solo_heur(const SoloArgs &a, int dx, int dy, int dz, uint32_t v)
+{
+    if (a.h_xy == 0.0f && a.h_z == 0.0f) return 0.0f;
+    const int z = (int)(v / (uint32_t)a.plane);
+    const int r = (int)(v - (uint32_t)z * (uint32_t)a.plane);
+    const int y = r / a.X;
+    const int x = r - y * a.X;
+    return a.h_xy * (float)(abs(x - dx) + abs(y - dy)) + a.h_z * (float)abs(z - dz);
+}
+
+__global__ void __launch_bounds__(64)
+k_solo(SoloArgs a)
+{
+    __shared__ uint32_t s_touch[SOLO_TOUCH_WORDS];
+    __shared__ uint32_t s_hc[SOLO_NL], s_hp[SOLO_NL], s_tc[SOLO_NL], s_tp[SOLO_NL], s_cnt[SOLO_NL], s_al[2];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t slot = blockIdx.x;
+    uint64_t *keys = a.keys + (uint64_t)slot * (uint64_t)a.NP;
+    SoloLists L{a.arena + (uint64_t)slot * (uint64_t)a.nch * SOLO_CH, a.links + (uint64_t)slot * 2u * a.nch,
+                a.links + (uint64_t)slot * 2u * a.nch + a.nch, a.nch, s_hc, s_hp, s_tc, s_tp, s_cnt, s_al};
+    int32_t *stage = a.stage + (uint64_t)slot * (uint64_t)a.max_path;
+    const uint32_t tshift = a.touch_shift;
+    for (int i = lane; i < SOLO_TOUCH_WORDS; i += 64) s_touch[i] = 0u;
+    const bool goal = a.h_xy != 0.0f || a.h_z != 0.0f;
+    const uint64_t margin = goal ? 1u : 0u;
+    unsigned long long my_items = 0, my_edges = 0;
+
+    for (;;) {
+        uint32_t r = 0;
+        if (lane == 0) r = atomicAdd(&a.ctl[0], 1u);
+        r = (uint32_t)__shfl((int)r, 0);
+        if (r >= (uint32_t)a.n_reqs) break;
+        const orgpu_net_req q = a.reqs[r];
+        const uint32_t src = (uint32_t)q.src, dst = (uint32_t)q.dst;
+        int dx = 0, dy = 0, dz = 0;
+        if (goal) {
+            dz = (int)(dst / (uint32_t)a.plane);
+            const int rr = (int)(dst - (uint32_t)dz * (uint32_t)a.plane);
+            dy = rr / a.X;
+            dx = rr - dy * a.X;
+        }
+        const uint32_t *roi = q.roi_index >= 0 ? a.roi + (uint64_t)q.roi_index * (uint64_t)a.roi_words : nullptr;
+        const bool use_ko = q.use_keepout && a.ko_owner;
+        L.reset(lane);
+
+        // seed the source in its own bucket
+        const uint64_t k0 = key_pack(0, 0.0f, KEY_NOPRED);
+        uint64_t cur = key_bucket_rel(k0, a.inv_delta, solo_heur(a, dx, dy, dz, src));
+        uint64_t W = cur;              // window of the ring: [W, W + SOLO_NBR)
+        uint32_t ovf = SOLO_NBR;       // overflow list in use
+        uint64_t ovf_min = ~0ull;      // lower bound of its buckets
+        if (lane == 0) {
+            kstore(&keys[src], k0);
+            atomicOr(&s_touch[(src >> tshift) >> 5], 1u << ((src >> tshift) & 31));
+        }
+        bool ok = lists_append(L, lane == 0, (uint32_t)(cur % SOLO_NBR), entry_pack(k0, src));
+        wave_fence();
+        bool reached = false;
+        // a bound on the expansion steps of one search, so that every wave
+        // ends whatever the input (a search that reaches it is handed back)
+        uint32_t steps = 0;
+
+        while (ok) {
+            // expand the current bucket to its fixed point (entries pushed
+            // into it meanwhile are appended to the same list)
+            const uint32_t lc = (uint32_t)(cur % SOLO_NBR);
+            while (ok && !L.empty(lc)) {
+                if (++steps > SOLO_MAX_STEPS) { ok = false; break; }
+                uint32_t n = 0;
+                const uint64_t e = L.pop(lc, lane, n);
+                bool has = lane < n;
+                const uint32_t u = (uint32_t)(e & KEY_PRED_MASK);
+                uint64_t ku = 0;
+                int row = 0, row_end = 0;
+                int32_t own = -1;
+                if (has) {
+                    ku = kload(&keys[u]);
+                    row = a.indptr[u];
+                    row_end = a.indptr[u + 1];
+                    if (use_ko) own = a.ko_owner[u];
+                }
+                // stale: a cheaper entry of the node was pushed since
+                has = has && key_cost(ku) == key_cost(e);
+                const int deg = has ? row_end - row : 0;
+                if (has) { my_items++; my_edges += (unsigned long long)deg; }
+                const uint32_t ko0 = key_ko(e);
+                const float d0 = key_dist(e);
+                const bool ko_node = has && use_ko && own >= 0 && own != q.net_id;
+                const int maxdeg = wave_max_i32(deg);
+                for (int k0e = 0; k0e < maxdeg; k0e += 4) {
+                    uint32_t v[4];
+                    float c[4];
+                    bool go[4], pko[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        go[j] = k0e + j < deg;
+                        pko[j] = false;
+                        v[j] = 0u;
+                        c[j] = 0.0f;
+                        if (go[j]) {
+                            v[j] = (uint32_t)a.indices[row + k0e + j];
+                            c[j] = a.cost[row + k0e + j];
+                            if (ko_node) pko[j] = a.kind[row + k0e + j] == 0;
+                        }
+                    }
+                    uint32_t rw[4];
+                    uint64_t kv[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        rw[j] = 0xFFFFFFFFu;
+                        kv[j] = KEY_INF;
+                        if (!go[j]) continue;
+                        if (roi) rw[j] = roi[v[j] >> 5];
+                        kv[j] = kload(&keys[v[j]]);
+                    }
+                    uint64_t nk[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        nk[j] = KEY_INF;
+                        if (!go[j]) continue;
+                        if (!((rw[j] >> (v[j] & 31)) & 1u)) { go[j] = false; continue; }
+                        uint32_t ko = ko0;
+                        float nd = d0;
+                        if (pko[j]) ko = ko < 31 ? ko + 1 : 31;
+                        else nd = nd + c[j];
+                        nk[j] = key_pack(ko, nd, u + 1u);
+                        go[j] = nd < __builtin_inff() && nk[j] < kv[j];
+                    }
+                    // store, then reload: a lane that sees a larger key than
+                    // its candidate lost a race inside the wave and stores
+                    // again; one that sees its own candidate holds the node
+                    bool pend[4], won[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) { pend[j] = go[j]; won[j] = false; }
+                    for (;;) {
+                        bool any = false;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) any = any || pend[j];
+                        if (__ballot(any) == 0ull) break;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (pend[j]) kstore(&keys[v[j]], nk[j]);
+                        wave_fence();
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            if (!pend[j]) continue;
+                            const uint64_t kr = kload(&keys[v[j]]);
+                            pend[j] = kr > nk[j];
+                            won[j] = kr == nk[j];
+                        }
+                    }
+                    // touched chunks and pushes of strict improvements
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (go[j] && kv[j] == KEY_INF) {
+                            const uint32_t ch = v[j] >> tshift;
+                            atomicOr(&s_touch[ch >> 5], 1u << (ch & 31));
+                        }
+                        const bool push = won[j] && key_cost(nk[j]) < key_cost(kv[j]);
+                        uint32_t t = 0;
+                        if (push) {
+                            const uint64_t b = key_bucket_rel(nk[j], a.inv_delta, solo_heur(a, dx, dy, dz, v[j]));
+                            if (b <= cur) t = lc;
+                            else if (b < W + SOLO_NBR) t = (uint32_t)(b % SOLO_NBR);
+                            else {
+                                t = ovf;
+                                ovf_min = b < ovf_min ? b : ovf_min;
+                            }
+                        }
+                        ok = lists_append(L, push, t, entry_pack(nk[j], v[j])) && ok;
+                    }
+                }
+                ovf_min = wave_min64(ovf_min);
+                wave_fence();
+            }
+            if (!ok) break;
+            // next bucket to expand: the first listed one in the window, else
+            // the overflow list's lowest (then redistributed)
+            uint64_t next = ~0ull;
+            for (uint64_t b = cur + 1; b < W + SOLO_NBR; ++b) {
+                const uint32_t t = (uint32_t)(b % SOLO_NBR);
+                if (!L.empty(t)) { next = b; break; }
+            }
+            const bool from_ovf = next == ~0ull && !L.empty(ovf);
+            if (from_ovf) next = ovf_min;
+            const uint64_t kd = kload(&keys[dst]);
+            if (kd != KEY_INF && (next == ~0ull || key_bucket_rel(kd, a.inv_delta) + margin < next)) {
+                reached = true;
+                break;
+            }
+            if (next == ~0ull) break;   // frontier empty: dst unreachable
+            cur = next;
+            if (!from_ovf) continue;
+            // window jump: live overflow entries move into the new window's
+            // ring lists or into the other overflow list
+            W = cur;
+            const uint32_t old = ovf;
+            ovf = old == SOLO_NBR ? SOLO_NBR + 1 : SOLO_NBR;
+            ovf_min = ~0ull;
+            while (ok && !L.empty(old)) {
+                uint32_t n = 0;
+                const uint64_t e = L.pop(old, lane, n);
+                bool has = lane < n;
+                const uint32_t v = (uint32_t)(e & KEY_PRED_MASK);
+                uint64_t kv = has ? kload(&keys[v]) : 0ull;
+                has = has && key_cost(kv) == key_cost(e);
+                uint32_t t = 0;
+                if (has) {
+                    const uint64_t b = key_bucket_rel(kv, a.inv_delta, solo_heur(a, dx, dy, dz, v));
+                    if (b < W + SOLO_NBR) t = (uint32_t)((b < cur ? cur : b) % SOLO_NBR);
+                    else {
+                        t = ovf;
+                        ovf_min = b < ovf_min ? b : ovf_min;
+                    }
+                }
+                ok = lists_append(L, has, t, e) && ok;
+                ovf_min = wave_min64(ovf_min);
+            }
+        }
+
+        // result: walk the predecessors (lane 0), then pack src -> dst
+        int32_t len = 0;
+        float dist = __builtin_inff();
+        if (!ok) {
+            len = SOLO_RETRY;
+        } else if (reached) {
+            const uint64_t kd = kload(&keys[dst]);
+            dist = key_dist(kd);
+            if (key_ko(kd) >= 31u) {
+                len = SOLO_KO_SATURATED;
+            } else if (lane == 0) {
+                uint32_t v = dst;
+                for (;;) {
+                    if (len >= a.max_path) { len = SOLO_BROKEN; break; }
+                    stage[len++] = (int32_t)v;
+                    if (v == src) break;
+                    const uint32_t pf = key_pred(kload(&keys[v]));
+                    if (pf == KEY_NOPRED) { len = SOLO_BROKEN; break; }
+                    v = pf - 1u;
+                }
+            }
+            len = __shfl(len, 0);
+        }
+        uint32_t off = 0;
+        if (len > 0) {
+            if (lane == 0) off = atomicAdd(&a.ctl[1], (uint32_t)len);
+            off = (uint32_t)__shfl((int)off, 0);
+            if ((uint64_t)off + (uint64_t)len > a.pack_half) len = SOLO_RETRY;
+        }
+        if (len > 0) {
+            wave_fence();
+            int32_t *on = a.pack + off;
+            int32_t *oe = a.pack + a.pack_half + off;
+            // node t of the packed path is stage[len-1-t]; the edge leaving
+            // it is the first CSR edge stage[len-1-t] -> stage[len-2-t]
+            for (int t = lane; t < len; t += 64) {
+                const int32_t un = stage[len - 1 - t];
+                on[t] = un;
+                int ee = -1;
+                if (t + 1 < len) {
+                    const int32_t vn = stage[len - 2 - t];
+                    const int r0 = a.indptr[un], r1 = a.indptr[un + 1];
+                    for (int e2 = r0; e2 < r1; ++e2)
+                        if (a.indices[e2] == vn) { ee = e2; break; }
+                }
+                oe[t] = ee;
+            }
+        }
+        if (lane == 0) {
+            a.out_len[r] = len;
+            a.out_off[r] = off;
+            a.out_dist[r] = dist;
+        }
+        // reset every key chunk the search touched, and the lists
+        for (int w0 = 0; w0 < SOLO_TOUCH_WORDS; w0 += 64) {
+            uint32_t word = s_touch[w0 + lane];
+            uint64_t any = __ballot(word != 0u);
+            while (any) {
+                const int l = __builtin_ctzll(any);
+                uint32_t wl = (uint32_t)__shfl((int)word, l);
+                while (wl) {
+                    const int bit = __builtin_ctz(wl);
+                    wl &= wl - 1u;
+                    const uint64_t lo = ((uint64_t)(w0 + l) * 32u + (uint64_t)bit) << tshift;
+                    const uint64_t hi = min(lo + (1ull << tshift), (uint64_t)a.N);
+                    for (uint64_t k = lo + lane; k < hi; k += 64) keys[k] = KEY_INF;
+                }
+                any &= any - 1ull;
+            }
+            s_touch[w0 + lane] = 0u;
+        }
+        wave_fence();
+    }
+    // counters
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        my_items += __shfl_xor(my_items, o, 64);
+        my_edges += __shfl_xor(my_edges, o, 64);
+    }
+    if (lane == 0) {
+        atomicAdd(&a.counters[0], my_items);
+        atomicAdd(&a.counters[1], my_edges);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+
+static int solo_reserve(Engine &en, int32_t slots, int32_t n_reqs, uint32_t nch)
+{
+    if (en.solo_arena == nullptr || en.solo_slots < slots || en.solo_nch != nch) {
+        if (en.solo_arena) hipFree(en.solo_arena);
+        if (en.solo_links) hipFree(en.solo_links);
+        en.solo_arena = nullptr; en.solo_links = nullptr;
+        HIP_CHECK_RET(hipMalloc((void **)&en.solo_arena, (size_t)slots * nch * SOLO_CH * sizeof(uint64_t)));
+        HIP_CHECK_RET(hipMalloc((void **)&en.solo_links, (size_t)slots * 2 * nch * sizeof(uint32_t)));
+        en.solo_slots = slots;
+        en.solo_nch = nch;
+    }
+    if (en.solo_req_cap < n_reqs) {
+        if (en.solo_req) hipFree(en.solo_req);
+        if (en.solo_out) hipFree(en.solo_out);
+        if (en.h_solo_out) hipHostFree(en.h_solo_out);
+        en.solo_req = nullptr; en.solo_out = nullptr; en.h_solo_out = nullptr;
+        const int32_t cap = std::max(n_reqs, 4096);
+        HIP_CHECK_RET(hipMalloc((void **)&en.solo_req, sizeof(orgpu_net_req) * cap));
+        HIP_CHECK_RET(hipMalloc((void **)&en.solo_out, (size_t)3 * cap * sizeof(int32_t)));
+        HIP_CHECK_RET(hipHostMalloc((void **)&en.h_solo_out, (size_t)3 * cap * sizeof(int32_t), hipHostMallocDefault));
+        en.solo_req_cap = cap;
+    }
+    if (!en.solo_ctl) {
+        HIP_CHECK_RET(hipMalloc((void **)&en.solo_ctl, 64));
+        HIP_CHECK_RET(hipHostMalloc((void **)&en.h_solo_ctl, 64, hipHostMallocDefault));
+    }
+    return ORGPU_OK;
+}
+
+void solo_release(Engine &en)
+{
+    if (en.solo_arena) hipFree(en.solo_arena);
+    if (en.solo_links) hipFree(en.solo_links);
+    if (en.solo_req) hipFree(en.solo_req);
+    if (en.solo_out) hipFree(en.solo_out);
+    if (en.h_solo_out) hipHostFree(en.h_solo_out);
+    if (en.solo_ctl) hipFree(en.solo_ctl);
+    if (en.h_solo_ctl) hipHostFree(en.h_solo_ctl);
+    en.solo_arena = nullptr; en.solo_links = nullptr; en.solo_req = nullptr; en.solo_out = nullptr;
+    en.h_solo_out = nullptr;
+    en.solo_ctl = nullptr; en.h_solo_ctl = nullptr;
+    en.solo_slots = 0; en.solo_nch = 0; en.solo_req_cap = 0;
+}
+
+// Two-pin requests on the solo engine.  Arguments were checked by
+// engine_route; outputs as engine_route writes them.  Requests the engine
+// gives back for capacity are routed on the bucket-round engine.
+int engine_route_solo(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, int32_t n_nets,
+                      const uint32_t *roi_bitmaps, int32_t n_roi, const int32_t *keepout_owner, float keepout_cost,
+                      const orgpu_sssp_params *p, int32_t *path_nodes, int32_t *path_edges, int64_t path_capacity,
+                      int64_t *path_offset, float *path_dist, orgpu_sssp_stats *stats)
+{
+    const int64_t N = g->n_nodes;
+    const int K = std::min(p->slots, en.cap_slots);
+    hipStream_t s = en.stream;
+    path_offset[0] = 0;
+    if (stats) memset(stats, 0, sizeof(*stats));
+    if (n_nets == 0) return ORGPU_OK;
+    // bucket-list storage per search: p->list_cap entries (0: 2^20), in
+    // whole chunks
+    const uint64_t want = p->list_cap > 0 ? (uint64_t)p->list_cap : (1ull << 20);
+    const uint32_t nch = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(2, (want + SOLO_CH - 1) / SOLO_CH), 1u << 24);
+    int rc = solo_reserve(en, K, n_nets, nch);
+    if (rc) return rc;
+    rc = engine_prepare_keys(en, N);
+    if (rc) return rc;
+    const int64_t NP = (N + 2047) / 2048 * 2048;
+    uint32_t tshift = 6;
+    while ((((uint64_t)N + (1ull << tshift) - 1) >> tshift) > (uint64_t)SOLO_TOUCH_WORDS * 32u) ++tshift;
+    const int64_t cap = en.solo_req_cap;
+    const uint64_t half = (uint64_t)en.cap_slots * (uint64_t)en.cap_path;
+
+    SoloArgs a;
+    memset(&a, 0, sizeof(a));
+    a.indptr = g->indptr; a.indices = g->indices; a.cost = g->cost; a.kind = g->edge_kind;
+    a.roi = roi_bitmaps; a.roi_words = (N + 31) / 32;
+    a.ko_owner = keepout_owner;
+    a.N = N; a.NP = NP;
+    a.X = g->X; a.plane = g->X * g->Y;
+    a.inv_delta = 1.0f / p->delta;
+    a.h_xy = p->h_xy; a.h_z = p->h_z;
+    a.reqs = en.solo_req; a.n_reqs = n_nets;
+    a.ctl = en.solo_ctl;
+    a.keys = en.keys;
+    a.arena = en.solo_arena; a.links = en.solo_links; a.nch = nch;
+    a.touch_shift = tshift;
+    a.stage = en.stage_nodes; a.max_path = en.cap_path;
+    a.pack = en.pack; a.pack_half = half;
+    a.out_len = en.solo_out;
+    a.out_off = (uint32_t *)(en.solo_out + cap);
+    a.out_dist = (float *)(en.solo_out + 2 * cap);
+    a.counters = (unsigned long long *)(en.solo_ctl + 4);
+
+    HIP_CHECK_RET(hipMemcpyAsync(en.solo_req, nets, sizeof(orgpu_net_req) * n_nets, hipMemcpyHostToDevice, s));
+    HIP_CHECK_RET(hipMemsetAsync(en.solo_ctl, 0, 64, s));
+    HIP_CHECK_RET(hipEventRecord(en.ev_t0, s));
+    hipLaunchKernelGGL(k_solo, dim3(std::min(K, n_nets)), dim3(64), 0, s, a);
+    HIP_CHECK_RET(hipGetLastError());
+    HIP_CHECK_RET(hipEventRecord(en.ev_r1, s));
+    HIP_CHECK_RET(hipMemcpyAsync(en.h_solo_ctl, en.solo_ctl, 64, hipMemcpyDeviceToHost, s));
+    for (int part = 0; part < 3; ++part)
+        HIP_CHECK_RET(hipMemcpyAsync(en.h_solo_out + part * cap, en.solo_out + part * cap,
+                                     sizeof(int32_t) * (size_t)n_nets, hipMemcpyDeviceToHost, s));
+    HIP_CHECK_RET(hipStreamSynchronize(s));
+    const uint32_t used = (uint32_t)std::min<uint64_t>(en.h_solo_ctl[1], half);
+    if (used > 0) {
+        HIP_CHECK_RET(hipMemcpyAsync(en.h_stage, en.pack, sizeof(int32_t) * used, hipMemcpyDeviceToHost, s));
+        if (path_edges)
+            HIP_CHECK_RET(hipMemcpyAsync(en.h_stage + half, en.pack + half, sizeof(int32_t) * used,
+                                         hipMemcpyDeviceToHost, s));
+    }
+    HIP_CHECK_RET(hipEventRecord(en.ev_t1, s));
+    HIP_CHECK_RET(hipStreamSynchronize(s));
+    const int32_t *h_len = en.h_solo_out;
+    const uint32_t *h_off = (const uint32_t *)(en.h_solo_out + cap);
+    const float *h_dist = (const float *)(en.h_solo_out + 2 * cap);
+    const unsigned long long *h_cnt = (const unsigned long long *)(en.h_solo_ctl + 4);
+    float kms = 0.0f, rms = 0.0f;
+    hipEventElapsedTime(&kms, en.ev_t0, en.ev_t1);
+    hipEventElapsedTime(&rms, en.ev_t0, en.ev_r1);
+
+    std::vector<int32_t> retry;
+    for (int i = 0; i < n_nets; ++i) {
+        if (h_len[i] == SOLO_KO_SATURATED)
+            return orgpu_fail(ORGPU_ECAPACITY, "route: request %d: its path crosses 31 or more keepout edges, "
+                              "which the search key cannot count", i);
+        if (h_len[i] == SOLO_BROKEN)
+            return orgpu_fail(ORGPU_ECAPACITY, "route: request %d: predecessor chain broken or longer than %d", i,
+                              en.cap_path);
+        if (h_len[i] == SOLO_RETRY) retry.push_back(i);
+    }
+    // requests given back for capacity: routed on the bucket-round engine
+    // (which reuses the pinned staging, so the packed paths are kept first)
+    orgpu_sssp_stats st2;
+    memset(&st2, 0, sizeof(st2));
+    std::vector<int64_t> off2(retry.size() + 1, 0);
+    std::vector<float> dist2(retry.size());
+    std::vector<int32_t> n2, e2, keep_n, keep_e;
+    const int32_t *stage_n = en.h_stage, *stage_e = en.h_stage + half;
+    if (!retry.empty()) {
+        keep_n.assign(en.h_stage, en.h_stage + used);
+        if (path_edges) keep_e.assign(en.h_stage + half, en.h_stage + half + used);
+        stage_n = keep_n.data();
+        stage_e = keep_e.data();
+        std::vector<orgpu_net_req> sub(retry.size());
+        for (size_t j = 0; j < retry.size(); ++j) sub[j] = nets[retry[j]];
+        orgpu_sssp_params p2 = *p;
+        p2.engine = 0;
+        n2.resize((size_t)std::max<int64_t>(path_capacity, 1));
+        if (path_edges) e2.resize(n2.size());
+        rc = engine_route(en, g, sub.data(), (int32_t)sub.size(), roi_bitmaps, n_roi, keepout_owner, keepout_cost,
+                          &p2, n2.data(), path_edges ? e2.data() : nullptr, path_capacity, off2.data(), dist2.data(),
+                          &st2);
+        if (rc) return rc;
+    }
+    int64_t pos = 0;
+    size_t j = 0;
+    for (int i = 0; i < n_nets; ++i) {
+        path_offset[i] = pos;
+        const int32_t *src_n = nullptr, *src_e = nullptr;
+        int64_t len = 0;
+        float d = 0.0f;
+        if (j < retry.size() && retry[j] == i) {
+            len = off2[j + 1] - off2[j];
+            src_n = n2.data() + off2[j];
+            if (path_edges) src_e = e2.data() + off2[j];
+            d = dist2[j];
+            ++j;
+        } else {
+            len = h_len[i];
+            src_n = stage_n + h_off[i];
+            src_e = stage_e + h_off[i];
+            d = h_dist[i];
+        }
+        if (len > 0) {
+            if (pos + len > path_capacity)
+                return orgpu_fail(ORGPU_ECAPACITY, "route: path output capacity %lld exceeded", (long long)path_capacity);
+            memcpy(path_nodes + pos, src_n, sizeof(int32_t) * len);
+            if (path_edges) memcpy(path_edges + pos, src_e, sizeof(int32_t) * len);
+            pos += len;
+        }
+        if (path_dist) path_dist[i] = d;
+    }
+    path_offset[n_nets] = pos;
+    if (stats) {
+        stats->rounds = st2.rounds;
+        stats->items = (int64_t)h_cnt[0] + st2.items;
+        stats->edges = (int64_t)h_cnt[1] + st2.edges;
+        stats->kernel_ms = kms + st2.kernel_ms;
+        stats->launches = 1 + st2.launches;
+        stats->round_ms = rms + st2.round_ms;
+        stats->retried = (int64_t)retry.size();
+    }
+    return ORGPU_OK;
+}

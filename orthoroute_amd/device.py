@@ -218,12 +218,15 @@ class SearchParams:
     runs the plain Dijkstra bucket order.  Paths are the same either way."""
 
     def __init__(self, slots=1, delta=1.0, n_buckets=8, rounds_per_poll=32, grid=0, count_minimal=False,
-                 h_xy=0.0, h_z=0.0, local_batches=0, graph_rounds=False):
+                 h_xy=0.0, h_z=0.0, local_batches=0, graph_rounds=False, engine=0, list_cap=0):
         self.slots, self.delta, self.n_buckets = int(slots), float(delta), int(n_buckets)
         self.rounds_per_poll, self.grid, self.count_minimal = int(rounds_per_poll), int(grid), bool(count_minimal)
         self.h_xy, self.h_z = float(h_xy), float(h_z)
         self.local_batches = int(local_batches)
         self.graph_rounds = bool(graph_rounds)   # replay a poll's rounds as one captured HIP graph
+        # two-pin searches: 0 = bucket rounds over all slots, 1 = one wavefront
+        # per search (list_cap: entries per bucket list, 0 = default)
+        self.engine, self.list_cap = int(engine), int(list_cap)
 
     def c(self):
         p = _abi.SsspParams()
@@ -232,6 +235,7 @@ class SearchParams:
         p.h_xy, p.h_z = self.h_xy, self.h_z
         p.local_batches = self.local_batches
         p.launch_mode = int(self.graph_rounds)
+        p.engine, p.list_cap = self.engine, self.list_cap
         return p
 
 
@@ -347,7 +351,8 @@ def route_flat(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner
     nodes, edges = nodes[:used].copy(), edges[:used].copy()
     stats = {"rounds": st.rounds, "items": st.items, "edges": st.edges,
              "minimal_nodes": st.minimal_nodes, "minimal_edges": st.minimal_edges,
-             "kernel_ms": st.kernel_ms, "launches": st.launches, "round_ms": st.round_ms}
+             "kernel_ms": st.kernel_ms, "launches": st.launches, "round_ms": st.round_ms,
+             "retried": st.retried}
     return nodes, edges, offs, dist[:n], stats
 
 
