@@ -81,8 +81,9 @@ def parse():
                          "against the reference's recorded trajectory (0: skip)")
     ap.add_argument("--cost-reps", type=int, default=5, help="timed repetitions of each cost-update kernel (0: skip)")
     ap.add_argument("--graph-rounds", action="store_true", help="replay the rounds between polls as HIP graphs")
-    ap.add_argument("--engine", type=int, default=0,
-                    help="two-pin search engine: 0 = bucket rounds over all slots, 1 = one wavefront per search")
+    ap.add_argument("--engine", default="auto", choices=["0", "1", "auto"],
+                    help="two-pin search engine: 0 = bucket rounds over all slots, 1 = one wavefront per "
+                         "search, auto = 1 when at least 1024 searches run at once")
     ap.add_argument("--list-cap", type=int, default=0, help="engine 1: entries per bucket list (0 = default)")
     ap.add_argument("--serve-jobs", type=int, default=16, help="serving stream: jobs (0 = skip)")
     ap.add_argument("--serve-max-nodes", type=float, default=4e6, help="serving stream: largest board (nodes)")
@@ -144,6 +145,10 @@ def access_probe(n):
         print(json.dumps(out), flush=True)
 
 
+def engine_arg(v):
+    return v if v == "auto" else int(v)
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -178,7 +183,7 @@ def main():
     cfg.sssp_grid = args.grid
     cfg.sssp_rounds_per_poll = args.rounds_per_poll
     cfg.sssp_graph_rounds = args.graph_rounds
-    cfg.sssp_engine, cfg.sssp_list_cap = args.engine, args.list_cap
+    cfg.sssp_engine, cfg.sssp_list_cap = engine_arg(args.engine), args.list_cap
     r = PathFinderRouter(cfg)
     t0 = time.perf_counter()
     r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
@@ -268,7 +273,9 @@ def main():
     costk = cost_kernels(r, pres_fac, hmult, args.cost_reps) if args.cost_reps > 0 else None
     # the sections below build boards of their own: the backplane's search
     # scratch (each slot's keys and bucket bitmaps) is released first
-    sp_line, slots_line = r._search_params(args.slots), min(args.slots, max_slots(r.N))
+    eng_line = r._engine_for(len(tasks))
+    sp_line = r._search_params(args.slots, engine=eng_line)
+    slots_line = min(args.slots, max_slots(r.N, engine=eng_line))
     r.ctx.close()
     import gc
     gc.collect()
@@ -283,22 +290,26 @@ def main():
     stress_line = stress(args) if args.stress_nets > 0 else None
     serve = serving(args) if args.serve_jobs > 0 else None
 
-    # HBM traffic per launch comes from a committed rocprofv3 --pmc profile;
-    # it is reported only when that profile measured this build's kernel
-    # source (the digest it records equals the digest of sssp.hip now)
+    # the dominant kernel of the step: the one-wave search kernel when the
+    # pass ran on engine 1, else the bucket round kernel.  Its HBM traffic
+    # per launch comes from a committed rocprofv3 --pmc profile and is
+    # reported only when that profile measured this build's kernel source
+    # (the digest it records equals the digest of the source file now)
+    step_engine = r._engine_for(len(tasks))
+    kname, ksrc = ("k_solo", "sssp_solo.hip") if step_engine == 1 else ("k_round", "sssp.hip")
     traffic, traffic_note = None, "no counter profile"
     here = os.path.dirname(os.path.abspath(__file__))
-    prof = os.path.join(here, "profiles", "k_round_cfg2_traffic.json")
+    prof = os.path.join(here, "profiles", f"{kname}_cfg2_traffic.json")
     if os.path.exists(prof):
         import hashlib
         with open(prof) as f:
             tj = json.load(f)
-        with open(os.path.join(here, "orthoroute_amd", "csrc", "sssp.hip"), "rb") as f:
+        with open(os.path.join(here, "orthoroute_amd", "csrc", ksrc), "rb") as f:
             src_digest = hashlib.sha256(f.read()).hexdigest()
         if tj.get("kernel_source_sha256") == src_digest:
             traffic = tj.get("hbm_bytes_per_launch")
             traffic_note = f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel source ({src_digest[:12]}), " \
-                           f"copy-calibrated: profiles/k_round_cfg2_traffic.json"
+                           f"copy-calibrated: profiles/{kname}_cfg2_traffic.json"
         else:
             traffic_note = "the counter profile measured another version of the kernel source: not reported"
 
@@ -320,7 +331,7 @@ def main():
                         f"{r.accounting.E} edges, {len(tasks)} two-pin nets; one step = one full routing "
                         f"pass on the iteration-1 cost snapshot",
             "nodes": r.N, "edges": r.accounting.E, "nets": len(tasks),
-            "slots": slots_line, "delta": sp_line.delta, "n_buckets": sp_line.n_buckets,
+            "engine": step_engine, "slots": slots_line, "delta": sp_line.delta, "n_buckets": sp_line.n_buckets,
             "local_batches": args.local_batches or 64, "grid": args.grid or 1024,
             "rounds_per_poll": args.rounds_per_poll,
         },
@@ -341,7 +352,7 @@ def main():
                              "batched_s": round(timing["batched"], 3),
                              "speedup": round(timing["plain"] / max(timing["batched"], 1e-9), 2),
                              "identical_paths": same},
-        "roofline": {"kernel": "k_round", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+        "roofline": {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "traffic_note": traffic_note,
                      "avg_launch_us": round(avg_launch_us, 2), "launches": int(launches),
@@ -439,7 +450,7 @@ def negotiation(args, which, cap, with_plain, trees=False):
         cfg.batch_slots = slots
         cfg.max_iterations_cap = cap
         cfg.sssp_graph_rounds = args.graph_rounds
-        cfg.sssp_engine, cfg.sssp_list_cap = args.engine, args.list_cap
+        cfg.sssp_engine, cfg.sssp_list_cap = engine_arg(args.engine), args.list_cap
         cfg.multi_terminal_enabled = trees
         r = PathFinderRouter(cfg)
         r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
@@ -609,7 +620,7 @@ def stress(args):
     cfg.lattice_policy = "adjacent"
     cfg.batch_slots = slots
     cfg.sssp_graph_rounds = args.graph_rounds
-    cfg.sssp_engine, cfg.sssp_list_cap = args.engine, args.list_cap
+    cfg.sssp_engine, cfg.sssp_list_cap = engine_arg(args.engine), args.list_cap
     r = PathFinderRouter(cfg)
     r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
     tasks = r._parse_requests(reqs)
@@ -669,7 +680,7 @@ def time_to_zero_overuse(args):
         cfg.lattice_policy = "reference"
         cfg.batch_slots = slots
         cfg.sssp_graph_rounds = args.graph_rounds
-        cfg.sssp_engine, cfg.sssp_list_cap = args.engine, args.list_cap
+        cfg.sssp_engine, cfg.sssp_list_cap = engine_arg(args.engine), args.list_cap
         r = PathFinderRouter(cfg)
         r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
         torch.cuda.synchronize()

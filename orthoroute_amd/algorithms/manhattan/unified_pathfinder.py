@@ -152,9 +152,11 @@ class PathFinderConfig:
     sssp_grid = 0                  # workgroups per round launch (0 = library default)
     sssp_rounds_per_poll = 32      # rounds queued between status polls / slot refills
     sssp_graph_rounds = False      # replay the rounds between polls as one captured HIP graph
-    sssp_engine = 0                # two-pin searches: 0 = bucket rounds over all slots, 1 = one
-                                   # wavefront per search (same paths)
-    sssp_list_cap = 0              # engine 1: entries per bucket list (0 = library default)
+    sssp_engine = "auto"           # two-pin searches: 0 = bucket rounds over all slots, 1 = one
+                                   # wavefront per search, "auto" = 1 when at least sssp_solo_min
+                                   # searches run at once, else 0 (same paths either way)
+    sssp_solo_min = 1024
+    sssp_list_cap = 0              # engine 1: bucket-list entries per search (0 = library default)
     sssp_target_buckets = None     # widen the bucket so a batch's deep searches span about this many
                                    # (None: 128 from 1 M nodes up, 32 below)
     max_iterations_cap = None
@@ -1165,15 +1167,27 @@ class PathFinderRouter:
         return victims
 
     # ---------------- batched search of one iteration (UPF:4441-4823)
-    def _search_params(self, slots, depth=None):
+    def _engine_for(self, n_searches):
+        """Search engine of a batch of n two-pin searches: the one-wave engine
+        fills the GPU with many searches (one wavefront each), the bucket-round
+        engine spreads a few large searches (late iterations, keepout
+        searches that exhaust the keepout-free region first) over every CU."""
+        e = self.config.sssp_engine
+        if e == "auto":
+            n = min(int(n_searches), int(self.config.batch_slots))
+            return 1 if n >= int(self.config.sssp_solo_min) else 0
+        return int(e or 0)
+
+    def _search_params(self, slots, depth=None, engine=0):
         """depth: the batch's estimated search depth in cost units beyond
         each source's lower bound (None = unknown); with no fixed delta
         configured, the bucket is widened so that this depth spans about
-        config.sssp_target_buckets buckets."""
+        config.sssp_target_buckets buckets.  engine: the search engine the
+        batch runs on (its base bucket width differs)."""
         d = self.config.sssp_delta
         nb = self.config.sssp_buckets
         if d is None or nb is None:
-            d2, nb2 = base_width(self, getattr(self, "_cost_scale", 0.0)), 8
+            d2, nb2 = base_width(self, getattr(self, "_cost_scale", 0.0), engine), 8
             target = self.config.sssp_target_buckets
             if target is None:
                 # a negotiation's tuner picks the target of batches with a
@@ -1197,8 +1211,7 @@ class PathFinderRouter:
                             grid=int(self.config.sssp_grid or 0), count_minimal=bool(self.config.count_minimal),
                             h_xy=hx, h_z=hz, local_batches=int(self.config.sssp_local_batches or 0),
                             graph_rounds=bool(getattr(self.config, "sssp_graph_rounds", False)),
-                            engine=int(getattr(self.config, "sssp_engine", 0) or 0),
-                            list_cap=int(getattr(self.config, "sssp_list_cap", 0) or 0))
+                            engine=int(engine), list_cap=int(getattr(self.config, "sssp_list_cap", 0) or 0))
 
     def _depth_hint(self, ordered, sd):
         """How far past their first bucket the searches of a batch of nets
@@ -1284,9 +1297,9 @@ class PathFinderRouter:
         todo = np.nonzero(keep)[0]
         if not len(todo):
             return out
-        # a work item is the u32 slot*N + node: at most (2^32 - 64) / N slots
-        slots = max(1, min(int(self.config.batch_slots), len(todo), max_slots(self.N)))
-        params = self._search_params(slots, depth_hint)
+        eng = 0 if ref_order else self._engine_for(len(todo))
+        slots = max(1, min(int(self.config.batch_slots), len(todo), max_slots(self.N, engine=eng)))
+        params = self._search_params(slots, depth_hint, engine=eng)
         sreqs = np.stack([src[todo], dst[todo], nid[todo], row_of[todo],
                           np.full(len(todo), int(keepout_owner is not None), np.int64)], axis=1)
         ko = None
@@ -1707,7 +1720,8 @@ class PathFinderRouter:
         # the bucket target of the batches with a depth estimate, tuned on
         # this negotiation's own search times (speed only: paths are the same)
         slots_key = int(min(int(cfg.batch_slots), max_slots(self.N)) > 1)
-        self._tuner = BucketTuner(f"{self._Nx}x{self._Ny}x{self._Nz}:{self.lattice.policy}:{slots_key}", self.N)
+        self._tuner = BucketTuner(f"{self._Nx}x{self._Ny}x{self._Nz}:{self.lattice.policy}:{slots_key}:"
+                                  f"{cfg.sssp_engine}", self.N)
         self._net_failure_count = {}
         self._excluded_nets = set()
         routed = failed = 0

@@ -30,18 +30,27 @@ than 8,192 in the first congested iterations).
 import math
 
 # targets a tuner moves between (x4 apart)
-TARGETS = (32, 128, 512, 2048, 8192, 32768)
+TARGETS = (32, 128, 512, 2048, 8192, 32768, 131072)
 _SETTLED = {}   # lattice key -> settled target, for later negotiations in this process
 
 
-def base_width(router, cost_scale=0.0):
-    """Bucket width with no depth estimate: 2.4 x the iteration's median
-    planar cost (cost_scale, rounded up to a power of two), or 4 base steps
-    when the costs are not known yet."""
+# base width in typical planar steps per search engine: the bucket-round
+# engine pays a launch pair per bucket and prefers wide buckets; the one-wave
+# engine advances a bucket inside the wave and prefers narrow ones (on the
+# 8,000-net backplane pass: 47 ms at 0.15-0.3 cost units, 77 ms at 1.2)
+WIDTH_STEPS = {0: 2.4, 1: 0.5}
+
+
+def base_width(router, cost_scale=0.0, engine=0):
+    """Bucket width with no depth estimate: WIDTH_STEPS of the search engine
+    times the iteration's median planar cost (cost_scale, rounded up to a
+    power of two), or 5/3 of that many base steps when the costs are not
+    known yet."""
+    k = WIDTH_STEPS.get(int(engine), 2.4)
     if cost_scale > 0.0:
-        return max(2.4 * float(cost_scale), 1e-3)
+        return max(k * float(cost_scale), 1e-3)
     step = float(router.config.base_cost_weight) * float(router.lattice.pitch)
-    return max(step * 4.0, 1e-3)
+    return max(step * k * (4.0 / 2.4), 1e-3)
 
 
 def default_target(n_nodes):

@@ -11,6 +11,7 @@ struct Engine {
     // capacities reserved
     int64_t cap_nodes = 0;
     int32_t cap_slots = 0, cap_nb = 0, cap_path = 0;
+    int32_t cap_bslots = 0;   // slots the bucket-round engine's structures hold (<= cap_slots)
     uint64_t words = 0;
     // device scratch
     uint64_t *keys = nullptr;

@@ -41,7 +41,8 @@
 
 #define WG 256
 #define WAVES_PER_WG (WG / 64)
-#define MAX_SLOTS 2048
+#define MAX_SLOTS 2048          // slots of the bucket-round engine
+#define RESERVE_MAX_SLOTS 4096  // slots a handle may reserve (one-wave engine)
 #define CHUNK_ITEMS 2048
 #define MAX_NB MAX_RING
 
@@ -1092,15 +1093,21 @@ static int launch_rounds(Engine &en, const RoundArgs &a, int grid, int rpp, int 
 
 int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets, int64_t qcap, int32_t max_path)
 {
-    if (slots < 1 || slots > MAX_SLOTS) return orgpu_fail(ORGPU_EINVAL, "reserve: slots %d outside [1, %d]", slots, MAX_SLOTS);
+    if (slots < 1 || slots > RESERVE_MAX_SLOTS)
+        return orgpu_fail(ORGPU_EINVAL, "reserve: slots %d outside [1, %d]", slots, RESERVE_MAX_SLOTS);
     if (n_buckets < 1 || n_buckets > MAX_NB) return orgpu_fail(ORGPU_EINVAL, "reserve: n_buckets %d outside [1, %d]", n_buckets, MAX_NB);
     if (n_nodes < 1 || n_nodes >= KEY_MAX_NODES) return orgpu_fail(ORGPU_EINVAL, "reserve: bad node count");
     if (max_path < 2) max_path = 1 << 16;
-    // work items slot * NP + node: NP is n_nodes rounded up to whole chunks
+    // work items slot * NP + node: NP is n_nodes rounded up to whole chunks.
+    // Keys, touch records and path staging are kept for every slot; the
+    // bucket-round engine's bitmaps and slot records for at most MAX_SLOTS
+    // of them (the one-wave engine needs none of those).
     const uint64_t np = ((uint64_t)n_nodes + CHUNK_ITEMS - 1) / CHUNK_ITEMS * CHUNK_ITEMS;
-    const uint64_t items = np * (uint64_t)slots;
-    if (items / CHUNK_ITEMS >= (1ull << 32))
-        return orgpu_fail(ORGPU_EINVAL, "reserve: slots*nodes = %llu exceeds 2^43 work items", (unsigned long long)items);
+    const int32_t bslots = std::min(slots, MAX_SLOTS);
+    const uint64_t keys_n = np * (uint64_t)slots;
+    const uint64_t items = np * (uint64_t)bslots;
+    if (keys_n / CHUNK_ITEMS >= (1ull << 32))
+        return orgpu_fail(ORGPU_EINVAL, "reserve: slots*nodes = %llu exceeds 2^43 work items", (unsigned long long)keys_n);
     (void)qcap;   // frontiers are bitmaps: no item queues to size
     if (en.cap_nodes >= n_nodes && en.cap_slots >= slots && en.cap_nb >= n_buckets &&
         en.cap_path >= max_path)
@@ -1110,7 +1117,7 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
     const uint64_t chunks = (words + 63) / 64;
     const uint64_t summ_words = (chunks + 31) / 32;
     const int nbufs = n_buckets + 3;
-    HIP_CHECK_RET(hipMalloc((void **)&en.keys, items * sizeof(uint64_t)));
+    HIP_CHECK_RET(hipMalloc((void **)&en.keys, keys_n * sizeof(uint64_t)));
     HIP_CHECK_RET(hipMalloc((void **)&en.bits, (uint64_t)nbufs * words * sizeof(uint32_t)));
     HIP_CHECK_RET(hipMemsetAsync(en.bits, 0, (uint64_t)nbufs * words * sizeof(uint32_t), en.stream));
     HIP_CHECK_RET(hipMalloc((void **)&en.summ, (uint64_t)nbufs * summ_words * sizeof(uint32_t)));
@@ -1151,7 +1158,7 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
     HIP_CHECK_RET(hipEventCreate(&en.ev_t1));
     HIP_CHECK_RET(hipEventCreate(&en.ev_r0));
     HIP_CHECK_RET(hipEventCreate(&en.ev_r1));
-    en.cap_nodes = n_nodes; en.cap_slots = slots; en.cap_nb = n_buckets; en.cap_path = max_path;
+    en.cap_nodes = n_nodes; en.cap_slots = slots; en.cap_bslots = bslots; en.cap_nb = n_buckets; en.cap_path = max_path;
     en.words = words;
     en.summ_words = summ_words;
     en.chunks = chunks;
@@ -1315,6 +1322,8 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     if (p->engine == 1 && !ms && !p->count_minimal)
         return engine_route_solo(en, g, nets, n_nets, roi_bitmaps, n_roi, keepout_owner, keepout_cost, p, path_nodes,
                                  path_edges, path_capacity, path_offset, path_dist, stats);
+    if (K > en.cap_bslots)
+        return orgpu_fail(ORGPU_EINVAL, "route: %d slots on the bucket-round engine (at most %d)", K, en.cap_bslots);
     hipStream_t s = en.stream;
     if (ms) {
         if ((ms->n_seeds > 0 && !ms->seeds) || (ms->n_targets > 0 && !ms->targets) || (n_nets > 0 && !ms->reqs) ||

@@ -12,14 +12,17 @@
 //    (keys u64[NP] of its own) and takes requests from a global counter
 //    until none are left.  No workgroup ever waits for another one.
 //  * the wave alone reads and writes its slot's keys, so a relaxation is a
-//    plain store followed by a reload: lanes of the wave that raced on one
-//    node see the winner and the better candidates store again until the
-//    minimum stands (no global atomics: every device-scope atomic on this
-//    GPU is performed at the memory side).
-//  * the frontier is a set of per-wave lists in device memory (entries
-//    (key cost, node) in the key's own bit layout): a ring of SOLO_NBR
-//    bucket lists for the window [W, W + SOLO_NBR) and two overflow lists
-//    used in turn, all chains of chunks of the slot's arena.  An entry whose cost no longer equals its node's key is
+//    workgroup-scope 64-bit atomicMin (no device-wide ordering) after a
+//    plain-load filter, and the ROI bit of a target is read only when the
+//    candidate would improve it.
+//  * the frontier is a radix heap of per-wave lists in device memory
+//    (entries (key cost, node) in the key's own bit layout; chains of chunks
+//    of the slot's arena): list 0 holds the current bucket, list (i, d) the
+//    entries whose bucket agrees with the current one above hex digit i and
+//    has digit d there.  Expanding the next bucket empties the lowest
+//    non-empty list into lower ones, so an entry moves at most once per
+//    digit, however far ahead its bucket lies (keepout edges put whole
+//    buckets 2^32 apart).  An entry whose cost no longer equals its node's key is
 //    stale and skipped (every strict improvement pushes a new entry).
 //    Appends are wave-aggregated (one ballot per target list), so list
 //    traffic is coalesced and costs no atomics.
@@ -39,8 +42,8 @@
 #include "common.h"
 #include "engine.h"
 
-#define SOLO_NBR 16                 // ring bucket lists
-#define SOLO_NL (SOLO_NBR + 2)      // plus two overflow lists
+#define SOLO_LEVELS 10              // hex digits of a bucket index (keepout count and 32 bits)
+#define SOLO_NL (1 + SOLO_LEVELS * 16)   // current bucket + one list per (digit, value)
 #define SOLO_TOUCH_WORDS 1024       // LDS touch bitmap: 32768 key chunks
 #define SOLO_MAX_STEPS (1u << 22)  // expansion steps of 64 entries per search
 
@@ -137,6 +140,7 @@ struct SoloLists {
     uint32_t nch;
     uint32_t *hc, *hp, *tc, *tp, *cnt;   // LDS [SOLO_NL]
     uint32_t *al;                        // LDS [2]: free-stack top, bump pointer
+    uint32_t *mask;                      // LDS [SOLO_LEVELS]: non-empty lists of each digit
 
     __device__ __forceinline__ bool empty(uint32_t t) const { return cnt[t] == 0u; }
     __device__ __forceinline__ uint32_t alloc() const
@@ -155,8 +159,11 @@ struct SoloLists {
     }
     __device__ __forceinline__ void reset(uint32_t lane) const
     {
-        if (lane < SOLO_NL) { hc[lane] = SOLO_NONE; tc[lane] = SOLO_NONE; hp[lane] = 0u; tp[lane] = 0u; cnt[lane] = 0u; }
+        for (uint32_t t = lane; t < SOLO_NL; t += 64) {
+            hc[t] = SOLO_NONE; tc[t] = SOLO_NONE; hp[t] = 0u; tp[t] = 0u; cnt[t] = 0u;
+        }
         if (lane < 2) al[lane] = 0u;
+        if (lane < SOLO_LEVELS) mask[lane] = 0u;
     }
     // Takes up to 64 entries from the head of list t: lane k < n gets
     // entry k.  The chunks it empties go back to the free stack; the
@@ -177,6 +184,7 @@ struct SoloLists {
         const uint32_t left = c - n;
         cnt[t] = left;
         if (left == 0u) {
+            if (t > 0u) mask[(t - 1u) >> 4] &= ~(1u << ((t - 1u) & 15u));
             release(h);
             if (cross) release(nx);
             hc[t] = SOLO_NONE; tc[t] = SOLO_NONE; hp[t] = 0u; tp[t] = 0u;
@@ -227,8 +235,19 @@ __device__ __forceinline__ bool lists_append(const SoloLists &L, bool act, uint3
         if (c2 != SOLO_NONE) { L.tc[t0] = c2; L.tp[t0] = tp + cnt - SOLO_CH; }
         else { L.tc[t0] = tc; L.tp[t0] = tp + cnt; }
         L.cnt[t0] += cnt;
+        if (t0 > 0u) L.mask[(t0 - 1u) >> 4] |= 1u << ((t0 - 1u) & 15u);
     }
     return ok;
+}
+
+// List of bucket b relative to the current bucket cur: 0 for b <= cur,
+// else (i, d) = the highest hex digit where b differs from cur and b's value
+// there.
+__device__ __forceinline__ uint32_t radix_list(uint64_t b, uint64_t cur)
+{
+    if (b <= cur) return 0u;
+    const int i = (63 - __builtin_clzll(b ^ cur)) >> 2;
+    return 1u + (uint32_t)i * 16u + ((uint32_t)(b >> (4 * i)) & 15u);
 }
 
 __device__ __forceinline__ float solo_heur(const SoloArgs &a, int dx, int dy, int dz, uint32_t v)
@@ -246,11 +265,12 @@ k_solo(SoloArgs a)
 {
     __shared__ uint32_t s_touch[SOLO_TOUCH_WORDS];
     __shared__ uint32_t s_hc[SOLO_NL], s_hp[SOLO_NL], s_tc[SOLO_NL], s_tp[SOLO_NL], s_cnt[SOLO_NL], s_al[2];
+    __shared__ uint32_t s_mask[SOLO_LEVELS];
     const uint32_t lane = threadIdx.x;
     const uint32_t slot = blockIdx.x;
     uint64_t *keys = a.keys + (uint64_t)slot * (uint64_t)a.NP;
     SoloLists L{a.arena + (uint64_t)slot * (uint64_t)a.nch * SOLO_CH, a.links + (uint64_t)slot * 2u * a.nch,
-                a.links + (uint64_t)slot * 2u * a.nch + a.nch, a.nch, s_hc, s_hp, s_tc, s_tp, s_cnt, s_al};
+                a.links + (uint64_t)slot * 2u * a.nch + a.nch, a.nch, s_hc, s_hp, s_tc, s_tp, s_cnt, s_al, s_mask};
     int32_t *stage = a.stage + (uint64_t)slot * (uint64_t)a.max_path;
     const uint32_t tshift = a.touch_shift;
     for (int i = lane; i < SOLO_TOUCH_WORDS; i += 64) s_touch[i] = 0u;
@@ -279,14 +299,11 @@ k_solo(SoloArgs a)
         // seed the source in its own bucket
         const uint64_t k0 = key_pack(0, 0.0f, KEY_NOPRED);
         uint64_t cur = key_bucket_rel(k0, a.inv_delta, solo_heur(a, dx, dy, dz, src));
-        uint64_t W = cur;              // window of the ring: [W, W + SOLO_NBR)
-        uint32_t ovf = SOLO_NBR;       // overflow list in use
-        uint64_t ovf_min = ~0ull;      // lower bound of its buckets
         if (lane == 0) {
             kstore(&keys[src], k0);
             atomicOr(&s_touch[(src >> tshift) >> 5], 1u << ((src >> tshift) & 31));
         }
-        bool ok = lists_append(L, lane == 0, (uint32_t)(cur % SOLO_NBR), entry_pack(k0, src));
+        bool ok = lists_append(L, lane == 0, 0u, entry_pack(k0, src));
         wave_fence();
         bool reached = false;
         // a bound on the expansion steps of one search, so that every wave
@@ -296,7 +313,7 @@ k_solo(SoloArgs a)
         while (ok) {
             // expand the current bucket to its fixed point (entries pushed
             // into it meanwhile are appended to the same list)
-            const uint32_t lc = (uint32_t)(cur % SOLO_NBR);
+            const uint32_t lc = 0u;
             while (ok && !L.empty(lc)) {
                 if (++steps > SOLO_MAX_STEPS) { ok = false; break; }
                 uint32_t n = 0;
@@ -336,22 +353,19 @@ k_solo(SoloArgs a)
                             if (ko_node) pko[j] = a.kind[row + k0e + j] == 0;
                         }
                     }
-                    uint32_t rw[4];
-                    uint64_t kv[4];
+                    // target keys first; the ROI bit is read only for a
+                    // candidate that would improve its target (an edge to a
+                    // settled neighbour, the common case, costs one load)
+                    uint64_t kv[4], nk[4];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        rw[j] = 0xFFFFFFFFu;
                         kv[j] = KEY_INF;
-                        if (!go[j]) continue;
-                        if (roi) rw[j] = roi[v[j] >> 5];
-                        kv[j] = kload(&keys[v[j]]);
+                        nk[j] = KEY_INF;
+                        if (go[j]) kv[j] = kload(&keys[v[j]]);
                     }
-                    uint64_t nk[4];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        nk[j] = KEY_INF;
                         if (!go[j]) continue;
-                        if (!((rw[j] >> (v[j] & 31)) & 1u)) { go[j] = false; continue; }
                         uint32_t ko = ko0;
                         float nd = d0;
                         if (pko[j]) ko = ko < 31 ? ko + 1 : 31;
@@ -359,95 +373,72 @@ k_solo(SoloArgs a)
                         nk[j] = key_pack(ko, nd, u + 1u);
                         go[j] = nd < __builtin_inff() && nk[j] < kv[j];
                     }
-                    // store, then reload: a lane that sees a larger key than
-                    // its candidate lost a race inside the wave and stores
-                    // again; one that sees its own candidate holds the node
-                    bool pend[4], won[4];
+                    if (roi) {
+                        uint32_t rw[4];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) { pend[j] = go[j]; won[j] = false; }
-                    for (;;) {
-                        bool any = false;
+                        for (int j = 0; j < 4; ++j) rw[j] = go[j] ? roi[v[j] >> 5] : 0u;
 #pragma unroll
-                        for (int j = 0; j < 4; ++j) any = any || pend[j];
-                        if (__ballot(any) == 0ull) break;
+                        for (int j = 0; j < 4; ++j) go[j] = go[j] && ((rw[j] >> (v[j] & 31)) & 1u);
+                    }
+                    // relax: only this wave writes the slot's keys, so a
+                    // workgroup-scope atomicMin orders the lanes' candidates
+                    uint64_t old[4];
+                    bool won[4];
 #pragma unroll
-                        for (int j = 0; j < 4; ++j)
-                            if (pend[j]) kstore(&keys[v[j]], nk[j]);
-                        wave_fence();
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            if (!pend[j]) continue;
-                            const uint64_t kr = kload(&keys[v[j]]);
-                            pend[j] = kr > nk[j];
-                            won[j] = kr == nk[j];
-                        }
+                    for (int j = 0; j < 4; ++j) {
+                        old[j] = KEY_INF;
+                        if (go[j]) old[j] = __hip_atomic_fetch_min(&keys[v[j]], nk[j], __ATOMIC_RELAXED,
+                                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                        won[j] = go[j] && nk[j] < old[j];
                     }
                     // touched chunks and pushes of strict improvements
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        if (go[j] && kv[j] == KEY_INF) {
+                        if (go[j] && old[j] == KEY_INF) {
                             const uint32_t ch = v[j] >> tshift;
                             atomicOr(&s_touch[ch >> 5], 1u << (ch & 31));
                         }
-                        const bool push = won[j] && key_cost(nk[j]) < key_cost(kv[j]);
-                        uint32_t t = 0;
-                        if (push) {
-                            const uint64_t b = key_bucket_rel(nk[j], a.inv_delta, solo_heur(a, dx, dy, dz, v[j]));
-                            if (b <= cur) t = lc;
-                            else if (b < W + SOLO_NBR) t = (uint32_t)(b % SOLO_NBR);
-                            else {
-                                t = ovf;
-                                ovf_min = b < ovf_min ? b : ovf_min;
-                            }
-                        }
+                        const bool push = won[j] && key_cost(nk[j]) < key_cost(old[j]);
+                        const uint32_t t = push ? radix_list(key_bucket_rel(nk[j], a.inv_delta,
+                                                                            solo_heur(a, dx, dy, dz, v[j])), cur) : 0u;
                         ok = lists_append(L, push, t, entry_pack(nk[j], v[j])) && ok;
                     }
                 }
-                ovf_min = wave_min64(ovf_min);
                 wave_fence();
             }
             if (!ok) break;
-            // next bucket to expand: the first listed one in the window, else
-            // the overflow list's lowest (then redistributed)
+            // next bucket to expand: the lowest non-empty list's lower bound
+            // (the current bucket's digits above i, digit d, zeros below)
             uint64_t next = ~0ull;
-            for (uint64_t b = cur + 1; b < W + SOLO_NBR; ++b) {
-                const uint32_t t = (uint32_t)(b % SOLO_NBR);
-                if (!L.empty(t)) { next = b; break; }
+            uint32_t tl = 0u;
+            for (uint32_t i = 0; i < SOLO_LEVELS; ++i) {
+                const uint32_t m = L.mask[i];
+                if (m) {
+                    const uint32_t d = (uint32_t)__builtin_ctz(m);
+                    const uint64_t low = (16ull << (4 * i)) - 1ull;
+                    next = (cur & ~low) | ((uint64_t)d << (4 * i));
+                    tl = 1u + i * 16u + d;
+                    break;
+                }
             }
-            const bool from_ovf = next == ~0ull && !L.empty(ovf);
-            if (from_ovf) next = ovf_min;
             const uint64_t kd = kload(&keys[dst]);
             if (kd != KEY_INF && (next == ~0ull || key_bucket_rel(kd, a.inv_delta) + margin < next)) {
                 reached = true;
                 break;
             }
             if (next == ~0ull) break;   // frontier empty: dst unreachable
+            // the list's live entries move to lower lists relative to the new
+            // current bucket (they agree with it above digit i)
             cur = next;
-            if (!from_ovf) continue;
-            // window jump: live overflow entries move into the new window's
-            // ring lists or into the other overflow list
-            W = cur;
-            const uint32_t old = ovf;
-            ovf = old == SOLO_NBR ? SOLO_NBR + 1 : SOLO_NBR;
-            ovf_min = ~0ull;
-            while (ok && !L.empty(old)) {
+            while (ok && !L.empty(tl)) {
                 uint32_t n = 0;
-                const uint64_t e = L.pop(old, lane, n);
+                const uint64_t e = L.pop(tl, lane, n);
                 bool has = lane < n;
                 const uint32_t v = (uint32_t)(e & KEY_PRED_MASK);
-                uint64_t kv = has ? kload(&keys[v]) : 0ull;
+                const uint64_t kv = has ? kload(&keys[v]) : 0ull;
                 has = has && key_cost(kv) == key_cost(e);
-                uint32_t t = 0;
-                if (has) {
-                    const uint64_t b = key_bucket_rel(kv, a.inv_delta, solo_heur(a, dx, dy, dz, v));
-                    if (b < W + SOLO_NBR) t = (uint32_t)((b < cur ? cur : b) % SOLO_NBR);
-                    else {
-                        t = ovf;
-                        ovf_min = b < ovf_min ? b : ovf_min;
-                    }
-                }
+                const uint32_t t = has ? radix_list(key_bucket_rel(kv, a.inv_delta, solo_heur(a, dx, dy, dz, v)), cur) : 0u;
                 ok = lists_append(L, has, t, e) && ok;
-                ovf_min = wave_min64(ovf_min);
             }
         }
 
@@ -688,6 +679,7 @@ int engine_route_solo(Engine &en, const orgpu_graph *g, const orgpu_net_req *net
         for (size_t j = 0; j < retry.size(); ++j) sub[j] = nets[retry[j]];
         orgpu_sssp_params p2 = *p;
         p2.engine = 0;
+        p2.slots = std::min(p->slots, en.cap_bslots);
         n2.resize((size_t)std::max<int64_t>(path_capacity, 1));
         if (path_edges) e2.resize(n2.size());
         rc = engine_route(en, g, sub.data(), (int32_t)sub.size(), roi_bitmaps, n_roi, keepout_owner, keepout_cost,

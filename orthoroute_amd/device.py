@@ -26,7 +26,9 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
-MAX_SLOTS = 2048       # concurrent searches one engine holds
+MAX_SLOTS = 2048       # concurrent searches of the bucket-round engine (engine 0)
+SOLO_MAX_SLOTS = 4096  # concurrent searches of the one-wave engine (engine 1)
+SOLO_ARENA = 1 << 20   # engine 1: bucket-list entries per search (the library default)
 CHUNK_ITEMS = 2048     # a search's keys and bitmap bits are laid out per whole chunk
 _DEVICE_BYTES = None
 
@@ -40,22 +42,34 @@ def device_memory():
     return _DEVICE_BYTES
 
 
-def slot_bytes(n_nodes, n_buckets=8):
+def slot_bytes(n_nodes, n_buckets=8, engine=0):
     """Engine memory one concurrent search takes on a graph of n_nodes: an
-    8-byte key and n_buckets + 3 bucket-bitmap bits per node (the per-slot
-    stride rounds n_nodes up to a whole chunk), and its touched-chunk
-    records."""
+    8-byte key per node (the per-slot stride rounds n_nodes up to a whole
+    chunk), its touched-chunk records, and then n_buckets + 3 bucket-bitmap
+    bits per node (engine 0) or its bucket-list arena (engine 1)."""
     np_ = -(-int(n_nodes) // CHUNK_ITEMS) * CHUNK_ITEMS
-    return 8 * np_ + (int(n_buckets) + 3) * np_ // 8 + 8 * (np_ // 1024 + 1)
+    common = 8 * np_ + 8 * (np_ // 1024 + 1)
+    if engine == 1:
+        return common + 8 * SOLO_ARENA + 8 * (SOLO_ARENA // 256)
+    return common + (int(n_buckets) + 3) * np_ // 8
 
 
-def max_slots(n_nodes, budget_bytes=None, n_buckets=8):
+def max_slots(n_nodes, budget_bytes=None, n_buckets=8, engine=0):
     """Concurrent searches a graph of n_nodes allows: as many as the
     engine's slot memory (slot_bytes each) fits in budget_bytes (default 70 %
-    of the device's memory), at most MAX_SLOTS."""
+    of the device's memory), at most MAX_SLOTS (engine 0) or SOLO_MAX_SLOTS
+    (engine 1, whose handle still holds the bucket engine's bitmaps for up to
+    MAX_SLOTS of its slots, for searches it hands back)."""
     if budget_bytes is None:
         budget_bytes = 0.7 * device_memory()
-    return max(1, min(MAX_SLOTS, int(budget_bytes // slot_bytes(n_nodes, n_buckets))))
+    if engine != 1:
+        return max(1, min(MAX_SLOTS, int(budget_bytes // slot_bytes(n_nodes, n_buckets))))
+    s1 = slot_bytes(n_nodes, n_buckets, 1)
+    bm = slot_bytes(n_nodes, n_buckets, 0) - slot_bytes(n_nodes, n_buckets, 1) + 8 * SOLO_ARENA + 8 * (SOLO_ARENA // 256)
+    k = int(budget_bytes // (s1 + bm))
+    if k > MAX_SLOTS:
+        k = int((budget_bytes - MAX_SLOTS * bm) // s1)
+    return max(1, min(SOLO_MAX_SLOTS, k))
 
 
 class Context:

@@ -33,7 +33,7 @@ def run_oracle(X, Y, L, policy, reqs, cap):
     return res, rec
 
 
-def run_gpu(X, Y, L, policy, reqs, cap, slots, delta=1.0, target_buckets=None, graph=False):
+def run_gpu(X, Y, L, policy, reqs, cap, slots, delta=1.0, target_buckets=None, graph=False, engine="auto"):
     from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig, PathFinderRouter
     cfg = PathFinderConfig()
     cfg.lattice_policy = policy
@@ -44,6 +44,7 @@ def run_gpu(X, Y, L, policy, reqs, cap, slots, delta=1.0, target_buckets=None, g
     if target_buckets is not None:
         cfg.sssp_target_buckets = target_buckets
     cfg.sssp_graph_rounds = graph
+    cfg.sssp_engine = engine
     r = PathFinderRouter(cfg)
     r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
     rec = []
@@ -70,6 +71,23 @@ def test_negotiation_matches_oracle(case, slots):
     assert bool(g_res.get("success")) == bool(o_res.get("success"))
 
 
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("delta", [1.0, None], ids=["fixed", "tuned"])
+def test_negotiation_solo_engine_matches_oracle(case, delta):
+    """Every search of the negotiation on the one-wave engine (keepout
+    iterations included): the oracle's trajectory, history bits and paths."""
+    name, X, Y, L, policy, n, seed, cap = case
+    print(f"[router] {name} engine 1 delta={delta}", flush=True)
+    reqs = requests(X, Y, n, seed)
+    o_res, o_rec = run_oracle(X, Y, L, policy, reqs, cap)
+    g_res, g_rec = run_gpu(X, Y, L, policy, reqs, cap, 64, delta=delta, engine=1)
+    assert [r[:3] for r in g_rec] == [r[:3] for r in o_rec]
+    for a, b in zip(g_rec, o_rec):
+        np.testing.assert_array_equal(a[3].view(np.uint32), b[3].view(np.uint32), err_msg=f"history it {a[0]}")
+    for nm, *_ in reqs:
+        assert list(g_res["paths"].get(nm, [])) == list(o_res["paths"].get(nm, [])), nm
+
+
 @pytest.mark.parametrize("shape", [(12, 10, 16, 30, 16), (10, 8, 24, 24, 24)], ids=["l16", "l24"])
 def test_negotiation_many_layers_matches_oracle(shape, monkeypatch):
     """14 and 22 signal layers with a fast pres_fac ramp (the reference's
@@ -89,15 +107,16 @@ def test_negotiation_many_layers_matches_oracle(shape, monkeypatch):
         assert list(g_res["paths"].get(nm, [])) == list(o_res["paths"].get(nm, [])), nm
 
 
-@pytest.mark.parametrize("policy", ["reference", "adjacent"])
-def test_cfg1_full_negotiation_matches_oracle(policy):
+@pytest.mark.parametrize("policy,engine", [("reference", "auto"), ("adjacent", "auto"), ("reference", 1)])
+def test_cfg1_full_negotiation_matches_oracle(policy, engine):
     """The whole cfg1 board: 96x96x6, its 300 seeded top-layer nets, 30
     iterations, batched on the GPU vs the oracle with the same search
-    semantics (per-iteration overuse, history bits, final paths)."""
+    semantics (per-iteration overuse, history bits, final paths); also
+    with every search on the one-wave engine."""
     from orthoroute_amd import workloads
     X, Y, L, reqs = workloads.cfg1()
-    print(f"[router] cfg1 {X}x{Y}x{L} {policy} nets={len(reqs)} cap=30", flush=True)
-    g_res, g_rec = run_gpu(X, Y, L, policy, reqs, 30, 1024, delta=None)
+    print(f"[router] cfg1 {X}x{Y}x{L} {policy} nets={len(reqs)} cap=30 engine={engine}", flush=True)
+    g_res, g_rec = run_gpu(X, Y, L, policy, reqs, 30, 1024, delta=None, engine=engine)
     print(f"[router] cfg1 gpu done: final overuse {g_rec[-1][1]}", flush=True)
     o_res, o_rec = run_oracle(X, Y, L, policy, reqs, 30)
     assert [r[:3] for r in g_rec] == [r[:3] for r in o_rec]
