@@ -133,6 +133,11 @@ typedef struct orgpu_sssp_params {
      * each, per slot; 0 = 2^20).  A search that would exceed it is handed to
      * engine 0 (counted in orgpu_sssp_stats.retried). */
     int32_t list_cap;
+    /* engine 1: a step with fewer than 64 entries of the current bucket also
+     * expands entries of up to this many following buckets (same paths: a
+     * node expanded early is expanded again if its key drops); 0 = 15, < 0 =
+     * none, at most 15. */
+    int32_t lookahead;
 } orgpu_sssp_params;
 
 typedef struct orgpu_sssp_stats {

@@ -20,6 +20,7 @@ struct Engine {
     uint32_t *lists = nullptr, *list_cnt = nullptr;
     uint32_t *touch_bits = nullptr, *touch_list = nullptr, *touch_n = nullptr;
     int64_t keys_n = -1;   // node count the keys' slot layout was last reset for
+    int64_t keys_cap = 0;  // keys allocated
     // multi-source / multi-sink calls
     orgpu_ms_point *ms_seed = nullptr, *ms_tgt = nullptr;
     int64_t ms_seed_cap = 0, ms_tgt_cap = 0;

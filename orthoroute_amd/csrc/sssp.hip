@@ -1118,6 +1118,7 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
     const uint64_t summ_words = (chunks + 31) / 32;
     const int nbufs = n_buckets + 3;
     HIP_CHECK_RET(hipMalloc((void **)&en.keys, keys_n * sizeof(uint64_t)));
+    en.keys_cap = (int64_t)keys_n;
     HIP_CHECK_RET(hipMalloc((void **)&en.bits, (uint64_t)nbufs * words * sizeof(uint32_t)));
     HIP_CHECK_RET(hipMemsetAsync(en.bits, 0, (uint64_t)nbufs * words * sizeof(uint32_t), en.stream));
     HIP_CHECK_RET(hipMalloc((void **)&en.summ, (uint64_t)nbufs * summ_words * sizeof(uint32_t)));
@@ -1268,7 +1269,7 @@ int engine_prepare_keys(Engine &en, int64_t N)
     a.touch_words = (a.touch_cap + 31) / 32;
     a.touch_bits = en.touch_bits; a.touch_list = en.touch_list; a.touch_n = en.touch_n;
     if (en.keys_n != N) {
-        hipLaunchKernelGGL(k_fill_keys, dim3(4096), dim3(256), 0, s, en.keys, (int64_t)en.cap_slots * a.NP);
+        hipLaunchKernelGGL(k_fill_keys, dim3(4096), dim3(256), 0, s, en.keys, en.keys_cap);
         HIP_CHECK_RET(hipGetLastError());
         en.keys_n = N;
     } else if (en.touch_dirty) {
@@ -1430,7 +1431,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     // Keys are reset per slot from the touch records of its last search; those
     // records describe the layout slot*NP of the node count they were made for.
     if (en.keys_n != N) {
-        hipLaunchKernelGGL(k_fill_keys, dim3(4096), dim3(256), 0, s, en.keys, (int64_t)en.cap_slots * a.NP);
+        hipLaunchKernelGGL(k_fill_keys, dim3(4096), dim3(256), 0, s, en.keys, en.keys_cap);
         HIP_CHECK_RET(hipGetLastError());
         HIP_CHECK_RET(hipMemsetAsync(en.touch_n, 0, sizeof(uint32_t) * en.cap_slots, s));
         HIP_CHECK_RET(hipMemsetAsync(en.touch_bits, 0, sizeof(uint32_t) * (size_t)en.cap_slots * a.touch_words, s));

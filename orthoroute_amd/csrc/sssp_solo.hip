@@ -11,10 +11,12 @@
 //  * a persistent grid of one-wave workgroups; workgroup w owns key slot w
 //    (keys u64[NP] of its own) and takes requests from a global counter
 //    until none are left.  No workgroup ever waits for another one.
-//  * the wave alone reads and writes its slot's keys, so a relaxation is a
-//    workgroup-scope 64-bit atomicMin (no device-wide ordering) after a
-//    plain-load filter, and the ROI bit of a target is read only when the
-//    candidate would improve it.
+//  * the wave alone reads and writes its slot's keys: a candidate that
+//    passes a plain-load filter (and then its target's ROI bit) meets the
+//    wave's other candidates for the same node in an LDS table (one slot per
+//    node, claimed by linear probing, LDS 64-bit min), and the winner writes
+//    the key with a plain store.  No global atomics: every global atomic,
+//    workgroup-scope ones included, is performed at the memory side here.
 //  * the frontier is a radix heap of per-wave lists in device memory
 //    (entries (key cost, node) in the key's own bit layout; chains of chunks
 //    of the slot's arena): list 0 holds the current bucket, list (i, d) the
@@ -45,6 +47,8 @@
 #define SOLO_LEVELS 10              // hex digits of a bucket index (keepout count and 32 bits)
 #define SOLO_NL (1 + SOLO_LEVELS * 16)   // current bucket + one list per (digit, value)
 #define SOLO_TOUCH_WORDS 1024       // LDS touch bitmap: 32768 key chunks
+#define SOLO_DD_BITS 9
+#define SOLO_DD (1u << SOLO_DD_BITS)  // LDS relaxation table entries (>= 4 x 64 candidates)
 #define SOLO_MAX_STEPS (1u << 22)  // expansion steps of 64 entries per search
 
 // request outcomes (out_len) besides a path length >= 0
@@ -60,8 +64,11 @@ struct SoloArgs {
     const uint32_t *roi;           // [n_roi][roi_words]
     int64_t roi_words;
     const int32_t *ko_owner;       // [N] or null
-    int64_t N, NP;
-    int32_t X, plane;              // node = z*plane + y*X + x (goal-directed order)
+    int64_t N, NP;                 // NP: a slot's key stride (the tiled index space, rounded up)
+    int32_t X, Y, plane;           // node = z*plane + y*X + x
+    int32_t tiled;                 // keys in 2x2x4 (x, y, z) tiles of 16 (one 128-byte line)
+    int32_t TX, TY;                // tiles along x and y
+    int32_t lookahead;             // buckets past the current one an under-full step takes entries from
     float inv_delta, h_xy, h_z;
     const orgpu_net_req *reqs;
     int32_t n_reqs;
@@ -165,20 +172,22 @@ struct SoloLists {
         if (lane < 2) al[lane] = 0u;
         if (lane < SOLO_LEVELS) mask[lane] = 0u;
     }
-    // Takes up to 64 entries from the head of list t: lane k < n gets
-    // entry k.  The chunks it empties go back to the free stack; the
-    // entries are read before any later append can reuse them.
-    __device__ __forceinline__ uint64_t pop(uint32_t t, uint32_t lane, uint32_t &n_out) const
+    // Takes up to maxn (<= 64) entries from the head of list t: lane
+    // base + k gets entry k, for k < n.  The chunks it empties go back to the
+    // free stack; the entries are read before any later append can reuse
+    // them.
+    __device__ __forceinline__ uint64_t pop(uint32_t t, uint32_t lane, uint32_t base, uint32_t maxn,
+                                            uint32_t &n_out) const
     {
         const uint32_t c = cnt[t];
-        const uint32_t n = c < 64u ? c : 64u;
+        const uint32_t n = c < maxn ? c : maxn;
         n_out = n;
         const uint32_t h = hc[t], p = hp[t];
         const bool cross = p + n > SOLO_CH;
         const uint32_t nx = cross ? link[h] : SOLO_NONE;
         uint64_t e = 0ull;
-        if (lane < n) {
-            const uint32_t pos = p + lane;
+        if (lane >= base && lane < base + n) {
+            const uint32_t pos = p + (lane - base);
             e = pos < SOLO_CH ? arena[(uint64_t)h * SOLO_CH + pos] : arena[(uint64_t)nx * SOLO_CH + (pos - SOLO_CH)];
         }
         const uint32_t left = c - n;
@@ -199,45 +208,79 @@ struct SoloLists {
         }
         return e;
     }
+    // Moves every entry of list t to the empty list 0 (their buckets all
+    // equal the new current bucket).
+    __device__ __forceinline__ void take(uint32_t t, uint32_t lane) const
+    {
+        if (lane == 0) {
+            hc[0] = hc[t]; hp[0] = hp[t]; tc[0] = tc[t]; tp[0] = tp[t]; cnt[0] = cnt[t];
+            hc[t] = SOLO_NONE; tc[t] = SOLO_NONE; hp[t] = 0u; tp[t] = 0u; cnt[t] = 0u;
+            mask[(t - 1u) >> 4] &= ~(1u << ((t - 1u) & 15u));
+        }
+    }
 };
 
-// Appends each active lane's entry e to its list t, one ballot per distinct
-// list.  Returns false (wave-uniform) when the arena is exhausted; the
-// entries that did not fit are dropped and the caller abandons the search.
-__device__ __forceinline__ bool lists_append(const SoloLists &L, bool act, uint32_t t, uint64_t e)
+// Appends each active (lane, j) entry e[j] to its list t[j], one pass per
+// distinct target list over all four candidates of every lane (a step's
+// pushes mostly go to one to three lists).  Returns false (wave-uniform)
+// when the arena is exhausted; the entries that did not fit are dropped and
+// the caller abandons the search.
+__device__ __forceinline__ bool lists_append4(const SoloLists &L, bool act[4], const uint32_t t[4], const uint64_t e[4])
 {
     bool ok = true;
-    uint64_t m;
-    while ((m = __ballot(act)) != 0ull) {
-        const int first = __builtin_ctzll(m);
-        const uint32_t t0 = (uint32_t)__shfl((int)t, first);
-        const bool mine = act && t == t0;
-        const uint64_t mm = __ballot(mine);
-        const uint32_t cnt = (uint32_t)__popcll(mm);
-        if (mine) act = false;
+    for (;;) {
+        const bool anyl = act[0] || act[1] || act[2] || act[3];
+        const uint64_t m = __ballot(anyl);
+        if (m == 0ull) break;
+        const uint32_t tf = act[0] ? t[0] : act[1] ? t[1] : act[2] ? t[2] : t[3];
+        const uint32_t t0 = (uint32_t)__shfl((int)tf, __builtin_ctzll(m));
+        uint64_t mm[4];
+        uint32_t off[4];
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            mm[j] = __ballot(act[j] && t[j] == t0);
+            off[j] = cnt;
+            cnt += (uint32_t)__popcll(mm[j]);
+        }
         uint32_t tc = L.tc[t0], tp = L.tp[t0];
+        bool fits = true;
         if (tc == SOLO_NONE) {
             tc = L.alloc();
-            if (tc == SOLO_NONE) { ok = false; continue; }
-            L.hc[t0] = tc; L.hp[t0] = 0u; tp = 0u;
+            if (tc == SOLO_NONE) fits = false;
+            else { L.hc[t0] = tc; L.hp[t0] = 0u; tp = 0u; }
         }
         uint32_t c2 = SOLO_NONE;
-        if (tp + cnt > SOLO_CH) {
+        if (fits && tp + cnt > SOLO_CH) {
             c2 = L.alloc();
-            if (c2 == SOLO_NONE) { ok = false; L.tc[t0] = tc; L.tp[t0] = tp; continue; }
-            L.link[tc] = c2;
+            if (c2 == SOLO_NONE) { fits = false; L.tc[t0] = tc; L.tp[t0] = tp; }
+            else L.link[tc] = c2;
         }
-        if (mine) {
-            const uint32_t pos = tp + lanes_below(mm);
-            const uint64_t at = pos < SOLO_CH ? (uint64_t)tc * SOLO_CH + pos : (uint64_t)c2 * SOLO_CH + (pos - SOLO_CH);
-            L.arena[at] = e;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bool mine = act[j] && t[j] == t0;
+            if (mine && fits) {
+                const uint32_t pos = tp + off[j] + lanes_below(mm[j]);
+                const uint64_t at = pos < SOLO_CH ? (uint64_t)tc * SOLO_CH + pos : (uint64_t)c2 * SOLO_CH + (pos - SOLO_CH);
+                L.arena[at] = e[j];
+            }
+            if (mine) act[j] = false;
         }
+        if (!fits) { ok = false; continue; }
         if (c2 != SOLO_NONE) { L.tc[t0] = c2; L.tp[t0] = tp + cnt - SOLO_CH; }
         else { L.tc[t0] = tc; L.tp[t0] = tp + cnt; }
         L.cnt[t0] += cnt;
         if (t0 > 0u) L.mask[(t0 - 1u) >> 4] |= 1u << ((t0 - 1u) & 15u);
     }
     return ok;
+}
+
+__device__ __forceinline__ bool lists_append(const SoloLists &L, bool act, uint32_t t, uint64_t e)
+{
+    bool a4[4] = {act, false, false, false};
+    const uint32_t t4[4] = {t, 0u, 0u, 0u};
+    const uint64_t e4[4] = {e, 0ull, 0ull, 0ull};
+    return lists_append4(L, a4, t4, e4);
 }
 
 // List of bucket b relative to the current bucket cur: 0 for b <= cur,
@@ -250,14 +293,39 @@ __device__ __forceinline__ uint32_t radix_list(uint64_t b, uint64_t cur)
     return 1u + (uint32_t)i * 16u + ((uint32_t)(b >> (4 * i)) & 15u);
 }
 
-__device__ __forceinline__ float solo_heur(const SoloArgs &a, int dx, int dy, int dz, uint32_t v)
+struct Cell {
+    int x, y, z;
+};
+
+__device__ __forceinline__ Cell node_cell(const SoloArgs &a, uint32_t v)
+{
+    Cell c;
+    c.z = (int)(v / (uint32_t)a.plane);
+    const int r = (int)(v - (uint32_t)c.z * (uint32_t)a.plane);
+    c.y = r / a.X;
+    c.x = r - c.y * a.X;
+    return c;
+}
+
+// index of a node's key inside its slot: a tile of 2 x 2 cells on 4 layers
+// shares one 128-byte line, so most of a node's lattice neighbours (the
+// next cell along its layer, the layers above and below) share its line
+__device__ __forceinline__ uint32_t tile_index(const SoloArgs &a, const Cell &c)
+{
+    const uint32_t t = ((uint32_t)(c.z >> 2) * (uint32_t)a.TY + (uint32_t)(c.y >> 1)) * (uint32_t)a.TX + (uint32_t)(c.x >> 1);
+    return t * 16u + ((uint32_t)(c.z & 3) << 2) + ((uint32_t)(c.y & 1) << 1) + (uint32_t)(c.x & 1);
+}
+
+__device__ __forceinline__ uint32_t key_at(const SoloArgs &a, uint32_t v)
+{
+    return a.tiled ? tile_index(a, node_cell(a, v)) : v;
+}
+
+// lower bound of the remaining cost from cell c to the destination
+__device__ __forceinline__ float solo_heur(const SoloArgs &a, int dx, int dy, int dz, const Cell &c)
 {
     if (a.h_xy == 0.0f && a.h_z == 0.0f) return 0.0f;
-    const int z = (int)(v / (uint32_t)a.plane);
-    const int r = (int)(v - (uint32_t)z * (uint32_t)a.plane);
-    const int y = r / a.X;
-    const int x = r - y * a.X;
-    return a.h_xy * (float)(abs(x - dx) + abs(y - dy)) + a.h_z * (float)abs(z - dz);
+    return a.h_xy * (float)(abs(c.x - dx) + abs(c.y - dy)) + a.h_z * (float)abs(c.z - dz);
 }
 
 __global__ void __launch_bounds__(64)
@@ -266,6 +334,8 @@ k_solo(SoloArgs a)
     __shared__ uint32_t s_touch[SOLO_TOUCH_WORDS];
     __shared__ uint32_t s_hc[SOLO_NL], s_hp[SOLO_NL], s_tc[SOLO_NL], s_tp[SOLO_NL], s_cnt[SOLO_NL], s_al[2];
     __shared__ uint32_t s_mask[SOLO_LEVELS];
+    __shared__ uint32_t s_ddtag[SOLO_DD];          // relaxation table: node + 1 (0 = free)
+    __shared__ uint64_t s_ddkey[SOLO_DD];          // its best candidate key
     const uint32_t lane = threadIdx.x;
     const uint32_t slot = blockIdx.x;
     uint64_t *keys = a.keys + (uint64_t)slot * (uint64_t)a.NP;
@@ -274,6 +344,7 @@ k_solo(SoloArgs a)
     int32_t *stage = a.stage + (uint64_t)slot * (uint64_t)a.max_path;
     const uint32_t tshift = a.touch_shift;
     for (int i = lane; i < SOLO_TOUCH_WORDS; i += 64) s_touch[i] = 0u;
+    for (int i = lane; i < SOLO_DD; i += 64) { s_ddtag[i] = 0u; s_ddkey[i] = KEY_INF; }
     const bool goal = a.h_xy != 0.0f || a.h_z != 0.0f;
     const uint64_t margin = goal ? 1u : 0u;
     unsigned long long my_items = 0, my_edges = 0;
@@ -287,10 +358,8 @@ k_solo(SoloArgs a)
         const uint32_t src = (uint32_t)q.src, dst = (uint32_t)q.dst;
         int dx = 0, dy = 0, dz = 0;
         if (goal) {
-            dz = (int)(dst / (uint32_t)a.plane);
-            const int rr = (int)(dst - (uint32_t)dz * (uint32_t)a.plane);
-            dy = rr / a.X;
-            dx = rr - dy * a.X;
+            const Cell cd = node_cell(a, dst);
+            dx = cd.x; dy = cd.y; dz = cd.z;
         }
         const uint32_t *roi = q.roi_index >= 0 ? a.roi + (uint64_t)q.roi_index * (uint64_t)a.roi_words : nullptr;
         const bool use_ko = q.use_keepout && a.ko_owner;
@@ -298,10 +367,13 @@ k_solo(SoloArgs a)
 
         // seed the source in its own bucket
         const uint64_t k0 = key_pack(0, 0.0f, KEY_NOPRED);
-        uint64_t cur = key_bucket_rel(k0, a.inv_delta, solo_heur(a, dx, dy, dz, src));
+        const bool lat = a.tiled || goal;
+        const uint32_t kdst = key_at(a, dst);
+        uint64_t cur = key_bucket_rel(k0, a.inv_delta, lat ? solo_heur(a, dx, dy, dz, node_cell(a, src)) : 0.0f);
         if (lane == 0) {
-            kstore(&keys[src], k0);
-            atomicOr(&s_touch[(src >> tshift) >> 5], 1u << ((src >> tshift) & 31));
+            const uint32_t ks = key_at(a, src);
+            kstore(&keys[ks], k0);
+            atomicOr(&s_touch[(ks >> tshift) >> 5], 1u << ((ks >> tshift) & 31));
         }
         bool ok = lists_append(L, lane == 0, 0u, entry_pack(k0, src));
         wave_fence();
@@ -317,14 +389,31 @@ k_solo(SoloArgs a)
             while (ok && !L.empty(lc)) {
                 if (++steps > SOLO_MAX_STEPS) { ok = false; break; }
                 uint32_t n = 0;
-                const uint64_t e = L.pop(lc, lane, n);
+                uint64_t e = L.pop(lc, lane, 0u, 64u, n);
+                // an under-full step also takes entries of the next buckets
+                // of this 16-bucket block (up to a.lookahead ahead): expanding
+                // a node early is exact (it is expanded again if its key
+                // drops later) and it fills lanes that would idle
+                if (n < 64u && a.lookahead > 0) {
+                    const uint32_t d0 = (uint32_t)(cur & 15u);
+                    const uint32_t dl = min(15u, d0 + (uint32_t)a.lookahead);
+                    for (uint32_t d = d0 + 1u; d <= dl && n < 64u; ++d) {
+                        if (L.empty(1u + d)) continue;
+                        uint32_t n2 = 0;
+                        const uint64_t e2 = L.pop(1u + d, lane, n, 64u - n, n2);
+                        if (lane >= n && lane < n + n2) e = e2;
+                        n += n2;
+                    }
+                }
                 bool has = lane < n;
                 const uint32_t u = (uint32_t)(e & KEY_PRED_MASK);
                 uint64_t ku = 0;
                 int row = 0, row_end = 0;
                 int32_t own = -1;
+                Cell cu = Cell{0, 0, 0};
                 if (has) {
-                    ku = kload(&keys[u]);
+                    if (lat) cu = node_cell(a, u);
+                    ku = kload(&keys[a.tiled ? tile_index(a, cu) : u]);
                     row = a.indptr[u];
                     row_end = a.indptr[u + 1];
                     if (use_ko) own = a.ko_owner[u];
@@ -357,11 +446,20 @@ k_solo(SoloArgs a)
                     // candidate that would improve its target (an edge to a
                     // settled neighbour, the common case, costs one load)
                     uint64_t kv[4], nk[4];
+                    uint32_t kix[4];
+                    Cell cv[4];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         kv[j] = KEY_INF;
                         nk[j] = KEY_INF;
-                        if (go[j]) kv[j] = kload(&keys[v[j]]);
+                        kix[j] = v[j];
+                        cv[j] = Cell{0, 0, 0};
+                        if (!go[j]) continue;
+                        if (lat) {
+                            cv[j] = node_cell(a, v[j]);
+                            if (a.tiled) kix[j] = tile_index(a, cv[j]);
+                        }
+                        kv[j] = kload(&keys[kix[j]]);
                     }
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
@@ -380,29 +478,53 @@ k_solo(SoloArgs a)
 #pragma unroll
                         for (int j = 0; j < 4; ++j) go[j] = go[j] && ((rw[j] >> (v[j] & 31)) & 1u);
                     }
-                    // relax: only this wave writes the slot's keys, so a
-                    // workgroup-scope atomicMin orders the lanes' candidates
-                    uint64_t old[4];
+                    // relax: only this wave writes the slot's keys, so the
+                    // lanes' candidates for one node meet in the wave's LDS
+                    // table (claim a slot per node by linear probing, LDS
+                    // 64-bit min), and the winner writes the key with a plain
+                    // store: no global atomics
+                    uint32_t pos[4];
                     bool won[4];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        old[j] = KEY_INF;
-                        if (go[j]) old[j] = __hip_atomic_fetch_min(&keys[v[j]], nk[j], __ATOMIC_RELAXED,
-                                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-                        won[j] = go[j] && nk[j] < old[j];
+                        pos[j] = 0u;
+                        if (!go[j]) continue;
+                        uint32_t p = (v[j] * 2654435761u) >> (32 - SOLO_DD_BITS);
+                        for (;;) {
+                            const uint32_t o = atomicCAS(&s_ddtag[p], 0u, v[j] + 1u);
+                            if (o == 0u || o == v[j] + 1u) break;
+                            p = (p + 1u) & (SOLO_DD - 1u);
+                        }
+                        pos[j] = p;
                     }
-                    // touched chunks and pushes of strict improvements
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (go[j]) atomicMin((unsigned long long *)&s_ddkey[pos[j]], (unsigned long long)nk[j]);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) won[j] = go[j] && s_ddkey[pos[j]] == nk[j];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        if (go[j] && old[j] == KEY_INF) {
-                            const uint32_t ch = v[j] >> tshift;
+                        if (!won[j]) continue;
+                        s_ddtag[pos[j]] = 0u;
+                        s_ddkey[pos[j]] = KEY_INF;
+                        kstore(&keys[kix[j]], nk[j]);
+                    }
+                    // touched chunks and pushes of strict improvements
+                    bool push[4];
+                    uint32_t tl4[4];
+                    uint64_t e4[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (won[j] && kv[j] == KEY_INF) {
+                            const uint32_t ch = kix[j] >> tshift;
                             atomicOr(&s_touch[ch >> 5], 1u << (ch & 31));
                         }
-                        const bool push = won[j] && key_cost(nk[j]) < key_cost(old[j]);
-                        const uint32_t t = push ? radix_list(key_bucket_rel(nk[j], a.inv_delta,
-                                                                            solo_heur(a, dx, dy, dz, v[j])), cur) : 0u;
-                        ok = lists_append(L, push, t, entry_pack(nk[j], v[j])) && ok;
+                        push[j] = won[j] && key_cost(nk[j]) < key_cost(kv[j]);
+                        tl4[j] = push[j] ? radix_list(key_bucket_rel(nk[j], a.inv_delta,
+                                                                      solo_heur(a, dx, dy, dz, cv[j])), cur) : 0u;
+                        e4[j] = entry_pack(nk[j], v[j]);
                     }
+                    ok = lists_append4(L, push, tl4, e4) && ok;
                 }
                 wave_fence();
             }
@@ -421,23 +543,29 @@ k_solo(SoloArgs a)
                     break;
                 }
             }
-            const uint64_t kd = kload(&keys[dst]);
+            const uint64_t kd = kload(&keys[kdst]);
             if (kd != KEY_INF && (next == ~0ull || key_bucket_rel(kd, a.inv_delta) + margin < next)) {
                 reached = true;
                 break;
             }
             if (next == ~0ull) break;   // frontier empty: dst unreachable
             // the list's live entries move to lower lists relative to the new
-            // current bucket (they agree with it above digit i)
+            // current bucket (they agree with it above digit i); a list of
+            // the lowest digit holds that bucket alone and becomes list 0
             cur = next;
+            if (tl <= 16u) {
+                L.take(tl, lane);
+                continue;
+            }
             while (ok && !L.empty(tl)) {
                 uint32_t n = 0;
-                const uint64_t e = L.pop(tl, lane, n);
+                const uint64_t e = L.pop(tl, lane, 0u, 64u, n);
                 bool has = lane < n;
                 const uint32_t v = (uint32_t)(e & KEY_PRED_MASK);
-                const uint64_t kv = has ? kload(&keys[v]) : 0ull;
+                const Cell cc = has && lat ? node_cell(a, v) : Cell{0, 0, 0};
+                const uint64_t kv = has ? kload(&keys[a.tiled ? tile_index(a, cc) : v]) : 0ull;
                 has = has && key_cost(kv) == key_cost(e);
-                const uint32_t t = has ? radix_list(key_bucket_rel(kv, a.inv_delta, solo_heur(a, dx, dy, dz, v)), cur) : 0u;
+                const uint32_t t = has ? radix_list(key_bucket_rel(kv, a.inv_delta, solo_heur(a, dx, dy, dz, cc)), cur) : 0u;
                 ok = lists_append(L, has, t, e) && ok;
             }
         }
@@ -448,7 +576,7 @@ k_solo(SoloArgs a)
         if (!ok) {
             len = SOLO_RETRY;
         } else if (reached) {
-            const uint64_t kd = kload(&keys[dst]);
+            const uint64_t kd = kload(&keys[kdst]);
             dist = key_dist(kd);
             if (key_ko(kd) >= 31u) {
                 len = SOLO_KO_SATURATED;
@@ -458,7 +586,7 @@ k_solo(SoloArgs a)
                     if (len >= a.max_path) { len = SOLO_BROKEN; break; }
                     stage[len++] = (int32_t)v;
                     if (v == src) break;
-                    const uint32_t pf = key_pred(kload(&keys[v]));
+                    const uint32_t pf = key_pred(kload(&keys[key_at(a, v)]));
                     if (pf == KEY_NOPRED) { len = SOLO_BROKEN; break; }
                     v = pf - 1u;
                 }
@@ -506,7 +634,7 @@ k_solo(SoloArgs a)
                     const int bit = __builtin_ctz(wl);
                     wl &= wl - 1u;
                     const uint64_t lo = ((uint64_t)(w0 + l) * 32u + (uint64_t)bit) << tshift;
-                    const uint64_t hi = min(lo + (1ull << tshift), (uint64_t)a.N);
+                    const uint64_t hi = min(lo + (1ull << tshift), (uint64_t)a.NP);
                     for (uint64_t k = lo + lane; k < hi; k += 64) keys[k] = KEY_INF;
                 }
                 any &= any - 1ull;
@@ -595,11 +723,27 @@ int engine_route_solo(Engine &en, const orgpu_graph *g, const orgpu_net_req *net
     const uint32_t nch = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(2, (want + SOLO_CH - 1) / SOLO_CH), 1u << 24);
     int rc = solo_reserve(en, K, n_nets, nch);
     if (rc) return rc;
+    // key layout: 2x2x4 tiles on a lattice (the graph's shape is X*Y*L nodes
+    // in node order z*X*Y + y*X + x), node order otherwise; a slot's stride
+    // NP covers the index space in whole 2048-key chunks
+    const bool tiled = g->X > 0 && g->Y > 0 && g->L > 0 && (int64_t)g->X * g->Y * g->L == N;
+    const int32_t TX = (g->X + 1) / 2, TY = (g->Y + 1) / 2, TZ = (g->L + 3) / 4;
+    const int64_t span = tiled ? (int64_t)TX * TY * TZ * 16 : N;
+    const int64_t NP = (span + 2047) / 2048 * 2048;
+    if ((uint64_t)K * (uint64_t)NP > (uint64_t)en.keys_cap) {
+        // the tiled index space is larger than the node count: grow the keys
+        // (every key is refilled with KEY_INF below)
+        if (en.keys) hipFree(en.keys);
+        en.keys = nullptr;
+        const int64_t cap = std::max<int64_t>((int64_t)K * NP, en.keys_cap);
+        HIP_CHECK_RET(hipMalloc((void **)&en.keys, (size_t)cap * sizeof(uint64_t)));
+        en.keys_cap = cap;
+        en.keys_n = -1;
+    }
     rc = engine_prepare_keys(en, N);
     if (rc) return rc;
-    const int64_t NP = (N + 2047) / 2048 * 2048;
     uint32_t tshift = 6;
-    while ((((uint64_t)N + (1ull << tshift) - 1) >> tshift) > (uint64_t)SOLO_TOUCH_WORDS * 32u) ++tshift;
+    while ((((uint64_t)NP + (1ull << tshift) - 1) >> tshift) > (uint64_t)SOLO_TOUCH_WORDS * 32u) ++tshift;
     const int64_t cap = en.solo_req_cap;
     const uint64_t half = (uint64_t)en.cap_slots * (uint64_t)en.cap_path;
 
@@ -609,7 +753,9 @@ int engine_route_solo(Engine &en, const orgpu_graph *g, const orgpu_net_req *net
     a.roi = roi_bitmaps; a.roi_words = (N + 31) / 32;
     a.ko_owner = keepout_owner;
     a.N = N; a.NP = NP;
-    a.X = g->X; a.plane = g->X * g->Y;
+    a.X = g->X; a.Y = g->Y; a.plane = g->X * g->Y;
+    a.tiled = tiled ? 1 : 0; a.TX = TX; a.TY = TY;
+    a.lookahead = p->lookahead == 0 ? 15 : (p->lookahead < 0 ? 0 : std::min(p->lookahead, 15));
     a.inv_delta = 1.0f / p->delta;
     a.h_xy = p->h_xy; a.h_z = p->h_z;
     a.reqs = en.solo_req; a.n_reqs = n_nets;

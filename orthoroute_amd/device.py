@@ -232,7 +232,7 @@ class SearchParams:
     runs the plain Dijkstra bucket order.  Paths are the same either way."""
 
     def __init__(self, slots=1, delta=1.0, n_buckets=8, rounds_per_poll=32, grid=0, count_minimal=False,
-                 h_xy=0.0, h_z=0.0, local_batches=0, graph_rounds=False, engine=0, list_cap=0):
+                 h_xy=0.0, h_z=0.0, local_batches=0, graph_rounds=False, engine=0, list_cap=0, lookahead=0):
         self.slots, self.delta, self.n_buckets = int(slots), float(delta), int(n_buckets)
         self.rounds_per_poll, self.grid, self.count_minimal = int(rounds_per_poll), int(grid), bool(count_minimal)
         self.h_xy, self.h_z = float(h_xy), float(h_z)
@@ -241,6 +241,7 @@ class SearchParams:
         # two-pin searches: 0 = bucket rounds over all slots, 1 = one wavefront
         # per search (list_cap: entries per bucket list, 0 = default)
         self.engine, self.list_cap = int(engine), int(list_cap)
+        self.lookahead = int(lookahead)   # engine 1: buckets an under-full step looks ahead (0 = 15, < 0 = none)
 
     def c(self):
         p = _abi.SsspParams()
@@ -249,7 +250,7 @@ class SearchParams:
         p.h_xy, p.h_z = self.h_xy, self.h_z
         p.local_batches = self.local_batches
         p.launch_mode = int(self.graph_rounds)
-        p.engine, p.list_cap = self.engine, self.list_cap
+        p.engine, p.list_cap, p.lookahead = self.engine, self.list_cap, self.lookahead
         return p
 
 

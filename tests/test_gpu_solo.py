@@ -40,12 +40,14 @@ def test_solo_keepout_matches_oracle(ctx, goal):
         run_case(ctx, 20, 16, 6, "reference", 16, 30, seed, kind, keepout=True, goal=goal, engine=1)
 
 
+@pytest.mark.parametrize("lookahead", [-1, 0, 15], ids=["no-lookahead", "default", "lookahead15"])
 @pytest.mark.parametrize("dn", [(0.05, 1), (0.3, 8), (4.0, 32), (100.0, 8)], ids=["narrow", "mid", "wide", "one"])
-def test_solo_bucket_widths(ctx, dn):
-    """Bucket widths from far below one edge cost (many overflow jumps) to
-    wider than every path (one bucket): the same paths."""
-    run_case(ctx, 30, 24, 6, "adjacent", 16, 40, 31, "congested", dn=dn, goal=True, engine=1)
-    run_case(ctx, 30, 24, 6, "adjacent", 16, 40, 32, "jitter", dn=dn, goal=False, engine=1)
+def test_solo_bucket_widths(ctx, dn, lookahead):
+    """Bucket widths from far below one edge cost (many radix-list moves) to
+    wider than every path (one bucket), with and without early expansion of
+    the next buckets' entries: the same paths."""
+    run_case(ctx, 30, 24, 6, "adjacent", 16, 40, 31, "congested", dn=dn, goal=True, engine=1, lookahead=lookahead)
+    run_case(ctx, 30, 24, 6, "adjacent", 16, 40, 32, "jitter", dn=dn, goal=False, engine=1, lookahead=lookahead)
 
 
 def test_solo_capacity_hands_back(ctx):
