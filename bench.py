@@ -246,13 +246,19 @@ def main():
     d = {k: s1.get(k, 0.0) - s0.get(k, 0.0) for k in s1}
     K = args.steps
     gteps = min_edges * K / wall / 1e9
-    launches = d.get("sssp_launches", 0.0)
-    round_ms = d.get("sssp_round_ms", 0.0)
+    # the step's dominant kernel: the one-wave search kernel when the pass
+    # runs on engine 1 (the wider-ROI fallback of the first five failures,
+    # a handful of searches, runs on the bucket engine beside it), else the
+    # bucket round kernel
+    step_engine = r._engine_for(len(tasks))
+    pre = "solo_" if step_engine == 1 else "sssp_"
+    launches = d.get(pre + "launches", 0.0)
+    round_ms = d.get("solo_kernel_ms" if step_engine == 1 else "sssp_round_ms", 0.0)
     avg_launch_us = 1e3 * round_ms / max(launches, 1)
-    # algorithmic bytes of the work the round launches performed (expanded
-    # items and scanned edges, re-expansions included); the Dijkstra-minimal
-    # equivalent is reported beside it
-    alg_bytes = BYTES_PER_NODE * d.get("sssp_items", 0) + BYTES_PER_EDGE * d.get("sssp_edges", 0)
+    # algorithmic bytes of the work that kernel's launches performed
+    # (expanded items and scanned edges, re-expansions included); the
+    # Dijkstra-minimal equivalent is reported beside it
+    alg_bytes = BYTES_PER_NODE * d.get(pre + "items", 0) + BYTES_PER_EDGE * d.get(pre + "edges", 0)
     achieved = alg_bytes / max(round_ms * 1e-3, 1e-12) / 1e9
     dij_bytes = (BYTES_PER_NODE * min_nodes + BYTES_PER_EDGE * min_edges) * K
     dij_equiv = dij_bytes / max(round_ms * 1e-3, 1e-12) / 1e9
@@ -290,12 +296,10 @@ def main():
     stress_line = stress(args) if args.stress_nets > 0 else None
     serve = serving(args) if args.serve_jobs > 0 else None
 
-    # the dominant kernel of the step: the one-wave search kernel when the
-    # pass ran on engine 1, else the bucket round kernel.  Its HBM traffic
-    # per launch comes from a committed rocprofv3 --pmc profile and is
-    # reported only when that profile measured this build's kernel source
-    # (the digest it records equals the digest of the source file now)
-    step_engine = r._engine_for(len(tasks))
+    # HBM traffic per launch of the dominant kernel comes from a committed
+    # rocprofv3 --pmc profile and is reported only when that profile
+    # measured this build's kernel source (the digest it records equals the
+    # digest of the source file now)
     kname, ksrc = ("k_solo", "sssp_solo.hip") if step_engine == 1 else ("k_round", "sssp.hip")
     traffic, traffic_note = None, "no counter profile"
     here = os.path.dirname(os.path.abspath(__file__))

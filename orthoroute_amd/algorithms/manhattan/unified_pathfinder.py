@@ -1353,6 +1353,13 @@ class PathFinderRouter:
         self.stats["sssp_minimal_edges"] += st["minimal_edges"]
         self.stats["sssp_retried"] += st["retried"]
         self.stats["sssp_searches"] += len(sreqs)
+        if params.engine == 1:
+            # the one-wave engine's own share (searches it handed back to the
+            # bucket engine included: st["retried"])
+            self.stats["solo_kernel_ms"] += st["round_ms"]
+            self.stats["solo_items"] += st["items"]
+            self.stats["solo_edges"] += st["edges"]
+            self.stats["solo_launches"] += st["launches"]
         ol = offs.tolist()
         for j, i in enumerate(todo.tolist()):
             a, b = ol[j], ol[j + 1]
