@@ -88,6 +88,12 @@ def parse():
     ap.add_argument("--serve-jobs", type=int, default=16, help="serving stream: jobs (0 = skip)")
     ap.add_argument("--serve-max-nodes", type=float, default=4e6, help="serving stream: largest board (nodes)")
     ap.add_argument("--serve-active", type=int, default=4, help="serving slots (jobs on the GPU at once)")
+    ap.add_argument("--serve-nets", default="density", choices=["density", "cfg5"],
+                    help="nets per serving job: density = 0.75 X Y / (X + Y) (boards the schedule mostly "
+                         "solves), cfg5 = log-uniform 500-30,000")
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="serving: HBM the running jobs may hold")
+    ap.add_argument("--serve-extras", type=int, default=1,
+                    help="serving: also the burst capacity runs and the alone-vs-mix re-runs (0: no)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU oracle sample budget (0 = skip)")
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--delta", type=float, default=0.0, help="bucket width (0 = the router's choice)")
@@ -734,9 +740,17 @@ def serving(args):
     from orthoroute_amd import workloads
     from orthoroute_amd.serving import RoutingServer, replay, run_job, summarize
     jobs = workloads.cfg5(n_jobs=args.serve_jobs, n_min=1e5, n_max=args.serve_max_nodes,
-                          net_density=0.75, max_iterations=40)
+                          net_density=0.75 if args.serve_nets == "density" else None, max_iterations=40)
     torch.cuda.synchronize()
-    srv = RoutingServer(hbm_budget=64 << 30, max_active=args.serve_active, roi_limit=1 << 30)
+    budget = int(args.serve_budget_gib * 2 ** 30)
+
+    def done(job, r):
+        if isinstance(r, BaseException):
+            log(f"[bench] serving: job {job.job_id} failed: {r}")
+        else:
+            log(f"[bench] serving: job {job.job_id} ({r.nodes} nodes, {r.nets} nets) done after {r.iterations} "
+                f"iterations in {r.route_s:.2f}s, zero overuse at {r.zero_overuse_s}")
+    srv = RoutingServer(hbm_budget=budget, max_active=args.serve_active, roi_limit=1 << 30, on_done=done)
     t0 = time.perf_counter()
     res = replay(srv, jobs)
     srv.close()
@@ -752,8 +766,8 @@ def serving(args):
     import dataclasses
     burst = [dataclasses.replace(j, arrival_s=0.0) for j in jobs]
     cap = {}
-    for active in (args.serve_active, 1):
-        bsrv = RoutingServer(hbm_budget=64 << 30, max_active=active, roi_limit=1 << 30)
+    for active in ((args.serve_active, 1) if args.serve_extras else ()):
+        bsrv = RoutingServer(hbm_budget=budget, max_active=active, roi_limit=1 << 30)
         tb = time.perf_counter()
         bres = replay(bsrv, burst)
         bsrv.close()
@@ -762,9 +776,10 @@ def serving(args):
         cap[active] = {"slots": active, "makespan_s": round(mk, 3),
                        "jobs_per_hour": round(3600.0 * len(jobs) / max(mk, 1e-9), 1),
                        "same_trajectories_as_stream": bool(same)}
-    log(f"[bench] serving burst: {len(jobs)} jobs in {cap[args.serve_active]['makespan_s']}s with "
-        f"{args.serve_active} slots, {cap[1]['makespan_s']}s with one")
-    quick = sorted((r for r in res if r.error is None), key=lambda r: r.route_s)[:3]
+    if cap:
+        log(f"[bench] serving burst: {len(jobs)} jobs in {cap[args.serve_active]['makespan_s']}s with "
+            f"{args.serve_active} slots, {cap[1]['makespan_s']}s with one")
+    quick = sorted((r for r in res if r.error is None), key=lambda r: r.route_s)[:3] if args.serve_extras else []
     same = []
     for r in quick:
         job = jobs[r.job_id]
@@ -777,11 +792,12 @@ def serving(args):
                     f"{min(j.nodes for j in jobs)}-{max(j.nodes for j in jobs)} nodes, 4-24 layers, "
                     f"{min(len(j.nets) for j in jobs)}-{max(len(j.nets) for j in jobs)} nets per job, cap 40 iterations",
         "serving_slots": args.serve_active, "peak_jobs_on_gpu": srv.peak_active,
-        "peak_admitted_gib": round(srv.peak_used / 2 ** 30, 2), "hbm_budget_gib": 64,
+        "peak_admitted_gib": round(srv.peak_used / 2 ** 30, 2), "hbm_budget_gib": args.serve_budget_gib,
         "sum_route_s": round(busy, 3), "overlap": round(busy / max(makespan, 1e-9), 2),
-        "alone_equals_mix": f"{sum(same)}/{len(same)}",
-        "burst": cap[args.serve_active], "burst_one_slot": cap[1],
-        "burst_speedup": round(cap[1]["makespan_s"] / max(cap[args.serve_active]["makespan_s"], 1e-9), 2),
+        "alone_equals_mix": f"{sum(same)}/{len(same)}" if same else None,
+        "burst": cap.get(args.serve_active), "burst_one_slot": cap.get(1),
+        "burst_speedup": round(cap[1]["makespan_s"] / max(cap[args.serve_active]["makespan_s"], 1e-9), 2)
+        if cap else None,
         "jobs": [{"id": r.job_id, "nodes": r.nodes, "layers": jobs[r.job_id].L, "nets": r.nets,
                   "iterations": r.iterations, "converged": r.converged,
                   "zero_overuse_s": None if r.zero_overuse_s is None else round(r.zero_overuse_s, 3),

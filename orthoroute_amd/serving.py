@@ -38,11 +38,12 @@ from typing import List, Optional, Tuple
 
 import numpy as np
 
-from .device import Context, lattice_counts, max_slots
+from .device import SOLO_ARENA, Context, lattice_counts, max_slots
 
 MAX_PATH = 1 << 16          # path staging length the router reserves per search slot
 RING_BUCKETS = 8            # bucket ring the router's search uses by default
 TOUCH_KEYS = 1024           # keys per touched-chunk record of a search slot
+SOLO_MIN = 1024             # searches at once from which the router's batches run on the one-wave engine
 
 
 @dataclass
@@ -97,13 +98,18 @@ def footprint(X, Y, L, policy, n_nets, slots, roi_limit):
     """Device bytes a job holds while it runs: the CSR (indptr, indices,
     base cost, layer, kind), the edge state (count, present, EMA, history,
     total cost), the keepout owner map, the search scratch of `slots`
-    concurrent searches (keys, bucket bitmaps, touch records, path staging)
-    and the ROI BFS scratch and bitmap rows.  An upper estimate of what the
-    engine and the router allocate."""
+    concurrent searches (keys, bucket bitmaps, touch records, path staging;
+    with the one-wave engine, which the router uses when at least SOLO_MIN
+    searches run at once, keys over the tiled index space and each search's
+    bucket-list arena) and the ROI BFS scratch and bitmap rows.  An upper
+    estimate of what the engine and the router allocate."""
     N, E = lattice_counts(X, Y, L, policy)
     csr = 4 * (N + 1) + 10 * E
     state = 20 * E + 4 * N
     per_slot = 8 * N + (RING_BUCKETS + 3) * N // 8 + 4 * (N // TOUCH_KEYS + 1) + 4 * MAX_PATH * 4 + 64
+    if min(int(n_nets), int(slots)) >= SOLO_MIN:
+        tiled = ((X + 1) // 2) * ((Y + 1) // 2) * ((L + 3) // 4) * 16
+        per_slot += 8 * (max(tiled, N) - N + 2048) + 8 * SOLO_ARENA + 8 * (SOLO_ARENA // 256)
     search = slots * per_slot + (RING_BUCKETS + 4) * (N * slots // 2048 + 64) * 4 // 32 + (1 << 20)
     roi_slots = max(1, min(1024, int(n_nets), roi_limit // (9 * N + 1)))
     roi = roi_slots * 9 * N + int(n_nets) * ((N + 31) // 32) * 4
@@ -163,10 +169,13 @@ class RoutingServer:
 
     hbm_budget: device bytes the running jobs may hold together (default:
     80 % of the device's free memory at start).  roi_limit: ROI BFS scratch
-    per job (bytes).  slot_cap: most concurrent searches per job."""
+    per job (bytes).  slot_cap: most concurrent searches per job.  on_done:
+    called with (job, JobResult or the exception) as each job ends."""
 
-    def __init__(self, device=0, hbm_budget=None, max_active=4, roi_limit=1 << 30, slot_cap=1024, runner=None):
+    def __init__(self, device=0, hbm_budget=None, max_active=4, roi_limit=1 << 30, slot_cap=1024, runner=None,
+                 on_done=None):
         self.device = int(device)
+        self._on_done = on_done          # on_done(job, JobResult or exception), from the job's thread
         if runner is None or hbm_budget is None:
             import torch
             if not torch.cuda.is_available():
@@ -264,6 +273,11 @@ class RoutingServer:
             self._active -= 1
             self._used -= fp
             self._cv.notify_all()
+        if self._on_done is not None:
+            try:
+                self._on_done(job, out if out is not None else err)
+            except Exception:
+                pass
         if out is not None:
             fut.set_result(out)
         else:
