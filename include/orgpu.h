@@ -179,6 +179,38 @@ int orgpu_route(orgpu_t *h, const orgpu_graph *g, const orgpu_net_req *nets, int
                 int64_t path_capacity, int64_t *path_offset, float *path_dist,
                 orgpu_sssp_stats *stats);
 
+/* One-wave engine, asynchronous form.  Replaces the same reference code as
+ * orgpu_route (SimpleDijkstra.find_path_roi per net with the owner-aware
+ * keepout, UPF:1801-1887, 3150-3241; the parent walk and _path_to_edges,
+ * UPF:1880-1887, 5463-5476), for callers that keep requests and results on
+ * the device.
+ *
+ * orgpu_reserve_solo: after orgpu_reserve (nodes, slots, path length), the
+ * one-wave engine's own scratch for a graph of n_nodes nodes (a lattice
+ * when X*Y*L == n_nodes: its keys are then laid out in tiles) and `slots`
+ * concurrent searches with bucket lists of list_cap entries each (0 = 2^20).
+ * Blocks (allocation).
+ *
+ * orgpu_route_async: enqueues the n requests of the device array `reqs` on
+ * the handle's stream and returns at once; allocates nothing (it needs
+ * orgpu_reserve_solo for this graph, params->slots and params->list_cap).
+ * After orgpu_sync(h), for request i (device arrays): path_len[i] nodes of
+ * its path src..dst at path_nodes[path_off[i]], the first CSR edge of each
+ * hop at path_edges[path_off[i]] (-1 after the last node; path_edges may be
+ * NULL), path_dist[i] its cost; *used (device u32, may be NULL) the entries
+ * written.  path_len[i]: 0 = no path, -1 = broken chain or longer than the
+ * reserved path length, -2 = 31 or more keepout edges, -3 = given back (its
+ * bucket lists or the output capacity ran out: route it with orgpu_route),
+ * -4 = invalid request (endpoint outside the graph, ROI row >= n_roi, or
+ * use_keepout without keepout_owner).  Paths are those of orgpu_route. */
+int orgpu_reserve_solo(orgpu_t *h, int64_t n_nodes, int32_t X, int32_t Y, int32_t L, int32_t slots,
+                       int32_t list_cap);
+int orgpu_route_async(orgpu_t *h, const orgpu_graph *g, const orgpu_net_req *reqs, int32_t n,
+                      const uint32_t *roi_bitmaps, int32_t n_roi, const int32_t *keepout_owner,
+                      float keepout_cost, const orgpu_sssp_params *params, int32_t *path_nodes,
+                      int32_t *path_edges, int64_t capacity, uint32_t *path_off, int32_t *path_len,
+                      float *path_dist, uint32_t *used);
+
 /* Multi-source / multi-sink search: SimpleDijkstra.find_path_multisource_
  * multisink (UPF:1889-1989) and the GPU seam find_path_fullgraph_gpu_seeds
  * (UPF:4558-4565).  Each request seeds seeds[seed_off .. +seed_n) at their

@@ -232,7 +232,7 @@ int orc_sssp_canon(int64_t N, const int32_t *indptr, const int32_t *indices,
      * here the heap compares (ko, d, node) through a combined float key is
      * not possible, so levels are processed in increasing ko order */
     int reached = 0;
-    for (int level = 0; level < 32 && !(reached); ++level) {
+    for (int level = 0; level < 255 && !(reached); ++level) {
         /* seed this level with every node that sits at it */
         h.n = 0;
         for (int64_t i = 0; i < N; ++i)
@@ -254,7 +254,7 @@ int orc_sssp_canon(int64_t N, const int32_t *indptr, const int32_t *indices,
                 if (roi && !roi[v]) continue;
                 int nk = level;
                 float nd;
-                if (kou && edge_kind[e] == 0) { nk = level < 31 ? level + 1 : 31; nd = du; }
+                if (kou && edge_kind[e] == 0) { nk = level < 254 ? level + 1 : 254; nd = du; }
                 else { nd = du + costs[e]; if (!(nd < INFINITY)) continue; }
                 int better = nk < ko[v] || (nk == ko[v] && nd < dist[v]);
                 if (better) {

@@ -24,6 +24,7 @@ EXPORTED = [
     "orgpu_history_update", "orgpu_overuse_sparse", "orgpu_cost_bounds", "orgpu_route_ms",
     "orgpu_set_roi_scratch", "orgpu_roi_lattice", "orgpu_reserve_ref", "orgpu_route_ref",
     "orgpu_route_ref_noalloc", "orgpu_segment_sums", "orgpu_via_pressure",
+    "orgpu_reserve_solo", "orgpu_route_async",
 ]
 
 
@@ -163,6 +164,9 @@ def load():
         "orgpu_overuse_sparse": [P, i64, P, P, ctypes.c_int, f32, f32, ctypes.POINTER(SparseOut)],
         "orgpu_segment_sums": [P, i64, P, P, P, i64, P],
         "orgpu_via_pressure": [P, i64, i32, P, i32, P, i32, P, P, P, P],
+        "orgpu_reserve_solo": [P, i64, i32, i32, i32, i32, i32],
+        "orgpu_route_async": [P, ctypes.POINTER(Graph), P, i32, P, i32, P, f32, ctypes.POINTER(SsspParams),
+                              P, P, i64, P, P, P, P],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

@@ -103,6 +103,41 @@ extern "C" int orgpu_route(orgpu_t *h, const orgpu_graph *g, const orgpu_net_req
                         path_nodes, path_edges, path_capacity, path_offset, path_dist, stats);
 }
 
+extern "C" int orgpu_reserve_solo(orgpu_t *h, int64_t n_nodes, int32_t X, int32_t Y, int32_t L, int32_t slots,
+                                  int32_t list_cap)
+{
+    if (!h) return orgpu_fail(ORGPU_EINVAL, "reserve_solo: null handle");
+    HIP_CHECK_RET(hipSetDevice(h->device));
+    HIP_CHECK_RET(hipStreamSynchronize(h->stream));
+    return engine_reserve_solo(h->en, n_nodes, X, Y, L, slots, list_cap);
+}
+
+extern "C" int orgpu_route_async(orgpu_t *h, const orgpu_graph *g, const orgpu_net_req *reqs, int32_t n,
+                                 const uint32_t *roi_bitmaps, int32_t n_roi, const int32_t *keepout_owner,
+                                 float keepout_cost, const orgpu_sssp_params *params, int32_t *path_nodes,
+                                 int32_t *path_edges, int64_t capacity, uint32_t *path_off, int32_t *path_len,
+                                 float *path_dist, uint32_t *used)
+{
+    if (!h) return orgpu_fail(ORGPU_EINVAL, "route_async: null handle");
+    if (!g || !params || n < 0 || (n > 0 && (!reqs || !path_nodes || !path_off || !path_len || !path_dist)))
+        return orgpu_fail(ORGPU_EINVAL, "route_async: null argument or negative count");
+    if (!g->indptr || !g->indices || !g->cost || !g->edge_kind || g->n_nodes < 1)
+        return orgpu_fail(ORGPU_EINVAL, "route_async: bad graph");
+    if (!(params->delta > 0.0f)) return orgpu_fail(ORGPU_EINVAL, "route_async: delta must be > 0");
+    if (!(params->h_xy >= 0.0f && params->h_z >= 0.0f && params->h_xy < 3.0e38f && params->h_z < 3.0e38f))
+        return orgpu_fail(ORGPU_EINVAL, "route_async: h_xy / h_z must be finite and >= 0");
+    if ((params->h_xy > 0.0f || params->h_z > 0.0f) &&
+        (g->X <= 0 || g->Y <= 0 || g->L <= 0 || (int64_t)g->X * g->Y * g->L != g->n_nodes))
+        return orgpu_fail(ORGPU_EINVAL, "route_async: goal-directed search needs the lattice shape");
+    if (keepout_owner && !(keepout_cost >= 1.0e6f))
+        return orgpu_fail(ORGPU_EINVAL, "route_async: keepout_cost must dominate path costs (>= 1e6)");
+    if (params->slots < 1 || params->list_cap < 0)
+        return orgpu_fail(ORGPU_EINVAL, "route_async: slots %d / list_cap %d", params->slots, params->list_cap);
+    HIP_CHECK_RET(hipSetDevice(h->device));
+    return engine_route_async(h->en, g, reqs, n, roi_bitmaps, n_roi, keepout_owner, params, path_nodes, path_edges,
+                              capacity, path_off, path_len, path_dist, used);
+}
+
 extern "C" int orgpu_route_ms(orgpu_t *h, const orgpu_graph *g, const orgpu_ms_req *reqs, int32_t n_reqs,
                               const orgpu_ms_point *seeds, int64_t n_seeds, const orgpu_ms_point *targets,
                               int64_t n_targets, const uint32_t *roi_bitmaps, int32_t n_roi,

@@ -9,10 +9,10 @@
 // ---------------------------------------------------------------------------
 // Packed search key (one u64 per (slot, node)), minimised with atomicMin:
 //
-//   bits 63..59  keepout edges on the path (saturating at 31)
-//   bits 58..28  FP32 bit pattern of the path cost without keepout edges
+//   bits 63..56  keepout edges on the path (counting up to KEY_KO_MAX)
+//   bits 55..25  FP32 bit pattern of the path cost without keepout edges
 //                (non-negative floats order like their bit patterns)
-//   bits 27..0   predecessor field: predecessor node index + 1 (lowest
+//   bits 24..0   predecessor field: predecessor node index + 1 (lowest
 //                index wins ties); 0 for a source, so that a source keeps
 //                "no predecessor" against an equal-cost path into it
 //
@@ -20,23 +20,26 @@
 // (UPF:3182, 3220) and sums in float64; with path costs far below 1e9 that
 // orders paths by (number of keepout edges, remaining cost), which is what the
 // two top fields encode without the FP32 absorption a 1e9 summand would cause.
+// A path whose count reaches KEY_KO_MAX is reported instead of routed (the
+// field could no longer order it).
 // ---------------------------------------------------------------------------
 #define KEY_INF 0xFFFFFFFFFFFFFFFFull
-#define KEY_PRED_BITS 28
-#define KEY_PRED_MASK 0x0FFFFFFFu
+#define KEY_PRED_BITS 25
+#define KEY_PRED_MASK 0x01FFFFFFu
 #define KEY_NOPRED 0u
-#define KEY_MAX_NODES ((1ll << 28) - 2)
+#define KEY_MAX_NODES ((1ll << 25) - 2)
+#define KEY_KO_MAX 255u
 
 __host__ __device__ inline uint64_t key_pack(uint32_t ko, float d, uint32_t pred)
 {
     uint32_t db;
     __builtin_memcpy(&db, &d, 4);
-    return ((uint64_t)ko << 59) | ((uint64_t)(db & 0x7FFFFFFFu) << 28) | (uint64_t)(pred & KEY_PRED_MASK);
+    return ((uint64_t)ko << 56) | ((uint64_t)(db & 0x7FFFFFFFu) << 25) | (uint64_t)(pred & KEY_PRED_MASK);
 }
-__host__ __device__ inline uint32_t key_ko(uint64_t k) { return (uint32_t)(k >> 59); }
+__host__ __device__ inline uint32_t key_ko(uint64_t k) { return (uint32_t)(k >> 56); }
 __host__ __device__ inline float key_dist(uint64_t k)
 {
-    uint32_t db = (uint32_t)((k >> 28) & 0x7FFFFFFFu);
+    uint32_t db = (uint32_t)((k >> 25) & 0x7FFFFFFFu);
     float d;
     __builtin_memcpy(&d, &db, 4);
     return d;
@@ -44,7 +47,9 @@ __host__ __device__ inline float key_dist(uint64_t k)
 // predecessor field (KEY_NOPRED or node + 1)
 __host__ __device__ inline uint32_t key_pred(uint64_t k) { return (uint32_t)(k & KEY_PRED_MASK); }
 // (keepout count, distance) part, comparable as an integer
-__host__ __device__ inline uint64_t key_cost(uint64_t k) { return k >> 28; }
+__host__ __device__ inline uint64_t key_cost(uint64_t k) { return k >> 25; }
+// one more keepout edge, counting up to KEY_KO_MAX
+__host__ __device__ inline uint32_t ko_add(uint32_t ko) { return ko < KEY_KO_MAX ? ko + 1u : KEY_KO_MAX; }
 
 // bucket of a key relative to the slot's base bucket; h: lower bound of the
 // remaining cost to the destination (0 for plain Dijkstra order)

@@ -124,6 +124,12 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
                  const orgpu_sssp_params *p, int32_t *path_nodes, int32_t *path_edges,
                  int64_t path_capacity, int64_t *path_offset, float *path_dist, orgpu_sssp_stats *stats,
                  const MsCall *ms = nullptr);
+int engine_reserve_solo(Engine &en, int64_t n_nodes, int32_t X, int32_t Y, int32_t L, int32_t slots,
+                        int32_t list_cap);
+int engine_route_async(Engine &en, const orgpu_graph *g, const orgpu_net_req *d_reqs, int32_t n,
+                       const uint32_t *roi_bitmaps, int32_t n_roi, const int32_t *keepout_owner,
+                       const orgpu_sssp_params *p, int32_t *path_nodes, int32_t *path_edges, int64_t capacity,
+                       uint32_t *path_off, int32_t *path_len, float *path_dist, uint32_t *used);
 int engine_route_solo(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, int32_t n_nets,
                       const uint32_t *roi_bitmaps, int32_t n_roi, const int32_t *keepout_owner, float keepout_cost,
                       const orgpu_sssp_params *p, int32_t *path_nodes, int32_t *path_edges, int64_t path_capacity,

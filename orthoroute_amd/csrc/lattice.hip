@@ -50,7 +50,7 @@ static int validate_desc(const orgpu_lattice_desc *d)
     if (d->X < 1 || d->Y < 1 || d->L < 1 || d->L > ORGPU_MAX_LAYERS)
         return orgpu_fail(ORGPU_EINVAL, "lattice: bad shape %d x %d x %d", d->X, d->Y, d->L);
     int64_t N = (int64_t)d->X * d->Y * d->L;
-    if (N >= KEY_MAX_NODES) return orgpu_fail(ORGPU_EINVAL, "lattice: %lld nodes exceed 2^28", (long long)N);
+    if (N >= KEY_MAX_NODES) return orgpu_fail(ORGPU_EINVAL, "lattice: %lld nodes exceed 2^25 - 2 (the search key's predecessor field)", (long long)N);
     if (d->npairs < 0 || (d->npairs > 0 && (!d->pair_from || !d->pair_to)))
         return orgpu_fail(ORGPU_EINVAL, "lattice: bad via pair list");
     for (int p = 0; p < d->npairs; ++p) {

@@ -314,7 +314,7 @@ __device__ __forceinline__ void relax_item(const RoundArgs &a, uint32_t slot, ui
             if (roi && !((rw[q] >> (v[q] & 31)) & 1u)) { ok[q] = false; continue; }
             uint32_t ko = ko0;
             float nd = d0;
-            if (planar_ko[q]) ko = ko < 31 ? ko + 1 : 31;
+            if (planar_ko[q]) ko = ko_add(ko);
             else nd = nd + c[q];
             nk[q] = key_pack(ko, nd, u + 1u);
             ok[q] = nd < __builtin_inff() && nk[q] < kv[q];
@@ -933,7 +933,7 @@ __global__ void k_seed(RoundArgs a, const SeedReq *reqs, int n, const orgpu_ms_p
 // hop); then the lanes find each hop's CSR edge id (first match in the sorted
 // row, UPF:5463-5476) and pack the path src -> dst into the shared output.
 // len = 0: unreachable; -1: broken chain or longer than the staging row;
-// -2: the path crosses 31 or more keepout edges (the key saturates there).
+// -2: the path crosses KEY_KO_MAX or more keepout edges (the key's count stops there).
 __global__ void __launch_bounds__(64)
 k_backtrace(RoundArgs a, const int32_t *slot_list, int n, int32_t *stage_nodes, int32_t *stage_edges,
             int32_t *lens, float *dists, int32_t max_len, double *ms_total, int32_t *ms_choice,
@@ -979,8 +979,8 @@ k_backtrace(RoundArgs a, const int32_t *slot_list, int n, int32_t *stage_nodes, 
         }
         if (!found) {
             dists[s] = __builtin_inff();
-        } else if (key_ko(ks[end]) >= 31u) {
-            // the key's keepout count saturates at 31: past it, paths would
+        } else if (key_ko(ks[end]) >= KEY_KO_MAX) {
+            // the key's keepout count stops at KEY_KO_MAX: past it, paths would
             // no longer be ordered by their number of keepout edges
             dists[s] = key_dist(ks[end]);
             len = -2;
@@ -1546,7 +1546,7 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
         for (int i = 0; i < nf; ++i) {
             const int k = en.h_slot_list[i];
             if (en.h_lens[k] == -2)
-                return orgpu_fail(ORGPU_ECAPACITY, "route: request %d: its path crosses 31 or more keepout edges, "
+                return orgpu_fail(ORGPU_ECAPACITY, "route: request %d: its path crosses 255 or more keepout edges, "
                                   "which the search key cannot count", slot_req[k]);
             if (en.h_lens[k] < 0)
                 return orgpu_fail(ORGPU_ECAPACITY, "route: request %d: predecessor chain broken or longer than %d",

@@ -44,7 +44,7 @@
 #include "common.h"
 #include "engine.h"
 
-#define SOLO_LEVELS 10              // hex digits of a bucket index (keepout count and 32 bits)
+#define SOLO_LEVELS 10              // hex digits of a bucket index (8-bit keepout count and 32 bits)
 #define SOLO_NL (1 + SOLO_LEVELS * 16)   // current bucket + one list per (digit, value)
 #define SOLO_TOUCH_WORDS 1024       // LDS touch bitmap: 32768 key chunks
 #define SOLO_DD_BITS 9
@@ -53,8 +53,10 @@
 
 // request outcomes (out_len) besides a path length >= 0
 #define SOLO_BROKEN (-1)            // predecessor chain broken or longer than the staging row
-#define SOLO_KO_SATURATED (-2)      // path crosses 31 or more keepout edges
+#define SOLO_KO_SATURATED (-2)      // path crosses KEY_KO_MAX or more keepout edges
 #define SOLO_RETRY (-3)             // list or output capacity exceeded: route it on the other engine
+#define SOLO_INVALID (-4)           // endpoint outside the graph, ROI row outside the rows given, or a
+                                    // keepout request without the owner map
 
 struct SoloArgs {
     const int32_t *indptr;
@@ -68,6 +70,8 @@ struct SoloArgs {
     int32_t X, Y, plane;           // node = z*plane + y*X + x
     int32_t tiled;                 // keys in 2x2x4 (x, y, z) tiles of 16 (one 128-byte line)
     int32_t TX, TY;                // tiles along x and y
+    uint64_t m_plane, m_x;         // division by plane and by X (magic_div)
+    uint32_t s_plane, s_x;
     int32_t lookahead;             // buckets past the current one an under-full step takes entries from
     float inv_delta, h_xy, h_z;
     const orgpu_net_req *reqs;
@@ -80,8 +84,10 @@ struct SoloArgs {
     uint32_t touch_shift;          // keys per touch chunk = 2^touch_shift
     int32_t *stage;                // [slots][max_path] predecessor walk
     int32_t max_path;
-    int32_t *pack;                 // nodes [0, pack_half), edges [pack_half, 2 pack_half)
-    uint64_t pack_half;
+    int32_t *pack_nodes;           // packed paths src -> dst, pack_cap entries
+    int32_t *pack_edges;           // first CSR edge of each hop (-1 after the last node), or null
+    uint64_t pack_cap;
+    int32_t n_roi;                 // ROI rows (requests are checked against them and N)
     int32_t *out_len;
     uint32_t *out_off;
     float *out_dist;
@@ -297,13 +303,21 @@ struct Cell {
     int x, y, z;
 };
 
+// floor(v / d) for v < 2^28 as (v * m) >> sh with m = ceil(2^sh / d),
+// sh = 28 + ceil(log2 d) (solo_magic): exact, since m * d - 2^sh < d and
+// v < 2^28 keep the error term below 1 / d
+__device__ __forceinline__ uint32_t magic_div(uint32_t v, uint64_t m, uint32_t sh)
+{
+    return (uint32_t)(((uint64_t)v * m) >> sh);
+}
+
 __device__ __forceinline__ Cell node_cell(const SoloArgs &a, uint32_t v)
 {
     Cell c;
-    c.z = (int)(v / (uint32_t)a.plane);
-    const int r = (int)(v - (uint32_t)c.z * (uint32_t)a.plane);
-    c.y = r / a.X;
-    c.x = r - c.y * a.X;
+    c.z = (int)magic_div(v, a.m_plane, a.s_plane);
+    const uint32_t r = v - (uint32_t)c.z * (uint32_t)a.plane;
+    c.y = (int)magic_div(r, a.m_x, a.s_x);
+    c.x = (int)(r - (uint32_t)c.y * (uint32_t)a.X);
     return c;
 }
 
@@ -355,6 +369,15 @@ k_solo(SoloArgs a)
         r = (uint32_t)__shfl((int)r, 0);
         if (r >= (uint32_t)a.n_reqs) break;
         const orgpu_net_req q = a.reqs[r];
+        if (q.src < 0 || (int64_t)q.src >= a.N || q.dst < 0 || (int64_t)q.dst >= a.N || q.roi_index >= a.n_roi ||
+            (q.use_keepout && !a.ko_owner)) {
+            if (lane == 0) {
+                a.out_len[r] = SOLO_INVALID;
+                a.out_off[r] = 0u;
+                a.out_dist[r] = __builtin_inff();
+            }
+            continue;
+        }
         const uint32_t src = (uint32_t)q.src, dst = (uint32_t)q.dst;
         int dx = 0, dy = 0, dz = 0;
         if (goal) {
@@ -397,8 +420,11 @@ k_solo(SoloArgs a)
                 if (n < 64u && a.lookahead > 0) {
                     const uint32_t d0 = (uint32_t)(cur & 15u);
                     const uint32_t dl = min(15u, d0 + (uint32_t)a.lookahead);
-                    for (uint32_t d = d0 + 1u; d <= dl && n < 64u; ++d) {
-                        if (L.empty(1u + d)) continue;
+                    // the non-empty lists of digits d0+1 .. dl
+                    uint32_t mk = L.mask[0] & (((2u << dl) - 1u) & ~((2u << d0) - 1u));
+                    while (mk && n < 64u) {
+                        const uint32_t d = (uint32_t)__builtin_ctz(mk);
+                        mk &= mk - 1u;
                         uint32_t n2 = 0;
                         const uint64_t e2 = L.pop(1u + d, lane, n, 64u - n, n2);
                         if (lane >= n && lane < n + n2) e = e2;
@@ -466,7 +492,7 @@ k_solo(SoloArgs a)
                         if (!go[j]) continue;
                         uint32_t ko = ko0;
                         float nd = d0;
-                        if (pko[j]) ko = ko < 31 ? ko + 1 : 31;
+                        if (pko[j]) ko = ko_add(ko);
                         else nd = nd + c[j];
                         nk[j] = key_pack(ko, nd, u + 1u);
                         go[j] = nd < __builtin_inff() && nk[j] < kv[j];
@@ -578,7 +604,7 @@ k_solo(SoloArgs a)
         } else if (reached) {
             const uint64_t kd = kload(&keys[kdst]);
             dist = key_dist(kd);
-            if (key_ko(kd) >= 31u) {
+            if (key_ko(kd) >= KEY_KO_MAX) {
                 len = SOLO_KO_SATURATED;
             } else if (lane == 0) {
                 uint32_t v = dst;
@@ -597,17 +623,18 @@ k_solo(SoloArgs a)
         if (len > 0) {
             if (lane == 0) off = atomicAdd(&a.ctl[1], (uint32_t)len);
             off = (uint32_t)__shfl((int)off, 0);
-            if ((uint64_t)off + (uint64_t)len > a.pack_half) len = SOLO_RETRY;
+            if ((uint64_t)off + (uint64_t)len > a.pack_cap) len = SOLO_RETRY;
         }
         if (len > 0) {
             wave_fence();
-            int32_t *on = a.pack + off;
-            int32_t *oe = a.pack + a.pack_half + off;
+            int32_t *on = a.pack_nodes + off;
+            int32_t *oe = a.pack_edges ? a.pack_edges + off : nullptr;
             // node t of the packed path is stage[len-1-t]; the edge leaving
             // it is the first CSR edge stage[len-1-t] -> stage[len-2-t]
             for (int t = lane; t < len; t += 64) {
                 const int32_t un = stage[len - 1 - t];
                 on[t] = un;
+                if (!oe) continue;
                 int ee = -1;
                 if (t + 1 < len) {
                     const int32_t vn = stage[len - 2 - t];
@@ -659,7 +686,59 @@ k_solo(SoloArgs a)
 // host driver
 // ---------------------------------------------------------------------------
 
-static int solo_reserve(Engine &en, int32_t slots, int32_t n_reqs, uint32_t nch)
+// m = ceil(2^sh / d), sh = 28 + ceil(log2 d): magic_div(v, m, sh) = v / d
+// for every v < 2^28 (the node-index range)
+static void solo_magic(uint32_t d, uint64_t *m, uint32_t *sh)
+{
+    uint32_t lg = 0;
+    while ((1ull << lg) < (uint64_t)d) ++lg;
+    *sh = 28u + lg;
+    *m = ((1ull << *sh) + (uint64_t)d - 1ull) / (uint64_t)d;
+}
+
+// the call's control words: zeroed by a kernel (a stream a caller captures
+// records no memset node)
+__global__ void k_solo_begin(uint32_t *ctl)
+{
+    if (threadIdx.x < 16) ctl[threadIdx.x] = 0u;
+}
+
+__global__ void k_solo_used(const uint32_t *ctl, uint32_t *used)
+{
+    if (threadIdx.x == 0) *used = ctl[1];
+}
+
+// Key layout of a graph: 2x2x4 tiles on a lattice (X*Y*L nodes in node
+// order z*X*Y + y*X + x), node order otherwise; a slot's stride NP covers
+// the index space in whole 2048-key chunks, touched in chunks of 2^tshift.
+struct SoloLayout {
+    bool tiled;
+    int32_t TX, TY;
+    int64_t NP;
+    uint32_t tshift;
+};
+
+static SoloLayout solo_layout(int64_t N, int32_t X, int32_t Y, int32_t L)
+{
+    SoloLayout y;
+    y.tiled = X > 0 && Y > 0 && L > 0 && (int64_t)X * Y * L == N;
+    y.TX = (X + 1) / 2;
+    y.TY = (Y + 1) / 2;
+    const int64_t span = y.tiled ? (int64_t)y.TX * y.TY * ((L + 3) / 4) * 16 : N;
+    y.NP = (span + 2047) / 2048 * 2048;
+    y.tshift = 6;
+    while ((((uint64_t)y.NP + (1ull << y.tshift) - 1) >> y.tshift) > (uint64_t)SOLO_TOUCH_WORDS * 32u) ++y.tshift;
+    return y;
+}
+
+// bucket-list storage per search: list_cap entries (0: 2^20), in chunks
+static uint32_t solo_chunks(int32_t list_cap)
+{
+    const uint64_t want = list_cap > 0 ? (uint64_t)list_cap : (1ull << 20);
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(2, (want + SOLO_CH - 1) / SOLO_CH), 1u << 24);
+}
+
+static int solo_reserve_arena(Engine &en, int32_t slots, uint32_t nch)
 {
     if (en.solo_arena == nullptr || en.solo_slots < slots || en.solo_nch != nch) {
         if (en.solo_arena) hipFree(en.solo_arena);
@@ -670,21 +749,40 @@ static int solo_reserve(Engine &en, int32_t slots, int32_t n_reqs, uint32_t nch)
         en.solo_slots = slots;
         en.solo_nch = nch;
     }
-    if (en.solo_req_cap < n_reqs) {
-        if (en.solo_req) hipFree(en.solo_req);
-        if (en.solo_out) hipFree(en.solo_out);
-        if (en.h_solo_out) hipHostFree(en.h_solo_out);
-        en.solo_req = nullptr; en.solo_out = nullptr; en.h_solo_out = nullptr;
-        const int32_t cap = std::max(n_reqs, 4096);
-        HIP_CHECK_RET(hipMalloc((void **)&en.solo_req, sizeof(orgpu_net_req) * cap));
-        HIP_CHECK_RET(hipMalloc((void **)&en.solo_out, (size_t)3 * cap * sizeof(int32_t)));
-        HIP_CHECK_RET(hipHostMalloc((void **)&en.h_solo_out, (size_t)3 * cap * sizeof(int32_t), hipHostMallocDefault));
-        en.solo_req_cap = cap;
-    }
     if (!en.solo_ctl) {
         HIP_CHECK_RET(hipMalloc((void **)&en.solo_ctl, 64));
         HIP_CHECK_RET(hipHostMalloc((void **)&en.h_solo_ctl, 64, hipHostMallocDefault));
     }
+    return ORGPU_OK;
+}
+
+// keys for `slots` searches at stride NP (the tiled index space can exceed
+// the node count the handle reserved for); every key is refilled with
+// KEY_INF by the next engine_prepare_keys
+static int solo_reserve_keys(Engine &en, int32_t slots, int64_t NP)
+{
+    if ((uint64_t)slots * (uint64_t)NP <= (uint64_t)en.keys_cap) return ORGPU_OK;
+    if (en.keys) hipFree(en.keys);
+    en.keys = nullptr;
+    const int64_t cap = std::max<int64_t>((int64_t)slots * NP, en.keys_cap);
+    HIP_CHECK_RET(hipMalloc((void **)&en.keys, (size_t)cap * sizeof(uint64_t)));
+    en.keys_cap = cap;
+    en.keys_n = -1;
+    return ORGPU_OK;
+}
+
+static int solo_reserve_requests(Engine &en, int32_t n_reqs)
+{
+    if (en.solo_req_cap >= n_reqs) return ORGPU_OK;
+    if (en.solo_req) hipFree(en.solo_req);
+    if (en.solo_out) hipFree(en.solo_out);
+    if (en.h_solo_out) hipHostFree(en.h_solo_out);
+    en.solo_req = nullptr; en.solo_out = nullptr; en.h_solo_out = nullptr;
+    const int32_t cap = std::max(n_reqs, 4096);
+    HIP_CHECK_RET(hipMalloc((void **)&en.solo_req, sizeof(orgpu_net_req) * cap));
+    HIP_CHECK_RET(hipMalloc((void **)&en.solo_out, (size_t)3 * cap * sizeof(int32_t)));
+    HIP_CHECK_RET(hipHostMalloc((void **)&en.h_solo_out, (size_t)3 * cap * sizeof(int32_t), hipHostMallocDefault));
+    en.solo_req_cap = cap;
     return ORGPU_OK;
 }
 
@@ -703,9 +801,83 @@ void solo_release(Engine &en)
     en.solo_slots = 0; en.solo_nch = 0; en.solo_req_cap = 0;
 }
 
-// Two-pin requests on the solo engine.  Arguments were checked by
-// engine_route; outputs as engine_route writes them.  Requests the engine
-// gives back for capacity are routed on the bucket-round engine.
+// The kernel arguments of a call (everything but requests and outputs).
+static void solo_args(SoloArgs &a, const Engine &en, const orgpu_graph *g, const SoloLayout &lay,
+                      const orgpu_sssp_params *p, const uint32_t *roi_bitmaps, int32_t n_roi,
+                      const int32_t *keepout_owner)
+{
+    const int64_t N = g->n_nodes;
+    memset(&a, 0, sizeof(a));
+    a.indptr = g->indptr; a.indices = g->indices; a.cost = g->cost; a.kind = g->edge_kind;
+    a.roi = roi_bitmaps; a.roi_words = (N + 31) / 32;
+    a.n_roi = roi_bitmaps ? n_roi : 0;
+    a.ko_owner = keepout_owner;
+    a.N = N; a.NP = lay.NP;
+    a.X = g->X; a.Y = g->Y; a.plane = g->X * g->Y;
+    a.tiled = lay.tiled ? 1 : 0; a.TX = lay.TX; a.TY = lay.TY;
+    if (lay.tiled || p->h_xy > 0.0f || p->h_z > 0.0f) {
+        solo_magic((uint32_t)a.plane, &a.m_plane, &a.s_plane);
+        solo_magic((uint32_t)a.X, &a.m_x, &a.s_x);
+    }
+    a.lookahead = p->lookahead == 0 ? 15 : (p->lookahead < 0 ? 0 : std::min(p->lookahead, 15));
+    a.inv_delta = 1.0f / p->delta;
+    a.h_xy = p->h_xy; a.h_z = p->h_z;
+    a.ctl = en.solo_ctl;
+    a.keys = en.keys;
+    a.arena = en.solo_arena; a.links = en.solo_links; a.nch = en.solo_nch;
+    a.touch_shift = lay.tshift;
+    a.stage = en.stage_nodes; a.max_path = en.cap_path;
+    a.counters = (unsigned long long *)(en.solo_ctl + 4);
+}
+
+int engine_reserve_solo(Engine &en, int64_t n_nodes, int32_t X, int32_t Y, int32_t L, int32_t slots, int32_t list_cap)
+{
+    if (en.cap_slots == 0) return orgpu_fail(ORGPU_EINVAL, "reserve_solo: call orgpu_reserve first");
+    if (slots < 1 || slots > en.cap_slots || n_nodes < 1 || n_nodes > en.cap_nodes || list_cap < 0)
+        return orgpu_fail(ORGPU_EINVAL, "reserve_solo: (nodes %lld, slots %d, list_cap %d) outside the handle's "
+                          "reserve (nodes %lld, slots %d)", (long long)n_nodes, slots, list_cap,
+                          (long long)en.cap_nodes, en.cap_slots);
+    const SoloLayout lay = solo_layout(n_nodes, X, Y, L);
+    int rc = solo_reserve_keys(en, slots, lay.NP);
+    if (rc) return rc;
+    rc = solo_reserve_arena(en, slots, solo_chunks(list_cap));
+    if (rc) return rc;
+    return engine_prepare_keys(en, n_nodes);
+}
+
+int engine_route_async(Engine &en, const orgpu_graph *g, const orgpu_net_req *d_reqs, int32_t n,
+                       const uint32_t *roi_bitmaps, int32_t n_roi, const int32_t *keepout_owner,
+                       const orgpu_sssp_params *p, int32_t *path_nodes, int32_t *path_edges, int64_t capacity,
+                       uint32_t *path_off, int32_t *path_len, float *path_dist, uint32_t *used)
+{
+    const int64_t N = g->n_nodes;
+    const int K = std::min(p->slots, en.cap_slots);
+    const SoloLayout lay = solo_layout(N, g->X, g->Y, g->L);
+    if (!en.solo_arena || en.solo_slots < K || en.solo_nch != solo_chunks(p->list_cap) ||
+        (uint64_t)K * (uint64_t)lay.NP > (uint64_t)en.keys_cap)
+        return orgpu_fail(ORGPU_EINVAL, "route_async: call orgpu_reserve_solo for this graph, %d slots and list_cap %d "
+                          "first", K, p->list_cap);
+    if (capacity < 0 || capacity > (int64_t)UINT32_MAX)
+        return orgpu_fail(ORGPU_EINVAL, "route_async: capacity %lld outside [0, 2^32)", (long long)capacity);
+    hipStream_t s = en.stream;
+    int rc = engine_prepare_keys(en, N);
+    if (rc) return rc;
+    SoloArgs a;
+    solo_args(a, en, g, lay, p, roi_bitmaps, n_roi, keepout_owner);
+    a.reqs = d_reqs; a.n_reqs = n;
+    a.pack_nodes = path_nodes; a.pack_edges = path_edges; a.pack_cap = (uint64_t)capacity;
+    a.out_len = path_len; a.out_off = path_off; a.out_dist = path_dist;
+    hipLaunchKernelGGL(k_solo_begin, dim3(1), dim3(64), 0, s, en.solo_ctl);
+    if (n > 0) hipLaunchKernelGGL(k_solo, dim3(std::min(K, n)), dim3(64), 0, s, a);
+    if (used) hipLaunchKernelGGL(k_solo_used, dim3(1), dim3(64), 0, s, en.solo_ctl, used);
+    HIP_CHECK_RET(hipGetLastError());
+    return ORGPU_OK;
+}
+
+// Two-pin requests on the solo engine, blocking (orgpu_route with engine 1).
+// Arguments were checked by engine_route; outputs as engine_route writes
+// them.  Requests the engine gives back for capacity are routed on the
+// bucket-round engine.
 int engine_route_solo(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, int32_t n_nets,
                       const uint32_t *roi_bitmaps, int32_t n_roi, const int32_t *keepout_owner, float keepout_cost,
                       const orgpu_sssp_params *p, int32_t *path_nodes, int32_t *path_edges, int64_t path_capacity,
@@ -717,61 +889,28 @@ int engine_route_solo(Engine &en, const orgpu_graph *g, const orgpu_net_req *net
     path_offset[0] = 0;
     if (stats) memset(stats, 0, sizeof(*stats));
     if (n_nets == 0) return ORGPU_OK;
-    // bucket-list storage per search: p->list_cap entries (0: 2^20), in
-    // whole chunks
-    const uint64_t want = p->list_cap > 0 ? (uint64_t)p->list_cap : (1ull << 20);
-    const uint32_t nch = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(2, (want + SOLO_CH - 1) / SOLO_CH), 1u << 24);
-    int rc = solo_reserve(en, K, n_nets, nch);
+    const SoloLayout lay = solo_layout(N, g->X, g->Y, g->L);
+    int rc = solo_reserve_keys(en, K, lay.NP);
     if (rc) return rc;
-    // key layout: 2x2x4 tiles on a lattice (the graph's shape is X*Y*L nodes
-    // in node order z*X*Y + y*X + x), node order otherwise; a slot's stride
-    // NP covers the index space in whole 2048-key chunks
-    const bool tiled = g->X > 0 && g->Y > 0 && g->L > 0 && (int64_t)g->X * g->Y * g->L == N;
-    const int32_t TX = (g->X + 1) / 2, TY = (g->Y + 1) / 2, TZ = (g->L + 3) / 4;
-    const int64_t span = tiled ? (int64_t)TX * TY * TZ * 16 : N;
-    const int64_t NP = (span + 2047) / 2048 * 2048;
-    if ((uint64_t)K * (uint64_t)NP > (uint64_t)en.keys_cap) {
-        // the tiled index space is larger than the node count: grow the keys
-        // (every key is refilled with KEY_INF below)
-        if (en.keys) hipFree(en.keys);
-        en.keys = nullptr;
-        const int64_t cap = std::max<int64_t>((int64_t)K * NP, en.keys_cap);
-        HIP_CHECK_RET(hipMalloc((void **)&en.keys, (size_t)cap * sizeof(uint64_t)));
-        en.keys_cap = cap;
-        en.keys_n = -1;
-    }
+    rc = solo_reserve_arena(en, K, solo_chunks(p->list_cap));
+    if (rc) return rc;
+    rc = solo_reserve_requests(en, n_nets);
+    if (rc) return rc;
     rc = engine_prepare_keys(en, N);
     if (rc) return rc;
-    uint32_t tshift = 6;
-    while ((((uint64_t)NP + (1ull << tshift) - 1) >> tshift) > (uint64_t)SOLO_TOUCH_WORDS * 32u) ++tshift;
     const int64_t cap = en.solo_req_cap;
     const uint64_t half = (uint64_t)en.cap_slots * (uint64_t)en.cap_path;
 
     SoloArgs a;
-    memset(&a, 0, sizeof(a));
-    a.indptr = g->indptr; a.indices = g->indices; a.cost = g->cost; a.kind = g->edge_kind;
-    a.roi = roi_bitmaps; a.roi_words = (N + 31) / 32;
-    a.ko_owner = keepout_owner;
-    a.N = N; a.NP = NP;
-    a.X = g->X; a.Y = g->Y; a.plane = g->X * g->Y;
-    a.tiled = tiled ? 1 : 0; a.TX = TX; a.TY = TY;
-    a.lookahead = p->lookahead == 0 ? 15 : (p->lookahead < 0 ? 0 : std::min(p->lookahead, 15));
-    a.inv_delta = 1.0f / p->delta;
-    a.h_xy = p->h_xy; a.h_z = p->h_z;
+    solo_args(a, en, g, lay, p, roi_bitmaps, n_roi, keepout_owner);
     a.reqs = en.solo_req; a.n_reqs = n_nets;
-    a.ctl = en.solo_ctl;
-    a.keys = en.keys;
-    a.arena = en.solo_arena; a.links = en.solo_links; a.nch = nch;
-    a.touch_shift = tshift;
-    a.stage = en.stage_nodes; a.max_path = en.cap_path;
-    a.pack = en.pack; a.pack_half = half;
+    a.pack_nodes = en.pack; a.pack_edges = en.pack + half; a.pack_cap = half;
     a.out_len = en.solo_out;
     a.out_off = (uint32_t *)(en.solo_out + cap);
     a.out_dist = (float *)(en.solo_out + 2 * cap);
-    a.counters = (unsigned long long *)(en.solo_ctl + 4);
 
     HIP_CHECK_RET(hipMemcpyAsync(en.solo_req, nets, sizeof(orgpu_net_req) * n_nets, hipMemcpyHostToDevice, s));
-    HIP_CHECK_RET(hipMemsetAsync(en.solo_ctl, 0, 64, s));
+    hipLaunchKernelGGL(k_solo_begin, dim3(1), dim3(64), 0, s, en.solo_ctl);
     HIP_CHECK_RET(hipEventRecord(en.ev_t0, s));
     hipLaunchKernelGGL(k_solo, dim3(std::min(K, n_nets)), dim3(64), 0, s, a);
     HIP_CHECK_RET(hipGetLastError());
@@ -801,11 +940,13 @@ int engine_route_solo(Engine &en, const orgpu_graph *g, const orgpu_net_req *net
     std::vector<int32_t> retry;
     for (int i = 0; i < n_nets; ++i) {
         if (h_len[i] == SOLO_KO_SATURATED)
-            return orgpu_fail(ORGPU_ECAPACITY, "route: request %d: its path crosses 31 or more keepout edges, "
+            return orgpu_fail(ORGPU_ECAPACITY, "route: request %d: its path crosses 255 or more keepout edges, "
                               "which the search key cannot count", i);
         if (h_len[i] == SOLO_BROKEN)
             return orgpu_fail(ORGPU_ECAPACITY, "route: request %d: predecessor chain broken or longer than %d", i,
                               en.cap_path);
+        if (h_len[i] == SOLO_INVALID)
+            return orgpu_fail(ORGPU_EINVAL, "route: request %d outside the graph or the ROI rows", i);
         if (h_len[i] == SOLO_RETRY) retry.push_back(i);
     }
     // requests given back for capacity: routed on the bucket-round engine

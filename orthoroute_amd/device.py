@@ -371,6 +371,46 @@ def route_flat(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner
     return nodes, edges, offs, dist[:n], stats
 
 
+def reserve_solo(ctx, lat, slots, list_cap=0):
+    """The one-wave engine's scratch for `slots` concurrent searches on
+    lattice `lat` (orgpu_reserve_solo, after the handle's orgpu_reserve)."""
+    ctx.reserve(lat.N, slots, 8)
+    _abi.check(ctx.lib.orgpu_reserve_solo(ctx.handle, int(lat.N), int(lat.X), int(lat.Y), int(lat.L), int(slots),
+                                          int(list_cap)))
+
+
+def route_async(ctx, lat, cost, reqs_d, params, out, roi_bitmaps=None, keepout_owner=None):
+    """Enqueues the searches of the device request rows reqs_d (int32
+    [n, 6]: src, dst, net_id, roi_index, use_keepout, 0) on the handle's
+    stream (orgpu_route_async) and returns at once.  out: a dict of device
+    tensors from route_async_outputs.  After ctx.sync(): out["len"][i] nodes
+    of request i's path at out["nodes"][out["off"][i]:] (edges likewise),
+    out["dist"][i] its cost, out["used"] the entries written; a negative
+    length is the code of include/orgpu.h."""
+    n = int(reqs_d.shape[0])
+    g = lat.graph(cost)
+    n_roi = 0 if roi_bitmaps is None else int(roi_bitmaps.shape[0])
+    _abi.check(ctx.lib.orgpu_route_async(ctx.handle, ctypes.byref(g), _ptr(reqs_d), n, _ptr(roi_bitmaps), n_roi,
+                                         _ptr(keepout_owner), ctypes.c_float(KEEPOUT_COST), ctypes.byref(params.c()),
+                                         _ptr(out["nodes"]), _ptr(out.get("edges")), int(out["nodes"].numel()),
+                                         _ptr(out["off"]), _ptr(out["len"]), _ptr(out["dist"]), _ptr(out["used"])))
+
+
+def route_async_outputs(ctx, n, capacity, edges=True):
+    """Device result buffers for route_async: n requests, `capacity` packed
+    path entries."""
+    torch = ctx.torch
+    dev = ctx.device
+    out = {"nodes": torch.empty(max(int(capacity), 1), dtype=torch.int32, device=dev),
+           "off": torch.zeros(max(n, 1), dtype=torch.int32, device=dev),
+           "len": torch.zeros(max(n, 1), dtype=torch.int32, device=dev),
+           "dist": torch.zeros(max(n, 1), dtype=torch.float32, device=dev),
+           "used": torch.zeros(1, dtype=torch.int32, device=dev)}
+    if edges:
+        out["edges"] = torch.empty(max(int(capacity), 1), dtype=torch.int32, device=dev)
+    return out
+
+
 MS_POINT = np.dtype([("node", "<i4"), ("cost", "<f4")])
 # orgpu_roi_req / orgpu_roi_info rows (the ctypes layouts of _abi.RoiReq / RoiInfo)
 ROI_REQ = np.dtype({"names": ["src", "dst", "max_depth", "roi_index", "max_nodes", "readd", "reserved", "keep"],

@@ -116,6 +116,22 @@ def footprint(X, Y, L, policy, n_nets, slots, roi_limit):
     return int(1.05 * (csr + state + search + roi)) + (64 << 20)
 
 
+def fit_slots(job, slots, budget, roi_limit):
+    """The most concurrent searches, at most `slots`, whose job footprint
+    fits `budget` bytes (1 when none does: the job is then rejected)."""
+    fp = lambda k: footprint(job.X, job.Y, job.L, job.policy, len(job.nets), k, roi_limit)
+    if fp(slots) <= budget:
+        return slots
+    lo, hi = 1, slots
+    while lo < hi:
+        mid = (lo + hi + 1) // 2
+        if fp(mid) <= budget:
+            lo = mid
+        else:
+            hi = mid - 1
+    return lo
+
+
 def choose_slots(job, cap=1024, budget_bytes=None):
     """Concurrent searches for a job: all its nets at once when the slot
     memory (device.slot_bytes per search) fits budget_bytes (default 70 % of
@@ -208,7 +224,8 @@ class RoutingServer:
     def submit(self, job):
         """Queue a job; returns a Future of its JobResult."""
         fut = Future()
-        slots = choose_slots(job, self.slot_cap)
+        # as many searches at once as the board allows and the budget holds
+        slots = fit_slots(job, choose_slots(job, self.slot_cap), self.budget, self.roi_limit)
         fp = footprint(job.X, job.Y, job.L, job.policy, len(job.nets), slots, self.roi_limit)
         with self._cv:
             if self._closed:

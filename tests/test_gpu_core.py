@@ -146,29 +146,31 @@ def test_route_keepout_matches_oracle(ctx, goal):
     run_case(ctx, 20, 18, 6, "adjacent", 1, 10, 78, "random", keepout=True, goal=goal)
 
 
-def test_route_keepout_saturation_fails_loudly(ctx):
-    """The key counts keepout edges up to 31: a search whose best path
-    crosses more (every node of the adjacent lattice owned by another net, a
-    destination 40 planar steps away) raises instead of returning a path
-    ordered by cost alone; the same request without keepouts routes."""
+@pytest.mark.parametrize("engine", [0, 1])
+def test_route_keepout_saturation_fails_loudly(ctx, engine):
+    """The key counts keepout edges up to 255: a search whose best path
+    crosses that many (every node of the adjacent lattice owned by another
+    net, a destination 260 planar steps away) raises instead of returning a
+    path ordered by cost alone; the same request without keepouts routes, and
+    200 keepout edges route and equal the oracle.  Both search engines."""
     import torch
     from orthoroute_amd._abi import OrgpuError
     from orthoroute_amd.device import SearchParams, route
-    X, Y, L = 48, 3, 2
+    X, Y, L = 300, 3, 2
     g = ctx.build_lattice(X, Y, L, "adjacent")
     o = oracle.Lattice(X, Y, L, "adjacent")
     cost_d = torch.from_numpy(o.costs.astype(np.float32)).to(ctx.device)
     own_d = torch.full((o.N,), 5, dtype=torch.int32, device=ctx.device)
-    p = SearchParams(slots=1, delta=1.0, n_buckets=8)
-    src, dst = X + 2, X + 42          # row 1 of layer 0, 40 steps apart
+    p = SearchParams(slots=1, delta=1.0, n_buckets=8, engine=engine)
+    src, dst = X + 2, X + 262         # row 1 of layer 0, 260 steps apart
     res, _ = route(ctx, g, cost_d, [(src, dst, 1, -1, 0)], p, keepout_owner=own_d)
-    assert res[0] is not None and len(res[0][0]) >= 41
-    with pytest.raises(OrgpuError, match="31 or more keepout edges"):
+    assert res[0] is not None and len(res[0][0]) >= 261
+    with pytest.raises(OrgpuError, match="255 or more keepout edges"):
         route(ctx, g, cost_d, [(src, dst, 1, -1, 1)], p, keepout_owner=own_d)
-    # 20 keepout edges stay below the limit and equal the oracle
-    res, _ = route(ctx, g, cost_d, [(src, src + 20, 1, -1, 1)], p, keepout_owner=own_d)
+    # 200 keepout edges stay below the limit and equal the oracle
+    res, _ = route(ctx, g, cost_d, [(src, src + 200, 1, -1, 1)], p, keepout_owner=own_d)
     owner = np.full(o.N, 5, np.int32)
-    path = oracle.sssp_canon(o, o.costs.astype(np.float32), src, src + 20, keepout_owner=owner, net_id=1)[0]
+    path = oracle.sssp_canon(o, o.costs.astype(np.float32), src, src + 200, keepout_owner=owner, net_id=1)[0]
     assert res[0][0].tolist() == path
 
 

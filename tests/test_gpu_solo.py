@@ -128,3 +128,77 @@ def test_solo_edge_cases(ctx):
     # and the bucket engine again after the solo engine
     res2, _ = route(ctx, g, cost_d, reqs, p0, roi_bitmaps=roi_d)
     assert [None if r is None else r[0].tolist() for r in res2] == [None if r is None else r[0].tolist() for r in res0]
+
+
+def test_route_async_matches_route():
+    """orgpu_route_async (device requests and results, nothing allocated,
+    nothing waited for) returns the paths, edge ids and distances of
+    orgpu_route on the one-wave engine; invalid requests come back as -4, an
+    output too small as -3 (given back), and a call before
+    orgpu_reserve_solo fails."""
+    import torch
+    from orthoroute_amd._abi import OrgpuError
+    from orthoroute_amd.device import (Context, SearchParams, cost_bounds, reserve_solo, route_async,
+                                       route_async_outputs, route_flat)
+    ctx2 = Context(0)
+    X, Y, L = 60, 48, 6
+    rng = np.random.default_rng(71)
+    o = oracle.Lattice(X, Y, L, "adjacent")
+    g = ctx2.build_lattice(X, Y, L, "adjacent")
+    c = make_costs(o, rng, "jitter")
+    cost_d = torch.from_numpy(c).to(ctx2.device)
+    owner = np.full(o.N, -1, np.int32)
+    sel = rng.random(o.N) < 0.05
+    owner[sel] = rng.integers(0, 6, sel.sum())
+    own_d = torch.from_numpy(owner).to(ctx2.device)
+    reqs, rois = [], []
+    for i in range(300):
+        s, d = (int(v) for v in rng.integers(0, o.N, 2))
+        roi_idx = -1
+        if i % 3 == 0:
+            roi = oracle.roi_bfs(o, s, d, max_nodes=4000)
+            mask = np.zeros(o.N, np.uint8); mask[roi] = 1; mask[s] = 1; mask[d] = 1
+            roi_idx = len(rois)
+            rois.append(pack_bits(mask))
+        reqs.append((s, d, i % 6, roi_idx, int(i % 7 == 0)))
+    roi_d = torch.from_numpy(np.stack(rois)).to(ctx2.device)
+    hx, hz = cost_bounds(ctx2, g, cost_d)
+    p = SearchParams(slots=64, delta=0.5, n_buckets=8, h_xy=hx, h_z=hz, engine=1)
+    out = route_async_outputs(ctx2, len(reqs), 300 * 400)
+    req_d = torch.from_numpy(np.concatenate([np.asarray(reqs, np.int32), np.zeros((len(reqs), 1), np.int32)], 1)).to(
+        ctx2.device)
+    with pytest.raises(OrgpuError):
+        route_async(ctx2, g, cost_d, req_d, p, out, roi_bitmaps=roi_d, keepout_owner=own_d)
+    nodes, edges, offs, dist, st = route_flat(ctx2, g, cost_d, reqs, p, roi_bitmaps=roi_d, keepout_owner=own_d)
+    reserve_solo(ctx2, g, 64)
+    for call in range(2):
+        n = len(reqs) if call == 0 else 120     # a second, smaller call on the same reserve
+        route_async(ctx2, g, cost_d, req_d[:n], p, out, roi_bitmaps=roi_d, keepout_owner=own_d)
+        ctx2.sync()
+        ln = out["len"].cpu().numpy()[:n]
+        off = out["off"].cpu().numpy()[:n]
+        dd = out["dist"].cpu().numpy()[:n]
+        an, ae = out["nodes"].cpu().numpy(), out["edges"].cpu().numpy()
+        assert int(out["used"].cpu().numpy()[0]) == int(ln[ln > 0].sum())
+        for i in range(n):
+            exp = nodes[offs[i]:offs[i + 1]]
+            assert ln[i] == len(exp), (call, i)
+            np.testing.assert_array_equal(an[off[i]:off[i] + ln[i]], exp)
+            np.testing.assert_array_equal(ae[off[i]:off[i] + ln[i]], edges[offs[i]:offs[i + 1]])
+            if len(exp):
+                assert dd[i] == dist[i]
+    # invalid requests: endpoint outside the graph, ROI row past the rows,
+    # keepout without the owner map
+    bad = torch.tensor([[o.N, 0, 0, -1, 0, 0], [0, 5, 0, len(rois), 0, 0], [3, 9, 0, -1, 1, 0], [3, 9, 0, -1, 0, 0]],
+                       dtype=torch.int32, device=ctx2.device)
+    route_async(ctx2, g, cost_d, bad, p, out, roi_bitmaps=roi_d)
+    ctx2.sync()
+    assert out["len"].cpu().numpy()[:4].tolist()[:3] == [-4, -4, -4]
+    assert out["len"].cpu().numpy()[3] > 0
+    # an output too small for the paths: given back (-3), nothing written past it
+    small = route_async_outputs(ctx2, len(reqs), 40)
+    route_async(ctx2, g, cost_d, req_d, p, small, roi_bitmaps=roi_d, keepout_owner=own_d)
+    ctx2.sync()
+    sl = small["len"].cpu().numpy()[:len(reqs)]
+    assert (sl == -3).sum() > 0 and sl[sl > 0].sum() <= 40
+    ctx2.close()

@@ -271,3 +271,40 @@ def test_roi_with_portal_seeds_matches_reference(golden_dir):
         got = oracle.roi_bfs(lat, int(g["src"][i]), int(g["dst"][i]), stagnation_bonus=float(g["bonus"][i]),
                              max_nodes=int(g["max_nodes"][i]), keep=seeds)
         np.testing.assert_array_equal(got, ref, err_msg=f"case {i}")
+
+
+def test_canon_counts_many_keepout_edges_like_the_reference():
+    """Paths that cross far more than 31 keepout edges: the canonical search
+    counts every one (up to 254) and its path has the reference's keepout
+    count and cost (the reference sums 1e9 per keepout edge in float64), on a
+    lattice where every node belongs to another net and on one where a
+    keepout-free detour exists but costs more."""
+    X, Y, L = 300, 3, 2
+    o = oracle.Lattice(X, Y, L, "adjacent")
+    costs = o.costs.astype(np.float32)
+    for owner_kind in ("all", "rows"):
+        owner = np.full(o.N, 5, np.int32)
+        if owner_kind == "rows":
+            owner[2 * X:3 * X] = -1          # row 2 of layer 0 is free of keepouts
+        src = X + 2
+        for steps in (40, 120, 200):
+            dst = src + steps
+            path, dist, pred, *_ = oracle.sssp_canon(o, costs, src, dst, keepout_owner=owner, net_id=1)
+            rpath, ralt = oracle.sssp_ref(o, costs, src, dst, np.arange(o.N), keepout_owner=owner, net_id=1)
+            assert path is not None and rpath is not None
+
+            def ko_cost(p):
+                k, c = 0, 0.0
+                for u, v in zip(p, p[1:]):
+                    e = o.indptr[u] + int(np.nonzero(o.indices[o.indptr[u]:o.indptr[u + 1]] == v)[0][0])
+                    if owner[u] >= 0 and owner[u] != 1 and o.edge_kind[e] == 0:
+                        k += 1
+                    else:
+                        c += float(costs[e])
+                return k, c
+            kc, cc = ko_cost(path)
+            kr, cr = ko_cost(rpath)
+            assert kc == kr, (owner_kind, steps, kc, kr)
+            assert abs(cc - cr) <= 1e-5 * max(cr, 1.0)
+            if owner_kind == "all":
+                assert kc == steps                  # every planar step leaves an owned node

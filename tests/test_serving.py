@@ -158,3 +158,31 @@ def test_cfg5_generator_is_seeded_and_in_range():
     d = workloads.cfg5(n_jobs=4, net_density=0.75)
     for j in d:
         assert len(j.nets) == max(1, min(int(round(0.75 * j.X * j.Y / (2 * j.X))), j.X * j.Y // 2))
+
+
+def test_large_job_gets_fewer_searches_instead_of_rejection():
+    """A job whose footprint at the slot cap exceeds the budget is admitted
+    with as many concurrent searches as the budget holds (the job then runs
+    alone), and is rejected only when one search does not fit."""
+    from orthoroute_amd.serving import fit_slots
+    big = _job(0, X=600, Y=600, L=8, n=2000)
+    full = footprint(big.X, big.Y, big.L, big.policy, len(big.nets), 1024, 1 << 20)
+    budget = full // 3
+    k = fit_slots(big, 1024, budget, 1 << 20)
+    assert 1 <= k < 1024
+    assert footprint(big.X, big.Y, big.L, big.policy, len(big.nets), k, 1 << 20) <= budget
+    assert footprint(big.X, big.Y, big.L, big.policy, len(big.nets), k + 1, 1 << 20) > budget
+    ref = [None]
+    rec = Recorder(ref, {0: 0.02})
+    srv = RoutingServer(hbm_budget=budget, max_active=2, roi_limit=1 << 20, runner=rec)
+    ref[0] = srv
+    res = replay(srv, [big])
+    srv.close()
+    # (choose_slots caps the searches by the device's memory first: one
+    # search without a GPU)
+    assert res[0].error is None and res[0].slots == fit_slots(big, choose_slots(big, 1024), budget, 1 << 20)
+    tiny = RoutingServer(hbm_budget=1 << 20, max_active=1, roi_limit=1 << 20, runner=rec)
+    ref[0] = tiny
+    res = replay(tiny, [big])
+    tiny.close()
+    assert res[0].error is not None and "exceeds" in res[0].error
