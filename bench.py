@@ -85,6 +85,8 @@ def parse():
                     help="two-pin search engine: 0 = bucket rounds over all slots, 1 = one wavefront per "
                          "search, auto = 1 when at least 1024 searches run at once")
     ap.add_argument("--list-cap", type=int, default=0, help="engine 1: entries per bucket list (0 = default)")
+    ap.add_argument("--waves", type=int, default=0, choices=[0, 1, 2, 4],
+                    help="engine 1: wavefronts per search (0 = by the searches in flight)")
     ap.add_argument("--serve-jobs", type=int, default=16, help="serving stream: jobs (0 = skip)")
     ap.add_argument("--serve-max-nodes", type=float, default=4e6, help="serving stream: largest board (nodes)")
     ap.add_argument("--serve-active", type=int, default=4, help="serving slots (jobs on the GPU at once)")
@@ -174,7 +176,7 @@ def main():
     import numpy as np
     import torch
     from orthoroute_amd import workloads
-    from orthoroute_amd.device import max_slots
+    from orthoroute_amd.device import max_slots, solo_waves
     from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig, PathFinderRouter
 
     X, Y, L, reqs = workloads.CONFIGS[args.workload]()
@@ -190,6 +192,7 @@ def main():
     cfg.sssp_rounds_per_poll = args.rounds_per_poll
     cfg.sssp_graph_rounds = args.graph_rounds
     cfg.sssp_engine, cfg.sssp_list_cap = engine_arg(args.engine), args.list_cap
+    cfg.sssp_solo_waves = args.waves
     r = PathFinderRouter(cfg)
     t0 = time.perf_counter()
     r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
@@ -306,10 +309,15 @@ def main():
     # rocprofv3 --pmc profile and is reported only when that profile
     # measured this build's kernel source (the digest it records equals the
     # digest of the source file now)
-    kname, ksrc = ("k_solo", "sssp_solo.hip") if step_engine == 1 else ("k_round", "sssp.hip")
+    # (engine 1 runs k_solo with one wave per search, k_solo_mw<W> with W)
+    kname, kfile, ksrc = "k_round", "k_round", "sssp.hip"
+    if step_engine == 1:
+        w = solo_waves(args.waves, slots_line, len(tasks))
+        kname, kfile, ksrc = ("k_solo", "k_solo", "sssp_solo.hip") if w == 1 else \
+            (f"k_solo_mw<{w}>", f"k_solo_mw{w}", "sssp_solo.hip")
     traffic, traffic_note = None, "no counter profile"
     here = os.path.dirname(os.path.abspath(__file__))
-    prof = os.path.join(here, "profiles", f"{kname}_cfg2_traffic.json")
+    prof = os.path.join(here, "profiles", f"{kfile}_cfg2_traffic.json")
     if os.path.exists(prof):
         import hashlib
         with open(prof) as f:
@@ -319,7 +327,7 @@ def main():
         if tj.get("kernel_source_sha256") == src_digest:
             traffic = tj.get("hbm_bytes_per_launch")
             traffic_note = f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel source ({src_digest[:12]}), " \
-                           f"copy-calibrated: profiles/{kname}_cfg2_traffic.json"
+                           f"copy-calibrated: profiles/{kfile}_cfg2_traffic.json"
         else:
             traffic_note = "the counter profile measured another version of the kernel source: not reported"
 
@@ -341,7 +349,8 @@ def main():
                         f"{r.accounting.E} edges, {len(tasks)} two-pin nets; one step = one full routing "
                         f"pass on the iteration-1 cost snapshot",
             "nodes": r.N, "edges": r.accounting.E, "nets": len(tasks),
-            "engine": step_engine, "slots": slots_line, "delta": sp_line.delta, "n_buckets": sp_line.n_buckets,
+            "engine": step_engine, "waves": solo_waves(args.waves, slots_line, len(tasks)) if step_engine == 1
+            else None, "slots": slots_line, "delta": sp_line.delta, "n_buckets": sp_line.n_buckets,
             "local_batches": args.local_batches or 64, "grid": args.grid or 1024,
             "rounds_per_poll": args.rounds_per_poll,
         },
@@ -461,6 +470,7 @@ def negotiation(args, which, cap, with_plain, trees=False):
         cfg.max_iterations_cap = cap
         cfg.sssp_graph_rounds = args.graph_rounds
         cfg.sssp_engine, cfg.sssp_list_cap = engine_arg(args.engine), args.list_cap
+        cfg.sssp_solo_waves = args.waves
         cfg.multi_terminal_enabled = trees
         r = PathFinderRouter(cfg)
         r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
@@ -631,6 +641,7 @@ def stress(args):
     cfg.batch_slots = slots
     cfg.sssp_graph_rounds = args.graph_rounds
     cfg.sssp_engine, cfg.sssp_list_cap = engine_arg(args.engine), args.list_cap
+    cfg.sssp_solo_waves = args.waves
     r = PathFinderRouter(cfg)
     r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
     tasks = r._parse_requests(reqs)
@@ -691,6 +702,7 @@ def time_to_zero_overuse(args):
         cfg.batch_slots = slots
         cfg.sssp_graph_rounds = args.graph_rounds
         cfg.sssp_engine, cfg.sssp_list_cap = engine_arg(args.engine), args.list_cap
+        cfg.sssp_solo_waves = args.waves
         r = PathFinderRouter(cfg)
         r.initialize_lattice(X, Y, L, pitch=1.0, via_cost=3.0)
         torch.cuda.synchronize()

@@ -138,6 +138,11 @@ typedef struct orgpu_sssp_params {
      * node expanded early is expanded again if its key drops); 0 = 15, < 0 =
      * none, at most 15. */
     int32_t lookahead;
+    /* engine 1: wavefronts that share one search, 1, 2 or 4 (0 = automatic:
+     * 4 when all n_nets searches start at once, n_nets <= slots and
+     * n_nets <= 1024, else 2).  Same paths; more waves per search shorten the
+     * longest searches, which end a call whose slots do not refill. */
+    int32_t waves;
 } orgpu_sssp_params;
 
 typedef struct orgpu_sssp_stats {

@@ -73,7 +73,7 @@ class SsspParams(ctypes.Structure):
                 ("grid", ctypes.c_int32), ("count_minimal", ctypes.c_int32),
                 ("h_xy", ctypes.c_float), ("h_z", ctypes.c_float), ("local_batches", ctypes.c_int32),
                 ("launch_mode", ctypes.c_int32), ("engine", ctypes.c_int32), ("list_cap", ctypes.c_int32),
-                ("lookahead", ctypes.c_int32)]
+                ("lookahead", ctypes.c_int32), ("waves", ctypes.c_int32)]
 
 
 class SsspStats(ctypes.Structure):

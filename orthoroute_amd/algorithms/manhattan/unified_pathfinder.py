@@ -159,6 +159,8 @@ class PathFinderConfig:
     sssp_list_cap = 0              # engine 1: bucket-list entries per search (0 = library default)
     sssp_lookahead = 0             # engine 1: buckets an under-full step looks ahead (0 = library
                                    # default, < 0 = none)
+    sssp_solo_waves = 0            # engine 1: wavefronts per search, 1, 2 or 4 (0 = library default:
+                                   # 4 when a batch's searches all start at once, else 2)
     sssp_target_buckets = None     # widen the bucket so a batch's deep searches span about this many
                                    # (None: 128 from 1 M nodes up, 32 below)
     max_iterations_cap = None
@@ -1214,7 +1216,8 @@ class PathFinderRouter:
                             h_xy=hx, h_z=hz, local_batches=int(self.config.sssp_local_batches or 0),
                             graph_rounds=bool(getattr(self.config, "sssp_graph_rounds", False)),
                             engine=int(engine), list_cap=int(getattr(self.config, "sssp_list_cap", 0) or 0),
-                            lookahead=int(getattr(self.config, "sssp_lookahead", 0) or 0))
+                            lookahead=int(getattr(self.config, "sssp_lookahead", 0) or 0),
+                            waves=int(getattr(self.config, "sssp_solo_waves", 0) or 0))
 
     def _depth_hint(self, ordered, sd):
         """How far past their first bucket the searches of a batch of nets

@@ -133,6 +133,8 @@ extern "C" int orgpu_route_async(orgpu_t *h, const orgpu_graph *g, const orgpu_n
         return orgpu_fail(ORGPU_EINVAL, "route_async: keepout_cost must dominate path costs (>= 1e6)");
     if (params->slots < 1 || params->list_cap < 0)
         return orgpu_fail(ORGPU_EINVAL, "route_async: slots %d / list_cap %d", params->slots, params->list_cap);
+    if (params->waves != 0 && params->waves != 1 && params->waves != 2 && params->waves != 4)
+        return orgpu_fail(ORGPU_EINVAL, "route_async: waves %d is not 0, 1, 2 or 4", params->waves);
     HIP_CHECK_RET(hipSetDevice(h->device));
     return engine_route_async(h->en, g, reqs, n, roi_bitmaps, n_roi, keepout_owner, params, path_nodes, path_edges,
                               capacity, path_off, path_len, path_dist, used);

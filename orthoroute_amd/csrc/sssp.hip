@@ -1320,6 +1320,8 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     }
     if (p->engine != 0 && p->engine != 1) return orgpu_fail(ORGPU_EINVAL, "route: engine %d is not 0 or 1", p->engine);
     if (p->list_cap < 0) return orgpu_fail(ORGPU_EINVAL, "route: list_cap %d < 0", p->list_cap);
+    if (p->waves != 0 && p->waves != 1 && p->waves != 2 && p->waves != 4)
+        return orgpu_fail(ORGPU_EINVAL, "route: waves %d is not 0, 1, 2 or 4", p->waves);
     if (p->engine == 1 && !ms && !p->count_minimal)
         return engine_route_solo(en, g, nets, n_nets, roi_bitmaps, n_roi, keepout_owner, keepout_cost, p, path_nodes,
                                  path_edges, path_capacity, path_offset, path_dist, stats);

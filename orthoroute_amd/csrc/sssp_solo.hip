@@ -196,6 +196,32 @@ struct SoloLists {
             const uint32_t pos = p + (lane - base);
             e = pos < SOLO_CH ? arena[(uint64_t)h * SOLO_CH + pos] : arena[(uint64_t)nx * SOLO_CH + (pos - SOLO_CH)];
         }
+        consume(t, c, n, h, p, cross, nx);
+        return e;
+    }
+    // pop() for up to SOLO_CH entries, for the whole wave: entry k goes to
+    // out[base + k], for k < the count returned
+    __device__ __forceinline__ uint32_t pop_to(uint32_t t, uint32_t lane, uint32_t base, uint32_t maxn,
+                                               uint64_t *out) const
+    {
+        const uint32_t c = cnt[t];
+        const uint32_t n = c < maxn ? c : maxn;
+        const uint32_t h = hc[t], p = hp[t];
+        const bool cross = p + n > SOLO_CH;
+        const uint32_t nx = cross ? link[h] : SOLO_NONE;
+        for (uint32_t k = lane; k < n; k += 64) {
+            const uint32_t pos = p + k;
+            out[base + k] = pos < SOLO_CH ? arena[(uint64_t)h * SOLO_CH + pos]
+                                          : arena[(uint64_t)nx * SOLO_CH + (pos - SOLO_CH)];
+        }
+        consume(t, c, n, h, p, cross, nx);
+        return n;
+    }
+    // list state after n of its c entries were taken from position p of
+    // head chunk h (nx: the next chunk when they cross into it)
+    __device__ __forceinline__ void consume(uint32_t t, uint32_t c, uint32_t n, uint32_t h, uint32_t p, bool cross,
+                                            uint32_t nx) const
+    {
         const uint32_t left = c - n;
         cnt[t] = left;
         if (left == 0u) {
@@ -212,7 +238,6 @@ struct SoloLists {
         } else {
             hp[t] = p + n;
         }
-        return e;
     }
     // Moves every entry of list t to the empty list 0 (their buckets all
     // equal the new current bucket).
@@ -683,6 +708,377 @@ k_solo(SoloArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// W wavefronts per search (W = 2 or 4): the same search as k_solo, for calls
+// with fewer searches in flight than the GPU holds waves (a large board
+// whose keys fit fewer slots, a batch shorter than the slots).  Wave 0 alone
+// owns the bucket lists: it takes up to 64*W entries per step into LDS, and
+// the waves append their pushes one wave at a time.  Every wave expands
+// 64 entries; candidates for one node meet in the workgroup's LDS table,
+// and block barriers separate the table's claim / minimum, its read and its
+// clear.  Every barrier is reached by all W waves: each loop condition is
+// read from LDS by all of them between two barriers.
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ void __launch_bounds__(64 * W)
+k_solo_mw(SoloArgs a)
+{
+    static_assert(W == 2 || W == 4, "2 or 4 waves per search");
+    constexpr uint32_t NT = 64u * W;                 // entries per step (<= SOLO_CH)
+    constexpr uint32_t DD = SOLO_DD * (uint32_t)W;   // >= 2 x the 4*NT candidates of a pass
+    constexpr uint32_t DD_BITS = SOLO_DD_BITS + (W == 2 ? 1 : 2);
+    __shared__ uint32_t s_touch[SOLO_TOUCH_WORDS];
+    __shared__ uint32_t s_hc[SOLO_NL], s_hp[SOLO_NL], s_tc[SOLO_NL], s_tp[SOLO_NL], s_cnt[SOLO_NL], s_al[2];
+    __shared__ uint32_t s_mask[SOLO_LEVELS];
+    __shared__ uint32_t s_ddtag[DD];
+    __shared__ uint64_t s_ddkey[DD];
+    __shared__ uint64_t s_ent[NT];                   // a step's entries
+    __shared__ uint32_t s_ctl[4];                    // [0] request, [1] entries this step, [2] lists ok
+    __shared__ int32_t s_deg[W];                     // per wave: largest degree of a live entry
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t slot = blockIdx.x;
+    uint64_t *keys = a.keys + (uint64_t)slot * (uint64_t)a.NP;
+    SoloLists L{a.arena + (uint64_t)slot * (uint64_t)a.nch * SOLO_CH, a.links + (uint64_t)slot * 2u * a.nch,
+                a.links + (uint64_t)slot * 2u * a.nch + a.nch, a.nch, s_hc, s_hp, s_tc, s_tp, s_cnt, s_al, s_mask};
+    int32_t *stage = a.stage + (uint64_t)slot * (uint64_t)a.max_path;
+    const uint32_t tshift = a.touch_shift;
+    for (uint32_t i = tid; i < SOLO_TOUCH_WORDS; i += NT) s_touch[i] = 0u;
+    for (uint32_t i = tid; i < DD; i += NT) { s_ddtag[i] = 0u; s_ddkey[i] = KEY_INF; }
+    const bool goal = a.h_xy != 0.0f || a.h_z != 0.0f;
+    const uint64_t margin = goal ? 1u : 0u;
+    const bool lat = a.tiled || goal;
+    unsigned long long my_items = 0, my_edges = 0;
+    __syncthreads();
+
+    for (;;) {
+        if (tid == 0) s_ctl[0] = atomicAdd(&a.ctl[0], 1u);
+        __syncthreads();
+        const uint32_t r = s_ctl[0];
+        __syncthreads();                             // read by all before the next request overwrites it
+        if (r >= (uint32_t)a.n_reqs) break;
+        const orgpu_net_req q = a.reqs[r];
+        if (q.src < 0 || (int64_t)q.src >= a.N || q.dst < 0 || (int64_t)q.dst >= a.N || q.roi_index >= a.n_roi ||
+            (q.use_keepout && !a.ko_owner)) {
+            if (tid == 0) {
+                a.out_len[r] = SOLO_INVALID;
+                a.out_off[r] = 0u;
+                a.out_dist[r] = __builtin_inff();
+            }
+            continue;
+        }
+        const uint32_t src = (uint32_t)q.src, dst = (uint32_t)q.dst;
+        int dx = 0, dy = 0, dz = 0;
+        if (goal) {
+            const Cell cd = node_cell(a, dst);
+            dx = cd.x; dy = cd.y; dz = cd.z;
+        }
+        const uint32_t *roi = q.roi_index >= 0 ? a.roi + (uint64_t)q.roi_index * (uint64_t)a.roi_words : nullptr;
+        const bool use_ko = q.use_keepout && a.ko_owner;
+        const uint64_t k0 = key_pack(0, 0.0f, KEY_NOPRED);
+        const uint32_t kdst = key_at(a, dst);
+        uint64_t cur = key_bucket_rel(k0, a.inv_delta, lat ? solo_heur(a, dx, dy, dz, node_cell(a, src)) : 0.0f);
+        if (wave == 0) {
+            L.reset(lane);
+            if (lane == 0) {
+                const uint32_t ks = key_at(a, src);
+                kstore(&keys[ks], k0);
+                atomicOr(&s_touch[(ks >> tshift) >> 5], 1u << ((ks >> tshift) & 31));
+            }
+            const bool okw = lists_append(L, lane == 0, 0u, entry_pack(k0, src));
+            if (lane == 0) s_ctl[2] = okw ? 1u : 0u;
+        }
+        __syncthreads();
+        bool ok = s_ctl[2] != 0u;
+        bool reached = false;
+        uint32_t steps = 0;
+
+        while (ok) {
+            // expand the current bucket to its fixed point
+            for (;;) {
+                const bool more = ok && !L.empty(0u);
+                __syncthreads();                     // all have read the list state
+                if (!more) break;
+                if (++steps > SOLO_MAX_STEPS) { ok = false; break; }
+                if (wave == 0) {
+                    uint32_t n = L.pop_to(0u, lane, 0u, NT, s_ent);
+                    if (n < NT && a.lookahead > 0) {
+                        const uint32_t d0 = (uint32_t)(cur & 15u);
+                        const uint32_t dl = min(15u, d0 + (uint32_t)a.lookahead);
+                        uint32_t mk = L.mask[0] & (((2u << dl) - 1u) & ~((2u << d0) - 1u));
+                        while (mk && n < NT) {
+                            const uint32_t d = (uint32_t)__builtin_ctz(mk);
+                            mk &= mk - 1u;
+                            n += L.pop_to(1u + d, lane, n, NT - n, s_ent);
+                        }
+                    }
+                    if (lane == 0) s_ctl[1] = n;
+                }
+                __syncthreads();                     // the step's entries
+                const uint32_t n = s_ctl[1];
+                bool has = tid < n;
+                const uint64_t e = has ? s_ent[tid] : 0ull;
+                const uint32_t u = (uint32_t)(e & KEY_PRED_MASK);
+                uint64_t ku = 0;
+                int row = 0, row_end = 0;
+                int32_t own = -1;
+                Cell cu = Cell{0, 0, 0};
+                if (has) {
+                    if (lat) cu = node_cell(a, u);
+                    ku = kload(&keys[a.tiled ? tile_index(a, cu) : u]);
+                    row = a.indptr[u];
+                    row_end = a.indptr[u + 1];
+                    if (use_ko) own = a.ko_owner[u];
+                }
+                has = has && key_cost(ku) == key_cost(e);
+                const int deg = has ? row_end - row : 0;
+                if (has) { my_items++; my_edges += (unsigned long long)deg; }
+                const uint32_t ko0 = key_ko(e);
+                const float d0 = key_dist(e);
+                const bool ko_node = has && use_ko && own >= 0 && own != q.net_id;
+                const int wdeg = wave_max_i32(deg);
+                if (lane == 0) s_deg[wave] = wdeg;
+                __syncthreads();
+                int maxdeg = 0;
+#pragma unroll
+                for (int w = 0; w < W; ++w) maxdeg = max(maxdeg, s_deg[w]);
+                for (int k0e = 0; k0e < maxdeg; k0e += 4) {
+                    uint32_t v[4];
+                    float c[4];
+                    bool go[4], pko[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        go[j] = k0e + j < deg;
+                        pko[j] = false;
+                        v[j] = 0u;
+                        c[j] = 0.0f;
+                        if (go[j]) {
+                            v[j] = (uint32_t)a.indices[row + k0e + j];
+                            c[j] = a.cost[row + k0e + j];
+                            if (ko_node) pko[j] = a.kind[row + k0e + j] == 0;
+                        }
+                    }
+                    uint64_t kv[4], nk[4];
+                    uint32_t kix[4];
+                    Cell cv[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        kv[j] = KEY_INF;
+                        nk[j] = KEY_INF;
+                        kix[j] = v[j];
+                        cv[j] = Cell{0, 0, 0};
+                        if (!go[j]) continue;
+                        if (lat) {
+                            cv[j] = node_cell(a, v[j]);
+                            if (a.tiled) kix[j] = tile_index(a, cv[j]);
+                        }
+                        kv[j] = kload(&keys[kix[j]]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (!go[j]) continue;
+                        uint32_t ko = ko0;
+                        float nd = d0;
+                        if (pko[j]) ko = ko_add(ko);
+                        else nd = nd + c[j];
+                        nk[j] = key_pack(ko, nd, u + 1u);
+                        go[j] = nd < __builtin_inff() && nk[j] < kv[j];
+                    }
+                    if (roi) {
+                        uint32_t rw[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) rw[j] = go[j] ? roi[v[j] >> 5] : 0u;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) go[j] = go[j] && ((rw[j] >> (v[j] & 31)) & 1u);
+                    }
+                    // the workgroup's candidates for one node meet in the
+                    // LDS table: claim and minimum, barrier, read, barrier,
+                    // clear and store by the winner
+                    uint32_t pos[4];
+                    bool won[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        pos[j] = 0u;
+                        if (!go[j]) continue;
+                        uint32_t p = (v[j] * 2654435761u) >> (32 - DD_BITS);
+                        for (;;) {
+                            const uint32_t o = atomicCAS(&s_ddtag[p], 0u, v[j] + 1u);
+                            if (o == 0u || o == v[j] + 1u) break;
+                            p = (p + 1u) & (DD - 1u);
+                        }
+                        pos[j] = p;
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (go[j]) atomicMin((unsigned long long *)&s_ddkey[pos[j]], (unsigned long long)nk[j]);
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) won[j] = go[j] && s_ddkey[pos[j]] == nk[j];
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (!won[j]) continue;
+                        s_ddtag[pos[j]] = 0u;
+                        s_ddkey[pos[j]] = KEY_INF;
+                        kstore(&keys[kix[j]], nk[j]);
+                    }
+                    bool push[4];
+                    uint32_t tl4[4];
+                    uint64_t e4[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (won[j] && kv[j] == KEY_INF) {
+                            const uint32_t ch = kix[j] >> tshift;
+                            atomicOr(&s_touch[ch >> 5], 1u << (ch & 31));
+                        }
+                        push[j] = won[j] && key_cost(nk[j]) < key_cost(kv[j]);
+                        tl4[j] = push[j] ? radix_list(key_bucket_rel(nk[j], a.inv_delta,
+                                                                      solo_heur(a, dx, dy, dz, cv[j])), cur) : 0u;
+                        e4[j] = entry_pack(nk[j], v[j]);
+                    }
+                    // the waves append one at a time (the lists' state is
+                    // the workgroup's); the last barrier also orders this
+                    // pass's table clears and key stores before the next
+                    for (int w = 0; w < W; ++w) {
+                        if ((int)wave == w) {
+                            const bool okw = lists_append4(L, push, tl4, e4);
+                            if (!okw && lane == 0) s_ctl[2] = 0u;
+                        }
+                        __syncthreads();
+                    }
+                }
+                ok = s_ctl[2] != 0u;
+            }
+            if (!ok) break;
+            uint64_t next = ~0ull;
+            uint32_t tl = 0u;
+            for (uint32_t i = 0; i < SOLO_LEVELS; ++i) {
+                const uint32_t m = L.mask[i];
+                if (m) {
+                    const uint32_t d = (uint32_t)__builtin_ctz(m);
+                    const uint64_t low = (16ull << (4 * i)) - 1ull;
+                    next = (cur & ~low) | ((uint64_t)d << (4 * i));
+                    tl = 1u + i * 16u + d;
+                    break;
+                }
+            }
+            const uint64_t kd = kload(&keys[kdst]);
+            if (kd != KEY_INF && (next == ~0ull || key_bucket_rel(kd, a.inv_delta) + margin < next)) {
+                reached = true;
+                break;
+            }
+            if (next == ~0ull) break;
+            cur = next;
+            __syncthreads();                         // all have read the list state
+            if (wave == 0) {
+                if (tl <= 16u) {
+                    L.take(tl, lane);
+                } else {
+                    bool okw = true;
+                    while (okw && !L.empty(tl)) {
+                        uint32_t n = 0;
+                        const uint64_t e = L.pop(tl, lane, 0u, 64u, n);
+                        bool has = lane < n;
+                        const uint32_t v = (uint32_t)(e & KEY_PRED_MASK);
+                        const Cell cc = has && lat ? node_cell(a, v) : Cell{0, 0, 0};
+                        const uint64_t kv = has ? kload(&keys[a.tiled ? tile_index(a, cc) : v]) : 0ull;
+                        has = has && key_cost(kv) == key_cost(e);
+                        const uint32_t t = has ? radix_list(key_bucket_rel(kv, a.inv_delta,
+                                                                           solo_heur(a, dx, dy, dz, cc)), cur) : 0u;
+                        okw = lists_append(L, has, t, e) && okw;
+                    }
+                    if (!okw && lane == 0) s_ctl[2] = 0u;
+                }
+            }
+            __syncthreads();
+            ok = s_ctl[2] != 0u;
+        }
+
+        // result (wave 0): walk the predecessors, pack src -> dst
+        if (wave == 0) {
+            int32_t len = 0;
+            float dist = __builtin_inff();
+            if (!ok) {
+                len = SOLO_RETRY;
+            } else if (reached) {
+                const uint64_t kd = kload(&keys[kdst]);
+                dist = key_dist(kd);
+                if (key_ko(kd) >= KEY_KO_MAX) {
+                    len = SOLO_KO_SATURATED;
+                } else if (lane == 0) {
+                    uint32_t v = dst;
+                    for (;;) {
+                        if (len >= a.max_path) { len = SOLO_BROKEN; break; }
+                        stage[len++] = (int32_t)v;
+                        if (v == src) break;
+                        const uint32_t pf = key_pred(kload(&keys[key_at(a, v)]));
+                        if (pf == KEY_NOPRED) { len = SOLO_BROKEN; break; }
+                        v = pf - 1u;
+                    }
+                }
+                len = __shfl(len, 0);
+            }
+            uint32_t off = 0;
+            if (len > 0) {
+                if (lane == 0) off = atomicAdd(&a.ctl[1], (uint32_t)len);
+                off = (uint32_t)__shfl((int)off, 0);
+                if ((uint64_t)off + (uint64_t)len > a.pack_cap) len = SOLO_RETRY;
+            }
+            if (len > 0) {
+                wave_fence();
+                int32_t *on = a.pack_nodes + off;
+                int32_t *oe = a.pack_edges ? a.pack_edges + off : nullptr;
+                for (int t = lane; t < len; t += 64) {
+                    const int32_t un = stage[len - 1 - t];
+                    on[t] = un;
+                    if (!oe) continue;
+                    int ee = -1;
+                    if (t + 1 < len) {
+                        const int32_t vn = stage[len - 2 - t];
+                        const int r0 = a.indptr[un], r1 = a.indptr[un + 1];
+                        for (int e2 = r0; e2 < r1; ++e2)
+                            if (a.indices[e2] == vn) { ee = e2; break; }
+                    }
+                    oe[t] = ee;
+                }
+            }
+            if (lane == 0) {
+                a.out_len[r] = len;
+                a.out_off[r] = off;
+                a.out_dist[r] = dist;
+            }
+        }
+        __syncthreads();                             // the walk has read the keys
+        // reset every key chunk the search touched (the waves split the words)
+        for (uint32_t w0 = wave * 64u; w0 < SOLO_TOUCH_WORDS; w0 += NT) {
+            uint32_t word = s_touch[w0 + lane];
+            uint64_t any = __ballot(word != 0u);
+            while (any) {
+                const int l = __builtin_ctzll(any);
+                uint32_t wl = (uint32_t)__shfl((int)word, l);
+                while (wl) {
+                    const int bit = __builtin_ctz(wl);
+                    wl &= wl - 1u;
+                    const uint64_t lo = ((uint64_t)(w0 + l) * 32u + (uint64_t)bit) << tshift;
+                    const uint64_t hi = min(lo + (1ull << tshift), (uint64_t)a.NP);
+                    for (uint64_t k = lo + lane; k < hi; k += 64) keys[k] = KEY_INF;
+                }
+                any &= any - 1ull;
+            }
+            s_touch[w0 + lane] = 0u;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        my_items += __shfl_xor(my_items, o, 64);
+        my_edges += __shfl_xor(my_edges, o, 64);
+    }
+    if (lane == 0) {
+        atomicAdd(&a.counters[0], my_items);
+        atomicAdd(&a.counters[1], my_edges);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
 
@@ -830,6 +1226,24 @@ static void solo_args(SoloArgs &a, const Engine &en, const orgpu_graph *g, const
     a.counters = (unsigned long long *)(en.solo_ctl + 4);
 }
 
+// wavefronts per search: params->waves, or automatic.  A call whose
+// searches all start at once (n <= slots) ends with its longest search, so
+// up to 1024 of them (as many four-wave workgroups as the GPU holds at once)
+// take four waves each; a call whose slots refill is bound by the memory
+// system's request rate, where two waves per search measured best.
+static int solo_waves(const orgpu_sssp_params *p, int slots, int n)
+{
+    if (p->waves == 1 || p->waves == 2 || p->waves == 4) return p->waves;
+    return n <= slots && n <= 1024 ? 4 : 2;
+}
+
+static void solo_launch(const SoloArgs &a, int grid, int waves, hipStream_t s)
+{
+    if (waves == 4) hipLaunchKernelGGL(k_solo_mw<4>, dim3(grid), dim3(256), 0, s, a);
+    else if (waves == 2) hipLaunchKernelGGL(k_solo_mw<2>, dim3(grid), dim3(128), 0, s, a);
+    else hipLaunchKernelGGL(k_solo, dim3(grid), dim3(64), 0, s, a);
+}
+
 int engine_reserve_solo(Engine &en, int64_t n_nodes, int32_t X, int32_t Y, int32_t L, int32_t slots, int32_t list_cap)
 {
     if (en.cap_slots == 0) return orgpu_fail(ORGPU_EINVAL, "reserve_solo: call orgpu_reserve first");
@@ -868,7 +1282,7 @@ int engine_route_async(Engine &en, const orgpu_graph *g, const orgpu_net_req *d_
     a.pack_nodes = path_nodes; a.pack_edges = path_edges; a.pack_cap = (uint64_t)capacity;
     a.out_len = path_len; a.out_off = path_off; a.out_dist = path_dist;
     hipLaunchKernelGGL(k_solo_begin, dim3(1), dim3(64), 0, s, en.solo_ctl);
-    if (n > 0) hipLaunchKernelGGL(k_solo, dim3(std::min(K, n)), dim3(64), 0, s, a);
+    if (n > 0) solo_launch(a, std::min(K, n), solo_waves(p, K, n), s);
     if (used) hipLaunchKernelGGL(k_solo_used, dim3(1), dim3(64), 0, s, en.solo_ctl, used);
     HIP_CHECK_RET(hipGetLastError());
     return ORGPU_OK;
@@ -912,7 +1326,7 @@ int engine_route_solo(Engine &en, const orgpu_graph *g, const orgpu_net_req *net
     HIP_CHECK_RET(hipMemcpyAsync(en.solo_req, nets, sizeof(orgpu_net_req) * n_nets, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_solo_begin, dim3(1), dim3(64), 0, s, en.solo_ctl);
     HIP_CHECK_RET(hipEventRecord(en.ev_t0, s));
-    hipLaunchKernelGGL(k_solo, dim3(std::min(K, n_nets)), dim3(64), 0, s, a);
+    solo_launch(a, std::min(K, n_nets), solo_waves(p, K, n_nets), s);
     HIP_CHECK_RET(hipGetLastError());
     HIP_CHECK_RET(hipEventRecord(en.ev_r1, s));
     HIP_CHECK_RET(hipMemcpyAsync(en.h_solo_ctl, en.solo_ctl, 64, hipMemcpyDeviceToHost, s));

@@ -232,7 +232,8 @@ class SearchParams:
     runs the plain Dijkstra bucket order.  Paths are the same either way."""
 
     def __init__(self, slots=1, delta=1.0, n_buckets=8, rounds_per_poll=32, grid=0, count_minimal=False,
-                 h_xy=0.0, h_z=0.0, local_batches=0, graph_rounds=False, engine=0, list_cap=0, lookahead=0):
+                 h_xy=0.0, h_z=0.0, local_batches=0, graph_rounds=False, engine=0, list_cap=0, lookahead=0,
+                 waves=0):
         self.slots, self.delta, self.n_buckets = int(slots), float(delta), int(n_buckets)
         self.rounds_per_poll, self.grid, self.count_minimal = int(rounds_per_poll), int(grid), bool(count_minimal)
         self.h_xy, self.h_z = float(h_xy), float(h_z)
@@ -242,6 +243,7 @@ class SearchParams:
         # per search (list_cap: entries per bucket list, 0 = default)
         self.engine, self.list_cap = int(engine), int(list_cap)
         self.lookahead = int(lookahead)   # engine 1: buckets an under-full step looks ahead (0 = 15, < 0 = none)
+        self.waves = int(waves)           # engine 1: wavefronts per search, 1, 2 or 4 (0 = by searches in flight)
 
     def c(self):
         p = _abi.SsspParams()
@@ -250,7 +252,7 @@ class SearchParams:
         p.h_xy, p.h_z = self.h_xy, self.h_z
         p.local_batches = self.local_batches
         p.launch_mode = int(self.graph_rounds)
-        p.engine, p.list_cap, p.lookahead = self.engine, self.list_cap, self.lookahead
+        p.engine, p.list_cap, p.lookahead, p.waves = self.engine, self.list_cap, self.lookahead, self.waves
         return p
 
 
@@ -369,6 +371,15 @@ def route_flat(ctx, lat, cost, requests, params, roi_bitmaps=None, keepout_owner
              "kernel_ms": st.kernel_ms, "launches": st.launches, "round_ms": st.round_ms,
              "retried": st.retried}
     return nodes, edges, offs, dist[:n], stats
+
+
+def solo_waves(waves, slots, n):
+    """Wavefronts per search the one-wave engine runs n searches with on
+    `slots` slots, as orgpu_route picks them: params.waves when it is 1, 2
+    or 4; else 4 when all n start at once and n <= 1024, otherwise 2."""
+    if int(waves) in (1, 2, 4):
+        return int(waves)
+    return 4 if n <= slots and n <= 1024 else 2
 
 
 def reserve_solo(ctx, lat, slots, list_cap=0):

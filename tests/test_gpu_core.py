@@ -69,7 +69,7 @@ def make_costs(o, rng, kind="jitter"):
 
 
 def run_case(ctx, X, Y, L, policy, slots, n_nets, seed, cost_kind, roi_frac=0.5, keepout=False, dn=None,
-             goal=False, graph=False, engine=0, list_cap=0, lookahead=0):
+             goal=False, graph=False, engine=0, list_cap=0, lookahead=0, waves=0):
     import torch
     from orthoroute_amd.device import SearchParams, cost_bounds, route
     say(f"[route] {policy} {X}x{Y}x{L} slots={slots} nets={n_nets} seed={seed} costs={cost_kind} keepout={keepout}")
@@ -102,9 +102,9 @@ def run_case(ctx, X, Y, L, policy, slots, n_nets, seed, cost_kind, roi_frac=0.5,
     if dn is None:
         dn = (float(rng.choice([0.3, 1.0, 4.0])), int(rng.choice([1, 4, 8])))
     hx, hz = cost_bounds(ctx, g, cost_d) if goal else (0.0, 0.0)
-    say(f"[route]   delta={dn[0]} n_buckets={dn[1]} h=({hx}, {hz}) graph={graph} engine={engine}")
+    say(f"[route]   delta={dn[0]} n_buckets={dn[1]} h=({hx}, {hz}) graph={graph} engine={engine} waves={waves}")
     p = SearchParams(slots=slots, delta=dn[0], n_buckets=dn[1], rounds_per_poll=16, h_xy=hx, h_z=hz,
-                     graph_rounds=graph, engine=engine, list_cap=list_cap, lookahead=lookahead)
+                     graph_rounds=graph, engine=engine, list_cap=list_cap, lookahead=lookahead, waves=waves)
     res, stats = route(ctx, g, cost_d, reqs, p, roi_bitmaps=roi_d, keepout_owner=own_d)
     for i, (s, d, nid, roi_idx, ko) in enumerate(reqs):
         path, dist, pred, _, _ = oracle.sssp_canon(o, c, s, d, roi_mask=masks[i],

@@ -1,8 +1,8 @@
-"""GPU parity of the one-wavefront-per-search engine (orgpu_sssp_params.engine
-= 1): paths, first-match edge ids and FP32 distances bit-equal to the
-canonical oracle (sssp_canon) and to the bucket-round engine, including ROIs,
-keepouts, congested costs, capacity hand-back to the bucket engine and the
-edge cases of the two-pin search."""
+"""GPU parity of the one-wave-per-search engine (orgpu_sssp_params.engine
+= 1), with one, two and four wavefronts per search: paths, first-match edge
+ids and FP32 distances bit-equal to the canonical oracle (sssp_canon) and to
+the bucket-round engine, including ROIs, keepouts, congested costs, capacity
+hand-back to the bucket engine and the edge cases of the two-pin search."""
 import sys
 
 import numpy as np
@@ -25,38 +25,48 @@ def ctx():
     return Context(0)
 
 
+WAVES = [1, 2, 4]
+
+
+@pytest.mark.parametrize("waves", WAVES)
 @pytest.mark.parametrize("goal", [False, True], ids=["dijkstra", "goal"])
 @pytest.mark.parametrize("policy", ["reference", "adjacent"])
 @pytest.mark.parametrize("slots", [1, 7, 64])
-def test_solo_matches_oracle(ctx, policy, slots, goal):
+def test_solo_matches_oracle(ctx, policy, slots, goal, waves):
     for seed, kind in [(1, "jitter"), (2, "ties"), (3, "random"), (4, "congested")]:
-        st = run_case(ctx, 24, 20, 6, policy, slots, 24, seed + 10 * slots, kind, goal=goal, engine=1)
+        st = run_case(ctx, 24, 20, 6, policy, slots, 24, seed + 10 * slots, kind, goal=goal, engine=1,
+                      waves=waves)
         assert st["retried"] == 0
 
 
+@pytest.mark.parametrize("waves", WAVES)
 @pytest.mark.parametrize("goal", [False, True], ids=["dijkstra", "goal"])
-def test_solo_keepout_matches_oracle(ctx, goal):
+def test_solo_keepout_matches_oracle(ctx, goal, waves):
     for seed, kind in [(21, "jitter"), (22, "ties"), (23, "congested")]:
-        run_case(ctx, 20, 16, 6, "reference", 16, 30, seed, kind, keepout=True, goal=goal, engine=1)
+        run_case(ctx, 20, 16, 6, "reference", 16, 30, seed, kind, keepout=True, goal=goal, engine=1, waves=waves)
 
 
+@pytest.mark.parametrize("waves", [1, 4])
 @pytest.mark.parametrize("lookahead", [-1, 0, 15], ids=["no-lookahead", "default", "lookahead15"])
 @pytest.mark.parametrize("dn", [(0.05, 1), (0.3, 8), (4.0, 32), (100.0, 8)], ids=["narrow", "mid", "wide", "one"])
-def test_solo_bucket_widths(ctx, dn, lookahead):
+def test_solo_bucket_widths(ctx, dn, lookahead, waves):
     """Bucket widths from far below one edge cost (many radix-list moves) to
     wider than every path (one bucket), with and without early expansion of
     the next buckets' entries: the same paths."""
-    run_case(ctx, 30, 24, 6, "adjacent", 16, 40, 31, "congested", dn=dn, goal=True, engine=1, lookahead=lookahead)
-    run_case(ctx, 30, 24, 6, "adjacent", 16, 40, 32, "jitter", dn=dn, goal=False, engine=1, lookahead=lookahead)
+    run_case(ctx, 30, 24, 6, "adjacent", 16, 40, 31, "congested", dn=dn, goal=True, engine=1, lookahead=lookahead,
+             waves=waves)
+    run_case(ctx, 30, 24, 6, "adjacent", 16, 40, 32, "jitter", dn=dn, goal=False, engine=1, lookahead=lookahead,
+             waves=waves)
 
 
-def test_solo_capacity_hands_back(ctx):
+@pytest.mark.parametrize("waves", WAVES)
+def test_solo_capacity_hands_back(ctx, waves):
     """Bucket-list storage of two 256-entry chunks per search runs out on most
     searches of this board: those searches are routed by the bucket engine and
     every path still equals the oracle's."""
     st = run_case(ctx, 40, 32, 6, "adjacent", 16, 32, 41, "jitter", roi_frac=0.0, dn=(0.3, 8), goal=False,
-                  engine=1, list_cap=1)
-    say(f"[solo] retried {st['retried']} of 32")
+                  engine=1, list_cap=1, waves=waves)
+    say(f"[solo] waves {waves}: retried {st['retried']} of 32")
     assert st["retried"] > 0
 
 
@@ -85,17 +95,19 @@ def test_solo_equals_bucket_engine(ctx):
     roi_d = torch.from_numpy(np.stack(rois)).to(ctx.device)
     hx, hz = cost_bounds(ctx, g, cost_d)
     out = []
-    for eng in (0, 1):
-        p = SearchParams(slots=128, delta=1.0, n_buckets=8, h_xy=hx, h_z=hz, engine=eng)
+    for eng, waves in ((0, 0), (1, 1), (1, 2), (1, 4)):
+        p = SearchParams(slots=128, delta=1.0, n_buckets=8, h_xy=hx, h_z=hz, engine=eng, waves=waves)
         nodes, edges, offs, dist, st = route_flat(ctx, g, cost_d, reqs, p, roi_bitmaps=roi_d)
-        say(f"[solo] engine {eng}: {st['kernel_ms']:.2f} ms, {st['items']} items, {st['edges']} edges, "
-            f"retried {st['retried']}")
+        say(f"[solo] engine {eng} waves {waves}: {st['kernel_ms']:.2f} ms, {st['items']} items, "
+            f"{st['edges']} edges, retried {st['retried']}")
         out.append((nodes, edges, offs, dist))
-    for a, b in zip(out[0], out[1]):
-        np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
+    for o2 in out[1:]:
+        for a, b in zip(out[0], o2):
+            np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
 
 
-def test_solo_edge_cases(ctx):
+@pytest.mark.parametrize("waves", [1, 4])
+def test_solo_edge_cases(ctx, waves):
     """Source equal to destination (a one-node path), a destination outside
     the ROI (no path), an empty request list, and a call after a bucket-engine
     call on the same handle (pending touch records are flushed first)."""
@@ -107,7 +119,7 @@ def test_solo_edge_cases(ctx):
     rng = np.random.default_rng(61)
     c = make_costs(o, rng, "jitter")
     cost_d = torch.from_numpy(c).to(ctx.device)
-    p1 = SearchParams(slots=4, delta=1.0, n_buckets=8, engine=1)
+    p1 = SearchParams(slots=4, delta=1.0, n_buckets=8, engine=1, waves=waves)
     nodes, edges, offs, dist, st = route_flat(ctx, g, cost_d, [], p1)
     assert len(offs) == 1 and offs[0] == 0
     mask = np.zeros(o.N, np.uint8)
@@ -132,8 +144,8 @@ def test_solo_edge_cases(ctx):
 
 def test_route_async_matches_route():
     """orgpu_route_async (device requests and results, nothing allocated,
-    nothing waited for) returns the paths, edge ids and distances of
-    orgpu_route on the one-wave engine; invalid requests come back as -4, an
+    nothing waited for; four waves per search) returns the paths, edge ids
+    and distances of orgpu_route on the one-wave engine with one wave each; invalid requests come back as -4, an
     output too small as -3 (given back), and a call before
     orgpu_reserve_solo fails."""
     import torch
@@ -163,13 +175,14 @@ def test_route_async_matches_route():
         reqs.append((s, d, i % 6, roi_idx, int(i % 7 == 0)))
     roi_d = torch.from_numpy(np.stack(rois)).to(ctx2.device)
     hx, hz = cost_bounds(ctx2, g, cost_d)
-    p = SearchParams(slots=64, delta=0.5, n_buckets=8, h_xy=hx, h_z=hz, engine=1)
+    p = SearchParams(slots=64, delta=0.5, n_buckets=8, h_xy=hx, h_z=hz, engine=1, waves=4)
+    p1 = SearchParams(slots=64, delta=0.5, n_buckets=8, h_xy=hx, h_z=hz, engine=1, waves=1)
     out = route_async_outputs(ctx2, len(reqs), 300 * 400)
     req_d = torch.from_numpy(np.concatenate([np.asarray(reqs, np.int32), np.zeros((len(reqs), 1), np.int32)], 1)).to(
         ctx2.device)
     with pytest.raises(OrgpuError):
         route_async(ctx2, g, cost_d, req_d, p, out, roi_bitmaps=roi_d, keepout_owner=own_d)
-    nodes, edges, offs, dist, st = route_flat(ctx2, g, cost_d, reqs, p, roi_bitmaps=roi_d, keepout_owner=own_d)
+    nodes, edges, offs, dist, st = route_flat(ctx2, g, cost_d, reqs, p1, roi_bitmaps=roi_d, keepout_owner=own_d)
     reserve_solo(ctx2, g, 64)
     for call in range(2):
         n = len(reqs) if call == 0 else 120     # a second, smaller call on the same reserve
@@ -202,3 +215,15 @@ def test_route_async_matches_route():
     sl = small["len"].cpu().numpy()[:len(reqs)]
     assert (sl == -3).sum() > 0 and sl[sl > 0].sum() <= 40
     ctx2.close()
+
+
+def test_solo_waves_rejected(ctx):
+    """A wavefront count other than 0, 1, 2 or 4 is an argument error."""
+    import torch
+    from orthoroute_amd._abi import OrgpuError
+    from orthoroute_amd.device import SearchParams, route_flat
+    o = oracle.Lattice(8, 8, 2, "reference")
+    g = ctx.build_lattice(8, 8, 2, "reference")
+    cost_d = torch.from_numpy(make_costs(o, np.random.default_rng(3), "jitter")).to(ctx.device)
+    with pytest.raises(OrgpuError):
+        route_flat(ctx, g, cost_d, [(0, 9, 0, -1, 0)], SearchParams(slots=2, engine=1, waves=3))

@@ -87,3 +87,17 @@ def test_radix_lists_only_move_down():
         new = (cur & ~low) | (d << (4 * i))
         assert new <= b
         assert radix_list(b, new) < t
+
+
+def test_wave_count_rule():
+    """Wavefronts per search: an explicit 1, 2 or 4 is kept; otherwise four
+    when every search of the call starts at once (n <= slots, n <= 1024) and
+    two when slots refill or more than 1024 run at once."""
+    from orthoroute_amd.device import solo_waves
+    assert [solo_waves(w, 64, 300) for w in (1, 2, 4)] == [1, 2, 4]
+    assert solo_waves(0, 1024, 1024) == 4
+    assert solo_waves(0, 2048, 1024) == 4
+    assert solo_waves(0, 2048, 1025) == 2
+    assert solo_waves(0, 1024, 40000) == 2
+    assert solo_waves(0, 64, 65) == 2
+    assert solo_waves(0, 64, 1) == 4
