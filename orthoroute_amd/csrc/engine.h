@@ -94,6 +94,8 @@ struct Engine {
     int32_t *solo_out = nullptr, *h_solo_out = nullptr;
     int32_t solo_req_cap = 0;
     uint32_t *solo_ctl = nullptr, *h_solo_ctl = nullptr;
+    uint4 *solo_recs = nullptr;       // adjacency records, two uint4 per key index
+    int64_t solo_recs_cap = 0;        // key indices they cover
 };
 
 struct orgpu_ctx {

@@ -92,6 +92,7 @@ struct SoloArgs {
     uint32_t *out_off;
     float *out_dist;
     unsigned long long *counters;  // [0] items expanded, [1] edges scanned
+    const uint4 *recs;             // [NP][2] adjacency records in key order (null: CSR rows only)
 };
 
 __device__ __forceinline__ uint64_t kload(const uint64_t *p)
@@ -367,6 +368,118 @@ __device__ __forceinline__ float solo_heur(const SoloArgs &a, int dx, int dy, in
     return a.h_xy * (float)(abs(c.x - dx) + abs(c.y - dy)) + a.h_z * (float)abs(c.z - dz);
 }
 
+// Adjacency records.  On a lattice most nodes have at most six edges whose
+// targets are u -+ 1, u -+ X or u + dz * plane; such a node's row is kept,
+// in the keys' tile order, as one 32-byte record: six FP32 costs, six
+// one-byte target codes, the degree and a bit per planar edge (edge kind 0,
+// for keepouts), so an expanded node costs one cache line instead of its row
+// bounds, target indices and costs.  A node with another degree or target
+// keeps its CSR row (degree byte SOLO_REC_CSR).  Records are written from
+// the call's CSR and costs before each search launch.
+#define SOLO_REC_MAXDEG 6
+#define SOLO_REC_CSR 0xFFu
+// target codes: 1..4 = u - X, u - 1, u + 1, u + X; 0x80 | dz = u + dz * plane
+// (dz a signed 7-bit layer step)
+__device__ __forceinline__ uint32_t rec_code(int64_t dv, int64_t X, int64_t plane)
+{
+    if (dv == -X) return 1u;
+    if (dv == -1) return 2u;
+    if (dv == 1) return 3u;
+    if (dv == X) return 4u;
+    if (dv != 0 && dv % plane == 0 && dv / plane >= -63 && dv / plane <= 63)
+        return 0x80u | ((uint32_t)(dv / plane) & 0x7Fu);
+    return 0u;
+}
+
+__device__ __forceinline__ uint32_t rec_target(uint32_t u, uint32_t code, uint32_t X, uint32_t plane)
+{
+    if (code & 0x80u) {
+        const int dz = (int)(code << 25) >> 25;
+        return (uint32_t)((int)u + dz * (int)plane);
+    }
+    return code == 1u ? u - X : code == 2u ? u - 1u : code == 3u ? u + 1u : u + X;
+}
+
+// one thread per node; ctl[2] counts the nodes left on their CSR rows
+__global__ void __launch_bounds__(256)
+k_solo_adjacency(SoloArgs a, uint4 *recs)
+{
+    uint32_t n_csr = 0;
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < a.N; n += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t u = (uint32_t)n;
+        const uint32_t t = tile_index(a, node_cell(a, u));
+        const int row = a.indptr[u];
+        const int deg = a.indptr[u + 1] - row;
+        uint32_t cw[SOLO_REC_MAXDEG] = {0u, 0u, 0u, 0u, 0u, 0u};
+        uint32_t code = 0u, kmask = 0u;
+        bool fits = deg <= SOLO_REC_MAXDEG;
+        for (int k = 0; fits && k < deg; ++k) {
+            const uint32_t ck = rec_code((int64_t)a.indices[row + k] - (int64_t)u, a.X, a.plane);
+            if (ck == 0u) { fits = false; break; }
+            cw[k] = __float_as_uint(a.cost[row + k]);
+            if (k < 4) code |= ck << (8 * k);
+            else kmask |= ck << (8 * (k - 4));
+            if (a.kind[row + k] == 0) kmask |= 1u << (24 + k);
+        }
+        uint4 r0, r1;
+        if (fits) {
+            r0 = make_uint4(cw[0], cw[1], cw[2], cw[3]);
+            r1 = make_uint4(cw[4], cw[5], code, kmask | ((uint32_t)deg << 16));
+        } else {
+            r0 = make_uint4(0u, 0u, 0u, 0u);
+            r1 = make_uint4(0u, 0u, 0u, SOLO_REC_CSR << 16);
+            ++n_csr;
+        }
+        recs[2ull * t] = r0;
+        recs[2ull * t + 1] = r1;
+    }
+    if (n_csr) atomicAdd(&a.ctl[2], n_csr);
+}
+
+// The adjacency of expanded node u (key index kidx): its record, or its CSR
+// row (row, deg) when it has none or records are off for the launch
+__device__ __forceinline__ void adj_load(const SoloArgs &a, bool use_rec, uint32_t u, uint32_t kidx, uint4 &r0,
+                                         uint4 &r1, int &row, int &deg, bool &csr)
+{
+    csr = true;
+    if (use_rec) {
+        const uint4 *p = a.recs + 2ull * kidx;
+        r0 = p[0];
+        r1 = p[1];
+        const uint32_t d = (r1.w >> 16) & 0xFFu;
+        csr = d == SOLO_REC_CSR;
+        deg = (int)d;
+    }
+    if (csr) {
+        row = a.indptr[u];
+        deg = a.indptr[u + 1] - row;
+    }
+}
+
+__device__ __forceinline__ uint32_t u4_at(const uint4 &r, int j)
+{
+    return j == 0 ? r.x : j == 1 ? r.y : j == 2 ? r.z : r.w;
+}
+
+// edge k0e + j of the node (j a constant of the unrolled caller; a record
+// has at most six edges, so k0e is 0 or 4 there): target, cost, and
+// whether it is a planar edge (kind 0) when keepouts are counted
+__device__ __forceinline__ void adj_edge(const SoloArgs &a, bool csr, const uint4 &r0, const uint4 &r1, int row,
+                                         uint32_t u, int k0e, int j, bool ko, uint32_t &v, float &c, bool &planar)
+{
+    if (csr) {
+        v = (uint32_t)a.indices[row + k0e + j];
+        c = a.cost[row + k0e + j];
+        planar = ko && a.kind[row + k0e + j] == 0;
+        return;
+    }
+    const bool lo = k0e == 0;
+    c = __uint_as_float(lo ? u4_at(r0, j) : u4_at(r1, j & 1));
+    const uint32_t code = ((lo ? r1.z : r1.w) >> (8 * (lo ? j : (j & 1)))) & 0xFFu;
+    v = rec_target(u, code, (uint32_t)a.X, (uint32_t)a.plane);
+    planar = ko && ((r1.w >> (24 + k0e + j)) & 1u);
+}
+
 __global__ void __launch_bounds__(64)
 k_solo(SoloArgs a)
 {
@@ -386,6 +499,8 @@ k_solo(SoloArgs a)
     for (int i = lane; i < SOLO_DD; i += 64) { s_ddtag[i] = 0u; s_ddkey[i] = KEY_INF; }
     const bool goal = a.h_xy != 0.0f || a.h_z != 0.0f;
     const uint64_t margin = goal ? 1u : 0u;
+    // records, unless more than one node in eight keeps its CSR row
+    const bool use_rec = a.recs && a.tiled && (uint64_t)a.ctl[2] * 8u <= (uint64_t)a.N;
     unsigned long long my_items = 0, my_edges = 0;
 
     for (;;) {
@@ -459,19 +574,21 @@ k_solo(SoloArgs a)
                 bool has = lane < n;
                 const uint32_t u = (uint32_t)(e & KEY_PRED_MASK);
                 uint64_t ku = 0;
-                int row = 0, row_end = 0;
+                int row = 0, deg = 0;
+                bool csr = true;
+                uint4 r0 = make_uint4(0u, 0u, 0u, 0u), r1 = r0;
                 int32_t own = -1;
                 Cell cu = Cell{0, 0, 0};
                 if (has) {
                     if (lat) cu = node_cell(a, u);
-                    ku = kload(&keys[a.tiled ? tile_index(a, cu) : u]);
-                    row = a.indptr[u];
-                    row_end = a.indptr[u + 1];
+                    const uint32_t ki = a.tiled ? tile_index(a, cu) : u;
+                    ku = kload(&keys[ki]);
+                    adj_load(a, use_rec, u, ki, r0, r1, row, deg, csr);
                     if (use_ko) own = a.ko_owner[u];
                 }
                 // stale: a cheaper entry of the node was pushed since
                 has = has && key_cost(ku) == key_cost(e);
-                const int deg = has ? row_end - row : 0;
+                if (!has) deg = 0;
                 if (has) { my_items++; my_edges += (unsigned long long)deg; }
                 const uint32_t ko0 = key_ko(e);
                 const float d0 = key_dist(e);
@@ -487,11 +604,7 @@ k_solo(SoloArgs a)
                         pko[j] = false;
                         v[j] = 0u;
                         c[j] = 0.0f;
-                        if (go[j]) {
-                            v[j] = (uint32_t)a.indices[row + k0e + j];
-                            c[j] = a.cost[row + k0e + j];
-                            if (ko_node) pko[j] = a.kind[row + k0e + j] == 0;
-                        }
+                        if (go[j]) adj_edge(a, csr, r0, r1, row, u, k0e, j, ko_node, v[j], c[j], pko[j]);
                     }
                     // target keys first; the ROI bit is read only for a
                     // candidate that would improve its target (an edge to a
@@ -746,6 +859,7 @@ k_solo_mw(SoloArgs a)
     const bool goal = a.h_xy != 0.0f || a.h_z != 0.0f;
     const uint64_t margin = goal ? 1u : 0u;
     const bool lat = a.tiled || goal;
+    const bool use_rec = a.recs && a.tiled && (uint64_t)a.ctl[2] * 8u <= (uint64_t)a.N;
     unsigned long long my_items = 0, my_edges = 0;
     __syncthreads();
 
@@ -818,18 +932,20 @@ k_solo_mw(SoloArgs a)
                 const uint64_t e = has ? s_ent[tid] : 0ull;
                 const uint32_t u = (uint32_t)(e & KEY_PRED_MASK);
                 uint64_t ku = 0;
-                int row = 0, row_end = 0;
+                int row = 0, deg = 0;
+                bool csr = true;
+                uint4 r0 = make_uint4(0u, 0u, 0u, 0u), r1 = r0;
                 int32_t own = -1;
                 Cell cu = Cell{0, 0, 0};
                 if (has) {
                     if (lat) cu = node_cell(a, u);
-                    ku = kload(&keys[a.tiled ? tile_index(a, cu) : u]);
-                    row = a.indptr[u];
-                    row_end = a.indptr[u + 1];
+                    const uint32_t ki = a.tiled ? tile_index(a, cu) : u;
+                    ku = kload(&keys[ki]);
+                    adj_load(a, use_rec, u, ki, r0, r1, row, deg, csr);
                     if (use_ko) own = a.ko_owner[u];
                 }
                 has = has && key_cost(ku) == key_cost(e);
-                const int deg = has ? row_end - row : 0;
+                if (!has) deg = 0;
                 if (has) { my_items++; my_edges += (unsigned long long)deg; }
                 const uint32_t ko0 = key_ko(e);
                 const float d0 = key_dist(e);
@@ -850,11 +966,7 @@ k_solo_mw(SoloArgs a)
                         pko[j] = false;
                         v[j] = 0u;
                         c[j] = 0.0f;
-                        if (go[j]) {
-                            v[j] = (uint32_t)a.indices[row + k0e + j];
-                            c[j] = a.cost[row + k0e + j];
-                            if (ko_node) pko[j] = a.kind[row + k0e + j] == 0;
-                        }
+                        if (go[j]) adj_edge(a, csr, r0, r1, row, u, k0e, j, ko_node, v[j], c[j], pko[j]);
                     }
                     uint64_t kv[4], nk[4];
                     uint32_t kix[4];
@@ -1167,6 +1279,18 @@ static int solo_reserve_keys(Engine &en, int32_t slots, int64_t NP)
     return ORGPU_OK;
 }
 
+// adjacency records for a key stride of NP
+static int solo_reserve_recs(Engine &en, int64_t NP)
+{
+    if (en.solo_recs_cap >= NP) return ORGPU_OK;
+    if (en.solo_recs) hipFree(en.solo_recs);
+    en.solo_recs = nullptr;
+    en.solo_recs_cap = 0;
+    HIP_CHECK_RET(hipMalloc((void **)&en.solo_recs, (size_t)NP * 2 * sizeof(uint4)));
+    en.solo_recs_cap = NP;
+    return ORGPU_OK;
+}
+
 static int solo_reserve_requests(Engine &en, int32_t n_reqs)
 {
     if (en.solo_req_cap >= n_reqs) return ORGPU_OK;
@@ -1191,6 +1315,8 @@ void solo_release(Engine &en)
     if (en.h_solo_out) hipHostFree(en.h_solo_out);
     if (en.solo_ctl) hipFree(en.solo_ctl);
     if (en.h_solo_ctl) hipHostFree(en.h_solo_ctl);
+    if (en.solo_recs) hipFree(en.solo_recs);
+    en.solo_recs = nullptr; en.solo_recs_cap = 0;
     en.solo_arena = nullptr; en.solo_links = nullptr; en.solo_req = nullptr; en.solo_out = nullptr;
     en.h_solo_out = nullptr;
     en.solo_ctl = nullptr; en.h_solo_ctl = nullptr;
@@ -1224,6 +1350,16 @@ static void solo_args(SoloArgs &a, const Engine &en, const orgpu_graph *g, const
     a.touch_shift = lay.tshift;
     a.stage = en.stage_nodes; a.max_path = en.cap_path;
     a.counters = (unsigned long long *)(en.solo_ctl + 4);
+    a.recs = lay.tiled && g->edge_kind && en.solo_recs && en.solo_recs_cap >= lay.NP ? en.solo_recs : nullptr;
+}
+
+// writes the call's adjacency records (after k_solo_begin, before the search)
+static void solo_build_recs(const SoloArgs &a, hipStream_t s)
+{
+    if (!a.recs) return;
+    int64_t blocks = (a.N + 255) / 256;
+    if (blocks > 65535 * 8) blocks = 65535 * 8;
+    hipLaunchKernelGGL(k_solo_adjacency, dim3((unsigned)blocks), dim3(256), 0, s, a, const_cast<uint4 *>(a.recs));
 }
 
 // wavefronts per search: params->waves, or automatic.  A call whose
@@ -1256,6 +1392,8 @@ int engine_reserve_solo(Engine &en, int64_t n_nodes, int32_t X, int32_t Y, int32
     if (rc) return rc;
     rc = solo_reserve_arena(en, slots, solo_chunks(list_cap));
     if (rc) return rc;
+    rc = solo_reserve_recs(en, lay.NP);
+    if (rc) return rc;
     return engine_prepare_keys(en, n_nodes);
 }
 
@@ -1282,7 +1420,10 @@ int engine_route_async(Engine &en, const orgpu_graph *g, const orgpu_net_req *d_
     a.pack_nodes = path_nodes; a.pack_edges = path_edges; a.pack_cap = (uint64_t)capacity;
     a.out_len = path_len; a.out_off = path_off; a.out_dist = path_dist;
     hipLaunchKernelGGL(k_solo_begin, dim3(1), dim3(64), 0, s, en.solo_ctl);
-    if (n > 0) solo_launch(a, std::min(K, n), solo_waves(p, K, n), s);
+    if (n > 0) {
+        solo_build_recs(a, s);
+        solo_launch(a, std::min(K, n), solo_waves(p, K, n), s);
+    }
     if (used) hipLaunchKernelGGL(k_solo_used, dim3(1), dim3(64), 0, s, en.solo_ctl, used);
     HIP_CHECK_RET(hipGetLastError());
     return ORGPU_OK;
@@ -1310,6 +1451,10 @@ int engine_route_solo(Engine &en, const orgpu_graph *g, const orgpu_net_req *net
     if (rc) return rc;
     rc = solo_reserve_requests(en, n_nets);
     if (rc) return rc;
+    if (lay.tiled) {
+        rc = solo_reserve_recs(en, lay.NP);
+        if (rc) return rc;
+    }
     rc = engine_prepare_keys(en, N);
     if (rc) return rc;
     const int64_t cap = en.solo_req_cap;
@@ -1325,6 +1470,7 @@ int engine_route_solo(Engine &en, const orgpu_graph *g, const orgpu_net_req *net
 
     HIP_CHECK_RET(hipMemcpyAsync(en.solo_req, nets, sizeof(orgpu_net_req) * n_nets, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_solo_begin, dim3(1), dim3(64), 0, s, en.solo_ctl);
+    solo_build_recs(a, s);
     HIP_CHECK_RET(hipEventRecord(en.ev_t0, s));
     solo_launch(a, std::min(K, n_nets), solo_waves(p, K, n_nets), s);
     HIP_CHECK_RET(hipGetLastError());

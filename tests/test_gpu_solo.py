@@ -227,3 +227,54 @@ def test_solo_waves_rejected(ctx):
     cost_d = torch.from_numpy(make_costs(o, np.random.default_rng(3), "jitter")).to(ctx.device)
     with pytest.raises(OrgpuError):
         route_flat(ctx, g, cost_d, [(0, 9, 0, -1, 0)], SearchParams(slots=2, engine=1, waves=3))
+
+
+@pytest.mark.parametrize("frac", [0.05, 0.3], ids=["mixed", "csr-launch"])
+@pytest.mark.parametrize("waves", WAVES)
+def test_solo_adjacency_records_on_irregular_graph(ctx, waves, frac):
+    """A lattice CSR with extra edges on a fraction of its nodes (a target
+    off the lattice pattern, or more than six edges): those nodes keep their
+    CSR rows while the others read adjacency records (5 %), or the launch
+    reads every row from the CSR (30 %, above one node in eight).  Paths,
+    first-match edge ids and distances equal the oracle's on that graph."""
+    import types
+    import torch
+    from orthoroute_amd.device import DeviceGraph, SearchParams, route
+    X, Y, L = 30, 24, 6
+    o = oracle.Lattice(X, Y, L, "adjacent")
+    rng = np.random.default_rng(81 + int(100 * frac))
+    plane, N = X * Y, o.N
+    rows = [list(o.indices[o.indptr[u]:o.indptr[u + 1]]) for u in range(N)]
+    pick = rng.random(N) < frac
+    for u in np.nonzero(pick)[0].tolist():
+        if u % 2:
+            rows[u].append(int(rng.integers(0, N)))                     # off the lattice pattern
+        else:
+            z = u // plane
+            rows[u] += [((z + k) % L) * plane + u % plane for k in (1, 2, 3, 4, 5) if (z + k) % L != z]
+    indptr = np.zeros(N + 1, np.int32)
+    indptr[1:] = np.cumsum([len(r) for r in rows])
+    indices = np.concatenate([np.asarray(r, np.int32) for r in rows])
+    src_z = np.repeat(np.arange(N) // plane, np.diff(indptr))
+    kind = (src_z != indices // plane).astype(np.uint8)
+    og = types.SimpleNamespace(N=N, indptr=indptr, indices=indices, edge_kind=kind)
+    c = (1.0 + rng.random(len(indices)) * 3.0).astype(np.float32)
+    g = DeviceGraph(ctx, indptr, indices, shape=(X, Y, L))
+    cost_d = torch.from_numpy(c).to(ctx.device)
+    reqs = []
+    for i in range(40):
+        s, d = (int(v) for v in rng.integers(0, N, 2))
+        reqs.append((s, d, i % 4, -1, 0))
+    # plain bucket order: the off-lattice edges void the lattice lower bounds
+    p = SearchParams(slots=16, delta=0.5, n_buckets=8, engine=1, waves=waves)
+    res, st = route(ctx, g, cost_d, reqs, p)
+    say(f"[solo] irregular graph, {pick.sum()} of {N} nodes changed, waves {waves}: retried {st['retried']}")
+    for i, (s, d, nid, ri, ko) in enumerate(reqs):
+        path, dist, *_ = oracle.sssp_canon(og, c, s, d)
+        got = None if res[i] is None else res[i][0].tolist()
+        assert got == (path if path is not None and len(path) > 1 else None), f"request {i}"
+        if got:
+            exp_e = [int(indptr[u] + np.nonzero(indices[indptr[u]:indptr[u + 1]] == v)[0][0])
+                     for u, v in zip(path, path[1:])]
+            assert res[i][1].tolist() == exp_e
+            assert np.float32(res[i][2]) == dist[d]
