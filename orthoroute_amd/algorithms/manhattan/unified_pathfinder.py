@@ -954,7 +954,8 @@ class PathFinderRouter:
         if not paths:
             return
         lens = np.fromiter((len(p) for p in paths), dtype=np.int64, count=len(paths))
-        allp = np.concatenate(paths).astype(np.int64)
+        # node ids stay below 2^25, so their differences fit the paths' int32
+        allp = np.concatenate(paths)
         ends = np.cumsum(lens)
         # via hops among the consecutive pairs inside each path (a planar
         # step moves the index by 1 or X, a via by a multiple of X*Y; the
@@ -1363,11 +1364,13 @@ class PathFinderRouter:
             self.stats["solo_items"] += st["items"]
             self.stats["solo_edges"] += st["edges"]
             self.stats["solo_launches"] += st["launches"]
+        t0 = time.perf_counter()
         ol = offs.tolist()
         for j, i in enumerate(todo.tolist()):
             a, b = ol[j], ol[j + 1]
             if b - a > 1:
                 out[i] = (nodes[a:b], edges[a:b - 1])
+        self.stats["search_out_s"] += time.perf_counter() - t0
         return out
 
     def _route_trees(self, names):
@@ -1471,6 +1474,11 @@ class PathFinderRouter:
                     p.retarget_count += 1
 
     def _route_all(self, tasks, all_tasks=None, pres_fac=1.0, iteration=0, over_map=None):
+        # host phases of the pass (seconds, accumulated in self.stats):
+        # route_req_s until the searches start, route_post_s from their end
+        # to the commit (reordering, trees, fallbacks), route_commit_s the
+        # commit and barrel marks
+        t_start = time.perf_counter()
         if all_tasks is None:
             all_tasks = tasks
         self.full_graph_fallback_count = 0
@@ -1486,12 +1494,13 @@ class PathFinderRouter:
         ko_owner = self._keepout_owner_array() if use_ko else None
         # The searches of an iteration read one cost snapshot, so their
         # results do not depend on the routing order.  For a large batch they
-        # start while a helper thread computes that order (the library calls
-        # release the GIL), longest first by Manhattan distance, the order's
-        # leading term: slots refill in request order, and long searches
-        # started last would hold a refill batch open.  Results and requests
-        # are then put in routing order, which the fallback and the commit
-        # follow.
+        # run while a helper thread computes that order (started once the
+        # requests are built, so that it takes the GIL only while the library
+        # calls have released it), longest first by Manhattan distance, the
+        # order's leading term: slots refill in request order, and long
+        # searches started last would hold a refill batch open.  Results and
+        # requests are then put in routing order, which the fallback and the
+        # commit follow.
         helper = None
         if total > 1024 and not use_ko:
             box = {}
@@ -1502,7 +1511,6 @@ class PathFinderRouter:
                 except BaseException as exc:     # re-raised below
                     box["error"] = exc
             helper = threading.Thread(target=order_job, name="net-order")
-            helper.start()
             names = [nm for nm in tasks if nm not in trees]
             st = np.fromiter((v for nid in names for v in tasks[nid]), dtype=np.int64,
                              count=2 * len(names)).reshape(-1, 2)
@@ -1510,13 +1518,15 @@ class PathFinderRouter:
             zs_, rs_ = np.divmod(st[:, 0], plane_)
             zd_, rd_ = np.divmod(st[:, 1], plane_)
             dist_ = np.abs(rd_ % X_ - rs_ % X_) + np.abs(rd_ // X_ - rs_ // X_) + np.abs(zd_ - zs_)
-            search_order = [names[i] for i in np.argsort(-dist_, kind="stable").tolist()]
+            by_len = np.argsort(-dist_, kind="stable")
+            search_order = [names[i] for i in by_len.tolist()]
+            sd = st[by_len]
         else:
             ordered = self._order_nets_by_difficulty(tasks, over_map)
             search_order = [nm for nm in ordered if nm not in trees] if trees else ordered
+            sd = np.fromiter((v for nid in search_order for v in tasks[nid]), dtype=np.int64,
+                             count=2 * len(search_order)).reshape(-1, 2)
         n = len(search_order)
-        sd = np.fromiter((v for nid in search_order for v in tasks[nid]), dtype=np.int64,
-                         count=2 * n).reshape(n, 2)
         X, plane = self._Nx, self._Nx * self._Ny
         rs, rd = sd[:, 0] % plane, sd[:, 1] % plane
         md = np.abs(rd % X - rs % X) + np.abs(rd // X - rs // X)
@@ -1549,11 +1559,15 @@ class PathFinderRouter:
             if hint is None:
                 hint = self._last_depth_hint
             self._last_depth_hint = hint
+        self.stats["route_req_s"] += time.perf_counter() - t_start
+        if helper is not None:
+            helper.start()
         try:
             results = self._search_batch(reqs, ko_owner, readd=True, depth_hint=hint)
         finally:
             if helper is not None:
                 helper.join()
+        t_post = time.perf_counter()
         if helper is not None:
             if "error" in box:
                 raise box["error"]
@@ -1562,16 +1576,12 @@ class PathFinderRouter:
             # results and requests in routing order (a tree net's entry is
             # filled by the tree search)
             pos = {nm: i for i, nm in enumerate(search_order)}
+            perm = np.fromiter((pos.get(nm, -1) for nm in ordered), dtype=np.int64, count=len(ordered))
             rows = np.full((len(ordered), reqs.shape[1]), -1.0)
-            full = []
-            for j, nm in enumerate(ordered):
-                i = pos.get(nm)
-                if i is None:
-                    full.append(None)
-                else:
-                    full.append(results[i])
-                    rows[j] = reqs[i]
-            results, reqs = full, rows
+            have = perm >= 0
+            rows[have] = reqs[perm[have]]
+            results = [results[i] if i >= 0 else None for i in perm.tolist()]
+            reqs = rows
         if trees:
             grown = self._route_trees([nm for nm in ordered if nm in trees])
             for j, nm in enumerate(ordered):
@@ -1598,15 +1608,19 @@ class PathFinderRouter:
             fres = self._search_batch(freqs, None, readd=False)
             for i, r in zip(fb, fres):
                 results[i] = r
+        t_commit = time.perf_counter()
+        self.stats["route_post_s"] += t_commit - t_post
         # commit in routing order (UPF:4783-4812); usage counts are integer
         # adds, so the per-net +1/-1 scatters run as two batched launches
         # (the per-net bookkeeping of _clear_net_edge_tracking /
         # _update_net_edge_tracking, inlined: this loop runs over every net)
         dec, inc, marked, mark_ids = [], [], [], []
         paths, n2e, n2n, get_id = self.net_paths, self._net_to_edges, self._net_nodes, self._get_net_id
+        id_map = self.net_id_map
         tb = self.__dict__.setdefault("_tree_branches", {})
         for nid, r in zip(ordered, results):
-            if paths.get(nid):
+            held = paths.get(nid)
+            if held is not None and len(held):
                 old = n2e.pop(nid, None)
                 if old is not None:
                     dec.append(old)
@@ -1632,7 +1646,8 @@ class PathFinderRouter:
                 inc.append(edges)
                 paths[nid] = path.view(NodePath)
                 marked.append(path)
-                mark_ids.append(get_id(nid))
+                k = id_map.get(nid)
+                mark_ids.append(k if k is not None else get_id(nid))
                 n2e.pop(nid, None)
                 n2e[nid] = edges
                 n2n[nid] = path
@@ -1663,6 +1678,7 @@ class PathFinderRouter:
             self.accounting.commit_path(np.concatenate(inc))
         self._mark_barrels_many(marked, mark_ids)
         routed = sum(1 for p in self.net_paths.values() if len(p))
+        self.stats["route_commit_s"] += time.perf_counter() - t_commit
         return routed, len(all_tasks) - routed
 
     # ---------------- negotiation (UPF:3548-4259)
