@@ -101,8 +101,9 @@ def footprint(X, Y, L, policy, n_nets, slots, roi_limit):
     concurrent searches (keys, bucket bitmaps, touch records, path staging;
     with the one-wave engine, which the router uses when at least SOLO_MIN
     searches run at once, keys over the tiled index space and each search's
-    bucket-list arena) and the ROI BFS scratch and bitmap rows.  An upper
-    estimate of what the engine and the router allocate."""
+    bucket-list arena, and its adjacency records) and the ROI BFS scratch and
+    bitmap rows.  An upper estimate of what the engine and the router
+    allocate."""
     N, E = lattice_counts(X, Y, L, policy)
     csr = 4 * (N + 1) + 10 * E
     state = 20 * E + 4 * N
@@ -111,6 +112,10 @@ def footprint(X, Y, L, policy, n_nets, slots, roi_limit):
         tiled = ((X + 1) // 2) * ((Y + 1) // 2) * ((L + 3) // 4) * 16
         per_slot += 8 * (max(tiled, N) - N + 2048) + 8 * SOLO_ARENA + 8 * (SOLO_ARENA // 256)
     search = slots * per_slot + (RING_BUCKETS + 4) * (N * slots // 2048 + 64) * 4 // 32 + (1 << 20)
+    if min(int(n_nets), int(slots)) >= SOLO_MIN:
+        # the one-wave engine's adjacency records: 32 B per key index
+        tiled = ((X + 1) // 2) * ((Y + 1) // 2) * ((L + 3) // 4) * 16
+        search += 32 * (max(tiled, N) + 2048)
     roi_slots = max(1, min(1024, int(n_nets), roi_limit // (9 * N + 1)))
     roi = roi_slots * 9 * N + int(n_nets) * ((N + 31) // 32) * 4
     return int(1.05 * (csr + state + search + roi)) + (64 << 20)
