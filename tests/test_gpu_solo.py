@@ -41,9 +41,13 @@ def test_solo_matches_oracle(ctx, policy, slots, goal, waves):
 
 @pytest.mark.parametrize("waves", WAVES)
 @pytest.mark.parametrize("goal", [False, True], ids=["dijkstra", "goal"])
-def test_solo_keepout_matches_oracle(ctx, goal, waves):
+@pytest.mark.parametrize("policy", ["reference", "adjacent"])
+def test_solo_keepout_matches_oracle(ctx, goal, waves, policy):
+    """Owner-aware keepouts: on the reference via rules most nodes have more
+    than six edges (the launch reads CSR rows), on adjacent-layer vias every
+    node has an adjacency record (planar edges marked in its record)."""
     for seed, kind in [(21, "jitter"), (22, "ties"), (23, "congested")]:
-        run_case(ctx, 20, 16, 6, "reference", 16, 30, seed, kind, keepout=True, goal=goal, engine=1, waves=waves)
+        run_case(ctx, 20, 16, 6, policy, 16, 30, seed, kind, keepout=True, goal=goal, engine=1, waves=waves)
 
 
 @pytest.mark.parametrize("waves", [1, 4])
