@@ -160,10 +160,14 @@ typedef struct orgpu_sssp_stats {
 } orgpu_sssp_stats;
 
 /* Reserves scratch for graphs of up to n_nodes nodes, `slots` concurrent
- * searches (at most 2048) and `n_buckets` ring buckets (at most 32): with
- * NP = n_nodes rounded up to a multiple of 2048, keys slots*NP*8 bytes,
- * (n_buckets+3) bucket bitmaps of slots*NP bits, path staging
- * 2*slots*max_path_len ints.  queue_capacity is ignored (the frontiers are
+ * searches (at most 4096; the bucket-round engine runs at most 2048 of them)
+ * and `n_buckets` ring buckets (at most 32): with
+ * NP = n_nodes rounded up to a multiple of 2048, (n_buckets+3) bucket
+ * bitmaps of slots*NP bits and path staging 2*slots*max_path_len ints; the
+ * keys (8 bytes per slot and node: slots*NP for the bucket-round engine, the
+ * tiled index space for the one-wave engine) are allocated by the first
+ * search call, at the stride its engine needs, and grown once if a later
+ * call's engine needs more.  queue_capacity is ignored (the frontiers are
  * bitmaps; the argument stays for callers of the earlier queue form).
  * Blocks (allocation). */
 int orgpu_reserve(orgpu_t *h, int64_t n_nodes, int32_t slots, int32_t n_buckets,

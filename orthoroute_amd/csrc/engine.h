@@ -111,6 +111,7 @@ void solo_release(Engine &en);
 // Makes every key of the reserved slots KEY_INF for a graph of n_nodes nodes
 // (whole fill when the node count changed, else the chunks pending in the
 // bucket engine's touch records) and clears those records.
+int engine_keys_alloc(Engine &en, uint64_t need);
 int engine_prepare_keys(Engine &en, int64_t n_nodes);
 // the multi-source part of a route call (null for two-pin requests)
 struct MsCall {

@@ -1117,8 +1117,12 @@ int engine_reserve(Engine &en, int64_t n_nodes, int32_t slots, int32_t n_buckets
     const uint64_t chunks = (words + 63) / 64;
     const uint64_t summ_words = (chunks + 31) / 32;
     const int nbufs = n_buckets + 3;
-    HIP_CHECK_RET(hipMalloc((void **)&en.keys, keys_n * sizeof(uint64_t)));
-    en.keys_cap = (int64_t)keys_n;
+    // keys are allocated by the first search call, at the stride its engine
+    // lays them out with (engine_keys_alloc): a handle whose first call runs
+    // the one-wave engine allocates them once, at that engine's tiled stride
+    en.keys = nullptr;
+    en.keys_cap = 0;
+    en.keys_n = -1;
     HIP_CHECK_RET(hipMalloc((void **)&en.bits, (uint64_t)nbufs * words * sizeof(uint32_t)));
     HIP_CHECK_RET(hipMemsetAsync(en.bits, 0, (uint64_t)nbufs * words * sizeof(uint32_t), en.stream));
     HIP_CHECK_RET(hipMalloc((void **)&en.summ, (uint64_t)nbufs * summ_words * sizeof(uint32_t)));
@@ -1254,6 +1258,20 @@ __global__ void k_flush_touched(RoundArgs a)
         for (uint64_t v = lo + threadIdx.x; v < hi; v += blockDim.x) ks[v] = KEY_INF;
         if (threadIdx.x == 0) tb[ch >> 5] = 0u;
     }
+}
+
+// The key buffer holds at least `need` keys (grown when smaller; every key
+// is refilled with KEY_INF by the next call that uses them)
+int engine_keys_alloc(Engine &en, uint64_t need)
+{
+    if (need <= (uint64_t)en.keys_cap && en.keys) return ORGPU_OK;
+    if (en.keys) hipFree(en.keys);
+    en.keys = nullptr;
+    en.keys_cap = 0;
+    HIP_CHECK_RET(hipMalloc((void **)&en.keys, (size_t)need * sizeof(uint64_t)));
+    en.keys_cap = (int64_t)need;
+    en.keys_n = -1;
+    return ORGPU_OK;
 }
 
 int engine_prepare_keys(Engine &en, int64_t N)
@@ -1432,6 +1450,11 @@ int engine_route(Engine &en, const orgpu_graph *g, const orgpu_net_req *nets, in
     HIP_CHECK_RET(hipMemsetAsync(en.list_cnt, 0, (size_t)(nb + 4) * LIST_BINS * sizeof(uint32_t), s));
     // Keys are reset per slot from the touch records of its last search; those
     // records describe the layout slot*NP of the node count they were made for.
+    {
+        const int rk = engine_keys_alloc(en, (uint64_t)en.cap_slots * (uint64_t)a.NP);
+        if (rk) return rk;
+        a.keys = en.keys;
+    }
     if (en.keys_n != N) {
         hipLaunchKernelGGL(k_fill_keys, dim3(4096), dim3(256), 0, s, en.keys, en.keys_cap);
         HIP_CHECK_RET(hipGetLastError());

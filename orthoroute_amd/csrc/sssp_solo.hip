@@ -1269,14 +1269,7 @@ static int solo_reserve_arena(Engine &en, int32_t slots, uint32_t nch)
 // KEY_INF by the next engine_prepare_keys
 static int solo_reserve_keys(Engine &en, int32_t slots, int64_t NP)
 {
-    if ((uint64_t)slots * (uint64_t)NP <= (uint64_t)en.keys_cap) return ORGPU_OK;
-    if (en.keys) hipFree(en.keys);
-    en.keys = nullptr;
-    const int64_t cap = std::max<int64_t>((int64_t)slots * NP, en.keys_cap);
-    HIP_CHECK_RET(hipMalloc((void **)&en.keys, (size_t)cap * sizeof(uint64_t)));
-    en.keys_cap = cap;
-    en.keys_n = -1;
-    return ORGPU_OK;
+    return engine_keys_alloc(en, std::max<uint64_t>((uint64_t)slots * (uint64_t)NP, (uint64_t)en.keys_cap));
 }
 
 // adjacency records for a key stride of NP
