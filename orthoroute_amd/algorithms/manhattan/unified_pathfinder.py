@@ -164,6 +164,8 @@ class PathFinderConfig:
     sssp_target_buckets = None     # widen the bucket so a batch's deep searches span about this many
                                    # (None: 128 from 1 M nodes up, 32 below)
     max_iterations_cap = None
+    iteration_geometry = True      # build the provisional tracks and vias that iteration_cb receives
+                                   # each iteration (False: it receives empty lists)
     count_minimal = False          # also count Dijkstra-minimal nodes/edges per search
     goal_directed = True           # bucket on g + lower bound to the destination (same paths)
     sssp_order = "canonical"       # "canonical": FP32 costs, lowest-predecessor tie-break (batched
@@ -1869,7 +1871,8 @@ class PathFinderRouter:
                 try:
                     # the current routing state as geometry (UPF:4066-4117;
                     # no escape geometry to merge in this build)
-                    prov_tracks, prov_vias = self._generate_geometry_from_paths()
+                    prov_tracks, prov_vias = (self._generate_geometry_from_paths()
+                                              if cfg.iteration_geometry else ([], []))
                     iteration_cb(it, prov_tracks, prov_vias, over_sum, over_cnt)
                 except Exception as e:
                     logger.warning(f"[ITER {it}] iteration callback failed: {e}")

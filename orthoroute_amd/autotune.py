@@ -29,8 +29,11 @@ than 8,192 in the first congested iterations).
 """
 import math
 
-# targets a tuner moves between (x4 apart)
-TARGETS = (32, 128, 512, 2048, 8192, 32768, 131072)
+# targets a tuner moves between (x4 apart); the deep congested searches of
+# late iterations on the largest boards settle at 524,288 or more (a 19 M-node
+# board with 25,000 nets: iterations 2-8 in 0.18-0.82 s of search at those
+# targets against 0.26-2.5 s with 131,072 as the narrowest)
+TARGETS = (32, 128, 512, 2048, 8192, 32768, 131072, 524288, 2097152)
 _SETTLED = {}   # lattice key -> settled target, for later negotiations in this process
 
 

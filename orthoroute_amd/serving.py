@@ -162,6 +162,7 @@ def run_job(job, device=0, slots=None, roi_limit=1 << 30, stream=None):
             cfg.lattice_policy = job.policy
             cfg.batch_slots = int(slots or choose_slots(job))
             cfg.max_iterations_cap = int(job.max_iterations)
+            cfg.iteration_geometry = False     # the progress callback reads only the overuse
             r = PathFinderRouter(cfg)
             r.ctx = ctx
             r.initialize_lattice(job.X, job.Y, job.L, pitch=1.0, via_cost=job.via_cost)
