@@ -1117,7 +1117,7 @@ class PathFinderRouter:
         if not hasattr(self, "_net_clean_iters"):
             self._net_clean_iters = {}
         if len(hot_idx) == 0:
-            unrouted = {n for n in tasks if not self.net_paths.get(n)}
+            unrouted = {n for n in tasks if not len(self.net_paths.get(n, ()))}
             return unrouted | ripped
         E, _, _, NE, _, _ = flat
         # hot edges: a 1 marker in a slot buffer (an edge can be hot with
@@ -1133,7 +1133,7 @@ class PathFinderRouter:
         thr = int(getattr(self.config, "freeze_after_clean", 3))
         offenders -= {n for n in offenders if self._net_clean_iters.get(n, 0) >= thr}
         offenders |= ripped
-        unrouted = {n for n in tasks if not self.net_paths.get(n)}
+        unrouted = {n for n in tasks if not len(self.net_paths.get(n, ()))}
         scored = [n for n in offenders if n in self._net_to_edges]
         # impact: the sum of the hot edges' overuse along the net's path
         # (UPF:5071), on the device for every offender at once
@@ -1160,7 +1160,7 @@ class PathFinderRouter:
 
     def _rip_top_k_offenders(self, o_idx, o_val, k=20):
         cand = [n for n, path in self.net_paths.items()
-                if path and n not in self.locked_nets and n in self._net_to_edges]
+                if len(path) and n not in self.locked_nets and n in self._net_to_edges]
         imps = edge_list_sums(self.ctx, self.accounting.E, [self._net_to_edges[n] for n in cand], o_idx, o_val)
         scores = [(v, n) for v, n in zip(imps.tolist(), cand) if v > 0]
         scores.sort(reverse=True)
@@ -1771,7 +1771,7 @@ class PathFinderRouter:
                 for n in tasks:
                     if n in self._last_ripped:
                         continue
-                    if not self.net_paths.get(n):
+                    if not len(self.net_paths.get(n, ())):
                         self._net_failure_count[n] = self._net_failure_count.get(n, 0) + 1
                     else:
                         self._net_failure_count[n] = 0
@@ -1852,7 +1852,7 @@ class PathFinderRouter:
                                        "routed": routed, "failed": failed, "rerouted": len(sub),
                                        "t": time.perf_counter() - t_it})
             if over_sum == 0:
-                unrouted = {n for n in tasks if not self.net_paths.get(n)}
+                unrouted = {n for n in tasks if not len(self.net_paths.get(n, ()))}
                 if not unrouted:
                     break
                 pres_fac = min(pres_fac, 10.0)
