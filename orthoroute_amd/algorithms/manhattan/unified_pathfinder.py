@@ -156,6 +156,7 @@ class PathFinderConfig:
                                    # wavefront per search, "auto" = 1 when at least sssp_solo_min
                                    # searches run at once, else 0 (same paths either way)
     sssp_solo_min = 1024
+    sssp_solo_nodes = 2_000_000    # "auto": every batch of a board up to this many nodes on engine 1
     sssp_list_cap = 0              # engine 1: bucket-list entries per search (0 = library default)
     sssp_lookahead = 0             # engine 1: buckets an under-full step looks ahead (0 = library
                                    # default, < 0 = none)
@@ -1176,12 +1177,21 @@ class PathFinderRouter:
     # ---------------- batched search of one iteration (UPF:4441-4823)
     def _engine_for(self, n_searches):
         """Search engine of a batch of n two-pin searches: the one-wave engine
-        fills the GPU with many searches (one wavefront each), the bucket-round
-        engine spreads a few large searches (late iterations, keepout
-        searches that exhaust the keepout-free region first) over every CU."""
+        fills the GPU with many searches (a workgroup each), and on boards of
+        up to sssp_solo_nodes nodes it also takes the late iterations' few
+        searches (a search that covers such a board is a few thousand steps
+        of one workgroup, where the bucket-round engine pays thousands of
+        launch-bound rounds: 96x96x6 board 0.47 s against 1.00 s for 30
+        iterations, 300x300x12 0.35 against 0.53 s for 5); on larger boards
+        the bucket-round engine spreads each of a few large searches over
+        every CU (600x400x18: 1.41 against 3.26 s for 8 iterations).  The
+        plain form (one slot) stays on the bucket-round engine."""
         e = self.config.sssp_engine
         if e == "auto":
-            n = min(int(n_searches), int(self.config.batch_slots))
+            slots = int(self.config.batch_slots)
+            if slots > 1 and self.N <= int(getattr(self.config, "sssp_solo_nodes", 0) or 0):
+                return 1
+            n = min(int(n_searches), slots)
             return 1 if n >= int(self.config.sssp_solo_min) else 0
         return int(e or 0)
 

@@ -44,6 +44,7 @@ MAX_PATH = 1 << 16          # path staging length the router reserves per search
 RING_BUCKETS = 8            # bucket ring the router's search uses by default
 TOUCH_KEYS = 1024           # keys per touched-chunk record of a search slot
 SOLO_MIN = 1024             # searches at once from which the router's batches run on the one-wave engine
+SOLO_NODES = 2_000_000      # boards up to this many nodes run every batch on it (several slots)
 
 
 @dataclass
@@ -100,19 +101,21 @@ def footprint(X, Y, L, policy, n_nets, slots, roi_limit):
     total cost), the keepout owner map, the search scratch of `slots`
     concurrent searches (keys, bucket bitmaps, touch records, path staging;
     with the one-wave engine, which the router uses when at least SOLO_MIN
-    searches run at once, keys over the tiled index space and each search's
+    searches run at once or the board has at most SOLO_NODES nodes, keys
+    over the tiled index space and each search's
     bucket-list arena, and its adjacency records) and the ROI BFS scratch and
     bitmap rows.  An upper estimate of what the engine and the router
     allocate."""
     N, E = lattice_counts(X, Y, L, policy)
+    solo = min(int(n_nets), int(slots)) >= SOLO_MIN or (int(slots) > 1 and N <= SOLO_NODES)
     csr = 4 * (N + 1) + 10 * E
     state = 20 * E + 4 * N
     per_slot = 8 * N + (RING_BUCKETS + 3) * N // 8 + 4 * (N // TOUCH_KEYS + 1) + 4 * MAX_PATH * 4 + 64
-    if min(int(n_nets), int(slots)) >= SOLO_MIN:
+    if solo:
         tiled = ((X + 1) // 2) * ((Y + 1) // 2) * ((L + 3) // 4) * 16
         per_slot += 8 * (max(tiled, N) - N + 2048) + 8 * SOLO_ARENA + 8 * (SOLO_ARENA // 256)
     search = slots * per_slot + (RING_BUCKETS + 4) * (N * slots // 2048 + 64) * 4 // 32 + (1 << 20)
-    if min(int(n_nets), int(slots)) >= SOLO_MIN:
+    if solo:
         # the one-wave engine's adjacency records: 32 B per key index
         tiled = ((X + 1) // 2) * ((Y + 1) // 2) * ((L + 3) // 4) * 16
         search += 32 * (max(tiled, N) + 2048)

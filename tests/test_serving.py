@@ -70,7 +70,8 @@ def test_choose_slots_respects_memory_budget():
 
 
 def test_admission_budget_slots_and_fifo():
-    fps = [footprint(40, 40, 4, "adjacent", 10, 10, 1 << 20)] * 6
+    # the footprint at the slot count the server picks for these jobs
+    fps = [footprint(40, 40, 4, "adjacent", 10, choose_slots(_job(0)), 1 << 20)] * 6
     budget = int(2.5 * fps[0])          # two jobs fit at once
     ref = [None]
     rec = Recorder(ref, {i: 0.05 for i in range(6)})
@@ -114,7 +115,7 @@ def test_late_arrival_is_admitted_while_others_run():
 def test_oversize_job_is_rejected_and_the_rest_run():
     ref = [None]
     rec = Recorder(ref, {})
-    small = footprint(40, 40, 4, "adjacent", 10, 10, 1 << 20)
+    small = footprint(40, 40, 4, "adjacent", 10, choose_slots(_job(0)), 1 << 20)
     srv = RoutingServer(hbm_budget=2 * small, max_active=2, roi_limit=1 << 20, runner=rec)
     ref[0] = srv
     res = replay(srv, [_job(0), _job(1, X=400, Y=400, L=8, n=200), _job(2)])
@@ -186,3 +187,15 @@ def test_large_job_gets_fewer_searches_instead_of_rejection():
     res = replay(tiny, [big])
     tiny.close()
     assert res[0].error is not None and "exceeds" in res[0].error
+
+
+def test_footprint_counts_the_one_wave_engine_on_small_boards():
+    """A board of at most SOLO_NODES nodes runs every batch of several slots
+    on the one-wave engine: its footprint holds each slot's bucket-list
+    arena; one slot (the plain form) does not."""
+    from orthoroute_amd.device import SOLO_ARENA
+    from orthoroute_amd.serving import SOLO_NODES
+    assert 100 * 100 * 4 <= SOLO_NODES
+    one = footprint(100, 100, 4, "adjacent", 10, 1, 1 << 20)
+    ten = footprint(100, 100, 4, "adjacent", 10, 10, 1 << 20)
+    assert ten - one >= 10 * 8 * SOLO_ARENA
