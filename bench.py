@@ -509,7 +509,9 @@ def negotiation(args, which, cap, with_plain, trees=False):
            "sssp_s": round(stats.get("sssp_s", 0.0), 3), "roi_s": round(stats.get("roi_s", 0.0), 3),
            "tree_s": round(stats.get("tree_s", 0.0), 3),
            "routed_trees": sum(1 for q in reqs[n_two:] if res["paths"].get(q[0])),
-           "searches": int(stats.get("sssp_searches", 0))}
+           "searches": int(stats.get("sssp_searches", 0)),
+           "small_batches_to_one_wave_engine": ("predicted" if stats.get("small_engine_switch_predicted") else
+                                                "measured" if stats.get("small_engine_switch_measured") else None)}
     if with_plain:
         pdt, pres, prec, pt_it, ppaths, _ = run(1)
         same = prec == rec and ppaths == paths

@@ -156,7 +156,9 @@ class PathFinderConfig:
                                    # wavefront per search, "auto" = 1 when at least sssp_solo_min
                                    # searches run at once, else 0 (same paths either way)
     sssp_solo_min = 1024
-    sssp_solo_nodes = 2_000_000    # "auto": every batch of a board up to this many nodes on engine 1
+    sssp_solo_nodes = 2_000_000    # "auto": small batches of a board up to this many nodes, in a
+    sssp_solo_density = 0.001      # negotiation of at least this many nets per node, may move to
+                                   # engine 1 (_small_engine_predict, _small_engine_record)
     sssp_list_cap = 0              # engine 1: bucket-list entries per search (0 = library default)
     sssp_lookahead = 0             # engine 1: buckets an under-full step looks ahead (0 = library
                                    # default, < 0 = none)
@@ -1177,23 +1179,69 @@ class PathFinderRouter:
     # ---------------- batched search of one iteration (UPF:4441-4823)
     def _engine_for(self, n_searches):
         """Search engine of a batch of n two-pin searches: the one-wave engine
-        fills the GPU with many searches (a workgroup each), and on boards of
-        up to sssp_solo_nodes nodes it also takes the late iterations' few
-        searches (a search that covers such a board is a few thousand steps
-        of one workgroup, where the bucket-round engine pays thousands of
-        launch-bound rounds: 96x96x6 board 0.47 s against 1.00 s for 30
-        iterations, 300x300x12 0.35 against 0.53 s for 5); on larger boards
-        the bucket-round engine spreads each of a few large searches over
-        every CU (600x400x18: 1.41 against 3.26 s for 8 iterations).  The
-        plain form (one slot) stays on the bucket-round engine."""
+        fills the GPU with many searches (a workgroup each); a few searches
+        run on the bucket-round engine, which spreads each over every CU,
+        unless the board has at most sssp_solo_nodes nodes, the negotiation
+        at least sssp_solo_density nets per node (a crowded board, whose late
+        searches are deep; on lightly loaded boards the bucket engine's few
+        rounds are the faster way), it has no multi-terminal trees, and a
+        small batch has moved them to the one-wave engine
+        (_small_engine_record; _search_batch also predicts the rounds of each
+        such batch, _small_engine_predict).  The plain form (one slot) stays
+        on the bucket-round engine."""
         e = self.config.sssp_engine
         if e == "auto":
             slots = int(self.config.batch_slots)
-            if slots > 1 and self.N <= int(getattr(self.config, "sssp_solo_nodes", 0) or 0):
-                return 1
             n = min(int(n_searches), slots)
-            return 1 if n >= int(self.config.sssp_solo_min) else 0
+            if n >= int(self.config.sssp_solo_min):
+                return 1
+            if slots > 1 and self.N <= int(getattr(self.config, "sssp_solo_nodes", 0) or 0):
+                return int(self.__dict__.get("_small_engine", 0))
+            return 0
         return int(e or 0)
+
+    def _tuner_for(self, engine):
+        """The bucket-target tuner of this negotiation's batches on `engine`
+        (each engine has its own: their best widths differ), or None outside
+        a negotiation."""
+        key = self.__dict__.get("_tuner_key")
+        if key is None:
+            return None
+        t = self._tuners.get(int(engine))
+        if t is None:
+            t = self._tuners[int(engine)] = BucketTuner(f"{key}:{int(engine)}", self.N)
+        return t
+
+    def _small_solo_time(self):
+        """A conservative one-wave engine time for a batch of searches that
+        each cover the board (about N/256 steps of a four-wave workgroup at
+        about 20 us a step), and at least 1/4 s: below that the bucket
+        engine's batches are cheap whatever the board (lightly loaded boards,
+        short searches, where it is the faster engine)."""
+        return max((self.N / 256.0) * 20e-6, 0.25)
+
+    def _small_engine_record(self, engine, seconds):
+        """Moves a negotiation's small batches (fewer than sssp_solo_min
+        searches, several slots, a board of at most sssp_solo_nodes nodes) to
+        the one-wave engine for good when a bucket-engine batch took more than
+        twice _small_solo_time: its rounds were the deep, launch-bound ones
+        of a congested board (a 100,820-node board with 2,520 nets: 40
+        iterations in 5.8 s on the one-wave engine against 46.6 s; single
+        bucket-engine batches there took up to 30 s).  Paths are the same on
+        either engine."""
+        if engine == 0 and seconds > 2.0 * self._small_solo_time() and not self.__dict__.get("_small_engine"):
+            self._small_engine = 1
+            self.stats["small_engine_switch_measured"] += 1
+
+    def _small_engine_predict(self, depth_hint, slots):
+        """The engine of a small batch before it runs: the one-wave engine when
+        the bucket engine's rounds for the batch's depth estimate (the deepest
+        search spans depth / bucket width buckets, each a round of about
+        0.1 ms) would take more than twice _small_solo_time."""
+        if depth_hint is None:
+            return 0
+        width = self._search_params(slots, depth_hint, engine=0).delta
+        return 1 if (float(depth_hint) / max(width, 1e-9)) * 1e-4 > 2.0 * self._small_solo_time() else 0
 
     def _search_params(self, slots, depth=None, engine=0):
         """depth: the batch's estimated search depth in cost units beyond
@@ -1207,9 +1255,9 @@ class PathFinderRouter:
             d2, nb2 = base_width(self, getattr(self, "_cost_scale", 0.0), engine), 8
             target = self.config.sssp_target_buckets
             if target is None:
-                # a negotiation's tuner picks the target of batches with a
-                # depth estimate; otherwise the starting default
-                tuner = getattr(self, "_tuner", None)
+                # a negotiation's tuner (one per engine) picks the target of
+                # batches with a depth estimate; otherwise the starting default
+                tuner = self._tuner_for(engine)
                 if tuner is not None and depth is not None:
                     target = tuner.target(ratio=float(depth) / d2)
                 else:
@@ -1317,6 +1365,22 @@ class PathFinderRouter:
         if not len(todo):
             return out
         eng = 0 if ref_order else self._engine_for(len(todo))
+        # (not with multi-terminal trees in the negotiation: there the
+        # two-pin keepout searches around the trees measured slower on the
+        # one-wave engine, cfg3 with its 200 trees 1.42 against 1.15 s)
+        small = (not ref_order and not getattr(self, "_tree_pins", None) and self.config.sssp_engine == "auto"
+                 and int(self.config.batch_slots) > 1
+                 and min(len(todo), int(self.config.batch_slots)) < int(self.config.sssp_solo_min)
+                 and self.N <= int(getattr(self.config, "sssp_solo_nodes", 0) or 0)
+                 and self.__dict__.get("_negotiation_nets", 0)
+                 >= float(getattr(self.config, "sssp_solo_density", 0.0)) * self.N)
+        if small and eng == 0:
+            eng = self._small_engine_predict(depth_hint, max(1, min(int(self.config.batch_slots), len(todo))))
+            if eng == 1:
+                # the negotiation's later small batches (the wider-ROI
+                # fallback's included, which have no depth estimate) follow
+                self._small_engine = 1
+                self.stats["small_engine_switch_predicted"] += 1
         slots = max(1, min(int(self.config.batch_slots), len(todo), max_slots(self.N, engine=eng)))
         params = self._search_params(slots, depth_hint, engine=eng)
         sreqs = np.stack([src[todo], dst[todo], nid[todo], row_of[todo],
@@ -1355,10 +1419,12 @@ class PathFinderRouter:
                                                       roi_bitmaps=rows, keepout_owner=ko)
         dt = time.perf_counter() - t0
         self.stats["sssp_s"] += dt
-        tuner = getattr(self, "_tuner", None)
+        tuner = self._tuner_for(params.engine)
         if (tuner is not None and depth_hint is not None and self.config.sssp_target_buckets is None
                 and self.config.sssp_delta is None):
             tuner.record(dt)
+        if small:
+            self._small_engine_record(params.engine, dt)
         self.stats["sssp_kernel_ms"] += st["kernel_ms"]
         self.stats["sssp_rounds"] += st["rounds"]
         self.stats["sssp_edges"] += st["edges"]
@@ -1761,8 +1827,9 @@ class PathFinderRouter:
         # the bucket target of the batches with a depth estimate, tuned on
         # this negotiation's own search times (speed only: paths are the same)
         slots_key = int(min(int(cfg.batch_slots), max_slots(self.N)) > 1)
-        self._tuner = BucketTuner(f"{self._Nx}x{self._Ny}x{self._Nz}:{self.lattice.policy}:{slots_key}:"
-                                  f"{cfg.sssp_engine}", self.N)
+        self._tuner_key = f"{self._Nx}x{self._Ny}x{self._Nz}:{self.lattice.policy}:{slots_key}"
+        self._tuners = {}
+        self._negotiation_nets = len(tasks)
         self._net_failure_count = {}
         self._excluded_nets = set()
         routed = failed = 0
