@@ -101,3 +101,26 @@ def test_wave_count_rule():
     assert solo_waves(0, 1024, 40000) == 2
     assert solo_waves(0, 64, 65) == 2
     assert solo_waves(0, 64, 1) == 4
+
+
+def test_small_batch_engine_switch_thresholds():
+    """A negotiation's small batches move to the one-wave engine for good once
+    a bucket-engine batch took more than twice a board-covering one-wave
+    batch (N/256 steps of 20 us, at least 1/4 s); one-wave batches and quick
+    bucket-engine batches leave the choice alone."""
+    from orthoroute_amd.algorithms.manhattan.unified_pathfinder import PathFinderConfig, PathFinderRouter
+    r = PathFinderRouter(PathFinderConfig())
+    r.N = 100_000                        # 7.8 ms of one-wave steps: the 1/4 s floor holds
+    assert r._small_solo_time() == 0.25
+    r._small_engine_record(0, 0.45)
+    r._small_engine_record(1, 5.0)
+    assert not r.__dict__.get("_small_engine")
+    r._small_engine_record(0, 0.55)
+    assert r._small_engine == 1 and r.stats["small_engine_switch_measured"] == 1
+    r._small_engine_record(1, 9.0)      # no way back
+    assert r._small_engine == 1
+    big = PathFinderRouter(PathFinderConfig())
+    big.N = 1_920_000                    # 150 ms of one-wave steps
+    assert abs(big._small_solo_time() - 0.25) < 1e-12
+    big.N = 4_000_000
+    assert abs(big._small_solo_time() - 4_000_000 / 256 * 20e-6) < 1e-12
