@@ -511,6 +511,9 @@ class PathFinderRouter:
         self.ctx = self.ctx or Context(0)
         self.lattice = Lattice3D(bounds, pitch, layers, getattr(self.config, "lattice_policy", "reference"))
         self.graph = self.lattice.build_graph(self.config.via_cost, ctx=self.ctx)
+        # per-lattice engine choices start over
+        for k in ("_rows_fit", "_small_engine"):
+            self.__dict__.pop(k, None)
         self.N = self.lattice.num_nodes
         self.accounting = EdgeAccountant(self.ctx, self.graph.dlat)
         X, Y, L = self.lattice.x_steps, self.lattice.y_steps, layers
@@ -1184,7 +1187,8 @@ class PathFinderRouter:
         unless the board has at most sssp_solo_nodes nodes, the negotiation
         at least sssp_solo_density nets per node (a crowded board, whose late
         searches are deep; on lightly loaded boards the bucket engine's few
-        rounds are the faster way), it has no multi-terminal trees, and a
+        rounds are the faster way), it has no multi-terminal trees, the
+        board's rows fit the one-wave engine's adjacency records, and a
         small batch has moved them to the one-wave engine
         (_small_engine_record; _search_batch also predicts the rounds of each
         such batch, _small_engine_predict).  The plain form (one slot) stays
@@ -1211,6 +1215,19 @@ class PathFinderRouter:
         if t is None:
             t = self._tuners[int(engine)] = BucketTuner(f"{key}:{int(engine)}", self.N)
         return t
+
+    def _rows_fit_records(self):
+        """True when at most one node in eight has more than six edges, so
+        that the one-wave engine reads adjacency records rather than CSR rows
+        (the reference via rules give most nodes of a 6-layer board 7-10
+        edges: cfg1 in canonical order took 1.37 s on the one-wave engine's
+        CSR path against 0.95 s on the bucket engine)."""
+        v = self.__dict__.get("_rows_fit")
+        if v is None:
+            ip = self.graph.dlat.indptr
+            v = bool(int(((ip[1:] - ip[:-1]) > 6).sum().item()) * 8 <= self.N)
+            self._rows_fit = v
+        return v
 
     def _small_solo_time(self):
         """A conservative one-wave engine time for a batch of searches that
@@ -1368,7 +1385,8 @@ class PathFinderRouter:
         # (not with multi-terminal trees in the negotiation: there the
         # two-pin keepout searches around the trees measured slower on the
         # one-wave engine, cfg3 with its 200 trees 1.42 against 1.15 s)
-        small = (not ref_order and not getattr(self, "_tree_pins", None) and self.config.sssp_engine == "auto"
+        small = (not ref_order and not getattr(self, "_tree_pins", None) and self._rows_fit_records()
+                 and self.config.sssp_engine == "auto"
                  and int(self.config.batch_slots) > 1
                  and min(len(todo), int(self.config.batch_slots)) < int(self.config.sssp_solo_min)
                  and self.N <= int(getattr(self.config, "sssp_solo_nodes", 0) or 0)
