@@ -152,7 +152,11 @@ struct SoloLists {
     uint32_t *link;        // [nch] next chunk of the chain
     uint32_t *freestk;     // [nch]
     uint32_t nch;
-    uint32_t *hc, *hp, *tc, *tp, *cnt;   // LDS [SOLO_NL]
+    uint32_t *hc;                        // LDS [SOLO_NL]: head chunk
+    uint16_t *hp;                        //   position in it (< SOLO_CH)
+    uint32_t *tc;                        //   tail chunk
+    uint16_t *tp;                        //   entries in it (<= SOLO_CH)
+    uint32_t *cnt;                       //   entries of the list
     uint32_t *al;                        // LDS [2]: free-stack top, bump pointer
     uint32_t *mask;                      // LDS [SOLO_LEVELS]: non-empty lists of each digit
 
@@ -484,7 +488,8 @@ __global__ void __launch_bounds__(64)
 k_solo(SoloArgs a)
 {
     __shared__ uint32_t s_touch[SOLO_TOUCH_WORDS];
-    __shared__ uint32_t s_hc[SOLO_NL], s_hp[SOLO_NL], s_tc[SOLO_NL], s_tp[SOLO_NL], s_cnt[SOLO_NL], s_al[2];
+    __shared__ uint32_t s_hc[SOLO_NL], s_tc[SOLO_NL], s_cnt[SOLO_NL], s_al[2];
+    __shared__ uint16_t s_hp[SOLO_NL], s_tp[SOLO_NL];      // (two-byte positions: 8 two-wave workgroups per CU)
     __shared__ uint32_t s_mask[SOLO_LEVELS];
     __shared__ uint32_t s_ddtag[SOLO_DD];          // relaxation table: node + 1 (0 = free)
     __shared__ uint64_t s_ddkey[SOLO_DD];          // its best candidate key
@@ -840,7 +845,8 @@ k_solo_mw(SoloArgs a)
     constexpr uint32_t DD = SOLO_DD * (uint32_t)W;   // >= 2 x the 4*NT candidates of a pass
     constexpr uint32_t DD_BITS = SOLO_DD_BITS + (W == 2 ? 1 : 2);
     __shared__ uint32_t s_touch[SOLO_TOUCH_WORDS];
-    __shared__ uint32_t s_hc[SOLO_NL], s_hp[SOLO_NL], s_tc[SOLO_NL], s_tp[SOLO_NL], s_cnt[SOLO_NL], s_al[2];
+    __shared__ uint32_t s_hc[SOLO_NL], s_tc[SOLO_NL], s_cnt[SOLO_NL], s_al[2];
+    __shared__ uint16_t s_hp[SOLO_NL], s_tp[SOLO_NL];      // (two-byte positions: 8 two-wave workgroups per CU)
     __shared__ uint32_t s_mask[SOLO_LEVELS];
     __shared__ uint32_t s_ddtag[DD];
     __shared__ uint64_t s_ddkey[DD];
