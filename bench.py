@@ -470,6 +470,7 @@ def negotiation(args, which, cap, with_plain, trees=False):
 
     def run(slots):
         cfg = PathFinderConfig()
+        cfg.iteration_geometry = False     # the iteration callbacks below read only the overuse
         cfg.lattice_policy = "adjacent"
         cfg.batch_slots = slots
         cfg.max_iterations_cap = cap
@@ -548,6 +549,7 @@ def reference_order(args):
     def run(order):
         nonlocal E
         cfg = PathFinderConfig()
+        cfg.iteration_geometry = False     # the iteration callbacks below read only the overuse
         cfg.lattice_policy = "reference"
         cfg.max_iterations_cap = cap
         cfg.sssp_order = order
@@ -705,6 +707,7 @@ def time_to_zero_overuse(args):
 
     def gpu_run(slots=1024):
         cfg = PathFinderConfig()
+        cfg.iteration_geometry = False     # the iteration callbacks below read only the overuse
         cfg.lattice_policy = "reference"
         cfg.batch_slots = slots
         cfg.sssp_graph_rounds = args.graph_rounds
